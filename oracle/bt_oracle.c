@@ -1,0 +1,1173 @@
+/*
+ * bt_oracle.c -- CPU restatement of BaryTree's FP64 interaction path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product): only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this file's
+ * library.  Compile with -ffp-contract=off so the arithmetic is the
+ * reference's arithmetic without fused multiply-adds.
+ *
+ * Scope: Lagrange approximation; PC, CP and CC compute types; the Coulomb,
+ * Yukawa and regularized-Coulomb kernels with skipping and subtraction, in
+ * exactly the combinations the reference dispatchers accept
+ * (src/interaction_compute/interaction_compute_{pc,cp,cc}.c).  Hermite is
+ * rejected with an error code ("parity unpinned" -- not restated here).
+ *
+ * Every function cites the reference lines it follows.  Deviations, each
+ * deliberate:
+ *   - caller arrays are copied, never permuted in place;
+ *   - target-cluster charges start at zero (the reference's CPU path leaves
+ *     them uninitialised, src/clusters/clusters.c:245-253 allocates without
+ *     clearing; its OpenACC path clears them, :258-271);
+ *   - a NULL source weight is treated as 1 for skipping (the reference uses
+ *     an uninitialised dummy that the skipping path never reads);
+ *   - errors return a negative code instead of exit(1).
+ */
+#define _GNU_SOURCE
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "bt_oracle.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+enum { KER_NONE = 0, KER_COULOMB = 1, KER_YUKAWA = 2, KER_REGCOULOMB = 3 };
+enum { SNG_NONE = 0, SNG_SKIP = 1, SNG_SUB = 2 };
+enum { APX_NONE = 0, APX_LAGRANGE = 1, APX_HERMITE = 2 };
+enum { CT_NONE = 0, CT_PC = 1, CT_CP = 2, CT_CC = 3 };
+
+#define ORC_ERR_ARGS (-1)
+#define ORC_ERR_UNSUPPORTED (-2)
+#define ORC_ERR_NOMEM (-3)
+
+static __thread orc_stats g_stats;
+static __thread int g_stride = 1;
+
+static double wall(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------ */
+/* Parameters: src/run_params/run_params.c:13-131                       */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int kernel, singularity, approximation, compute_type;
+    double kp[4];
+    double theta, size_check_factor;
+    int degree, P;
+    int msl, mtl;
+} Prm;
+
+void orc_params_resolve(int compute_type, int approximation, double theta, int degree,
+                        int msl, int mtl, double size_check, double beta,
+                        double *theta_out, int *degree_out, double *size_check_out,
+                        int *msl_out, int *mtl_out)
+{
+    if (beta < 0 || beta > 1) {
+        *theta_out = theta;
+        *degree_out = degree;
+        *size_check_out = size_check;
+        *msl_out = msl;
+        *mtl_out = mtl;
+        return;
+    }
+    double tmin = 0.55, tmax = 0.95, nmin = 1, nmax = 12, es = 2, et = 3, scf = 1.0;
+    if (compute_type == CT_PC || compute_type == CT_CP) {
+        if (approximation != APX_LAGRANGE) { scf = 4.0; nmax = 9; }
+    } else {
+        es = 1.75;
+    }
+    *size_check_out = scf;
+    *theta_out = tmax - (tmax - tmin) * pow(beta, es);
+    *degree_out = (int)(nmax - (nmax - nmin) * pow(1. - beta, et));
+    /* CPU-build leaf sizes (run_params.c:107-119) */
+    if (compute_type == CT_PC) { *msl_out = 50; *mtl_out = 5; }
+    else if (compute_type == CT_CP) { *msl_out = 5; *mtl_out = 50; }
+    else { *msl_out = 50; *mtl_out = 50; }
+}
+
+/* ------------------------------------------------------------------ */
+/* Particles: SoA copies with the 1-based order permutation            */
+/* (src/particles/particles.c:58-144)                                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n;
+    double *x, *y, *z, *q, *w; /* w may be NULL (targets) */
+    int *order;
+} Parts;
+
+static int parts_copy(Parts *p, int n, const double *x, const double *y, const double *z,
+                      const double *q, const double *w, int want_w)
+{
+    size_t b = (size_t)(n > 0 ? n : 1) * sizeof(double);
+    p->n = n;
+    p->x = malloc(b); p->y = malloc(b); p->z = malloc(b); p->q = malloc(b);
+    p->w = want_w ? malloc(b) : NULL;
+    p->order = malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    if (!p->x || !p->y || !p->z || !p->q || (want_w && !p->w) || !p->order) return ORC_ERR_NOMEM;
+    for (int i = 0; i < n; i++) {
+        p->x[i] = x[i]; p->y[i] = y[i]; p->z[i] = z[i];
+        p->q[i] = q ? q[i] : 0.0;
+        if (want_w) p->w[i] = w ? w[i] : 1.0;
+        p->order[i] = i + 1;
+    }
+    return 0;
+}
+
+static void parts_free(Parts *p)
+{
+    free(p->x); free(p->y); free(p->z); free(p->q); free(p->w); free(p->order);
+    memset(p, 0, sizeof(*p));
+}
+
+/* ------------------------------------------------------------------ */
+/* Partition: src/tree/partition.c:14-175 (in-place, pivot hole).       */
+/* Key dimension d; all arrays follow the same permutation.  Returns   */
+/* the 1-based index of the last element of the lower part.            */
+/* ------------------------------------------------------------------ */
+
+static double *key_of(Parts *p, int d) { return d == 0 ? p->x : (d == 1 ? p->y : p->z); }
+
+static void move_el(Parts *p, int dst, int src)
+{
+    p->x[dst] = p->x[src]; p->y[dst] = p->y[src]; p->z[dst] = p->z[src];
+    p->q[dst] = p->q[src];
+    if (p->w) p->w[dst] = p->w[src];
+    p->order[dst] = p->order[src];
+}
+
+static int split_range(Parts *p, int d, int ibeg, int iend, double val)
+{
+    double *a = key_of(p, d);
+    if (ibeg < iend) {
+        double hx = p->x[ibeg - 1], hy = p->y[ibeg - 1], hz = p->z[ibeg - 1];
+        double hq = p->q[ibeg - 1], hw = p->w ? p->w[ibeg - 1] : 0.0;
+        int ho = p->order[ibeg - 1];
+        double hkey = a[ibeg - 1];
+        a[ibeg - 1] = val;
+        int up = ibeg, lo = iend;
+        while (up != lo) {
+            while (up < lo && val < a[lo - 1]) lo--;
+            if (up != lo) move_el(p, up - 1, lo - 1);
+            while (up < lo && val >= a[up - 1]) up++;
+            if (up != lo) move_el(p, lo - 1, up - 1);
+        }
+        int mid = up;
+        if (hkey > val) mid = up - 1;
+        p->x[up - 1] = hx; p->y[up - 1] = hy; p->z[up - 1] = hz; p->q[up - 1] = hq;
+        if (p->w) p->w[up - 1] = hw;
+        p->order[up - 1] = ho;
+        return mid;
+    } else if (ibeg == iend) {
+        return a[ibeg - 1] < val ? ibeg : ibeg - 1;
+    }
+    return ibeg - 1;
+}
+
+static double max3d(double a, double b, double c)
+{
+    double m = a;
+    if (m < b) m = b;
+    if (m < c) m = c;
+    return m;
+}
+
+/* src/tree/partition.c:178-286 (pc_partition_8 / cp_partition_8) */
+static int split_8(Parts *p, double xl, double yl, double zl, int maxch,
+                   double xm, double ym, double zm, int ind[8][2])
+{
+    int n = 1;
+    double crit = max3d(xl, yl, zl) / sqrt(2.0);
+    int dx = xl >= crit, dy = yl >= crit, dz = zl >= crit;
+    if (maxch == 4) {
+        if (xl < yl && xl < zl) dx = 0;
+        if (yl < xl && yl < zl) dy = 0;
+        if (zl < xl && zl < yl) dz = 0;
+        if (dx + dy + dz == 3) dx = 0;
+    }
+    if (maxch == 2) {
+        if (xl < yl || xl < zl) dx = 0;
+        if (yl < xl || yl < zl) dy = 0;
+        if (zl < xl || zl < yl) dz = 0;
+        if (dx + dy + dz == 3) { dx = 0; dy = 0; }
+        if (dx + dy + dz == 2) {
+            if (dx == 0) dy = 0;
+            else if (dy == 0) dz = 0;
+            else if (dz == 0) dx = 0;
+        }
+    }
+    const int div[3] = {dx, dy, dz};
+    const double mid[3] = {xm, ym, zm};
+    for (int d = 0; d < 3; d++) {
+        if (!div[d]) continue;
+        for (int i = 0; i < n; i++) {
+            int t = split_range(p, d, ind[i][0], ind[i][1], mid[d]);
+            ind[n + i][0] = t + 1;
+            ind[n + i][1] = ind[i][1];
+            ind[i][1] = t;
+        }
+        n *= 2;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Tree: src/tree/tree_linked_list.c:16-280, tree.c:19-158, 235-268    */
+/* Nodes are stored in creation order, which is the preorder that      */
+/* TreeLinkedList_SetIndex assigns.                                    */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n, cap;
+    int *ibeg, *iend, *numpar, *nchild, *child, *level;
+    double *xmin, *xmax, *ymin, *ymax, *zmin, *zmax, *xmid, *ymid, *zmid, *radius;
+    int max_depth, numleaves;
+    int *leaves, nleaves;
+    int **lev, *lev_n, *lev_cap;
+    int leaves_only; /* batches: keep only leaves */
+} Tree;
+
+static int tree_grow(Tree *t)
+{
+    if (t->n < t->cap) return 0;
+    int nc = t->cap ? t->cap * 2 : 64;
+#define GROW(f, T) do { void *q_ = realloc(t->f, (size_t)nc * sizeof(T)); if (!q_) return ORC_ERR_NOMEM; t->f = q_; } while (0)
+    GROW(ibeg, int); GROW(iend, int); GROW(numpar, int); GROW(nchild, int); GROW(level, int);
+    { void *q_ = realloc(t->child, (size_t)nc * 8 * sizeof(int)); if (!q_) return ORC_ERR_NOMEM; t->child = q_; }
+    GROW(xmin, double); GROW(xmax, double); GROW(ymin, double); GROW(ymax, double);
+    GROW(zmin, double); GROW(zmax, double); GROW(xmid, double); GROW(ymid, double);
+    GROW(zmid, double); GROW(radius, double);
+#undef GROW
+    t->cap = nc;
+    return 0;
+}
+
+static void tree_free(Tree *t)
+{
+    free(t->ibeg); free(t->iend); free(t->numpar); free(t->nchild); free(t->child); free(t->level);
+    free(t->xmin); free(t->xmax); free(t->ymin); free(t->ymax); free(t->zmin); free(t->zmax);
+    free(t->xmid); free(t->ymid); free(t->zmid); free(t->radius); free(t->leaves);
+    if (t->lev) { for (int i = 0; i < t->max_depth; i++) free(t->lev[i]); }
+    free(t->lev); free(t->lev_n); free(t->lev_cap);
+    memset(t, 0, sizeof(*t));
+}
+
+static double minv(const double *x, int n) { double m = x[0]; for (int i = 1; i < n; ++i) if (m > x[i]) m = x[i]; return m; }
+static double maxv(const double *x, int n) { double m = x[0]; for (int i = 1; i < n; ++i) if (m < x[i]) m = x[i]; return m; }
+
+/* One node (or, for batches, possibly a leaf record): returns node index
+ * (>= 0), -1 when a batch interior node is not recorded, or an error. */
+static int tree_build_rec(Tree *t, Parts *p, int ibeg, int iend, int maxpar, int level)
+{
+    int numpar = iend - ibeg + 1;
+    double x_min = minv(p->x + ibeg - 1, numpar), x_max = maxv(p->x + ibeg - 1, numpar);
+    double y_min = minv(p->y + ibeg - 1, numpar), y_max = maxv(p->y + ibeg - 1, numpar);
+    double z_min = minv(p->z + ibeg - 1, numpar), z_max = maxv(p->z + ibeg - 1, numpar);
+    double xl = x_max - x_min, yl = y_max - y_min, zl = z_max - z_min;
+    double x_mid = (x_max + x_min) / 2.0, y_mid = (y_max + y_min) / 2.0, z_mid = (z_max + z_min) / 2.0;
+    double radius = sqrt(xl * xl + yl * yl + zl * zl) / 2.0;
+    int is_leaf = !(numpar > maxpar);
+    if (level + 1 > t->max_depth) t->max_depth = level + 1;
+
+    int idx = -1;
+    if (!t->leaves_only || is_leaf) {
+        int e = tree_grow(t);
+        if (e) return e;
+        idx = t->n++;
+        t->ibeg[idx] = ibeg; t->iend[idx] = iend; t->numpar[idx] = numpar;
+        t->xmin[idx] = x_min; t->xmax[idx] = x_max; t->ymin[idx] = y_min;
+        t->ymax[idx] = y_max; t->zmin[idx] = z_min; t->zmax[idx] = z_max;
+        t->xmid[idx] = x_mid; t->ymid[idx] = y_mid; t->zmid[idx] = z_mid;
+        t->radius[idx] = radius; t->nchild[idx] = 0; t->level[idx] = level;
+    }
+    if (!is_leaf) {
+        int maxch = numpar < 2 * maxpar ? 2 : (numpar < 4 * maxpar ? 4 : 8);
+        int ind[8][2];
+        memset(ind, 0, sizeof(ind));
+        ind[0][0] = ibeg; ind[0][1] = iend;
+        int nposs = split_8(p, xl, yl, zl, maxch, x_mid, y_mid, z_mid, ind);
+        for (int i = 0; i < nposs; i++) {
+            if (ind[i][0] <= ind[i][1]) {
+                int c = tree_build_rec(t, p, ind[i][0], ind[i][1], maxpar, level + 1);
+                if (c < -1) return c;
+                if (idx >= 0) t->child[8 * idx + t->nchild[idx]++] = c;
+            }
+        }
+    } else {
+        t->numleaves++;
+    }
+    return idx;
+}
+
+static int push_int(int **arr, int *n, int *cap, int v)
+{
+    if (*n >= *cap) {
+        int nc = *cap ? *cap * 2 : 16;
+        int *q = realloc(*arr, (size_t)nc * sizeof(int));
+        if (!q) return ORC_ERR_NOMEM;
+        *arr = q; *cap = nc;
+    }
+    (*arr)[(*n)++] = v;
+    return 0;
+}
+
+/* src/tree/tree.c:106-158 */
+static int tree_levels_rec(Tree *t, int idx, int level, int *leaves_cap)
+{
+    if (t->nchild[idx] == 0) return push_int(&t->leaves, &t->nleaves, leaves_cap, idx);
+    int e = push_int(&t->lev[level], &t->lev_n[level], &t->lev_cap[level], idx);
+    if (e) return e;
+    for (int i = 0; i < t->nchild[idx]; i++) {
+        e = tree_levels_rec(t, t->child[8 * idx + i], level + 1, leaves_cap);
+        if (e) return e;
+    }
+    return 0;
+}
+
+static int tree_construct(Tree *t, Parts *p, int maxpar, int leaves_only)
+{
+    memset(t, 0, sizeof(*t));
+    t->leaves_only = leaves_only;
+    if (p->n <= 0) return ORC_ERR_ARGS;
+    int r = tree_build_rec(t, p, 1, p->n, maxpar, 0);
+    if (r < -1) return r;
+    if (leaves_only) return 0;
+    t->lev = calloc((size_t)t->max_depth, sizeof(int *));
+    t->lev_n = calloc((size_t)t->max_depth, sizeof(int));
+    t->lev_cap = calloc((size_t)t->max_depth, sizeof(int));
+    if (!t->lev || !t->lev_n || !t->lev_cap) return ORC_ERR_NOMEM;
+    int leaves_cap = 0;
+    return tree_levels_rec(t, 0, 0, &leaves_cap);
+}
+
+/* ------------------------------------------------------------------ */
+/* Interaction lists: src/interaction_lists/interaction_lists.c:48-234, */
+/* 464-748.  Jagged lists keyed by the "target" side.                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct { int *v; int n, cap; } IList;
+
+static int il_push(IList *l, int v) { return push_int(&l->v, &l->n, &l->cap, v); }
+
+typedef struct {
+    int n;             /* number of keys (target nodes / batches) */
+    IList *pp, *ap;    /* direct, approx (pc / cp / cc)             */
+    IList *spc, *tcp;  /* CC only: source-approx (PC), target-approx (CP) */
+} Lists;
+
+static void lists_free(Lists *L)
+{
+    for (int i = 0; i < L->n; i++) {
+        free(L->pp[i].v); free(L->ap[i].v);
+        if (L->spc) free(L->spc[i].v);
+        if (L->tcp) free(L->tcp[i].v);
+    }
+    free(L->pp); free(L->ap); free(L->spc); free(L->tcp);
+    memset(L, 0, sizeof(*L));
+}
+
+/* interaction_lists.c:464-533 */
+static int pc_list_rec(const Tree *tr, int node, double br, double bx, double by, double bz,
+                       IList *ap, IList *pp, const Prm *prm)
+{
+    double tx = bx - tr->xmid[node];
+    double ty = by - tr->ymid[node];
+    double tz = bz - tr->zmid[node];
+    double dist = sqrt(tx * tx + ty * ty + tz * tz);
+    if (((tr->radius[node] + br) < dist * prm->theta) && (tr->radius[node] != 0.00)
+        && (prm->size_check_factor * prm->P < tr->numpar[node])) {
+        return il_push(ap, node);
+    }
+    if (tr->nchild[node] == 0) return il_push(pp, node);
+    for (int i = 0; i < tr->nchild[node]; i++) {
+        int e = pc_list_rec(tr, tr->child[8 * node + i], br, bx, by, bz, ap, pp, prm);
+        if (e) return e;
+    }
+    return 0;
+}
+
+/* interaction_lists.c:537-748 */
+static int cc_list_rec(const Tree *st, int sn, const Tree *tt, int tn, Lists *L, const Prm *prm)
+{
+    int size_check = (int)(prm->size_check_factor * prm->P);
+    double tx = tt->xmid[tn] - st->xmid[sn];
+    double ty = tt->ymid[tn] - st->ymid[sn];
+    double tz = tt->zmid[tn] - st->zmid[sn];
+    double dist = sqrt(tx * tx + ty * ty + tz * tz);
+    if ((st->radius[sn] + tt->radius[tn]) < dist * prm->theta) {
+        if (st->numpar[sn] <= size_check && tt->numpar[tn] <= size_check) return il_push(&L->pp[tn], sn);
+        if (st->numpar[sn] <= size_check) return il_push(&L->tcp[tn], sn);
+        if (tt->numpar[tn] <= size_check) return il_push(&L->spc[tn], sn);
+        return il_push(&L->ap[tn], sn);
+    }
+    int e = 0;
+    if (tt->nchild[tn] == 0 && st->nchild[sn] == 0) return il_push(&L->pp[tn], sn);
+    if (st->nchild[sn] == 0 || (tt->nchild[tn] != 0 && st->numpar[sn] < tt->numpar[tn])) {
+        for (int i = 0; i < tt->nchild[tn] && !e; i++)
+            e = cc_list_rec(st, sn, tt, tt->child[8 * tn + i], L, prm);
+    } else {
+        for (int i = 0; i < st->nchild[sn] && !e; i++)
+            e = cc_list_rec(st, st->child[8 * sn + i], tt, tn, L, prm);
+    }
+    return e;
+}
+
+static int lists_make(Lists *L, const Tree *src, const Tree *tgt, const Prm *prm)
+{
+    memset(L, 0, sizeof(*L));
+    L->n = tgt->n;
+    L->pp = calloc((size_t)L->n, sizeof(IList));
+    L->ap = calloc((size_t)L->n, sizeof(IList));
+    if (!L->pp || !L->ap) return ORC_ERR_NOMEM;
+    if (prm->compute_type == CT_CC) {
+        L->spc = calloc((size_t)L->n, sizeof(IList));
+        L->tcp = calloc((size_t)L->n, sizeof(IList));
+        if (!L->spc || !L->tcp) return ORC_ERR_NOMEM;
+        return cc_list_rec(src, 0, tgt, 0, L, prm);
+    }
+    for (int i = 0; i < tgt->n; i++) {
+        int e = pc_list_rec(src, 0, tgt->radius[i], tgt->xmid[i], tgt->ymid[i], tgt->zmid[i],
+                            &L->ap[i], &L->pp[i], prm);
+        if (e) return e;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Chebyshev nodes and barycentric weights: clusters.c:436-457          */
+/* ------------------------------------------------------------------ */
+
+static void cheb_nodes(int n, double a0, double a1, double *node)
+{
+    for (int i = 0; i < n + 1; i++) {
+        double tt = cos(i * M_PI / n);
+        node[i] = a0 + (tt + 1.0) / 2.0 * (a1 - a0);
+    }
+}
+
+static void bary_weights(int n, double *wts)
+{
+    for (int j = 0; j < n + 1; j++) {
+        double dj = 1.0;
+        if (j == 0) dj = 0.5;
+        if (j == n) dj = 0.5;
+        wts[j] = ((j % 2 == 0) ? 1 : -1) * dj;
+    }
+}
+
+typedef struct {
+    int numnodes, P;
+    double *x, *y, *z, *q, *w; /* w NULL unless subtraction */
+} Clus;
+
+static void clus_free(Clus *c) { free(c->x); free(c->y); free(c->z); free(c->q); free(c->w); memset(c, 0, sizeof(*c)); }
+
+static int clus_alloc(Clus *c, int numnodes, int P, int with_w)
+{
+    size_t n = (size_t)numnodes * (size_t)P;
+    c->numnodes = numnodes; c->P = P;
+    c->x = calloc(n, sizeof(double)); c->y = calloc(n, sizeof(double)); c->z = calloc(n, sizeof(double));
+    c->q = calloc(n, sizeof(double));
+    c->w = with_w ? calloc(n, sizeof(double)) : NULL;
+    if (!c->x || !c->y || !c->z || !c->q || (with_w && !c->w)) return ORC_ERR_NOMEM;
+    return 0;
+}
+
+/*
+ * Anterpolation of m "points" (particles, or a child's grid) onto the grid of
+ * the box [x0,x1]x[y0,y1]x[z0,z1]: clusters.c:800-1004 (leaf, skipping),
+ * :1008-1219 (leaf, subtraction), :379-582 and :585-795 (child to parent).
+ * f1 holds the charges (q, q*w or child Q); f2 (optional) the second plane
+ * (w or child W).  Grid coordinates are written to gx/gy/gz.
+ */
+static int anterp(int n, double x0, double x1, double y0, double y1, double z0, double z1,
+                  int m, const double *px, const double *py, const double *pz,
+                  const double *f1in, const double *f2in,
+                  double *gx, double *gy, double *gz, double *Q, double *W)
+{
+    int nl = n + 1, P = nl * nl * nl;
+    double nodeX[32], nodeY[32], nodeZ[32], wts[32];
+    double *f1 = malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    double *f2 = f2in ? malloc((size_t)(m > 0 ? m : 1) * sizeof(double)) : NULL;
+    int *ex = malloc((size_t)(m > 0 ? m : 1) * 3 * sizeof(int));
+    if (!f1 || (f2in && !f2) || !ex) { free(f1); free(f2); free(ex); return ORC_ERR_NOMEM; }
+    int *ey = ex + m, *ez = ex + 2 * m;
+    for (int j = 0; j < m; j++) {
+        f1[j] = f1in[j];
+        if (f2) f2[j] = f2in[j];
+        ex[j] = -1; ey[j] = -1; ez[j] = -1;
+    }
+    cheb_nodes(n, x0, x1, nodeX);
+    cheb_nodes(n, y0, y1, nodeY);
+    cheb_nodes(n, z0, z1, nodeZ);
+    bary_weights(n, wts);
+
+    for (int i = 0; i < m; i++) {
+        double sumX = 0.0, sumY = 0.0, sumZ = 0.0;
+        double sx = px[i], sy = py[i], sz = pz[i];
+        for (int j = 0; j < nl; j++) {
+            double cx = sx - nodeX[j], cy = sy - nodeY[j], cz = sz - nodeZ[j];
+            if (fabs(cx) < DBL_MIN) ex[i] = j;
+            if (fabs(cy) < DBL_MIN) ey[i] = j;
+            if (fabs(cz) < DBL_MIN) ez[i] = j;
+            double w = wts[j];
+            sumX += w / cx;
+            sumY += w / cy;
+            sumZ += w / cz;
+        }
+        double denominator = 1.0;
+        if (ex[i] == -1) denominator *= sumX;
+        if (ey[i] == -1) denominator *= sumY;
+        if (ez[i] == -1) denominator *= sumZ;
+        f1[i] /= denominator;
+        if (f2) f2[i] /= denominator;
+    }
+
+    for (int j = 0; j < P; j++) {
+        int k1 = j % nl;
+        int kk = (j - k1) / nl;
+        int k2 = kk % nl;
+        kk = kk - k2;
+        int k3 = kk / nl;
+        double cz = nodeZ[k3], w3 = wts[k3];
+        double cy = nodeY[k2], w2 = wts[k2];
+        double cx = nodeX[k1], w1 = wts[k1];
+        gx[j] = cx; gy[j] = cy; gz[j] = cz;
+        double temp = 0.0, temp2 = 0.0;
+        for (int i = 0; i < m; i++) {
+            double sx = px[i], sy = py[i], sz = pz[i];
+            double numerator = 1.0;
+            if (ex[i] == -1) numerator *= w1 / (sx - cx);
+            else if (ex[i] != k1) numerator *= 0;
+            if (ey[i] == -1) numerator *= w2 / (sy - cy);
+            else if (ey[i] != k2) numerator *= 0;
+            if (ez[i] == -1) numerator *= w3 / (sz - cz);
+            else if (ez[i] != k3) numerator *= 0;
+            temp += numerator * f1[i];
+            if (f2) temp2 += numerator * f2[i];
+        }
+        Q[j] += temp;
+        if (f2) W[j] += temp2;
+    }
+    free(f1); free(f2); free(ex);
+    return 0;
+}
+
+/* clusters.c:50-200 (Lagrange branches) */
+static int clusters_sources(Clus *c, const Tree *t, const Parts *s, const Prm *prm)
+{
+    int sub = prm->singularity == SNG_SUB;
+    int P = prm->P, n = prm->degree;
+    int e = clus_alloc(c, t->n, P, sub);
+    if (e) return e;
+    double *qw = NULL;
+    if (sub) {
+        qw = malloc((size_t)s->n * sizeof(double));
+        if (!qw) return ORC_ERR_NOMEM;
+        for (int i = 0; i < s->n; i++) qw[i] = s->q[i] * s->w[i];
+    }
+    for (int li = 0; li < t->nleaves && !e; li++) {
+        int idx = t->leaves[li];
+        int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
+        size_t cs = (size_t)idx * P;
+        e = anterp(n, t->xmin[idx], t->xmax[idx], t->ymin[idx], t->ymax[idx], t->zmin[idx], t->zmax[idx],
+                   m, s->x + b, s->y + b, s->z + b, sub ? qw + b : s->q + b, sub ? s->w + b : NULL,
+                   c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL);
+    }
+    for (int level = t->max_depth - 1; level >= 0 && !e; --level) {
+        for (int ci = 0; ci < t->lev_n[level] && !e; ++ci) {
+            int par = t->lev[level][ci];
+            size_t ps = (size_t)par * P;
+            for (int k = 0; k < t->nchild[par] && !e; ++k) {
+                int ch = t->child[8 * par + k];
+                size_t cs = (size_t)ch * P;
+                e = anterp(n, t->xmin[par], t->xmax[par], t->ymin[par], t->ymax[par], t->zmin[par], t->zmax[par],
+                           P, c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL,
+                           c->x + ps, c->y + ps, c->z + ps, c->q + ps, sub ? c->w + ps : NULL);
+            }
+        }
+    }
+    free(qw);
+    return e;
+}
+
+/* clusters.c:205-288, 1840-1908: grid coordinates of every target node;
+ * charges (and weights) start at zero. */
+static int clusters_targets(Clus *c, const Tree *t, const Prm *prm)
+{
+    int P = prm->P, n = prm->degree, nl = n + 1;
+    int e = clus_alloc(c, t->n, P, prm->singularity == SNG_SUB);
+    if (e) return e;
+    double nodeX[32], nodeY[32], nodeZ[32];
+    for (int idx = 0; idx < t->n; idx++) {
+        cheb_nodes(n, t->xmin[idx], t->xmax[idx], nodeX);
+        cheb_nodes(n, t->ymin[idx], t->ymax[idx], nodeY);
+        cheb_nodes(n, t->zmin[idx], t->zmax[idx], nodeZ);
+        size_t cs = (size_t)idx * P;
+        for (int j = 0; j < P; j++) {
+            int k1 = j % nl;
+            int kk = (j - k1) / nl;
+            int k2 = kk % nl;
+            kk = kk - k2;
+            int k3 = kk / nl;
+            c->x[cs + j] = nodeX[k1]; c->y[cs + j] = nodeY[k2]; c->z[cs + j] = nodeZ[k3];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Kernels: the coulomb, yukawa and regularized-coulomb dirs of src/kernels */
+/* ------------------------------------------------------------------ */
+
+/* Target particles [ts, ts+nt) against source particles [ss, ss+ns):
+ * K_*_PP and K_*_SS_PP (coulomb_pp.c:16-58, coulomb_ss_pp.c:18-57,
+ * yukawa_pp.c:19-54, yukawa_ss_pp.c:19-57, regularized-coulomb_pp.c:18-58,
+ * regularized-coulomb_ss_pp.c:19-56). */
+static void k_pp(const Prm *prm, int nt, int ns, int ts, int ss, const Parts *T, const Parts *S, double *pot)
+{
+    const double *sx = S->x, *sy = S->y, *sz = S->z, *sq = S->q, *sw = S->w;
+    int sub = prm->singularity == SNG_SUB;
+    for (int i = 0; i < nt; i++) {
+        int ii = ts + i;
+        double temporary_potential = 0.0;
+        double tx = T->x[ii], ty = T->y[ii], tz = T->z[ii], tq = T->q[ii];
+        for (int j = 0; j < ns; j++) {
+            int jj = ss + j;
+            double dx = tx - sx[jj], dy = ty - sy[jj], dz = tz - sz[jj];
+            if (prm->kernel == KER_COULOMB) {
+                if (!sub) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 > DBL_MIN) temporary_potential += sq[jj] / sqrt(r2);
+                } else {
+                    double k2 = prm->kp[0] * prm->kp[0];
+                    double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (r > DBL_MIN)
+                        temporary_potential += (sq[jj] - tq * exp(-r * r / k2)) * sw[jj] / r;
+                }
+            } else if (prm->kernel == KER_YUKAWA) {
+                double kappa = prm->kp[0];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (!sub) {
+                    if (r > DBL_MIN) temporary_potential += sq[jj] * exp(-kappa * r) / r;
+                } else {
+                    if (r > DBL_MIN)
+                        temporary_potential += (sq[jj] - tq) * sw[jj] * exp(-kappa * r) / r;
+                }
+            } else {
+                if (!sub) {
+                    double eps = prm->kp[0];
+                    double r2 = dx * dx + dy * dy + dz * dz + eps * eps;
+                    temporary_potential += sq[jj] / sqrt(r2);
+                } else {
+                    double alpha2 = prm->kp[0] * prm->kp[0];
+                    double epsilon2 = prm->kp[1] * prm->kp[1];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    temporary_potential += (sq[jj] - tq * exp(-r2 / alpha2)) * sw[jj] / sqrt(r2 + epsilon2);
+                }
+            }
+        }
+        pot[ii] += temporary_potential;
+    }
+}
+
+/* Target particles against a source cluster grid: K_*_PC_Lagrange and
+ * K_*_SS_PC_Lagrange (coulomb_pc.c:17-58, coulomb_ss_pc.c:19-56,
+ * yukawa_pc.c:19-53, yukawa_ss_pc.c:18-55, regularized-coulomb_pc.c:19-59,
+ * regularized-coulomb_ss_pc.c:19-56). */
+static void k_pc(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *T, const Clus *C, double *pot)
+{
+    int sub = prm->singularity == SNG_SUB;
+    for (int i = 0; i < nt; i++) {
+        int ii = ts + i;
+        double temporary_potential = 0.0;
+        double tx = T->x[ii], ty = T->y[ii], tz = T->z[ii], tq = T->q[ii];
+        for (int j = 0; j < P; j++) {
+            size_t jj = cs + j;
+            double dx = tx - C->x[jj], dy = ty - C->y[jj], dz = tz - C->z[jj];
+            if (prm->kernel == KER_COULOMB) {
+                if (!sub) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    temporary_potential += C->q[jj] / sqrt(r2);
+                } else {
+                    double k2 = prm->kp[0] * prm->kp[0];
+                    double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    temporary_potential += (C->q[jj] - tq * C->w[jj] * exp(-r * r / k2)) / r;
+                }
+            } else if (prm->kernel == KER_YUKAWA) {
+                double kappa = prm->kp[0];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (!sub) temporary_potential += C->q[jj] * exp(-kappa * r) / r;
+                else temporary_potential += (C->q[jj] - tq * C->w[jj]) * exp(-kappa * r) / r;
+            } else {
+                if (!sub) {
+                    double epsilon2 = prm->kp[0] * prm->kp[0];
+                    double r2 = dx * dx + dy * dy + dz * dz + epsilon2;
+                    temporary_potential += C->q[jj] / sqrt(r2);
+                } else {
+                    double alpha2 = prm->kp[0] * prm->kp[0];
+                    double epsilon2 = prm->kp[1] * prm->kp[1];
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    temporary_potential += (C->q[jj] - tq * C->w[jj] * exp(-r2 / alpha2)) / sqrt(r2 + epsilon2);
+                }
+            }
+        }
+        pot[ii] += temporary_potential;
+    }
+}
+
+/* Target cluster grid against source particles: K_*_CP_Lagrange and
+ * K_*_SS_CP_Lagrange (coulomb_cp.c:17-58, coulomb_ss_cp.c:20-68,
+ * yukawa_cp.c:19-60, yukawa_ss_cp.c:19-67, regularized-coulomb_cp.c:19-60). */
+static void k_cp(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *S, Clus *C)
+{
+    int sub = prm->singularity == SNG_SUB;
+    for (int i = 0; i < P; i++) {
+        double temporary_potential = 0.0, temporary_weight = 0.0;
+        double cx = C->x[cs + i], cy = C->y[cs + i], cz = C->z[cs + i];
+        for (int j = 0; j < ns; j++) {
+            int jj = ss + j;
+            double dx = cx - S->x[jj], dy = cy - S->y[jj], dz = cz - S->z[jj];
+            if (prm->kernel == KER_COULOMB) {
+                if (!sub) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    temporary_potential += S->q[jj] / sqrt(r2);
+                } else {
+                    double k2 = prm->kp[0] * prm->kp[0];
+                    double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    temporary_potential += S->q[jj] * S->w[jj] / r;
+                    temporary_weight += exp(-r * r / k2) * S->w[jj] / r;
+                }
+            } else if (prm->kernel == KER_YUKAWA) {
+                double kappa = prm->kp[0];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (!sub) {
+                    temporary_potential += S->q[jj] * exp(-kappa * r) / r;
+                } else {
+                    temporary_potential += S->q[jj] * S->w[jj] * exp(-kappa * r) / r;
+                    temporary_weight += S->w[jj] * exp(-kappa * r) / r;
+                }
+            } else {
+                double epsilon2 = prm->kp[0] * prm->kp[0];
+                double r2 = dx * dx + dy * dy + dz * dz + epsilon2;
+                temporary_potential += S->q[jj] / sqrt(r2);
+            }
+        }
+        C->q[cs + i] += temporary_potential;
+        if (sub) C->w[cs + i] += temporary_weight;
+    }
+}
+
+/* Target cluster grid against a source cluster grid: K_*_CP_Lagrange on
+ * cluster charges (interaction_compute_cc.c:121-127), K_Coulomb_SS_CC_Lagrange
+ * (coulomb_ss_cc.c:20-68), K_Yukawa_SS_CC_Lagrange (yukawa_ss_cc.c:19-67). */
+static void k_cc(const Prm *prm, int P, size_t ss, size_t ts, const Clus *S, Clus *C)
+{
+    int sub = prm->singularity == SNG_SUB;
+    for (int i = 0; i < P; i++) {
+        double temporary_potential = 0.0, temporary_weight = 0.0;
+        double cx = C->x[ts + i], cy = C->y[ts + i], cz = C->z[ts + i];
+        for (int j = 0; j < P; j++) {
+            size_t jj = ss + j;
+            double dx = cx - S->x[jj], dy = cy - S->y[jj], dz = cz - S->z[jj];
+            if (prm->kernel == KER_COULOMB) {
+                if (!sub) {
+                    double r2 = dx * dx + dy * dy + dz * dz;
+                    temporary_potential += S->q[jj] / sqrt(r2);
+                } else {
+                    double k2 = prm->kp[0] * prm->kp[0];
+                    double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    temporary_potential += S->q[jj] / r;
+                    temporary_weight += exp(-r * r / k2) * S->w[jj] / r;
+                }
+            } else if (prm->kernel == KER_YUKAWA) {
+                double kappa = prm->kp[0];
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (!sub) {
+                    temporary_potential += S->q[jj] * exp(-kappa * r) / r;
+                } else {
+                    temporary_potential += S->q[jj] * exp(-kappa * r) / r;
+                    temporary_weight += S->w[jj] * exp(-kappa * r) / r;
+                }
+            } else {
+                double epsilon2 = prm->kp[0] * prm->kp[0];
+                double r2 = dx * dx + dy * dy + dz * dz + epsilon2;
+                temporary_potential += S->q[jj] / sqrt(r2);
+            }
+        }
+        C->q[ts + i] += temporary_potential;
+        if (sub) C->w[ts + i] += temporary_weight;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Downpass: interaction_compute_downpass.c:37-142, 151-497, 502-856   */
+/* Evaluates the interpolant of the grid values (Q, W) of the box       */
+/* [x0,x1]x... at m points; adds to out1 (and out2).  When tq is given  */
+/* (subtraction, leaf to targets), the value interpolated is Q - tq*W. */
+/* ------------------------------------------------------------------ */
+
+static void interp_eval(int n, double x0, double x1, double y0, double y1, double z0, double z1,
+                        const double *gq, const double *gw, int m,
+                        const double *px, const double *py, const double *pz, const double *tq,
+                        double *out1, double *out2)
+{
+    int nl = n + 1, P = nl * nl * nl;
+    double nodeX[32], nodeY[32], nodeZ[32], wts[32];
+    cheb_nodes(n, x0, x1, nodeX);
+    cheb_nodes(n, y0, y1, nodeY);
+    cheb_nodes(n, z0, z1, nodeZ);
+    bary_weights(n, wts);
+    for (int i = 0; i < m; i++) {
+        double sumX = 0.0, sumY = 0.0, sumZ = 0.0;
+        double tx = px[i], ty = py[i], tz = pz[i];
+        int eix = -1, eiy = -1, eiz = -1;
+        for (int j = 0; j < nl; j++) {
+            double cx = tx - nodeX[j], cy = ty - nodeY[j], cz = tz - nodeZ[j];
+            if (fabs(cx) < DBL_MIN) eix = j;
+            if (fabs(cy) < DBL_MIN) eiy = j;
+            if (fabs(cz) < DBL_MIN) eiz = j;
+            double w = wts[j];
+            sumX += w / cx;
+            sumY += w / cy;
+            sumZ += w / cz;
+        }
+        double denominator = 1.0;
+        if (eix == -1) denominator /= sumX;
+        if (eiy == -1) denominator /= sumY;
+        if (eiz == -1) denominator /= sumZ;
+        double temp = 0.0, temp2 = 0.0;
+        double tqi = tq ? tq[i] : 0.0;
+        for (int j = 0; j < P; j++) {
+            int k1 = j % nl;
+            int kk = (j - k1) / nl;
+            int k2 = kk % nl;
+            kk = kk - k2;
+            int k3 = kk / nl;
+            double w3 = wts[k3], w2 = wts[k2], w1 = wts[k1];
+            double cx = nodeX[k1], cy = nodeY[k2], cz = nodeZ[k3];
+            double numerator = 1.0;
+            if (eix == -1) numerator *= w1 / (tx - cx);
+            else if (eix != k1) numerator *= 0;
+            if (eiy == -1) numerator *= w2 / (ty - cy);
+            else if (eiy != k2) numerator *= 0;
+            if (eiz == -1) numerator *= w3 / (tz - cz);
+            else if (eiz != k3) numerator *= 0;
+            if (tq) {
+                temp += numerator * denominator * (gq[j] - tqi * gw[j]);
+            } else {
+                temp += numerator * denominator * gq[j];
+                if (gw && out2) temp2 += numerator * denominator * gw[j];
+            }
+        }
+        out1[i] += temp;
+        if (!tq && gw && out2) out2[i] += temp2;
+    }
+}
+
+static void downpass(const Tree *t, Clus *c, const Parts *T, const Prm *prm, double *pot)
+{
+    int P = prm->P, n = prm->degree;
+    int sub = prm->singularity == SNG_SUB;
+    for (int level = 0; level < t->max_depth; ++level) {
+        for (int ci = 0; ci < t->lev_n[level]; ++ci) {
+            int par = t->lev[level][ci];
+            size_t ps = (size_t)par * P;
+            for (int k = 0; k < t->nchild[par]; ++k) {
+                int ch = t->child[8 * par + k];
+                size_t cs = (size_t)ch * P;
+                interp_eval(n, t->xmin[par], t->xmax[par], t->ymin[par], t->ymax[par], t->zmin[par], t->zmax[par],
+                            c->q + ps, sub ? c->w + ps : NULL, P, c->x + cs, c->y + cs, c->z + cs, NULL,
+                            c->q + cs, sub ? c->w + cs : NULL);
+            }
+        }
+    }
+    for (int li = 0; li < t->nleaves; ++li) {
+        int idx = t->leaves[li];
+        int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
+        size_t cs = (size_t)idx * P;
+        interp_eval(n, t->xmin[idx], t->xmax[idx], t->ymin[idx], t->ymax[idx], t->zmin[idx], t->zmax[idx],
+                    c->q + cs, sub ? c->w + cs : NULL, m, T->x + b, T->y + b, T->z + b,
+                    sub ? T->q + b : NULL, pot + b, NULL);
+    }
+}
+
+/* interaction_compute_correction.c:22-45 and the *_ss_correction.c kernels */
+static void ss_correction(const Prm *prm, const Parts *T, double *pot)
+{
+    if (prm->singularity != SNG_SUB) return;
+    double param = 0.0;
+    if (prm->kernel == KER_COULOMB) param = 2.0 * M_PI * prm->kp[0] * prm->kp[0];
+    else if (prm->kernel == KER_REGCOULOMB) param = 2.0 * M_PI * prm->kp[0] * prm->kp[0];
+    else if (prm->kernel == KER_YUKAWA) param = 4.0 * M_PI / prm->kp[0] / prm->kp[0];
+    for (int i = 0; i < T->n; i++) pot[i] += param * T->q[i];
+}
+
+/* Which (kernel, singularity, compute type) the reference dispatchers accept
+ * for the Lagrange approximation (interaction_compute_{pc,cp,cc}.c). */
+static int supported(const Prm *p)
+{
+    if (p->approximation != APX_LAGRANGE) return 0;
+    if (p->kernel != KER_COULOMB && p->kernel != KER_YUKAWA && p->kernel != KER_REGCOULOMB) return 0;
+    if (p->singularity != SNG_SKIP && p->singularity != SNG_SUB) return 0;
+    if (p->compute_type < CT_PC || p->compute_type > CT_CC) return 0;
+    if (p->kernel == KER_REGCOULOMB && p->singularity == SNG_SUB && p->compute_type != CT_PC) return 0;
+    if (p->degree < 1 || p->degree > 30) return 0;
+    return 1;
+}
+
+static void stats_counts(const Lists *L, const Tree *tgt, const Tree *src, const Prm *prm)
+{
+    long long P = prm->P;
+    for (int i = 0; i < L->n; i++) {
+        long long nt = tgt->numpar[i];
+        g_stats.num_pp += L->pp[i].n;
+        for (int j = 0; j < L->pp[i].n; j++) g_stats.pp_ptwise += nt * src->numpar[L->pp[i].v[j]];
+        if (prm->compute_type == CT_PC) {
+            g_stats.num_pc += L->ap[i].n;
+            for (int j = 0; j < L->ap[i].n; j++) {
+                g_stats.pc_ptwise += nt * P;
+                g_stats.pc_replaced += nt * src->numpar[L->ap[i].v[j]];
+            }
+        } else if (prm->compute_type == CT_CP) {
+            g_stats.num_cp += L->ap[i].n;
+            for (int j = 0; j < L->ap[i].n; j++) {
+                g_stats.cp_ptwise += nt * P;
+                g_stats.cp_replaced += nt * src->numpar[L->ap[i].v[j]];
+            }
+        } else {
+            g_stats.num_cc += L->ap[i].n;
+            g_stats.num_pc += L->spc[i].n;
+            g_stats.num_cp += L->tcp[i].n;
+            for (int j = 0; j < L->ap[i].n; j++) {
+                g_stats.cc_ptwise += P * P;
+                g_stats.cc_replaced += nt * src->numpar[L->ap[i].v[j]];
+            }
+            for (int j = 0; j < L->spc[i].n; j++) {
+                g_stats.pc_ptwise += nt * P;
+                g_stats.pc_replaced += nt * src->numpar[L->spc[i].v[j]];
+            }
+            for (int j = 0; j < L->tcp[i].n; j++) {
+                g_stats.cp_ptwise += P * src->numpar[L->tcp[i].v[j]];
+                g_stats.cp_replaced += nt * src->numpar[L->tcp[i].v[j]];
+            }
+        }
+    }
+}
+
+int orc_treecode(int M, int N,
+                 const double *tx, const double *ty, const double *tz, const double *tq,
+                 const double *sx, const double *sy, const double *sz, const double *sq,
+                 const double *sw, double *out,
+                 int kernel, int nkp, const double *kp,
+                 int singularity, int approximation, int compute_type,
+                 double theta, int degree, int msl, int mtl,
+                 double size_check, double beta, int verbosity)
+{
+    (void)verbosity;
+    memset(&g_stats, 0, sizeof(g_stats));
+    if (M <= 0 || N <= 0 || !out || !tx || !ty || !tz || !sx || !sy || !sz || !sq) return ORC_ERR_ARGS;
+    memset(out, 0, (size_t)M * sizeof(double));
+    Prm prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.kernel = kernel; prm.singularity = singularity; prm.approximation = approximation;
+    prm.compute_type = compute_type;
+    for (int i = 0; i < nkp && i < 4; i++) prm.kp[i] = kp[i];
+    orc_params_resolve(compute_type, approximation, theta, degree, msl, mtl, size_check, beta,
+                       &prm.theta, &prm.degree, &prm.size_check_factor, &prm.msl, &prm.mtl);
+    prm.P = (prm.degree + 1) * (prm.degree + 1) * (prm.degree + 1);
+    if (!supported(&prm)) return ORC_ERR_UNSUPPORTED;
+    if (singularity == SNG_SUB && (!sw || !tq)) return ORC_ERR_ARGS;
+    if (prm.msl < 1 || prm.mtl < 1) return ORC_ERR_ARGS;
+    g_stats.interp_degree = prm.degree; g_stats.pts_per_cluster = prm.P; g_stats.theta = prm.theta;
+    g_stats.max_per_source_leaf = prm.msl; g_stats.max_per_target_leaf = prm.mtl;
+
+    int rc = 0;
+    double t0 = wall();
+    Parts S, T;
+    memset(&S, 0, sizeof(S)); memset(&T, 0, sizeof(T));
+    if ((rc = parts_copy(&S, N, sx, sy, sz, sq, sw, 1))) goto done0;
+    if ((rc = parts_copy(&T, M, tx, ty, tz, tq, NULL, 0))) goto done0;
+    double *pot = calloc((size_t)M, sizeof(double));
+    if (!pot) { rc = ORC_ERR_NOMEM; goto done0; }
+
+    Tree src, tgt;
+    Clus sc, tc;
+    Lists L;
+    memset(&src, 0, sizeof(src)); memset(&tgt, 0, sizeof(tgt));
+    memset(&sc, 0, sizeof(sc)); memset(&tc, 0, sizeof(tc)); memset(&L, 0, sizeof(L));
+    int P = prm.P;
+    int stride = g_stride > 0 ? g_stride : 1;
+
+    if (compute_type == CT_PC) {
+        /* treedriver.c:350-627 */
+        if ((rc = tree_construct(&src, &S, prm.msl, 0))) goto done;
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 1))) goto done;
+        g_stats.t_setup = wall() - t0; t0 = wall();
+        if ((rc = clusters_sources(&sc, &src, &S, &prm))) goto done;
+        g_stats.t_upward = wall() - t0; t0 = wall();
+        if ((rc = lists_make(&L, &src, &tgt, &prm))) goto done;
+        g_stats.t_lists = wall() - t0; t0 = wall();
+        stats_counts(&L, &tgt, &src, &prm);
+        for (int i = 0; i < tgt.n; i++) {
+            if (i % stride) continue;
+            int bs = tgt.ibeg[i] - 1, nt = tgt.iend[i] - tgt.ibeg[i] + 1;
+            for (int j = 0; j < L.ap[i].n; j++)
+                k_pc(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, bs, src.ibeg[nd] - 1, &T, &S, pot);
+            }
+        }
+        g_stats.t_compute = wall() - t0; t0 = wall();
+    } else if (compute_type == CT_CP) {
+        /* treedriver.c:114-339 */
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 0))) goto done;
+        if ((rc = tree_construct(&src, &S, prm.msl, 1))) goto done;
+        if ((rc = clusters_targets(&tc, &tgt, &prm))) goto done;
+        g_stats.t_setup = wall() - t0; t0 = wall();
+        if ((rc = lists_make(&L, &tgt, &src, &prm))) goto done;
+        g_stats.t_lists = wall() - t0; t0 = wall();
+        stats_counts(&L, &src, &tgt, &prm);
+        for (int i = 0; i < src.n; i++) {
+            if (i % stride) continue;
+            int bs = src.ibeg[i] - 1, ns = src.iend[i] - src.ibeg[i] + 1;
+            for (int j = 0; j < L.ap[i].n; j++)
+                k_cp(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, tgt.iend[nd] - tgt.ibeg[nd] + 1, ns, tgt.ibeg[nd] - 1, bs, &T, &S, pot);
+            }
+        }
+        g_stats.t_compute = wall() - t0; t0 = wall();
+        downpass(&tgt, &tc, &T, &prm, pot);
+        g_stats.t_downpass = wall() - t0; t0 = wall();
+    } else {
+        /* treedriver.c:638-967 */
+        if ((rc = tree_construct(&src, &S, prm.msl, 0))) goto done;
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 0))) goto done;
+        g_stats.t_setup = wall() - t0; t0 = wall();
+        if ((rc = clusters_sources(&sc, &src, &S, &prm))) goto done;
+        if ((rc = clusters_targets(&tc, &tgt, &prm))) goto done;
+        g_stats.t_upward = wall() - t0; t0 = wall();
+        if ((rc = lists_make(&L, &src, &tgt, &prm))) goto done;
+        g_stats.t_lists = wall() - t0; t0 = wall();
+        stats_counts(&L, &tgt, &src, &prm);
+        /* interaction_compute_cc.c:87-1205 */
+        for (int i = 0; i < tgt.n; i++) {
+            if (i % stride) continue;
+            int ts = tgt.ibeg[i] - 1, nt = tgt.iend[i] - tgt.ibeg[i] + 1;
+            size_t tcs = (size_t)P * i;
+            for (int j = 0; j < L.ap[i].n; j++)
+                k_cc(&prm, P, (size_t)P * L.ap[i].v[j], tcs, &sc, &tc);
+            for (int j = 0; j < L.spc[i].n; j++)
+                k_pc(&prm, nt, P, ts, (size_t)P * L.spc[i].v[j], &T, &sc, pot);
+            for (int j = 0; j < L.tcp[i].n; j++) {
+                int nd = L.tcp[i].v[j];
+                k_cp(&prm, src.iend[nd] - src.ibeg[nd] + 1, P, src.ibeg[nd] - 1, tcs, &S, &tc);
+            }
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, ts, src.ibeg[nd] - 1, &T, &S, pot);
+            }
+        }
+        g_stats.t_compute = wall() - t0; t0 = wall();
+        downpass(&tgt, &tc, &T, &prm, pot);
+        g_stats.t_downpass = wall() - t0; t0 = wall();
+    }
+    g_stats.source_nodes = src.n; g_stats.target_nodes = tgt.n;
+    g_stats.source_leaves = src.numleaves; g_stats.target_leaves = tgt.numleaves;
+    g_stats.max_depth_source = src.max_depth; g_stats.max_depth_target = tgt.max_depth;
+    g_stats.batches = compute_type == CT_PC ? tgt.n : (compute_type == CT_CP ? src.n : 0);
+
+    /* treedriver.c:589-592: correction, then restore the caller's order */
+    ss_correction(&prm, &T, pot);
+    for (int i = 0; i < M; i++) out[T.order[i] - 1] = pot[i];
+    g_stats.t_finish = wall() - t0;
+
+done:
+    lists_free(&L);
+    clus_free(&sc); clus_free(&tc);
+    tree_free(&src); tree_free(&tgt);
+    free(pot);
+done0:
+    parts_free(&S); parts_free(&T);
+    return rc;
+}
+
+int orc_direct(int M, int N,
+               const double *tx, const double *ty, const double *tz, const double *tq,
+               const double *sx, const double *sy, const double *sz, const double *sq,
+               const double *sw, double *out,
+               int kernel, int nkp, const double *kp, int singularity)
+{
+    if (M <= 0 || N <= 0 || !out) return ORC_ERR_ARGS;
+    Prm prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.kernel = kernel; prm.singularity = singularity;
+    for (int i = 0; i < nkp && i < 4; i++) prm.kp[i] = kp[i];
+    if (kernel < KER_COULOMB || kernel > KER_REGCOULOMB) return ORC_ERR_UNSUPPORTED;
+    if (singularity != SNG_SKIP && singularity != SNG_SUB) return ORC_ERR_UNSUPPORTED;
+    if (singularity == SNG_SUB && (!sw || !tq)) return ORC_ERR_ARGS;
+    Parts S, T;
+    memset(&S, 0, sizeof(S)); memset(&T, 0, sizeof(T));
+    int rc = parts_copy(&S, N, sx, sy, sz, sq, sw, 1);
+    if (!rc) rc = parts_copy(&T, M, tx, ty, tz, tq, NULL, 0);
+    if (!rc) {
+        memset(out, 0, (size_t)M * sizeof(double));
+        /* interaction_compute_direct.c:62-218: one PP call over all pairs */
+        k_pp(&prm, M, N, 0, 0, &T, &S, out);
+        /* directdriver.c:117 */
+        ss_correction(&prm, &T, out);
+    }
+    parts_free(&S); parts_free(&T);
+    return rc;
+}
+
+void orc_last_stats(orc_stats *s) { *s = g_stats; }
+
+void orc_set_sample_stride(int stride) { g_stride = stride > 0 ? stride : 1; }
+
+int orc_source_clusters(int n, const double *x, const double *y, const double *z,
+                        const double *q, const double *w, int degree, int max_leaf,
+                        int singularity, double *cx, double *cy, double *cz,
+                        double *q_out, double *w_out)
+{
+    if (n <= 0 || degree < 1 || degree > 30 || max_leaf < 1) return ORC_ERR_ARGS;
+    Prm prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.singularity = singularity; prm.degree = degree;
+    prm.P = (degree + 1) * (degree + 1) * (degree + 1);
+    Parts S;
+    memset(&S, 0, sizeof(S));
+    int rc = parts_copy(&S, n, x, y, z, q, w, 1);
+    Tree t;
+    Clus c;
+    memset(&t, 0, sizeof(t)); memset(&c, 0, sizeof(c));
+    if (!rc) rc = tree_construct(&t, &S, max_leaf, 0);
+    if (!rc && q_out) rc = clusters_sources(&c, &t, &S, &prm);
+    if (!rc) {
+        rc = t.n;
+        if (q_out) {
+            size_t tot = (size_t)t.n * prm.P;
+            memcpy(q_out, c.q, tot * sizeof(double));
+            if (cx) memcpy(cx, c.x, tot * sizeof(double));
+            if (cy) memcpy(cy, c.y, tot * sizeof(double));
+            if (cz) memcpy(cz, c.z, tot * sizeof(double));
+            if (w_out && c.w) memcpy(w_out, c.w, tot * sizeof(double));
+        }
+    }
+    clus_free(&c); tree_free(&t); parts_free(&S);
+    return rc;
+}

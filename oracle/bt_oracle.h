@@ -1,0 +1,95 @@
+/*
+ * bt_oracle.h -- CPU restatement of BaryTree's FP64 interaction path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in barytree_amd/.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  The product library never links,
+ * loads or calls anything here.
+ *
+ * Each function follows the reference's operation order so that results are
+ * bit-identical to the reference CPU build where the reference is compiled
+ * with the same libm (glibc) and without FMA contraction.
+ *
+ * Parity pins (see tests/test_oracle_*.py):
+ *   - closed forms of tests/serial_tests.c:25-197 (direct sum, 10 particles);
+ *   - the two bit-exact known-answer values of
+ *     interfaces/python/testBaryTreeInterface.py:48,69 (Yukawa-SS, PC);
+ *   - bit equality against the reference's own MPI-free sources
+ *     (interaction lists, partition, tree linked list, all interaction
+ *     compute dispatchers and kernels, downpass) compiled into oracle/_ref;
+ *   - invariants: sum of modified charges equals sum of charges, coverage
+ *     of every (target, source) pair exactly once, permutation restore.
+ */
+#ifndef BT_ORACLE_H
+#define BT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Interaction-count diagnostics, as treedriver.c:191-200, 472-483, 763-792
+ * define them (list entries and point-wise evaluations). */
+typedef struct {
+    long long num_pp, num_pc, num_cp, num_cc;
+    long long pp_ptwise, pc_ptwise, cp_ptwise, cc_ptwise;
+    long long pc_replaced, cp_replaced, cc_replaced;
+    int source_nodes, source_leaves, target_nodes, target_leaves, batches;
+    int max_depth_source, max_depth_target;
+    int interp_degree, pts_per_cluster;
+    double theta;
+    int max_per_source_leaf, max_per_target_leaf;
+    double t_setup, t_upward, t_lists, t_compute, t_downpass, t_finish;
+} orc_stats;
+
+/* BaryTreeInterface semantics (src/interface/BaryTreeInterface.c:18-62) with
+ * an int return instead of exit(1).  Caller arrays are not modified.
+ * Returns 0 on success, a negative code on an unsupported combination or a
+ * missing array. */
+int orc_treecode(int num_targets, int num_sources,
+                 const double *tx, const double *ty, const double *tz, const double *tq,
+                 const double *sx, const double *sy, const double *sz, const double *sq,
+                 const double *sw, double *out,
+                 int kernel, int num_kernel_params, const double *kernel_params,
+                 int singularity, int approximation, int compute_type,
+                 double theta, int degree, int max_per_source_leaf, int max_per_target_leaf,
+                 double size_check, double beta, int verbosity);
+
+/* directdriver semantics (src/drivers/directdriver.c:23-122 at one rank):
+ * full FP64 sum over all sources for every target, then the subtraction
+ * correction.  out is overwritten. */
+int orc_direct(int num_targets, int num_sources,
+               const double *tx, const double *ty, const double *tz, const double *tq,
+               const double *sx, const double *sy, const double *sz, const double *sq,
+               const double *sw, double *out,
+               int kernel, int num_kernel_params, const double *kernel_params,
+               int singularity);
+
+/* Statistics of the last orc_treecode call on this thread. */
+void orc_last_stats(orc_stats *s);
+
+/* Evaluate only every stride-th target batch / node in the interaction
+ * phase (the upward pass and the downpass still run in full).  Used only to
+ * time a bounded sample of a large problem; 1 restores the full sum. */
+void orc_set_sample_stride(int stride);
+
+/* RunParams_Setup semantics (src/run_params/run_params.c:13-131): writes
+ * theta, degree, size_check_factor, source and target leaf sizes. */
+void orc_params_resolve(int compute_type, int approximation, double theta, int degree,
+                        int max_per_source_leaf, int max_per_target_leaf, double size_check,
+                        double beta, double *theta_out, int *degree_out,
+                        double *size_check_out, int *msl_out, int *mtl_out);
+
+/* Source-cluster upward pass alone: builds the source tree of (x,y,z,q,w)
+ * and returns the modified charges (and weights for subtraction) of every
+ * node, numnodes*(degree+1)^3 values each, in node (preorder) order.
+ * Returns numnodes, or a negative code.  q_out / w_out may be NULL to query
+ * the node count only. */
+int orc_source_clusters(int n, const double *x, const double *y, const double *z,
+                        const double *q, const double *w, int degree, int max_leaf,
+                        int singularity, double *cx, double *cy, double *cz,
+                        double *q_out, double *w_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,154 @@
+"""ctypes front end of the CPU oracle (libbt_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module; the product package (barytree_amd) never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbt_oracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libbt_refparts.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_longlong) for n in (
+        "num_pp", "num_pc", "num_cp", "num_cc",
+        "pp_ptwise", "pc_ptwise", "cp_ptwise", "cc_ptwise",
+        "pc_replaced", "cp_replaced", "cc_replaced")] + [
+        (n, ctypes.c_int) for n in (
+            "source_nodes", "source_leaves", "target_nodes", "target_leaves", "batches",
+            "max_depth_source", "max_depth_target", "interp_degree", "pts_per_cluster")] + [
+        ("theta", ctypes.c_double),
+        ("max_per_source_leaf", ctypes.c_int), ("max_per_target_leaf", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in (
+            "t_setup", "t_upward", "t_lists", "t_compute", "t_downpass", "t_finish")]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+def build(ref=False):
+    """Compile the oracle (and, with ref=True, the reference-parts library)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"] + (["ref"] if ref else []), check=True)
+
+
+_LIBS = {}
+
+
+def _load(path):
+    if path not in _LIBS:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        lib = ctypes.CDLL(path)
+        _LIBS[path] = lib
+    return _LIBS[path]
+
+
+def lib():
+    l = _load(LIB_PATH)
+    l.orc_treecode.restype = ctypes.c_int
+    l.orc_direct.restype = ctypes.c_int
+    return l
+
+
+def ref_available():
+    return os.path.exists(REF_LIB_PATH)
+
+
+def ref_lib():
+    l = _load(REF_LIB_PATH)
+    l.ref_treecode.restype = ctypes.c_int
+    return l
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _call_tree(fn, tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, kernel_params, singularity,
+               approximation, compute_type, theta, degree, max_source_leaf, max_target_leaf,
+               size_check, beta):
+    tx, ty, tz, tq, sx, sy, sz, sq, sw = map(_c, (tx, ty, tz, tq, sx, sy, sz, sq, sw))
+    kp = np.ascontiguousarray(np.atleast_1d(np.asarray(kernel_params, dtype=np.float64)))
+    out = np.zeros(len(tx))
+    rc = fn(ctypes.c_int(len(tx)), ctypes.c_int(len(sx)),
+            _p(tx), _p(ty), _p(tz), _p(tq), _p(sx), _p(sy), _p(sz), _p(sq), _p(sw), _p(out),
+            ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _p(kp),
+            ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+            ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+            ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+            ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(0))
+    if rc != 0:
+        raise RuntimeError(f"oracle returned {rc}")
+    return out
+
+
+def treecode(tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, kernel_params, singularity,
+             approximation, compute_type, theta, degree, max_source_leaf, max_target_leaf,
+             size_check=1.0, beta=-1.0):
+    """BaryTreeInterface semantics on the CPU (FP64, serial)."""
+    return _call_tree(lib().orc_treecode, tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel,
+                      kernel_params, singularity, approximation, compute_type, theta, degree,
+                      max_source_leaf, max_target_leaf, size_check, beta)
+
+
+def ref_treecode(tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, kernel_params, singularity,
+                 approximation, compute_type, theta, degree, max_source_leaf, max_target_leaf,
+                 size_check=1.0, beta=-1.0):
+    """Same call through the reference's own compiled sources (oracle/_ref)."""
+    return _call_tree(ref_lib().ref_treecode, tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel,
+                      kernel_params, singularity, approximation, compute_type, theta, degree,
+                      max_source_leaf, max_target_leaf, size_check, beta)
+
+
+def direct(tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, kernel_params, singularity):
+    """directdriver semantics: FP64 direct sum plus subtraction correction."""
+    tx, ty, tz, tq, sx, sy, sz, sq, sw = map(_c, (tx, ty, tz, tq, sx, sy, sz, sq, sw))
+    kp = np.ascontiguousarray(np.atleast_1d(np.asarray(kernel_params, dtype=np.float64)))
+    out = np.zeros(len(tx))
+    rc = lib().orc_direct(ctypes.c_int(len(tx)), ctypes.c_int(len(sx)),
+                          _p(tx), _p(ty), _p(tz), _p(tq), _p(sx), _p(sy), _p(sz), _p(sq),
+                          _p(sw), _p(out), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)),
+                          _p(kp), ctypes.c_int(int(singularity)))
+    if rc != 0:
+        raise RuntimeError(f"oracle direct returned {rc}")
+    return out
+
+
+def last_stats():
+    s = Stats()
+    lib().orc_last_stats(ctypes.byref(s))
+    return s.as_dict()
+
+
+def set_sample_stride(stride):
+    lib().orc_set_sample_stride(ctypes.c_int(int(stride)))
+
+
+def source_clusters(x, y, z, q, w, degree, max_leaf, singularity):
+    """Upward pass alone: (cx, cy, cz, Q, W) per node, each (numnodes, P)."""
+    x, y, z, q, w = map(_c, (x, y, z, q, w))
+    l = lib()
+    nn = l.orc_source_clusters(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
+                               singularity, None, None, None, None, None)
+    if nn < 0:
+        raise RuntimeError(f"oracle returned {nn}")
+    P = (degree + 1) ** 3
+    bufs = [np.zeros(nn * P) for _ in range(5)]
+    nn2 = l.orc_source_clusters(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
+                                singularity, *[_p(b) for b in bufs])
+    assert nn2 == nn
+    return [b.reshape(nn, P) for b in bufs]
