@@ -1,0 +1,59 @@
+"""Builds libBaryTree_gpu.so in-tree for gfx950.
+
+Host C++ (trees, lists, orchestration, C ABI) is compiled with g++ and
+-ffp-contract=off, so the tree boxes and Chebyshev node coordinates are the
+reference's unfused IEEE results bit for bit; the kernels are compiled with
+hipcc for gfx950.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libBaryTree_gpu.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = os.environ.get("BARYTREE_ARCH", "gfx950")
+
+HOST_SRCS = ["bt_host.cpp", "bt_solver.cpp", "bt_capi.cpp"]
+HIP_SRCS = ["bt_kernels.hip"]
+
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function"]
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+             # packed FP32 instructions stay off
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-munsafe-fp-atomics"]
+
+
+def run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} {cmd[-1]}")
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for s in HOST_SRCS:
+        o = os.path.join(BUILD, s + ".o")
+        run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in HIP_SRCS:
+        o = os.path.join(BUILD, s + ".o")
+        run([f"{ROCM}/bin/hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    run([f"{ROCM}/bin/hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+         f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + f"{ROCM}/lib"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

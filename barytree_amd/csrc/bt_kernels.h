@@ -1,0 +1,89 @@
+// bt_kernels.h -- launch interface of the gfx950 kernels (bt_kernels.hip).
+// Plain C++ declarations so that host code compiled without hipcc can call
+// them; every pointer here is a device pointer.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+namespace btk {
+
+// Work tile {key, first target, target count, node} and source range
+// {first, count}; 16- and 8-byte aligned so kernels load them whole.
+struct alignas(16) Tile { int x, y, z, w; };
+struct alignas(8) Seg { int x, y; };
+
+// Kernel parameters, pre-derived on the host.
+struct KParams {
+    double p0, p1;      // kernel_params[0], kernel_params[1]
+    double inv_a2;      // 1/kappa^2 (Coulomb SS), 1/alpha^2 (reg. Coulomb SS)
+    double eps2;        // epsilon^2 (regularized Coulomb)
+    double ss_param;    // subtraction correction factor
+};
+
+// Interaction work: a list of tiles {key, first target, target count, node}
+// against CSR lists of source particle ranges and source clusters.
+struct InteractArgs {
+    int kernel, singularity;
+    int grid_targets;        // 0: particle targets, 1: grid points of target nodes
+    int precision;           // 0: FP64, 1: FP32 pairs with compensated accumulation
+    int ntiles;
+    const Tile *tiles;
+    const int *seg_off;
+    const Seg *segs;        // {first source, count}
+    const int *clu_off, *clu_id;
+    int P, nl;
+    const double *tnode_tab; // target node tables (grid targets)
+    const double *tx, *ty, *tz, *tq;
+    const double *sx, *sy, *sz, *sq, *sw;
+    const double *cx, *cy, *cz, *cq, *cw;  // source cluster points and charges
+    KParams kp;
+    double *out1, *out2;     // potential (particles) or target-cluster Q and W
+};
+
+hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
+
+// Source grid point coordinates from node tables: c{x,y,z}[node*P + j].
+hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double *cx, double *cy,
+                                 double *cz, hipStream_t st);
+
+// Upward pass.  mode 0: one block per leaf in `nodes`, points = particles
+// [ibeg, iend); mode 1: one block per parent in `nodes`, points = the grids
+// of its children (child CSR child_off/child_id).
+struct AnterpArgs {
+    int mode, degree, subtraction, nblocks;
+    const int *nodes;
+    const int *ibeg, *iend;
+    const int *child_off, *child_id;
+    const double *tab;                                  // node tables
+    const double *px, *py, *pz, *pq, *pw;              // particles (mode 0)
+    double *Q, *W;                                      // cluster charges (in/out)
+};
+hipError_t launch_anterp(const AnterpArgs &a, hipStream_t st);
+
+// Downward pass.  mode 0: one block per child in `nodes`; the child grid
+// gets the interpolant of its parent's values.  mode 1: one block per tile
+// {leaf, first target, count} of `tiles`; targets get the leaf interpolant.
+struct DownArgs {
+    int mode, degree, subtraction, nblocks;
+    const int *nodes, *parent;
+    const Tile *tiles;
+    const double *tab;
+    const double *tx, *ty, *tz, *tq;
+    double *Q, *W;
+    double *pot;
+};
+hipError_t launch_down(const DownArgs &a, hipStream_t st);
+
+// Subtraction correction and scatter to the caller's order:
+// out[order[i]] = pot[i] + ss_param*tq[i] (ss_param = 0 without subtraction).
+hipError_t launch_finish(int n, const double *pot, const double *tq, double ss_param,
+                         const int *order, double *out, hipStream_t st);
+
+// FP64 direct sum (directdriver semantics) for sampled targets, no tree:
+// out[i] = sum_j G(t_i, s_j) (+ correction).  Used for error reporting.
+hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const double *tx,
+                         const double *ty, const double *tz, const double *tq, int n,
+                         const double *sx, const double *sy, const double *sz, const double *sq,
+                         const double *sw, double *out, hipStream_t st);
+
+}  // namespace btk

@@ -1,0 +1,810 @@
+// bt_kernels.hip -- gfx950 kernels of the BaryTree interaction path.
+//
+// Layout in HBM (all tree-ordered, SoA, FP64):
+//   targets  tx,ty,tz,tq[M]         sources sx,sy,sz,sq,sw[N]
+//   source clusters cx,cy,cz,cq,cw[numnodes*P]  (grid point j = k1 + nl*k2 + nl^2*k3)
+//   target clusters Q,W[numnodes*P]; node tables tab[numnodes*3*nl]
+// Work lists are CSR arrays built on the host (bt_solver.cpp).
+//
+// Kernel map to the reference (src/kernels, src/interaction_compute):
+//   k_interact<..., GRID=0>  : K_*_PP and K_*_PC_Lagrange (+SS) for a target leaf
+//   k_interact<..., GRID=1>  : K_*_CP_Lagrange, K_*_SS_CP/CC_Lagrange for a target grid
+//   k_anterp                 : pc_comp_ms_modifiedF(_SS)(_child_to_parent), clusters.c
+//   k_down                   : cp_comp_pot(_SS)(_parent_to_child), downpass.c
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "bt_kernels.h"
+
+namespace btk {
+
+namespace {
+
+constexpr int TPB = 256;  // threads per block of the interaction kernels (4 waves)
+constexpr int TPT = 2;    // targets per thread
+constexpr int KC = 1, KY = 2, KR = 3;
+
+// ---------------------------------------------------------------------
+// 1/sqrt(x) in FP64: hardware estimate (about 2^-23 relative) refined by
+// two Newton steps y <- y + y*(1/2 - (x*y)*(y/2)); about 1 ulp.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double rsq64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    y = fma(y, e, y);
+    g = x * y; h = 0.5 * y;
+    e = fma(-g, h, 0.5);
+    return fma(y, e, y);
+}
+
+// ---------------------------------------------------------------------
+// Pair kernels.  Source point carries (a, b):
+//   particles: a = q (skipping) or q*w (subtraction), b = w
+//   clusters : a = Q, b = W (subtraction weights)
+// Target-particle form (potential):
+//   Coulomb   SK: a/r  [r^2 > DBL_MIN]        SS: (a - tq*b*exp(-r^2/k^2))/r [r > DBL_MIN]
+//   Yukawa    SK: a e^{-kr}/r [r > DBL_MIN]   SS: (a - tq*b) e^{-kr}/r [r > DBL_MIN]
+//   RegCoul   SK: a/sqrt(r^2+e^2)             SS: (a - tq*b*exp(-r^2/al^2))/sqrt(r^2+e^2)
+// (coulomb_pp.c:48-49, coulomb_ss_pp.c:46-47, coulomb_pc.c:47-50, coulomb_ss_pc.c:47,
+//  yukawa_pp.c:44-45, yukawa_ss_pp.c:47-48, yukawa_ss_pc.c:46, regularized-coulomb_*.c)
+// The skip never triggers for cluster sources: an accepted cluster is
+// separated from its targets, so the reference's unskipped PC sum agrees.
+// ---------------------------------------------------------------------
+template <int KER, bool SS>
+__device__ __forceinline__ double pair_pot(double dx, double dy, double dz, double a, double b,
+                                           double tq, const KParams &kp)
+{
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        const double ir = rsq64(r2);
+        if constexpr (!SS) {
+            return r2 > DBL_MIN ? a * ir : 0.0;
+        } else {
+            const double r = r2 * ir;
+            const double v = fma(-tq * b, exp(-r2 * kp.inv_a2), a) * ir;
+            return r > DBL_MIN ? v : 0.0;
+        }
+    } else if constexpr (KER == KY) {
+        const double ir = rsq64(r2);
+        const double r = r2 * ir;
+        const double e = exp(-kp.p0 * r) * ir;
+        double v;
+        if constexpr (!SS) v = a * e;
+        else v = fma(-tq, b, a) * e;
+        return r > DBL_MIN ? v : 0.0;
+    } else {
+        const double ir = rsq64(r2 + kp.eps2);
+        if constexpr (!SS) return a * ir;
+        else return fma(-tq * b, exp(-r2 * kp.inv_a2), a) * ir;
+    }
+}
+
+// Target-grid form (accumulates Q and, for subtraction, W):
+//   Coulomb   SK: Q += a/r          SS: Q += a/r, W += exp(-r^2/k^2) b/r
+//   Yukawa    SK: Q += a e^{-kr}/r  SS: Q += a e^{-kr}/r, W += b e^{-kr}/r
+//   RegCoul   SK: Q += a/sqrt(r^2+e^2)
+// (coulomb_cp.c:46-55, coulomb_ss_cp.c:54-55, coulomb_ss_cc.c:54-55,
+//  yukawa_cp.c:52, yukawa_ss_cp.c:55-56, yukawa_ss_cc.c:55-56, regularized-coulomb_cp.c:52)
+template <int KER, bool SS>
+__device__ __forceinline__ void pair_grid(double dx, double dy, double dz, double a, double b,
+                                          const KParams &kp, double &q, double &w)
+{
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        const double ir = rsq64(r2);
+        q = fma(a, ir, q);
+        if constexpr (SS) w = fma(exp(-r2 * kp.inv_a2) * b, ir, w);
+    } else if constexpr (KER == KY) {
+        const double ir = rsq64(r2);
+        const double e = exp(-kp.p0 * (r2 * ir)) * ir;
+        q = fma(a, e, q);
+        if constexpr (SS) w = fma(b, e, w);
+    } else {
+        q = fma(a, rsq64(r2 + kp.eps2), q);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Interaction kernel.  One block = one tile of up to TPB*TPT targets of one
+// key (a target leaf, or the grid of a target node).  Sources stream through
+// LDS in chunks of TPB points: first the key's source clusters, then its
+// source particle ranges.  Each target's sum stays in registers and is
+// stored once, so no atomics are needed.
+// ---------------------------------------------------------------------
+template <int KER, bool SS, bool GRID>
+__global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
+{
+    const Tile tile = A.tiles[blockIdx.x];
+    const int key = tile.x;
+    const int tid = threadIdx.x;
+
+    __shared__ double4 Ls[TPB];
+    __shared__ double Lb[SS ? TPB : 1];
+
+    double Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT], acc[TPT], accw[TPT];
+    int tix[TPT];
+    const int nl = A.nl;
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        const int local = tid + u * TPB;
+        const int lc = local < tile.z ? local : 0;
+        tix[u] = local < tile.z ? tile.y + local : -1;
+        if constexpr (GRID) {
+            const int j = tile.y + lc;
+            const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
+            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
+            Tx[u] = tb[k1]; Ty[u] = tb[nl + k2]; Tz[u] = tb[2 * nl + k3]; Tq[u] = 0.0;
+        } else {
+            const int i = tile.y + lc;
+            Tx[u] = A.tx[i]; Ty[u] = A.ty[i]; Tz[u] = A.tz[i];
+            Tq[u] = SS ? A.tq[i] : 0.0;
+        }
+        acc[u] = 0.0; accw[u] = 0.0;
+    }
+
+    auto consume = [&](int n) {
+#pragma unroll 2
+        for (int j = 0; j < n; ++j) {
+            const double4 s = Ls[j];
+            const double b = SS ? Lb[j] : 0.0;
+#pragma unroll
+            for (int u = 0; u < TPT; u++) {
+                const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
+                else acc[u] += pair_pot<KER, SS>(dx, dy, dz, s.w, b, Tq[u], A.kp);
+            }
+        }
+    };
+
+    // source clusters (PC in PC mode, CC in CC mode)
+    const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
+    for (int c = c0; c < c1; ++c) {
+        const size_t base = (size_t)A.clu_id[c] * A.P;
+        for (int off = 0; off < A.P; off += TPB) {
+            const int n = min(TPB, A.P - off);
+            __syncthreads();
+            if (tid < n) {
+                const size_t g = base + off + tid;
+                Ls[tid] = make_double4(A.cx[g], A.cy[g], A.cz[g], A.cq[g]);
+                if constexpr (SS) Lb[tid] = A.cw[g];
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    // source particle ranges (PP, or CP into grids)
+    const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
+    for (int s = s0; s < s1; ++s) {
+        const Seg sg = A.segs[s];
+        for (int off = 0; off < sg.y; off += TPB) {
+            const int n = min(TPB, sg.y - off);
+            __syncthreads();
+            if (tid < n) {
+                const int g = sg.x + off + tid;
+                const double q = A.sq[g];
+                double a = q, b = 0.0;
+                if constexpr (SS) { b = A.sw[g]; a = q * b; }
+                Ls[tid] = make_double4(A.sx[g], A.sy[g], A.sz[g], a);
+                if constexpr (SS) Lb[tid] = b;
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        if (tix[u] < 0) continue;
+        if constexpr (GRID) {
+            const size_t o = (size_t)tile.w * A.P + tix[u];
+            A.out1[o] = acc[u];
+            if constexpr (SS) A.out2[o] = accw[u];
+        } else {
+            A.out1[tix[u]] = acc[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Mixed precision interaction kernel: FP32 pair evaluation, FP64 target
+// coordinates.  The displacement is formed in FP64 and rounded once, the
+// pair value is FP32 (v_rsq_f32 plus one Newton step), and each target
+// accumulates a chunk of TPB pairs in FP32 with Kahan compensation before
+// adding the chunk into an FP64 total.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float rsq32(float x)
+{
+    float y = __builtin_amdgcn_rsqf(x);
+    // one Newton step: y*(1.5 - 0.5*x*y*y)
+    const float h = 0.5f * x * y;
+    return y * fmaf(-h, y, 1.5f);
+}
+
+template <int KER, bool SS>
+__device__ __forceinline__ float pair_pot32(float dx, float dy, float dz, float a, float b, float tq,
+                                            const KParams &kp)
+{
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        const float ir = rsq32(r2);
+        if constexpr (!SS) {
+            return r2 > FLT_MIN ? a * ir : 0.0f;
+        } else {
+            const float v = fmaf(-tq * b, __expf(-r2 * (float)kp.inv_a2), a) * ir;
+            return r2 > FLT_MIN ? v : 0.0f;
+        }
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2);
+        const float e = __expf(-(float)kp.p0 * (r2 * ir)) * ir;
+        const float v = SS ? fmaf(-tq, b, a) * e : a * e;
+        return r2 > FLT_MIN ? v : 0.0f;
+    } else {
+        const float ir = rsq32(r2 + (float)kp.eps2);
+        if constexpr (!SS) return a * ir;
+        else return fmaf(-tq * b, __expf(-r2 * (float)kp.inv_a2), a) * ir;
+    }
+}
+
+template <int KER, bool SS>
+__device__ __forceinline__ void pair_grid32(float dx, float dy, float dz, float a, float b,
+                                            const KParams &kp, float &q, float &w)
+{
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        const float ir = rsq32(r2);
+        q = fmaf(a, ir, q);
+        if constexpr (SS) w = fmaf(__expf(-r2 * (float)kp.inv_a2) * b, ir, w);
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2);
+        const float e = __expf(-(float)kp.p0 * (r2 * ir)) * ir;
+        q = fmaf(a, e, q);
+        if constexpr (SS) w = fmaf(b, e, w);
+    } else {
+        q = fmaf(a, rsq32(r2 + (float)kp.eps2), q);
+    }
+}
+
+// Sources are staged relative to the tile's first target (FP64 difference,
+// rounded once), so the FP32 displacement loses nothing to large absolute
+// coordinates.
+template <int KER, bool SS, bool GRID>
+__global__ __launch_bounds__(TPB) void k_interact32(InteractArgs A)
+{
+    const Tile tile = A.tiles[blockIdx.x];
+    const int key = tile.x;
+    const int tid = threadIdx.x;
+
+    __shared__ float4 Ls[TPB];
+    __shared__ float Lb[SS ? TPB : 1];
+
+    const int nl = A.nl;
+    double ox, oy, oz;  // tile origin
+    {
+        if constexpr (GRID) {
+            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
+            ox = 0.5 * (tb[0] + tb[nl - 1]);
+            oy = 0.5 * (tb[nl] + tb[2 * nl - 1]);
+            oz = 0.5 * (tb[2 * nl] + tb[3 * nl - 1]);
+        } else {
+            ox = A.tx[tile.y]; oy = A.ty[tile.y]; oz = A.tz[tile.y];
+        }
+    }
+    float Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT];
+    float cs[TPT], cc[TPT], csw[TPT], ccw[TPT];   // FP32 chunk sums + Kahan carries
+    double acc[TPT], accw[TPT];
+    int tix[TPT];
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        const int local = tid + u * TPB;
+        const int lc = local < tile.z ? local : 0;
+        tix[u] = local < tile.z ? tile.y + local : -1;
+        double x, y, z, q = 0.0;
+        if constexpr (GRID) {
+            const int j = tile.y + lc;
+            const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
+            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
+            x = tb[k1]; y = tb[nl + k2]; z = tb[2 * nl + k3];
+        } else {
+            const int i = tile.y + lc;
+            x = A.tx[i]; y = A.ty[i]; z = A.tz[i];
+            if (SS) q = A.tq[i];
+        }
+        Tx[u] = (float)(x - ox); Ty[u] = (float)(y - oy); Tz[u] = (float)(z - oz); Tq[u] = (float)q;
+        acc[u] = 0.0; accw[u] = 0.0;
+    }
+
+    auto consume = [&](int n) {
+#pragma unroll
+        for (int u = 0; u < TPT; u++) { cs[u] = 0.f; cc[u] = 0.f; csw[u] = 0.f; ccw[u] = 0.f; }
+#pragma unroll 4
+        for (int j = 0; j < n; ++j) {
+            const float4 s = Ls[j];
+            const float b = SS ? Lb[j] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < TPT; u++) {
+                const float dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                if constexpr (GRID) {
+                    float q = 0.f, w = 0.f;
+                    pair_grid32<KER, SS>(dx, dy, dz, s.w, b, A.kp, q, w);
+                    // Kahan: cs += q
+                    float y = q - cc[u], t = cs[u] + y;
+                    cc[u] = (t - cs[u]) - y; cs[u] = t;
+                    if constexpr (SS) {
+                        float y2 = w - ccw[u], t2 = csw[u] + y2;
+                        ccw[u] = (t2 - csw[u]) - y2; csw[u] = t2;
+                    }
+                } else {
+                    const float v = pair_pot32<KER, SS>(dx, dy, dz, s.w, b, Tq[u], A.kp);
+                    float y = v - cc[u], t = cs[u] + y;
+                    cc[u] = (t - cs[u]) - y; cs[u] = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TPT; u++) {
+            acc[u] += (double)cs[u] - (double)cc[u];
+            if (SS && GRID) accw[u] += (double)csw[u] - (double)ccw[u];
+        }
+    };
+
+    const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
+    for (int c = c0; c < c1; ++c) {
+        const size_t base = (size_t)A.clu_id[c] * A.P;
+        for (int off = 0; off < A.P; off += TPB) {
+            const int n = min(TPB, A.P - off);
+            __syncthreads();
+            if (tid < n) {
+                const size_t g = base + off + tid;
+                Ls[tid] = make_float4((float)(A.cx[g] - ox), (float)(A.cy[g] - oy),
+                                      (float)(A.cz[g] - oz), (float)A.cq[g]);
+                if constexpr (SS) Lb[tid] = (float)A.cw[g];
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
+    for (int s = s0; s < s1; ++s) {
+        const Seg sg = A.segs[s];
+        for (int off = 0; off < sg.y; off += TPB) {
+            const int n = min(TPB, sg.y - off);
+            __syncthreads();
+            if (tid < n) {
+                const int g = sg.x + off + tid;
+                double a = A.sq[g], b = 0.0;
+                if constexpr (SS) { b = A.sw[g]; a = a * b; }
+                Ls[tid] = make_float4((float)(A.sx[g] - ox), (float)(A.sy[g] - oy),
+                                      (float)(A.sz[g] - oz), (float)a);
+                if constexpr (SS) Lb[tid] = (float)b;
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        if (tix[u] < 0) continue;
+        if constexpr (GRID) {
+            const size_t o = (size_t)tile.w * A.P + tix[u];
+            A.out1[o] = acc[u];
+            if constexpr (SS) A.out2[o] = accw[u];
+        } else {
+            A.out1[tix[u]] = acc[u];
+        }
+    }
+}
+
+template <int KER, bool SS>
+hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
+{
+    dim3 grid(a.ntiles), block(TPB);
+    if (a.precision == 1) {
+        if (a.grid_targets) k_interact32<KER, SS, true><<<grid, block, 0, st>>>(a);
+        else k_interact32<KER, SS, false><<<grid, block, 0, st>>>(a);
+    } else {
+        if (a.grid_targets) k_interact<KER, SS, true><<<grid, block, 0, st>>>(a);
+        else k_interact<KER, SS, false><<<grid, block, 0, st>>>(a);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// Cluster grid coordinates from node tables.
+// ---------------------------------------------------------------------
+__global__ void k_cluster_points(int numnodes, int nl, const double *tab, double *cx, double *cy,
+                                 double *cz)
+{
+    const int P = nl * nl * nl;
+    const size_t total = (size_t)numnodes * P;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (size_t)gridDim.x * blockDim.x) {
+        const int node = (int)(g / P), j = (int)(g % P);
+        const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
+        const double *tb = tab + (size_t)node * 3 * nl;
+        cx[g] = tb[k1]; cy[g] = tb[nl + k2]; cz[g] = tb[2 * nl + k3];
+    }
+}
+
+// ---------------------------------------------------------------------
+// Barycentric weights of the Chebyshev points of the second kind:
+// w_j = (-1)^j d_j, d_0 = d_n = 1/2 (clusters.c:446-457).
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double bary_w(int j, int n)
+{
+    const double d = (j == 0 || j == n) ? 0.5 : 1.0;
+    return (j & 1) ? -d : d;
+}
+
+// ---------------------------------------------------------------------
+// Upward pass (anterpolation), one block of 256 threads per output node.
+// Points arrive in chunks of CH; for each chunk, 3*CH threads compute the
+// per-dimension factors w_k/(x_d - node_d[k]) (or the exact-hit Kronecker
+// row) into LDS, CH threads form the modified charge q/prod(sums), and
+// every thread accumulates its grid points j = tid + 256*m as
+//   Q[j] += ((Ax[k1]*Ay[k2])*Az[k3]) * qmod
+// which is the reference's numerator order (clusters.c:506-565).
+// ---------------------------------------------------------------------
+constexpr int ACH = 64;
+
+template <int NL, bool SS>
+__global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
+{
+    constexpr int P = NL * NL * NL;
+    constexpr int MJ = (P + 255) / 256;
+    constexpr int LDA = ACH + 1;
+    constexpr int n = NL - 1;
+    __shared__ double LA[3 * NL * LDA];
+    __shared__ double Lsum[3 * ACH];
+    __shared__ double Lf1[ACH], Lf2[ACH];
+    __shared__ double Lnode[3 * NL];
+
+    const int tid = threadIdx.x;
+    const int node = A.nodes[blockIdx.x];
+    if (tid < 3 * NL) Lnode[tid] = A.tab[(size_t)node * 3 * NL + tid];
+
+    int o1[MJ], o2[MJ], o3[MJ];
+    double accq[MJ], accw[MJ];
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+        const int j = tid + 256 * m;
+        const int jj = j < P ? j : 0;
+        const int k1 = jj % NL, k2 = (jj / NL) % NL, k3 = jj / (NL * NL);
+        o1[m] = k1 * LDA; o2[m] = (NL + k2) * LDA; o3[m] = (2 * NL + k3) * LDA;
+        accq[m] = 0.0; accw[m] = 0.0;
+    }
+
+    // point sets: mode 0 one range of particles; mode 1 the grids of all children
+    int nsets = 1;
+    if (A.mode == 1) nsets = A.child_off[blockIdx.x + 1] - A.child_off[blockIdx.x];
+    for (int set = 0; set < nsets; ++set) {
+        int pbeg = 0, pcnt = P, child = -1;
+        if (A.mode == 0) {
+            pbeg = A.ibeg[node];
+            pcnt = A.iend[node] - pbeg;
+        } else {
+            child = A.child_id[A.child_off[blockIdx.x] + set];
+        }
+        for (int c0 = 0; c0 < pcnt; c0 += ACH) {
+            __syncthreads();
+            if (tid < 3 * ACH) {
+                const int p = tid % ACH, d = tid / ACH;
+                const bool valid = c0 + p < pcnt;
+                double coord = 0.0;
+                if (valid) {
+                    if (A.mode == 0) {
+                        const int g = pbeg + c0 + p;
+                        coord = d == 0 ? A.px[g] : (d == 1 ? A.py[g] : A.pz[g]);
+                    } else {
+                        const int jp = c0 + p;
+                        const int kd = d == 0 ? jp % NL : (d == 1 ? (jp / NL) % NL : jp / (NL * NL));
+                        coord = A.tab[(size_t)child * 3 * NL + d * NL + kd];
+                    }
+                }
+                double sum = 0.0;
+                int hit = -1;
+#pragma unroll
+                for (int k = 0; k < NL; k++) {
+                    const double c = coord - Lnode[d * NL + k];
+                    if (fabs(c) < DBL_MIN) hit = k;
+                    const double v = bary_w(k, n) / c;
+                    sum += v;
+                    LA[(d * NL + k) * LDA + p] = v;
+                }
+                if (hit >= 0 || !valid) {
+#pragma unroll
+                    for (int k = 0; k < NL; k++) LA[(d * NL + k) * LDA + p] = (valid && k == hit) ? 1.0 : 0.0;
+                    sum = 1.0;
+                }
+                Lsum[d * ACH + p] = sum;
+            }
+            __syncthreads();
+            if (tid < ACH) {
+                const int p = tid;
+                double f1 = 0.0, f2 = 0.0;
+                if (c0 + p < pcnt) {
+                    double den = 1.0;
+                    den *= Lsum[p];
+                    den *= Lsum[ACH + p];
+                    den *= Lsum[2 * ACH + p];
+                    if (A.mode == 0) {
+                        const int g = pbeg + c0 + p;
+                        const double q = A.pq[g];
+                        if (SS) { const double w = A.pw[g]; f1 = q * w; f2 = w; }
+                        else f1 = q;
+                    } else {
+                        const size_t g = (size_t)child * P + c0 + p;
+                        f1 = A.Q[g];
+                        if (SS) f2 = A.W[g];
+                    }
+                    f1 /= den;
+                    if (SS) f2 /= den;
+                }
+                Lf1[p] = f1;
+                Lf2[p] = f2;
+            }
+            __syncthreads();
+            const int np = min(ACH, pcnt - c0);
+            for (int p = 0; p < np; ++p) {
+                const double f1 = Lf1[p];
+                const double f2 = SS ? Lf2[p] : 0.0;
+#pragma unroll
+                for (int m = 0; m < MJ; m++) {
+                    const double num = (LA[o1[m] + p] * LA[o2[m] + p]) * LA[o3[m] + p];
+                    accq[m] = fma(num, f1, accq[m]);
+                    if (SS) accw[m] = fma(num, f2, accw[m]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+        const int j = tid + 256 * m;
+        if (j < P) {
+            A.Q[(size_t)node * P + j] = accq[m];
+            if (SS) A.W[(size_t)node * P + j] = accw[m];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Downward pass: evaluate the interpolant of a node's grid values at points
+// (the child's grid points, or the leaf's targets).  Per point the factors
+// w_k/(x_d - node_d[k]) live in registers (NL is a template parameter);
+// the value is  D * sum_k3 wz sum_k2 wy sum_k1 wx Q[k1,k2,k3]  with
+// D = ((1/sx)/sy)/sz over the dimensions without an exact hit
+// (interaction_compute_downpass.c:210-303, 395-479).
+// ---------------------------------------------------------------------
+template <int NL>
+__device__ __forceinline__ void point_factors(double x, const double *nodes, double *wv, double &den,
+                                              bool &hit_any)
+{
+    double sum = 0.0;
+    int hit = -1;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const double c = x - nodes[k];
+        if (fabs(c) < DBL_MIN) hit = k;
+        const double v = bary_w(k, NL - 1) / c;
+        wv[k] = v;
+        sum += v;
+    }
+    if (hit >= 0) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) wv[k] = k == hit ? 1.0 : 0.0;
+    } else {
+        den /= sum;
+    }
+    hit_any = hit_any || hit >= 0;
+}
+
+constexpr int DTPB = 128;  // threads per block of the downward pass
+
+template <int NL, bool SS>
+__global__ __launch_bounds__(DTPB) void k_down(DownArgs A)
+{
+    constexpr int P = NL * NL * NL;
+    __shared__ double LQ[P];
+    __shared__ double LW[SS ? P : 1];
+    __shared__ double Lnode[3 * NL];
+    __shared__ double Lwy[NL * DTPB], Lwz[NL * DTPB];  // per-thread y and z factors
+    const int tid = threadIdx.x;
+    int src_node, pbeg = 0, pcnt = P, dst_node = -1;
+    if (A.mode == 0) {
+        dst_node = A.nodes[blockIdx.x];
+        src_node = A.parent[dst_node];
+    } else {
+        const Tile t = A.tiles[blockIdx.x];
+        src_node = t.x; pbeg = t.y; pcnt = t.z;
+    }
+    for (int j = tid; j < P; j += DTPB) {
+        LQ[j] = A.Q[(size_t)src_node * P + j];
+        if (SS) LW[j] = A.W[(size_t)src_node * P + j];
+    }
+    if (tid < 3 * NL) Lnode[tid] = A.tab[(size_t)src_node * 3 * NL + tid];
+    __syncthreads();
+    for (int i = tid; i < pcnt; i += DTPB) {
+        double x, y, z;
+        if (A.mode == 0) {
+            const int k1 = i % NL, k2 = (i / NL) % NL, k3 = i / (NL * NL);
+            const double *tb = A.tab + (size_t)dst_node * 3 * NL;
+            x = tb[k1]; y = tb[NL + k2]; z = tb[2 * NL + k3];
+        } else {
+            const int g = pbeg + i;
+            x = A.tx[g]; y = A.ty[g]; z = A.tz[g];
+        }
+        double wx[NL], wt[NL];
+        double den = 1.0;
+        bool h = false;
+        point_factors<NL>(x, Lnode, wx, den, h);
+        point_factors<NL>(y, Lnode + NL, wt, den, h);
+#pragma unroll
+        for (int k = 0; k < NL; k++) Lwy[k * DTPB + tid] = wt[k];
+        point_factors<NL>(z, Lnode + 2 * NL, wt, den, h);
+#pragma unroll
+        for (int k = 0; k < NL; k++) Lwz[k * DTPB + tid] = wt[k];
+        double sq = 0.0, sw = 0.0;
+#pragma unroll 1
+        for (int k3 = 0; k3 < NL; k3++) {
+            double s2q = 0.0, s2w = 0.0;
+#pragma unroll 1
+            for (int k2 = 0; k2 < NL; k2++) {
+                double s1q = 0.0, s1w = 0.0;
+                const int base = (k3 * NL + k2) * NL;
+#pragma unroll
+                for (int k1 = 0; k1 < NL; k1++) {
+                    s1q = fma(wx[k1], LQ[base + k1], s1q);
+                    if (SS) s1w = fma(wx[k1], LW[base + k1], s1w);
+                }
+                const double wyk = Lwy[k2 * DTPB + tid];
+                s2q = fma(wyk, s1q, s2q);
+                if (SS) s2w = fma(wyk, s1w, s2w);
+            }
+            const double wzk = Lwz[k3 * DTPB + tid];
+            sq = fma(wzk, s2q, sq);
+            if (SS) sw = fma(wzk, s2w, sw);
+        }
+        const double vq = den * sq, vw = den * sw;
+        if (A.mode == 0) {
+            const size_t o = (size_t)dst_node * P + i;
+            A.Q[o] += vq;
+            if (SS) A.W[o] += vw;
+        } else {
+            const int g = pbeg + i;
+            // subtraction: interpolate Q - tq*W (downpass.c:836)
+            A.pot[g] += SS ? fma(-A.tq[g], vw, vq) : vq;
+        }
+    }
+}
+
+template <bool SS>
+hipError_t anterp_dispatch(const AnterpArgs &a, hipStream_t st)
+{
+    dim3 g(a.nblocks), b(256);
+    switch (a.degree) {
+#define CASE(D) case D: k_anterp<D + 1, SS><<<g, b, 0, st>>>(a); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <bool SS>
+hipError_t down_dispatch(const DownArgs &a, hipStream_t st)
+{
+    dim3 g(a.nblocks), b(DTPB);
+    switch (a.degree) {
+#define CASE(D) case D: k_down<D + 1, SS><<<g, b, 0, st>>>(a); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
+#undef CASE
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_finish(int n, const double *pot, const double *tq, double ss, const int *order, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[order[i]] = ss != 0.0 ? fma(ss, tq[i], pot[i]) : pot[i];
+}
+
+// Direct sum: one target per thread, all sources through LDS.
+template <int KER, bool SS>
+__global__ __launch_bounds__(256) void k_direct(KParams kp, int m, const double *tx, const double *ty,
+                                                const double *tz, const double *tq, int n,
+                                                const double *sx, const double *sy, const double *sz,
+                                                const double *sq, const double *sw, double *out)
+{
+    __shared__ double4 Ls[256];
+    __shared__ double Lb[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int ic = i < m ? i : 0;
+    const double x = tx[ic], y = ty[ic], z = tz[ic], q = SS ? tq[ic] : 0.0;
+    double acc = 0.0;
+    for (int off = 0; off < n; off += 256) {
+        const int cnt = min(256, n - off);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            const int g = off + threadIdx.x;
+            double a = sq[g], b = 0.0;
+            if (SS) { b = sw[g]; a = a * b; }
+            Ls[threadIdx.x] = make_double4(sx[g], sy[g], sz[g], a);
+            Lb[threadIdx.x] = b;
+        }
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            const double4 s = Ls[j];
+            acc += pair_pot<KER, SS>(x - s.x, y - s.y, z - s.z, s.w, Lb[j], q, kp);
+        }
+    }
+    if (i < m) out[i] = SS ? fma(kp.ss_param, q, acc) : acc;
+}
+
+}  // namespace
+
+hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    const bool ss = a.singularity == 2;
+    switch (a.kernel) {
+        case KC: return ss ? interact_dispatch<KC, true>(a, st) : interact_dispatch<KC, false>(a, st);
+        case KY: return ss ? interact_dispatch<KY, true>(a, st) : interact_dispatch<KY, false>(a, st);
+        case KR: return ss ? interact_dispatch<KR, true>(a, st) : interact_dispatch<KR, false>(a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double *cx, double *cy,
+                                 double *cz, hipStream_t st)
+{
+    const size_t total = (size_t)numnodes * nl * nl * nl;
+    if (total == 0) return hipSuccess;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65535) blocks = 65535;
+    k_cluster_points<<<blocks, 256, 0, st>>>(numnodes, nl, tab, cx, cy, cz);
+    return hipGetLastError();
+}
+
+hipError_t launch_anterp(const AnterpArgs &a, hipStream_t st)
+{
+    if (a.nblocks <= 0) return hipSuccess;
+    return a.subtraction ? anterp_dispatch<true>(a, st) : anterp_dispatch<false>(a, st);
+}
+
+hipError_t launch_down(const DownArgs &a, hipStream_t st)
+{
+    if (a.nblocks <= 0) return hipSuccess;
+    return a.subtraction ? down_dispatch<true>(a, st) : down_dispatch<false>(a, st);
+}
+
+hipError_t launch_finish(int n, const double *pot, const double *tq, double ss_param, const int *order,
+                         double *out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_finish<<<(n + 255) / 256, 256, 0, st>>>(n, pot, tq, ss_param, order, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const double *tx,
+                         const double *ty, const double *tz, const double *tq, int n,
+                         const double *sx, const double *sy, const double *sz, const double *sq,
+                         const double *sw, double *out, hipStream_t st)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    dim3 g((m + 255) / 256), b(256);
+    const bool ss = singularity == 2;
+#define L(K, S) k_direct<K, S><<<g, b, 0, st>>>(kp, m, tx, ty, tz, tq, n, sx, sy, sz, sq, sw, out)
+    switch (kernel) {
+        case KC: if (ss) L(KC, true); else L(KC, false); break;
+        case KY: if (ss) L(KY, true); else L(KY, false); break;
+        case KR: if (ss) L(KR, true); else L(KR, false); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef L
+    return hipGetLastError();
+}
+
+}  // namespace btk

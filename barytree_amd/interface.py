@@ -1,0 +1,267 @@
+"""Python front end of libBaryTree_gpu.so (ctypes over the C ABI).
+
+Mirrors the reference's binding, interfaces/python/BaryTreeInterface.py:14-152
+(the Kernel / Singularity / Approximation / ComputeType enums and
+callTreedriver with the same arguments and defaults), so scripts written for
+the reference run unchanged; GPUpresent is accepted and ignored because every
+call runs on the GPU.  The extensions (precision choice, plans, direct sums,
+statistics) are exposed as solve(), Plan and analyze().
+
+The library is loaded from this package directory.  Nothing here falls back
+to a CPU path: a missing library raises at load time.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+LIB_NAME = "libBaryTree_gpu.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+class Kernel(enum.IntEnum):
+    NO_KERNEL = 0
+    COULOMB = 1
+    YUKAWA = 2
+    REGULARIZED_COULOMB = 3
+    REGULARIZED_YUKAWA = 4
+    ATAN = 5
+    TCF = 6
+    DCF = 7
+    SIN_OVER_R = 8
+    MQ = 9
+    RBS_U = 10
+    RBS_V = 11
+    USER = 12
+
+
+class Singularity(enum.IntEnum):
+    NO_SINGULARITY = 0
+    SKIPPING = 1
+    SUBTRACTION = 2
+
+
+class Approximation(enum.IntEnum):
+    NO_APPROX = 0
+    LAGRANGE = 1
+    HERMITE = 2
+
+
+class ComputeType(enum.IntEnum):
+    NO_COMPUTE_TYPE = 0
+    PARTICLE_CLUSTER = 1
+    CLUSTER_PARTICLE = 2
+    CLUSTER_CLUSTER = 3
+
+
+class Stats(ctypes.Structure):
+    """BaryTreeStats of include/BaryTreeInterface.h."""
+    _fields_ = [(n, ctypes.c_longlong) for n in (
+        "num_pp", "num_pc", "num_cp", "num_cc",
+        "pp_ptwise", "pc_ptwise", "cp_ptwise", "cc_ptwise",
+        "pc_replaced", "cp_replaced", "cc_replaced")] + [
+        (n, ctypes.c_int) for n in ("source_nodes", "target_nodes", "interp_degree",
+                                    "pts_per_cluster")] + [
+        ("theta", ctypes.c_double),
+        ("max_per_source_leaf", ctypes.c_int), ("max_per_target_leaf", ctypes.c_int),
+        ("near_tiles", ctypes.c_int), ("far_tiles", ctypes.c_int)] + [
+        (n, ctypes.c_double) for n in (
+            "host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s",
+            "gpu_upward_s", "gpu_near_s", "gpu_far_s", "gpu_down_s", "gpu_finish_s",
+            "gpu_total_s")]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    """The loaded libBaryTree_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        l = ctypes.CDLL(LIB_PATH)
+        l.BaryTree_GetLastError.restype = ctypes.c_char_p
+        for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
+                  "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
+                  "BaryTreePlan_DirectSample"):
+            getattr(l, f).restype = ctypes.c_int
+        l.BaryTreeInterface.restype = None
+        l.BaryTreePlan_Destroy.restype = None
+        _lib = l
+    return _lib
+
+
+def last_error():
+    return lib().BaryTree_GetLastError().decode()
+
+
+def _arr(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _kp(kernel_params):
+    if kernel_params is None:
+        return np.zeros(1)
+    return np.ascontiguousarray(np.atleast_1d(np.asarray(kernel_params, dtype=np.float64)))
+
+
+def solve(targets, sources, kernel, kernel_params, singularity, approximation, compute_type,
+          theta, degree, max_source_leaf, max_target_leaf, size_check=1.0, beta=-1.0,
+          precision=PRECISION_FP64, verbosity=0):
+    """One BaryTree_Solve call.  targets = (x, y, z, q) (q may be None),
+    sources = (x, y, z, q, w) (w may be None).  Returns the potential."""
+    tx, ty, tz, tq = map(_arr, targets)
+    sx, sy, sz, sq, sw = map(_arr, sources)
+    kp = _kp(kernel_params)
+    out = np.zeros(len(tx))
+    rc = lib().BaryTree_Solve(
+        ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(tq),
+        _ptr(sx), _ptr(sy), _ptr(sz), _ptr(sq), _ptr(sw), _ptr(out), ctypes.c_int(int(kernel)),
+        ctypes.c_int(len(kp)), _ptr(kp), ctypes.c_int(int(singularity)),
+        ctypes.c_int(int(approximation)), ctypes.c_int(int(compute_type)),
+        ctypes.c_double(theta), ctypes.c_int(degree), ctypes.c_int(max_source_leaf),
+        ctypes.c_int(max_target_leaf), ctypes.c_double(size_check), ctypes.c_double(beta),
+        ctypes.c_int(verbosity), ctypes.c_int(precision))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_Solve failed: {last_error()}")
+    return out
+
+
+def last_stats():
+    s = Stats()
+    if lib().BaryTree_GetLastStats(ctypes.byref(s)) != 0:
+        return None
+    return s.as_dict()
+
+
+def analyze(targets, sources, kernel, kernel_params, singularity, approximation, compute_type,
+            theta, degree, max_source_leaf, max_target_leaf, size_check=1.0, beta=-1.0):
+    """Host-only tree and list construction; returns the interaction counts."""
+    tx, ty, tz = map(_arr, targets[:3])
+    sx, sy, sz = map(_arr, sources[:3])
+    kp = _kp(kernel_params)
+    s = Stats()
+    rc = lib().BaryTree_Analyze(
+        ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(sx),
+        _ptr(sy), _ptr(sz), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+        ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+        ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+        ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+        ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.byref(s))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_Analyze failed: {last_error()}")
+    return s.as_dict()
+
+
+class Plan:
+    """Host setup once, GPU execution many times (BaryTreePlan_*)."""
+
+    def __init__(self, targets, sources, kernel, kernel_params, singularity, approximation,
+                 compute_type, theta, degree, max_source_leaf, max_target_leaf, size_check=1.0,
+                 beta=-1.0, precision=PRECISION_FP64):
+        tx, ty, tz, tq = map(_arr, targets)
+        sx, sy, sz, sq, sw = map(_arr, sources)
+        kp = _kp(kernel_params)
+        self.n_targets = len(tx)
+        self._h = ctypes.c_void_p()
+        rc = lib().BaryTreePlan_Create(
+            ctypes.byref(self._h), ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx),
+            _ptr(ty), _ptr(tz), _ptr(tq), _ptr(sx), _ptr(sy), _ptr(sz), _ptr(sq), _ptr(sw),
+            ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+            ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+            ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+            ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+            ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(precision))
+        if rc != 0:
+            self._h = None
+            raise RuntimeError(f"BaryTreePlan_Create failed: {last_error()}")
+
+    def execute(self, download=True):
+        out = np.zeros(self.n_targets) if download else None
+        rc = lib().BaryTreePlan_Execute(self._h, _ptr(out))
+        if rc != 0:
+            raise RuntimeError(f"BaryTreePlan_Execute failed: {last_error()}")
+        return out
+
+    def stats(self):
+        s = Stats()
+        lib().BaryTreePlan_GetStats(self._h, ctypes.byref(s))
+        return s.as_dict()
+
+    def direct_sample(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros(len(idx))
+        rc = lib().BaryTreePlan_DirectSample(self._h, ctypes.c_int(len(idx)),
+                                             idx.ctypes.data_as(_ip), _ptr(out))
+        if rc != 0:
+            raise RuntimeError(f"BaryTreePlan_DirectSample failed: {last_error()}")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().BaryTreePlan_Destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def callTreedriver(numTargets, numSources,
+                   targetX, targetY, targetZ, targetValue,
+                   sourceX, sourceY, sourceZ, sourceValue, sourceWeight,
+                   kernelName, numberOfKernelParameters, kernelParameters, singularityHandling,
+                   approximationName, computeType, GPUpresent, verbosity,
+                   beta=None, theta=None, degree=None, sourceLeafSize=None, targetLeafSize=None,
+                   sizeCheck=None):
+    """Same arguments, defaults and result as the reference's callTreedriver
+    (interfaces/python/BaryTreeInterface.py:82-152): either beta, or theta,
+    degree and both leaf sizes; sizeCheck defaults to 1 (Lagrange) or 4
+    (Hermite).  Returns the potential at the targets."""
+    if beta:
+        theta, degree, sourceLeafSize, targetLeafSize = -1.0, -1, 1, 1
+    else:
+        assert theta is not None, "If beta isn't provided, theta must be."
+        assert degree is not None, "If beta isn't provided, degree must be."
+        assert sourceLeafSize is not None, "If beta isn't provided, sourceLeafSize must be."
+        assert targetLeafSize is not None, "If beta isn't provided, targetLeafSize must be."
+        beta = -1
+    if not sizeCheck:
+        sizeCheck = 4.0 if approximationName == Approximation.HERMITE else 1.0
+    kp = _kp(kernelParameters)
+    arrays = [_arr(a) for a in (targetX, targetY, targetZ, targetValue, sourceX, sourceY,
+                                sourceZ, sourceValue, sourceWeight)]
+    result = np.zeros(numTargets)
+    lib().BaryTreeInterface(
+        ctypes.c_int(numTargets), ctypes.c_int(numSources), *[_ptr(a) for a in arrays],
+        _ptr(result), ctypes.c_int(int(kernelName)), ctypes.c_int(numberOfKernelParameters),
+        _ptr(kp), ctypes.c_int(int(singularityHandling)), ctypes.c_int(int(approximationName)),
+        ctypes.c_int(int(computeType)), ctypes.c_double(theta), ctypes.c_int(degree),
+        ctypes.c_int(sourceLeafSize), ctypes.c_int(targetLeafSize), ctypes.c_double(sizeCheck),
+        ctypes.c_double(beta), ctypes.c_int(verbosity))
+    return result

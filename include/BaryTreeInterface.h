@@ -1,0 +1,155 @@
+/*
+ * BaryTreeInterface.h -- C ABI of the MI355X BaryTree interaction path
+ * (libBaryTree_gpu.so).
+ *
+ * BaryTreeInterface() has the reference's prototype and enum numbering and
+ * replaces src/interface/BaryTreeInterface.h:52-59 (enums :7-47), so the
+ * reference's ctypes binding (interfaces/python/BaryTreeInterface.py:47-55,
+ * 134-152) and Fortran interface (interfaces/fortran/BaryTreeInterface.fh:
+ * 14-39) bind to it unchanged.  The remaining entry points have no
+ * counterpart in the reference; each says what it is for.
+ *
+ * All arrays are host pointers owned by the caller.  Nothing here modifies
+ * caller input arrays (the reference permutes them during the call and
+ * restores them on return; the observable result is the same).
+ */
+#ifndef BARYTREE_INTERFACE_H
+#define BARYTREE_INTERFACE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef H_BARYTREE_TYPES_H
+#define H_BARYTREE_TYPES_H
+
+typedef enum KERNEL {
+    NO_KERNEL,
+    COULOMB,
+    YUKAWA,
+    REGULARIZED_COULOMB,
+    REGULARIZED_YUKAWA,
+    ATAN,
+    TCF,
+    DCF,
+    SIN_OVER_R,
+    MQ,
+    RBS_U,
+    RBS_V,
+    USER
+} KERNEL;
+
+typedef enum SINGULARITY { NO_SINGULARITY, SKIPPING, SUBTRACTION } SINGULARITY;
+
+typedef enum APPROXIMATION { NO_APPROX, LAGRANGE, HERMITE } APPROXIMATION;
+
+typedef enum COMPUTE_TYPE {
+    NO_COMPUTE_TYPE,
+    PARTICLE_CLUSTER,
+    CLUSTER_PARTICLE,
+    CLUSTER_CLUSTER,
+} COMPUTE_TYPE;
+
+#endif /* H_BARYTREE_TYPES_H */
+
+/* Reference entry point (src/interface/BaryTreeInterface.h:52-59,
+ * src/interface/BaryTreeInterface.c:18-62): zeroes outputArray, then adds the
+ * treecode potential at every target, in the caller's target order.
+ * Supported: LAGRANGE; COULOMB, YUKAWA, REGULARIZED_COULOMB; SKIPPING and
+ * SUBTRACTION; PARTICLE_CLUSTER, CLUSTER_PARTICLE, CLUSTER_CLUSTER, in the
+ * combinations the reference dispatchers accept; degree 1..12.  Where the
+ * reference prints an error and calls exit(1), this prints the error (with
+ * the "[BaryTree]" prefix), leaves outputArray zero and records the message
+ * for BaryTree_GetLastError(); the process keeps running. */
+void BaryTreeInterface(int numTargets, int numSources,
+                       double *targetX, double *targetY, double *targetZ, double *targetValue,
+                       double *sourceX, double *sourceY, double *sourceZ, double *sourceValue,
+                       double *sourceWeight, double *outputArray,
+                       KERNEL kernel, int numKernelParams, double *kernelParams,
+                       SINGULARITY singularity, APPROXIMATION approximation,
+                       COMPUTE_TYPE compute_type, double theta, int interpOrder,
+                       int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck,
+                       double beta, int verbosity);
+
+/* ---- Extensions (no counterpart in the reference) ------------------- */
+
+/* Arithmetic of the pair kernels. */
+#define BARYTREE_PRECISION_FP64 0  /* plain mode: FP64 throughout */
+#define BARYTREE_PRECISION_MIXED 1 /* FP32 pair evaluation with compensated
+                                      per-target accumulation; cluster charges,
+                                      interpolation weights and totals FP64 */
+
+/* Interaction counts (the reference's verbosity>2 diagnostics,
+ * src/drivers/treedriver.c:191-200, 472-483, 763-792) and phase times. */
+typedef struct BaryTreeStats {
+    long long num_pp, num_pc, num_cp, num_cc;
+    long long pp_ptwise, pc_ptwise, cp_ptwise, cc_ptwise;
+    long long pc_replaced, cp_replaced, cc_replaced;
+    int source_nodes, target_nodes, interp_degree, pts_per_cluster;
+    double theta;
+    int max_per_source_leaf, max_per_target_leaf;
+    int near_tiles, far_tiles;
+    double host_copy_s, host_tree_s, host_lists_s, host_work_s, h2d_s;
+    double gpu_upward_s, gpu_near_s, gpu_far_s, gpu_down_s, gpu_finish_s, gpu_total_s;
+} BaryTreeStats;
+
+/* For callers that want an error code instead of a printed message:
+ * BaryTreeInterface semantics plus a precision choice.  Returns 0 on
+ * success, a negative value on error (message in BaryTree_GetLastError). */
+int BaryTree_Solve(int numTargets, int numSources,
+                   const double *targetX, const double *targetY, const double *targetZ,
+                   const double *targetValue, const double *sourceX, const double *sourceY,
+                   const double *sourceZ, const double *sourceValue, const double *sourceWeight,
+                   double *outputArray, int kernel, int numKernelParams,
+                   const double *kernelParams, int singularity, int approximation,
+                   int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                   int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                   int precision);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *BaryTree_GetLastError(void);
+
+/* Statistics of the last successful BaryTreeInterface/BaryTree_Solve call on
+ * this thread.  Returns 0, or -1 if there was none. */
+int BaryTree_GetLastStats(BaryTreeStats *stats);
+
+/* For callers that size a run before launching it: builds the trees and
+ * interaction lists on the host only (no GPU work) and reports the counts
+ * and host times of BaryTree_Solve with the same arguments.  Returns 0 or a
+ * negative value. */
+int BaryTree_Analyze(int numTargets, int numSources,
+                     const double *targetX, const double *targetY, const double *targetZ,
+                     const double *sourceX, const double *sourceY, const double *sourceZ,
+                     int kernel, int numKernelParams, const double *kernelParams,
+                     int singularity, int approximation, int compute_type, double theta,
+                     int interpOrder, int maxPerSourceLeaf, int maxPerTargetLeaf,
+                     double sizeCheck, double beta, BaryTreeStats *stats);
+
+/* For callers that re-run one particle configuration (and for timing the GPU
+ * path with its inputs resident in HBM): Create does the host work (trees,
+ * lists) and uploads everything; Execute runs the GPU path and, if
+ * outputArray is not NULL, copies the potential to it (caller order). */
+typedef struct BaryTreePlan BaryTreePlan;
+
+int BaryTreePlan_Create(BaryTreePlan **plan, int numTargets, int numSources,
+                        const double *targetX, const double *targetY, const double *targetZ,
+                        const double *targetValue, const double *sourceX,
+                        const double *sourceY, const double *sourceZ,
+                        const double *sourceValue, const double *sourceWeight, int kernel,
+                        int numKernelParams, const double *kernelParams, int singularity,
+                        int approximation, int compute_type, double theta, int interpOrder,
+                        int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck,
+                        double beta, int precision);
+int BaryTreePlan_Execute(BaryTreePlan *plan, double *outputArray);
+int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats);
+/* FP64 direct sum on the GPU (directdriver semantics,
+ * src/drivers/directdriver.c:23-122) at m targets given by caller-order
+ * indices; for error checks of the plan's result. */
+int BaryTreePlan_DirectSample(BaryTreePlan *plan, int m, const int *targetIndex, double *out);
+void BaryTreePlan_Destroy(BaryTreePlan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BARYTREE_INTERFACE_H */

@@ -1,0 +1,44 @@
+"""Shared input generators and error metrics for the tests."""
+import numpy as np
+
+KP = {  # kernel -> singularity -> parameters
+    1: {1: [0.5], 2: [0.5]},
+    2: {1: [0.5], 2: [0.5]},
+    3: {1: [0.05], 2: [0.3, 0.05]},
+}
+
+
+def supported_combos():
+    """(compute_type, kernel, singularity) the reference accepts for Lagrange
+    (src/interaction_compute/interaction_compute_{pc,cp,cc}.c)."""
+    out = []
+    for ct in (1, 2, 3):
+        for k in (1, 2, 3):
+            for s in (1, 2):
+                if k == 3 and s == 2 and ct != 1:
+                    continue
+                out.append((ct, k, s))
+    return out
+
+
+def uniform_cube(n, m, seed, wlo=0.5, whi=1.5):
+    """Sources (x,y,z,q,w) and targets (x,y,z,q), uniform in [-1,1]^3,
+    charges uniform in [-1,1], weights uniform in [wlo, whi]."""
+    rng = np.random.default_rng(seed)
+    s = [rng.uniform(-1, 1, n) for _ in range(4)] + [rng.uniform(wlo, whi, n)]
+    t = [rng.uniform(-1, 1, m) for _ in range(4)]
+    return t, s
+
+
+def kat_inputs():
+    """Inputs of interfaces/python/testBaryTreeInterface.py:41-46: numpy legacy
+    MT19937 seed(1), drawn in the order RHO, X, Y, Z; weights 1."""
+    rs = np.random.RandomState(1)
+    n = 5000
+    rho = rs.rand(n); x = rs.rand(n); y = rs.rand(n); z = rs.rand(n)
+    return x, y, z, rho, np.ones(n)
+
+
+def rel_l2(ref, val):
+    ref = np.asarray(ref); val = np.asarray(val)
+    return float(np.sqrt(np.sum((ref - val) ** 2) / np.sum(ref ** 2)))
