@@ -182,6 +182,16 @@ int BaryTreePlan_Execute(BaryTreePlan *plan, double *outputArray)
     return 0;
 }
 
+int BaryTreePlan_SetPrecision(BaryTreePlan *plan, int precision)
+{
+    g_last_error.clear();
+    if (!plan) return fail("plan is NULL");
+    if (precision != BARYTREE_PRECISION_FP64 && precision != BARYTREE_PRECISION_MIXED)
+        return fail("unknown precision mode");
+    plan->plan.set_precision(precision);
+    return 0;
+}
+
 int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats)
 {
     if (!plan || !stats) return fail("NULL argument");

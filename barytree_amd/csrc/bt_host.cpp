@@ -70,7 +70,7 @@ std::string params_check(const Params &p)
     // interaction_compute_cp.c:218, interaction_compute_cc.c:241
     if (p.kernel == K_REG_COULOMB && p.singularity == S_SUBTRACTION && p.compute_type != C_PC)
         return "NOT SET UP FOR CP/CC REGULARIZED COULOMB SS";
-    if (p.degree < 1 || p.degree > 15) return "interpolation degree must be in [1, 15]";
+    if (p.degree < 1 || p.degree > 12) return "interpolation degree must be in [1, 12]";
     if (p.max_source_leaf < 1 || p.max_target_leaf < 1) return "leaf sizes must be positive";
     if (p.kernel == K_YUKAWA && p.num_kp < 1) return "YUKAWA needs kappa";
     if (p.singularity == S_SUBTRACTION && p.num_kp < 1) return "SUBTRACTION needs a kernel parameter";

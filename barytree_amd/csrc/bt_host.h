@@ -89,6 +89,7 @@ struct WorkCSR {
     std::vector<int> tbeg, tcnt;   // target range (particles) or grid size
     std::vector<int> seg_off;      // CSR into seg_beg/seg_cnt (source particle ranges)
     std::vector<int> seg_beg, seg_cnt;
+    std::vector<char> seg_chk;     // 1: the singularity skip may trigger
     std::vector<int> clu_off;      // CSR into clu_id (source clusters)
     std::vector<int> clu_id;
     long long pairs = 0;           // point-wise evaluations described

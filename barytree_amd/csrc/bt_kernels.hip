@@ -15,6 +15,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 #include "bt_kernels.h"
 
@@ -27,59 +28,61 @@ constexpr int TPT = 2;    // targets per thread
 constexpr int KC = 1, KY = 2, KR = 3;
 
 // ---------------------------------------------------------------------
-// 1/sqrt(x) in FP64: hardware estimate (about 2^-23 relative) refined by
-// two Newton steps y <- y + y*(1/2 - (x*y)*(y/2)); about 1 ulp.
+// 1/sqrt(x) in FP64: hardware estimate refined by Newton steps
+// y <- y + y*(1/2 - (x*y)*(y/2)).  RSQ_NEWTON steps give full FP64 accuracy
+// (about 1 ulp) from the v_rsq_f64 estimate.
 // ---------------------------------------------------------------------
+#ifndef BT_RSQ_NEWTON
+#define BT_RSQ_NEWTON 2
+#endif
 __device__ __forceinline__ double rsq64(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    double e = fma(-g, h, 0.5);
-    y = fma(y, e, y);
-    g = x * y; h = 0.5 * y;
-    e = fma(-g, h, 0.5);
-    return fma(y, e, y);
+#pragma unroll
+    for (int it = 0; it < BT_RSQ_NEWTON; it++) {
+        const double g = x * y, h = 0.5 * y;
+        const double e = fma(-g, h, 0.5);
+        y = fma(y, e, y);
+    }
+    return y;
 }
 
 // ---------------------------------------------------------------------
-// Pair kernels.  Source point carries (a, b):
+// Pair kernels.  A staged source point carries (a, b):
 //   particles: a = q (skipping) or q*w (subtraction), b = w
 //   clusters : a = Q, b = W (subtraction weights)
-// Target-particle form (potential):
-//   Coulomb   SK: a/r  [r^2 > DBL_MIN]        SS: (a - tq*b*exp(-r^2/k^2))/r [r > DBL_MIN]
-//   Yukawa    SK: a e^{-kr}/r [r > DBL_MIN]   SS: (a - tq*b) e^{-kr}/r [r > DBL_MIN]
+// Target-particle form (adds to the potential):
+//   Coulomb   SK: a/r                         SS: (a - tq*b*exp(-r^2/k^2))/r
+//   Yukawa    SK: a e^{-kr}/r                 SS: (a - tq*b) e^{-kr}/r
 //   RegCoul   SK: a/sqrt(r^2+e^2)             SS: (a - tq*b*exp(-r^2/al^2))/sqrt(r^2+e^2)
 // (coulomb_pp.c:48-49, coulomb_ss_pp.c:46-47, coulomb_pc.c:47-50, coulomb_ss_pc.c:47,
 //  yukawa_pp.c:44-45, yukawa_ss_pp.c:47-48, yukawa_ss_pc.c:46, regularized-coulomb_*.c)
-// The skip never triggers for cluster sources: an accepted cluster is
-// separated from its targets, so the reference's unskipped PC sum agrees.
+// CHECK applies the reference's singularity skip (r^2 > DBL_MIN, r > DBL_MIN)
+// for Coulomb and Yukawa.  The host clears CHECK for source ranges whose
+// bounding box cannot share a point with the targets' box, where the skip
+// can never trigger; cluster sources never need it (an accepted cluster is
+// separated from its targets).
 // ---------------------------------------------------------------------
-template <int KER, bool SS>
-__device__ __forceinline__ double pair_pot(double dx, double dy, double dz, double a, double b,
-                                           double tq, const KParams &kp)
+template <int KER, bool SS, bool CHECK>
+__device__ __forceinline__ void pair_pot(double dx, double dy, double dz, double a, double b,
+                                         double tq, const KParams &kp, double &acc)
 {
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if constexpr (KER == KC) {
-        const double ir = rsq64(r2);
-        if constexpr (!SS) {
-            return r2 > DBL_MIN ? a * ir : 0.0;
-        } else {
-            const double r = r2 * ir;
-            const double v = fma(-tq * b, exp(-r2 * kp.inv_a2), a) * ir;
-            return r > DBL_MIN ? v : 0.0;
-        }
+        double ir = rsq64(r2);
+        if constexpr (CHECK) ir = r2 > (SS ? 0.0 : DBL_MIN) ? ir : 0.0;
+        if constexpr (!SS) acc = fma(a, ir, acc);
+        else acc = fma(fma(-tq * b, exp(-r2 * kp.inv_a2), a), ir, acc);
     } else if constexpr (KER == KY) {
         const double ir = rsq64(r2);
-        const double r = r2 * ir;
-        const double e = exp(-kp.p0 * r) * ir;
-        double v;
-        if constexpr (!SS) v = a * e;
-        else v = fma(-tq, b, a) * e;
-        return r > DBL_MIN ? v : 0.0;
+        double e = exp(-kp.p0 * (r2 * ir)) * ir;
+        if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
+        if constexpr (!SS) acc = fma(a, e, acc);
+        else acc = fma(fma(-tq, b, a), e, acc);
     } else {
         const double ir = rsq64(r2 + kp.eps2);
-        if constexpr (!SS) return a * ir;
-        else return fma(-tq * b, exp(-r2 * kp.inv_a2), a) * ir;
+        if constexpr (!SS) acc = fma(a, ir, acc);
+        else acc = fma(fma(-tq * b, exp(-r2 * kp.inv_a2), a), ir, acc);
     }
 }
 
@@ -89,6 +92,7 @@ __device__ __forceinline__ double pair_pot(double dx, double dy, double dz, doub
 //   RegCoul   SK: Q += a/sqrt(r^2+e^2)
 // (coulomb_cp.c:46-55, coulomb_ss_cp.c:54-55, coulomb_ss_cc.c:54-55,
 //  yukawa_cp.c:52, yukawa_ss_cp.c:55-56, yukawa_ss_cc.c:55-56, regularized-coulomb_cp.c:52)
+// The reference applies no skip in these kernels.
 template <int KER, bool SS>
 __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, double a, double b,
                                           const KParams &kp, double &q, double &w)
@@ -109,13 +113,74 @@ __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, doubl
 }
 
 // ---------------------------------------------------------------------
+// Mixed precision pair kernels.  The displacement is formed in FP64 and
+// rounded once to FP32 (so its relative error is 2^-24 even for the closest
+// pairs), the kernel value G is evaluated in FP32 with the hardware
+// v_rsq_f32 / v_exp_f32, and the product with the FP64 charge is
+// accumulated per target in FP64.  Each FP32 term is exact in FP64, so the
+// accumulation carries no FP32 rounding (the compensation of a Kahan sum,
+// at two instructions per pair instead of four).  For subtraction the
+// combination (a - tq*b*g) is FP32.
+// kp32: {c_gauss = -log2(e)/k^2, c_yuk = -log2(e)*kappa, eps2}
+// ---------------------------------------------------------------------
+struct KP32 { float cg, cy, eps2; };
+
+__device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float exp2_32(float x) { return __builtin_amdgcn_exp2f(x); }
+
+template <int KER, bool SS, bool CHECK>
+__device__ __forceinline__ void pair_pot32(float dx, float dy, float dz, double a, float af,
+                                           float bf, float tqf, const KP32 &k, double &acc)
+{
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        float ir = rsq32(r2);
+        if constexpr (CHECK) ir = r2 > 0.0f ? ir : 0.0f;
+        if constexpr (!SS) acc = fma(a, (double)ir, acc);
+        else acc += (double)(fmaf(-tqf * bf, exp2_32(r2 * k.cg), af) * ir);
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2);
+        float e = exp2_32((r2 * ir) * k.cy) * ir;
+        if constexpr (CHECK) e = r2 > 0.0f ? e : 0.0f;
+        if constexpr (!SS) acc = fma(a, (double)e, acc);
+        else acc += (double)(fmaf(-tqf, bf, af) * e);
+    } else {
+        const float ir = rsq32(r2 + k.eps2);
+        if constexpr (!SS) acc = fma(a, (double)ir, acc);
+        else acc += (double)(fmaf(-tqf * bf, exp2_32(r2 * k.cg), af) * ir);
+    }
+}
+
+template <int KER, bool SS>
+__device__ __forceinline__ void pair_grid32(float dx, float dy, float dz, double a, double b,
+                                            const KP32 &k, double &q, double &w)
+{
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if constexpr (KER == KC) {
+        const float ir = rsq32(r2);
+        q = fma(a, (double)ir, q);
+        if constexpr (SS) w = fma(b, (double)(exp2_32(r2 * k.cg) * ir), w);
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2);
+        const double e = (double)(exp2_32((r2 * ir) * k.cy) * ir);
+        q = fma(a, e, q);
+        if constexpr (SS) w = fma(b, e, w);
+    } else {
+        q = fma(a, (double)rsq32(r2 + k.eps2), q);
+    }
+}
+
+// ---------------------------------------------------------------------
 // Interaction kernel.  One block = one tile of up to TPB*TPT targets of one
 // key (a target leaf, or the grid of a target node).  Sources stream through
 // LDS in chunks of TPB points: first the key's source clusters, then its
 // source particle ranges.  Each target's sum stays in registers and is
-// stored once, so no atomics are needed.
+// stored once, so no atomics are needed.  MIXED selects the FP32 pair
+// kernels above; everything else (staging, targets, totals) is FP64.
 // ---------------------------------------------------------------------
-template <int KER, bool SS, bool GRID>
+constexpr unsigned SEG_CHECK = 0x80000000u;  // Seg.y flag: apply the singularity skip
+
+template <int KER, bool SS, bool GRID, bool MIXED>
 __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 {
     const Tile tile = A.tiles[blockIdx.x];
@@ -124,10 +189,14 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 
     __shared__ double4 Ls[TPB];
     __shared__ double Lb[SS ? TPB : 1];
+    __shared__ float2 Lf[(MIXED && SS && !GRID) ? TPB : 1];
 
     double Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT], acc[TPT], accw[TPT];
+    float Tqf[TPT];
     int tix[TPT];
     const int nl = A.nl;
+    const KP32 k32{(float)(-1.4426950408889634 * A.kp.inv_a2), (float)(-1.4426950408889634 * A.kp.p0),
+                   (float)A.kp.eps2};
 #pragma unroll
     for (int u = 0; u < TPT; u++) {
         const int local = tid + u * TPB;
@@ -143,22 +212,35 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
             Tx[u] = A.tx[i]; Ty[u] = A.ty[i]; Tz[u] = A.tz[i];
             Tq[u] = SS ? A.tq[i] : 0.0;
         }
+        Tqf[u] = (float)Tq[u];
         acc[u] = 0.0; accw[u] = 0.0;
     }
 
-    auto consume = [&](int n) {
+    auto consume = [&](int n, auto check_tag) {
+        constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll 2
         for (int j = 0; j < n; ++j) {
             const double4 s = Ls[j];
             const double b = SS ? Lb[j] : 0.0;
+            float2 f = make_float2(0.f, 0.f);
+            if constexpr (MIXED && SS && !GRID) f = Lf[j];
 #pragma unroll
             for (int u = 0; u < TPT; u++) {
-                const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
-                if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
-                else acc[u] += pair_pot<KER, SS>(dx, dy, dz, s.w, b, Tq[u], A.kp);
+                if constexpr (MIXED) {
+                    const float dx = (float)(Tx[u] - s.x), dy = (float)(Ty[u] - s.y),
+                                dz = (float)(Tz[u] - s.z);
+                    if constexpr (GRID) pair_grid32<KER, SS>(dx, dy, dz, s.w, b, k32, acc[u], accw[u]);
+                    else pair_pot32<KER, SS, CHECK>(dx, dy, dz, s.w, f.x, f.y, Tqf[u], k32, acc[u]);
+                } else {
+                    const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                    if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
+                    else pair_pot<KER, SS, CHECK>(dx, dy, dz, s.w, b, Tq[u], A.kp, acc[u]);
+                }
             }
         }
     };
+    using NoCheck = std::integral_constant<bool, false>;
+    using Check = std::integral_constant<bool, KER != KR>;
 
     // source clusters (PC in PC mode, CC in CC mode)
     const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
@@ -169,19 +251,26 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
             __syncthreads();
             if (tid < n) {
                 const size_t g = base + off + tid;
-                Ls[tid] = make_double4(A.cx[g], A.cy[g], A.cz[g], A.cq[g]);
-                if constexpr (SS) Lb[tid] = A.cw[g];
+                const double a = A.cq[g];
+                Ls[tid] = make_double4(A.cx[g], A.cy[g], A.cz[g], a);
+                if constexpr (SS) {
+                    const double b = A.cw[g];
+                    Lb[tid] = b;
+                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)a, (float)b);
+                }
             }
             __syncthreads();
-            consume(n);
+            consume(n, NoCheck{});
         }
     }
     // source particle ranges (PP, or CP into grids)
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
-        for (int off = 0; off < sg.y; off += TPB) {
-            const int n = min(TPB, sg.y - off);
+        const bool check = !GRID && ((unsigned)sg.y & SEG_CHECK);
+        const int cnt = (int)((unsigned)sg.y & ~SEG_CHECK);
+        for (int off = 0; off < cnt; off += TPB) {
+            const int n = min(TPB, cnt - off);
             __syncthreads();
             if (tid < n) {
                 const int g = sg.x + off + tid;
@@ -189,199 +278,14 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
                 double a = q, b = 0.0;
                 if constexpr (SS) { b = A.sw[g]; a = q * b; }
                 Ls[tid] = make_double4(A.sx[g], A.sy[g], A.sz[g], a);
-                if constexpr (SS) Lb[tid] = b;
-            }
-            __syncthreads();
-            consume(n);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < TPT; u++) {
-        if (tix[u] < 0) continue;
-        if constexpr (GRID) {
-            const size_t o = (size_t)tile.w * A.P + tix[u];
-            A.out1[o] = acc[u];
-            if constexpr (SS) A.out2[o] = accw[u];
-        } else {
-            A.out1[tix[u]] = acc[u];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------
-// Mixed precision interaction kernel: FP32 pair evaluation, FP64 target
-// coordinates.  The displacement is formed in FP64 and rounded once, the
-// pair value is FP32 (v_rsq_f32 plus one Newton step), and each target
-// accumulates a chunk of TPB pairs in FP32 with Kahan compensation before
-// adding the chunk into an FP64 total.
-// ---------------------------------------------------------------------
-__device__ __forceinline__ float rsq32(float x)
-{
-    float y = __builtin_amdgcn_rsqf(x);
-    // one Newton step: y*(1.5 - 0.5*x*y*y)
-    const float h = 0.5f * x * y;
-    return y * fmaf(-h, y, 1.5f);
-}
-
-template <int KER, bool SS>
-__device__ __forceinline__ float pair_pot32(float dx, float dy, float dz, float a, float b, float tq,
-                                            const KParams &kp)
-{
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    if constexpr (KER == KC) {
-        const float ir = rsq32(r2);
-        if constexpr (!SS) {
-            return r2 > FLT_MIN ? a * ir : 0.0f;
-        } else {
-            const float v = fmaf(-tq * b, __expf(-r2 * (float)kp.inv_a2), a) * ir;
-            return r2 > FLT_MIN ? v : 0.0f;
-        }
-    } else if constexpr (KER == KY) {
-        const float ir = rsq32(r2);
-        const float e = __expf(-(float)kp.p0 * (r2 * ir)) * ir;
-        const float v = SS ? fmaf(-tq, b, a) * e : a * e;
-        return r2 > FLT_MIN ? v : 0.0f;
-    } else {
-        const float ir = rsq32(r2 + (float)kp.eps2);
-        if constexpr (!SS) return a * ir;
-        else return fmaf(-tq * b, __expf(-r2 * (float)kp.inv_a2), a) * ir;
-    }
-}
-
-template <int KER, bool SS>
-__device__ __forceinline__ void pair_grid32(float dx, float dy, float dz, float a, float b,
-                                            const KParams &kp, float &q, float &w)
-{
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    if constexpr (KER == KC) {
-        const float ir = rsq32(r2);
-        q = fmaf(a, ir, q);
-        if constexpr (SS) w = fmaf(__expf(-r2 * (float)kp.inv_a2) * b, ir, w);
-    } else if constexpr (KER == KY) {
-        const float ir = rsq32(r2);
-        const float e = __expf(-(float)kp.p0 * (r2 * ir)) * ir;
-        q = fmaf(a, e, q);
-        if constexpr (SS) w = fmaf(b, e, w);
-    } else {
-        q = fmaf(a, rsq32(r2 + (float)kp.eps2), q);
-    }
-}
-
-// Sources are staged relative to the tile's first target (FP64 difference,
-// rounded once), so the FP32 displacement loses nothing to large absolute
-// coordinates.
-template <int KER, bool SS, bool GRID>
-__global__ __launch_bounds__(TPB) void k_interact32(InteractArgs A)
-{
-    const Tile tile = A.tiles[blockIdx.x];
-    const int key = tile.x;
-    const int tid = threadIdx.x;
-
-    __shared__ float4 Ls[TPB];
-    __shared__ float Lb[SS ? TPB : 1];
-
-    const int nl = A.nl;
-    double ox, oy, oz;  // tile origin
-    {
-        if constexpr (GRID) {
-            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
-            ox = 0.5 * (tb[0] + tb[nl - 1]);
-            oy = 0.5 * (tb[nl] + tb[2 * nl - 1]);
-            oz = 0.5 * (tb[2 * nl] + tb[3 * nl - 1]);
-        } else {
-            ox = A.tx[tile.y]; oy = A.ty[tile.y]; oz = A.tz[tile.y];
-        }
-    }
-    float Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT];
-    float cs[TPT], cc[TPT], csw[TPT], ccw[TPT];   // FP32 chunk sums + Kahan carries
-    double acc[TPT], accw[TPT];
-    int tix[TPT];
-#pragma unroll
-    for (int u = 0; u < TPT; u++) {
-        const int local = tid + u * TPB;
-        const int lc = local < tile.z ? local : 0;
-        tix[u] = local < tile.z ? tile.y + local : -1;
-        double x, y, z, q = 0.0;
-        if constexpr (GRID) {
-            const int j = tile.y + lc;
-            const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
-            x = tb[k1]; y = tb[nl + k2]; z = tb[2 * nl + k3];
-        } else {
-            const int i = tile.y + lc;
-            x = A.tx[i]; y = A.ty[i]; z = A.tz[i];
-            if (SS) q = A.tq[i];
-        }
-        Tx[u] = (float)(x - ox); Ty[u] = (float)(y - oy); Tz[u] = (float)(z - oz); Tq[u] = (float)q;
-        acc[u] = 0.0; accw[u] = 0.0;
-    }
-
-    auto consume = [&](int n) {
-#pragma unroll
-        for (int u = 0; u < TPT; u++) { cs[u] = 0.f; cc[u] = 0.f; csw[u] = 0.f; ccw[u] = 0.f; }
-#pragma unroll 4
-        for (int j = 0; j < n; ++j) {
-            const float4 s = Ls[j];
-            const float b = SS ? Lb[j] : 0.0f;
-#pragma unroll
-            for (int u = 0; u < TPT; u++) {
-                const float dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
-                if constexpr (GRID) {
-                    float q = 0.f, w = 0.f;
-                    pair_grid32<KER, SS>(dx, dy, dz, s.w, b, A.kp, q, w);
-                    // Kahan: cs += q
-                    float y = q - cc[u], t = cs[u] + y;
-                    cc[u] = (t - cs[u]) - y; cs[u] = t;
-                    if constexpr (SS) {
-                        float y2 = w - ccw[u], t2 = csw[u] + y2;
-                        ccw[u] = (t2 - csw[u]) - y2; csw[u] = t2;
-                    }
-                } else {
-                    const float v = pair_pot32<KER, SS>(dx, dy, dz, s.w, b, Tq[u], A.kp);
-                    float y = v - cc[u], t = cs[u] + y;
-                    cc[u] = (t - cs[u]) - y; cs[u] = t;
+                if constexpr (SS) {
+                    Lb[tid] = b;
+                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)a, (float)b);
                 }
             }
-        }
-#pragma unroll
-        for (int u = 0; u < TPT; u++) {
-            acc[u] += (double)cs[u] - (double)cc[u];
-            if (SS && GRID) accw[u] += (double)csw[u] - (double)ccw[u];
-        }
-    };
-
-    const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
-    for (int c = c0; c < c1; ++c) {
-        const size_t base = (size_t)A.clu_id[c] * A.P;
-        for (int off = 0; off < A.P; off += TPB) {
-            const int n = min(TPB, A.P - off);
             __syncthreads();
-            if (tid < n) {
-                const size_t g = base + off + tid;
-                Ls[tid] = make_float4((float)(A.cx[g] - ox), (float)(A.cy[g] - oy),
-                                      (float)(A.cz[g] - oz), (float)A.cq[g]);
-                if constexpr (SS) Lb[tid] = (float)A.cw[g];
-            }
-            __syncthreads();
-            consume(n);
-        }
-    }
-    const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
-    for (int s = s0; s < s1; ++s) {
-        const Seg sg = A.segs[s];
-        for (int off = 0; off < sg.y; off += TPB) {
-            const int n = min(TPB, sg.y - off);
-            __syncthreads();
-            if (tid < n) {
-                const int g = sg.x + off + tid;
-                double a = A.sq[g], b = 0.0;
-                if constexpr (SS) { b = A.sw[g]; a = a * b; }
-                Ls[tid] = make_float4((float)(A.sx[g] - ox), (float)(A.sy[g] - oy),
-                                      (float)(A.sz[g] - oz), (float)a);
-                if constexpr (SS) Lb[tid] = (float)b;
-            }
-            __syncthreads();
-            consume(n);
+            if (check) consume(n, Check{});
+            else consume(n, NoCheck{});
         }
     }
 #pragma unroll
@@ -402,11 +306,11 @@ hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
 {
     dim3 grid(a.ntiles), block(TPB);
     if (a.precision == 1) {
-        if (a.grid_targets) k_interact32<KER, SS, true><<<grid, block, 0, st>>>(a);
-        else k_interact32<KER, SS, false><<<grid, block, 0, st>>>(a);
+        if (a.grid_targets) k_interact<KER, SS, true, true><<<grid, block, 0, st>>>(a);
+        else k_interact<KER, SS, false, true><<<grid, block, 0, st>>>(a);
     } else {
-        if (a.grid_targets) k_interact<KER, SS, true><<<grid, block, 0, st>>>(a);
-        else k_interact<KER, SS, false><<<grid, block, 0, st>>>(a);
+        if (a.grid_targets) k_interact<KER, SS, true, false><<<grid, block, 0, st>>>(a);
+        else k_interact<KER, SS, false, false><<<grid, block, 0, st>>>(a);
     }
     return hipGetLastError();
 }
@@ -737,7 +641,7 @@ __global__ __launch_bounds__(256) void k_direct(KParams kp, int m, const double 
         __syncthreads();
         for (int j = 0; j < cnt; ++j) {
             const double4 s = Ls[j];
-            acc += pair_pot<KER, SS>(x - s.x, y - s.y, z - s.z, s.w, Lb[j], q, kp);
+            pair_pot<KER, SS, true>(x - s.x, y - s.y, z - s.z, s.w, Lb[j], q, kp, acc);
         }
     }
     if (i < m) out[i] = SS ? fma(kp.ss_param, q, acc) : acc;

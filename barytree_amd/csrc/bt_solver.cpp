@@ -70,8 +70,18 @@ void Plan::build_work()
         w.seg_off.push_back((int)w.seg_beg.size());
         w.clu_off.push_back((int)w.clu_id.size());
     };
-    auto add_seg = [](WorkCSR &w, int b, int c, long long nt) {
-        w.seg_beg.push_back(b); w.seg_cnt.push_back(c); w.pairs += nt * c;
+    auto add_seg = [](WorkCSR &w, int b, int c, long long nt, bool check = false) {
+        w.seg_beg.push_back(b); w.seg_cnt.push_back(c); w.seg_chk.push_back(check ? 1 : 0);
+        w.pairs += nt * c;
+    };
+    // May a source of node s coincide with a target of node t?  The
+    // singularity skip can only trigger when their closed boxes overlap (up to
+    // a 1e-150 gap: any pair across a wider gap has r^2 > DBL_MIN).
+    auto touch = [this](int t, int s) {
+        const double g = 1e-150;
+        return ttree_.xmin[t] - stree_.xmax[s] <= g && stree_.xmin[s] - ttree_.xmax[t] <= g &&
+               ttree_.ymin[t] - stree_.ymax[s] <= g && stree_.ymin[s] - ttree_.ymax[t] <= g &&
+               ttree_.zmin[t] - stree_.zmax[s] <= g && stree_.zmin[s] - ttree_.zmax[t] <= g;
     };
     auto add_clu = [P](WorkCSR &w, int id, long long nt) { w.clu_id.push_back(id); w.pairs += nt * P; };
     near_ = WorkCSR();
@@ -85,7 +95,7 @@ void Plan::build_work()
             const long long nt = ttree_.numpar[b];
             add_key(near_, b, ttree_.ibeg[b], ttree_.numpar[b]);
             for (int c : L_.approx[b]) add_clu(near_, c, nt);
-            for (int s : L_.pp[b]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt);
+            for (int s : L_.pp[b]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt, touch(b, s));
             close_key(near_);
         }
     } else if (p.compute_type == C_CP) {
@@ -98,7 +108,7 @@ void Plan::build_work()
         for (int leaf : ttree_.leaves) {
             const long long nt = ttree_.numpar[leaf];
             add_key(near_, leaf, ttree_.ibeg[leaf], ttree_.numpar[leaf]);
-            for (int b : pp_of[leaf]) add_seg(near_, stree_.ibeg[b], stree_.numpar[b], nt);
+            for (int b : pp_of[leaf]) add_seg(near_, stree_.ibeg[b], stree_.numpar[b], nt, touch(leaf, b));
             close_key(near_);
         }
         for (int t = 0; t < ttree_.n; t++) {
@@ -115,7 +125,7 @@ void Plan::build_work()
             add_key(near_, leaf, ttree_.ibeg[leaf], ttree_.numpar[leaf]);
             for (int a = leaf; a >= 0; a = ttree_.parent[a]) {
                 for (int c : L_.cc_pc[a]) add_clu(near_, c, nt);
-                for (int s : L_.pp[a]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt);
+                for (int s : L_.pp[a]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt, touch(leaf, s));
             }
             close_key(near_);
         }
@@ -383,7 +393,10 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
     }
     auto up_csr = [&](const WorkCSR &w, int **so, btk::Seg **sg, int **co, int **ci) -> std::string {
         std::vector<btk::Seg> segs(w.seg_beg.size());
-        for (size_t i = 0; i < segs.size(); i++) { segs[i].x = w.seg_beg[i]; segs[i].y = w.seg_cnt[i]; }
+        for (size_t i = 0; i < segs.size(); i++) {
+            segs[i].x = w.seg_beg[i];
+            segs[i].y = (int)((unsigned)w.seg_cnt[i] | (w.seg_chk[i] ? 0x80000000u : 0u));
+        }
         std::string rr;
         if (!(rr = up(so, w.seg_off)).empty()) return rr;
         if (!(rr = up(sg, segs)).empty()) return rr;

@@ -35,6 +35,7 @@ public:
     // FP64 direct sum on the GPU for caller-order target indices `idx`.
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
 
+    void set_precision(int p) { prm_.precision = p; }
     const Counts &counts() const { return counts_; }
     const Timings &timings() const { return tm_; }
     const Params &params() const { return prm_; }

@@ -94,7 +94,7 @@ def lib():
         l.BaryTree_GetLastError.restype = ctypes.c_char_p
         for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
-                  "BaryTreePlan_DirectSample"):
+                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
@@ -199,6 +199,10 @@ class Plan:
         if rc != 0:
             raise RuntimeError(f"BaryTreePlan_Execute failed: {last_error()}")
         return out
+
+    def set_precision(self, precision):
+        if lib().BaryTreePlan_SetPrecision(self._h, ctypes.c_int(precision)) != 0:
+            raise RuntimeError(f"BaryTreePlan_SetPrecision failed: {last_error()}")
 
     def stats(self):
         s = Stats()

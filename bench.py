@@ -1,0 +1,192 @@
+"""Benchmark of the BaryTree interaction path on one MI355X.
+
+Workload (the headline configuration): Coulomb kernel with singularity
+subtraction (kappa = 0.5), N = 16M particles uniform in [-1,1]^3 (sources =
+targets, separate arrays), charges uniform in [-1,1], weights 1,
+cluster-cluster treecode, theta = 0.8, degree 10, max leaf 8000.
+
+A step is one execution of the GPU path (upward pass, near and far field,
+downpass, correction, scatter to caller order) with its inputs resident in
+HBM; the host work (trees, interaction lists, upload) is done once by the
+plan and reported separately.  Two forms are timed on the same plan: the
+fast form (FP32 pair evaluation with FP64 charges and accumulation) -- the
+one the metric is quoted on -- and the plain FP64 form.  Both are checked
+against an FP64 direct sum at 1000 stride-sampled targets.
+
+Prints one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_PER_EVAL = 17  # Coulomb-SS, operations as written in coulomb_ss_pp.c:40-47
+PEAK = {"fp32": 157.3, "fp64": 78.6}  # MI355X vector TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen_uniform(n, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = (rng.uniform(-1.0, 1.0, n) for _ in range(3))
+    q = rng.uniform(-1.0, 1.0, n)
+    w = np.ones(n)
+    return x, y, z, q, w
+
+
+def rel_l2(ref, val):
+    return float(np.sqrt(np.sum((ref - val) ** 2) / np.sum(ref ** 2)))
+
+
+def time_steps(plan, precision, steps, warmup):
+    import torch
+    plan.set_precision(precision)
+    for _ in range(warmup):
+        plan.execute(download=False)
+    torch.cuda.synchronize()
+    times, near = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(download=False)  # synchronizes on its own stream
+        st = plan.stats()
+        times.append(st["gpu_total_s"])
+        near.append(st["gpu_near_s"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps, float(np.mean(times)), float(np.mean(near)), st
+
+
+def cpu_baseline(n_evals):
+    """Times the oracle (FP64 restatement of the reference kernels, 1 core)
+    on a bounded sample and scales to the workload's evaluation count."""
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    m, n = 1000, 2_000_000
+    x, y, z, q, w = gen_uniform(n, 99)
+    t0 = time.perf_counter()
+    O.direct(x[:m], y[:m], z[:m], q[:m], x, y, z, q, w, 1, [0.5], 2)
+    dt = time.perf_counter() - t0
+    rate = m * n / dt
+    return {
+        "value": n_evals / rate,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 Coulomb-SS pair kernel (coulomb_ss_pp.c arithmetic), "
+                   f"{m} targets x {n} sources = {m * n:.1e} evaluations in {dt:.1f} s "
+                   f"({rate:.3e} eval/s, 1 core); workload time = its {n_evals:.3e} "
+                   f"point-wise evaluations / that rate (upward/downward passes excluded)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16_000_000)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md, 'How the path would shard').", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+
+    n = args.n
+    log(f"[bench] generating N={n}")
+    x, y, z, q, w = gen_uniform(n, 5)
+    targets = (x.copy(), y.copy(), z.copy(), q.copy())
+    sources = (x, y, z, q, w)
+    kernel_args = (bt.Kernel.COULOMB, [0.5], bt.Singularity.SUBTRACTION,
+                   bt.Approximation.LAGRANGE, bt.ComputeType.CLUSTER_CLUSTER, 0.8, 10, 8000, 8000)
+    t0 = time.perf_counter()
+    plan = bt.Plan(targets, sources, *kernel_args, precision=bt.PRECISION_MIXED)
+    host_setup = time.perf_counter() - t0
+    st0 = plan.stats()
+    log(f"[bench] plan {host_setup:.2f}s (tree {st0['host_tree_s']:.2f} lists "
+        f"{st0['host_lists_s']:.2f} work {st0['host_work_s']:.2f} h2d {st0['h2d_s']:.2f})")
+
+    # fast form (the metric)
+    wall_fast, gpu_fast, near_fast, st = time_steps(plan, bt.PRECISION_MIXED, args.steps, args.warmup)
+    log(f"[bench] mixed: {wall_fast:.3f} s/step (gpu {gpu_fast:.3f}, near {near_fast:.3f})")
+    idx = np.arange(0, n, n // 1000)[:1000]
+    out_fast = plan.execute()
+    direct = plan.direct_sample(idx)
+    err_fast = rel_l2(direct, out_fast[idx])
+    log(f"[bench] mixed relL2 vs FP64 direct (1000 targets): {err_fast:.3e}")
+
+    # plain FP64 form on the same plan
+    wall_plain, gpu_plain, near_plain, _ = time_steps(plan, bt.PRECISION_FP64, args.plain_steps, 1)
+    out_plain = plan.execute()
+    err_plain = rel_l2(direct, out_plain[idx])
+    log(f"[bench] fp64: {wall_plain:.3f} s/step (gpu {gpu_plain:.3f}), relL2 {err_plain:.3e}")
+    plan.close()
+
+    evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
+    near_evals = st["pp_ptwise"] + st["pc_ptwise"]
+    achieved = FLOPS_PER_EVAL * near_evals / near_fast / 1e12
+    roofline = {
+        "bound": "valu",
+        "kernel": "k_interact<Coulomb,SS,particle targets,mixed> (near field + PC)",
+        "achieved": round(achieved, 2),
+        "peak": PEAK["fp32"],
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK["fp32"], 4),
+        "traffic": None,
+    }
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the CPU oracle sample")
+        cpu = cpu_baseline(evals)
+        log(f"[bench] cpu baseline {cpu['value']:.1f} s (extrapolated)")
+
+    line = {
+        "metric": "time-to-solution N=16M at rel-L2 err <=1e-6 vs FP64 direct; speedup vs plain",
+        "value": round(wall_fast, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_fast * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 pair kernel, f64 charges/accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"Coulomb singularity subtraction kappa=0.5, N={n} uniform [-1,1]^3, "
+                         "cluster-cluster theta=0.8 degree 10 max leaf 8000"),
+            "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
+        },
+        "rel_l2_err": err_fast,
+        "speedup_vs_plain": round(wall_plain / wall_fast, 3),
+        "plain_fp64_s": round(wall_plain, 4),
+        "plain_fp64_rel_l2_err": err_plain,
+        "gpu_time_s": round(gpu_fast, 4),
+        "near_kernel_s": round(near_fast, 4),
+        "host_setup_s": round(host_setup, 3),
+        "end_to_end_s": round(host_setup + wall_fast, 3),
+        "interactions": evals,
+        "interactions_per_s": evals / wall_fast,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -141,6 +141,9 @@ int BaryTreePlan_Create(BaryTreePlan **plan, int numTargets, int numSources,
                         int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck,
                         double beta, int precision);
 int BaryTreePlan_Execute(BaryTreePlan *plan, double *outputArray);
+/* Switches the pair arithmetic of later Execute calls (BARYTREE_PRECISION_*)
+ * without rebuilding the plan. */
+int BaryTreePlan_SetPrecision(BaryTreePlan *plan, int precision);
 int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats);
 /* FP64 direct sum on the GPU (directdriver semantics,
  * src/drivers/directdriver.c:23-122) at m targets given by caller-order

@@ -144,3 +144,15 @@ def test_mixed_precision_accuracy(bt, ct, k, s):
     d = rel_l2(fp64, mixed)
     print(f"mixed vs fp64 ct={ct} k={k} s={s}: {d:.3e}")
     assert d < 1e-6
+
+
+def test_against_reference_outputs(bt):
+    """The HIP path against the reference's own compiled sources (committed
+    outputs, tests/golden/make_golden.py)."""
+    import os
+    from tests.golden.make_golden import M, N, cases
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_outputs.npz"))
+    for name, seed, args in cases():
+        t, s = uniform_cube(N, M, seed)
+        got = bt.solve(t, s, *args)
+        assert rel_l2(g[name], got) <= TOL_FP64, name
