@@ -80,10 +80,11 @@ hipError_t launch_finish(int n, const double *pot, const double *tq, double ss_p
                          const int *order, double *out, hipStream_t st);
 
 // FP64 direct sum (directdriver semantics) for sampled targets, no tree:
-// out[i] = sum_j G(t_i, s_j) (+ correction).  Used for error reporting.
+// out[i] = sum_j G(t_i, s_j) (+ correction).  The sources are split into
+// nchunks chunks over blocks; part holds nchunks*m partial sums.
 hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const double *tx,
                          const double *ty, const double *tz, const double *tq, int n,
                          const double *sx, const double *sy, const double *sz, const double *sq,
-                         const double *sw, double *out, hipStream_t st);
+                         const double *sw, double *part, int nchunks, double *out, hipStream_t st);
 
 }  // namespace btk

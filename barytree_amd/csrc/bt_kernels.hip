@@ -113,9 +113,9 @@ __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, doubl
 }
 
 // ---------------------------------------------------------------------
-// Mixed precision pair kernels.  The displacement is formed in FP64 and
-// rounded once to FP32 (so its relative error is 2^-24 even for the closest
-// pairs), the kernel value G is evaluated in FP32 with the hardware
+// Mixed precision pair kernels.  The displacement and r^2 are formed in FP64
+// and r^2 is rounded once to FP32 (so its relative error is 2^-24 even for
+// the closest pairs), the kernel value G is evaluated in FP32 with the hardware
 // v_rsq_f32 / v_exp_f32, and the product with the FP64 charge is
 // accumulated per target in FP64.  Each FP32 term is exact in FP64, so the
 // accumulation carries no FP32 rounding (the compensation of a Kahan sum,
@@ -129,10 +129,9 @@ __device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x
 __device__ __forceinline__ float exp2_32(float x) { return __builtin_amdgcn_exp2f(x); }
 
 template <int KER, bool SS, bool CHECK>
-__device__ __forceinline__ void pair_pot32(float dx, float dy, float dz, double a, float af,
-                                           float bf, float tqf, const KP32 &k, double &acc)
+__device__ __forceinline__ void pair_pot32(float r2, double a, float af, float bf, float tqf,
+                                           const KP32 &k, double &acc)
 {
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     if constexpr (KER == KC) {
         float ir = rsq32(r2);
         if constexpr (CHECK) ir = r2 > 0.0f ? ir : 0.0f;
@@ -152,10 +151,9 @@ __device__ __forceinline__ void pair_pot32(float dx, float dy, float dz, double 
 }
 
 template <int KER, bool SS>
-__device__ __forceinline__ void pair_grid32(float dx, float dy, float dz, double a, double b,
-                                            const KP32 &k, double &q, double &w)
+__device__ __forceinline__ void pair_grid32(float r2, double a, double b, const KP32 &k, double &q,
+                                            double &w)
 {
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     if constexpr (KER == KC) {
         const float ir = rsq32(r2);
         q = fma(a, (double)ir, q);
@@ -184,6 +182,7 @@ template <int KER, bool SS, bool GRID, bool MIXED>
 __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 {
     const Tile tile = A.tiles[blockIdx.x];
+    if (tile.z == 0) return;  // padding tile (uniform across the block)
     const int key = tile.x;
     const int tid = threadIdx.x;
 
@@ -227,10 +226,11 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 #pragma unroll
             for (int u = 0; u < TPT; u++) {
                 if constexpr (MIXED) {
-                    const float dx = (float)(Tx[u] - s.x), dy = (float)(Ty[u] - s.y),
-                                dz = (float)(Tz[u] - s.z);
-                    if constexpr (GRID) pair_grid32<KER, SS>(dx, dy, dz, s.w, b, k32, acc[u], accw[u]);
-                    else pair_pot32<KER, SS, CHECK>(dx, dy, dz, s.w, f.x, f.y, Tqf[u], k32, acc[u]);
+                    // FP64 displacement and r^2 (full-rate FP64 on gfx950), rounded once
+                    const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                    const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
+                    if constexpr (GRID) pair_grid32<KER, SS>(r2, s.w, b, k32, acc[u], accw[u]);
+                    else pair_pot32<KER, SS, CHECK>(r2, s.w, f.x, f.y, Tqf[u], k32, acc[u]);
                 } else {
                     const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
                     if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
@@ -615,21 +615,25 @@ __global__ void k_finish(int n, const double *pot, const double *tq, double ss, 
     if (i < n) out[order[i]] = ss != 0.0 ? fma(ss, tq[i], pot[i]) : pot[i];
 }
 
-// Direct sum: one target per thread, all sources through LDS.
+// Direct sum: one target per thread; blockIdx.y selects a chunk of the
+// sources, so the sum over all N sources spreads over many blocks.  Partial
+// sums go to part[chunk][m] and k_direct_reduce adds them in chunk order
+// (deterministic), then the subtraction correction.
 template <int KER, bool SS>
 __global__ __launch_bounds__(256) void k_direct(KParams kp, int m, const double *tx, const double *ty,
-                                                const double *tz, const double *tq, int n,
+                                                const double *tz, const double *tq, int n, int chunk,
                                                 const double *sx, const double *sy, const double *sz,
-                                                const double *sq, const double *sw, double *out)
+                                                const double *sq, const double *sw, double *part)
 {
     __shared__ double4 Ls[256];
     __shared__ double Lb[256];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int ic = i < m ? i : 0;
     const double x = tx[ic], y = ty[ic], z = tz[ic], q = SS ? tq[ic] : 0.0;
+    const int s0 = blockIdx.y * chunk, s1 = min(n, s0 + chunk);
     double acc = 0.0;
-    for (int off = 0; off < n; off += 256) {
-        const int cnt = min(256, n - off);
+    for (int off = s0; off < s1; off += 256) {
+        const int cnt = min(256, s1 - off);
         __syncthreads();
         if ((int)threadIdx.x < cnt) {
             const int g = off + threadIdx.x;
@@ -644,7 +648,17 @@ __global__ __launch_bounds__(256) void k_direct(KParams kp, int m, const double 
             pair_pot<KER, SS, true>(x - s.x, y - s.y, z - s.z, s.w, Lb[j], q, kp, acc);
         }
     }
-    if (i < m) out[i] = SS ? fma(kp.ss_param, q, acc) : acc;
+    if (i < m) part[(size_t)blockIdx.y * m + i] = acc;
+}
+
+__global__ void k_direct_reduce(int m, int nchunks, const double *part, const double *tq, double ss,
+                                double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int c = 0; c < nchunks; c++) acc += part[(size_t)c * m + i];
+    out[i] = ss != 0.0 ? fma(ss, tq[i], acc) : acc;
 }
 
 }  // namespace
@@ -695,12 +709,13 @@ hipError_t launch_finish(int n, const double *pot, const double *tq, double ss_p
 hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const double *tx,
                          const double *ty, const double *tz, const double *tq, int n,
                          const double *sx, const double *sy, const double *sz, const double *sq,
-                         const double *sw, double *out, hipStream_t st)
+                         const double *sw, double *part, int nchunks, double *out, hipStream_t st)
 {
-    if (m <= 0 || n <= 0) return hipSuccess;
-    dim3 g((m + 255) / 256), b(256);
+    if (m <= 0 || n <= 0 || nchunks <= 0) return hipSuccess;
+    const int chunk = (n + nchunks - 1) / nchunks;
+    dim3 g((m + 255) / 256, nchunks), b(256);
     const bool ss = singularity == 2;
-#define L(K, S) k_direct<K, S><<<g, b, 0, st>>>(kp, m, tx, ty, tz, tq, n, sx, sy, sz, sq, sw, out)
+#define L(K, S) k_direct<K, S><<<g, b, 0, st>>>(kp, m, tx, ty, tz, tq, n, chunk, sx, sy, sz, sq, sw, part)
     switch (kernel) {
         case KC: if (ss) L(KC, true); else L(KC, false); break;
         case KY: if (ss) L(KY, true); else L(KY, false); break;
@@ -708,6 +723,9 @@ hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const d
         default: return hipErrorInvalidValue;
     }
 #undef L
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_direct_reduce<<<(m + 255) / 256, 256, 0, st>>>(m, nchunks, part, tq, ss ? kp.ss_param : 0.0, out);
     return hipGetLastError();
 }
 

@@ -138,26 +138,46 @@ void Plan::build_work()
         }
     }
 
-    // tiles, heaviest first
+    // Tiles.  All tiles of one key read the same sources, so they should run
+    // at the same time on the same XCD and share its L2.  Workgroups are
+    // dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), so
+    // keys (groups of tiles) are dealt to per-XCD lists, heaviest group to the
+    // least loaded XCD, and block 8*j + x takes the j-th tile of XCD x's list.
+    // Lists are padded with empty tiles (count 0), which return at once.
     auto make_tiles = [&](const WorkCSR &w, bool grid, std::vector<btk::Tile> &tiles) {
-        std::vector<std::pair<long long, btk::Tile>> tmp;
+        constexpr int NXCD = 8;
+        struct Group { long long work; std::vector<btk::Tile> t; };
+        std::vector<Group> groups;
         for (size_t k = 0; k < w.key.size(); k++) {
             long long src = 0;
             for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
             src += (long long)(w.clu_off[k + 1] - w.clu_off[k]) * P;
+            Group g{0, {}};
             for (int o = 0; o < w.tcnt[k]; o += TILE_TARGETS) {
                 btk::Tile t;
                 t.x = (int)k;
                 t.y = w.tbeg[k] + o;
                 t.z = std::min(TILE_TARGETS, w.tcnt[k] - o);
                 t.w = grid ? w.key[k] : 0;
-                tmp.push_back({src * t.z, t});
+                g.work += (src + 1) * t.z;
+                g.t.push_back(t);
             }
+            if (!g.t.empty()) groups.push_back(std::move(g));
         }
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const auto &a, const auto &b) { return a.first > b.first; });
-        tiles.clear();
-        for (auto &e : tmp) tiles.push_back(e.second);
+        std::stable_sort(groups.begin(), groups.end(),
+                         [](const Group &a, const Group &b) { return a.work > b.work; });
+        std::vector<std::vector<btk::Tile>> lists(NXCD);
+        std::vector<long long> load(NXCD, 0);
+        for (const Group &g : groups) {
+            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += g.work;
+            lists[x].insert(lists[x].end(), g.t.begin(), g.t.end());
+        }
+        size_t L = 0;
+        for (auto &l : lists) L = std::max(L, l.size());
+        tiles.assign(L * NXCD, btk::Tile{0, 0, 0, 0});
+        for (int x = 0; x < NXCD; x++)
+            for (size_t j = 0; j < lists[x].size(); j++) tiles[j * NXCD + x] = lists[x][j];
     };
     make_tiles(near_, false, near_tiles_);
     make_tiles(grid_, true, grid_tiles_);
@@ -216,7 +236,8 @@ std::string Plan::validate() const
         for (int c : w.clu_id)
             if (c < 0 || c >= src_tree.n) return "cluster id";
         for (const btk::Tile &t : tiles) {
-            if (t.x < 0 || t.x >= nk || t.z <= 0 || t.z > TILE_TARGETS || t.y < 0) return "tile";
+            if (t.z == 0) continue;  // padding
+            if (t.x < 0 || t.x >= nk || t.z < 0 || t.z > TILE_TARGETS || t.y < 0) return "tile";
             if (grid) {
                 if (t.w < 0 || t.w >= ttree_.n || t.y + t.z > P) return "grid tile";
             } else if ((long long)t.y + t.z > M_) {
@@ -542,8 +563,13 @@ std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
         std::vector<double> ones(N_, 1.0);
         if (!(r = up(&dw, ones)).empty()) return r;
     }
+    // enough source chunks for about 2048 blocks, each chunk at least 4096 sources
+    const int tblocks = (m + 255) / 256;
+    int nchunks = std::max(1, std::min((2048 + tblocks - 1) / tblocks, (N_ + 4095) / 4096));
+    double *dpart = dalloc<double>((size_t)nchunks * m);
+    if (!dpart) return "hipMalloc failed";
     HIPCHK(btk::launch_direct(prm_.kernel, prm_.singularity, kp_, m, dx, dy, dz, dq, N_, d_sx_, d_sy_,
-                              d_sz_, d_sq_, dw, dout, st_));
+                              d_sz_, d_sq_, dw, dpart, nchunks, dout, st_));
     out.resize(m);
     HIPCHK(hipMemcpyAsync(out.data(), dout, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st_));
     HIPCHK(hipStreamSynchronize(st_));

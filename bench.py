@@ -96,7 +96,17 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of 2 GiB (HBM-traffic counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda").uniform_()
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": 2 ** 31, "bytes_written": 2 ** 31}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md, 'How the path would shard').", file=sys.stderr)
@@ -130,10 +140,12 @@ def main():
     log(f"[bench] mixed relL2 vs FP64 direct (1000 targets): {err_fast:.3e}")
 
     # plain FP64 form on the same plan
-    wall_plain, gpu_plain, near_plain, _ = time_steps(plan, bt.PRECISION_FP64, args.plain_steps, 1)
-    out_plain = plan.execute()
-    err_plain = rel_l2(direct, out_plain[idx])
-    log(f"[bench] fp64: {wall_plain:.3f} s/step (gpu {gpu_plain:.3f}), relL2 {err_plain:.3e}")
+    wall_plain, err_plain = None, None
+    if args.plain_steps > 0:
+        wall_plain, gpu_plain, near_plain, _ = time_steps(plan, bt.PRECISION_FP64, args.plain_steps, 1)
+        out_plain = plan.execute()
+        err_plain = rel_l2(direct, out_plain[idx])
+        log(f"[bench] fp64: {wall_plain:.3f} s/step (gpu {gpu_plain:.3f}), relL2 {err_plain:.3e}")
     plan.close()
 
     evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
@@ -173,8 +185,8 @@ def main():
             "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
         },
         "rel_l2_err": err_fast,
-        "speedup_vs_plain": round(wall_plain / wall_fast, 3),
-        "plain_fp64_s": round(wall_plain, 4),
+        "speedup_vs_plain": round(wall_plain / wall_fast, 3) if wall_plain else None,
+        "plain_fp64_s": round(wall_plain, 4) if wall_plain else None,
         "plain_fp64_rel_l2_err": err_plain,
         "gpu_time_s": round(gpu_fast, 4),
         "near_kernel_s": round(near_fast, 4),
