@@ -20,7 +20,8 @@ HOST_SRCS = ["bt_host.cpp", "bt_solver.cpp", "bt_capi.cpp"]
 HIP_SRCS = ["bt_kernels.hip"]
 
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function"]
+              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function",
+              "-fopenmp"]
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              # packed FP32 instructions stay off
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -47,8 +48,9 @@ def build(verbose=False):
         run([f"{ROCM}/bin/hipcc", *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    run([f"{ROCM}/bin/hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs,
-         f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + f"{ROCM}/lib"])
+    # g++ links: the host objects use GCC's OpenMP runtime (libgomp)
+    run(["g++", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp",
+         "-Wl,-rpath," + f"{ROCM}/lib"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -137,44 +137,65 @@ struct Splitter {
     }
 };
 
+// One node of a subtree under construction; child ids are local to the
+// subtree's own node vector (preorder, root = 0).
+struct NodeRec {
+    int b, e, level, nchild;
+    int child[8];
+    double lo[3], hi[3], mid[3], radius;
+};
+
+// Ranges at least this long are split or descended in parallel tasks.
+constexpr int PAR_RANGE = 1 << 17;
+
 struct Builder {
-    Tree &t;
     Particles &p;
     Splitter sp;
     int maxpar;
     bool leaves_only;
 
-    // returns node id, or -1 for an unrecorded interior node (batches)
-    int build(int b, int e, int level, int parent)
+    void bounds(int b, int e, double lo[3], double hi[3]) const
+    {
+        double xn = p.x[b], xx = p.x[b], yn = p.y[b], yx = p.y[b], zn = p.z[b], zx = p.z[b];
+        if (e - b >= PAR_RANGE) {
+#pragma omp parallel for reduction(min : xn, yn, zn) reduction(max : xx, yx, zx) schedule(static)
+            for (int i = b + 1; i < e; i++) {
+                xn = std::min(xn, p.x[i]); xx = std::max(xx, p.x[i]);
+                yn = std::min(yn, p.y[i]); yx = std::max(yx, p.y[i]);
+                zn = std::min(zn, p.z[i]); zx = std::max(zx, p.z[i]);
+            }
+        } else {
+            for (int i = b + 1; i < e; i++) {
+                if (xn > p.x[i]) xn = p.x[i];
+                if (xx < p.x[i]) xx = p.x[i];
+                if (yn > p.y[i]) yn = p.y[i];
+                if (yx < p.y[i]) yx = p.y[i];
+                if (zn > p.z[i]) zn = p.z[i];
+                if (zx < p.z[i]) zx = p.z[i];
+            }
+        }
+        lo[0] = xn; hi[0] = xx; lo[1] = yn; hi[1] = yx; lo[2] = zn; hi[2] = zx;
+    }
+
+    // Builds the subtree of [b, e) into `out` (preorder, local ids).  For
+    // batches (leaves_only) only leaves are appended.  The min and max are
+    // order independent and every split works in place on its own range, so
+    // the result is the serial result whatever runs in parallel.
+    void build(int b, int e, int level, std::vector<NodeRec> &out, int &max_depth)
     {
         const int numpar = e - b;
-        double x_min = p.x[b], x_max = p.x[b], y_min = p.y[b], y_max = p.y[b];
-        double z_min = p.z[b], z_max = p.z[b];
-        for (int i = b + 1; i < e; i++) {
-            if (x_min > p.x[i]) x_min = p.x[i];
-            if (x_max < p.x[i]) x_max = p.x[i];
-            if (y_min > p.y[i]) y_min = p.y[i];
-            if (y_max < p.y[i]) y_max = p.y[i];
-            if (z_min > p.z[i]) z_min = p.z[i];
-            if (z_max < p.z[i]) z_max = p.z[i];
-        }
+        NodeRec r{};
+        r.b = b; r.e = e; r.level = level; r.nchild = 0;
+        bounds(b, e, r.lo, r.hi);
         // tree_linked_list.c:53-69
-        const double xl = x_max - x_min, yl = y_max - y_min, zl = z_max - z_min;
-        const double x_mid = (x_max + x_min) / 2.0, y_mid = (y_max + y_min) / 2.0;
-        const double z_mid = (z_max + z_min) / 2.0;
-        const double radius = std::sqrt(xl * xl + yl * yl + zl * zl) / 2.0;
+        const double xl = r.hi[0] - r.lo[0], yl = r.hi[1] - r.lo[1], zl = r.hi[2] - r.lo[2];
+        for (int d = 0; d < 3; d++) r.mid[d] = (r.hi[d] + r.lo[d]) / 2.0;
+        r.radius = std::sqrt(xl * xl + yl * yl + zl * zl) / 2.0;
         const bool leaf = !(numpar > maxpar);
-        if (level + 1 > t.max_depth) t.max_depth = level + 1;
-        int idx = -1;
-        if (!leaves_only || leaf) {
-            idx = t.add_node();
-            t.ibeg[idx] = b; t.iend[idx] = e; t.numpar[idx] = numpar; t.level[idx] = level;
-            t.parent[idx] = parent;
-            t.xmin[idx] = x_min; t.xmax[idx] = x_max; t.ymin[idx] = y_min; t.ymax[idx] = y_max;
-            t.zmin[idx] = z_min; t.zmax[idx] = z_max;
-            t.xmid[idx] = x_mid; t.ymid[idx] = y_mid; t.zmid[idx] = z_mid; t.radius[idx] = radius;
-        }
-        if (leaf) return idx;
+        max_depth = std::max(max_depth, level + 1);
+        if (!leaves_only || leaf) out.push_back(r);
+        if (leaf) return;
+        const size_t self = out.size() - 1;
 
         // tree_linked_list.c:81-91 and partition.c:178-286
         const int maxch = numpar < 2 * maxpar ? 2 : (numpar < 4 * maxpar ? 4 : 8);
@@ -204,22 +225,39 @@ struct Builder {
         }
         int rb[8], re[8], nr = 1;
         rb[0] = b; re[0] = e;
-        const double mid[3] = {x_mid, y_mid, z_mid};
         for (int d = 0; d < 3; d++) {
             if (!dv[d]) continue;
+            int m[8];
+            // the nr ranges are disjoint: split them concurrently
+#pragma omp taskloop shared(m, rb, re) if (numpar >= PAR_RANGE && nr > 1) grainsize(1)
+            for (int i = 0; i < nr; i++) m[i] = sp.split(d, rb[i], re[i], r.mid[d]);
             for (int i = 0; i < nr; i++) {
-                int m = sp.split(d, rb[i], re[i], mid[d]);
-                rb[nr + i] = m; re[nr + i] = re[i]; re[i] = m;
+                rb[nr + i] = m[i]; re[nr + i] = re[i]; re[i] = m[i];
             }
             nr *= 2;
         }
-        for (int i = 0; i < nr; i++) {
-            if (rb[i] < re[i]) {
-                int c = build(rb[i], re[i], level + 1, idx);
-                if (idx >= 0) t.child[8 * idx + t.nchild[idx]++] = c;
+        // children: independent ranges, built into their own vectors
+        std::vector<NodeRec> sub[8];
+        int depth[8];
+        int nc = 0, cb[8], ce[8];
+        for (int i = 0; i < nr; i++)
+            if (rb[i] < re[i]) { cb[nc] = rb[i]; ce[nc] = re[i]; nc++; }
+        for (int c = 0; c < nc; c++) {
+            depth[c] = 0;
+#pragma omp task shared(sub, depth) if (ce[c] - cb[c] >= PAR_RANGE)
+            build(cb[c], ce[c], level + 1, sub[c], depth[c]);
+        }
+#pragma omp taskwait
+        // splice in preorder: self, child 0's subtree, child 1's subtree, ...
+        for (int c = 0; c < nc; c++) {
+            max_depth = std::max(max_depth, depth[c]);
+            const int off = (int)out.size();
+            if (!leaves_only) out[self].child[out[self].nchild++] = off;
+            for (NodeRec n : sub[c]) {
+                for (int k = 0; k < n.nchild; k++) n.child[k] += off;
+                out.push_back(n);
             }
         }
-        return idx;
     }
 };
 
@@ -235,18 +273,59 @@ void levels_rec(Tree &t, int idx, int level)
 
 }  // namespace
 
-void build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only)
+namespace {
+
+void finish_tree(Tree &t, const std::vector<NodeRec> &nodes, int max_depth, bool leaves_only)
 {
-    t = Tree();
-    if (p.n <= 0) return;
-    Builder bld{t, p, Splitter{p, !p.w.empty()}, max_per_leaf, leaves_only};
-    bld.build(0, p.n, 0, -1);
+    t.max_depth = max_depth;
+    for (const NodeRec &r : nodes) {
+        const int i = t.add_node();
+        t.ibeg[i] = r.b; t.iend[i] = r.e; t.numpar[i] = r.e - r.b; t.level[i] = r.level;
+        t.xmin[i] = r.lo[0]; t.ymin[i] = r.lo[1]; t.zmin[i] = r.lo[2];
+        t.xmax[i] = r.hi[0]; t.ymax[i] = r.hi[1]; t.zmax[i] = r.hi[2];
+        t.xmid[i] = r.mid[0]; t.ymid[i] = r.mid[1]; t.zmid[i] = r.mid[2]; t.radius[i] = r.radius;
+        t.nchild[i] = r.nchild;
+        for (int k = 0; k < r.nchild; k++) t.child[8 * i + k] = r.child[k];
+    }
+    for (int i = 0; i < t.n; i++)
+        for (int k = 0; k < t.nchild[i]; k++) t.parent[t.child[8 * i + k]] = i;
     if (leaves_only) {
         for (int i = 0; i < t.n; i++) t.leaves.push_back(i);
         return;
     }
     t.levels.assign(t.max_depth, {});
     levels_rec(t, 0, 0);
+}
+
+}  // namespace
+
+void build_trees(const std::vector<TreeJob> &jobs)
+{
+    const size_t nj = jobs.size();
+    std::vector<std::vector<NodeRec>> nodes(nj);
+    std::vector<int> depth(nj, 0);
+    std::vector<Builder> blds;
+    for (const TreeJob &j : jobs) {
+        *j.t = Tree();
+        blds.push_back(Builder{*j.p, Splitter{*j.p, !j.p->w.empty()}, j.max_per_leaf, j.leaves_only});
+    }
+#pragma omp parallel
+#pragma omp single
+    {
+        for (size_t k = 0; k < nj; k++) {
+            if (jobs[k].p->n <= 0) continue;
+#pragma omp task shared(nodes, depth, blds)
+            blds[k].build(0, jobs[k].p->n, 0, nodes[k], depth[k]);
+        }
+#pragma omp taskwait
+    }
+    for (size_t k = 0; k < nj; k++)
+        if (jobs[k].p->n > 0) finish_tree(*jobs[k].t, nodes[k], depth[k], jobs[k].leaves_only);
+}
+
+void build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only)
+{
+    build_trees({TreeJob{&t, &p, max_per_leaf, leaves_only}});
 }
 
 namespace {

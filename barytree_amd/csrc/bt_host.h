@@ -7,7 +7,9 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace bt {
@@ -45,12 +47,26 @@ void params_setup(Params &p, int kernel, int num_kp, const double *kp, int appro
 // error text (the reference prints and exits for these combinations).
 std::string params_check(const Params &p);
 
+// Allocator whose resize() leaves new elements uninitialised, so that large
+// arrays are first touched by the (parallel) loop that fills them.
+template <class T> struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U> void construct(U *p) { ::new (static_cast<void *>(p)) U; }
+    template <class U, class... A> void construct(U *p, A &&...a)
+    {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T> using HostVec = std::vector<T, NoInitAlloc<T>>;
+
 // Private SoA particle copy, permuted into tree order; order[i] is the
 // caller index (0-based) of sorted particle i.
 struct Particles {
     int n = 0;
-    std::vector<double> x, y, z, q, w;
-    std::vector<int> order;
+    HostVec<double> x, y, z, q, w;
+    HostVec<int> order;
 };
 
 // Array tree (src/tree/struct_tree.h), nodes in preorder.  ibeg/iend are
@@ -68,6 +84,16 @@ struct Tree {
 // Build the tree over p (permuting p).  leaves_only keeps only the leaves
 // (batches, src/tree/batches.c:25-90).
 void build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only);
+
+// Several trees at once (for example the source and target trees of a
+// cluster-cluster run), sharing one pool of OpenMP threads.
+struct TreeJob {
+    Tree *t;
+    Particles *p;
+    int max_per_leaf;
+    bool leaves_only;
+};
+void build_trees(const std::vector<TreeJob> &jobs);
 
 // Jagged interaction lists (src/interaction_lists/interaction_lists.c),
 // keyed by the "target" argument of InteractionLists_Make.

@@ -47,7 +47,7 @@ template <class T> T *Plan::dalloc(size_t n)
     return static_cast<T *>(p);
 }
 
-template <class T> std::string Plan::up(T **d, const std::vector<T> &h)
+template <class T, class A> std::string Plan::up(T **d, const std::vector<T, A> &h)
 {
     *d = dalloc<T>(h.size());
     if (!*d) return "hipMalloc failed";
@@ -280,31 +280,35 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
     if (!host_only) HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
 
     double t0 = now();
-    // private copies (the caller's arrays are never permuted)
+    // private copies (the caller's arrays are never permuted), in parallel
     S_ = Particles(); T_ = Particles();
     S_.n = N; T_.n = M;
-    S_.x.assign(sx, sx + N); S_.y.assign(sy, sy + N); S_.z.assign(sz, sz + N); S_.q.assign(sq, sq + N);
-    if (sw) S_.w.assign(sw, sw + N); else S_.w.assign(N, 1.0);
+    for (auto *v : {&S_.x, &S_.y, &S_.z, &S_.q, &S_.w}) v->resize(N);
+    for (auto *v : {&T_.x, &T_.y, &T_.z, &T_.q}) v->resize(M);
     S_.order.resize(N);
-    for (int i = 0; i < N; i++) S_.order[i] = i;
-    T_.x.assign(tx, tx + M); T_.y.assign(ty, ty + M); T_.z.assign(tz, tz + M);
-    if (tq) T_.q.assign(tq, tq + M); else T_.q.assign(M, 0.0);
     T_.order.resize(M);
-    for (int i = 0; i < M; i++) T_.order[i] = i;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        S_.x[i] = sx[i]; S_.y[i] = sy[i]; S_.z[i] = sz[i]; S_.q[i] = sq[i];
+        S_.w[i] = sw ? sw[i] : 1.0;
+        S_.order[i] = i;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < M; i++) {
+        T_.x[i] = tx[i]; T_.y[i] = ty[i]; T_.z[i] = tz[i]; T_.q[i] = tq ? tq[i] : 0.0;
+        T_.order[i] = i;
+    }
     double t1 = now();
     tm_ = Timings();
     tm_.host_copy = t1 - t0;
 
     // trees (treedriver.c:128/146, 364/375, 654/665)
     if (prm.compute_type == C_PC) {
-        build_tree(stree_, S_, prm.max_source_leaf, false);
-        build_tree(ttree_, T_, prm.max_target_leaf, true);
+        build_trees({{&stree_, &S_, prm.max_source_leaf, false}, {&ttree_, &T_, prm.max_target_leaf, true}});
     } else if (prm.compute_type == C_CP) {
-        build_tree(ttree_, T_, prm.max_target_leaf, false);
-        build_tree(stree_, S_, prm.max_source_leaf, true);
+        build_trees({{&ttree_, &T_, prm.max_target_leaf, false}, {&stree_, &S_, prm.max_source_leaf, true}});
     } else {
-        build_tree(stree_, S_, prm.max_source_leaf, false);
-        build_tree(ttree_, T_, prm.max_target_leaf, false);
+        build_trees({{&stree_, &S_, prm.max_source_leaf, false}, {&ttree_, &T_, prm.max_target_leaf, false}});
     }
     double t2 = now();
     tm_.host_tree = t2 - t1;

@@ -50,7 +50,7 @@ public:
 
 private:
     template <class T> T *dalloc(size_t n);
-    template <class T> std::string up(T **d, const std::vector<T> &h);
+    template <class T, class A> std::string up(T **d, const std::vector<T, A> &h);
     void free_all();
     void build_work();
     std::string validate() const;
