@@ -8,4 +8,5 @@ Python front end (interface.py).
 """
 from .interface import (  # noqa: F401
     LIB_PATH, PRECISION_FP64, PRECISION_MIXED, Approximation, ComputeType, Kernel, Plan,
-    Singularity, analyze, callTreedriver, last_error, last_stats, lib, solve)
+    Schedule, Singularity, analyze, analyze_batched, autotune, callTreedriver, last_error, last_stats, lib, solve,
+    solve_batched)

@@ -1,9 +1,15 @@
 // bt_capi.cpp -- extern "C" entry points of libBaryTree_gpu.so.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstdio>
-#include <new>
-#include <vector>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "../../include/BaryTreeInterface.h"
 #include "bt_solver.h"
@@ -43,6 +49,8 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->host_work_s = t.host_work; s->h2d_s = t.h2d;
     s->gpu_upward_s = t.gpu_upward; s->gpu_near_s = t.gpu_near; s->gpu_far_s = t.gpu_far;
     s->gpu_down_s = t.gpu_down; s->gpu_finish_s = t.gpu_finish; s->gpu_total_s = t.gpu_total;
+    s->num_systems = p.num_systems(); s->targets_per_thread = p.tpt();
+    s->persistent = p.persistent() ? 1 : 0;
 }
 
 int plan_create(bt::Plan &plan, int M, int N, const double *tx, const double *ty, const double *tz,
@@ -94,6 +102,259 @@ int solve(int M, int N, const double *tx, const double *ty, const double *tz, co
                     g_stats.host_tree_s, g_stats.host_lists_s, g_stats.gpu_total_s,
                     g_stats.gpu_near_s, g_stats.gpu_far_s);
     }
+    return 0;
+}
+
+// ---- batching ---------------------------------------------------------
+
+// 2^c <= max(M, N) < 2^(c+1)
+int size_class(int M, int N)
+{
+    const long long n = std::max(M, N);
+    int c = 0;
+    while (c + 1 < BARYTREE_SIZE_CLASSES && (2LL << c) <= n) c++;
+    return c;
+}
+
+std::string check_offsets(int nsys, const int *toffs, const int *soffs)
+{
+    if (nsys < 1) return "numSystems must be positive";
+    if (!toffs || !soffs) return "system offsets are NULL";
+    if (toffs[0] != 0 || soffs[0] != 0) return "system offsets must start at 0";
+    for (int s = 0; s < nsys; s++)
+        if (toffs[s + 1] <= toffs[s] || soffs[s + 1] <= soffs[s])
+            return "every system needs at least one target and one source (offsets increasing)";
+    return "";
+}
+
+// leaf size of size class c from a schedule: its own entry, else the nearest
+// set class, else 2P
+int schedule_leaf(const BaryTreeSchedule &sc, int c, int P)
+{
+    for (int d = 0; d < BARYTREE_SIZE_CLASSES; d++) {
+        if (c - d >= 0 && sc.leaf[c - d] > 0) return sc.leaf[c - d];
+        if (c + d < BARYTREE_SIZE_CLASSES && sc.leaf[c + d] > 0) return sc.leaf[c + d];
+    }
+    return 2 * P;
+}
+
+using TuneKey = std::tuple<int, int, int, int, int, double, double, double, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, BaryTreeSchedule> g_tune_cache;
+
+TuneKey tune_key(const bt::Params &p)
+{
+    return TuneKey{p.kernel, p.singularity, p.approximation, p.compute_type, p.degree, p.theta,
+                   p.size_check_factor, p.beta, p.precision};
+}
+
+struct BatchArrays {
+    const double *tx, *ty, *tz, *tq, *sx, *sy, *sz, *sq, *sw;
+};
+
+// Minimum GPU time of `reps` timed executions after one warm-up; < 0 on error.
+double time_plan(bt::Plan &plan, int reps)
+{
+    if (!plan.execute(false).empty()) return -1.0;
+    double best = 1e300;
+    for (int r = 0; r < reps; r++) {
+        if (!plan.execute(true).empty()) return -1.0;
+        best = std::min(best, plan.timings().gpu_total);
+    }
+    return best;
+}
+
+// A tuning sample: the listed systems, each repeated `rep` times, as one
+// concatenated batch with its own offsets.
+struct Sample {
+    std::vector<int> toffs{0}, soffs{0}, leaf;
+    std::vector<double> tx, ty, tz, tq, sx, sy, sz, sq, sw;
+    void add(const BatchArrays &a, const int *toffs_in, const int *soffs_in, int s, int lf)
+    {
+        const int t0 = toffs_in[s], t1 = toffs_in[s + 1], s0 = soffs_in[s], s1 = soffs_in[s + 1];
+        tx.insert(tx.end(), a.tx + t0, a.tx + t1); ty.insert(ty.end(), a.ty + t0, a.ty + t1);
+        tz.insert(tz.end(), a.tz + t0, a.tz + t1);
+        if (a.tq) tq.insert(tq.end(), a.tq + t0, a.tq + t1);
+        else tq.insert(tq.end(), (size_t)(t1 - t0), 0.0);
+        sx.insert(sx.end(), a.sx + s0, a.sx + s1); sy.insert(sy.end(), a.sy + s0, a.sy + s1);
+        sz.insert(sz.end(), a.sz + s0, a.sz + s1); sq.insert(sq.end(), a.sq + s0, a.sq + s1);
+        if (a.sw) sw.insert(sw.end(), a.sw + s0, a.sw + s1);
+        else sw.insert(sw.end(), (size_t)(s1 - s0), 1.0);
+        toffs.push_back(toffs.back() + (t1 - t0));
+        soffs.push_back(soffs.back() + (s1 - s0));
+        leaf.push_back(lf);
+    }
+    int nsys() const { return (int)leaf.size(); }
+    bt::PlanInput input(int tpt, int persistent) const
+    {
+        bt::PlanInput in;
+        in.nsys = nsys(); in.toffs = toffs.data(); in.soffs = soffs.data();
+        in.msl = leaf.data(); in.mtl = leaf.data(); in.tpt = tpt; in.persistent = persistent;
+        in.tx = tx.data(); in.ty = ty.data(); in.tz = tz.data(); in.tq = tq.data();
+        in.sx = sx.data(); in.sy = sy.data(); in.sz = sz.data(); in.sq = sq.data(); in.sw = sw.data();
+        return in;
+    }
+};
+
+constexpr long long TUNE_FILL = 1 << 20;  // targets per tuning plan
+constexpr int TUNE_SAMPLE = 8;            // sampled systems per size class
+constexpr int TUNE_REPS = 3;
+
+int autotune(const bt::Params &prm, int nsys, const int *toffs, const int *soffs,
+             const BatchArrays &a, BaryTreeSchedule *out)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    BaryTreeSchedule sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.targets_per_thread = 2;
+    sc.persistent = 1;
+    std::map<int, std::vector<int>> classes;
+    for (int s = 0; s < nsys; s++)
+        classes[size_class(toffs[s + 1] - toffs[s], soffs[s + 1] - soffs[s])].push_back(s);
+    const int P = prm.P;
+    const int cands[5] = {std::max(1, P / 2), P, 2 * P, 4 * P, 8 * P};
+    std::map<int, std::vector<int>> picked;  // sampled systems per class
+    for (auto &kv : classes) {
+        const std::vector<int> &members = kv.second;
+        std::vector<int> smp;
+        const int k = std::min<int>(TUNE_SAMPLE, (int)members.size());
+        for (int i = 0; i < k; i++) smp.push_back(members[(size_t)i * members.size() / k]);
+        picked[kv.first] = smp;
+        long long nt = 0;
+        for (int s : smp) nt += toffs[s + 1] - toffs[s];
+        const int rep = (int)std::max<long long>(1, (TUNE_FILL + nt - 1) / nt);
+        double best_t = 1e300;
+        int best_leaf = 2 * P;
+        for (int lf : cands) {
+            Sample S;
+            for (int r = 0; r < rep; r++)
+                for (int s : smp) S.add(a, toffs, soffs, s, lf);
+            bt::Plan plan;
+            std::string e = plan.create(prm, S.input(2, 1));
+            if (!e.empty()) return fail("autotune: " + e);
+            const double t = time_plan(plan, TUNE_REPS);
+            if (t < 0) return fail("autotune: execution failed");
+            sc.trials++;
+            // per target of the sample (the same for every candidate)
+            if (t < best_t) { best_t = t; best_leaf = lf; }
+        }
+        sc.leaf[kv.first] = best_leaf;
+    }
+    // tile shape and launch form on all class samples together
+    {
+        Sample S;
+        long long nt = 0;
+        for (auto &kv : picked)
+            for (int s : kv.second) nt += toffs[s + 1] - toffs[s];
+        const int rep = (int)std::max<long long>(1, (TUNE_FILL + nt - 1) / nt);
+        for (int r = 0; r < rep; r++)
+            for (auto &kv : picked)
+                for (int s : kv.second) S.add(a, toffs, soffs, s, sc.leaf[kv.first]);
+        double best_t = 1e300;
+        for (int tpt = 1; tpt <= 2; tpt++) {
+            bt::Plan plan;
+            std::string e = plan.create(prm, S.input(tpt, 0));
+            if (!e.empty()) return fail("autotune: " + e);
+            for (int pers = 0; pers <= 1; pers++) {
+                plan.set_persistent(pers != 0);
+                const double t = time_plan(plan, TUNE_REPS);
+                if (t < 0) return fail("autotune: execution failed");
+                sc.trials++;
+                if (t < best_t) { best_t = t; sc.targets_per_thread = tpt; sc.persistent = pers; }
+            }
+        }
+    }
+    sc.tune_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    *out = sc;
+    return 0;
+}
+
+int batched_params(bt::Params &prm, int kernel, int nkp, const double *kp, int singularity,
+                   int approximation, int compute_type, double theta, int degree,
+                   double size_check, double beta, int verbosity, int precision)
+{
+    // leaf sizes are per system; 1 stands in until they are resolved
+    bt::params_setup(prm, kernel, nkp, kp, approximation, singularity, compute_type, theta, degree,
+                     1, 1, size_check, beta, verbosity);
+    if (precision != BARYTREE_PRECISION_FP64 && precision != BARYTREE_PRECISION_MIXED)
+        return fail("unknown precision mode");
+    prm.precision = precision;
+    std::string e = bt::params_check(prm);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+int plan_create_batched(bt::Plan &plan, int nsys, const int *toffs, const int *soffs,
+                        const BatchArrays &a, bt::Params prm, const int *msl, const int *mtl,
+                        const BaryTreeSchedule *schedule, bool host_only = false)
+{
+    std::string e = check_offsets(nsys, toffs, soffs);
+    if (!e.empty()) return fail(e);
+    if (!a.tx || !a.ty || !a.tz || !a.sx || !a.sy || !a.sz || !a.sq)
+        return fail("One or more required particle arrays is unset");
+    std::vector<int> vs(nsys), vt(nsys);
+    BaryTreeSchedule sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.targets_per_thread = 2;
+    const bool beta_leaves = prm.beta >= 0;  // run_params.c:51-120 sets the leaf sizes
+    if (schedule) {
+        sc = *schedule;
+        if (sc.targets_per_thread != 1 && sc.targets_per_thread != 2)
+            return fail("schedule: targets_per_thread must be 1 or 2");
+    } else if (!beta_leaves && (!msl || !mtl) && host_only) {
+        return fail("leaf sizes or a schedule are needed (the autotuner runs on the GPU)");
+    } else if (!beta_leaves && (!msl || !mtl)) {
+        const TuneKey key = tune_key(prm);
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                sc = it->second;
+                have = true;
+                for (int s = 0; s < nsys && have; s++)
+                    if (sc.leaf[size_class(toffs[s + 1] - toffs[s], soffs[s + 1] - soffs[s])] <= 0)
+                        have = false;
+            }
+        }
+        if (!have) {
+            BaryTreeSchedule tuned;
+            int rc = autotune(prm, nsys, toffs, soffs, a, &tuned);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            BaryTreeSchedule &c = g_tune_cache[key];
+            const bool fresh = c.targets_per_thread == 0;
+            for (int k = 0; k < BARYTREE_SIZE_CLASSES; k++)
+                if (tuned.leaf[k] > 0) c.leaf[k] = tuned.leaf[k];
+            if (fresh) {
+                c.targets_per_thread = tuned.targets_per_thread;
+                c.persistent = tuned.persistent;
+            }
+            c.trials += tuned.trials;
+            c.tune_seconds += tuned.tune_seconds;
+            sc = c;
+        }
+    }
+    for (int s = 0; s < nsys; s++) {
+        if (beta_leaves) {
+            vs[s] = prm.max_source_leaf;
+            vt[s] = prm.max_target_leaf;
+            continue;
+        }
+        const int c = size_class(toffs[s + 1] - toffs[s], soffs[s + 1] - soffs[s]);
+        vs[s] = msl ? msl[s] : schedule_leaf(sc, c, prm.P);
+        vt[s] = mtl ? mtl[s] : schedule_leaf(sc, c, prm.P);
+    }
+    prm.max_source_leaf = vs[0];
+    prm.max_target_leaf = vt[0];
+    bt::PlanInput in;
+    in.nsys = nsys; in.toffs = toffs; in.soffs = soffs; in.msl = vs.data(); in.mtl = vt.data();
+    in.tpt = sc.targets_per_thread; in.persistent = sc.persistent;
+    in.tx = a.tx; in.ty = a.ty; in.tz = a.tz; in.tq = a.tq;
+    in.sx = a.sx; in.sy = a.sy; in.sz = a.sz; in.sq = a.sq; in.sw = a.sw;
+    e = plan.create(prm, in, host_only);
+    if (!e.empty()) return fail(e);
     return 0;
 }
 
@@ -234,6 +495,135 @@ int BaryTree_Analyze(int numTargets, int numSources, const double *targetX, cons
     if (!e.empty()) return fail(e);
     fill_stats(plan, stats);
     return 0;
+}
+
+
+int BaryTree_SolveBatched(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                          const double *targetX, const double *targetY, const double *targetZ,
+                          const double *targetValue, const double *sourceX,
+                          const double *sourceY, const double *sourceZ,
+                          const double *sourceValue, const double *sourceWeight,
+                          double *outputArray, int kernel, int numKernelParams,
+                          const double *kernelParams, int singularity, int approximation,
+                          int compute_type, double theta, int interpOrder,
+                          const int *maxPerSourceLeaf, const int *maxPerTargetLeaf,
+                          const BaryTreeSchedule *schedule, double sizeCheck, double beta,
+                          int verbosity, int precision)
+{
+    g_last_error.clear();
+    if (!outputArray) return fail("outputArray is NULL");
+    std::string e = check_offsets(numSystems, targetOffsets, sourceOffsets);
+    if (!e.empty()) return fail(e);
+    std::memset(outputArray, 0, (size_t)targetOffsets[numSystems] * sizeof(double));
+    bt::Params prm;
+    int rc = batched_params(prm, kernel, numKernelParams, kernelParams, singularity, approximation,
+                            compute_type, theta, interpOrder, sizeCheck, beta, verbosity, precision);
+    if (rc) return rc;
+    const BatchArrays a{targetX, targetY, targetZ, targetValue, sourceX, sourceY, sourceZ,
+                        sourceValue, sourceWeight};
+    bt::Plan plan;
+    rc = plan_create_batched(plan, numSystems, targetOffsets, sourceOffsets, a, prm,
+                             maxPerSourceLeaf, maxPerTargetLeaf, schedule);
+    if (rc) return rc;
+    e = plan.execute(true);
+    if (e.empty()) e = plan.download(outputArray);
+    if (!e.empty()) return fail(e);
+    fill_stats(plan, &g_stats);
+    g_have_stats = true;
+    if (verbosity > 0) {
+        std::printf("[BaryTree] batched: %d systems, host tree %.3fs lists %.3fs; gpu total %.3fs\n",
+                    numSystems, g_stats.host_tree_s, g_stats.host_lists_s, g_stats.gpu_total_s);
+    }
+    return 0;
+}
+
+int BaryTreePlan_CreateBatched(BaryTreePlan **plan, int numSystems, const int *targetOffsets,
+                               const int *sourceOffsets, const double *targetX,
+                               const double *targetY, const double *targetZ,
+                               const double *targetValue, const double *sourceX,
+                               const double *sourceY, const double *sourceZ,
+                               const double *sourceValue, const double *sourceWeight,
+                               int kernel, int numKernelParams, const double *kernelParams,
+                               int singularity, int approximation, int compute_type,
+                               double theta, int interpOrder, const int *maxPerSourceLeaf,
+                               const int *maxPerTargetLeaf, const BaryTreeSchedule *schedule,
+                               double sizeCheck, double beta, int precision)
+{
+    g_last_error.clear();
+    if (!plan) return fail("plan is NULL");
+    *plan = nullptr;
+    bt::Params prm;
+    int rc = batched_params(prm, kernel, numKernelParams, kernelParams, singularity, approximation,
+                            compute_type, theta, interpOrder, sizeCheck, beta, 0, precision);
+    if (rc) return rc;
+    BaryTreePlan *p = new (std::nothrow) BaryTreePlan;
+    if (!p) return fail("out of host memory");
+    const BatchArrays a{targetX, targetY, targetZ, targetValue, sourceX, sourceY, sourceZ,
+                        sourceValue, sourceWeight};
+    rc = plan_create_batched(p->plan, numSystems, targetOffsets, sourceOffsets, a, prm,
+                             maxPerSourceLeaf, maxPerTargetLeaf, schedule);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *plan = p;
+    return 0;
+}
+
+int BaryTree_AnalyzeBatched(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                            const double *targetX, const double *targetY, const double *targetZ,
+                            const double *sourceX, const double *sourceY, const double *sourceZ,
+                            int kernel, int numKernelParams, const double *kernelParams,
+                            int singularity, int approximation, int compute_type, double theta,
+                            int interpOrder, const int *maxPerSourceLeaf,
+                            const int *maxPerTargetLeaf, const BaryTreeSchedule *schedule,
+                            double sizeCheck, double beta, BaryTreeStats *stats)
+{
+    g_last_error.clear();
+    if (!stats) return fail("stats is NULL");
+    std::string e = check_offsets(numSystems, targetOffsets, sourceOffsets);
+    if (!e.empty()) return fail(e);
+    bt::Params prm;
+    int rc = batched_params(prm, kernel, numKernelParams, kernelParams, singularity, approximation,
+                            compute_type, theta, interpOrder, sizeCheck, beta, 0,
+                            BARYTREE_PRECISION_FP64);
+    if (rc) return rc;
+    // charges do not influence trees or lists
+    std::vector<double> zs((size_t)sourceOffsets[numSystems], 0.0);
+    const BatchArrays a{targetX, targetY, targetZ, nullptr, sourceX, sourceY, sourceZ, zs.data(),
+                        nullptr};
+    bt::Plan plan;
+    rc = plan_create_batched(plan, numSystems, targetOffsets, sourceOffsets, a, prm,
+                             maxPerSourceLeaf, maxPerTargetLeaf, schedule, true);
+    if (rc) return rc;
+    fill_stats(plan, stats);
+    return 0;
+}
+
+int BaryTree_Autotune(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                      const double *targetX, const double *targetY, const double *targetZ,
+                      const double *targetValue, const double *sourceX, const double *sourceY,
+                      const double *sourceZ, const double *sourceValue,
+                      const double *sourceWeight, int kernel, int numKernelParams,
+                      const double *kernelParams, int singularity, int approximation,
+                      int compute_type, double theta, int interpOrder, double sizeCheck,
+                      double beta, int precision, BaryTreeSchedule *schedule)
+{
+    g_last_error.clear();
+    if (!schedule) return fail("schedule is NULL");
+    std::string e = check_offsets(numSystems, targetOffsets, sourceOffsets);
+    if (!e.empty()) return fail(e);
+    if (!targetX || !targetY || !targetZ || !sourceX || !sourceY || !sourceZ || !sourceValue)
+        return fail("One or more required particle arrays is unset");
+    bt::Params prm;
+    int rc = batched_params(prm, kernel, numKernelParams, kernelParams, singularity, approximation,
+                            compute_type, theta, interpOrder, sizeCheck, beta, 0, precision);
+    if (rc) return rc;
+    if (prm.singularity == bt::S_SUBTRACTION && !sourceWeight)
+        return fail("Singularity subtraction requires weights for the sources");
+    const BatchArrays a{targetX, targetY, targetZ, targetValue, sourceX, sourceY, sourceZ,
+                        sourceValue, sourceWeight};
+    return autotune(prm, numSystems, targetOffsets, sourceOffsets, a, schedule);
 }
 
 }  // extern "C"

@@ -12,6 +12,10 @@ namespace btk {
 struct alignas(16) Tile { int x, y, z, w; };
 struct alignas(8) Seg { int x, y; };
 
+// Tile lists are 8 interleaved queues (tile 8*j + x = j-th tile of queue x),
+// one per XCD of an MI355X.
+constexpr int TILE_QUEUES = 8;
+
 // Kernel parameters, pre-derived on the host.
 struct KParams {
     double p0, p1;      // kernel_params[0], kernel_params[1]
@@ -26,8 +30,13 @@ struct InteractArgs {
     int kernel, singularity;
     int grid_targets;        // 0: particle targets, 1: grid points of target nodes
     int precision;           // 0: FP64, 1: FP32 pairs with compensated accumulation
-    int ntiles;
+    int tpt;                 // targets per thread of particle-target tiles (1 or 2;
+                             // grid-target tiles always use 2)
+    int ntiles;              // tile slots (8 * longest queue)
     const Tile *tiles;
+    int persistent;          // 0: one block per slot; 1: resident blocks pop from queues
+    int qlen[TILE_QUEUES];   // tiles per queue (persistent)
+    int *cursor;             // TILE_QUEUES counters (persistent; zeroed by the launch)
     const int *seg_off;
     const Seg *segs;        // {first source, count}
     const int *clu_off, *clu_id;

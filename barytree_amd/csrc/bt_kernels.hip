@@ -13,6 +13,7 @@
 //   k_down                   : cp_comp_pot(_SS)(_parent_to_child), downpass.c
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 #include <type_traits>
@@ -24,7 +25,8 @@ namespace btk {
 namespace {
 
 constexpr int TPB = 256;  // threads per block of the interaction kernels (4 waves)
-constexpr int TPT = 2;    // targets per thread
+constexpr int GRID_TPT = 2;  // targets per thread of grid-target tiles
+constexpr int NQ = TILE_QUEUES;  // tile queues, one per XCD
 constexpr int KC = 1, KY = 2, KR = 3;
 
 // ---------------------------------------------------------------------
@@ -169,26 +171,27 @@ __device__ __forceinline__ void pair_grid32(float r2, double a, double b, const 
 }
 
 // ---------------------------------------------------------------------
-// Interaction kernel.  One block = one tile of up to TPB*TPT targets of one
-// key (a target leaf, or the grid of a target node).  Sources stream through
-// LDS in chunks of TPB points: first the key's source clusters, then its
-// source particle ranges.  Each target's sum stays in registers and is
-// stored once, so no atomics are needed.  MIXED selects the FP32 pair
-// kernels above; everything else (staging, targets, totals) is FP64.
+// Interaction tile.  One block = one tile of up to TPB*TPT targets of one
+// key (a target leaf, or the grid of a target node); thread tid owns targets
+// tid + u*TPB.  Sources stream through LDS in chunks of TPB points: first the
+// key's source clusters, then its source particle ranges.  Each target's sum
+// stays in registers and is stored once, so no atomics are needed.  A wave
+// evaluates only the target slots u that hold a target for some lane (a
+// wave-uniform count), so a partly filled tile costs about its targets.
+// MIXED selects the FP32 pair kernels above; everything else (staging,
+// targets, totals) is FP64.
 // ---------------------------------------------------------------------
 constexpr unsigned SEG_CHECK = 0x80000000u;  // Seg.y flag: apply the singularity skip
 
-template <int KER, bool SS, bool GRID, bool MIXED>
-__global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
+template <int KER, bool SS, bool GRID, bool MIXED, int TPT>
+__device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile tile, double4 *Ls,
+                                              double *Lb, float2 *Lf)
 {
-    const Tile tile = A.tiles[blockIdx.x];
-    if (tile.z == 0) return;  // padding tile (uniform across the block)
     const int key = tile.x;
     const int tid = threadIdx.x;
-
-    __shared__ double4 Ls[TPB];
-    __shared__ double Lb[SS ? TPB : 1];
-    __shared__ float2 Lf[(MIXED && SS && !GRID) ? TPB : 1];
+    // target slots with at least one target in this wave (0..TPT), wave-uniform
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
+    const int nu = max(0, min(TPT, (tile.z - wbase + TPB - 1) / TPB));
 
     double Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT], acc[TPT], accw[TPT];
     float Tqf[TPT];
@@ -215,8 +218,10 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
         acc[u] = 0.0; accw[u] = 0.0;
     }
 
-    auto consume = [&](int n, auto check_tag) {
+    // n staged sources against the first NU target slots
+    auto consume_nu = [&](int n, auto check_tag, auto nu_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
+        constexpr int NU = decltype(nu_tag)::value;
 #pragma unroll 2
         for (int j = 0; j < n; ++j) {
             const double4 s = Ls[j];
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
             float2 f = make_float2(0.f, 0.f);
             if constexpr (MIXED && SS && !GRID) f = Lf[j];
 #pragma unroll
-            for (int u = 0; u < TPT; u++) {
+            for (int u = 0; u < NU; u++) {
                 if constexpr (MIXED) {
                     // FP64 displacement and r^2 (full-rate FP64 on gfx950), rounded once
                     const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
@@ -238,6 +243,10 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
                 }
             }
         }
+    };
+    auto consume = [&](int n, auto check_tag) {
+        if (nu == TPT) consume_nu(n, check_tag, std::integral_constant<int, TPT>{});
+        else if (TPT > 1 && nu == 1) consume_nu(n, check_tag, std::integral_constant<int, 1>{});
     };
     using NoCheck = std::integral_constant<bool, false>;
     using Check = std::integral_constant<bool, KER != KR>;
@@ -301,18 +310,102 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
     }
 }
 
+// Interaction kernel.  Tiles are laid out as 8 interleaved queues: tile
+// 8*j + x is the j-th tile of queue x, and each queue's tiles read sources
+// that its neighbours in the queue also read.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so block b runs on XCD b % 8.
+//   one-shot  : one block per tile slot (padding slots have count 0);
+//   persistent: a grid sized to the resident capacity; block b pops tiles
+//               from queue b % 8 through an atomic cursor (vector memory
+//               atomics), then steals from the other queues, and exits when
+//               every queue is empty -- every block reaches that exit.
+template <int KER, bool SS, bool GRID, bool MIXED, int TPT>
+__global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
+{
+    __shared__ double4 Ls[TPB];
+    __shared__ double Lb[SS ? TPB : 1];
+    __shared__ float2 Lf[(MIXED && SS && !GRID) ? TPB : 1];
+    if (!A.persistent) {
+        const Tile tile = A.tiles[blockIdx.x];
+        if (tile.z == 0) return;  // padding tile (uniform across the block)
+        interact_tile<KER, SS, GRID, MIXED, TPT>(A, tile, Ls, Lb, Lf);
+        return;
+    }
+    __shared__ int Lt;
+    const int home = blockIdx.x % NQ;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int t = -1;
+            for (int k = 0; k < NQ && t < 0; k++) {
+                const int q = (home + k) % NQ;
+                const int len = A.qlen[q];
+                if (__hip_atomic_load(&A.cursor[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len)
+                    continue;
+                const int j = atomicAdd(&A.cursor[q], 1);
+                if (j < len) t = j * NQ + q;
+            }
+            Lt = t;
+        }
+        __syncthreads();
+        const int t = Lt;
+        __syncthreads();
+        if (t < 0) break;
+        interact_tile<KER, SS, GRID, MIXED, TPT>(A, A.tiles[t], Ls, Lb, Lf);
+    }
+}
+
+__global__ void k_zero_ints(int *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+template <class F> int persistent_blocks(F kernel, int ntiles)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TPB, 0) != hipSuccess) return -1;
+    int nb = std::max(1, cus * std::max(1, per_cu));
+    nb = std::min(nb, ntiles);
+    return (nb + NQ - 1) / NQ * NQ;
+}
+
+template <int KER, bool SS, bool GRID, bool MIXED, int TPT>
+hipError_t interact_launch(const InteractArgs &a, hipStream_t st)
+{
+    auto kern = k_interact<KER, SS, GRID, MIXED, TPT>;
+    if (!a.persistent) {
+        kern<<<dim3(a.ntiles), dim3(TPB), 0, st>>>(a);
+        return hipGetLastError();
+    }
+    int real = 0;
+    for (int q = 0; q < NQ; q++) {
+        // the last tile of queue q, slot (qlen-1)*NQ + q, must exist
+        if (a.qlen[q] < 0 || (a.qlen[q] > 0 && (long long)(a.qlen[q] - 1) * NQ + q >= a.ntiles))
+            return hipErrorInvalidValue;
+        real += a.qlen[q];
+    }
+    if (!a.cursor) return hipErrorInvalidValue;
+    const int nb = persistent_blocks(kern, std::max(real, 1));
+    if (nb <= 0) return hipErrorInvalidValue;
+    k_zero_ints<<<1, 64, 0, st>>>(a.cursor, NQ);
+    kern<<<dim3(nb), dim3(TPB), 0, st>>>(a);
+    return hipGetLastError();
+}
+
 template <int KER, bool SS>
 hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
 {
-    dim3 grid(a.ntiles), block(TPB);
-    if (a.precision == 1) {
-        if (a.grid_targets) k_interact<KER, SS, true, true><<<grid, block, 0, st>>>(a);
-        else k_interact<KER, SS, false, true><<<grid, block, 0, st>>>(a);
-    } else {
-        if (a.grid_targets) k_interact<KER, SS, true, false><<<grid, block, 0, st>>>(a);
-        else k_interact<KER, SS, false, false><<<grid, block, 0, st>>>(a);
+    if (a.grid_targets) {
+        if (a.precision == 1) return interact_launch<KER, SS, true, true, GRID_TPT>(a, st);
+        return interact_launch<KER, SS, true, false, GRID_TPT>(a, st);
     }
-    return hipGetLastError();
+    if (a.tpt == 1) {
+        if (a.precision == 1) return interact_launch<KER, SS, false, true, 1>(a, st);
+        return interact_launch<KER, SS, false, false, 1>(a, st);
+    }
+    if (a.precision == 1) return interact_launch<KER, SS, false, true, 2>(a, st);
+    return interact_launch<KER, SS, false, false, 2>(a, st);
 }
 
 // ---------------------------------------------------------------------

@@ -22,8 +22,9 @@ double now()
         if (e_ != hipSuccess) return std::string(#expr ": ") + hipGetErrorString(e_); \
     } while (0)
 
-constexpr int TILE_TARGETS = 512;  // TPB*TPT of k_interact
-constexpr int DOWN_TILE = 1024;    // targets per block of the leaf downpass
+constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
+constexpr int NXCD = btk::TILE_QUEUES;  // XCDs of an MI355X, one tile queue each
+constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
 
 }  // namespace
 
@@ -51,20 +52,25 @@ template <class T, class A> std::string Plan::up(T **d, const std::vector<T, A> 
 {
     *d = dalloc<T>(h.size());
     if (!*d) return "hipMalloc failed";
-    if (!h.empty()) HIPCHK(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st_));
+    return up_at(*d, h);
+}
+
+template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A> &h)
+{
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st_));
     return "";
 }
 
-// Flattens the mode-specific lists into two CSR work descriptions:
+// Flattens the lists of every system into two global CSR work descriptions:
 //   near_ : target leaves (particle targets) <- source particle ranges + source clusters
 //   grid_ : target nodes (grid targets)      <- source particle ranges + source clusters
+// with particle indices and node ids offset to the concatenated numbering.
 void Plan::build_work()
 {
     const Params &p = prm_;
     const int P = p.P;
     auto add_key = [](WorkCSR &w, int key, int tbeg, int tcnt) {
         w.key.push_back(key); w.tbeg.push_back(tbeg); w.tcnt.push_back(tcnt);
-        if (w.seg_off.empty()) { w.seg_off.push_back(0); w.clu_off.push_back(0); }
     };
     auto close_key = [](WorkCSR &w) {
         w.seg_off.push_back((int)w.seg_beg.size());
@@ -74,67 +80,72 @@ void Plan::build_work()
         w.seg_beg.push_back(b); w.seg_cnt.push_back(c); w.seg_chk.push_back(check ? 1 : 0);
         w.pairs += nt * c;
     };
-    // May a source of node s coincide with a target of node t?  The
-    // singularity skip can only trigger when their closed boxes overlap (up to
-    // a 1e-150 gap: any pair across a wider gap has r^2 > DBL_MIN).
-    auto touch = [this](int t, int s) {
-        const double g = 1e-150;
-        return ttree_.xmin[t] - stree_.xmax[s] <= g && stree_.xmin[s] - ttree_.xmax[t] <= g &&
-               ttree_.ymin[t] - stree_.ymax[s] <= g && stree_.ymin[s] - ttree_.ymax[t] <= g &&
-               ttree_.zmin[t] - stree_.zmax[s] <= g && stree_.zmin[s] - ttree_.zmax[t] <= g;
-    };
     auto add_clu = [P](WorkCSR &w, int id, long long nt) { w.clu_id.push_back(id); w.pairs += nt * P; };
     near_ = WorkCSR();
     grid_ = WorkCSR();
     near_.seg_off.push_back(0); near_.clu_off.push_back(0);
     grid_.seg_off.push_back(0); grid_.clu_off.push_back(0);
 
-    if (p.compute_type == C_PC) {
-        // keys: target batches; pc list -> clusters, pp list -> source leaves
-        for (int b = 0; b < ttree_.n; b++) {
-            const long long nt = ttree_.numpar[b];
-            add_key(near_, b, ttree_.ibeg[b], ttree_.numpar[b]);
-            for (int c : L_.approx[b]) add_clu(near_, c, nt);
-            for (int s : L_.pp[b]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt, touch(b, s));
-            close_key(near_);
-        }
-    } else if (p.compute_type == C_CP) {
-        // lists keyed by source batch; invert to target-node keys
-        std::vector<std::vector<int>> pp_of(ttree_.n), cp_of(ttree_.n);
-        for (int b = 0; b < stree_.n; b++) {
-            for (int t : L_.pp[b]) pp_of[t].push_back(b);
-            for (int t : L_.approx[b]) cp_of[t].push_back(b);
-        }
-        for (int leaf : ttree_.leaves) {
-            const long long nt = ttree_.numpar[leaf];
-            add_key(near_, leaf, ttree_.ibeg[leaf], ttree_.numpar[leaf]);
-            for (int b : pp_of[leaf]) add_seg(near_, stree_.ibeg[b], stree_.numpar[b], nt, touch(leaf, b));
-            close_key(near_);
-        }
-        for (int t = 0; t < ttree_.n; t++) {
-            if (cp_of[t].empty()) continue;
-            add_key(grid_, t, 0, P);
-            for (int b : cp_of[t]) add_seg(grid_, stree_.ibeg[b], stree_.numpar[b], P);
-            close_key(grid_);
-        }
-    } else {
-        // CC: a target leaf gets the PP and source-approx (PC) lists of
-        // itself and of every ancestor; grids get CC and target-approx (CP)
-        for (int leaf : ttree_.leaves) {
-            const long long nt = ttree_.numpar[leaf];
-            add_key(near_, leaf, ttree_.ibeg[leaf], ttree_.numpar[leaf]);
-            for (int a = leaf; a >= 0; a = ttree_.parent[a]) {
-                for (int c : L_.cc_pc[a]) add_clu(near_, c, nt);
-                for (int s : L_.pp[a]) add_seg(near_, stree_.ibeg[s], stree_.numpar[s], nt, touch(leaf, s));
+    for (const System &sy : sys_) {
+        const Tree &st = sy.stree, &tt = sy.ttree;
+        const Lists &L = sy.L;
+        const int so = sy.soff, to = sy.toff, sn = sy.snode_off, tn = sy.tnode_off;
+        // May a source of node s coincide with a target of node t?  The
+        // singularity skip can only trigger when their closed boxes overlap
+        // (up to a 1e-150 gap: any pair across a wider gap has r^2 > DBL_MIN).
+        auto touch = [&](int t, int s) {
+            const double g = 1e-150;
+            return tt.xmin[t] - st.xmax[s] <= g && st.xmin[s] - tt.xmax[t] <= g &&
+                   tt.ymin[t] - st.ymax[s] <= g && st.ymin[s] - tt.ymax[t] <= g &&
+                   tt.zmin[t] - st.zmax[s] <= g && st.zmin[s] - tt.zmax[t] <= g;
+        };
+        if (p.compute_type == C_PC) {
+            // keys: target batches; pc list -> clusters, pp list -> source leaves
+            for (int b = 0; b < tt.n; b++) {
+                const long long nt = tt.numpar[b];
+                add_key(near_, tn + b, to + tt.ibeg[b], tt.numpar[b]);
+                for (int c : L.approx[b]) add_clu(near_, sn + c, nt);
+                for (int s : L.pp[b]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(b, s));
+                close_key(near_);
             }
-            close_key(near_);
-        }
-        for (int t = 0; t < ttree_.n; t++) {
-            if (L_.approx[t].empty() && L_.cc_cp[t].empty()) continue;
-            add_key(grid_, t, 0, P);
-            for (int c : L_.approx[t]) add_clu(grid_, c, P);
-            for (int s : L_.cc_cp[t]) add_seg(grid_, stree_.ibeg[s], stree_.numpar[s], P);
-            close_key(grid_);
+        } else if (p.compute_type == C_CP) {
+            // lists keyed by source batch; invert to target-node keys
+            std::vector<std::vector<int>> pp_of(tt.n), cp_of(tt.n);
+            for (int b = 0; b < st.n; b++) {
+                for (int t : L.pp[b]) pp_of[t].push_back(b);
+                for (int t : L.approx[b]) cp_of[t].push_back(b);
+            }
+            for (int leaf : tt.leaves) {
+                const long long nt = tt.numpar[leaf];
+                add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
+                for (int b : pp_of[leaf]) add_seg(near_, so + st.ibeg[b], st.numpar[b], nt, touch(leaf, b));
+                close_key(near_);
+            }
+            for (int t = 0; t < tt.n; t++) {
+                if (cp_of[t].empty()) continue;
+                add_key(grid_, tn + t, 0, P);
+                for (int b : cp_of[t]) add_seg(grid_, so + st.ibeg[b], st.numpar[b], P);
+                close_key(grid_);
+            }
+        } else {
+            // CC: a target leaf gets the PP and source-approx (PC) lists of
+            // itself and of every ancestor; grids get CC and target-approx (CP)
+            for (int leaf : tt.leaves) {
+                const long long nt = tt.numpar[leaf];
+                add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
+                for (int a = leaf; a >= 0; a = tt.parent[a]) {
+                    for (int c : L.cc_pc[a]) add_clu(near_, sn + c, nt);
+                    for (int s : L.pp[a]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(leaf, s));
+                }
+                close_key(near_);
+            }
+            for (int t = 0; t < tt.n; t++) {
+                if (L.approx[t].empty() && L.cc_cp[t].empty()) continue;
+                add_key(grid_, tn + t, 0, P);
+                for (int c : L.approx[t]) add_clu(grid_, sn + c, P);
+                for (int s : L.cc_cp[t]) add_seg(grid_, so + st.ibeg[s], st.numpar[s], P);
+                close_key(grid_);
+            }
         }
     }
 
@@ -144,8 +155,8 @@ void Plan::build_work()
     // keys (groups of tiles) are dealt to per-XCD lists, heaviest group to the
     // least loaded XCD, and block 8*j + x takes the j-th tile of XCD x's list.
     // Lists are padded with empty tiles (count 0), which return at once.
-    auto make_tiles = [&](const WorkCSR &w, bool grid, std::vector<btk::Tile> &tiles) {
-        constexpr int NXCD = 8;
+    auto make_tiles = [&](const WorkCSR &w, bool grid, std::vector<btk::Tile> &tiles, int *qlen) {
+        const int tile_t = grid ? GRID_TILE : tile_targets_;
         struct Group { long long work; std::vector<btk::Tile> t; };
         std::vector<Group> groups;
         for (size_t k = 0; k < w.key.size(); k++) {
@@ -153,11 +164,11 @@ void Plan::build_work()
             for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
             src += (long long)(w.clu_off[k + 1] - w.clu_off[k]) * P;
             Group g{0, {}};
-            for (int o = 0; o < w.tcnt[k]; o += TILE_TARGETS) {
+            for (int o = 0; o < w.tcnt[k]; o += tile_t) {
                 btk::Tile t;
                 t.x = (int)k;
                 t.y = w.tbeg[k] + o;
-                t.z = std::min(TILE_TARGETS, w.tcnt[k] - o);
+                t.z = std::min(tile_t, w.tcnt[k] - o);
                 t.w = grid ? w.key[k] : 0;
                 g.work += (src + 1) * t.z;
                 g.t.push_back(t);
@@ -178,43 +189,68 @@ void Plan::build_work()
         tiles.assign(L * NXCD, btk::Tile{0, 0, 0, 0});
         for (int x = 0; x < NXCD; x++)
             for (size_t j = 0; j < lists[x].size(); j++) tiles[j * NXCD + x] = lists[x][j];
+        for (int x = 0; x < NXCD; x++) qlen[x] = (int)lists[x].size();
     };
-    make_tiles(near_, false, near_tiles_);
-    make_tiles(grid_, true, grid_tiles_);
+    make_tiles(near_, false, near_tiles_, near_qlen_);
+    make_tiles(grid_, true, grid_tiles_, grid_qlen_);
 
-    // upward pass (source tree, PC and CC): non-leaf nodes per level, deepest first
+    // upward pass (source trees, PC and CC): non-leaf nodes per level,
+    // deepest first, every system in the same launch
     up_levels_off_.assign(1, 0);
     up_nodes_.clear(); up_child_off_.assign(1, 0); up_child_id_.clear();
+    g_sleaves_.clear();
     if (p.compute_type != C_CP) {
-        for (int lv = stree_.max_depth - 1; lv >= 0; --lv) {
-            for (int par : stree_.levels[lv]) {
-                up_nodes_.push_back(par);
-                for (int k = 0; k < stree_.nchild[par]; k++) up_child_id_.push_back(stree_.child[8 * par + k]);
-                up_child_off_.push_back((int)up_child_id_.size());
+        for (int lv = sdepth_ - 1; lv >= 0; --lv) {
+            for (const System &sy : sys_) {
+                if (lv >= sy.stree.max_depth) continue;
+                for (int par : sy.stree.levels[lv]) {
+                    up_nodes_.push_back(sy.snode_off + par);
+                    for (int k = 0; k < sy.stree.nchild[par]; k++)
+                        up_child_id_.push_back(sy.snode_off + sy.stree.child[8 * par + k]);
+                    up_child_off_.push_back((int)up_child_id_.size());
+                }
             }
             up_levels_off_.push_back((int)up_nodes_.size());
         }
+        for (const System &sy : sys_)
+            for (int leaf : sy.stree.leaves) g_sleaves_.push_back(sy.snode_off + leaf);
     }
-    // downward pass (target tree, CP and CC): children per level, top first
+    // downward pass (target trees, CP and CC): children per level, top first
     down_levels_off_.assign(1, 0);
     down_nodes_.clear();
     leaf_tiles_.clear();
     if (p.compute_type != C_PC) {
-        for (int lv = 0; lv < ttree_.max_depth; ++lv) {
-            for (int par : ttree_.levels[lv])
-                for (int k = 0; k < ttree_.nchild[par]; k++) down_nodes_.push_back(ttree_.child[8 * par + k]);
+        for (int lv = 0; lv < tdepth_; ++lv) {
+            for (const System &sy : sys_) {
+                if (lv >= sy.ttree.max_depth) continue;
+                for (int par : sy.ttree.levels[lv])
+                    for (int k = 0; k < sy.ttree.nchild[par]; k++)
+                        down_nodes_.push_back(sy.tnode_off + sy.ttree.child[8 * par + k]);
+            }
             down_levels_off_.push_back((int)down_nodes_.size());
         }
-        for (int leaf : ttree_.leaves) {
-            for (int o = 0; o < ttree_.numpar[leaf]; o += DOWN_TILE) {
-                btk::Tile t;
-                t.x = leaf;
-                t.y = ttree_.ibeg[leaf] + o;
-                t.z = std::min(DOWN_TILE, ttree_.numpar[leaf] - o);
-                t.w = 0;
-                leaf_tiles_.push_back(t);
+        for (const System &sy : sys_) {
+            for (int leaf : sy.ttree.leaves) {
+                for (int o = 0; o < sy.ttree.numpar[leaf]; o += DOWN_TILE) {
+                    btk::Tile t;
+                    t.x = sy.tnode_off + leaf;
+                    t.y = sy.toff + sy.ttree.ibeg[leaf] + o;
+                    t.z = std::min(DOWN_TILE, sy.ttree.numpar[leaf] - o);
+                    t.w = 0;
+                    leaf_tiles_.push_back(t);
+                }
             }
         }
+    }
+    // global node ranges and parents
+    g_sibeg_.clear(); g_siend_.clear(); g_tparent_.clear();
+    for (const System &sy : sys_) {
+        for (int i = 0; i < sy.stree.n; i++) {
+            g_sibeg_.push_back(sy.soff + sy.stree.ibeg[i]);
+            g_siend_.push_back(sy.soff + sy.stree.iend[i]);
+        }
+        for (int i = 0; i < sy.ttree.n; i++)
+            g_tparent_.push_back(sy.ttree.parent[i] < 0 ? -1 : sy.tnode_off + sy.ttree.parent[i]);
     }
 }
 
@@ -223,8 +259,7 @@ void Plan::build_work()
 std::string Plan::validate() const
 {
     const int P = prm_.P;
-    auto chk_csr = [&](const WorkCSR &w, const std::vector<btk::Tile> &tiles, bool grid,
-                       const Tree &src_tree) -> std::string {
+    auto chk_csr = [&](const WorkCSR &w, const std::vector<btk::Tile> &tiles, bool grid) -> std::string {
         const int nk = (int)w.key.size();
         if ((int)w.seg_off.size() != nk + 1 || (int)w.clu_off.size() != nk + 1) return "csr offsets";
         if (w.seg_off[nk] != (int)w.seg_beg.size() || w.clu_off[nk] != (int)w.clu_id.size()) return "csr ends";
@@ -234,33 +269,46 @@ std::string Plan::validate() const
             if (w.seg_beg[i] < 0 || w.seg_cnt[i] < 0 || (long long)w.seg_beg[i] + w.seg_cnt[i] > N_)
                 return "source range";
         for (int c : w.clu_id)
-            if (c < 0 || c >= src_tree.n) return "cluster id";
+            if (c < 0 || c >= nsnodes_) return "cluster id";
         for (const btk::Tile &t : tiles) {
             if (t.z == 0) continue;  // padding
-            if (t.x < 0 || t.x >= nk || t.z < 0 || t.z > TILE_TARGETS || t.y < 0) return "tile";
+            if (t.x < 0 || t.x >= nk || t.z < 0 || t.z > (grid ? GRID_TILE : tile_targets_) || t.y < 0)
+                return "tile";
             if (grid) {
-                if (t.w < 0 || t.w >= ttree_.n || t.y + t.z > P) return "grid tile";
+                if (t.w < 0 || t.w >= ntnodes_ || t.y + t.z > P) return "grid tile";
             } else if ((long long)t.y + t.z > M_) {
                 return "target tile";
             }
         }
         return "";
     };
-    std::string e = chk_csr(near_, near_tiles_, false, stree_);
+    auto chk_q = [&](const std::vector<btk::Tile> &tiles, const int *qlen) {
+        for (int x = 0; x < NXCD; x++)
+            for (int j = 0; j < qlen[x]; j++) {
+                const size_t t = (size_t)j * NXCD + x;
+                if (t >= tiles.size() || tiles[t].z <= 0) return false;
+            }
+        return true;
+    };
+    if (!chk_q(near_tiles_, near_qlen_) || !chk_q(grid_tiles_, grid_qlen_)) return "tile queues";
+    std::string e = chk_csr(near_, near_tiles_, false);
     if (!e.empty()) return "near " + e;
-    if (!(e = chk_csr(grid_, grid_tiles_, true, stree_)).empty()) return "far " + e;
-    for (int i = 0; i < stree_.n; i++)
-        if (stree_.ibeg[i] < 0 || stree_.iend[i] > N_ || stree_.ibeg[i] >= stree_.iend[i]) return "source node range";
-    for (int i = 0; i < ttree_.n; i++)
-        if (ttree_.ibeg[i] < 0 || ttree_.iend[i] > M_ || ttree_.ibeg[i] >= ttree_.iend[i]) return "target node range";
+    if (!(e = chk_csr(grid_, grid_tiles_, true)).empty()) return "far " + e;
+    for (size_t i = 0; i < g_sibeg_.size(); i++)
+        if (g_sibeg_[i] < 0 || g_siend_[i] > N_ || g_sibeg_[i] >= g_siend_[i]) return "source node range";
+    for (int l : g_sleaves_)
+        if (l < 0 || l >= nsnodes_) return "source leaf";
     for (int c : up_child_id_)
-        if (c < 0 || c >= stree_.n) return "upward child";
+        if (c < 0 || c >= nsnodes_) return "upward child";
+    for (int n : up_nodes_)
+        if (n < 0 || n >= nsnodes_) return "upward parent";
     for (int c : down_nodes_)
-        if (c < 0 || c >= ttree_.n || ttree_.parent[c] < 0) return "downward child";
+        if (c < 0 || c >= ntnodes_ || g_tparent_[c] < 0 || g_tparent_[c] >= ntnodes_) return "downward child";
     for (const btk::Tile &t : leaf_tiles_)
-        if (t.x < 0 || t.x >= ttree_.n || t.y < 0 || (long long)t.y + t.z > M_ || t.z <= 0) return "leaf tile";
+        if (t.x < 0 || t.x >= ntnodes_ || t.y < 0 || (long long)t.y + t.z > M_ || t.z <= 0) return "leaf tile";
+    if ((int)g_torder_.size() != M_) return "target order size";
     for (int i = 0; i < M_; i++)
-        if (T_.order[i] < 0 || T_.order[i] >= M_) return "target order";
+        if (g_torder_[i] < 0 || g_torder_[i] >= M_) return "target order";
     return "";
 }
 
@@ -268,91 +316,172 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
                          const double *tz, const double *tq, const double *sx, const double *sy,
                          const double *sz, const double *sq, const double *sw, bool host_only)
 {
+    const int toffs[2] = {0, M}, soffs[2] = {0, N};
+    PlanInput in;
+    in.nsys = 1; in.toffs = toffs; in.soffs = soffs; in.tpt = 2;
+    in.tx = tx; in.ty = ty; in.tz = tz; in.tq = tq;
+    in.sx = sx; in.sy = sy; in.sz = sz; in.sq = sq; in.sw = sw;
+    return create(prm, in, host_only);
+}
+
+std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
+{
     free_all();
     prm_ = prm;
-    M_ = M; N_ = N;
     std::string e = params_check(prm);
     if (!e.empty()) return e;
-    if (M <= 0 || N <= 0) return "numTargets and numSources must be positive";
-    if (!tx || !ty || !tz || !sx || !sy || !sz || !sq) return "One or more required particle arrays is unset";
-    if (prm.singularity == S_SUBTRACTION && !sw && !host_only)
+    if (in.nsys < 1 || !in.toffs || !in.soffs) return "need at least one system";
+    if (in.tpt != 1 && in.tpt != 2) return "targets per thread must be 1 or 2";
+    tpt_ = in.tpt;
+    tile_targets_ = 256 * tpt_;
+    persistent_ = in.persistent != 0;
+    for (int s = 0; s < in.nsys; s++)
+        if (in.toffs[s + 1] - in.toffs[s] <= 0 || in.soffs[s + 1] - in.soffs[s] <= 0)
+            return "numTargets and numSources must be positive (every system)";
+    if (in.toffs[0] != 0 || in.soffs[0] != 0) return "system offsets must start at 0";
+    M_ = in.toffs[in.nsys];
+    N_ = in.soffs[in.nsys];
+    if (!in.tx || !in.ty || !in.tz || !in.sx || !in.sy || !in.sz || !in.sq)
+        return "One or more required particle arrays is unset";
+    if (prm.singularity == S_SUBTRACTION && !in.sw && !host_only)
         return "Singularity subtraction requires weights for the sources";
+    for (int s = 0; s < in.nsys; s++) {
+        if (in.msl && in.msl[s] < 1) return "leaf sizes must be positive";
+        if (in.mtl && in.mtl[s] < 1) return "leaf sizes must be positive";
+    }
     if (!host_only) HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
 
     double t0 = now();
-    // private copies (the caller's arrays are never permuted), in parallel
-    S_ = Particles(); T_ = Particles();
-    S_.n = N; T_.n = M;
-    for (auto *v : {&S_.x, &S_.y, &S_.z, &S_.q, &S_.w}) v->resize(N);
-    for (auto *v : {&T_.x, &T_.y, &T_.z, &T_.q}) v->resize(M);
-    S_.order.resize(N);
-    T_.order.resize(M);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < N; i++) {
-        S_.x[i] = sx[i]; S_.y[i] = sy[i]; S_.z[i] = sz[i]; S_.q[i] = sq[i];
-        S_.w[i] = sw ? sw[i] : 1.0;
-        S_.order[i] = i;
+    tm_ = Timings();
+    sys_.assign(in.nsys, System());
+    for (int s = 0; s < in.nsys; s++) {
+        System &y = sys_[s];
+        y.toff = in.toffs[s]; y.soff = in.soffs[s];
+        y.M = in.toffs[s + 1] - y.toff; y.N = in.soffs[s + 1] - y.soff;
+        y.msl = in.msl ? in.msl[s] : prm.max_source_leaf;
+        y.mtl = in.mtl ? in.mtl[s] : prm.max_target_leaf;
+        y.S.n = y.N; y.T.n = y.M;
+        for (auto *v : {&y.S.x, &y.S.y, &y.S.z, &y.S.q, &y.S.w}) v->resize(y.N);
+        for (auto *v : {&y.T.x, &y.T.y, &y.T.z, &y.T.q}) v->resize(y.M);
+        y.S.order.resize(y.N);
+        y.T.order.resize(y.M);
     }
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < M; i++) {
-        T_.x[i] = tx[i]; T_.y[i] = ty[i]; T_.z[i] = tz[i]; T_.q[i] = tq ? tq[i] : 0.0;
-        T_.order[i] = i;
+    // private copies (the caller's arrays are never permuted), in parallel:
+    // over systems when there are several, else over particles
+    const int ns = (int)sys_.size();
+#pragma omp parallel for schedule(dynamic) if (ns > 1)
+    for (int s = 0; s < ns; s++) {
+        System &y = sys_[s];
+        const double *sx = in.sx + y.soff, *sy = in.sy + y.soff, *sz = in.sz + y.soff;
+        const double *sq = in.sq + y.soff, *sw = in.sw ? in.sw + y.soff : nullptr;
+        const double *tx = in.tx + y.toff, *ty = in.ty + y.toff, *tz = in.tz + y.toff;
+        const double *tq = in.tq ? in.tq + y.toff : nullptr;
+        const int n = y.N, m = y.M;
+#pragma omp parallel for schedule(static) if (n > 65536)
+        for (int i = 0; i < n; i++) {
+            y.S.x[i] = sx[i]; y.S.y[i] = sy[i]; y.S.z[i] = sz[i]; y.S.q[i] = sq[i];
+            y.S.w[i] = sw ? sw[i] : 1.0;
+            y.S.order[i] = i;
+        }
+#pragma omp parallel for schedule(static) if (m > 65536)
+        for (int i = 0; i < m; i++) {
+            y.T.x[i] = tx[i]; y.T.y[i] = ty[i]; y.T.z[i] = tz[i]; y.T.q[i] = tq ? tq[i] : 0.0;
+            y.T.order[i] = i;
+        }
     }
     double t1 = now();
-    tm_ = Timings();
     tm_.host_copy = t1 - t0;
 
-    // trees (treedriver.c:128/146, 364/375, 654/665)
-    if (prm.compute_type == C_PC) {
-        build_trees({{&stree_, &S_, prm.max_source_leaf, false}, {&ttree_, &T_, prm.max_target_leaf, true}});
-    } else if (prm.compute_type == C_CP) {
-        build_trees({{&ttree_, &T_, prm.max_target_leaf, false}, {&stree_, &S_, prm.max_source_leaf, true}});
-    } else {
-        build_trees({{&stree_, &S_, prm.max_source_leaf, false}, {&ttree_, &T_, prm.max_target_leaf, false}});
+    // trees (treedriver.c:128/146, 364/375, 654/665), all systems at once
+    std::vector<TreeJob> jobs;
+    for (System &y : sys_) {
+        if (prm.compute_type == C_PC) {
+            jobs.push_back({&y.stree, &y.S, y.msl, false});
+            jobs.push_back({&y.ttree, &y.T, y.mtl, true});
+        } else if (prm.compute_type == C_CP) {
+            jobs.push_back({&y.ttree, &y.T, y.mtl, false});
+            jobs.push_back({&y.stree, &y.S, y.msl, true});
+        } else {
+            jobs.push_back({&y.stree, &y.S, y.msl, false});
+            jobs.push_back({&y.ttree, &y.T, y.mtl, false});
+        }
     }
+    build_trees(jobs);
     double t2 = now();
     tm_.host_tree = t2 - t1;
 
     // interaction lists (treedriver.c:181, 462, 753)
-    if (prm.compute_type == C_PC) make_pc_lists(L_, stree_, ttree_, prm);
-    else if (prm.compute_type == C_CP) make_pc_lists(L_, ttree_, stree_, prm);
-    else make_cc_lists(L_, stree_, ttree_, prm);
+#pragma omp parallel for schedule(dynamic) if (ns > 1)
+    for (int s = 0; s < ns; s++) {
+        System &y = sys_[s];
+        if (prm.compute_type == C_PC) make_pc_lists(y.L, y.stree, y.ttree, prm);
+        else if (prm.compute_type == C_CP) make_pc_lists(y.L, y.ttree, y.stree, prm);
+        else make_cc_lists(y.L, y.stree, y.ttree, prm);
+    }
     double t3 = now();
     tm_.host_lists = t3 - t2;
 
-    // reference diagnostics (treedriver.c:191-200, 472-483, 763-792)
+    // global node numbering
+    nsnodes_ = ntnodes_ = sdepth_ = tdepth_ = 0;
+    for (System &y : sys_) {
+        y.snode_off = nsnodes_; y.tnode_off = ntnodes_;
+        nsnodes_ += y.stree.n; ntnodes_ += y.ttree.n;
+        sdepth_ = std::max(sdepth_, y.stree.max_depth);
+        tdepth_ = std::max(tdepth_, y.ttree.max_depth);
+    }
+
+    // reference diagnostics (treedriver.c:191-200, 472-483, 763-792), summed
     counts_ = Counts();
-    {
-        const long long P = prm.P;
-        const Tree &key = prm.compute_type == C_CP ? stree_ : ttree_;
-        const Tree &ent = prm.compute_type == C_CP ? ttree_ : stree_;
+    const long long P = prm.P;
+    for (const System &y : sys_) {
+        const Tree &key = prm.compute_type == C_CP ? y.stree : y.ttree;
+        const Tree &ent = prm.compute_type == C_CP ? y.ttree : y.stree;
+        const Lists &L = y.L;
         for (int i = 0; i < key.n; i++) {
             const long long nt = key.numpar[i];
-            counts_.num_pp += (long long)L_.pp[i].size();
-            for (int s : L_.pp[i]) counts_.pp_ptwise += nt * ent.numpar[s];
+            counts_.num_pp += (long long)L.pp[i].size();
+            for (int s : L.pp[i]) counts_.pp_ptwise += nt * ent.numpar[s];
             if (prm.compute_type == C_PC) {
-                counts_.num_pc += (long long)L_.approx[i].size();
-                for (int s : L_.approx[i]) { counts_.pc_ptwise += nt * P; counts_.pc_replaced += nt * ent.numpar[s]; }
+                counts_.num_pc += (long long)L.approx[i].size();
+                for (int s : L.approx[i]) { counts_.pc_ptwise += nt * P; counts_.pc_replaced += nt * ent.numpar[s]; }
             } else if (prm.compute_type == C_CP) {
-                counts_.num_cp += (long long)L_.approx[i].size();
-                for (int s : L_.approx[i]) { counts_.cp_ptwise += nt * P; counts_.cp_replaced += nt * ent.numpar[s]; }
+                counts_.num_cp += (long long)L.approx[i].size();
+                for (int s : L.approx[i]) { counts_.cp_ptwise += nt * P; counts_.cp_replaced += nt * ent.numpar[s]; }
             } else {
-                counts_.num_cc += (long long)L_.approx[i].size();
-                counts_.num_pc += (long long)L_.cc_pc[i].size();
-                counts_.num_cp += (long long)L_.cc_cp[i].size();
-                for (int s : L_.approx[i]) { counts_.cc_ptwise += P * P; counts_.cc_replaced += nt * ent.numpar[s]; }
-                for (int s : L_.cc_pc[i]) { counts_.pc_ptwise += nt * P; counts_.pc_replaced += nt * ent.numpar[s]; }
-                for (int s : L_.cc_cp[i]) { counts_.cp_ptwise += P * ent.numpar[s]; counts_.cp_replaced += nt * ent.numpar[s]; }
+                counts_.num_cc += (long long)L.approx[i].size();
+                counts_.num_pc += (long long)L.cc_pc[i].size();
+                counts_.num_cp += (long long)L.cc_cp[i].size();
+                for (int s : L.approx[i]) { counts_.cc_ptwise += P * P; counts_.cc_replaced += nt * ent.numpar[s]; }
+                for (int s : L.cc_pc[i]) { counts_.pc_ptwise += nt * P; counts_.pc_replaced += nt * ent.numpar[s]; }
+                for (int s : L.cc_cp[i]) { counts_.cp_ptwise += P * ent.numpar[s]; counts_.cp_replaced += nt * ent.numpar[s]; }
             }
         }
     }
 
     build_work();
-    std::vector<double> stab, ttab;
     const bool need_src_clusters = prm.compute_type != C_CP;
     const bool need_tgt_clusters = prm.compute_type != C_PC;
-    if (need_src_clusters) node_table(stree_, prm.degree, stab);
-    if (need_tgt_clusters) node_table(ttree_, prm.degree, ttab);
+    std::vector<double> stab, ttab, tab;
+    for (const System &y : sys_) {
+        if (need_src_clusters) {
+            node_table(y.stree, prm.degree, tab);
+            stab.insert(stab.end(), tab.begin(), tab.end());
+        }
+        if (need_tgt_clusters) {
+            node_table(y.ttree, prm.degree, tab);
+            ttab.insert(ttab.end(), tab.begin(), tab.end());
+        }
+    }
+    g_torder_.resize(M_);
+#pragma omp parallel for schedule(dynamic) if (ns > 1)
+    for (int s = 0; s < ns; s++) {
+        const System &y = sys_[s];
+        int *o = g_torder_.data() + y.toff;
+        const int *lo = y.T.order.data();
+        const int off = y.toff, m = y.M;
+#pragma omp parallel for schedule(static) if (m > 65536)
+        for (int i = 0; i < m; i++) o[i] = off + lo[i];
+    }
     double t4 = now();
     tm_.host_work = t4 - t3;
     if (!(e = validate()).empty()) return "internal work-list check failed: " + e;
@@ -378,41 +507,50 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
     }
     if (prm.singularity != S_SUBTRACTION) kp_.ss_param = 0.0;
 
-    // upload
+    // upload: particles straight from each system's sorted copies
     const bool ss = prm.singularity == S_SUBTRACTION;
     std::string r;
-    if (!(r = up(&d_tx_, T_.x)).empty()) return r;
-    if (!(r = up(&d_ty_, T_.y)).empty()) return r;
-    if (!(r = up(&d_tz_, T_.z)).empty()) return r;
-    if (!(r = up(&d_tq_, T_.q)).empty()) return r;
-    if (!(r = up(&d_sx_, S_.x)).empty()) return r;
-    if (!(r = up(&d_sy_, S_.y)).empty()) return r;
-    if (!(r = up(&d_sz_, S_.z)).empty()) return r;
-    if (!(r = up(&d_sq_, S_.q)).empty()) return r;
-    if (ss && !(r = up(&d_sw_, S_.w)).empty()) return r;
-    if (!(r = up(&d_torder_, T_.order)).empty()) return r;
-    const size_t P = prm.P;
+    d_tx_ = dalloc<double>(M_); d_ty_ = dalloc<double>(M_); d_tz_ = dalloc<double>(M_);
+    d_tq_ = dalloc<double>(M_);
+    d_sx_ = dalloc<double>(N_); d_sy_ = dalloc<double>(N_); d_sz_ = dalloc<double>(N_);
+    d_sq_ = dalloc<double>(N_);
+    if (ss) d_sw_ = dalloc<double>(N_);
+    if (!d_tx_ || !d_ty_ || !d_tz_ || !d_tq_ || !d_sx_ || !d_sy_ || !d_sz_ || !d_sq_ || (ss && !d_sw_))
+        return "hipMalloc failed (particles)";
+    for (const System &y : sys_) {
+        if (!(r = up_at(d_tx_ + y.toff, y.T.x)).empty()) return r;
+        if (!(r = up_at(d_ty_ + y.toff, y.T.y)).empty()) return r;
+        if (!(r = up_at(d_tz_ + y.toff, y.T.z)).empty()) return r;
+        if (!(r = up_at(d_tq_ + y.toff, y.T.q)).empty()) return r;
+        if (!(r = up_at(d_sx_ + y.soff, y.S.x)).empty()) return r;
+        if (!(r = up_at(d_sy_ + y.soff, y.S.y)).empty()) return r;
+        if (!(r = up_at(d_sz_ + y.soff, y.S.z)).empty()) return r;
+        if (!(r = up_at(d_sq_ + y.soff, y.S.q)).empty()) return r;
+        if (ss && !(r = up_at(d_sw_ + y.soff, y.S.w)).empty()) return r;
+    }
+    if (!(r = up(&d_torder_, g_torder_)).empty()) return r;
+    const size_t Pz = prm.P;
     if (need_src_clusters) {
         if (!(r = up(&d_stab_, stab)).empty()) return r;
-        const size_t np = (size_t)stree_.n * P;
+        const size_t np = (size_t)nsnodes_ * Pz;
         d_cx_ = dalloc<double>(np); d_cy_ = dalloc<double>(np); d_cz_ = dalloc<double>(np);
         d_cq_ = dalloc<double>(np);
         if (ss) d_cw_ = dalloc<double>(np);
         if (!d_cx_ || !d_cy_ || !d_cz_ || !d_cq_ || (ss && !d_cw_)) return "hipMalloc failed (source clusters)";
-        if (!(r = up(&d_sibeg_, stree_.ibeg)).empty()) return r;
-        if (!(r = up(&d_siend_, stree_.iend)).empty()) return r;
-        if (!(r = up(&d_sleaves_, stree_.leaves)).empty()) return r;
+        if (!(r = up(&d_sibeg_, g_sibeg_)).empty()) return r;
+        if (!(r = up(&d_siend_, g_siend_)).empty()) return r;
+        if (!(r = up(&d_sleaves_, g_sleaves_)).empty()) return r;
         if (!(r = up(&d_up_nodes_, up_nodes_)).empty()) return r;
         if (!(r = up(&d_up_child_off_, up_child_off_)).empty()) return r;
         if (!(r = up(&d_up_child_id_, up_child_id_)).empty()) return r;
     }
     if (need_tgt_clusters) {
         if (!(r = up(&d_ttab_, ttab)).empty()) return r;
-        const size_t np = (size_t)ttree_.n * P;
+        const size_t np = (size_t)ntnodes_ * Pz;
         d_Qt_ = dalloc<double>(np);
         if (ss) d_Wt_ = dalloc<double>(np);
         if (!d_Qt_ || (ss && !d_Wt_)) return "hipMalloc failed (target clusters)";
-        if (!(r = up(&d_tparent_, ttree_.parent)).empty()) return r;
+        if (!(r = up(&d_tparent_, g_tparent_)).empty()) return r;
         if (!(r = up(&d_down_nodes_, down_nodes_)).empty()) return r;
         if (!(r = up(&d_leaf_tiles_, leaf_tiles_)).empty()) return r;
     }
@@ -434,9 +572,10 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
         if (!(r = up_csr(grid_, &d_gseg_off_, &d_gsegs_, &d_gclu_off_, &d_gclu_id_)).empty()) return r;
         if (!(r = up(&d_grid_tiles_, grid_tiles_)).empty()) return r;
     }
-    d_pot_ = dalloc<double>(M);
-    d_out_ = dalloc<double>(M);
-    if (!d_pot_ || !d_out_) return "hipMalloc failed (potential)";
+    d_pot_ = dalloc<double>(M_);
+    d_out_ = dalloc<double>(M_);
+    d_cursor_ = dalloc<int>(2 * NXCD);
+    if (!d_pot_ || !d_out_ || !d_cursor_) return "hipMalloc failed (potential)";
     HIPCHK(hipStreamSynchronize(st_));
     tm_.h2d = now() - t4;
     return "";
@@ -451,15 +590,15 @@ std::string Plan::execute(bool timed)
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(ev[0], st_));
 
-    // 1. upward pass over the source tree (clusters.c:50-200)
+    // 1. upward pass over the source trees (clusters.c:50-200)
     if (p.compute_type != C_CP) {
-        HIPCHK(btk::launch_cluster_points(stree_.n, nl, d_stab_, d_cx_, d_cy_, d_cz_, st_));
+        HIPCHK(btk::launch_cluster_points(nsnodes_, nl, d_stab_, d_cx_, d_cy_, d_cz_, st_));
         btk::AnterpArgs a{};
         a.degree = p.degree; a.subtraction = ss;
         a.ibeg = d_sibeg_; a.iend = d_siend_; a.tab = d_stab_;
         a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
         a.Q = d_cq_; a.W = d_cw_;
-        a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)stree_.leaves.size();
+        a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)g_sleaves_.size();
         HIPCHK(btk::launch_anterp(a, st_));
         for (size_t lv = 0; lv + 1 < up_levels_off_.size(); lv++) {
             const int b = up_levels_off_[lv], e = up_levels_off_[lv + 1];
@@ -471,9 +610,15 @@ std::string Plan::execute(bool timed)
     }
     HIPCHK(hipEventRecord(ev[1], st_));
 
-    // 2. near field and particle-cluster far field into the potential
+    // 2. near field and particle-cluster far field into the potential.  The
+    // potential starts as NaN (all bytes 0xFF): the near-field tiles write
+    // every target once, so a target that no tile covered would show NaN,
+    // never a value left from an earlier execution.
+    HIPCHK(hipMemsetAsync(d_pot_, 0xFF, (size_t)M_ * sizeof(double), st_));
     btk::InteractArgs ia{};
     ia.kernel = p.kernel; ia.singularity = p.singularity; ia.precision = p.precision;
+    ia.tpt = tpt_;
+    ia.persistent = persistent_ ? 1 : 0;
     ia.P = p.P; ia.nl = nl; ia.kp = kp_;
     ia.tx = d_tx_; ia.ty = d_ty_; ia.tz = d_tz_; ia.tq = d_tq_;
     ia.sx = d_sx_; ia.sy = d_sy_; ia.sz = d_sz_; ia.sq = d_sq_; ia.sw = d_sw_;
@@ -483,13 +628,15 @@ std::string Plan::execute(bool timed)
         a.grid_targets = 0; a.ntiles = (int)near_tiles_.size(); a.tiles = d_near_tiles_;
         a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_nclu_off_; a.clu_id = d_nclu_id_;
         a.out1 = d_pot_;
+        std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
+        a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
     }
     HIPCHK(hipEventRecord(ev[2], st_));
 
     // 3. far field into target-cluster grids (CP, CC)
     if (p.compute_type != C_PC) {
-        const size_t np = (size_t)ttree_.n * p.P;
+        const size_t np = (size_t)ntnodes_ * p.P;
         HIPCHK(hipMemsetAsync(d_Qt_, 0, np * sizeof(double), st_));
         if (ss) HIPCHK(hipMemsetAsync(d_Wt_, 0, np * sizeof(double), st_));
         if (!grid_tiles_.empty()) {
@@ -498,12 +645,14 @@ std::string Plan::execute(bool timed)
             a.seg_off = d_gseg_off_; a.segs = d_gsegs_; a.clu_off = d_gclu_off_; a.clu_id = d_gclu_id_;
             a.tnode_tab = d_ttab_;
             a.out1 = d_Qt_; a.out2 = d_Wt_;
+            std::copy(grid_qlen_, grid_qlen_ + NXCD, a.qlen);
+            a.cursor = d_cursor_ + NXCD;
             HIPCHK(btk::launch_interact(a, st_));
         }
     }
     HIPCHK(hipEventRecord(ev[3], st_));
 
-    // 4. downpass over the target tree (interaction_compute_downpass.c:37-142)
+    // 4. downpass over the target trees (interaction_compute_downpass.c:37-142)
     if (p.compute_type != C_PC) {
         btk::DownArgs d{};
         d.degree = p.degree; d.subtraction = ss; d.tab = d_ttab_; d.parent = d_tparent_;
@@ -543,40 +692,51 @@ std::string Plan::download(double *out)
 
 std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
 {
-    // targets in caller order: gather from the sorted copy through order
     const int m = (int)idx.size();
     for (int k = 0; k < m; k++)
         if (idx[k] < 0 || idx[k] >= M_) return "target index out of range";
-    if (m == 0) { out.clear(); return ""; }
+    out.assign(m, 0.0);
+    if (m == 0) return "";
+    // position of each caller target in the sorted arrays
     std::vector<int> inv(M_);
-    for (int i = 0; i < M_; i++) inv[T_.order[i]] = i;
-    std::vector<double> hx(m), hy(m), hz(m), hq(m);
-    for (int k = 0; k < m; k++) {
-        const int i = inv[idx[k]];
-        hx[k] = T_.x[i]; hy[k] = T_.y[i]; hz[k] = T_.z[i]; hq[k] = T_.q[i];
-    }
-    double *dx = nullptr, *dy = nullptr, *dz = nullptr, *dq = nullptr, *dout = nullptr, *dw = d_sw_;
+    for (int i = 0; i < M_; i++) inv[g_torder_[i]] = i;
+    double *dw = d_sw_;
     std::string r;
-    if (!(r = up(&dx, hx)).empty()) return r;
-    if (!(r = up(&dy, hy)).empty()) return r;
-    if (!(r = up(&dz, hz)).empty()) return r;
-    if (!(r = up(&dq, hq)).empty()) return r;
-    dout = dalloc<double>(m);
-    if (!dout) return "hipMalloc failed";
     if (!dw) {
         std::vector<double> ones(N_, 1.0);
         if (!(r = up(&dw, ones)).empty()) return r;
     }
-    // enough source chunks for about 2048 blocks, each chunk at least 4096 sources
-    const int tblocks = (m + 255) / 256;
-    int nchunks = std::max(1, std::min((2048 + tblocks - 1) / tblocks, (N_ + 4095) / 4096));
-    double *dpart = dalloc<double>((size_t)nchunks * m);
-    if (!dpart) return "hipMalloc failed";
-    HIPCHK(btk::launch_direct(prm_.kernel, prm_.singularity, kp_, m, dx, dy, dz, dq, N_, d_sx_, d_sy_,
-                              d_sz_, d_sq_, dw, dpart, nchunks, dout, st_));
-    out.resize(m);
-    HIPCHK(hipMemcpyAsync(out.data(), dout, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st_));
-    HIPCHK(hipStreamSynchronize(st_));
+    // each target against the sources of its own system
+    for (const System &y : sys_) {
+        std::vector<int> which;
+        for (int k = 0; k < m; k++)
+            if (idx[k] >= y.toff && idx[k] < y.toff + y.M) which.push_back(k);
+        if (which.empty()) continue;
+        const int mm = (int)which.size();
+        std::vector<double> hx(mm), hy(mm), hz(mm), hq(mm);
+        for (int k = 0; k < mm; k++) {
+            const int i = inv[idx[which[k]]] - y.toff;  // local sorted index
+            hx[k] = y.T.x[i]; hy[k] = y.T.y[i]; hz[k] = y.T.z[i]; hq[k] = y.T.q[i];
+        }
+        double *dx = nullptr, *dy = nullptr, *dz = nullptr, *dq = nullptr;
+        if (!(r = up(&dx, hx)).empty()) return r;
+        if (!(r = up(&dy, hy)).empty()) return r;
+        if (!(r = up(&dz, hz)).empty()) return r;
+        if (!(r = up(&dq, hq)).empty()) return r;
+        double *dout = dalloc<double>(mm);
+        // enough source chunks for about 2048 blocks, each chunk at least 4096 sources
+        const int tblocks = (mm + 255) / 256;
+        const int nchunks = std::max(1, std::min((2048 + tblocks - 1) / tblocks, (y.N + 4095) / 4096));
+        double *dpart = dalloc<double>((size_t)nchunks * mm);
+        if (!dout || !dpart) return "hipMalloc failed";
+        HIPCHK(btk::launch_direct(prm_.kernel, prm_.singularity, kp_, mm, dx, dy, dz, dq, y.N,
+                                  d_sx_ + y.soff, d_sy_ + y.soff, d_sz_ + y.soff, d_sq_ + y.soff,
+                                  dw + y.soff, dpart, nchunks, dout, st_));
+        std::vector<double> res(mm);
+        HIPCHK(hipMemcpyAsync(res.data(), dout, (size_t)mm * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        for (int k = 0; k < mm; k++) out[which[k]] = res[k];
+    }
     return "";
 }
 

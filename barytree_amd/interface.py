@@ -5,7 +5,8 @@ Mirrors the reference's binding, interfaces/python/BaryTreeInterface.py:14-152
 callTreedriver with the same arguments and defaults), so scripts written for
 the reference run unchanged; GPUpresent is accepted and ignored because every
 call runs on the GPU.  The extensions (precision choice, plans, direct sums,
-statistics) are exposed as solve(), Plan and analyze().
+statistics, batches of independent systems, the autotuner) are exposed as
+solve(), Plan, analyze(), solve_batched(), Plan.batched() and autotune().
 
 The library is loaded from this package directory.  Nothing here falls back
 to a CPU path: a missing library raises at load time.
@@ -72,10 +73,36 @@ class Stats(ctypes.Structure):
         (n, ctypes.c_double) for n in (
             "host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s",
             "gpu_upward_s", "gpu_near_s", "gpu_far_s", "gpu_down_s", "gpu_finish_s",
-            "gpu_total_s")]
+            "gpu_total_s")] + [
+        (n, ctypes.c_int) for n in ("num_systems", "targets_per_thread", "persistent")]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+SIZE_CLASSES = 32
+
+
+class Schedule(ctypes.Structure):
+    """BaryTreeSchedule: leaf size per size class (2^c <= max(M, N) < 2^(c+1)),
+    tile shape and launch form of a batched solve."""
+    _fields_ = [("targets_per_thread", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("leaf", ctypes.c_int * SIZE_CLASSES), ("trials", ctypes.c_int),
+                ("tune_seconds", ctypes.c_double)]
+
+    @classmethod
+    def make(cls, leaf_by_class, targets_per_thread=2, persistent=1):
+        s = cls()
+        s.targets_per_thread = targets_per_thread
+        s.persistent = persistent
+        for c, v in dict(leaf_by_class).items():
+            s.leaf[int(c)] = int(v)
+        return s
+
+    def as_dict(self):
+        return {"targets_per_thread": self.targets_per_thread, "persistent": self.persistent,
+                "leaf": {c: self.leaf[c] for c in range(SIZE_CLASSES) if self.leaf[c] > 0},
+                "trials": self.trials, "tune_seconds": self.tune_seconds}
 
 
 _lib = None
@@ -94,7 +121,9 @@ def lib():
         l.BaryTree_GetLastError.restype = ctypes.c_char_p
         for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
-                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision"):
+                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision",
+                  "BaryTree_SolveBatched", "BaryTreePlan_CreateBatched", "BaryTree_Autotune",
+                  "BaryTree_AnalyzeBatched"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
@@ -170,8 +199,128 @@ def analyze(targets, sources, kernel, kernel_params, singularity, approximation,
     return s.as_dict()
 
 
+def _batch(systems):
+    """Concatenates [(targets, sources), ...] into offset arrays and particle
+    arrays (targets = (x, y, z, q), sources = (x, y, z, q, w))."""
+    toffs, soffs = [0], [0]
+    cols = [[] for _ in range(9)]
+    for tgt, src in systems:
+        tx, ty, tz, tq = tgt
+        sx, sy, sz, sq, sw = src
+        m, n = len(tx), len(sx)
+        toffs.append(toffs[-1] + m)
+        soffs.append(soffs[-1] + n)
+        for k, a in enumerate((tx, ty, tz, tq if tq is not None else np.zeros(m),
+                               sx, sy, sz, sq, sw if sw is not None else np.ones(n))):
+            cols[k].append(np.asarray(a, dtype=np.float64))
+    arrays = [np.ascontiguousarray(np.concatenate(c)) if c else np.zeros(0) for c in cols]
+    return (np.asarray(toffs, dtype=np.int32), np.asarray(soffs, dtype=np.int32), arrays)
+
+
+def _ints(a, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.int32), (n,)))
+    return a
+
+
+def _iptr(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def solve_batched(systems, kernel, kernel_params, singularity, approximation, compute_type,
+                  theta, degree, max_source_leaf=None, max_target_leaf=None, schedule=None,
+                  size_check=1.0, beta=-1.0, precision=PRECISION_FP64, verbosity=0):
+    """BaryTree_SolveBatched over systems = [(targets, sources), ...].  Leaf
+    sizes: per-system arrays (or one int for all), else the schedule, else the
+    autotuner.  Returns one potential array per system."""
+    toffs, soffs, arr = _batch(systems)
+    ns = len(systems)
+    kp = _kp(kernel_params)
+    msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
+    out = np.zeros(int(toffs[-1]))
+    rc = lib().BaryTree_SolveBatched(
+        ctypes.c_int(ns), _iptr(toffs), _iptr(soffs), *[_ptr(a) for a in arr], _ptr(out),
+        ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+        ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+        ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+        _iptr(msl), _iptr(mtl), None if schedule is None else ctypes.byref(schedule),
+        ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(verbosity),
+        ctypes.c_int(precision))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_SolveBatched failed: {last_error()}")
+    return [out[toffs[i]:toffs[i + 1]] for i in range(ns)]
+
+
+def analyze_batched(systems, kernel, kernel_params, singularity, approximation, compute_type,
+                    theta, degree, max_source_leaf=None, max_target_leaf=None, schedule=None,
+                    size_check=1.0, beta=-1.0):
+    """BaryTree_AnalyzeBatched: host-only trees and lists, counts summed over
+    the systems (no GPU work)."""
+    toffs, soffs, arr = _batch(systems)
+    ns = len(systems)
+    kp = _kp(kernel_params)
+    msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
+    s = Stats()
+    tx, ty, tz, _, sx, sy, sz, _, _ = arr
+    rc = lib().BaryTree_AnalyzeBatched(
+        ctypes.c_int(ns), _iptr(toffs), _iptr(soffs), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(sx),
+        _ptr(sy), _ptr(sz), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+        ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+        ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+        _iptr(msl), _iptr(mtl), None if schedule is None else ctypes.byref(schedule),
+        ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.byref(s))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_AnalyzeBatched failed: {last_error()}")
+    return s.as_dict()
+
+
+def autotune(systems, kernel, kernel_params, singularity, approximation, compute_type, theta,
+             degree, size_check=1.0, beta=-1.0, precision=PRECISION_FP64):
+    """BaryTree_Autotune: returns the Schedule picked for this batch."""
+    toffs, soffs, arr = _batch(systems)
+    kp = _kp(kernel_params)
+    sc = Schedule()
+    rc = lib().BaryTree_Autotune(
+        ctypes.c_int(len(systems)), _iptr(toffs), _iptr(soffs), *[_ptr(a) for a in arr],
+        ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+        ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+        ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+        ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(precision),
+        ctypes.byref(sc))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_Autotune failed: {last_error()}")
+    return sc
+
+
 class Plan:
     """Host setup once, GPU execution many times (BaryTreePlan_*)."""
+
+    @classmethod
+    def batched(cls, systems, kernel, kernel_params, singularity, approximation, compute_type,
+                theta, degree, max_source_leaf=None, max_target_leaf=None, schedule=None,
+                size_check=1.0, beta=-1.0, precision=PRECISION_FP64):
+        """BaryTreePlan_CreateBatched; execute() returns the concatenated
+        potential (split with .offsets)."""
+        toffs, soffs, arr = _batch(systems)
+        ns = len(systems)
+        kp = _kp(kernel_params)
+        msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
+        self = cls.__new__(cls)
+        self.n_targets = int(toffs[-1])
+        self.offsets = toffs
+        self._h = ctypes.c_void_p()
+        rc = lib().BaryTreePlan_CreateBatched(
+            ctypes.byref(self._h), ctypes.c_int(ns), _iptr(toffs), _iptr(soffs),
+            *[_ptr(a) for a in arr], ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+            ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+            ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+            _iptr(msl), _iptr(mtl), None if schedule is None else ctypes.byref(schedule),
+            ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(precision))
+        if rc != 0:
+            self._h = None
+            raise RuntimeError(f"BaryTreePlan_CreateBatched failed: {last_error()}")
+        return self
 
     def __init__(self, targets, sources, kernel, kernel_params, singularity, approximation,
                  compute_type, theta, degree, max_source_leaf, max_target_leaf, size_check=1.0,
@@ -180,6 +329,7 @@ class Plan:
         sx, sy, sz, sq, sw = map(_arr, sources)
         kp = _kp(kernel_params)
         self.n_targets = len(tx)
+        self.offsets = np.asarray([0, len(tx)], dtype=np.int32)
         self._h = ctypes.c_void_p()
         rc = lib().BaryTreePlan_Create(
             ctypes.byref(self._h), ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx),

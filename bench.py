@@ -14,6 +14,12 @@ one the metric is quoted on -- and the plain FP64 form.  Both are checked
 against an FP64 direct sum at 1000 stride-sampled targets.
 
 Prints one JSON line on stdout.
+
+--workload batched measures the batching extension instead: 256 independent
+Coulomb systems (N log-uniform in [1k, 64k], 8-Gaussian mixtures, sigma 0.05,
+sources = targets), particle-cluster, theta 0.7, degree 6: the autotuned
+batched solve (one launch per phase for all systems) against 256 sequential
+plain single-system solves with the same leaf sizes.
 """
 import argparse
 import json
@@ -88,6 +94,108 @@ def cpu_baseline(n_evals):
     }
 
 
+def bench_batched(args):
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from barytree_amd.workloads import batched_sizes, gaussian_mixture
+
+    nsys = args.systems
+    sizes = batched_sizes(nsys, 3, nmin=1000, nmax=64000)
+    systems = []
+    for i, n in enumerate(sizes):
+        x, y, z, q, w = gaussian_mixture(int(n), 3000 + i)
+        systems.append(((x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, w)))
+    total = int(sizes.sum())
+    log(f"[bench] batched: {nsys} systems, {total} particles")
+    kargs = (bt.Kernel.COULOMB, [0.0], bt.Singularity.SKIPPING, bt.Approximation.LAGRANGE,
+             bt.ComputeType.PARTICLE_CLUSTER, 0.7, 6)
+    t0 = time.perf_counter()
+    sc = bt.autotune(systems, *kargs)
+    tune_s = time.perf_counter() - t0
+    log(f"[bench] autotune {tune_s:.2f}s: {sc.as_dict()}")
+    leaf_of = [sc.leaf[int(np.floor(np.log2(n)))] for n in sizes]
+
+    # one-shot batched solve, end to end (host trees and lists, upload, GPU, download)
+    def batched_once(precision):
+        t0 = time.perf_counter()
+        out = bt.solve_batched(systems, *kargs, schedule=sc, precision=precision)
+        return time.perf_counter() - t0, out, bt.last_stats()
+    batched_once(bt.PRECISION_FP64)  # warm-up
+    e2e = [batched_once(bt.PRECISION_FP64) for _ in range(args.steps)]
+    e2e_fp64 = min(r[0] for r in e2e)
+    out_fp64, st_b = e2e[-1][1], e2e[-1][2]
+    e2e_mixed_runs = [batched_once(bt.PRECISION_MIXED) for _ in range(args.steps)]
+    e2e_mixed = min(r[0] for r in e2e_mixed_runs)
+    out_mixed = e2e_mixed_runs[-1][1]
+    log(f"[bench] batched solve fp64 {e2e_fp64:.3f}s mixed {e2e_mixed:.3f}s "
+        f"(host tree {st_b['host_tree_s']:.3f} lists {st_b['host_lists_s']:.3f} "
+        f"gpu {st_b['gpu_total_s']:.4f})")
+
+    # 256 sequential plain solves, same leaf sizes
+    def sequential_once():
+        t0 = time.perf_counter()
+        outs, gpu = [], 0.0
+        for (t, s), lf in zip(systems, leaf_of):
+            outs.append(bt.solve(t, s, *kargs, lf, lf))
+            gpu += bt.last_stats()["gpu_total_s"]
+        return time.perf_counter() - t0, outs, gpu
+    sequential_once()  # warm-up
+    seq_runs = [sequential_once() for _ in range(max(1, args.steps - 1))]
+    seq = min(r[0] for r in seq_runs)
+    seq_gpu = min(r[2] for r in seq_runs)
+    outs_seq = seq_runs[-1][1]
+    exact = all(np.array_equal(a, b) for a, b in zip(outs_seq, out_fp64))
+    log(f"[bench] sequential {seq:.3f}s (sum of gpu phases {seq_gpu:.4f}s), batched==sequential: {exact}")
+
+    # resident-input GPU time: batched plan vs per-system plans
+    plan = bt.Plan.batched(systems, *kargs, schedule=sc)
+    wall_b, gpu_b, near_b, st = time_steps(plan, bt.PRECISION_FP64, args.steps, args.warmup)
+    off = plan.offsets
+    idx = np.linspace(0, total - 1, 1000).astype(np.int64)
+    direct = plan.direct_sample(idx)
+    _, gpu_bm, _, _ = time_steps(plan, bt.PRECISION_MIXED, args.steps, args.warmup)
+    plan.close()
+    err64 = rel_l2(direct, np.concatenate(out_fp64)[idx])
+    err32 = rel_l2(direct, np.concatenate(out_mixed)[idx])
+    log(f"[bench] resident batched plan gpu fp64 {gpu_b:.4f}s mixed {gpu_bm:.4f}s; "
+        f"relL2 fp64 {err64:.2e} mixed {err32:.2e}")
+    evals = st["pp_ptwise"] + st["pc_ptwise"]
+    line = {
+        "metric": "batched time-to-solution, 256 systems (vs 256 sequential plain solves)",
+        "value": round(e2e_fp64, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(e2e_fp64 * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{nsys} Coulomb systems, N log-uniform [1k,64k] (total {total}), "
+                         "8-Gaussian mixtures sigma=0.05, particle-cluster theta=0.7 degree 6"),
+            "systems": nsys, "particles": total, "compute_type": "PC", "theta": 0.7, "degree": 6,
+        },
+        "schedule": sc.as_dict(),
+        "autotune_s": round(tune_s, 3),
+        "sequential_s": round(seq, 4),
+        "speedup_vs_sequential": round(seq / e2e_fp64, 3),
+        "batched_mixed_s": round(e2e_mixed, 4),
+        "batched_equals_sequential_bitwise": exact,
+        "gpu_batched_fp64_s": round(gpu_b, 5),
+        "gpu_batched_mixed_s": round(gpu_bm, 5),
+        "gpu_sequential_sum_s": round(seq_gpu, 5),
+        "rel_l2_err_fp64": err64,
+        "rel_l2_err_mixed": err32,
+        "interactions": evals,
+        "interactions_per_s_gpu_fp64": evals / gpu_b,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +204,8 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["large", "batched"], default="large")
+    ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (HBM-traffic counter calibration)")
     args = ap.parse_args()
@@ -111,6 +221,10 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md, 'How the path would shard').", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batched":
+        bench_batched(args)
+        return
 
     import torch  # device setup and synchronisation only
     assert torch.cuda.is_available(), "no GPU"

@@ -91,6 +91,9 @@ typedef struct BaryTreeStats {
     int near_tiles, far_tiles;
     double host_copy_s, host_tree_s, host_lists_s, host_work_s, h2d_s;
     double gpu_upward_s, gpu_near_s, gpu_far_s, gpu_down_s, gpu_finish_s, gpu_total_s;
+    int num_systems;         /* independent systems in the plan (1 unless batched) */
+    int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets */
+    int persistent;          /* 1: persistent interaction launches */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
@@ -150,6 +153,82 @@ int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats);
  * indices; for error checks of the plan's result. */
 int BaryTreePlan_DirectSample(BaryTreePlan *plan, int m, const int *targetIndex, double *out);
 void BaryTreePlan_Destroy(BaryTreePlan *plan);
+
+/* ---- Batching: many independent systems in one set of launches -------
+ *
+ * System s owns targets [targetOffsets[s], targetOffsets[s+1]) and sources
+ * [sourceOffsets[s], sourceOffsets[s+1]) of the particle arrays
+ * (offsets[0] = 0, numSystems+1 entries, every range non-empty); its targets
+ * see only its own sources.  Each system is solved as BaryTreeInterface
+ * would solve it alone with its leaf sizes; all systems share the kernel,
+ * approximation, theta, degree and precision, and go through every GPU phase
+ * in one launch per phase (one interaction launch walks all their lists).
+ *
+ * Leaf sizes: maxPerSourceLeaf/maxPerTargetLeaf arrays (numSystems entries)
+ * if given, else the schedule's leaf size for each system's size class,
+ * else a schedule from BaryTree_Autotune (cached per kernel, singularity,
+ * compute type, theta, degree and precision for the life of the process). */
+#define BARYTREE_SIZE_CLASSES 32
+typedef struct BaryTreeSchedule {
+    int targets_per_thread;            /* 1 or 2 (near-field tiles of 256 or 512 targets) */
+    int persistent;                    /* 1: resident blocks pop tiles from per-XCD queues */
+    int leaf[BARYTREE_SIZE_CLASSES];   /* leaf size of systems with
+                                          2^c <= max(targets, sources) < 2^(c+1); 0 = unset */
+    int trials;                        /* plans the autotuner timed */
+    double tune_seconds;               /* wall time the autotuner spent */
+} BaryTreeSchedule;
+
+int BaryTree_SolveBatched(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                          const double *targetX, const double *targetY, const double *targetZ,
+                          const double *targetValue, const double *sourceX,
+                          const double *sourceY, const double *sourceZ,
+                          const double *sourceValue, const double *sourceWeight,
+                          double *outputArray, int kernel, int numKernelParams,
+                          const double *kernelParams, int singularity, int approximation,
+                          int compute_type, double theta, int interpOrder,
+                          const int *maxPerSourceLeaf, const int *maxPerTargetLeaf,
+                          const BaryTreeSchedule *schedule, double sizeCheck, double beta,
+                          int verbosity, int precision);
+
+int BaryTreePlan_CreateBatched(BaryTreePlan **plan, int numSystems, const int *targetOffsets,
+                               const int *sourceOffsets, const double *targetX,
+                               const double *targetY, const double *targetZ,
+                               const double *targetValue, const double *sourceX,
+                               const double *sourceY, const double *sourceZ,
+                               const double *sourceValue, const double *sourceWeight,
+                               int kernel, int numKernelParams, const double *kernelParams,
+                               int singularity, int approximation, int compute_type,
+                               double theta, int interpOrder, const int *maxPerSourceLeaf,
+                               const int *maxPerTargetLeaf, const BaryTreeSchedule *schedule,
+                               double sizeCheck, double beta, int precision);
+
+/* BaryTree_Analyze for a batch: host-only trees and lists of every system,
+ * counts summed over the systems.  Leaf sizes as BaryTree_SolveBatched, but
+ * without the autotuner (it needs the GPU): arrays or a schedule are
+ * required unless beta is in [0, 1]. */
+int BaryTree_AnalyzeBatched(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                            const double *targetX, const double *targetY, const double *targetZ,
+                            const double *sourceX, const double *sourceY, const double *sourceZ,
+                            int kernel, int numKernelParams, const double *kernelParams,
+                            int singularity, int approximation, int compute_type, double theta,
+                            int interpOrder, const int *maxPerSourceLeaf,
+                            const int *maxPerTargetLeaf, const BaryTreeSchedule *schedule,
+                            double sizeCheck, double beta, BaryTreeStats *stats);
+
+/* Autotuner: picks the leaf size of every size class present in the batch
+ * (candidates P/2, P, 2P, 4P, 8P with P = (interpOrder+1)^3), then the tile
+ * shape (targets per thread) and the launch form (one-shot or persistent),
+ * by timing GPU plans of up to 8 sampled systems per class, replicated to
+ * about 2^20 targets.  Particles and charges are those of the batch.
+ * Returns 0 or a negative value. */
+int BaryTree_Autotune(int numSystems, const int *targetOffsets, const int *sourceOffsets,
+                      const double *targetX, const double *targetY, const double *targetZ,
+                      const double *targetValue, const double *sourceX, const double *sourceY,
+                      const double *sourceZ, const double *sourceValue,
+                      const double *sourceWeight, int kernel, int numKernelParams,
+                      const double *kernelParams, int singularity, int approximation,
+                      int compute_type, double theta, int interpOrder, double sizeCheck,
+                      double beta, int precision, BaryTreeSchedule *schedule);
 
 #ifdef __cplusplus
 }

@@ -92,3 +92,69 @@ def test_unsupported_combinations_are_errors(bt):
         bt.analyze(t, s, 1, [0.5], 1, 1, 1, 0.7, 13, 10, 10)
     with pytest.raises(RuntimeError, match="positive"):
         bt.analyze([a[:0] for a in t], s, 1, [0.5], 1, 1, 1, 0.7, 3, 10, 10)
+
+
+def _systems(sizes, seed):
+    out = []
+    for i, n in enumerate(sizes):
+        t, s = uniform_cube(n, max(1, (3 * n) // 4), seed=seed + i)
+        out.append((t, s))
+    return out
+
+
+@pytest.mark.parametrize("ct", [1, 2, 3])
+def test_batched_lists_are_the_sum_of_single_systems(bt, oracle_mod, ct):
+    # one plan over several systems: every system gets its own trees and lists
+    sizes = [1, 40, 900, 2500, 3000]
+    leaf = [10, 8, 30, 60, 45]
+    systems = _systems(sizes, 300 + ct)
+    a = bt.analyze_batched(systems, 1, [0.5], 1, 1, ct, 0.7, 3, max_source_leaf=leaf,
+                           max_target_leaf=leaf)
+    tot = {k: 0 for k in KEYS}
+    for (t, s), lf in zip(systems, leaf):
+        oracle_mod.treecode(*t, *s, 1, [0.5], 1, 1, ct, 0.7, 3, lf, lf)
+        o = oracle_mod.last_stats()
+        for k in KEYS:
+            tot[k] += o[k]
+    assert {k: a[k] for k in KEYS} == tot
+    assert a["num_systems"] == len(sizes)
+    pairs = sum(len(t[0]) * len(s[0]) for t, s in systems)
+    assert a["pp_ptwise"] + a["pc_replaced"] + a["cp_replaced"] + a["cc_replaced"] == pairs
+
+
+def test_batched_schedule_classes_pick_leaf_sizes(bt, oracle_mod):
+    # size class c holds systems with 2^c <= max(M, N) < 2^(c+1); an unset
+    # class takes the nearest set one
+    systems = _systems([700, 3000, 20000], 41)
+    sc = bt.Schedule.make({9: 64, 14: 500})
+    a = bt.analyze_batched(systems, 1, [0.5], 1, 1, 1, 0.7, 4, schedule=sc)
+    tot = {k: 0 for k in KEYS}
+    for (t, s), lf in zip(systems, (64, 64, 500)):   # 3000 is class 11: nearest set is 9
+        oracle_mod.treecode(*t, *s, 1, [0.5], 1, 1, 1, 0.7, 4, lf, lf)
+        o = oracle_mod.last_stats()
+        for k in KEYS:
+            tot[k] += o[k]
+    assert {k: a[k] for k in KEYS} == tot
+
+
+def test_batched_argument_errors(bt):
+    systems = _systems([100, 200], 5)
+    args = (1, [0.5], 1, 1, 1, 0.7, 3)
+    with pytest.raises(RuntimeError, match="leaf sizes or a schedule"):
+        bt.analyze_batched(systems, *args)
+    with pytest.raises(RuntimeError, match="targets_per_thread"):
+        bt.analyze_batched(systems, *args, schedule=bt.Schedule.make({}, targets_per_thread=3))
+    with pytest.raises(RuntimeError, match="HERMITE"):
+        bt.analyze_batched(systems, 1, [0.5], 1, 2, 1, 0.7, 3, max_source_leaf=10,
+                           max_target_leaf=10)
+    with pytest.raises(RuntimeError, match="leaf sizes must be positive"):
+        bt.analyze_batched(systems, *args, max_source_leaf=[10, 0], max_target_leaf=10)
+    with pytest.raises(RuntimeError, match="at least one target"):
+        bt.analyze_batched([systems[0], ([a[:0] for a in systems[1][0]], systems[1][1])], *args,
+                           max_source_leaf=10, max_target_leaf=10)
+    # solve entry points reject the same inputs before any GPU work
+    with pytest.raises(RuntimeError, match="at least one target"):
+        bt.solve_batched([systems[0], ([a[:0] for a in systems[1][0]], systems[1][1])], *args,
+                         max_source_leaf=10, max_target_leaf=10)
+    with pytest.raises(RuntimeError, match="numSystems"):
+        bt.solve_batched([], *args, max_source_leaf=10, max_target_leaf=10)
