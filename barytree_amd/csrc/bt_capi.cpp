@@ -12,10 +12,16 @@
 #include <vector>
 
 #include "../../include/BaryTreeInterface.h"
+#include "bt_learn.h"
+#include "bt_model.h"
 #include "bt_solver.h"
 
 struct BaryTreePlan {
     bt::Plan plan;
+};
+
+struct BaryTreeDegreeModel {
+    bt::DegreeModel m;
 };
 
 namespace {
@@ -51,6 +57,21 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->gpu_down_s = t.gpu_down; s->gpu_finish_s = t.gpu_finish; s->gpu_total_s = t.gpu_total;
     s->num_systems = p.num_systems(); s->targets_per_thread = p.tpt();
     s->persistent = p.persistent() ? 1 : 0;
+    s->num_far_pairs = (long long)p.far_pairs_list().size();
+    s->variable_degree = p.variable_degree() ? 1 : 0;
+    s->mean_pair_degree = p.mean_pair_degree();
+    s->far_ptwise_variable = p.variable_far_evals();
+}
+
+std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)
+{
+    std::vector<float> f;
+    plan.pair_features(f);
+    const int n = (int)plan.far_pairs_list().size();
+    std::vector<int> deg(n);
+    std::string e = m.predict(n, f.data(), delta, deg.data(), nullptr);
+    if (!e.empty()) return e;
+    return plan.set_pair_degrees(deg.data());
 }
 
 int plan_create(bt::Plan &plan, int M, int N, const double *tx, const double *ty, const double *tz,
@@ -624,6 +645,256 @@ int BaryTree_Autotune(int numSystems, const int *targetOffsets, const int *sourc
     const BatchArrays a{targetX, targetY, targetZ, targetValue, sourceX, sourceY, sourceZ,
                         sourceValue, sourceWeight};
     return autotune(prm, numSystems, targetOffsets, sourceOffsets, a, schedule);
+}
+
+
+int BaryTreePlan_NumPairs(const BaryTreePlan *plan)
+{
+    if (!plan) return fail("plan is NULL");
+    return (int)plan->plan.far_pairs_list().size();
+}
+
+int BaryTreePlan_PairInfo(const BaryTreePlan *plan, int *kind, int *system)
+{
+    if (!plan) return fail("plan is NULL");
+    const auto &pr = plan->plan.far_pairs_list();
+    for (size_t i = 0; i < pr.size(); i++) {
+        if (kind) kind[i] = pr[i].kind;
+        if (system) system[i] = pr[i].sys;
+    }
+    return 0;
+}
+
+int BaryTreePlan_PairFeatures(const BaryTreePlan *plan, float *features)
+{
+    if (!plan || !features) return fail("NULL argument");
+    std::vector<float> f;
+    plan->plan.pair_features(f);
+    std::memcpy(features, f.data(), f.size() * sizeof(float));
+    return 0;
+}
+
+int BaryTreePlan_LabelPairs(BaryTreePlan *plan, int first, int count, double tol, int *labels,
+                            double *errors)
+{
+    g_last_error.clear();
+    if (!plan || !labels) return fail("NULL argument");
+    std::string e = plan->plan.label_pairs(first, count, tol, labels, errors);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+int BaryTreePlan_SetPairDegrees(BaryTreePlan *plan, const int *degrees)
+{
+    g_last_error.clear();
+    if (!plan) return fail("plan is NULL");
+    std::string e = plan->plan.set_pair_degrees(degrees);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+int BaryTreePlan_UseDegreeModel(BaryTreePlan *plan, const BaryTreeDegreeModel *model, float delta)
+{
+    g_last_error.clear();
+    if (!plan || !model) return fail("NULL argument");
+    std::string e = use_model(plan->plan, model->m, delta);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
+                        const int *targetCount, const int *sourceBegin, const int *sourceCount,
+                        const double *targetBox, const double *sourceBox, int numTargets,
+                        const double *targetX, const double *targetY, const double *targetZ,
+                        int numSources, const double *sourceX, const double *sourceY,
+                        const double *sourceZ, const double *sourceValue, int kernel,
+                        double kernelParam, double tol, int *labels, double *errors)
+{
+    g_last_error.clear();
+    if (numPairs < 0 || !labels) return fail("bad argument");
+    if (numPairs == 0) return 0;
+    if (!kind || !targetBegin || !targetCount || !sourceBegin || !sourceCount || !targetBox ||
+        !sourceBox || !targetX || !targetY || !targetZ || !sourceX || !sourceY || !sourceZ ||
+        !sourceValue)
+        return fail("NULL argument");
+    if (kernel < 1 || kernel > 3) return fail("INVALID KERNEL (supported: COULOMB, YUKAWA, REGULARIZED_COULOMB)");
+    for (int i = 0; i < numPairs; i++) {
+        if (kind[i] < 1 || kind[i] > 3) return fail("pair kind must be 1, 2 or 3");
+        if (targetCount[i] <= 0 || sourceCount[i] <= 0 || targetBegin[i] < 0 || sourceBegin[i] < 0 ||
+            (long long)targetBegin[i] + targetCount[i] > numTargets ||
+            (long long)sourceBegin[i] + sourceCount[i] > numSources)
+            return fail("pair particle range out of bounds");
+    }
+    std::vector<void *> tmp;
+    auto cleanup = [&]() { for (void *q : tmp) (void)hipFree(q); };
+    auto put = [&](const void *h, size_t bytes) -> void * {
+        void *d = nullptr;
+        if (hipMalloc(&d, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        tmp.push_back(d);
+        if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return d;
+    };
+    btl::LabelArgs a{};
+    a.npairs = numPairs;
+    const size_t np = numPairs;
+    a.kind = (const int *)put(kind, np * sizeof(int));
+    a.tbeg = (const int *)put(targetBegin, np * sizeof(int));
+    a.tcnt = (const int *)put(targetCount, np * sizeof(int));
+    a.sbeg = (const int *)put(sourceBegin, np * sizeof(int));
+    a.scnt = (const int *)put(sourceCount, np * sizeof(int));
+    a.tbox = (const double *)put(targetBox, 6 * np * sizeof(double));
+    a.sbox = (const double *)put(sourceBox, 6 * np * sizeof(double));
+    a.tx = (const double *)put(targetX, (size_t)numTargets * sizeof(double));
+    a.ty = (const double *)put(targetY, (size_t)numTargets * sizeof(double));
+    a.tz = (const double *)put(targetZ, (size_t)numTargets * sizeof(double));
+    a.sx = (const double *)put(sourceX, (size_t)numSources * sizeof(double));
+    a.sy = (const double *)put(sourceY, (size_t)numSources * sizeof(double));
+    a.sz = (const double *)put(sourceZ, (size_t)numSources * sizeof(double));
+    a.sq = (const double *)put(sourceValue, (size_t)numSources * sizeof(double));
+    void *dl = nullptr, *de = nullptr;
+    if (hipMalloc(&dl, np * sizeof(int)) == hipSuccess) tmp.push_back(dl); else dl = nullptr;
+    if (hipMalloc(&de, np * btl::LABEL_ND * sizeof(double)) == hipSuccess) tmp.push_back(de); else de = nullptr;
+    if (!a.kind || !a.tbeg || !a.tcnt || !a.sbeg || !a.scnt || !a.tbox || !a.sbox || !a.tx || !a.ty ||
+        !a.tz || !a.sx || !a.sy || !a.sz || !a.sq || !dl || !de) {
+        cleanup();
+        return fail("device buffers for labels failed");
+    }
+    a.kernel = kernel;
+    a.p0 = kernelParam;
+    a.tol = tol;
+    a.labels = (int *)dl;
+    a.errors = (double *)de;
+    hipError_t he = btl::launch_label(a, nullptr);
+    if (he == hipSuccess) he = hipMemcpy(labels, dl, np * sizeof(int), hipMemcpyDeviceToHost);
+    if (he == hipSuccess && errors)
+        he = hipMemcpy(errors, de, np * btl::LABEL_ND * sizeof(double), hipMemcpyDeviceToHost);
+    cleanup();
+    if (he != hipSuccess) return fail(std::string("label kernel: ") + hipGetErrorString(he));
+    return 0;
+}
+
+int BaryTreeModel_Create(BaryTreeDegreeModel **model, unsigned long long seed)
+{
+    g_last_error.clear();
+    if (!model) return fail("model is NULL");
+    *model = new (std::nothrow) BaryTreeDegreeModel;
+    if (!*model) return fail("out of host memory");
+    (*model)->m.init(seed);
+    return 0;
+}
+
+int BaryTreeModel_Load(BaryTreeDegreeModel **model, const char *path)
+{
+    g_last_error.clear();
+    if (!model || !path) return fail("NULL argument");
+    *model = nullptr;
+    BaryTreeDegreeModel *m = new (std::nothrow) BaryTreeDegreeModel;
+    if (!m) return fail("out of host memory");
+    std::string e = m->m.load(path);
+    if (!e.empty()) {
+        delete m;
+        return fail(e);
+    }
+    *model = m;
+    return 0;
+}
+
+int BaryTreeModel_Save(const BaryTreeDegreeModel *model, const char *path)
+{
+    g_last_error.clear();
+    if (!model || !path) return fail("NULL argument");
+    std::string e = model->m.save(path);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+int BaryTreeModel_GetParams(const BaryTreeDegreeModel *model, float *params, float *featureMean,
+                            float *featureScale)
+{
+    if (!model) return fail("model is NULL");
+    if (params) std::memcpy(params, model->m.params.data(), btl::MLP_PARAMS * sizeof(float));
+    if (featureMean) std::memcpy(featureMean, model->m.mean, sizeof(model->m.mean));
+    if (featureScale) std::memcpy(featureScale, model->m.scale, sizeof(model->m.scale));
+    return 0;
+}
+
+int BaryTreeModel_SetParams(BaryTreeDegreeModel *model, const float *params,
+                            const float *featureMean, const float *featureScale)
+{
+    if (!model) return fail("model is NULL");
+    if (params) std::memcpy(model->m.params.data(), params, btl::MLP_PARAMS * sizeof(float));
+    if (featureMean) std::memcpy(model->m.mean, featureMean, sizeof(model->m.mean));
+    if (featureScale) std::memcpy(model->m.scale, featureScale, sizeof(model->m.scale));
+    return 0;
+}
+
+int BaryTreeModel_Gradient(const BaryTreeDegreeModel *model, int n, const float *features,
+                           const int *labels, float *grad, double *loss)
+{
+    g_last_error.clear();
+    if (!model || !features || !labels || !grad) return fail("NULL argument");
+    std::vector<float> g;
+    double l = 0.0;
+    std::string e = model->m.gradient(n, features, labels, g, l);
+    if (!e.empty()) return fail(e);
+    std::memcpy(grad, g.data(), g.size() * sizeof(float));
+    if (loss) *loss = l;
+    return 0;
+}
+
+int BaryTreeModel_Train(BaryTreeDegreeModel *model, int n, const float *features,
+                        const int *labels, int epochs, int batch, float learningRate,
+                        unsigned long long seed, float *epochLoss)
+{
+    g_last_error.clear();
+    if (!model || !features || !labels) return fail("NULL argument");
+    std::vector<float> el;
+    std::string e = model->m.train(n, features, labels, epochs, batch, learningRate, seed, &el);
+    if (!e.empty()) return fail(e);
+    if (epochLoss) std::memcpy(epochLoss, el.data(), el.size() * sizeof(float));
+    return 0;
+}
+
+int BaryTreeModel_Predict(const BaryTreeDegreeModel *model, int n, const float *features,
+                          float delta, int *degrees, float *logits)
+{
+    g_last_error.clear();
+    if (!model || (n > 0 && !features)) return fail("NULL argument");
+    std::string e = model->m.predict(n, features, delta, degrees, logits);
+    if (!e.empty()) return fail(e);
+    return 0;
+}
+
+void BaryTreeModel_Destroy(BaryTreeDegreeModel *model) { delete model; }
+
+int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
+                          const double *targetY, const double *targetZ, const double *targetValue,
+                          const double *sourceX, const double *sourceY, const double *sourceZ,
+                          const double *sourceValue, const double *sourceWeight,
+                          double *outputArray, int kernel, int numKernelParams,
+                          const double *kernelParams, int singularity, int approximation,
+                          int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                          int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                          int precision, const BaryTreeDegreeModel *model, float delta)
+{
+    g_last_error.clear();
+    if (!outputArray) return fail("outputArray is NULL");
+    if (numTargets > 0) std::memset(outputArray, 0, (size_t)numTargets * sizeof(double));
+    bt::Plan plan;
+    int rc = plan_create(plan, numTargets, numSources, targetX, targetY, targetZ, targetValue,
+                         sourceX, sourceY, sourceZ, sourceValue, sourceWeight, kernel,
+                         numKernelParams, kernelParams, singularity, approximation, compute_type,
+                         theta, interpOrder, maxPerSourceLeaf, maxPerTargetLeaf, sizeCheck, beta,
+                         verbosity, precision);
+    if (rc) return rc;
+    std::string e;
+    if (model) e = use_model(plan, model->m, delta);
+    if (e.empty()) e = plan.execute(true);
+    if (e.empty()) e = plan.download(outputArray);
+    if (!e.empty()) return fail(e);
+    fill_stats(plan, &g_stats);
+    g_have_stats = true;
+    return 0;
 }
 
 }  // extern "C"

@@ -47,6 +47,7 @@ struct InteractArgs {
     const double *cx, *cy, *cz, *cq, *cw;  // source cluster points and charges
     KParams kp;
     double *out1, *out2;     // potential (particles) or target-cluster Q and W
+    int accumulate;          // particle targets: 1 adds to out1, 0 stores
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);

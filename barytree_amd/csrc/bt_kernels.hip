@@ -304,6 +304,8 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
             const size_t o = (size_t)tile.w * A.P + tix[u];
             A.out1[o] = acc[u];
             if constexpr (SS) A.out2[o] = accw[u];
+        } else if (A.accumulate) {
+            A.out1[tix[u]] += acc[u];
         } else {
             A.out1[tix[u]] = acc[u];
         }

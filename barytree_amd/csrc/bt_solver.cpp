@@ -22,16 +22,59 @@ double now()
         if (e_ != hipSuccess) return std::string(#expr ": ") + hipGetErrorString(e_); \
     } while (0)
 
-constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
 constexpr int NXCD = btk::TILE_QUEUES;  // XCDs of an MI355X, one tile queue each
-constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
 
 }  // namespace
+
+// Tiles.  All tiles of one key read the same sources, so they should run at
+// the same time on the same XCD and share its L2.  Workgroups are dispatched
+// round-robin over the 8 XCDs (block b runs on XCD b % 8), so keys (groups of
+// tiles) are dealt to per-XCD lists, heaviest group to the least loaded XCD,
+// and block 8*j + x takes the j-th tile of XCD x's list.  Lists are padded
+// with empty tiles (count 0), which return at once.
+void make_queue_tiles(const WorkCSR &w, int tile_t, bool grid, int P, std::vector<btk::Tile> &tiles,
+                      int *qlen)
+{
+    struct Group { long long work; std::vector<btk::Tile> t; };
+    std::vector<Group> groups;
+    for (size_t k = 0; k < w.key.size(); k++) {
+        long long src = 0;
+        for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
+        src += (long long)(w.clu_off[k + 1] - w.clu_off[k]) * P;
+        Group g{0, {}};
+        for (int o = 0; o < w.tcnt[k]; o += tile_t) {
+            btk::Tile t;
+            t.x = (int)k;
+            t.y = w.tbeg[k] + o;
+            t.z = std::min(tile_t, w.tcnt[k] - o);
+            t.w = grid ? w.key[k] : 0;
+            g.work += (src + 1) * t.z;
+            g.t.push_back(t);
+        }
+        if (!g.t.empty()) groups.push_back(std::move(g));
+    }
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const Group &a, const Group &b) { return a.work > b.work; });
+    std::vector<std::vector<btk::Tile>> lists(NXCD);
+    std::vector<long long> load(NXCD, 0);
+    for (const Group &g : groups) {
+        int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += g.work;
+        lists[x].insert(lists[x].end(), g.t.begin(), g.t.end());
+    }
+    size_t L = 0;
+    for (auto &l : lists) L = std::max(L, l.size());
+    tiles.assign(L * NXCD, btk::Tile{0, 0, 0, 0});
+    for (int x = 0; x < NXCD; x++)
+        for (size_t j = 0; j < lists[x].size(); j++) tiles[j * NXCD + x] = lists[x][j];
+    for (int x = 0; x < NXCD; x++) qlen[x] = (int)lists[x].size();
+}
 
 Plan::~Plan() { free_all(); }
 
 void Plan::free_all()
 {
+    free_var();
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
     if (st_) {
@@ -44,7 +87,7 @@ template <class T> T *Plan::dalloc(size_t n)
 {
     void *p = nullptr;
     if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
-    allocs_.push_back(p);
+    (alloc_var_ ? var_allocs_ : allocs_).push_back(p);
     return static_cast<T *>(p);
 }
 
@@ -149,50 +192,8 @@ void Plan::build_work()
         }
     }
 
-    // Tiles.  All tiles of one key read the same sources, so they should run
-    // at the same time on the same XCD and share its L2.  Workgroups are
-    // dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), so
-    // keys (groups of tiles) are dealt to per-XCD lists, heaviest group to the
-    // least loaded XCD, and block 8*j + x takes the j-th tile of XCD x's list.
-    // Lists are padded with empty tiles (count 0), which return at once.
-    auto make_tiles = [&](const WorkCSR &w, bool grid, std::vector<btk::Tile> &tiles, int *qlen) {
-        const int tile_t = grid ? GRID_TILE : tile_targets_;
-        struct Group { long long work; std::vector<btk::Tile> t; };
-        std::vector<Group> groups;
-        for (size_t k = 0; k < w.key.size(); k++) {
-            long long src = 0;
-            for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
-            src += (long long)(w.clu_off[k + 1] - w.clu_off[k]) * P;
-            Group g{0, {}};
-            for (int o = 0; o < w.tcnt[k]; o += tile_t) {
-                btk::Tile t;
-                t.x = (int)k;
-                t.y = w.tbeg[k] + o;
-                t.z = std::min(tile_t, w.tcnt[k] - o);
-                t.w = grid ? w.key[k] : 0;
-                g.work += (src + 1) * t.z;
-                g.t.push_back(t);
-            }
-            if (!g.t.empty()) groups.push_back(std::move(g));
-        }
-        std::stable_sort(groups.begin(), groups.end(),
-                         [](const Group &a, const Group &b) { return a.work > b.work; });
-        std::vector<std::vector<btk::Tile>> lists(NXCD);
-        std::vector<long long> load(NXCD, 0);
-        for (const Group &g : groups) {
-            int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[x] += g.work;
-            lists[x].insert(lists[x].end(), g.t.begin(), g.t.end());
-        }
-        size_t L = 0;
-        for (auto &l : lists) L = std::max(L, l.size());
-        tiles.assign(L * NXCD, btk::Tile{0, 0, 0, 0});
-        for (int x = 0; x < NXCD; x++)
-            for (size_t j = 0; j < lists[x].size(); j++) tiles[j * NXCD + x] = lists[x][j];
-        for (int x = 0; x < NXCD; x++) qlen[x] = (int)lists[x].size();
-    };
-    make_tiles(near_, false, near_tiles_, near_qlen_);
-    make_tiles(grid_, true, grid_tiles_, grid_qlen_);
+    make_queue_tiles(near_, tile_targets_, false, P, near_tiles_, near_qlen_);
+    make_queue_tiles(grid_, GRID_TILE, true, P, grid_tiles_, grid_qlen_);
 
     // upward pass (source trees, PC and CC): non-leaf nodes per level,
     // deepest first, every system in the same launch
@@ -459,6 +460,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
 
     build_work();
+    enumerate_pairs();
     const bool need_src_clusters = prm.compute_type != C_CP;
     const bool need_tgt_clusters = prm.compute_type != C_PC;
     std::vector<double> stab, ttab, tab;
@@ -583,6 +585,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
 
 std::string Plan::execute(bool timed)
 {
+    if (var_) return execute_var(timed);
     const Params &p = prm_;
     const bool ss = p.singularity == S_SUBTRACTION;
     const int nl = p.degree + 1;

@@ -19,6 +19,9 @@
 
 namespace bt {
 
+constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
+constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
+
 struct Timings {
     double host_copy = 0, host_tree = 0, host_lists = 0, host_work = 0, h2d = 0;  // seconds
     double gpu_upward = 0, gpu_near = 0, gpu_far = 0, gpu_down = 0, gpu_finish = 0, gpu_total = 0;
@@ -34,8 +37,53 @@ struct System {
     int snode_off = 0, tnode_off = 0;      // global node id offsets
 };
 
+// An approximated (far-field) pair of the interaction lists: target node t
+// and source node s of system sys (node ids local to the system's trees).
+//   FAR_PC: source cluster -> target particles (PC lists, CC's PC entries)
+//   FAR_CP: source particles -> target grid     (CP lists, CC's CP entries)
+//   FAR_CC: source cluster -> target grid       (CC lists)
+enum FarKind { FAR_PC = 1, FAR_CP = 2, FAR_CC = 3 };
+struct FarPair { int sys, kind, t, s; };
+
+// Features of a far-field pair, for the degree model.
+constexpr int NUM_PAIR_FEATURES = 8;
+
+// Work of one interpolation degree when every far-field pair has its own
+// degree.  Source slots hold modified charges of (source node, degree),
+// computed from the node's particles; target slots hold grid values of
+// (target node, degree), interpolated straight to the node's particles.
+struct DegreeWork {
+    int degree = 0, nl = 0, P = 0;
+    std::vector<int> s_ibeg, s_iend;       // global particle range per source slot
+    std::vector<double> s_tab, t_tab;      // 3*nl Chebyshev nodes per slot
+    WorkCSR pc, grid;                      // particle targets <- source slots;
+                                           // target slots <- source slots + particle ranges
+    std::vector<btk::Tile> pc_tiles, grid_tiles;
+    int pc_qlen[btk::TILE_QUEUES] = {}, grid_qlen[btk::TILE_QUEUES] = {};
+    std::vector<std::vector<btk::Tile>> down;  // per tree level: {target slot, first target, count}
+    long long evals = 0;                   // point-wise evaluations of this degree's pairs
+    // device copies
+    int *d_s_ident = nullptr, *d_s_ibeg = nullptr, *d_s_iend = nullptr;
+    double *d_s_tab = nullptr, *d_cx = nullptr, *d_cy = nullptr, *d_cz = nullptr;
+    double *d_cq = nullptr, *d_cw = nullptr;
+    double *d_t_tab = nullptr, *d_Qt = nullptr, *d_Wt = nullptr;
+    int *d_pc_seg_off = nullptr, *d_pc_clu_off = nullptr, *d_pc_clu_id = nullptr;
+    btk::Seg *d_pc_segs = nullptr;
+    btk::Tile *d_pc_tiles = nullptr;
+    int *d_g_seg_off = nullptr, *d_g_clu_off = nullptr, *d_g_clu_id = nullptr;
+    btk::Seg *d_g_segs = nullptr;
+    btk::Tile *d_g_tiles = nullptr;
+    std::vector<btk::Tile *> d_down;
+};
+
 // Input of a plan: nsys systems, system s owning targets [toffs[s],
 // toffs[s+1]) and sources [soffs[s], soffs[s+1]) of the arrays.
+// Deals a CSR's keys into tiles of at most tile_targets targets, as 8
+// interleaved per-XCD queues (tile 8*j + x = j-th tile of queue x; padding
+// slots have count 0); qlen gets each queue's length.
+void make_queue_tiles(const WorkCSR &w, int tile_targets, bool grid, int P,
+                      std::vector<btk::Tile> &tiles, int *qlen);
+
 struct PlanInput {
     int nsys = 1;
     const int *toffs = nullptr, *soffs = nullptr;  // nsys+1 entries each
@@ -65,6 +113,24 @@ public:
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
 
     void set_precision(int p) { prm_.precision = p; }
+
+    // Far-field pairs in list order (per system; PC: per target batch its
+    // approx list; CP: per source batch its approx list; CC: per target node
+    // its CC, then PC, then CP entries).
+    const std::vector<FarPair> &far_pairs_list() const { return pairs_; }
+    // NUM_PAIR_FEATURES floats per pair (see bt_vardeg.cpp).
+    void pair_features(std::vector<float> &f) const;
+    // Gives every far-field pair its own degree (1..12) and builds and
+    // uploads the per-degree work; nullptr returns to the fixed degree.
+    std::string set_pair_degrees(const int *deg);
+    bool variable_degree() const { return var_; }
+    double mean_pair_degree() const { return var_mean_deg_; }
+    long long variable_far_evals() const { return var_evals_; }
+    // Minimal degree in [2, 12] whose pair error (relative L2 over sampled
+    // targets, sampled sources) is <= tol, for pairs [first, first+count);
+    // 13 where degree 12 does not reach tol.  err (optional) gets the
+    // errors at degrees 2..12 (11 per pair; NaN where not computed).
+    std::string label_pairs(int first, int count, double tol, int *labels, double *err);
     const Counts &counts() const { return counts_; }
     const Timings &timings() const { return tm_; }
     const Params &params() const { return prm_; }
@@ -88,6 +154,10 @@ private:
     void free_all();
     void build_work();
     std::string validate() const;
+    void enumerate_pairs();
+    void free_var();
+    std::string validate_var() const;
+    std::string execute_var(bool timed);
 
     Params prm_;
     int M_ = 0, N_ = 0, tpt_ = 2, tile_targets_ = 512;
@@ -109,6 +179,20 @@ private:
 
     hipStream_t st_ = nullptr;
     std::vector<void *> allocs_;
+    // variable degree
+    std::vector<FarPair> pairs_;
+    bool var_ = false;
+    double var_mean_deg_ = 0;
+    long long var_evals_ = 0;
+    std::vector<DegreeWork> dw_;
+    WorkCSR near_pp_;
+    std::vector<btk::Tile> near_pp_tiles_;
+    int near_pp_qlen_[btk::TILE_QUEUES] = {};
+    int *d_pp_seg_off_ = nullptr, *d_pp_clu_off_ = nullptr;
+    btk::Seg *d_pp_segs_ = nullptr;
+    btk::Tile *d_pp_tiles_ = nullptr;
+    std::vector<void *> var_allocs_;
+    bool alloc_var_ = false;  // route dalloc to var_allocs_
     // device arrays
     double *d_tx_ = nullptr, *d_ty_ = nullptr, *d_tz_ = nullptr, *d_tq_ = nullptr;
     double *d_sx_ = nullptr, *d_sy_ = nullptr, *d_sz_ = nullptr, *d_sq_ = nullptr, *d_sw_ = nullptr;

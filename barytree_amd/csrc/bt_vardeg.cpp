@@ -1,0 +1,579 @@
+// bt_vardeg.cpp -- far-field pairs with their own interpolation degree.
+//
+// The fixed-degree path follows the reference: one degree for every cluster,
+// charges passed child to parent (src/clusters/clusters.c:50-200), grid
+// values passed parent to child (interaction_compute_downpass.c:37-142).
+// With a degree per pair that chain breaks (a parent and its child may be
+// used at different degrees), so here
+//   - a source node used at degree p gets modified charges of degree p
+//     computed straight from its particles (source slot (node, p)),
+//   - a target node receiving at degree p gets a grid of degree p (target
+//     slot (node, p)) whose interpolant is evaluated straight at the node's
+//     particles.
+// Both are exact rewrites of the chains: a degree-p interpolant reproduces
+// every polynomial of degree <= p, so the Lagrange basis of the parent grid
+// is reproduced by the child grid and the two orders give the same sums
+// (up to rounding).  With every pair at the plan's degree the result is the
+// fixed-degree result.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "bt_learn.h"
+#include "bt_solver.h"
+
+namespace bt {
+
+namespace {
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) return std::string(#expr ": ") + hipGetErrorString(e_); \
+    } while (0)
+
+constexpr int NQ = btk::TILE_QUEUES;
+constexpr int MAXDEG = 12;
+
+}  // namespace
+
+void Plan::enumerate_pairs()
+{
+    pairs_.clear();
+    for (int sy = 0; sy < (int)sys_.size(); sy++) {
+        const System &y = sys_[sy];
+        const Lists &L = y.L;
+        if (prm_.compute_type == C_PC) {
+            for (int b = 0; b < y.ttree.n; b++)
+                for (int c : L.approx[b]) pairs_.push_back({sy, FAR_PC, b, c});
+        } else if (prm_.compute_type == C_CP) {
+            for (int b = 0; b < y.stree.n; b++)
+                for (int t : L.approx[b]) pairs_.push_back({sy, FAR_CP, t, b});
+        } else {
+            for (int t = 0; t < y.ttree.n; t++) {
+                for (int s : L.approx[t]) pairs_.push_back({sy, FAR_CC, t, s});
+                for (int s : L.cc_pc[t]) pairs_.push_back({sy, FAR_PC, t, s});
+                for (int s : L.cc_cp[t]) pairs_.push_back({sy, FAR_CP, t, s});
+            }
+        }
+    }
+}
+
+// Features of pair (t, s), all scale free:
+//   0 r_t/d   1 r_s/d      (node radii over the distance of the node centres,
+//                           the quantities of the acceptance criterion)
+//   2 log2(n_t)/16  3 log2(n_s)/16   (particle counts)
+//   4 [kind == PC]  5 [kind == CP]   (which side is interpolated; CC: both)
+//   6 kernel parameter at the pair's scale: Yukawa kappa*d, regularized
+//     Coulomb epsilon/d, Coulomb 0
+//   7 largest half-width of the interpolated box(es) over d
+void Plan::pair_features(std::vector<float> &f) const
+{
+    const int nf = NUM_PAIR_FEATURES;
+    f.assign(pairs_.size() * nf, 0.0f);
+    const bool ss = prm_.singularity == S_SUBTRACTION;
+    double par = 0.0;
+    int pk = 0;  // 0 none, 1 multiply by d, 2 divide by d
+    if (prm_.kernel == K_YUKAWA) { par = prm_.kp[0]; pk = 1; }
+    if (prm_.kernel == K_REG_COULOMB) { par = ss ? prm_.kp[1] : prm_.kp[0]; pk = 2; }
+    const long long np = (long long)pairs_.size();
+#pragma omp parallel for schedule(static) if (np > 20000)
+    for (long long i = 0; i < np; i++) {
+        const FarPair &q = pairs_[i];
+        const Tree &tt = sys_[q.sys].ttree, &st = sys_[q.sys].stree;
+        const int t = q.t, s = q.s;
+        const double dx = tt.xmid[t] - st.xmid[s], dy = tt.ymid[t] - st.ymid[s],
+                     dz = tt.zmid[t] - st.zmid[s];
+        const double d = std::max(std::sqrt(dx * dx + dy * dy + dz * dz), 1e-300);
+        auto halfw = [](const Tree &T, int n) {
+            return 0.5 * std::max({T.xmax[n] - T.xmin[n], T.ymax[n] - T.ymin[n], T.zmax[n] - T.zmin[n]});
+        };
+        double hw = 0.0;
+        if (q.kind != FAR_CP) hw = std::max(hw, halfw(st, s));
+        if (q.kind != FAR_PC) hw = std::max(hw, halfw(tt, t));
+        float *o = f.data() + i * nf;
+        o[0] = (float)(tt.radius[t] / d);
+        o[1] = (float)(st.radius[s] / d);
+        o[2] = (float)(std::log2((double)std::max(1, tt.numpar[t])) / 16.0);
+        o[3] = (float)(std::log2((double)std::max(1, st.numpar[s])) / 16.0);
+        o[4] = q.kind == FAR_PC ? 1.0f : 0.0f;
+        o[5] = q.kind == FAR_CP ? 1.0f : 0.0f;
+        o[6] = (float)(pk == 1 ? par * d : (pk == 2 ? par / d : 0.0));
+        o[7] = (float)(hw / d);
+    }
+}
+
+void Plan::free_var()
+{
+    for (void *p : var_allocs_) (void)hipFree(p);
+    var_allocs_.clear();
+    dw_.clear();
+    near_pp_ = WorkCSR();
+    near_pp_tiles_.clear();
+    var_ = false;
+    var_mean_deg_ = 0;
+    var_evals_ = 0;
+    d_pp_seg_off_ = d_pp_clu_off_ = nullptr;
+    d_pp_segs_ = nullptr;
+    d_pp_tiles_ = nullptr;
+}
+
+std::string Plan::set_pair_degrees(const int *deg)
+{
+    free_var();
+    if (!deg) return "";
+    if (!st_) return "plan has no device data";
+    const long long np = (long long)pairs_.size();
+    for (long long i = 0; i < np; i++)
+        if (deg[i] < 1 || deg[i] > MAXDEG) return "pair degree outside [1, 12]";
+    const bool ss = prm_.singularity == S_SUBTRACTION;
+
+    // near field: the fixed plan's keys and particle ranges, without clusters
+    near_pp_ = near_;
+    near_pp_.clu_off.assign(near_.clu_off.size(), 0);
+    near_pp_.clu_id.clear();
+    near_pp_.pairs = 0;
+    for (size_t k = 0; k < near_pp_.key.size(); k++)
+        for (int s = near_pp_.seg_off[k]; s < near_pp_.seg_off[k + 1]; s++)
+            near_pp_.pairs += (long long)near_pp_.tcnt[k] * near_pp_.seg_cnt[s];
+    make_queue_tiles(near_pp_, tile_targets_, false, 1, near_pp_tiles_, near_pp_qlen_);
+
+    // slots per degree
+    std::vector<std::vector<int>> sslot(MAXDEG + 1), tslot(MAXDEG + 1);
+    std::vector<int> used(MAXDEG + 1, 0);
+    for (long long i = 0; i < np; i++) used[deg[i]] = 1;
+    dw_.assign(MAXDEG + 1, DegreeWork());
+    for (int p = 1; p <= MAXDEG; p++) {
+        dw_[p].degree = p; dw_[p].nl = p + 1; dw_[p].P = (p + 1) * (p + 1) * (p + 1);
+        if (used[p]) { sslot[p].assign(nsnodes_, -1); tslot[p].assign(ntnodes_, -1); }
+    }
+    // per system and degree, node tables (computed on demand)
+    std::vector<std::vector<std::vector<double>>> stab(sys_.size()), ttab(sys_.size());
+    for (size_t y = 0; y < sys_.size(); y++) {
+        stab[y].resize(MAXDEG + 1);
+        ttab[y].resize(MAXDEG + 1);
+    }
+    auto src_slot = [&](int sy, int s, int p) {
+        const System &y = sys_[sy];
+        int &sl = sslot[p][y.snode_off + s];
+        if (sl < 0) {
+            DegreeWork &w = dw_[p];
+            sl = (int)w.s_ibeg.size();
+            w.s_ibeg.push_back(y.soff + y.stree.ibeg[s]);
+            w.s_iend.push_back(y.soff + y.stree.iend[s]);
+            if (stab[sy][p].empty()) node_table(y.stree, p, stab[sy][p]);
+            const double *row = stab[sy][p].data() + (size_t)s * 3 * (p + 1);
+            w.s_tab.insert(w.s_tab.end(), row, row + 3 * (p + 1));
+        }
+        return sl;
+    };
+    // target slots and their entries (CC clusters first, then CP ranges, in pair order)
+    struct GridEntry { int slot, kind, src, cnt; };
+    std::vector<std::vector<GridEntry>> gent(MAXDEG + 1);
+    std::vector<std::vector<int>> slot_node(MAXDEG + 1), slot_sys(MAXDEG + 1);
+    auto tgt_slot = [&](int sy, int t, int p) {
+        const System &y = sys_[sy];
+        int &sl = tslot[p][y.tnode_off + t];
+        if (sl < 0) {
+            DegreeWork &w = dw_[p];
+            sl = (int)slot_node[p].size();
+            slot_node[p].push_back(t);
+            slot_sys[p].push_back(sy);
+            if (ttab[sy][p].empty()) node_table(y.ttree, p, ttab[sy][p]);
+            const double *row = ttab[sy][p].data() + (size_t)t * 3 * (p + 1);
+            w.t_tab.insert(w.t_tab.end(), row, row + 3 * (p + 1));
+        }
+        return sl;
+    };
+    // PC entries per (degree, global target node): source slots, in pair order
+    std::vector<std::vector<std::pair<int, int>>> pc_at(MAXDEG + 1);  // (global tnode, slot)
+    double dsum = 0.0;
+    var_evals_ = 0;
+    for (long long i = 0; i < np; i++) {
+        const FarPair &q = pairs_[i];
+        const int p = deg[i];
+        const System &y = sys_[q.sys];
+        const long long P = dw_[p].P;
+        dsum += p;
+        if (q.kind == FAR_PC) {
+            pc_at[p].push_back({y.tnode_off + q.t, src_slot(q.sys, q.s, p)});
+            dw_[p].evals += (long long)y.ttree.numpar[q.t] * P;
+        } else if (q.kind == FAR_CP) {
+            const int sl = tgt_slot(q.sys, q.t, p);
+            gent[p].push_back({sl, FAR_CP, y.soff + y.stree.ibeg[q.s], y.stree.numpar[q.s]});
+            dw_[p].evals += P * y.stree.numpar[q.s];
+        } else {
+            const int sl = tgt_slot(q.sys, q.t, p);
+            gent[p].push_back({sl, FAR_CC, src_slot(q.sys, q.s, p), 0});
+            dw_[p].evals += P * P;
+        }
+    }
+    var_mean_deg_ = np ? dsum / np : 0.0;
+
+    for (int p = 1; p <= MAXDEG; p++) {
+        if (!used[p]) continue;
+        DegreeWork &w = dw_[p];
+        var_evals_ += w.evals;
+        // particle targets <- source slots: keys are target leaves (batches
+        // in PC), each with the entries of itself and its ancestors
+        w.pc = WorkCSR();
+        w.pc.seg_off.push_back(0); w.pc.clu_off.push_back(0);
+        if (!pc_at[p].empty()) {
+            std::vector<std::vector<int>> at(ntnodes_);
+            for (auto &e : pc_at[p]) at[e.first].push_back(e.second);
+            for (const System &y : sys_) {
+                const Tree &tt = y.ttree;
+                auto add_key = [&](int leaf, bool walk) {
+                    size_t before = w.pc.clu_id.size();
+                    for (int a = leaf; a >= 0; a = walk ? tt.parent[a] : -1)
+                        for (int sl : at[y.tnode_off + a]) w.pc.clu_id.push_back(sl);
+                    if (w.pc.clu_id.size() == before) return;
+                    w.pc.key.push_back(y.tnode_off + leaf);
+                    w.pc.tbeg.push_back(y.toff + tt.ibeg[leaf]);
+                    w.pc.tcnt.push_back(tt.numpar[leaf]);
+                    w.pc.seg_off.push_back(0);
+                    w.pc.clu_off.push_back((int)w.pc.clu_id.size());
+                    w.pc.pairs += (long long)tt.numpar[leaf] * w.P *
+                                  (long long)(w.pc.clu_id.size() - before);
+                };
+                if (prm_.compute_type == C_PC) {
+                    for (int b = 0; b < tt.n; b++) add_key(b, false);
+                } else {
+                    for (int leaf : tt.leaves) add_key(leaf, true);
+                }
+            }
+        }
+        make_queue_tiles(w.pc, tile_targets_, false, w.P, w.pc_tiles, w.pc_qlen);
+        // target slots <- source slots (CC) and particle ranges (CP)
+        w.grid = WorkCSR();
+        w.grid.seg_off.push_back(0); w.grid.clu_off.push_back(0);
+        const int ns = (int)slot_node[p].size();
+        if (ns > 0) {
+            std::vector<std::vector<int>> clus(ns), segs(ns), segc(ns);
+            for (const GridEntry &g : gent[p]) {
+                if (g.kind == FAR_CC) {
+                    clus[g.slot].push_back(g.src);
+                } else {
+                    segs[g.slot].push_back(g.src);
+                    segc[g.slot].push_back(g.cnt);
+                }
+            }
+            for (int sl = 0; sl < ns; sl++) {
+                w.grid.key.push_back(sl);
+                w.grid.tbeg.push_back(0);
+                w.grid.tcnt.push_back(w.P);
+                for (size_t j = 0; j < segs[sl].size(); j++) {
+                    w.grid.seg_beg.push_back(segs[sl][j]);
+                    w.grid.seg_cnt.push_back(segc[sl][j]);
+                    w.grid.seg_chk.push_back(0);
+                    w.grid.pairs += (long long)w.P * segc[sl][j];
+                }
+                for (int c : clus[sl]) {
+                    w.grid.clu_id.push_back(c);
+                    w.grid.pairs += (long long)w.P * w.P;
+                }
+                w.grid.seg_off.push_back((int)w.grid.seg_beg.size());
+                w.grid.clu_off.push_back((int)w.grid.clu_id.size());
+            }
+            // downpass: each target slot's interpolant at its node's particles,
+            // one launch per tree level (nodes of a level have disjoint ranges)
+            w.down.assign(tdepth_ + 1, {});
+            for (int sl = 0; sl < ns; sl++) {
+                const System &y = sys_[slot_sys[p][sl]];
+                const int t = slot_node[p][sl];
+                const int lv = y.ttree.level[t];
+                for (int o = 0; o < y.ttree.numpar[t]; o += DOWN_TILE) {
+                    btk::Tile tl;
+                    tl.x = sl;
+                    tl.y = y.toff + y.ttree.ibeg[t] + o;
+                    tl.z = std::min(DOWN_TILE, y.ttree.numpar[t] - o);
+                    tl.w = 0;
+                    w.down[lv].push_back(tl);
+                }
+            }
+        }
+        make_queue_tiles(w.grid, GRID_TILE, true, w.P, w.grid_tiles, w.grid_qlen);
+    }
+    std::string e = validate_var();
+    if (!e.empty()) { free_var(); return "internal variable-degree work check failed: " + e; }
+
+    // upload
+    alloc_var_ = true;
+    std::string r;
+    auto fail = [&](const std::string &m) { alloc_var_ = false; free_var(); return m; };
+    {
+        std::vector<int> zeros(near_pp_.clu_off.size(), 0);
+        if (!(r = up(&d_pp_clu_off_, zeros)).empty()) return fail(r);
+        if (!(r = up(&d_pp_tiles_, near_pp_tiles_)).empty()) return fail(r);
+    }
+    auto up_csr = [&](const WorkCSR &wk, int **so, btk::Seg **sg, int **co, int **ci) -> std::string {
+        std::vector<btk::Seg> sv(wk.seg_beg.size());
+        for (size_t i = 0; i < sv.size(); i++) { sv[i].x = wk.seg_beg[i]; sv[i].y = wk.seg_cnt[i]; }
+        std::string rr;
+        if (!(rr = up(so, wk.seg_off)).empty()) return rr;
+        if (!(rr = up(sg, sv)).empty()) return rr;
+        if (!(rr = up(co, wk.clu_off)).empty()) return rr;
+        return up(ci, wk.clu_id);
+    };
+    for (int p = 1; p <= MAXDEG; p++) {
+        if (!used[p]) continue;
+        DegreeWork &w = dw_[p];
+        const size_t S = w.s_ibeg.size(), T = w.t_tab.size() / (3 * w.nl);
+        if (S > 0) {
+            std::vector<int> ident(S);
+            std::iota(ident.begin(), ident.end(), 0);
+            if (!(r = up(&w.d_s_ident, ident)).empty()) return fail(r);
+            if (!(r = up(&w.d_s_ibeg, w.s_ibeg)).empty()) return fail(r);
+            if (!(r = up(&w.d_s_iend, w.s_iend)).empty()) return fail(r);
+            if (!(r = up(&w.d_s_tab, w.s_tab)).empty()) return fail(r);
+            const size_t n = S * w.P;
+            w.d_cx = dalloc<double>(n); w.d_cy = dalloc<double>(n); w.d_cz = dalloc<double>(n);
+            w.d_cq = dalloc<double>(n);
+            if (ss) w.d_cw = dalloc<double>(n);
+            if (!w.d_cx || !w.d_cy || !w.d_cz || !w.d_cq || (ss && !w.d_cw)) return fail("hipMalloc failed (slots)");
+        }
+        if (T > 0) {
+            if (!(r = up(&w.d_t_tab, w.t_tab)).empty()) return fail(r);
+            w.d_Qt = dalloc<double>(T * w.P);
+            if (ss) w.d_Wt = dalloc<double>(T * w.P);
+            if (!w.d_Qt || (ss && !w.d_Wt)) return fail("hipMalloc failed (slots)");
+        }
+        if (!(r = up_csr(w.pc, &w.d_pc_seg_off, &w.d_pc_segs, &w.d_pc_clu_off, &w.d_pc_clu_id)).empty()) return fail(r);
+        if (!(r = up(&w.d_pc_tiles, w.pc_tiles)).empty()) return fail(r);
+        if (!(r = up_csr(w.grid, &w.d_g_seg_off, &w.d_g_segs, &w.d_g_clu_off, &w.d_g_clu_id)).empty()) return fail(r);
+        if (!(r = up(&w.d_g_tiles, w.grid_tiles)).empty()) return fail(r);
+        w.d_down.assign(w.down.size(), nullptr);
+        for (size_t lv = 0; lv < w.down.size(); lv++)
+            if (!w.down[lv].empty() && !(r = up(&w.d_down[lv], w.down[lv])).empty()) return fail(r);
+    }
+    alloc_var_ = false;
+    HIPCHK(hipStreamSynchronize(st_));
+    var_ = true;
+    return "";
+}
+
+std::string Plan::validate_var() const
+{
+    auto chk = [&](const WorkCSR &w, const std::vector<btk::Tile> &tiles, const int *qlen, bool grid,
+                   int nclu, int nkeyslots, int P, int tile_t) -> std::string {
+        const int nk = (int)w.key.size();
+        if ((int)w.seg_off.size() != nk + 1 || (int)w.clu_off.size() != nk + 1) return "csr offsets";
+        if (w.seg_off[nk] != (int)w.seg_beg.size() || w.clu_off[nk] != (int)w.clu_id.size()) return "csr ends";
+        for (int k = 0; k < nk; k++) {
+            if (w.seg_off[k] > w.seg_off[k + 1] || w.clu_off[k] > w.clu_off[k + 1]) return "csr order";
+            if (grid && (w.key[k] < 0 || w.key[k] >= nkeyslots)) return "grid key";
+        }
+        for (size_t i = 0; i < w.seg_beg.size(); i++)
+            if (w.seg_beg[i] < 0 || w.seg_cnt[i] < 0 || (long long)w.seg_beg[i] + w.seg_cnt[i] > N_)
+                return "source range";
+        for (int c : w.clu_id)
+            if (c < 0 || c >= nclu) return "source slot";
+        for (const btk::Tile &t : tiles) {
+            if (t.z == 0) continue;
+            if (t.x < 0 || t.x >= nk || t.z < 0 || t.z > tile_t || t.y < 0) return "tile";
+            if (grid) {
+                if (t.w < 0 || t.w >= nkeyslots || t.y + t.z > P) return "grid tile";
+            } else if ((long long)t.y + t.z > M_) {
+                return "target tile";
+            }
+        }
+        for (int x = 0; x < NQ; x++)
+            for (int j = 0; j < qlen[x]; j++) {
+                const size_t t = (size_t)j * NQ + x;
+                if (t >= tiles.size() || tiles[t].z <= 0) return "tile queues";
+            }
+        return "";
+    };
+    std::string e = chk(near_pp_, near_pp_tiles_, near_pp_qlen_, false, 1, 0, 1, tile_targets_);
+    if (!e.empty()) return "near " + e;
+    for (const DegreeWork &w : dw_) {
+        if (w.degree == 0 || w.pc.seg_off.empty()) continue;  // degree not used
+        const int S = (int)w.s_ibeg.size(), T = (int)(w.t_tab.size() / (3 * w.nl));
+        if ((int)w.s_iend.size() != S || (int)w.s_tab.size() != S * 3 * w.nl) return "source slots";
+        for (int i = 0; i < S; i++)
+            if (w.s_ibeg[i] < 0 || w.s_iend[i] > N_ || w.s_ibeg[i] >= w.s_iend[i]) return "source slot range";
+        if (!(e = chk(w.pc, w.pc_tiles, w.pc_qlen, false, S, 0, w.P, tile_targets_)).empty()) return "pc " + e;
+        if (!(e = chk(w.grid, w.grid_tiles, w.grid_qlen, true, S, T, w.P, GRID_TILE)).empty()) return "grid " + e;
+        for (const auto &lv : w.down)
+            for (const btk::Tile &t : lv)
+                if (t.x < 0 || t.x >= T || t.y < 0 || t.z <= 0 || t.z > DOWN_TILE || (long long)t.y + t.z > M_)
+                    return "down tile";
+    }
+    return "";
+}
+
+std::string Plan::label_pairs(int first, int count, double tol, int *labels, double *err)
+{
+    if (!st_) return "plan has no device data";
+    if (first < 0 || count < 0 || (long long)first + count > (long long)pairs_.size())
+        return "pair range out of bounds";
+    if (count == 0) return "";
+    std::vector<int> kind(count), tb(count), tc(count), sb(count), sc(count);
+    std::vector<double> tbox(6 * (size_t)count), sbox(6 * (size_t)count);
+    for (int i = 0; i < count; i++) {
+        const FarPair &q = pairs_[first + i];
+        const System &y = sys_[q.sys];
+        const Tree &tt = y.ttree, &st = y.stree;
+        kind[i] = q.kind;
+        tb[i] = y.toff + tt.ibeg[q.t]; tc[i] = tt.numpar[q.t];
+        sb[i] = y.soff + st.ibeg[q.s]; sc[i] = st.numpar[q.s];
+        double *a = &tbox[6 * (size_t)i], *b = &sbox[6 * (size_t)i];
+        a[0] = tt.xmin[q.t]; a[1] = tt.xmax[q.t]; a[2] = tt.ymin[q.t];
+        a[3] = tt.ymax[q.t]; a[4] = tt.zmin[q.t]; a[5] = tt.zmax[q.t];
+        b[0] = st.xmin[q.s]; b[1] = st.xmax[q.s]; b[2] = st.ymin[q.s];
+        b[3] = st.ymax[q.s]; b[4] = st.zmin[q.s]; b[5] = st.zmax[q.s];
+    }
+    // temporary device buffers, freed before returning
+    std::vector<void *> tmp;
+    auto cleanup = [&]() { for (void *p : tmp) (void)hipFree(p); };
+    auto put = [&](const void *h, size_t bytes) -> void * {
+        void *d = nullptr;
+        if (hipMalloc(&d, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        tmp.push_back(d);
+        if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st_) != hipSuccess) return nullptr;
+        return d;
+    };
+    btl::LabelArgs a{};
+    a.npairs = count;
+    a.kind = (const int *)put(kind.data(), count * sizeof(int));
+    a.tbeg = (const int *)put(tb.data(), count * sizeof(int));
+    a.tcnt = (const int *)put(tc.data(), count * sizeof(int));
+    a.sbeg = (const int *)put(sb.data(), count * sizeof(int));
+    a.scnt = (const int *)put(sc.data(), count * sizeof(int));
+    a.tbox = (const double *)put(tbox.data(), tbox.size() * sizeof(double));
+    a.sbox = (const double *)put(sbox.data(), sbox.size() * sizeof(double));
+    void *dl = nullptr, *de = nullptr;
+    if (hipMalloc(&dl, count * sizeof(int)) == hipSuccess) tmp.push_back(dl); else dl = nullptr;
+    if (hipMalloc(&de, (size_t)count * btl::LABEL_ND * sizeof(double)) == hipSuccess) tmp.push_back(de); else de = nullptr;
+    if (!a.kind || !a.tbeg || !a.tcnt || !a.sbeg || !a.scnt || !a.tbox || !a.sbox || !dl || !de) {
+        cleanup();
+        return "device buffers for labels failed";
+    }
+    a.tx = d_tx_; a.ty = d_ty_; a.tz = d_tz_;
+    a.sx = d_sx_; a.sy = d_sy_; a.sz = d_sz_; a.sq = d_sq_;
+    a.kernel = prm_.kernel;
+    a.p0 = prm_.kernel == K_YUKAWA ? prm_.kp[0]
+         : (prm_.kernel == K_REG_COULOMB ? (prm_.singularity == S_SUBTRACTION ? prm_.kp[1] : prm_.kp[0]) : 0.0);
+    a.tol = tol;
+    a.labels = (int *)dl;
+    a.errors = (double *)de;
+    hipError_t e = btl::launch_label(a, st_);
+    if (e == hipSuccess) e = hipMemcpyAsync(labels, dl, count * sizeof(int), hipMemcpyDeviceToHost, st_);
+    std::vector<double> eh;
+    if (e == hipSuccess && err)
+        e = hipMemcpyAsync(err, de, (size_t)count * btl::LABEL_ND * sizeof(double), hipMemcpyDeviceToHost, st_);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_);
+    cleanup();
+    if (e != hipSuccess) return std::string("label kernel: ") + hipGetErrorString(e);
+    return "";
+}
+
+std::string Plan::execute_var(bool timed)
+{
+    const Params &p = prm_;
+    const bool ss = p.singularity == S_SUBTRACTION;
+    hipEvent_t ev[6];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(ev[0], st_));
+
+    // 1. source slots: modified charges of degree p from the node's particles
+    for (const DegreeWork &w : dw_) {
+        const int S = (int)w.s_ibeg.size();
+        if (w.degree == 0 || S == 0) continue;
+        HIPCHK(btk::launch_cluster_points(S, w.nl, w.d_s_tab, w.d_cx, w.d_cy, w.d_cz, st_));
+        btk::AnterpArgs a{};
+        a.degree = w.degree; a.subtraction = ss;
+        a.ibeg = w.d_s_ibeg; a.iend = w.d_s_iend; a.tab = w.d_s_tab;
+        a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
+        a.Q = w.d_cq; a.W = w.d_cw;
+        a.mode = 0; a.nodes = w.d_s_ident; a.nblocks = S;
+        HIPCHK(btk::launch_anterp(a, st_));
+    }
+    HIPCHK(hipEventRecord(ev[1], st_));
+
+    // 2. near field (stores every target), then each degree's source slots
+    //    into the particle targets (adds)
+    btk::InteractArgs ia{};
+    ia.kernel = p.kernel; ia.singularity = p.singularity; ia.precision = p.precision;
+    ia.tpt = tpt_;
+    ia.persistent = persistent_ ? 1 : 0;
+    ia.kp = kp_;
+    ia.tx = d_tx_; ia.ty = d_ty_; ia.tz = d_tz_; ia.tq = d_tq_;
+    ia.sx = d_sx_; ia.sy = d_sy_; ia.sz = d_sz_; ia.sq = d_sq_; ia.sw = d_sw_;
+    HIPCHK(hipMemsetAsync(d_pot_, 0xFF, (size_t)M_ * sizeof(double), st_));
+    {
+        btk::InteractArgs a = ia;
+        a.P = 1; a.nl = 1;
+        a.grid_targets = 0; a.ntiles = (int)near_pp_tiles_.size(); a.tiles = d_pp_tiles_;
+        a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_pp_clu_off_; a.clu_id = d_pp_clu_off_;
+        a.out1 = d_pot_;
+        std::copy(near_pp_qlen_, near_pp_qlen_ + NQ, a.qlen);
+        a.cursor = d_cursor_;
+        HIPCHK(btk::launch_interact(a, st_));
+    }
+    for (const DegreeWork &w : dw_) {
+        if (w.degree == 0 || w.pc.key.empty()) continue;
+        btk::InteractArgs a = ia;
+        a.P = w.P; a.nl = w.nl;
+        a.cx = w.d_cx; a.cy = w.d_cy; a.cz = w.d_cz; a.cq = w.d_cq; a.cw = w.d_cw;
+        a.grid_targets = 0; a.ntiles = (int)w.pc_tiles.size(); a.tiles = w.d_pc_tiles;
+        a.seg_off = w.d_pc_seg_off; a.segs = w.d_pc_segs; a.clu_off = w.d_pc_clu_off; a.clu_id = w.d_pc_clu_id;
+        a.out1 = d_pot_;
+        a.accumulate = 1;
+        std::copy(w.pc_qlen, w.pc_qlen + NQ, a.qlen);
+        a.cursor = d_cursor_;
+        HIPCHK(btk::launch_interact(a, st_));
+    }
+    HIPCHK(hipEventRecord(ev[2], st_));
+
+    // 3. target slots: grid values from source slots (CC) and particles (CP)
+    for (const DegreeWork &w : dw_) {
+        if (w.degree == 0 || w.grid.key.empty()) continue;
+        const size_t T = w.t_tab.size() / (3 * w.nl);
+        HIPCHK(hipMemsetAsync(w.d_Qt, 0, T * w.P * sizeof(double), st_));
+        if (ss) HIPCHK(hipMemsetAsync(w.d_Wt, 0, T * w.P * sizeof(double), st_));
+        btk::InteractArgs a = ia;
+        a.P = w.P; a.nl = w.nl;
+        a.cx = w.d_cx; a.cy = w.d_cy; a.cz = w.d_cz; a.cq = w.d_cq; a.cw = w.d_cw;
+        a.grid_targets = 1; a.ntiles = (int)w.grid_tiles.size(); a.tiles = w.d_g_tiles;
+        a.seg_off = w.d_g_seg_off; a.segs = w.d_g_segs; a.clu_off = w.d_g_clu_off; a.clu_id = w.d_g_clu_id;
+        a.tnode_tab = w.d_t_tab;
+        a.out1 = w.d_Qt; a.out2 = w.d_Wt;
+        std::copy(w.grid_qlen, w.grid_qlen + NQ, a.qlen);
+        a.cursor = d_cursor_ + NQ;
+        HIPCHK(btk::launch_interact(a, st_));
+    }
+    HIPCHK(hipEventRecord(ev[3], st_));
+
+    // 4. each target slot's interpolant at its node's particles
+    for (const DegreeWork &w : dw_) {
+        if (w.degree == 0) continue;
+        for (size_t lv = 0; lv < w.down.size(); lv++) {
+            if (w.down[lv].empty()) continue;
+            btk::DownArgs d{};
+            d.degree = w.degree; d.subtraction = ss; d.tab = w.d_t_tab; d.parent = nullptr;
+            d.tx = d_tx_; d.ty = d_ty_; d.tz = d_tz_; d.tq = d_tq_; d.Q = w.d_Qt; d.W = w.d_Wt;
+            d.pot = d_pot_;
+            d.mode = 1; d.tiles = w.d_down[lv]; d.nblocks = (int)w.down[lv].size();
+            HIPCHK(btk::launch_down(d, st_));
+        }
+    }
+    HIPCHK(hipEventRecord(ev[4], st_));
+
+    // 5. subtraction correction and the caller's order
+    HIPCHK(btk::launch_finish(M_, d_pot_, d_tq_, kp_.ss_param, d_torder_, d_out_, st_));
+    HIPCHK(hipEventRecord(ev[5], st_));
+    HIPCHK(hipEventSynchronize(ev[5]));
+    if (timed) {
+        float ms[5];
+        for (int i = 0; i < 5; i++) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        tm_.gpu_upward = ms[0] * 1e-3; tm_.gpu_near = ms[1] * 1e-3; tm_.gpu_far = ms[2] * 1e-3;
+        tm_.gpu_down = ms[3] * 1e-3; tm_.gpu_finish = ms[4] * 1e-3;
+        tm_.gpu_total = (ms[0] + ms[1] + ms[2] + ms[3] + ms[4]) * 1e-3;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return "";
+}
+
+}  // namespace bt

@@ -123,10 +123,24 @@ def lib():
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
                   "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision",
                   "BaryTree_SolveBatched", "BaryTreePlan_CreateBatched", "BaryTree_Autotune",
-                  "BaryTree_AnalyzeBatched"):
+                  "BaryTree_AnalyzeBatched", "BaryTreePlan_NumPairs", "BaryTreePlan_PairInfo",
+                  "BaryTreePlan_PairFeatures", "BaryTreePlan_LabelPairs",
+                  "BaryTreePlan_SetPairDegrees", "BaryTreePlan_UseDegreeModel",
+                  "BaryTree_LabelPairs", "BaryTreeModel_Create", "BaryTreeModel_Load",
+                  "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
+                  "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
+                  "BaryTree_SolveLearned"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
+        l.BaryTreeModel_Destroy.restype = None
+        l.BaryTreeModel_Create.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong]
+        l.BaryTreeModel_Train.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_float, ctypes.c_ulonglong, ctypes.c_void_p]
+        l.BaryTreeModel_Predict.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+        l.BaryTreePlan_UseDegreeModel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
         _lib = l
     return _lib
 
@@ -368,6 +382,48 @@ class Plan:
             raise RuntimeError(f"BaryTreePlan_DirectSample failed: {last_error()}")
         return out
 
+    # ---- far-field pairs and their degrees
+    def num_pairs(self):
+        return int(lib().BaryTreePlan_NumPairs(self._h))
+
+    def pair_info(self):
+        """(kind, system) arrays: kind 1 PC, 2 CP, 3 CC."""
+        n = self.num_pairs()
+        kind = np.zeros(n, dtype=np.int32)
+        system = np.zeros(n, dtype=np.int32)
+        lib().BaryTreePlan_PairInfo(self._h, _iptr(kind), _iptr(system))
+        return kind, system
+
+    def pair_features(self):
+        n = self.num_pairs()
+        f = np.zeros((n, PAIR_FEATURES), dtype=np.float32)
+        if n and lib().BaryTreePlan_PairFeatures(self._h, f.ctypes.data_as(_fp)) != 0:
+            raise RuntimeError(f"BaryTreePlan_PairFeatures failed: {last_error()}")
+        return f
+
+    def label_pairs(self, tol, first=0, count=None, with_errors=False):
+        n = self.num_pairs()
+        count = n - first if count is None else count
+        labels = np.zeros(count, dtype=np.int32)
+        errors = np.zeros((count, LABEL_DEGREES)) if with_errors else None
+        rc = lib().BaryTreePlan_LabelPairs(self._h, ctypes.c_int(first), ctypes.c_int(count),
+                                           ctypes.c_double(tol), _iptr(labels), _ptr(errors))
+        if rc != 0:
+            raise RuntimeError(f"BaryTreePlan_LabelPairs failed: {last_error()}")
+        return (labels, errors) if with_errors else labels
+
+    def set_pair_degrees(self, degrees):
+        """One degree per far-field pair; None restores the fixed degree."""
+        d = None if degrees is None else np.ascontiguousarray(degrees, dtype=np.int32)
+        if d is not None and len(d) != self.num_pairs():
+            raise ValueError("need one degree per far-field pair")
+        if lib().BaryTreePlan_SetPairDegrees(self._h, _iptr(d)) != 0:
+            raise RuntimeError(f"BaryTreePlan_SetPairDegrees failed: {last_error()}")
+
+    def use_model(self, model, delta=0.5):
+        if lib().BaryTreePlan_UseDegreeModel(self._h, model._h, ctypes.c_float(delta)) != 0:
+            raise RuntimeError(f"BaryTreePlan_UseDegreeModel failed: {last_error()}")
+
     def close(self):
         if self._h:
             lib().BaryTreePlan_Destroy(self._h)
@@ -384,6 +440,129 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+PAIR_FEATURES = 8
+LABEL_DEGREES = 11
+MODEL_PARAMS = 9546
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def label_pairs(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, targets, sources, kernel, kernel_param,
+                tol):
+    """BaryTree_LabelPairs on explicitly given pairs: returns (labels, errors)."""
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (kind, tbeg, tcnt, sbeg, scnt)]
+    tbox, sbox = (np.ascontiguousarray(a, dtype=np.float64) for a in (tbox, sbox))
+    tx, ty, tz = map(_arr, targets[:3])
+    sx, sy, sz, sq = map(_arr, sources[:4])
+    n = len(ints[0])
+    labels = np.zeros(n, dtype=np.int32)
+    errors = np.zeros((n, LABEL_DEGREES))
+    rc = lib().BaryTree_LabelPairs(
+        ctypes.c_int(n), *[_iptr(a) for a in ints], _ptr(tbox), _ptr(sbox), ctypes.c_int(len(tx)),
+        _ptr(tx), _ptr(ty), _ptr(tz), ctypes.c_int(len(sx)), _ptr(sx), _ptr(sy), _ptr(sz),
+        _ptr(sq), ctypes.c_int(int(kernel)), ctypes.c_double(kernel_param), ctypes.c_double(tol),
+        _iptr(labels), _ptr(errors))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_LabelPairs failed: {last_error()}")
+    return labels, errors
+
+
+class DegreeModel:
+    """BaryTreeDegreeModel: the MLP that picks far-field degrees."""
+
+    def __init__(self, seed=0, path=None):
+        self._h = ctypes.c_void_p()
+        if path is not None:
+            rc = lib().BaryTreeModel_Load(ctypes.byref(self._h), str(path).encode())
+        else:
+            rc = lib().BaryTreeModel_Create(ctypes.byref(self._h), ctypes.c_ulonglong(seed))
+        if rc != 0:
+            self._h = None
+            raise RuntimeError(f"degree model: {last_error()}")
+
+    def save(self, path):
+        if lib().BaryTreeModel_Save(self._h, str(path).encode()) != 0:
+            raise RuntimeError(f"BaryTreeModel_Save failed: {last_error()}")
+
+    def get_params(self):
+        p = np.zeros(MODEL_PARAMS, dtype=np.float32)
+        m = np.zeros(PAIR_FEATURES, dtype=np.float32)
+        sc = np.zeros(PAIR_FEATURES, dtype=np.float32)
+        lib().BaryTreeModel_GetParams(self._h, p.ctypes.data_as(_fp), m.ctypes.data_as(_fp),
+                                      sc.ctypes.data_as(_fp))
+        return p, m, sc
+
+    def set_params(self, params=None, mean=None, scale=None):
+        conv = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                for a in (params, mean, scale)]
+        lib().BaryTreeModel_SetParams(self._h, *[None if a is None else a.ctypes.data_as(_fp)
+                                                 for a in conv])
+
+    def gradient(self, features, labels):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        g = np.zeros(MODEL_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        rc = lib().BaryTreeModel_Gradient(self._h, ctypes.c_int(len(y)), x.ctypes.data_as(_fp),
+                                          _iptr(y), g.ctypes.data_as(_fp), ctypes.byref(loss))
+        if rc != 0:
+            raise RuntimeError(f"BaryTreeModel_Gradient failed: {last_error()}")
+        return loss.value, g
+
+    def train(self, features, labels, epochs=20, batch=4096, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        el = np.zeros(epochs, dtype=np.float32)
+        rc = lib().BaryTreeModel_Train(self._h, len(y), x.ctypes.data, y.ctypes.data, epochs,
+                                       batch, lr, seed, el.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"BaryTreeModel_Train failed: {last_error()}")
+        return el
+
+    def predict(self, features, delta=0.5, with_logits=False):
+        x = np.ascontiguousarray(features, dtype=np.float32)
+        n = x.shape[0]
+        d = np.zeros(n, dtype=np.int32)
+        z = np.zeros((n, 10), dtype=np.float32) if with_logits else None
+        rc = lib().BaryTreeModel_Predict(self._h, n, x.ctypes.data, delta, d.ctypes.data,
+                                         None if z is None else z.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"BaryTreeModel_Predict failed: {last_error()}")
+        return (d, z) if with_logits else d
+
+    def close(self):
+        if self._h:
+            lib().BaryTreeModel_Destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_learned(targets, sources, kernel, kernel_params, singularity, approximation,
+                  compute_type, theta, degree, max_source_leaf, max_target_leaf, model=None,
+                  delta=0.5, size_check=1.0, beta=-1.0, precision=PRECISION_FP64, verbosity=0):
+    """BaryTree_SolveLearned: far-field degrees from `model` (None: fixed)."""
+    tx, ty, tz, tq = map(_arr, targets)
+    sx, sy, sz, sq, sw = map(_arr, sources)
+    kp = _kp(kernel_params)
+    out = np.zeros(len(tx))
+    rc = lib().BaryTree_SolveLearned(
+        ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(tq),
+        _ptr(sx), _ptr(sy), _ptr(sz), _ptr(sq), _ptr(sw), _ptr(out), ctypes.c_int(int(kernel)),
+        ctypes.c_int(len(kp)), _ptr(kp), ctypes.c_int(int(singularity)),
+        ctypes.c_int(int(approximation)), ctypes.c_int(int(compute_type)),
+        ctypes.c_double(theta), ctypes.c_int(degree), ctypes.c_int(max_source_leaf),
+        ctypes.c_int(max_target_leaf), ctypes.c_double(size_check), ctypes.c_double(beta),
+        ctypes.c_int(verbosity), ctypes.c_int(precision),
+        None if model is None else model._h, ctypes.c_float(delta))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_SolveLearned failed: {last_error()}")
+    return out
 
 
 def callTreedriver(numTargets, numSources,

@@ -94,6 +94,10 @@ typedef struct BaryTreeStats {
     int num_systems;         /* independent systems in the plan (1 unless batched) */
     int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets */
     int persistent;          /* 1: persistent interaction launches */
+    long long num_far_pairs; /* entries of the approximation lists */
+    int variable_degree;     /* 1: far-field pairs carry their own degree */
+    double mean_pair_degree; /* mean degree over far-field pairs (variable degree) */
+    long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
@@ -214,6 +218,100 @@ int BaryTree_AnalyzeBatched(int numSystems, const int *targetOffsets, const int 
                             int interpOrder, const int *maxPerSourceLeaf,
                             const int *maxPerTargetLeaf, const BaryTreeSchedule *schedule,
                             double sizeCheck, double beta, BaryTreeStats *stats);
+
+/* ---- Learned degree selection ---------------------------------------
+ *
+ * Far-field pairs are the entries of the approximation lists: PC (a source
+ * cluster's charges act on target particles), CP (source particles act on a
+ * target grid) and CC (a source cluster acts on a target grid).  Each pair
+ * may carry its own interpolation degree: a source node used at degree p
+ * gets modified charges of degree p from its particles, a target node used
+ * at degree p gets a grid of degree p interpolated to its particles.  A
+ * degree model (an MLP over pair features) picks the degrees; without one,
+ * every pair uses the plan's degree (the reference's method).
+ *
+ * Pair order: per system, PC: target batches in order, each batch's
+ * approximation list; CP: source batches in order, each batch's list; CC:
+ * target nodes in order, each node's CC, then PC, then CP entries.
+ * Pair kinds: 1 PC, 2 CP, 3 CC. */
+#define BARYTREE_PAIR_FEATURES 8
+#define BARYTREE_LABEL_DEGREES 11  /* degrees 2..12 */
+#define BARYTREE_MODEL_PARAMS 9546 /* 8 -> 64 -> 64 -> 64 -> 10 */
+
+int BaryTreePlan_NumPairs(const BaryTreePlan *plan);
+/* kind (1 PC, 2 CP, 3 CC) and system of each pair (either may be NULL) */
+int BaryTreePlan_PairInfo(const BaryTreePlan *plan, int *kind, int *system);
+/* BARYTREE_PAIR_FEATURES floats per pair: r_t/d, r_s/d, log2(n_t)/16,
+ * log2(n_s)/16, [PC], [CP], kernel parameter at scale d (Yukawa kappa*d,
+ * regularized Coulomb epsilon/d, Coulomb 0), largest half-width of the
+ * interpolated box(es)/d; d = distance of the node centres. */
+int BaryTreePlan_PairFeatures(const BaryTreePlan *plan, float *features);
+/* Labels of pairs [first, first+count): the least degree in 2..12 whose pair
+ * error is <= tol, 13 if none.  Pair error at degree p: relative L2 error,
+ * over up to 32 stride-sampled targets of the target node, of the degree-p
+ * approximation of the potential of up to 128 stride-sampled sources of the
+ * source node (SK kernel, FP64).  errors (optional): 11 per pair, degrees
+ * 2..12, NaN beyond the label. */
+int BaryTreePlan_LabelPairs(BaryTreePlan *plan, int first, int count, double tol, int *labels,
+                            double *errors);
+/* One degree (1..12) per pair for later Execute calls; NULL restores the
+ * plan's fixed degree. */
+int BaryTreePlan_SetPairDegrees(BaryTreePlan *plan, const int *degrees);
+
+/* The same labels for explicitly given pairs: pair i has targets
+ * [targetBegin[i], +targetCount[i]) and sources [sourceBegin[i],
+ * +sourceCount[i]) of the arrays, and node boxes {xmin,xmax,ymin,ymax,zmin,
+ * zmax} (6 doubles per pair).  kernel 1/2/3 with kernelParam kappa or
+ * epsilon. */
+int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
+                        const int *targetCount, const int *sourceBegin, const int *sourceCount,
+                        const double *targetBox, const double *sourceBox, int numTargets,
+                        const double *targetX, const double *targetY, const double *targetZ,
+                        int numSources, const double *sourceX, const double *sourceY,
+                        const double *sourceZ, const double *sourceValue, int kernel,
+                        double kernelParam, double tol, int *labels, double *errors);
+
+/* The degree model: MLP 8 -> 64 -> 64 -> 64 -> 10 (ReLU, FP32).  Output k
+ * is the logit of "the pair needs a degree above 2+k".  Training (Adam on
+ * the batch-mean binary cross-entropy of the 10 outputs) and inference run
+ * in HIP kernels on the current GPU. */
+typedef struct BaryTreeDegreeModel BaryTreeDegreeModel;
+int BaryTreeModel_Create(BaryTreeDegreeModel **model, unsigned long long seed);
+int BaryTreeModel_Load(BaryTreeDegreeModel **model, const char *path);
+int BaryTreeModel_Save(const BaryTreeDegreeModel *model, const char *path);
+/* parameters (BARYTREE_MODEL_PARAMS floats: W1[64][8] b1 W2[64][64] b2
+ * W3[64][64] b3 W4[10][64] b4) and feature normalisation (x - mean)*scale */
+int BaryTreeModel_GetParams(const BaryTreeDegreeModel *model, float *params, float *featureMean,
+                            float *featureScale);
+int BaryTreeModel_SetParams(BaryTreeDegreeModel *model, const float *params,
+                            const float *featureMean, const float *featureScale);
+/* batch-mean loss and gradient over n samples (one batch) */
+int BaryTreeModel_Gradient(const BaryTreeDegreeModel *model, int n, const float *features,
+                           const int *labels, float *grad, double *loss);
+/* epochs of shuffled mini-batches; sets the feature normalisation from the
+ * data if the model has none; epochLoss (optional): mean loss per epoch */
+int BaryTreeModel_Train(BaryTreeDegreeModel *model, int n, const float *features,
+                        const int *labels, int epochs, int batch, float learningRate,
+                        unsigned long long seed, float *epochLoss);
+/* degree = 3+k for the largest k with sigmoid(logit_k) >= delta, else 2;
+ * logits (optional): 10 per sample */
+int BaryTreeModel_Predict(const BaryTreeDegreeModel *model, int n, const float *features,
+                          float delta, int *degrees, float *logits);
+void BaryTreeModel_Destroy(BaryTreeDegreeModel *model);
+/* features -> model -> SetPairDegrees */
+int BaryTreePlan_UseDegreeModel(BaryTreePlan *plan, const BaryTreeDegreeModel *model, float delta);
+
+/* BaryTree_Solve with the far-field degrees picked by a model (NULL: the
+ * fixed degree interpOrder, i.e. BaryTree_Solve). */
+int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
+                          const double *targetY, const double *targetZ, const double *targetValue,
+                          const double *sourceX, const double *sourceY, const double *sourceZ,
+                          const double *sourceValue, const double *sourceWeight,
+                          double *outputArray, int kernel, int numKernelParams,
+                          const double *kernelParams, int singularity, int approximation,
+                          int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                          int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                          int precision, const BaryTreeDegreeModel *model, float delta);
 
 /* Autotuner: picks the leaf size of every size class present in the batch
  * (candidates P/2, P, 2P, 4P, 8P with P = (interpOrder+1)^3), then the tile

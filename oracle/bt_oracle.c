@@ -1106,6 +1106,171 @@ done0:
     return rc;
 }
 
+/* ------------------------------------------------------------------ */
+/* Far-field pairs at their own degree (the product's variable-degree   */
+/* mode).  A pair's approximation is formed for that pair alone: source */
+/* charges of degree p from the source node's particles (anterp, as for */
+/* a leaf), a target grid of degree p for the target node (grid values  */
+/* by the CP/CC kernels, then the interpolant at the target node's      */
+/* particles).  Pair order: PC per target batch, CP per source batch,  */
+/* CC per target node its CC, then PC, then CP entries.                 */
+/* ------------------------------------------------------------------ */
+
+static int src_cluster_deg(const Prm *q, const Tree *src, int s, const Parts *S, const double *qw, Clus *c)
+{
+    int sub = q->singularity == SNG_SUB;
+    int e = clus_alloc(c, 1, q->P, sub);
+    if (e) return e;
+    int b = src->ibeg[s] - 1, m = src->iend[s] - src->ibeg[s] + 1;
+    return anterp(q->degree, src->xmin[s], src->xmax[s], src->ymin[s], src->ymax[s], src->zmin[s],
+                  src->zmax[s], m, S->x + b, S->y + b, S->z + b, sub ? qw + b : S->q + b,
+                  sub ? S->w + b : NULL, c->x, c->y, c->z, c->q, sub ? c->w : NULL);
+}
+
+static int tgt_grid_deg(const Prm *q, const Tree *tgt, int t, Clus *g)
+{
+    int e = clus_alloc(g, 1, q->P, q->singularity == SNG_SUB);
+    if (e) return e;
+    int n = q->degree, nl = n + 1;
+    double nodeX[32], nodeY[32], nodeZ[32];
+    cheb_nodes(n, tgt->xmin[t], tgt->xmax[t], nodeX);
+    cheb_nodes(n, tgt->ymin[t], tgt->ymax[t], nodeY);
+    cheb_nodes(n, tgt->zmin[t], tgt->zmax[t], nodeZ);
+    for (int j = 0; j < q->P; j++) {
+        int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
+        g->x[j] = nodeX[k1]; g->y[j] = nodeY[k2]; g->z[j] = nodeZ[k3];
+    }
+    return 0;
+}
+
+static int far_pair(const Prm *prm, int kind, int p, const Tree *src, int s, const Parts *S,
+                    const double *qw, const Tree *tgt, int t, const Parts *T, double *pot)
+{
+    Prm q = *prm;
+    q.degree = p;
+    q.P = (p + 1) * (p + 1) * (p + 1);
+    int sub = q.singularity == SNG_SUB, e = 0;
+    int ts = tgt->ibeg[t] - 1, nt = tgt->iend[t] - tgt->ibeg[t] + 1;
+    Clus c, g;
+    memset(&c, 0, sizeof(c)); memset(&g, 0, sizeof(g));
+    if (kind != 2 && (e = src_cluster_deg(&q, src, s, S, qw, &c))) goto out;
+    if (kind == 1) {
+        k_pc(&q, nt, q.P, ts, 0, T, &c, pot);
+        goto out;
+    }
+    if ((e = tgt_grid_deg(&q, tgt, t, &g))) goto out;
+    if (kind == 2) k_cp(&q, src->iend[s] - src->ibeg[s] + 1, q.P, src->ibeg[s] - 1, 0, S, &g);
+    else k_cc(&q, q.P, 0, 0, &c, &g);
+    interp_eval(p, tgt->xmin[t], tgt->xmax[t], tgt->ymin[t], tgt->ymax[t], tgt->zmin[t], tgt->zmax[t],
+                g.q, sub ? g.w : NULL, nt, T->x + ts, T->y + ts, T->z + ts, sub ? T->q + ts : NULL,
+                pot + ts, NULL);
+out:
+    clus_free(&c);
+    clus_free(&g);
+    return e;
+}
+
+int orc_treecode_vardeg(int M, int N,
+                        const double *tx, const double *ty, const double *tz, const double *tq,
+                        const double *sx, const double *sy, const double *sz, const double *sq,
+                        const double *sw, double *out,
+                        int kernel, int nkp, const double *kp,
+                        int singularity, int approximation, int compute_type,
+                        double theta, int degree, int msl, int mtl,
+                        double size_check, double beta, int npairs, const int *pair_degree)
+{
+    memset(&g_stats, 0, sizeof(g_stats));
+    if (M <= 0 || N <= 0 || !out || !tx || !ty || !tz || !sx || !sy || !sz || !sq) return ORC_ERR_ARGS;
+    memset(out, 0, (size_t)M * sizeof(double));
+    Prm prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.kernel = kernel; prm.singularity = singularity; prm.approximation = approximation;
+    prm.compute_type = compute_type;
+    for (int i = 0; i < nkp && i < 4; i++) prm.kp[i] = kp[i];
+    orc_params_resolve(compute_type, approximation, theta, degree, msl, mtl, size_check, beta,
+                       &prm.theta, &prm.degree, &prm.size_check_factor, &prm.msl, &prm.mtl);
+    prm.P = (prm.degree + 1) * (prm.degree + 1) * (prm.degree + 1);
+    if (!supported(&prm)) return ORC_ERR_UNSUPPORTED;
+    if (singularity == SNG_SUB && (!sw || !tq)) return ORC_ERR_ARGS;
+    int sub = singularity == SNG_SUB;
+
+    int rc = 0;
+    Parts S, T;
+    memset(&S, 0, sizeof(S)); memset(&T, 0, sizeof(T));
+    if ((rc = parts_copy(&S, N, sx, sy, sz, sq, sw, 1))) goto done0;
+    if ((rc = parts_copy(&T, M, tx, ty, tz, tq, NULL, 0))) goto done0;
+    double *pot = calloc((size_t)M, sizeof(double));
+    double *qw = calloc((size_t)N, sizeof(double));
+    if (!pot || !qw) { free(pot); free(qw); rc = ORC_ERR_NOMEM; goto done0; }
+
+    Tree src, tgt;
+    Lists L;
+    memset(&src, 0, sizeof(src)); memset(&tgt, 0, sizeof(tgt)); memset(&L, 0, sizeof(L));
+    if (compute_type == CT_PC) {
+        if ((rc = tree_construct(&src, &S, prm.msl, 0))) goto done;
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 1))) goto done;
+        if ((rc = lists_make(&L, &src, &tgt, &prm))) goto done;
+    } else if (compute_type == CT_CP) {
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 0))) goto done;
+        if ((rc = tree_construct(&src, &S, prm.msl, 1))) goto done;
+        if ((rc = lists_make(&L, &tgt, &src, &prm))) goto done;
+    } else {
+        if ((rc = tree_construct(&src, &S, prm.msl, 0))) goto done;
+        if ((rc = tree_construct(&tgt, &T, prm.mtl, 0))) goto done;
+        if ((rc = lists_make(&L, &src, &tgt, &prm))) goto done;
+    }
+    for (int i = 0; i < N; i++) qw[i] = sub ? S.q[i] * S.w[i] : S.q[i];  /* tree order */
+    {
+        long long count = 0;
+        for (int i = 0; i < L.n; i++)
+            count += L.ap[i].n + (compute_type == CT_CC ? L.spc[i].n + L.tcp[i].n : 0);
+        if (count != npairs) { rc = ORC_ERR_ARGS; goto done; }
+    }
+    int k = 0;
+    for (int i = 0; i < L.n && !rc; i++) {
+        if (compute_type == CT_PC) {
+            int bs = tgt.ibeg[i] - 1, nt = tgt.iend[i] - tgt.ibeg[i] + 1;
+            for (int j = 0; j < L.ap[i].n && !rc; j++)
+                rc = far_pair(&prm, 1, pair_degree[k++], &src, L.ap[i].v[j], &S, qw, &tgt, i, &T, pot);
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, bs, src.ibeg[nd] - 1, &T, &S, pot);
+            }
+        } else if (compute_type == CT_CP) {
+            int bs = src.ibeg[i] - 1, ns = src.iend[i] - src.ibeg[i] + 1;
+            for (int j = 0; j < L.ap[i].n && !rc; j++)
+                rc = far_pair(&prm, 2, pair_degree[k++], &src, i, &S, qw, &tgt, L.ap[i].v[j], &T, pot);
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, tgt.iend[nd] - tgt.ibeg[nd] + 1, ns, tgt.ibeg[nd] - 1, bs, &T, &S, pot);
+            }
+        } else {
+            int ts = tgt.ibeg[i] - 1, nt = tgt.iend[i] - tgt.ibeg[i] + 1;
+            for (int j = 0; j < L.ap[i].n && !rc; j++)
+                rc = far_pair(&prm, 3, pair_degree[k++], &src, L.ap[i].v[j], &S, qw, &tgt, i, &T, pot);
+            for (int j = 0; j < L.spc[i].n && !rc; j++)
+                rc = far_pair(&prm, 1, pair_degree[k++], &src, L.spc[i].v[j], &S, qw, &tgt, i, &T, pot);
+            for (int j = 0; j < L.tcp[i].n && !rc; j++)
+                rc = far_pair(&prm, 2, pair_degree[k++], &src, L.tcp[i].v[j], &S, qw, &tgt, i, &T, pot);
+            for (int j = 0; j < L.pp[i].n; j++) {
+                int nd = L.pp[i].v[j];
+                k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, ts, src.ibeg[nd] - 1, &T, &S, pot);
+            }
+        }
+    }
+    if (!rc) {
+        ss_correction(&prm, &T, pot);
+        for (int i = 0; i < M; i++) out[T.order[i] - 1] = pot[i];
+    }
+done:
+    lists_free(&L);
+    tree_free(&src); tree_free(&tgt);
+    free(pot); free(qw);
+done0:
+    parts_free(&S); parts_free(&T);
+    return rc;
+}
+
 int orc_direct(int M, int N,
                const double *tx, const double *ty, const double *tz, const double *tq,
                const double *sx, const double *sy, const double *sz, const double *sq,

@@ -89,6 +89,21 @@ int orc_source_clusters(int n, const double *x, const double *y, const double *z
                         int singularity, double *cx, double *cy, double *cz,
                         double *q_out, double *w_out);
 
+/* Variable-degree treecode: as orc_treecode, but far-field pair k (in the
+ * order PC per target batch, CP per source batch, CC per target node its
+ * CC, PC, then CP entries) is approximated at degree pair_degree[k], its
+ * charges formed from the source node's particles and its grid values
+ * interpolated straight to the target node's particles.  npairs must equal
+ * the number of far-field pairs (else ORC_ERR_ARGS). */
+int orc_treecode_vardeg(int M, int N,
+                        const double *tx, const double *ty, const double *tz, const double *tq,
+                        const double *sx, const double *sy, const double *sz, const double *sq,
+                        const double *sw, double *out,
+                        int kernel, int nkp, const double *kp,
+                        int singularity, int approximation, int compute_type,
+                        double theta, int degree, int msl, int mtl,
+                        double size_check, double beta, int npairs, const int *pair_degree);
+
 #ifdef __cplusplus
 }
 #endif

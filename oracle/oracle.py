@@ -152,3 +152,82 @@ def source_clusters(x, y, z, q, w, degree, max_leaf, singularity):
                                 singularity, *[_p(b) for b in bufs])
     assert nn2 == nn
     return [b.reshape(nn, P) for b in bufs]
+
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def label_pairs(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, tx, ty, tz, sx, sy, sz, sq, kernel, p0,
+                tol):
+    """Pair labels and per-degree errors (degrees 2..12), CPU FP64."""
+    l = lib()
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (kind, tbeg, tcnt, sbeg, scnt)]
+    dbl = [np.ascontiguousarray(a, dtype=np.float64) for a in (tbox, sbox, tx, ty, tz, sx, sy, sz, sq)]
+    n = len(ints[0])
+    labels = np.zeros(n, dtype=np.int32)
+    errors = np.zeros((n, 11))
+    l.orc_label_pairs.restype = ctypes.c_int
+    rc = l.orc_label_pairs(ctypes.c_int(n), *[a.ctypes.data_as(_ip) for a in ints],
+                           *[a.ctypes.data_as(_dp) for a in dbl], ctypes.c_int(int(kernel)),
+                           ctypes.c_double(p0), ctypes.c_double(tol), labels.ctypes.data_as(_ip),
+                           errors.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError("oracle label_pairs failed")
+    return labels, errors
+
+
+def mlp_num_params():
+    l = lib()
+    l.orc_mlp_num_params.restype = ctypes.c_int
+    return l.orc_mlp_num_params()
+
+
+def mlp_logits(params, mean, scale, x):
+    params, mean, scale = (np.ascontiguousarray(a, dtype=np.float64) for a in (params, mean, scale))
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[0]
+    out = np.zeros((n, 10))
+    lib().orc_mlp_logits(params.ctypes.data_as(_dp), mean.ctypes.data_as(_dp),
+                         scale.ctypes.data_as(_dp), ctypes.c_int(n), x.ctypes.data_as(_fp),
+                         out.ctypes.data_as(_dp))
+    return out
+
+
+def mlp_loss_grad(params, mean, scale, x, labels, want_grad=True):
+    """Batch-mean loss (and gradient) of the degree model in FP64."""
+    params, mean, scale = (np.ascontiguousarray(a, dtype=np.float64) for a in (params, mean, scale))
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    labels = np.ascontiguousarray(labels, dtype=np.int32)
+    grad = np.zeros(len(params)) if want_grad else None
+    l = lib()
+    l.orc_mlp_loss_grad.restype = ctypes.c_double
+    loss = l.orc_mlp_loss_grad(params.ctypes.data_as(_dp), mean.ctypes.data_as(_dp),
+                               scale.ctypes.data_as(_dp), ctypes.c_int(x.shape[0]),
+                               x.ctypes.data_as(_fp), labels.ctypes.data_as(_ip),
+                               None if grad is None else grad.ctypes.data_as(_dp))
+    return loss, grad
+
+
+def treecode_vardeg(tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, kernel_params, singularity,
+                    approximation, compute_type, theta, degree, max_source_leaf, max_target_leaf,
+                    pair_degree, size_check=1.0, beta=-1.0):
+    """Treecode with a degree per far-field pair (CPU FP64)."""
+    tx, ty, tz, tq, sx, sy, sz, sq, sw = map(_c, (tx, ty, tz, tq, sx, sy, sz, sq, sw))
+    kp = np.ascontiguousarray(np.atleast_1d(np.asarray(kernel_params, dtype=np.float64)))
+    deg = np.ascontiguousarray(pair_degree, dtype=np.int32)
+    out = np.zeros(len(tx))
+    l = lib()
+    l.orc_treecode_vardeg.restype = ctypes.c_int
+    rc = l.orc_treecode_vardeg(ctypes.c_int(len(tx)), ctypes.c_int(len(sx)),
+                               _p(tx), _p(ty), _p(tz), _p(tq), _p(sx), _p(sy), _p(sz), _p(sq),
+                               _p(sw), _p(out), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)),
+                               _p(kp), ctypes.c_int(int(singularity)),
+                               ctypes.c_int(int(approximation)), ctypes.c_int(int(compute_type)),
+                               ctypes.c_double(theta), ctypes.c_int(degree),
+                               ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+                               ctypes.c_double(size_check), ctypes.c_double(beta),
+                               ctypes.c_int(len(deg)), deg.ctypes.data_as(_ip))
+    if rc != 0:
+        raise RuntimeError(f"oracle treecode_vardeg returned {rc}")
+    return out
