@@ -897,4 +897,40 @@ int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
     return 0;
 }
 
+
+int BaryTree_AnalyzeDegrees(int numTargets, int numSources, const double *targetX,
+                            const double *targetY, const double *targetZ, const double *sourceX,
+                            const double *sourceY, const double *sourceZ, int kernel,
+                            int numKernelParams, const double *kernelParams, int singularity,
+                            int approximation, int compute_type, double theta, int interpOrder,
+                            int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck,
+                            double beta, int numPairs, const int *degrees, float *features,
+                            BaryTreeStats *stats)
+{
+    g_last_error.clear();
+    if (!stats) return fail("stats is NULL");
+    bt::Params prm;
+    bt::params_setup(prm, kernel, numKernelParams, kernelParams, approximation, singularity,
+                     compute_type, theta, interpOrder, maxPerSourceLeaf, maxPerTargetLeaf, sizeCheck,
+                     beta, 0);
+    bt::Plan plan;
+    std::vector<double> zs(numSources > 0 ? numSources : 0, 0.0);
+    std::string e = plan.create(prm, numTargets, numSources, targetX, targetY, targetZ, nullptr,
+                                sourceX, sourceY, sourceZ, zs.data(), nullptr, true);
+    if (!e.empty()) return fail(e);
+    const int n = (int)plan.far_pairs_list().size();
+    if (features) {
+        std::vector<float> f;
+        plan.pair_features(f);
+        std::memcpy(features, f.data(), f.size() * sizeof(float));
+    }
+    if (degrees) {
+        if (numPairs != n) return fail("numPairs does not match the plan's far-field pairs");
+        e = plan.set_pair_degrees(degrees);
+        if (!e.empty()) return fail(e);
+    }
+    fill_stats(plan, stats);
+    return 0;
+}
+
 }  // extern "C"

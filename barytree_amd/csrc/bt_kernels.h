@@ -16,6 +16,10 @@ struct alignas(8) Seg { int x, y; };
 // one per XCD of an MI355X.
 constexpr int TILE_QUEUES = 8;
 
+constexpr int MAX_DEGREE = 12;
+constexpr int DEG_SHIFT = 24;               // (degree << DEG_SHIFT) | slot
+constexpr int SLOT_MASK = (1 << DEG_SHIFT) - 1;
+
 // Kernel parameters, pre-derived on the host.
 struct KParams {
     double p0, p1;      // kernel_params[0], kernel_params[1]
@@ -46,8 +50,17 @@ struct InteractArgs {
     const double *sx, *sy, *sz, *sq, *sw;
     const double *cx, *cy, *cz, *cq, *cw;  // source cluster points and charges
     KParams kp;
-    double *out1, *out2;     // potential (particles) or target-cluster Q and W
+    double *out1, *out2;     // potential (particles) or target-grid rows of Q and W
     int accumulate;          // particle targets: 1 adds to out1, 0 stores
+    // grid targets: the output row of key k starts at out_row[k] (grid point
+    // j of a tile of key k goes to out1[out_row[k] + j]); null: row = node*P
+    const long long *out_row;
+    // several degrees in one launch: a cluster id or a tile's node field is
+    // (degree << DEG_SHIFT) | slot, and the arrays of degree d are d*[d]
+    int multi;
+    const double *dcx[MAX_DEGREE + 1], *dcy[MAX_DEGREE + 1], *dcz[MAX_DEGREE + 1];
+    const double *dcq[MAX_DEGREE + 1], *dcw[MAX_DEGREE + 1];
+    const double *dtab[MAX_DEGREE + 1];  // target slot node tables
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
@@ -57,8 +70,10 @@ hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double
                                  double *cz, hipStream_t st);
 
 // Upward pass.  mode 0: one block per leaf in `nodes`, points = particles
-// [ibeg, iend); mode 1: one block per parent in `nodes`, points = the grids
-// of its children (child CSR child_off/child_id).
+// [ibeg, iend) of the node, output row = the node; mode 1: one block per
+// parent in `nodes`, points = the grids of its children (child CSR
+// child_off/child_id); mode 2: block b takes the node tables of nodes[b],
+// points = particles [ibeg[b], iend[b]), output row b (partial charges).
 struct AnterpArgs {
     int mode, degree, subtraction, nblocks;
     const int *nodes;
@@ -69,6 +84,13 @@ struct AnterpArgs {
     double *Q, *W;                                      // cluster charges (in/out)
 };
 hipError_t launch_anterp(const AnterpArgs &a, hipStream_t st);
+// Q[s][j] = sum_{k in [off[s], off[s+1])} Qp[k][j] (W likewise if W is set), in k order.
+hipError_t launch_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
+                            double *Q, double *W, hipStream_t st);
+// Q[s][j] = sum_{k in [off[s], off[s+1])} R[row[k] + j] (W from RW likewise), in k order;
+// zero where a slot has no rows.
+hipError_t launch_sum_rows(int nslots, int P, const int *off, const long long *row, const double *R,
+                           const double *RW, double *Q, double *W, hipStream_t st);
 
 // Downward pass.  mode 0: one block per child in `nodes`; the child grid
 // gets the interpolant of its parent's values.  mode 1: one block per tile

@@ -196,7 +196,20 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     double Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT], acc[TPT], accw[TPT];
     float Tqf[TPT];
     int tix[TPT];
-    const int nl = A.nl;
+    // target grid: degree, node tables and output row of this tile
+    int nl = A.nl;
+    const double *ttab = nullptr;
+    size_t orow = 0;
+    if constexpr (GRID) {
+        if (A.multi) {
+            const int dg = tile.w >> DEG_SHIFT;
+            nl = dg + 1;
+            ttab = A.dtab[dg] + (size_t)(tile.w & SLOT_MASK) * 3 * nl;
+        } else {
+            ttab = A.tnode_tab + (size_t)tile.w * 3 * nl;
+        }
+        orow = A.out_row ? (size_t)A.out_row[key] : (size_t)tile.w * A.P;
+    }
     const KP32 k32{(float)(-1.4426950408889634 * A.kp.inv_a2), (float)(-1.4426950408889634 * A.kp.p0),
                    (float)A.kp.eps2};
 #pragma unroll
@@ -207,8 +220,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         if constexpr (GRID) {
             const int j = tile.y + lc;
             const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-            const double *tb = A.tnode_tab + (size_t)tile.w * 3 * nl;
-            Tx[u] = tb[k1]; Ty[u] = tb[nl + k2]; Tz[u] = tb[2 * nl + k3]; Tq[u] = 0.0;
+            Tx[u] = ttab[k1]; Ty[u] = ttab[nl + k2]; Tz[u] = ttab[2 * nl + k3]; Tq[u] = 0.0;
         } else {
             const int i = tile.y + lc;
             Tx[u] = A.tx[i]; Ty[u] = A.ty[i]; Tz[u] = A.tz[i];
@@ -254,16 +266,27 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     // source clusters (PC in PC mode, CC in CC mode)
     const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
     for (int c = c0; c < c1; ++c) {
-        const size_t base = (size_t)A.clu_id[c] * A.P;
-        for (int off = 0; off < A.P; off += TPB) {
-            const int n = min(TPB, A.P - off);
+        const int id = A.clu_id[c];
+        int P = A.P;
+        const double *cx = A.cx, *cy = A.cy, *cz = A.cz, *cq = A.cq, *cw = A.cw;
+        size_t base;
+        if (A.multi) {
+            const int dg = id >> DEG_SHIFT;
+            P = (dg + 1) * (dg + 1) * (dg + 1);
+            cx = A.dcx[dg]; cy = A.dcy[dg]; cz = A.dcz[dg]; cq = A.dcq[dg]; cw = A.dcw[dg];
+            base = (size_t)(id & SLOT_MASK) * P;
+        } else {
+            base = (size_t)id * P;
+        }
+        for (int off = 0; off < P; off += TPB) {
+            const int n = min(TPB, P - off);
             __syncthreads();
             if (tid < n) {
                 const size_t g = base + off + tid;
-                const double a = A.cq[g];
-                Ls[tid] = make_double4(A.cx[g], A.cy[g], A.cz[g], a);
+                const double a = cq[g];
+                Ls[tid] = make_double4(cx[g], cy[g], cz[g], a);
                 if constexpr (SS) {
-                    const double b = A.cw[g];
+                    const double b = cw[g];
                     Lb[tid] = b;
                     if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)a, (float)b);
                 }
@@ -301,7 +324,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     for (int u = 0; u < TPT; u++) {
         if (tix[u] < 0) continue;
         if constexpr (GRID) {
-            const size_t o = (size_t)tile.w * A.P + tix[u];
+            const size_t o = orow + tix[u];
             A.out1[o] = acc[u];
             if constexpr (SS) A.out2[o] = accw[u];
         } else if (A.accumulate) {
@@ -475,14 +498,20 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
         accq[m] = 0.0; accw[m] = 0.0;
     }
 
-    // point sets: mode 0 one range of particles; mode 1 the grids of all children
+    // point sets: mode 0 one range of particles (the node's); mode 2 one
+    // range of particles (the block's part of a node); mode 1 the grids of
+    // all children
     int nsets = 1;
     if (A.mode == 1) nsets = A.child_off[blockIdx.x + 1] - A.child_off[blockIdx.x];
+    const int orow = A.mode == 2 ? (int)blockIdx.x : node;  // output row
     for (int set = 0; set < nsets; ++set) {
         int pbeg = 0, pcnt = P, child = -1;
         if (A.mode == 0) {
             pbeg = A.ibeg[node];
             pcnt = A.iend[node] - pbeg;
+        } else if (A.mode == 2) {
+            pbeg = A.ibeg[blockIdx.x];
+            pcnt = A.iend[blockIdx.x] - pbeg;
         } else {
             child = A.child_id[A.child_off[blockIdx.x] + set];
         }
@@ -493,7 +522,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
                 const bool valid = c0 + p < pcnt;
                 double coord = 0.0;
                 if (valid) {
-                    if (A.mode == 0) {
+                    if (A.mode != 1) {
                         const int g = pbeg + c0 + p;
                         coord = d == 0 ? A.px[g] : (d == 1 ? A.py[g] : A.pz[g]);
                     } else {
@@ -528,7 +557,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
                     den *= Lsum[p];
                     den *= Lsum[ACH + p];
                     den *= Lsum[2 * ACH + p];
-                    if (A.mode == 0) {
+                    if (A.mode != 1) {
                         const int g = pbeg + c0 + p;
                         const double q = A.pq[g];
                         if (SS) { const double w = A.pw[g]; f1 = q * w; f2 = w; }
@@ -562,8 +591,8 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
     for (int m = 0; m < MJ; m++) {
         const int j = tid + 256 * m;
         if (j < P) {
-            A.Q[(size_t)node * P + j] = accq[m];
-            if (SS) A.W[(size_t)node * P + j] = accw[m];
+            A.Q[(size_t)orow * P + j] = accq[m];
+            if (SS) A.W[(size_t)orow * P + j] = accw[m];
         }
     }
 }
@@ -691,6 +720,42 @@ hipError_t anterp_dispatch(const AnterpArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+// Q[s*P + j] = sum over parts k in [off[s], off[s+1]) of Qp[k*P + j] (and W),
+// in part order
+__global__ void k_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
+                            double *Q, double *W)
+{
+    const size_t total = (size_t)nslots * P;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(g / P), j = (int)(g % P);
+        double q = 0.0, w = 0.0;
+        for (int k = off[s]; k < off[s + 1]; k++) {
+            q += Qp[(size_t)k * P + j];
+            if (W) w += Wp[(size_t)k * P + j];
+        }
+        Q[g] = q;
+        if (W) W[g] = w;
+    }
+}
+
+__global__ void k_sum_rows(int nslots, int P, const int *off, const long long *row, const double *R,
+                           const double *RW, double *Q, double *W)
+{
+    const size_t total = (size_t)nslots * P;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(g / P), j = (int)(g % P);
+        double q = 0.0, w = 0.0;
+        for (int k = off[s]; k < off[s + 1]; k++) {
+            q += R[row[k] + j];
+            if (W) w += RW[row[k] + j];
+        }
+        Q[g] = q;
+        if (W) W[g] = w;
+    }
+}
+
 template <bool SS>
 hipError_t down_dispatch(const DownArgs &a, hipStream_t st)
 {
@@ -768,6 +833,26 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
         case KR: return ss ? interact_dispatch<KR, true>(a, st) : interact_dispatch<KR, false>(a, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
+                            double *Q, double *W, hipStream_t st)
+{
+    const size_t total = (size_t)nslots * P;
+    if (total == 0) return hipSuccess;
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 65535);
+    k_sum_parts<<<blocks, 256, 0, st>>>(nslots, P, off, Qp, Wp, Q, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(int nslots, int P, const int *off, const long long *row, const double *R,
+                           const double *RW, double *Q, double *W, hipStream_t st)
+{
+    const size_t total = (size_t)nslots * P;
+    if (total == 0) return hipSuccess;
+    int blocks = (int)std::min<size_t>((total + 255) / 256, 65535);
+    k_sum_rows<<<blocks, 256, 0, st>>>(nslots, P, off, row, R, RW, Q, W);
+    return hipGetLastError();
 }
 
 hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double *cx, double *cy,

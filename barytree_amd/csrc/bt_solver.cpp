@@ -32,15 +32,61 @@ constexpr int NXCD = btk::TILE_QUEUES;  // XCDs of an MI355X, one tile queue eac
 // tiles) are dealt to per-XCD lists, heaviest group to the least loaded XCD,
 // and block 8*j + x takes the j-th tile of XCD x's list.  Lists are padded
 // with empty tiles (count 0), which return at once.
+namespace {
+long long cluster_points(int id, int P, bool multi)
+{
+    if (!multi) return P;
+    const long long nl = (id >> btk::DEG_SHIFT) + 1;
+    return nl * nl * nl;
+}
+}  // namespace
+
+void split_keys(const WorkCSR &in, long long max_src, int P, bool multi, WorkCSR &out)
+{
+    out = WorkCSR();
+    out.seg_off.push_back(0);
+    out.clu_off.push_back(0);
+    out.pairs = in.pairs;
+    auto open_key = [&](size_t k) {
+        out.key.push_back(in.key[k]); out.tbeg.push_back(in.tbeg[k]); out.tcnt.push_back(in.tcnt[k]);
+    };
+    auto close_key = [&]() {
+        out.seg_off.push_back((int)out.seg_beg.size());
+        out.clu_off.push_back((int)out.clu_id.size());
+    };
+    for (size_t k = 0; k < in.key.size(); k++) {
+        open_key(k);
+        long long load = 0;
+        bool empty = true;
+        for (int c = in.clu_off[k]; c < in.clu_off[k + 1]; c++) {
+            const long long w = cluster_points(in.clu_id[c], P, multi);
+            if (!empty && load + w > max_src) { close_key(); open_key(k); load = 0; }
+            out.clu_id.push_back(in.clu_id[c]);
+            load += w;
+            empty = false;
+        }
+        for (int s = in.seg_off[k]; s < in.seg_off[k + 1]; s++) {
+            const long long w = in.seg_cnt[s];
+            if (!empty && load + w > max_src) { close_key(); open_key(k); load = 0; }
+            out.seg_beg.push_back(in.seg_beg[s]);
+            out.seg_cnt.push_back(in.seg_cnt[s]);
+            out.seg_chk.push_back(in.seg_chk[s]);
+            load += w;
+            empty = false;
+        }
+        close_key();
+    }
+}
+
 void make_queue_tiles(const WorkCSR &w, int tile_t, bool grid, int P, std::vector<btk::Tile> &tiles,
-                      int *qlen)
+                      int *qlen, bool multi)
 {
     struct Group { long long work; std::vector<btk::Tile> t; };
     std::vector<Group> groups;
     for (size_t k = 0; k < w.key.size(); k++) {
         long long src = 0;
         for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
-        src += (long long)(w.clu_off[k + 1] - w.clu_off[k]) * P;
+        for (int c = w.clu_off[k]; c < w.clu_off[k + 1]; c++) src += cluster_points(w.clu_id[c], P, multi);
         Group g{0, {}};
         for (int o = 0; o < w.tcnt[k]; o += tile_t) {
             btk::Tile t;
@@ -192,6 +238,20 @@ void Plan::build_work()
         }
     }
 
+    // heavy grid keys are split by sources; every key writes its own row
+    {
+        WorkCSR raw = std::move(grid_);
+        split_keys(raw, MAX_KEY_SOURCES, P, false, grid_);
+    }
+    grid_row_.assign(grid_.key.size(), 0);
+    grid_rows_len_ = 0;
+    for (size_t k = 0; k < grid_.key.size(); k++) {
+        grid_row_[k] = grid_rows_len_;
+        grid_rows_len_ += grid_.tcnt[k];
+    }
+    node_rows_off_.assign(ntnodes_ + 1, 0);
+    for (int node : grid_.key) node_rows_off_[node + 1]++;
+    for (int n = 0; n < ntnodes_; n++) node_rows_off_[n + 1] += node_rows_off_[n];
     make_queue_tiles(near_, tile_targets_, false, P, near_tiles_, near_qlen_);
     make_queue_tiles(grid_, GRID_TILE, true, P, grid_tiles_, grid_qlen_);
 
@@ -295,6 +355,15 @@ std::string Plan::validate() const
     std::string e = chk_csr(near_, near_tiles_, false);
     if (!e.empty()) return "near " + e;
     if (!(e = chk_csr(grid_, grid_tiles_, true)).empty()) return "far " + e;
+    if (grid_row_.size() != grid_.key.size() || (int)node_rows_off_.size() != ntnodes_ + 1 ||
+        node_rows_off_[ntnodes_] != (int)grid_.key.size())
+        return "grid rows";
+    for (size_t k = 0; k < grid_.key.size(); k++) {
+        const int node = grid_.key[k];
+        if (grid_row_[k] < 0 || grid_row_[k] + grid_.tcnt[k] > grid_rows_len_ || grid_.tcnt[k] != P ||
+            (int)k < node_rows_off_[node] || (int)k >= node_rows_off_[node + 1])
+            return "grid row";
+    }
     for (size_t i = 0; i < g_sibeg_.size(); i++)
         if (g_sibeg_[i] < 0 || g_siend_[i] > N_ || g_sibeg_[i] >= g_siend_[i]) return "source node range";
     for (int l : g_sleaves_)
@@ -574,6 +643,13 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         if (!(r = up_csr(grid_, &d_gseg_off_, &d_gsegs_, &d_gclu_off_, &d_gclu_id_)).empty()) return r;
         if (!(r = up(&d_grid_tiles_, grid_tiles_)).empty()) return r;
     }
+    if (need_tgt_clusters) {
+        if (!(r = up(&d_grid_row_, grid_row_)).empty()) return r;
+        if (!(r = up(&d_node_rows_off_, node_rows_off_)).empty()) return r;
+        d_Qrows_ = dalloc<double>(grid_rows_len_);
+        if (ss) d_Wrows_ = dalloc<double>(grid_rows_len_);
+        if (!d_Qrows_ || (ss && !d_Wrows_)) return "hipMalloc failed (grid rows)";
+    }
     d_pot_ = dalloc<double>(M_);
     d_out_ = dalloc<double>(M_);
     d_cursor_ = dalloc<int>(2 * NXCD);
@@ -639,19 +715,19 @@ std::string Plan::execute(bool timed)
 
     // 3. far field into target-cluster grids (CP, CC)
     if (p.compute_type != C_PC) {
-        const size_t np = (size_t)ntnodes_ * p.P;
-        HIPCHK(hipMemsetAsync(d_Qt_, 0, np * sizeof(double), st_));
-        if (ss) HIPCHK(hipMemsetAsync(d_Wt_, 0, np * sizeof(double), st_));
         if (!grid_tiles_.empty()) {
             btk::InteractArgs a = ia;
             a.grid_targets = 1; a.ntiles = (int)grid_tiles_.size(); a.tiles = d_grid_tiles_;
             a.seg_off = d_gseg_off_; a.segs = d_gsegs_; a.clu_off = d_gclu_off_; a.clu_id = d_gclu_id_;
             a.tnode_tab = d_ttab_;
-            a.out1 = d_Qt_; a.out2 = d_Wt_;
+            a.out1 = d_Qrows_; a.out2 = d_Wrows_; a.out_row = d_grid_row_;
             std::copy(grid_qlen_, grid_qlen_ + NXCD, a.qlen);
             a.cursor = d_cursor_ + NXCD;
             HIPCHK(btk::launch_interact(a, st_));
         }
+        // every target grid: the sum of its rows, in row order (zero without rows)
+        HIPCHK(btk::launch_sum_rows(ntnodes_, p.P, d_node_rows_off_, d_grid_row_, d_Qrows_, d_Wrows_, d_Qt_,
+                                    ss ? d_Wt_ : nullptr, st_));
     }
     HIPCHK(hipEventRecord(ev[3], st_));
 

@@ -21,6 +21,8 @@ namespace bt {
 
 constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
 constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
+constexpr int PART_MAX = 16384;  // particles per block of a source slot's charges
+constexpr long long MAX_KEY_SOURCES = 1 << 18;  // sources per grid key (heavier keys are split)
 
 struct Timings {
     double host_copy = 0, host_tree = 0, host_lists = 0, host_work = 0, h2d = 0;  // seconds
@@ -48,41 +50,45 @@ struct FarPair { int sys, kind, t, s; };
 // Features of a far-field pair, for the degree model.
 constexpr int NUM_PAIR_FEATURES = 8;
 
-// Work of one interpolation degree when every far-field pair has its own
+// Data of one interpolation degree when every far-field pair has its own
 // degree.  Source slots hold modified charges of (source node, degree),
 // computed from the node's particles; target slots hold grid values of
 // (target node, degree), interpolated straight to the node's particles.
 struct DegreeWork {
     int degree = 0, nl = 0, P = 0;
+    bool used = false;                     // some pair has this degree
     std::vector<int> s_ibeg, s_iend;       // global particle range per source slot
+    // a source slot's charges are summed from parts of at most PART_MAX
+    // particles (one block each): part k covers [p_ibeg[k], p_iend[k]) of
+    // slot p_slot[k]; slot s owns parts [p_off[s], p_off[s+1])
+    std::vector<int> p_slot, p_ibeg, p_iend, p_off;
     std::vector<double> s_tab, t_tab;      // 3*nl Chebyshev nodes per slot
-    WorkCSR pc, grid;                      // particle targets <- source slots;
-                                           // target slots <- source slots + particle ranges
-    std::vector<btk::Tile> pc_tiles, grid_tiles;
-    int pc_qlen[btk::TILE_QUEUES] = {}, grid_qlen[btk::TILE_QUEUES] = {};
+    std::vector<int> rows_off;             // target slot s sums far keys [rows_off[s], rows_off[s+1])
     std::vector<std::vector<btk::Tile>> down;  // per tree level: {target slot, first target, count}
     long long evals = 0;                   // point-wise evaluations of this degree's pairs
+    int num_tslots() const { return nl ? (int)(t_tab.size() / (3 * nl)) : 0; }
     // device copies
-    int *d_s_ident = nullptr, *d_s_ibeg = nullptr, *d_s_iend = nullptr;
+    int *d_p_slot = nullptr, *d_p_ibeg = nullptr, *d_p_iend = nullptr, *d_p_off = nullptr;
+    double *d_Qp = nullptr, *d_Wp = nullptr;
     double *d_s_tab = nullptr, *d_cx = nullptr, *d_cy = nullptr, *d_cz = nullptr;
     double *d_cq = nullptr, *d_cw = nullptr;
     double *d_t_tab = nullptr, *d_Qt = nullptr, *d_Wt = nullptr;
-    int *d_pc_seg_off = nullptr, *d_pc_clu_off = nullptr, *d_pc_clu_id = nullptr;
-    btk::Seg *d_pc_segs = nullptr;
-    btk::Tile *d_pc_tiles = nullptr;
-    int *d_g_seg_off = nullptr, *d_g_clu_off = nullptr, *d_g_clu_id = nullptr;
-    btk::Seg *d_g_segs = nullptr;
-    btk::Tile *d_g_tiles = nullptr;
+    int *d_rows_off = nullptr;
     std::vector<btk::Tile *> d_down;
 };
 
-// Input of a plan: nsys systems, system s owning targets [toffs[s],
-// toffs[s+1]) and sources [soffs[s], soffs[s+1]) of the arrays.
 // Deals a CSR's keys into tiles of at most tile_targets targets, as 8
 // interleaved per-XCD queues (tile 8*j + x = j-th tile of queue x; padding
 // slots have count 0); qlen gets each queue's length.
+// With multi, cluster ids are (degree << DEG_SHIFT) | slot and weigh
+// (degree+1)^3 points each; else P.
 void make_queue_tiles(const WorkCSR &w, int tile_targets, bool grid, int P,
-                      std::vector<btk::Tile> &tiles, int *qlen);
+                      std::vector<btk::Tile> &tiles, int *qlen, bool multi = false);
+
+// Splits every key whose sources (particles plus cluster points) exceed
+// max_src into consecutive chunks of its particle ranges and clusters; the
+// chunks are keys of their own with the same node and targets, in order.
+void split_keys(const WorkCSR &in, long long max_src, int P, bool multi, WorkCSR &out);
 
 struct PlanInput {
     int nsys = 1;
@@ -123,7 +129,7 @@ public:
     // Gives every far-field pair its own degree (1..12) and builds and
     // uploads the per-degree work; nullptr returns to the fixed degree.
     std::string set_pair_degrees(const int *deg);
-    bool variable_degree() const { return var_; }
+    bool variable_degree() const { return var_planned_; }
     double mean_pair_degree() const { return var_mean_deg_; }
     long long variable_far_evals() const { return var_evals_; }
     // Minimal degree in [2, 12] whose pair error (relative L2 over sampled
@@ -167,6 +173,11 @@ private:
     std::vector<System> sys_;
     WorkCSR near_, grid_;
     std::vector<btk::Tile> near_tiles_, grid_tiles_, leaf_tiles_;
+    // grid keys write rows of partial grid values (row of key k at
+    // grid_row_[k]); node n sums rows [node_rows_off_[n], node_rows_off_[n+1])
+    std::vector<long long> grid_row_;
+    std::vector<int> node_rows_off_;
+    long long grid_rows_len_ = 0;
     std::vector<int> up_levels_off_;  // per level: range into up_nodes_ / child CSR
     std::vector<int> up_nodes_, up_child_off_, up_child_id_;
     std::vector<int> down_levels_off_, down_nodes_;
@@ -181,16 +192,25 @@ private:
     std::vector<void *> allocs_;
     // variable degree
     std::vector<FarPair> pairs_;
-    bool var_ = false;
+    bool var_ = false;          // variable-degree work is on the device
+    bool var_planned_ = false;  // variable-degree work is built (host)
     double var_mean_deg_ = 0;
     long long var_evals_ = 0;
     std::vector<DegreeWork> dw_;
-    WorkCSR near_pp_;
-    std::vector<btk::Tile> near_pp_tiles_;
-    int near_pp_qlen_[btk::TILE_QUEUES] = {};
-    int *d_pp_seg_off_ = nullptr, *d_pp_clu_off_ = nullptr;
-    btk::Seg *d_pp_segs_ = nullptr;
-    btk::Tile *d_pp_tiles_ = nullptr;
+    // one near launch (particle ranges + source slots of every degree) and
+    // one far launch (every target slot) for all degrees
+    WorkCSR vnear_, vfar_;
+    std::vector<btk::Tile> vnear_tiles_, vfar_tiles_;
+    int vnear_qlen_[btk::TILE_QUEUES] = {}, vfar_qlen_[btk::TILE_QUEUES] = {};
+    std::vector<long long> vfar_row_;
+    long long vfar_rows_len_ = 0;
+    int *d_vn_clu_off_ = nullptr, *d_vn_clu_id_ = nullptr;
+    btk::Tile *d_vn_tiles_ = nullptr;
+    int *d_vf_seg_off_ = nullptr, *d_vf_clu_off_ = nullptr, *d_vf_clu_id_ = nullptr;
+    btk::Seg *d_vf_segs_ = nullptr;
+    btk::Tile *d_vf_tiles_ = nullptr;
+    long long *d_vf_row_ = nullptr;
+    double *d_vf_R_ = nullptr, *d_vf_RW_ = nullptr;
     std::vector<void *> var_allocs_;
     bool alloc_var_ = false;  // route dalloc to var_allocs_
     // device arrays
@@ -210,6 +230,9 @@ private:
     btk::Seg *d_gsegs_ = nullptr;
     double *d_pot_ = nullptr, *d_out_ = nullptr;
     int *d_cursor_ = nullptr;  // tile-queue cursors: near, then far
+    long long *d_grid_row_ = nullptr;
+    int *d_node_rows_off_ = nullptr;
+    double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;
 };
 
 }  // namespace bt

@@ -109,46 +109,49 @@ void Plan::free_var()
     for (void *p : var_allocs_) (void)hipFree(p);
     var_allocs_.clear();
     dw_.clear();
-    near_pp_ = WorkCSR();
-    near_pp_tiles_.clear();
+    vnear_ = WorkCSR();
+    vfar_ = WorkCSR();
+    vnear_tiles_.clear();
+    vfar_tiles_.clear();
+    vfar_row_.clear();
+    vfar_rows_len_ = 0;
     var_ = false;
+    var_planned_ = false;
     var_mean_deg_ = 0;
     var_evals_ = 0;
-    d_pp_seg_off_ = d_pp_clu_off_ = nullptr;
-    d_pp_segs_ = nullptr;
-    d_pp_tiles_ = nullptr;
+    d_vn_clu_off_ = d_vn_clu_id_ = nullptr;
+    d_vn_tiles_ = nullptr;
+    d_vf_seg_off_ = d_vf_clu_off_ = d_vf_clu_id_ = nullptr;
+    d_vf_segs_ = nullptr;
+    d_vf_tiles_ = nullptr;
+    d_vf_row_ = nullptr;
+    d_vf_R_ = d_vf_RW_ = nullptr;
 }
+
+namespace {
+inline int multi_id(int deg, int slot) { return (deg << btk::DEG_SHIFT) | slot; }
+}  // namespace
 
 std::string Plan::set_pair_degrees(const int *deg)
 {
     free_var();
     if (!deg) return "";
-    if (!st_) return "plan has no device data";
     const long long np = (long long)pairs_.size();
     for (long long i = 0; i < np; i++)
         if (deg[i] < 1 || deg[i] > MAXDEG) return "pair degree outside [1, 12]";
     const bool ss = prm_.singularity == S_SUBTRACTION;
 
-    // near field: the fixed plan's keys and particle ranges, without clusters
-    near_pp_ = near_;
-    near_pp_.clu_off.assign(near_.clu_off.size(), 0);
-    near_pp_.clu_id.clear();
-    near_pp_.pairs = 0;
-    for (size_t k = 0; k < near_pp_.key.size(); k++)
-        for (int s = near_pp_.seg_off[k]; s < near_pp_.seg_off[k + 1]; s++)
-            near_pp_.pairs += (long long)near_pp_.tcnt[k] * near_pp_.seg_cnt[s];
-    make_queue_tiles(near_pp_, tile_targets_, false, 1, near_pp_tiles_, near_pp_qlen_);
-
-    // slots per degree
     std::vector<std::vector<int>> sslot(MAXDEG + 1), tslot(MAXDEG + 1);
     std::vector<int> used(MAXDEG + 1, 0);
     for (long long i = 0; i < np; i++) used[deg[i]] = 1;
     dw_.assign(MAXDEG + 1, DegreeWork());
     for (int p = 1; p <= MAXDEG; p++) {
         dw_[p].degree = p; dw_[p].nl = p + 1; dw_[p].P = (p + 1) * (p + 1) * (p + 1);
+        dw_[p].used = used[p] != 0;
         if (used[p]) { sslot[p].assign(nsnodes_, -1); tslot[p].assign(ntnodes_, -1); }
     }
-    // per system and degree, node tables (computed on demand)
+    if (nsnodes_ > btk::SLOT_MASK || ntnodes_ > btk::SLOT_MASK) return "too many tree nodes for degree slots";
+    // node tables per system and degree, computed on demand
     std::vector<std::vector<std::vector<double>>> stab(sys_.size()), ttab(sys_.size());
     for (size_t y = 0; y < sys_.size(); y++) {
         stab[y].resize(MAXDEG + 1);
@@ -168,9 +171,6 @@ std::string Plan::set_pair_degrees(const int *deg)
         }
         return sl;
     };
-    // target slots and their entries (CC clusters first, then CP ranges, in pair order)
-    struct GridEntry { int slot, kind, src, cnt; };
-    std::vector<std::vector<GridEntry>> gent(MAXDEG + 1);
     std::vector<std::vector<int>> slot_node(MAXDEG + 1), slot_sys(MAXDEG + 1);
     auto tgt_slot = [&](int sy, int t, int p) {
         const System &y = sys_[sy];
@@ -186,127 +186,134 @@ std::string Plan::set_pair_degrees(const int *deg)
         }
         return sl;
     };
-    // PC entries per (degree, global target node): source slots, in pair order
-    std::vector<std::vector<std::pair<int, int>>> pc_at(MAXDEG + 1);  // (global tnode, slot)
+    // PC entries per global target node; CC clusters and CP ranges per
+    // target slot; all in pair order
+    std::vector<std::vector<int>> pc_at(ntnodes_);
+    std::vector<std::vector<std::vector<int>>> tclu(MAXDEG + 1), tseg(MAXDEG + 1), tsegc(MAXDEG + 1);
     double dsum = 0.0;
     var_evals_ = 0;
     for (long long i = 0; i < np; i++) {
         const FarPair &q = pairs_[i];
         const int p = deg[i];
         const System &y = sys_[q.sys];
-        const long long P = dw_[p].P;
+        DegreeWork &w = dw_[p];
+        const long long P = w.P;
         dsum += p;
         if (q.kind == FAR_PC) {
-            pc_at[p].push_back({y.tnode_off + q.t, src_slot(q.sys, q.s, p)});
-            dw_[p].evals += (long long)y.ttree.numpar[q.t] * P;
-        } else if (q.kind == FAR_CP) {
-            const int sl = tgt_slot(q.sys, q.t, p);
-            gent[p].push_back({sl, FAR_CP, y.soff + y.stree.ibeg[q.s], y.stree.numpar[q.s]});
-            dw_[p].evals += P * y.stree.numpar[q.s];
+            pc_at[y.tnode_off + q.t].push_back(multi_id(p, src_slot(q.sys, q.s, p)));
+            w.evals += (long long)y.ttree.numpar[q.t] * P;
+            continue;
+        }
+        const int sl = tgt_slot(q.sys, q.t, p);
+        if ((int)tclu[p].size() <= sl) { tclu[p].resize(sl + 1); tseg[p].resize(sl + 1); tsegc[p].resize(sl + 1); }
+        if (q.kind == FAR_CP) {
+            tseg[p][sl].push_back(y.soff + y.stree.ibeg[q.s]);
+            tsegc[p][sl].push_back(y.stree.numpar[q.s]);
+            w.evals += P * y.stree.numpar[q.s];
         } else {
-            const int sl = tgt_slot(q.sys, q.t, p);
-            gent[p].push_back({sl, FAR_CC, src_slot(q.sys, q.s, p), 0});
-            dw_[p].evals += P * P;
+            tclu[p][sl].push_back(multi_id(p, src_slot(q.sys, q.s, p)));
+            w.evals += P * P;
         }
     }
     var_mean_deg_ = np ? dsum / np : 0.0;
+    for (int p = 1; p <= MAXDEG; p++) var_evals_ += dw_[p].evals;
+
+    // near: the fixed plan's keys and particle ranges, with the source slots
+    // of the key (PC) or of the key and its ancestors (CC)
+    vnear_ = near_;
+    vnear_.clu_id.clear();
+    vnear_.clu_off.assign(1, 0);
+    for (size_t k = 0; k < near_.key.size(); k++) {
+        for (int a = near_.key[k]; a >= 0; a = prm_.compute_type == C_CC ? g_tparent_[a] : -1)
+            for (int id : pc_at[a]) vnear_.clu_id.push_back(id);
+        vnear_.clu_off.push_back((int)vnear_.clu_id.size());
+    }
+    make_queue_tiles(vnear_, tile_targets_, false, 0, vnear_tiles_, vnear_qlen_, true);
+
+    // far: every target slot, degree by degree, split by sources; rows of
+    // partial grid values, summed per slot after the launch
+    WorkCSR raw;
+    raw.seg_off.push_back(0); raw.clu_off.push_back(0);
+    for (int p = 1; p <= MAXDEG; p++) {
+        const int T = (int)slot_node[p].size();
+        for (int sl = 0; sl < T; sl++) {
+            raw.key.push_back(multi_id(p, sl));
+            raw.tbeg.push_back(0);
+            raw.tcnt.push_back(dw_[p].P);
+            for (size_t j = 0; j < tseg[p][sl].size(); j++) {
+                raw.seg_beg.push_back(tseg[p][sl][j]);
+                raw.seg_cnt.push_back(tsegc[p][sl][j]);
+                raw.seg_chk.push_back(0);
+            }
+            for (int id : tclu[p][sl]) raw.clu_id.push_back(id);
+            raw.seg_off.push_back((int)raw.seg_beg.size());
+            raw.clu_off.push_back((int)raw.clu_id.size());
+        }
+    }
+    split_keys(raw, MAX_KEY_SOURCES, 0, true, vfar_);
+    vfar_row_.assign(vfar_.key.size(), 0);
+    vfar_rows_len_ = 0;
+    for (size_t k = 0; k < vfar_.key.size(); k++) {
+        vfar_row_[k] = vfar_rows_len_;
+        vfar_rows_len_ += vfar_.tcnt[k];
+    }
+    for (int p = 1; p <= MAXDEG; p++) dw_[p].rows_off.assign(slot_node[p].size() + 1, 0);
+    {
+        // keys come degree by degree, slot by slot
+        for (size_t k = 0; k < vfar_.key.size(); k++) {
+            const int p = vfar_.key[k] >> btk::DEG_SHIFT, sl = vfar_.key[k] & btk::SLOT_MASK;
+            dw_[p].rows_off[sl + 1]++;
+        }
+        int base = 0;
+        for (int p = 1; p <= MAXDEG; p++) {
+            std::vector<int> &o = dw_[p].rows_off;
+            o[0] = base;
+            for (size_t sl = 1; sl < o.size(); sl++) o[sl] += o[sl - 1];
+            base = o.back();
+        }
+    }
+    make_queue_tiles(vfar_, GRID_TILE, true, 0, vfar_tiles_, vfar_qlen_, true);
 
     for (int p = 1; p <= MAXDEG; p++) {
         if (!used[p]) continue;
         DegreeWork &w = dw_[p];
-        var_evals_ += w.evals;
-        // particle targets <- source slots: keys are target leaves (batches
-        // in PC), each with the entries of itself and its ancestors
-        w.pc = WorkCSR();
-        w.pc.seg_off.push_back(0); w.pc.clu_off.push_back(0);
-        if (!pc_at[p].empty()) {
-            std::vector<std::vector<int>> at(ntnodes_);
-            for (auto &e : pc_at[p]) at[e.first].push_back(e.second);
-            for (const System &y : sys_) {
-                const Tree &tt = y.ttree;
-                auto add_key = [&](int leaf, bool walk) {
-                    size_t before = w.pc.clu_id.size();
-                    for (int a = leaf; a >= 0; a = walk ? tt.parent[a] : -1)
-                        for (int sl : at[y.tnode_off + a]) w.pc.clu_id.push_back(sl);
-                    if (w.pc.clu_id.size() == before) return;
-                    w.pc.key.push_back(y.tnode_off + leaf);
-                    w.pc.tbeg.push_back(y.toff + tt.ibeg[leaf]);
-                    w.pc.tcnt.push_back(tt.numpar[leaf]);
-                    w.pc.seg_off.push_back(0);
-                    w.pc.clu_off.push_back((int)w.pc.clu_id.size());
-                    w.pc.pairs += (long long)tt.numpar[leaf] * w.P *
-                                  (long long)(w.pc.clu_id.size() - before);
-                };
-                if (prm_.compute_type == C_PC) {
-                    for (int b = 0; b < tt.n; b++) add_key(b, false);
-                } else {
-                    for (int leaf : tt.leaves) add_key(leaf, true);
-                }
+        // downpass: each target slot's interpolant at its node's particles,
+        // one launch per tree level (nodes of a level have disjoint ranges)
+        const int T = (int)slot_node[p].size();
+        w.down.assign(tdepth_ + 1, {});
+        for (int sl = 0; sl < T; sl++) {
+            const System &y = sys_[slot_sys[p][sl]];
+            const int t = slot_node[p][sl];
+            const int lv = y.ttree.level[t];
+            for (int o = 0; o < y.ttree.numpar[t]; o += DOWN_TILE) {
+                btk::Tile tl;
+                tl.x = sl;
+                tl.y = y.toff + y.ttree.ibeg[t] + o;
+                tl.z = std::min(DOWN_TILE, y.ttree.numpar[t] - o);
+                tl.w = 0;
+                w.down[lv].push_back(tl);
             }
         }
-        make_queue_tiles(w.pc, tile_targets_, false, w.P, w.pc_tiles, w.pc_qlen);
-        // target slots <- source slots (CC) and particle ranges (CP)
-        w.grid = WorkCSR();
-        w.grid.seg_off.push_back(0); w.grid.clu_off.push_back(0);
-        const int ns = (int)slot_node[p].size();
-        if (ns > 0) {
-            std::vector<std::vector<int>> clus(ns), segs(ns), segc(ns);
-            for (const GridEntry &g : gent[p]) {
-                if (g.kind == FAR_CC) {
-                    clus[g.slot].push_back(g.src);
-                } else {
-                    segs[g.slot].push_back(g.src);
-                    segc[g.slot].push_back(g.cnt);
-                }
+        // parts of the source slots
+        w.p_slot.clear(); w.p_ibeg.clear(); w.p_iend.clear(); w.p_off.assign(1, 0);
+        for (size_t sl = 0; sl < w.s_ibeg.size(); sl++) {
+            for (int b = w.s_ibeg[sl]; b < w.s_iend[sl]; b += PART_MAX) {
+                w.p_slot.push_back((int)sl);
+                w.p_ibeg.push_back(b);
+                w.p_iend.push_back(std::min(b + PART_MAX, w.s_iend[sl]));
             }
-            for (int sl = 0; sl < ns; sl++) {
-                w.grid.key.push_back(sl);
-                w.grid.tbeg.push_back(0);
-                w.grid.tcnt.push_back(w.P);
-                for (size_t j = 0; j < segs[sl].size(); j++) {
-                    w.grid.seg_beg.push_back(segs[sl][j]);
-                    w.grid.seg_cnt.push_back(segc[sl][j]);
-                    w.grid.seg_chk.push_back(0);
-                    w.grid.pairs += (long long)w.P * segc[sl][j];
-                }
-                for (int c : clus[sl]) {
-                    w.grid.clu_id.push_back(c);
-                    w.grid.pairs += (long long)w.P * w.P;
-                }
-                w.grid.seg_off.push_back((int)w.grid.seg_beg.size());
-                w.grid.clu_off.push_back((int)w.grid.clu_id.size());
-            }
-            // downpass: each target slot's interpolant at its node's particles,
-            // one launch per tree level (nodes of a level have disjoint ranges)
-            w.down.assign(tdepth_ + 1, {});
-            for (int sl = 0; sl < ns; sl++) {
-                const System &y = sys_[slot_sys[p][sl]];
-                const int t = slot_node[p][sl];
-                const int lv = y.ttree.level[t];
-                for (int o = 0; o < y.ttree.numpar[t]; o += DOWN_TILE) {
-                    btk::Tile tl;
-                    tl.x = sl;
-                    tl.y = y.toff + y.ttree.ibeg[t] + o;
-                    tl.z = std::min(DOWN_TILE, y.ttree.numpar[t] - o);
-                    tl.w = 0;
-                    w.down[lv].push_back(tl);
-                }
-            }
+            w.p_off.push_back((int)w.p_slot.size());
         }
-        make_queue_tiles(w.grid, GRID_TILE, true, w.P, w.grid_tiles, w.grid_qlen);
     }
     std::string e = validate_var();
     if (!e.empty()) { free_var(); return "internal variable-degree work check failed: " + e; }
+    var_planned_ = true;
+    if (!st_) return "";  // host-only plan: work built and checked, nothing to upload
 
     // upload
     alloc_var_ = true;
     std::string r;
     auto fail = [&](const std::string &m) { alloc_var_ = false; free_var(); return m; };
-    {
-        std::vector<int> zeros(near_pp_.clu_off.size(), 0);
-        if (!(r = up(&d_pp_clu_off_, zeros)).empty()) return fail(r);
-        if (!(r = up(&d_pp_tiles_, near_pp_tiles_)).empty()) return fail(r);
-    }
     auto up_csr = [&](const WorkCSR &wk, int **so, btk::Seg **sg, int **co, int **ci) -> std::string {
         std::vector<btk::Seg> sv(wk.seg_beg.size());
         for (size_t i = 0; i < sv.size(); i++) { sv[i].x = wk.seg_beg[i]; sv[i].y = wk.seg_cnt[i]; }
@@ -316,33 +323,40 @@ std::string Plan::set_pair_degrees(const int *deg)
         if (!(rr = up(co, wk.clu_off)).empty()) return rr;
         return up(ci, wk.clu_id);
     };
+    if (!(r = up(&d_vn_clu_off_, vnear_.clu_off)).empty()) return fail(r);
+    if (!(r = up(&d_vn_clu_id_, vnear_.clu_id)).empty()) return fail(r);
+    if (!(r = up(&d_vn_tiles_, vnear_tiles_)).empty()) return fail(r);
+    if (!(r = up_csr(vfar_, &d_vf_seg_off_, &d_vf_segs_, &d_vf_clu_off_, &d_vf_clu_id_)).empty()) return fail(r);
+    if (!(r = up(&d_vf_tiles_, vfar_tiles_)).empty()) return fail(r);
+    if (!(r = up(&d_vf_row_, vfar_row_)).empty()) return fail(r);
+    d_vf_R_ = dalloc<double>(vfar_rows_len_);
+    if (ss) d_vf_RW_ = dalloc<double>(vfar_rows_len_);
+    if (!d_vf_R_ || (ss && !d_vf_RW_)) return fail("hipMalloc failed (far rows)");
     for (int p = 1; p <= MAXDEG; p++) {
         if (!used[p]) continue;
         DegreeWork &w = dw_[p];
-        const size_t S = w.s_ibeg.size(), T = w.t_tab.size() / (3 * w.nl);
+        const size_t S = w.s_ibeg.size(), T = w.num_tslots();
         if (S > 0) {
-            std::vector<int> ident(S);
-            std::iota(ident.begin(), ident.end(), 0);
-            if (!(r = up(&w.d_s_ident, ident)).empty()) return fail(r);
-            if (!(r = up(&w.d_s_ibeg, w.s_ibeg)).empty()) return fail(r);
-            if (!(r = up(&w.d_s_iend, w.s_iend)).empty()) return fail(r);
+            if (!(r = up(&w.d_p_slot, w.p_slot)).empty()) return fail(r);
+            if (!(r = up(&w.d_p_ibeg, w.p_ibeg)).empty()) return fail(r);
+            if (!(r = up(&w.d_p_iend, w.p_iend)).empty()) return fail(r);
+            if (!(r = up(&w.d_p_off, w.p_off)).empty()) return fail(r);
             if (!(r = up(&w.d_s_tab, w.s_tab)).empty()) return fail(r);
-            const size_t n = S * w.P;
+            const size_t n = S * w.P, npt = w.p_slot.size() * (size_t)w.P;
             w.d_cx = dalloc<double>(n); w.d_cy = dalloc<double>(n); w.d_cz = dalloc<double>(n);
             w.d_cq = dalloc<double>(n);
-            if (ss) w.d_cw = dalloc<double>(n);
-            if (!w.d_cx || !w.d_cy || !w.d_cz || !w.d_cq || (ss && !w.d_cw)) return fail("hipMalloc failed (slots)");
+            w.d_Qp = dalloc<double>(npt);
+            if (ss) { w.d_cw = dalloc<double>(n); w.d_Wp = dalloc<double>(npt); }
+            if (!w.d_cx || !w.d_cy || !w.d_cz || !w.d_cq || !w.d_Qp || (ss && (!w.d_cw || !w.d_Wp)))
+                return fail("hipMalloc failed (slots)");
         }
         if (T > 0) {
             if (!(r = up(&w.d_t_tab, w.t_tab)).empty()) return fail(r);
+            if (!(r = up(&w.d_rows_off, w.rows_off)).empty()) return fail(r);
             w.d_Qt = dalloc<double>(T * w.P);
             if (ss) w.d_Wt = dalloc<double>(T * w.P);
             if (!w.d_Qt || (ss && !w.d_Wt)) return fail("hipMalloc failed (slots)");
         }
-        if (!(r = up_csr(w.pc, &w.d_pc_seg_off, &w.d_pc_segs, &w.d_pc_clu_off, &w.d_pc_clu_id)).empty()) return fail(r);
-        if (!(r = up(&w.d_pc_tiles, w.pc_tiles)).empty()) return fail(r);
-        if (!(r = up_csr(w.grid, &w.d_g_seg_off, &w.d_g_segs, &w.d_g_clu_off, &w.d_g_clu_id)).empty()) return fail(r);
-        if (!(r = up(&w.d_g_tiles, w.grid_tiles)).empty()) return fail(r);
         w.d_down.assign(w.down.size(), nullptr);
         for (size_t lv = 0; lv < w.down.size(); lv++)
             if (!w.down[lv].empty() && !(r = up(&w.d_down[lv], w.down[lv])).empty()) return fail(r);
@@ -355,25 +369,32 @@ std::string Plan::set_pair_degrees(const int *deg)
 
 std::string Plan::validate_var() const
 {
+    auto slot_ok = [&](int id, bool target) {
+        const int p = id >> btk::DEG_SHIFT, sl = id & btk::SLOT_MASK;
+        if (p < 1 || p > MAXDEG || (int)dw_.size() <= p) return false;
+        const DegreeWork &w = dw_[p];
+        return sl >= 0 && sl < (target ? w.num_tslots() : (int)w.s_ibeg.size());
+    };
     auto chk = [&](const WorkCSR &w, const std::vector<btk::Tile> &tiles, const int *qlen, bool grid,
-                   int nclu, int nkeyslots, int P, int tile_t) -> std::string {
+                   int tile_t) -> std::string {
         const int nk = (int)w.key.size();
         if ((int)w.seg_off.size() != nk + 1 || (int)w.clu_off.size() != nk + 1) return "csr offsets";
         if (w.seg_off[nk] != (int)w.seg_beg.size() || w.clu_off[nk] != (int)w.clu_id.size()) return "csr ends";
         for (int k = 0; k < nk; k++) {
             if (w.seg_off[k] > w.seg_off[k + 1] || w.clu_off[k] > w.clu_off[k + 1]) return "csr order";
-            if (grid && (w.key[k] < 0 || w.key[k] >= nkeyslots)) return "grid key";
+            if (grid && !slot_ok(w.key[k], true)) return "grid key";
+            if (grid && w.tcnt[k] != dw_[w.key[k] >> btk::DEG_SHIFT].P) return "grid key size";
         }
         for (size_t i = 0; i < w.seg_beg.size(); i++)
             if (w.seg_beg[i] < 0 || w.seg_cnt[i] < 0 || (long long)w.seg_beg[i] + w.seg_cnt[i] > N_)
                 return "source range";
         for (int c : w.clu_id)
-            if (c < 0 || c >= nclu) return "source slot";
+            if (!slot_ok(c, false)) return "source slot";
         for (const btk::Tile &t : tiles) {
             if (t.z == 0) continue;
             if (t.x < 0 || t.x >= nk || t.z < 0 || t.z > tile_t || t.y < 0) return "tile";
             if (grid) {
-                if (t.w < 0 || t.w >= nkeyslots || t.y + t.z > P) return "grid tile";
+                if (t.w != w.key[t.x] || t.y + t.z > w.tcnt[t.x]) return "grid tile";
             } else if ((long long)t.y + t.z > M_) {
                 return "target tile";
             }
@@ -385,16 +406,35 @@ std::string Plan::validate_var() const
             }
         return "";
     };
-    std::string e = chk(near_pp_, near_pp_tiles_, near_pp_qlen_, false, 1, 0, 1, tile_targets_);
+    std::string e = chk(vnear_, vnear_tiles_, vnear_qlen_, false, tile_targets_);
     if (!e.empty()) return "near " + e;
+    if ((e = chk(vfar_, vfar_tiles_, vfar_qlen_, true, GRID_TILE)).empty() == false) return "far " + e;
+    if (vfar_row_.size() != vfar_.key.size()) return "far rows";
+    for (size_t k = 0; k < vfar_.key.size(); k++)
+        if (vfar_row_[k] < 0 || vfar_row_[k] + vfar_.tcnt[k] > vfar_rows_len_) return "far row";
     for (const DegreeWork &w : dw_) {
-        if (w.degree == 0 || w.pc.seg_off.empty()) continue;  // degree not used
-        const int S = (int)w.s_ibeg.size(), T = (int)(w.t_tab.size() / (3 * w.nl));
+        if (!w.used) continue;
+        const int S = (int)w.s_ibeg.size(), T = w.num_tslots();
         if ((int)w.s_iend.size() != S || (int)w.s_tab.size() != S * 3 * w.nl) return "source slots";
         for (int i = 0; i < S; i++)
             if (w.s_ibeg[i] < 0 || w.s_iend[i] > N_ || w.s_ibeg[i] >= w.s_iend[i]) return "source slot range";
-        if (!(e = chk(w.pc, w.pc_tiles, w.pc_qlen, false, S, 0, w.P, tile_targets_)).empty()) return "pc " + e;
-        if (!(e = chk(w.grid, w.grid_tiles, w.grid_qlen, true, S, T, w.P, GRID_TILE)).empty()) return "grid " + e;
+        const int NP = (int)w.p_slot.size();
+        if ((int)w.p_ibeg.size() != NP || (int)w.p_iend.size() != NP || (int)w.p_off.size() != S + 1 ||
+            w.p_off[0] != 0 || w.p_off[S] != NP)
+            return "source parts";
+        for (int k = 0; k < NP; k++)
+            if (w.p_slot[k] < 0 || w.p_slot[k] >= S || w.p_ibeg[k] < w.s_ibeg[w.p_slot[k]] ||
+                w.p_iend[k] > w.s_iend[w.p_slot[k]] || w.p_ibeg[k] >= w.p_iend[k])
+                return "source part range";
+        for (int i = 0; i < S; i++)
+            if (w.p_off[i] > w.p_off[i + 1]) return "source part offsets";
+        if ((int)w.rows_off.size() != T + 1) return "slot rows";
+        for (int sl = 0; sl < T; sl++) {
+            if (w.rows_off[sl] > w.rows_off[sl + 1] || w.rows_off[sl + 1] > (int)vfar_.key.size())
+                return "slot rows order";
+            for (int k = w.rows_off[sl]; k < w.rows_off[sl + 1]; k++)
+                if (vfar_.key[k] != multi_id(w.degree, sl)) return "slot row key";
+        }
         for (const auto &lv : w.down)
             for (const btk::Tile &t : lv)
                 if (t.x < 0 || t.x >= T || t.y < 0 || t.z <= 0 || t.z > DOWN_TILE || (long long)t.y + t.z > M_)
@@ -480,20 +520,19 @@ std::string Plan::execute_var(bool timed)
     // 1. source slots: modified charges of degree p from the node's particles
     for (const DegreeWork &w : dw_) {
         const int S = (int)w.s_ibeg.size();
-        if (w.degree == 0 || S == 0) continue;
+        if (!w.used || S == 0) continue;
         HIPCHK(btk::launch_cluster_points(S, w.nl, w.d_s_tab, w.d_cx, w.d_cy, w.d_cz, st_));
         btk::AnterpArgs a{};
         a.degree = w.degree; a.subtraction = ss;
-        a.ibeg = w.d_s_ibeg; a.iend = w.d_s_iend; a.tab = w.d_s_tab;
+        a.ibeg = w.d_p_ibeg; a.iend = w.d_p_iend; a.tab = w.d_s_tab;
         a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
-        a.Q = w.d_cq; a.W = w.d_cw;
-        a.mode = 0; a.nodes = w.d_s_ident; a.nblocks = S;
+        a.Q = w.d_Qp; a.W = w.d_Wp;
+        a.mode = 2; a.nodes = w.d_p_slot; a.nblocks = (int)w.p_slot.size();
         HIPCHK(btk::launch_anterp(a, st_));
+        HIPCHK(btk::launch_sum_parts(S, w.P, w.d_p_off, w.d_Qp, w.d_Wp, w.d_cq, ss ? w.d_cw : nullptr, st_));
     }
     HIPCHK(hipEventRecord(ev[1], st_));
 
-    // 2. near field (stores every target), then each degree's source slots
-    //    into the particle targets (adds)
     btk::InteractArgs ia{};
     ia.kernel = p.kernel; ia.singularity = p.singularity; ia.precision = p.precision;
     ia.tpt = tpt_;
@@ -501,54 +540,48 @@ std::string Plan::execute_var(bool timed)
     ia.kp = kp_;
     ia.tx = d_tx_; ia.ty = d_ty_; ia.tz = d_tz_; ia.tq = d_tq_;
     ia.sx = d_sx_; ia.sy = d_sy_; ia.sz = d_sz_; ia.sq = d_sq_; ia.sw = d_sw_;
+    ia.multi = 1;
+    for (const DegreeWork &w : dw_) {
+        if (!w.used) continue;
+        ia.dcx[w.degree] = w.d_cx; ia.dcy[w.degree] = w.d_cy; ia.dcz[w.degree] = w.d_cz;
+        ia.dcq[w.degree] = w.d_cq; ia.dcw[w.degree] = w.d_cw; ia.dtab[w.degree] = w.d_t_tab;
+    }
+
+    // 2. near field and the source slots of every degree into the potential
+    //    (NaN first: every target is written by exactly one key)
     HIPCHK(hipMemsetAsync(d_pot_, 0xFF, (size_t)M_ * sizeof(double), st_));
     {
         btk::InteractArgs a = ia;
-        a.P = 1; a.nl = 1;
-        a.grid_targets = 0; a.ntiles = (int)near_pp_tiles_.size(); a.tiles = d_pp_tiles_;
-        a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_pp_clu_off_; a.clu_id = d_pp_clu_off_;
+        a.grid_targets = 0; a.ntiles = (int)vnear_tiles_.size(); a.tiles = d_vn_tiles_;
+        a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_vn_clu_off_; a.clu_id = d_vn_clu_id_;
         a.out1 = d_pot_;
-        std::copy(near_pp_qlen_, near_pp_qlen_ + NQ, a.qlen);
-        a.cursor = d_cursor_;
-        HIPCHK(btk::launch_interact(a, st_));
-    }
-    for (const DegreeWork &w : dw_) {
-        if (w.degree == 0 || w.pc.key.empty()) continue;
-        btk::InteractArgs a = ia;
-        a.P = w.P; a.nl = w.nl;
-        a.cx = w.d_cx; a.cy = w.d_cy; a.cz = w.d_cz; a.cq = w.d_cq; a.cw = w.d_cw;
-        a.grid_targets = 0; a.ntiles = (int)w.pc_tiles.size(); a.tiles = w.d_pc_tiles;
-        a.seg_off = w.d_pc_seg_off; a.segs = w.d_pc_segs; a.clu_off = w.d_pc_clu_off; a.clu_id = w.d_pc_clu_id;
-        a.out1 = d_pot_;
-        a.accumulate = 1;
-        std::copy(w.pc_qlen, w.pc_qlen + NQ, a.qlen);
+        std::copy(vnear_qlen_, vnear_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
     }
     HIPCHK(hipEventRecord(ev[2], st_));
 
-    // 3. target slots: grid values from source slots (CC) and particles (CP)
-    for (const DegreeWork &w : dw_) {
-        if (w.degree == 0 || w.grid.key.empty()) continue;
-        const size_t T = w.t_tab.size() / (3 * w.nl);
-        HIPCHK(hipMemsetAsync(w.d_Qt, 0, T * w.P * sizeof(double), st_));
-        if (ss) HIPCHK(hipMemsetAsync(w.d_Wt, 0, T * w.P * sizeof(double), st_));
+    // 3. every target slot's grid values (rows), then the sum per slot
+    if (!vfar_.key.empty()) {
         btk::InteractArgs a = ia;
-        a.P = w.P; a.nl = w.nl;
-        a.cx = w.d_cx; a.cy = w.d_cy; a.cz = w.d_cz; a.cq = w.d_cq; a.cw = w.d_cw;
-        a.grid_targets = 1; a.ntiles = (int)w.grid_tiles.size(); a.tiles = w.d_g_tiles;
-        a.seg_off = w.d_g_seg_off; a.segs = w.d_g_segs; a.clu_off = w.d_g_clu_off; a.clu_id = w.d_g_clu_id;
-        a.tnode_tab = w.d_t_tab;
-        a.out1 = w.d_Qt; a.out2 = w.d_Wt;
-        std::copy(w.grid_qlen, w.grid_qlen + NQ, a.qlen);
+        a.grid_targets = 1; a.ntiles = (int)vfar_tiles_.size(); a.tiles = d_vf_tiles_;
+        a.seg_off = d_vf_seg_off_; a.segs = d_vf_segs_; a.clu_off = d_vf_clu_off_; a.clu_id = d_vf_clu_id_;
+        a.out1 = d_vf_R_; a.out2 = d_vf_RW_; a.out_row = d_vf_row_;
+        std::copy(vfar_qlen_, vfar_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_ + NQ;
         HIPCHK(btk::launch_interact(a, st_));
+        for (const DegreeWork &w : dw_) {
+            const int T = w.num_tslots();
+            if (!w.used || T == 0) continue;
+            HIPCHK(btk::launch_sum_rows(T, w.P, w.d_rows_off, d_vf_row_, d_vf_R_, d_vf_RW_, w.d_Qt,
+                                        ss ? w.d_Wt : nullptr, st_));
+        }
     }
     HIPCHK(hipEventRecord(ev[3], st_));
 
     // 4. each target slot's interpolant at its node's particles
     for (const DegreeWork &w : dw_) {
-        if (w.degree == 0) continue;
+        if (!w.used) continue;
         for (size_t lv = 0; lv < w.down.size(); lv++) {
             if (w.down[lv].empty()) continue;
             btk::DownArgs d{};

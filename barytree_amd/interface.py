@@ -74,7 +74,9 @@ class Stats(ctypes.Structure):
             "host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s",
             "gpu_upward_s", "gpu_near_s", "gpu_far_s", "gpu_down_s", "gpu_finish_s",
             "gpu_total_s")] + [
-        (n, ctypes.c_int) for n in ("num_systems", "targets_per_thread", "persistent")]
+        (n, ctypes.c_int) for n in ("num_systems", "targets_per_thread", "persistent")] + [
+        ("num_far_pairs", ctypes.c_longlong), ("variable_degree", ctypes.c_int),
+        ("mean_pair_degree", ctypes.c_double), ("far_ptwise_variable", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -129,7 +131,7 @@ def lib():
                   "BaryTree_LabelPairs", "BaryTreeModel_Create", "BaryTreeModel_Load",
                   "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
                   "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
-                  "BaryTree_SolveLearned"):
+                  "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
@@ -211,6 +213,35 @@ def analyze(targets, sources, kernel, kernel_params, singularity, approximation,
     if rc != 0:
         raise RuntimeError(f"BaryTree_Analyze failed: {last_error()}")
     return s.as_dict()
+
+
+def analyze_degrees(targets, sources, kernel, kernel_params, singularity, approximation,
+                    compute_type, theta, degree, max_source_leaf, max_target_leaf, degrees=None,
+                    size_check=1.0, beta=-1.0):
+    """BaryTree_AnalyzeDegrees (host only): (stats, features); with degrees,
+    the variable-degree work is built and checked and the stats describe it."""
+    tx, ty, tz = map(_arr, targets[:3])
+    sx, sy, sz = map(_arr, sources[:3])
+    kp = _kp(kernel_params)
+    args = (ctypes.c_int(len(tx)), ctypes.c_int(len(sx)), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(sx),
+            _ptr(sy), _ptr(sz), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp),
+            ctypes.c_int(int(singularity)), ctypes.c_int(int(approximation)),
+            ctypes.c_int(int(compute_type)), ctypes.c_double(theta), ctypes.c_int(degree),
+            ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+            ctypes.c_double(size_check), ctypes.c_double(beta))
+    s = Stats()
+    if lib().BaryTree_AnalyzeDegrees(*args, 0, None, None, ctypes.byref(s)) != 0:
+        raise RuntimeError(f"BaryTree_AnalyzeDegrees failed: {last_error()}")
+    n = s.num_far_pairs
+    f = np.zeros((n, 8), dtype=np.float32)
+    d = None if degrees is None else np.ascontiguousarray(degrees, dtype=np.int32)
+    rc = lib().BaryTree_AnalyzeDegrees(*args, ctypes.c_int(n), None if d is None else
+                                       d.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       ctypes.byref(s))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_AnalyzeDegrees failed: {last_error()}")
+    return s.as_dict(), f
 
 
 def _batch(systems):

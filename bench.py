@@ -8,12 +8,19 @@ cluster-cluster treecode, theta = 0.8, degree 10, max leaf 8000.
 A step is one execution of the GPU path (upward pass, near and far field,
 downpass, correction, scatter to caller order) with its inputs resident in
 HBM; the host work (trees, interaction lists, upload) is done once by the
-plan and reported separately.  Two forms are timed on the same plan: the
-fast form (FP32 pair evaluation with FP64 charges and accumulation) -- the
-one the metric is quoted on -- and the plain FP64 form.  Both are checked
-against an FP64 direct sum at 1000 stride-sampled targets.
+plan and reported separately.  Three forms are timed on the same plan: the
+plain FP64 form; the mixed form (FP32 pair evaluation with FP64 charges and
+accumulation); and the mixed, learned-degree form (each far-field pair at the
+degree the shipped degree model picks) -- the one the metric is quoted on.
+Each is checked against an FP64 direct sum at 1000 stride-sampled targets.
 
 Prints one JSON line on stdout.
+
+--workload learned measures the learned degree selection on its own
+configuration: regularized Coulomb (epsilon 0.01), N = 4M Plummer sphere
+(scale 0.1, truncated at r = 1, seed 4), cluster-particle, theta 0.7, leaf
+2000: fixed degree 10 against the per-pair degrees of the shipped model
+(trained on other Plummer realizations), in FP64 and mixed.
 
 --workload batched measures the batching extension instead: 256 independent
 Coulomb systems (N log-uniform in [1k, 64k], 8-Gaussian mixtures, sigma 0.05,
@@ -196,6 +203,69 @@ def bench_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_learned(args):
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from barytree_amd import learned
+
+    cfg = learned.MODELS["regularized_coulomb_cp"]
+    n = args.n if args.n != 16_000_000 else 4_000_000
+    log(f"[bench] learned: Plummer N={n}")
+    pts = learned.particles("plummer", n, 4)
+    t0 = time.perf_counter()
+    plan = learned.plan_for(cfg, *pts)
+    host_setup = time.perf_counter() - t0
+    model = learned.load_model("regularized_coulomb_cp")
+    idx = np.arange(0, n, n // 1000)[:1000]
+    direct = plan.direct_sample(idx)
+    res = {}
+    for form, prec, use in (("fixed_fp64", bt.PRECISION_FP64, False),
+                            ("fixed_mixed", bt.PRECISION_MIXED, False),
+                            ("learned_fp64", bt.PRECISION_FP64, True),
+                            ("learned_mixed", bt.PRECISION_MIXED, True)):
+        t0 = time.perf_counter()
+        if use:
+            plan.use_model(model, args.delta)
+        else:
+            plan.set_pair_degrees(None)
+        setup = time.perf_counter() - t0
+        wall, gpu, near, st = time_steps(plan, prec, args.steps, args.warmup)
+        out = plan.execute()
+        err = rel_l2(direct, out[idx])
+        far = st["far_ptwise_variable"] if use else st["cp_ptwise"]
+        res[form] = {"s": round(wall, 5), "gpu_s": round(gpu, 5), "rel_l2_err": err,
+                     "far_evals": far, "degree_setup_s": round(setup, 4),
+                     "mean_pair_degree": st["mean_pair_degree"] if use else 10.0,
+                     "phases_s": {k: round(st[k], 5) for k in ("gpu_upward_s", "gpu_near_s",
+                                                               "gpu_far_s", "gpu_down_s")}}
+        log(f"[bench] {form}: {wall:.4f} s/step, relL2 {err:.2e}, far evals {far:.3e}")
+    plan.close()
+    line = {
+        "metric": "time-to-solution, learned per-pair degree vs fixed degree 10 (C4)",
+        "value": res["learned_mixed"]["s"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["learned_mixed"]["s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 pair kernel, f64 charges/accumulation",
+        "data": "synthetic",
+        "config": {"workload": (f"regularized Coulomb eps=0.01, N={n} Plummer (a=0.1, r<=1), "
+                                "cluster-particle theta=0.7, degree 10, leaf 2000"),
+                   "n": n, "compute_type": "CP", "theta": 0.7, "degree": 10, "max_leaf": 2000},
+        "forms": res,
+        "delta": args.delta,
+        "speedup_learned_vs_fixed_fp64": round(res["fixed_fp64"]["s"] / res["learned_fp64"]["s"], 3),
+        "speedup_learned_mixed_vs_fixed_fp64": round(res["fixed_fp64"]["s"] / res["learned_mixed"]["s"], 3),
+        "host_setup_s": round(host_setup, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +274,9 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["large", "batched"], default="large")
+    ap.add_argument("--workload", choices=["large", "batched", "learned"], default="large")
+    ap.add_argument("--delta", type=float, default=0.5,
+                    help="degree model threshold (probability that a lower degree is not enough)")
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (HBM-traffic counter calibration)")
@@ -224,6 +296,9 @@ def main():
 
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "learned":
+        bench_learned(args)
         return
 
     import torch  # device setup and synchronisation only
@@ -252,6 +327,18 @@ def main():
     direct = plan.direct_sample(idx)
     err_fast = rel_l2(direct, out_fast[idx])
     log(f"[bench] mixed relL2 vs FP64 direct (1000 targets): {err_fast:.3e}")
+
+    # mixed, learned-degree form (the metric)
+    from barytree_amd import learned
+    t0 = time.perf_counter()
+    plan.use_model(learned.load_model("coulomb_cc"), args.delta)
+    deg_setup = time.perf_counter() - t0
+    wall_ml, gpu_ml, near_ml, st_ml = time_steps(plan, bt.PRECISION_MIXED, args.steps, args.warmup)
+    out_ml = plan.execute()
+    err_ml = rel_l2(direct, out_ml[idx])
+    log(f"[bench] mixed+learned: {wall_ml:.3f} s/step (gpu {gpu_ml:.3f}, far {st_ml['gpu_far_s']:.3f}, "
+        f"mean degree {st_ml['mean_pair_degree']:.2f}), relL2 {err_ml:.3e}, degree setup {deg_setup:.2f}s")
+    plan.set_pair_degrees(None)
 
     # plain FP64 form on the same plan
     wall_plain, err_plain = None, None
@@ -282,12 +369,12 @@ def main():
 
     line = {
         "metric": "time-to-solution N=16M at rel-L2 err <=1e-6 vs FP64 direct; speedup vs plain",
-        "value": round(wall_fast, 4),
+        "value": round(wall_ml, 4),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall_fast * 1e3, 2),
+        "ms_per_step": round(wall_ml * 1e3, 2),
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
@@ -298,14 +385,21 @@ def main():
                          "cluster-cluster theta=0.8 degree 10 max leaf 8000"),
             "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
         },
-        "rel_l2_err": err_fast,
-        "speedup_vs_plain": round(wall_plain / wall_fast, 3) if wall_plain else None,
+        "form": "mixed precision + learned per-pair degree (shipped coulomb_cc model)",
+        "rel_l2_err": err_ml,
+        "speedup_vs_plain": round(wall_plain / wall_ml, 3) if wall_plain else None,
+        "mixed_fixed_degree_s": round(wall_fast, 4),
+        "mixed_fixed_degree_rel_l2_err": err_fast,
+        "mean_pair_degree": st_ml["mean_pair_degree"],
+        "far_evals_fixed": st["cc_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"],
+        "far_evals_learned": st_ml["far_ptwise_variable"],
+        "degree_setup_s": round(deg_setup, 3),
         "plain_fp64_s": round(wall_plain, 4) if wall_plain else None,
         "plain_fp64_rel_l2_err": err_plain,
         "gpu_time_s": round(gpu_fast, 4),
         "near_kernel_s": round(near_fast, 4),
         "host_setup_s": round(host_setup, 3),
-        "end_to_end_s": round(host_setup + wall_fast, 3),
+        "end_to_end_s": round(host_setup + deg_setup + wall_ml, 3),
         "interactions": evals,
         "interactions_per_s": evals / wall_fast,
         "roofline": roofline,

@@ -258,6 +258,21 @@ int BaryTreePlan_LabelPairs(BaryTreePlan *plan, int first, int count, double tol
  * plan's fixed degree. */
 int BaryTreePlan_SetPairDegrees(BaryTreePlan *plan, const int *degrees);
 
+/* Host only (no GPU work): the far-field pairs of the plan BaryTree_Analyze
+ * would build, their features (optional, BARYTREE_PAIR_FEATURES per pair),
+ * and, if degrees is not NULL (numPairs entries), the variable-degree work
+ * built and checked for those degrees (stats: variable_degree,
+ * mean_pair_degree, far_ptwise_variable).  numPairs may be 0 with degrees
+ * NULL to learn the count (stats->num_far_pairs). */
+int BaryTree_AnalyzeDegrees(int numTargets, int numSources, const double *targetX,
+                            const double *targetY, const double *targetZ, const double *sourceX,
+                            const double *sourceY, const double *sourceZ, int kernel,
+                            int numKernelParams, const double *kernelParams, int singularity,
+                            int approximation, int compute_type, double theta, int interpOrder,
+                            int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck,
+                            double beta, int numPairs, const int *degrees, float *features,
+                            BaryTreeStats *stats);
+
 /* The same labels for explicitly given pairs: pair i has targets
  * [targetBegin[i], +targetCount[i]) and sources [sourceBegin[i],
  * +sourceCount[i]) of the arrays, and node boxes {xmin,xmax,ymin,ymax,zmin,

@@ -158,3 +158,37 @@ def test_batched_argument_errors(bt):
                          max_source_leaf=10, max_target_leaf=10)
     with pytest.raises(RuntimeError, match="numSystems"):
         bt.solve_batched([], *args, max_source_leaf=10, max_target_leaf=10)
+
+
+@pytest.mark.parametrize("ct", [1, 2, 3])
+def test_stats_layout_far_pair_count(bt, ct):
+    # the last fields of BaryTreeStats: far-field pairs = entries of the
+    # approximation lists (PC, CP and CC)
+    t, s = uniform_cube(4000, 3000, seed=12)
+    a = bt.analyze(t, s, 1, [0.5], 1, 1, ct, 0.7, 3, 40, 40)
+    assert a["num_far_pairs"] == a["num_pc"] + a["num_cp"] + a["num_cc"] > 0
+    assert a["variable_degree"] == 0 and a["num_systems"] == 1
+
+
+@pytest.mark.parametrize("ct,k,s", [(1, 1, 1), (1, 3, 2), (2, 2, 1), (2, 3, 1), (3, 1, 2), (3, 2, 1)])
+def test_variable_degree_work_on_host(bt, ct, k, s):
+    # the per-degree work (slots, parts, one near and one far work list,
+    # split rows, downpass tiles) is built and every index checked on the host
+    from tests.helpers import KP
+    t, src = uniform_cube(6000, 5000, seed=70 + ct)
+    args = (k, KP[k][s], s, 1, ct, 0.7, 5, 60, 60)
+    st, f = bt.analyze_degrees(t, src, *args)
+    n = st["num_far_pairs"]
+    assert n == st["num_pc"] + st["num_cp"] + st["num_cc"] > 0
+    assert f.shape == (n, 8) and np.all(np.isfinite(f)) and np.all(f[:, 0] >= 0)
+    kinds = f[:, 4] + 2 * f[:, 5]            # 1 PC, 2 CP, 0 CC
+    assert set(np.unique(kinds)) <= ({1} if ct == 1 else {2} if ct == 2 else {0, 1, 2})
+    # every pair at the plan's degree: the fixed method's far-field work
+    st5, _ = bt.analyze_degrees(t, src, *args, degrees=np.full(n, 5))
+    assert st5["variable_degree"] == 1 and st5["mean_pair_degree"] == 5.0
+    assert st5["far_ptwise_variable"] == st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
+    deg = np.random.default_rng(ct).integers(1, 13, n)
+    stv, _ = bt.analyze_degrees(t, src, *args, degrees=deg)
+    assert np.isclose(stv["mean_pair_degree"], deg.mean())
+    with pytest.raises(RuntimeError, match="degree outside"):
+        bt.analyze_degrees(t, src, *args, degrees=np.full(n, 13))
