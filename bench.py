@@ -77,6 +77,21 @@ def time_steps(plan, precision, steps, warmup):
     return wall / steps, float(np.mean(times)), float(np.mean(near)), st
 
 
+def near_traffic():
+    """HBM bytes per launch of the mixed near-field kernel at the headline
+    size, from the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE,
+    each scaled by the same counter on a copy of known size); None if the
+    summary is absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_near_kernel_coulomb_ss_cc_n16m.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n_evals):
     """Times the oracle (FP64 restatement of the reference kernels, 1 core)
     on a bounded sample and scales to the workload's evaluation count."""
@@ -359,7 +374,7 @@ def main():
         "peak": PEAK["fp32"],
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK["fp32"], 4),
-        "traffic": None,
+        "traffic": near_traffic() if n == 16_000_000 else None,
     }
     cpu = None
     if not args.no_cpu:
