@@ -25,6 +25,8 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              # packed FP32 instructions stay off
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # matrix-core results straight into VGPRs (no AGPR copies per pair)
+             "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-munsafe-fp-atomics"]
 
 

@@ -108,6 +108,10 @@ void make_cc_lists(Lists &L, const Tree &src, const Tree &tgt, const Params &p);
 // y, then z), computed exactly as src/clusters/clusters.c:436-440.
 void node_table(const Tree &t, int degree, std::vector<double> &tab);
 
+// Source-range flags: a source may coincide with a target (the singularity
+// skip may trigger); a pair may come closer than the plan's r2_exact.
+constexpr int SEG_COINCIDE = 1, SEG_CLOSE = 2;
+
 // Device-side work description (CSR), built on the host.
 struct WorkCSR {
     // keys: target leaves (particle targets) or target nodes (grid targets)
@@ -115,7 +119,7 @@ struct WorkCSR {
     std::vector<int> tbeg, tcnt;   // target range (particles) or grid size
     std::vector<int> seg_off;      // CSR into seg_beg/seg_cnt (source particle ranges)
     std::vector<int> seg_beg, seg_cnt;
-    std::vector<char> seg_chk;     // 1: the singularity skip may trigger
+    std::vector<char> seg_chk;     // SEG_COINCIDE | SEG_CLOSE flags
     std::vector<int> clu_off;      // CSR into clu_id (source clusters)
     std::vector<int> clu_id;
     long long pairs = 0;           // point-wise evaluations described

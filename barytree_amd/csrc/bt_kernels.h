@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
 
 namespace btk {
 
@@ -11,6 +12,12 @@ namespace btk {
 // {first, count}; 16- and 8-byte aligned so kernels load them whole.
 struct alignas(16) Tile { int x, y, z, w; };
 struct alignas(8) Seg { int x, y; };
+
+// Seg.y = count | flags: the singularity skip may trigger in the range;
+// a pair of the range may come closer than InteractArgs::r2_exact.
+constexpr unsigned SEG_COINCIDE_BIT = 0x80000000u;
+constexpr unsigned SEG_CLOSE_BIT = 0x40000000u;
+constexpr unsigned SEG_COUNT_MASK = 0x3fffffffu;
 
 // Tile lists are 8 interleaved queues (tile 8*j + x = j-th tile of queue x),
 // one per XCD of an MI355X.
@@ -61,6 +68,12 @@ struct InteractArgs {
     const double *dcx[MAX_DEGREE + 1], *dcy[MAX_DEGREE + 1], *dcz[MAX_DEGREE + 1];
     const double *dcq[MAX_DEGREE + 1], *dcw[MAX_DEGREE + 1];
     const double *dtab[MAX_DEGREE + 1];  // target slot node tables
+    // particle targets, mixed precision: when key_ctr is set, r^2 is formed on
+    // the FP64 matrix cores from coordinates relative to the key's centre
+    // key_ctr[k] (x, y, z, unused); pairs with r^2 < r2_exact are recomputed
+    // from the coordinate differences
+    const double4 *key_ctr;
+    double r2_exact;
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);

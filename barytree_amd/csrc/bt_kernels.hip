@@ -152,6 +152,31 @@ __device__ __forceinline__ void pair_pot32(float r2, double a, float af, float b
     }
 }
 
+// Subtraction kernels in the mixed near field with coordinates pre-scaled by
+// lam (lam^2 = log2(e)/kappa^2 for Coulomb, log2(e)/alpha^2 for regularized
+// Coulomb; lam = kappa*log2(e) for Yukawa) and charges a, b pre-multiplied by
+// lam, so that the exponentials take their argument without a multiply:
+//   Coulomb : (a - tq b 2^(-r'^2)) / r'          = (a - tq b e^(-r^2/k^2)) / r
+//   Yukawa  : (a - tq b) 2^(-r') / r'            = (a - tq b) e^(-k r) / r
+//   RegCoul : (a - tq b 2^(-r'^2)) / sqrt(r'^2 + e'^2)
+// (r' = lam r, e'^2 = lam^2 eps^2).  Returns the FP32 term.
+template <int KER, bool CHECK>
+__device__ __forceinline__ float pair_ss32(float r2s, float af, float bf, float tqf, float eps2s)
+{
+    if constexpr (KER == KC) {
+        float ir = rsq32(r2s);
+        if constexpr (CHECK) ir = r2s > 0.0f ? ir : 0.0f;
+        return fmaf(-tqf * bf, exp2_32(-r2s), af) * ir;
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2s);
+        float e = exp2_32(-(r2s * ir)) * ir;
+        if constexpr (CHECK) e = r2s > 0.0f ? e : 0.0f;
+        return fmaf(-tqf, bf, af) * e;
+    } else {
+        return fmaf(-tqf * bf, exp2_32(-r2s), af) * rsq32(r2s + eps2s);
+    }
+}
+
 template <int KER, bool SS>
 __device__ __forceinline__ void pair_grid32(float r2, double a, double b, const KP32 &k, double &q,
                                             double &w)
@@ -181,7 +206,6 @@ __device__ __forceinline__ void pair_grid32(float r2, double a, double b, const 
 // MIXED selects the FP32 pair kernels above; everything else (staging,
 // targets, totals) is FP64.
 // ---------------------------------------------------------------------
-constexpr unsigned SEG_CHECK = 0x80000000u;  // Seg.y flag: apply the singularity skip
 
 template <int KER, bool SS, bool GRID, bool MIXED, int TPT>
 __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile tile, double4 *Ls,
@@ -229,11 +253,44 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         Tqf[u] = (float)Tq[u];
         acc[u] = 0.0; accw[u] = 0.0;
     }
+    // mixed subtraction kernels on particle targets: pre-scaled coordinates
+    // and charges (pair_ss32), FP32 sums over blocks of 16 sources
+    constexpr bool SCALED = MIXED && SS && !GRID;
+    double lam = 1.0;
+    float eps2s = 0.0f;
+    if constexpr (SCALED) {
+        lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : sqrt(1.4426950408889634 * A.kp.inv_a2);
+        eps2s = (float)(lam * lam * A.kp.eps2);
+#pragma unroll
+        for (int u = 0; u < TPT; u++) { Tx[u] *= lam; Ty[u] *= lam; Tz[u] *= lam; }
+    }
 
     // n staged sources against the first NU target slots
     auto consume_nu = [&](int n, auto check_tag, auto nu_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
         constexpr int NU = decltype(nu_tag)::value;
+        if constexpr (SCALED) {
+            for (int jb = 0; jb < n; jb += 16) {
+                const int je = min(jb + 16, n);
+                float part[NU];
+#pragma unroll
+                for (int u = 0; u < NU; u++) part[u] = 0.0f;
+#pragma unroll 2
+                for (int j = jb; j < je; ++j) {
+                    const double4 s = Ls[j];
+                    const float2 f = Lf[j];
+#pragma unroll
+                    for (int u = 0; u < NU; u++) {
+                        const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                        const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
+                        part[u] += pair_ss32<KER, CHECK>(r2, f.x, f.y, Tqf[u], eps2s);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < NU; u++) acc[u] += (double)part[u];
+            }
+            return;
+        }
 #pragma unroll 2
         for (int j = 0; j < n; ++j) {
             const double4 s = Ls[j];
@@ -284,11 +341,11 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
             if (tid < n) {
                 const size_t g = base + off + tid;
                 const double a = cq[g];
-                Ls[tid] = make_double4(cx[g], cy[g], cz[g], a);
+                Ls[tid] = make_double4(lam * cx[g], lam * cy[g], lam * cz[g], a);
                 if constexpr (SS) {
                     const double b = cw[g];
                     Lb[tid] = b;
-                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)a, (float)b);
+                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
                 }
             }
             __syncthreads();
@@ -299,8 +356,8 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
-        const bool check = !GRID && ((unsigned)sg.y & SEG_CHECK);
-        const int cnt = (int)((unsigned)sg.y & ~SEG_CHECK);
+        const bool check = !GRID && ((unsigned)sg.y & SEG_COINCIDE_BIT);
+        const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
         for (int off = 0; off < cnt; off += TPB) {
             const int n = min(TPB, cnt - off);
             __syncthreads();
@@ -309,10 +366,10 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
                 const double q = A.sq[g];
                 double a = q, b = 0.0;
                 if constexpr (SS) { b = A.sw[g]; a = q * b; }
-                Ls[tid] = make_double4(A.sx[g], A.sy[g], A.sz[g], a);
+                Ls[tid] = make_double4(lam * A.sx[g], lam * A.sy[g], lam * A.sz[g], a);
                 if constexpr (SS) {
                     Lb[tid] = b;
-                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)a, (float)b);
+                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
                 }
             }
             __syncthreads();
@@ -379,6 +436,239 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
     }
 }
 
+// ---------------------------------------------------------------------
+// Near field in mixed precision with r^2 on the matrix cores.  Each wave
+// owns 16*NB targets of the tile (NB row blocks of 16).  For a chunk of 16
+// staged sources, v_mfma_f64_16x16x4f64 forms D = A B + C with
+//   A row i  = [-2(t_i - c), 1]      (lane l supplies A[l%16][l/16])
+//   B col j  = [s_j - c, |s_j - c|^2] (lane l supplies B[l/16][l%16])
+//   C[i][j]  = |t_i - c|^2
+// so D[i][j] = |t_i - s_j|^2 in FP64, c the key's centre; lane l holds
+// D[l/16 + 4v][l%16], v = 0..3.  The FP64 rounding error of D is below
+// 6e-15 (|t-c|^2 + |s-c|^2), so pairs with D < r2_exact (set from the
+// domain size) take the coordinate differences instead, and a coincident
+// pair (r^2 = 0) is skipped as the reference skips it.  The VALU then only
+// rounds r^2 to FP32, evaluates the pair kernel and accumulates in FP64.
+// Each target's sum is spread over the 16 lanes of its row group and added
+// with a butterfly at the end of the tile.
+// ---------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int KER, bool SS, int NB>
+__device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile tile, double *LB,
+                                               double *LA, float2 *LC, double *LT)
+{
+    constexpr int WT = 16 * NB;  // targets per wave
+    constexpr int TT = 4 * WT;   // targets per tile
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int kk = lane >> 4, ii = lane & 15;
+    const int key = tile.x;
+    const double4 ctr = A.key_ctr[key];
+    const KP32 k32{(float)(-1.4426950408889634 * A.kp.inv_a2), (float)(-1.4426950408889634 * A.kp.p0),
+                   (float)A.kp.eps2};
+    // centred targets of the tile (LT[d*TT + i])
+    for (int i = tid; i < TT; i += TPB) {
+        const int g = tile.y + (i < tile.z ? i : 0);
+        LT[i] = A.tx[g] - ctr.x;
+        LT[TT + i] = A.ty[g] - ctr.y;
+        LT[2 * TT + i] = A.tz[g] - ctr.z;
+    }
+    __syncthreads();
+    const int tb = wv * WT;  // first target of this wave in the tile
+    const int nbv = __builtin_amdgcn_readfirstlane(max(0, min(NB, (tile.z - tb + 15) / 16)));
+    double Aop[NB];
+    v4d Cop[NB];
+    float tq[NB][4];
+    double acc[NB][4];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int ra = tb + 16 * b + ii;
+        Aop[b] = kk < 3 ? -2.0 * LT[kk * TT + ra] : 1.0;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = tb + 16 * b + kk + 4 * v;
+            const double x = LT[r], y = LT[TT + r], z = LT[2 * TT + r];
+            Cop[b][v] = fma(z, z, fma(y, y, x * x));
+            tq[b][v] = SS ? (float)A.tq[tile.y + (r < tile.z ? r : 0)] : 0.0f;
+            acc[b][v] = 0.0;
+        }
+    }
+
+    // staged sources: LB[d*TPB + j] = centred coordinate d (d = 3: |s-c|^2),
+    // LA[j] = FP64 charge term, LC[j] = (a, b) in FP32.  Entries past n are
+    // far dummies with zero charge, so 16-wide chunks need no tail handling.
+    auto put = [&](int n, double x, double y, double z, double a, double bw) {
+        const bool real = tid < n;
+        const double dx = real ? x - ctr.x : 1.0e3, dy = real ? y - ctr.y : 0.0, dz = real ? z - ctr.z : 0.0;
+        LB[tid] = dx; LB[TPB + tid] = dy; LB[2 * TPB + tid] = dz;
+        LB[3 * TPB + tid] = fma(dz, dz, fma(dy, dy, dx * dx));
+        a = real ? a : 0.0;
+        bw = real ? bw : 0.0;
+        LA[tid] = a;
+        LC[tid] = make_float2((float)a, (float)bw);
+    };
+    auto consume = [&](int n, bool check) {
+        if (nbv == 0) return;  // no target of this wave in the tile
+        for (int sc = 0; sc < n; sc += 16) {
+            const int j = sc + ii;
+            const double bop = LB[kk * TPB + j];
+            const float2 f = LC[j];
+            const double a64 = LA[j];
+            // every row block, valid or not (rows past the tile compute on a
+            // clamped target and are never stored): no branches in the chunk
+            v4d D[NB];
+#pragma unroll
+            for (int b = 0; b < NB; b++) D[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[b], bop, Cop[b], 0, 0, 0);
+            if (check) {
+                // one vote per chunk; the rare close pairs are fixed before the
+                // branch-free evaluation below
+                bool close = false;
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) close |= D[b][v] < A.r2_exact;
+                if (__any(close)) {
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            if (D[b][v] < A.r2_exact) {
+                                const int r = tb + 16 * b + kk + 4 * v;
+                                const double ex = LT[r] - LB[j], ey = LT[TT + r] - LB[TPB + j],
+                                             ez = LT[2 * TT + r] - LB[2 * TPB + j];
+                                const double r2 = fma(ez, ez, fma(ey, ey, ex * ex));
+                                // coincident: no contribution (rsq of FLT_MAX and the
+                                // exponentials below underflow to zero)
+                                D[b][v] = r2 > 0.0 ? r2 : 3.4e38;
+                            }
+                        }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    pair_pot32<KER, SS, false>((float)D[b][v], a64, f.x, f.y, tq[b][v], k32, acc[b][v]);
+        }
+    };
+    // a list of `total` source points in chunks of TPB; the next chunk's loads
+    // are issued before the current chunk is consumed
+    auto stream = [&](int total, bool check, auto &&load) {
+        double x = 0.0, y = 0.0, z = 0.0, a = 0.0, bw = 0.0;
+        if (tid < total) load(tid, x, y, z, a, bw);
+        for (int off = 0; off < total; off += TPB) {
+            const int n = min(TPB, total - off);
+            __syncthreads();
+            put(n, x, y, z, a, bw);
+            __syncthreads();
+            if (off + TPB + tid < total) load(off + TPB + tid, x, y, z, a, bw);
+            consume(n, check);
+        }
+    };
+
+    // source clusters
+    const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
+    for (int c = c0; c < c1; ++c) {
+        const int id = A.clu_id[c];
+        int P = A.P;
+        const double *cx = A.cx, *cy = A.cy, *cz = A.cz, *cq = A.cq, *cw = A.cw;
+        size_t base;
+        if (A.multi) {
+            const int dg = id >> DEG_SHIFT;
+            P = (dg + 1) * (dg + 1) * (dg + 1);
+            cx = A.dcx[dg]; cy = A.dcy[dg]; cz = A.dcz[dg]; cq = A.dcq[dg]; cw = A.dcw[dg];
+            base = (size_t)(id & SLOT_MASK) * P;
+        } else {
+            base = (size_t)id * P;
+        }
+        stream(P, false, [&](int t, double &x, double &y, double &z, double &a, double &bw) {
+            const size_t g = base + t;
+            x = cx[g]; y = cy[g]; z = cz[g]; a = cq[g];
+            if constexpr (SS) bw = cw[g];
+        });
+    }
+    // source particle ranges
+    const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
+    for (int s = s0; s < s1; ++s) {
+        const Seg sg = A.segs[s];
+        // coordinate differences below r2_exact (and the skip of coincident
+        // pairs, which are closer still); the regularized kernel has neither
+        // a singularity nor a sensitivity to r^2 below eps^2
+        const bool check = KER != KR && ((unsigned)sg.y & (SEG_COINCIDE_BIT | SEG_CLOSE_BIT)) != 0;
+        const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
+        stream(cnt, check, [&](int t, double &x, double &y, double &z, double &a, double &bw) {
+            const int g = sg.x + t;
+            x = A.sx[g]; y = A.sy[g]; z = A.sz[g];
+            const double q = A.sq[g];
+            a = q;
+            if constexpr (SS) { bw = A.sw[g]; a = q * bw; }
+        });
+    }
+    // each target's sum: its 16 lanes, butterfly
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double t = acc[b][v];
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 16);
+            const int r = tb + 16 * b + kk + 4 * v;
+            if (ii == 0 && b < nbv && r < tile.z) {
+                const int g = tile.y + r;
+                if (A.accumulate) A.out1[g] += t;
+                else A.out1[g] = t;
+            }
+        }
+    }
+}
+
+template <int KER, bool SS, int NB>
+__global__ __launch_bounds__(TPB) void k_near_mfma(InteractArgs A)
+{
+    constexpr int TT = 4 * 16 * NB;  // targets per pass
+    __shared__ double LB[4 * TPB];
+    __shared__ double LA[TPB];
+    __shared__ float2 LC[TPB];
+    __shared__ double LT[3 * TT];
+    __shared__ int Lt;
+    const int home = blockIdx.x % NQ;
+    // one-shot: the block's own tile; persistent: tiles popped from the
+    // queues (as k_interact) until every queue is empty
+    for (int round = 0;; round++) {
+        int t;
+        if (!A.persistent) {
+            if (round > 0) break;
+            t = blockIdx.x;
+        } else {
+            if (threadIdx.x == 0) {
+                int u = -1;
+                for (int k = 0; k < NQ && u < 0; k++) {
+                    const int q = (home + k) % NQ;
+                    const int len = A.qlen[q];
+                    if (__hip_atomic_load(&A.cursor[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len)
+                        continue;
+                    const int j = atomicAdd(&A.cursor[q], 1);
+                    if (j < len) u = j * NQ + q;
+                }
+                Lt = u;
+            }
+            __syncthreads();
+            t = Lt;
+            __syncthreads();
+            if (t < 0) break;
+        }
+        const Tile tile = A.tiles[t];
+        // a tile larger than one pass runs as consecutive parts
+        for (int h = 0; h * TT < tile.z; h++) {
+            Tile part = tile;
+            part.y = tile.y + h * TT;
+            part.z = min(TT, tile.z - h * TT);
+            near_mfma_tile<KER, SS, NB>(A, part, LB, LA, LC, LT);
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void k_zero_ints(int *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
@@ -418,9 +708,37 @@ hipError_t interact_launch(const InteractArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int KER, bool SS, int NB>
+hipError_t near_mfma_launch(const InteractArgs &a, hipStream_t st)
+{
+    auto kern = k_near_mfma<KER, SS, NB>;
+    if (!a.persistent) {
+        kern<<<dim3(a.ntiles), dim3(TPB), 0, st>>>(a);
+        return hipGetLastError();
+    }
+    int real = 0;
+    for (int q = 0; q < NQ; q++) {
+        if (a.qlen[q] < 0 || (a.qlen[q] > 0 && (long long)(a.qlen[q] - 1) * NQ + q >= a.ntiles))
+            return hipErrorInvalidValue;
+        real += a.qlen[q];
+    }
+    if (!a.cursor) return hipErrorInvalidValue;
+    const int nb = persistent_blocks(kern, std::max(real, 1));
+    if (nb <= 0) return hipErrorInvalidValue;
+    k_zero_ints<<<1, 64, 0, st>>>(a.cursor, NQ);
+    kern<<<dim3(nb), dim3(TPB), 0, st>>>(a);
+    return hipGetLastError();
+}
+
 template <int KER, bool SS>
 hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
 {
+    // the matrix-core near field pays where the pair kernel is short (the
+    // skipping kernels); the subtraction kernels stay on the VALU kernel
+    if (!a.grid_targets && a.precision == 1 && a.key_ctr && !SS) {
+        // 4 row blocks of 16 targets per wave: 256 targets per pass of a tile
+        return near_mfma_launch<KER, SS, 4>(a, st);
+    }
     if (a.grid_targets) {
         if (a.precision == 1) return interact_launch<KER, SS, true, true, GRID_TPT>(a, st);
         return interact_launch<KER, SS, true, false, GRID_TPT>(a, st);

@@ -165,34 +165,62 @@ void Plan::build_work()
         w.seg_off.push_back((int)w.seg_beg.size());
         w.clu_off.push_back((int)w.clu_id.size());
     };
-    auto add_seg = [](WorkCSR &w, int b, int c, long long nt, bool check = false) {
-        w.seg_beg.push_back(b); w.seg_cnt.push_back(c); w.seg_chk.push_back(check ? 1 : 0);
+    auto add_seg = [](WorkCSR &w, int b, int c, long long nt, int flags = 0) {
+        w.seg_beg.push_back(b); w.seg_cnt.push_back(c); w.seg_chk.push_back((char)flags);
         w.pairs += nt * c;
     };
     auto add_clu = [P](WorkCSR &w, int id, long long nt) { w.clu_id.push_back(id); w.pairs += nt * P; };
     near_ = WorkCSR();
     grid_ = WorkCSR();
+    near_ctr_.clear();
+    r2_exact_ = 0.0;
     near_.seg_off.push_back(0); near_.clu_off.push_back(0);
     grid_.seg_off.push_back(0); grid_.clu_off.push_back(0);
 
     for (const System &sy : sys_) {
         const Tree &st = sy.stree, &tt = sy.ttree;
         const Lists &L = sy.L;
+        double d2sys = 0.0;
+        // mixed near field on the matrix cores: r^2 = |t-c|^2 + |s-c|^2 - 2(t-c).(s-c)
+        // carries an FP64 rounding error below 6e-15 (|t-c|^2 + |s-c|^2) <= 1.2e-14 D^2
+        // (D the system's diameter); below r2_exact = 6e-8 D^2 that is more than a
+        // 2e-7 relative error, so such pairs take the coordinate differences
+        {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (const Tree *T : {&st, &tt})
+                for (int i = 0; i < T->n; i++) {
+                    lo[0] = std::min(lo[0], T->xmin[i]); hi[0] = std::max(hi[0], T->xmax[i]);
+                    lo[1] = std::min(lo[1], T->ymin[i]); hi[1] = std::max(hi[1], T->ymax[i]);
+                    lo[2] = std::min(lo[2], T->zmin[i]); hi[2] = std::max(hi[2], T->zmax[i]);
+                }
+            for (int d = 0; d < 3; d++) d2sys += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+            r2_exact_ = std::max(r2_exact_, 6e-8 * d2sys);
+        }
+        auto add_ctr = [&](int t) {
+            near_ctr_.push_back(tt.xmid[t]); near_ctr_.push_back(tt.ymid[t]);
+            near_ctr_.push_back(tt.zmid[t]); near_ctr_.push_back(0.0);
+        };
         const int so = sy.soff, to = sy.toff, sn = sy.snode_off, tn = sy.tnode_off;
-        // May a source of node s coincide with a target of node t?  The
-        // singularity skip can only trigger when their closed boxes overlap
-        // (up to a 1e-150 gap: any pair across a wider gap has r^2 > DBL_MIN).
-        auto touch = [&](int t, int s) {
-            const double g = 1e-150;
+        // Flags of a source range (node s) for a target node t: SEG_COINCIDE
+        // if a source may coincide with a target (closed boxes within 1e-150:
+        // the singularity skip can only trigger there), SEG_CLOSE if a pair may
+        // come closer than sqrt(r2_exact) (the matrix-core r^2 of the mixed
+        // near field is replaced by coordinate differences below r2_exact).
+        const double gclose = std::max(1e-150, std::sqrt(6e-8 * d2sys));
+        auto within = [&](int t, int s, double g) {
             return tt.xmin[t] - st.xmax[s] <= g && st.xmin[s] - tt.xmax[t] <= g &&
                    tt.ymin[t] - st.ymax[s] <= g && st.ymin[s] - tt.ymax[t] <= g &&
                    tt.zmin[t] - st.zmax[s] <= g && st.zmin[s] - tt.zmax[t] <= g;
+        };
+        auto touch = [&](int t, int s) {
+            return (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0);
         };
         if (p.compute_type == C_PC) {
             // keys: target batches; pc list -> clusters, pp list -> source leaves
             for (int b = 0; b < tt.n; b++) {
                 const long long nt = tt.numpar[b];
                 add_key(near_, tn + b, to + tt.ibeg[b], tt.numpar[b]);
+                add_ctr(b);
                 for (int c : L.approx[b]) add_clu(near_, sn + c, nt);
                 for (int s : L.pp[b]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(b, s));
                 close_key(near_);
@@ -207,6 +235,7 @@ void Plan::build_work()
             for (int leaf : tt.leaves) {
                 const long long nt = tt.numpar[leaf];
                 add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
+                add_ctr(leaf);
                 for (int b : pp_of[leaf]) add_seg(near_, so + st.ibeg[b], st.numpar[b], nt, touch(leaf, b));
                 close_key(near_);
             }
@@ -222,6 +251,7 @@ void Plan::build_work()
             for (int leaf : tt.leaves) {
                 const long long nt = tt.numpar[leaf];
                 add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
+                add_ctr(leaf);
                 for (int a = leaf; a >= 0; a = tt.parent[a]) {
                     for (int c : L.cc_pc[a]) add_clu(near_, sn + c, nt);
                     for (int s : L.pp[a]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(leaf, s));
@@ -327,7 +357,8 @@ std::string Plan::validate() const
         for (int k = 0; k < nk; k++)
             if (w.seg_off[k] > w.seg_off[k + 1] || w.clu_off[k] > w.clu_off[k + 1]) return "csr order";
         for (size_t i = 0; i < w.seg_beg.size(); i++)
-            if (w.seg_beg[i] < 0 || w.seg_cnt[i] < 0 || (long long)w.seg_beg[i] + w.seg_cnt[i] > N_)
+            if (w.seg_beg[i] < 0 || w.seg_cnt[i] < 0 || (long long)w.seg_beg[i] + w.seg_cnt[i] > N_ ||
+                (unsigned)w.seg_cnt[i] > btk::SEG_COUNT_MASK)
                 return "source range";
         for (int c : w.clu_id)
             if (c < 0 || c >= nsnodes_) return "cluster id";
@@ -354,6 +385,7 @@ std::string Plan::validate() const
     if (!chk_q(near_tiles_, near_qlen_) || !chk_q(grid_tiles_, grid_qlen_)) return "tile queues";
     std::string e = chk_csr(near_, near_tiles_, false);
     if (!e.empty()) return "near " + e;
+    if (near_ctr_.size() != 4 * near_.key.size()) return "near centres";
     if (!(e = chk_csr(grid_, grid_tiles_, true)).empty()) return "far " + e;
     if (grid_row_.size() != grid_.key.size() || (int)node_rows_off_.size() != ntnodes_ + 1 ||
         node_rows_off_[ntnodes_] != (int)grid_.key.size())
@@ -629,7 +661,8 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         std::vector<btk::Seg> segs(w.seg_beg.size());
         for (size_t i = 0; i < segs.size(); i++) {
             segs[i].x = w.seg_beg[i];
-            segs[i].y = (int)((unsigned)w.seg_cnt[i] | (w.seg_chk[i] ? 0x80000000u : 0u));
+            segs[i].y = (int)((unsigned)w.seg_cnt[i] | ((w.seg_chk[i] & SEG_COINCIDE) ? btk::SEG_COINCIDE_BIT : 0u) |
+                              ((w.seg_chk[i] & SEG_CLOSE) ? btk::SEG_CLOSE_BIT : 0u));
         }
         std::string rr;
         if (!(rr = up(so, w.seg_off)).empty()) return rr;
@@ -639,6 +672,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     };
     if (!(r = up_csr(near_, &d_nseg_off_, &d_nsegs_, &d_nclu_off_, &d_nclu_id_)).empty()) return r;
     if (!(r = up(&d_near_tiles_, near_tiles_)).empty()) return r;
+    if (!(r = up(&d_near_ctr_, near_ctr_)).empty()) return r;
     if (!grid_tiles_.empty()) {
         if (!(r = up_csr(grid_, &d_gseg_off_, &d_gsegs_, &d_gclu_off_, &d_gclu_id_)).empty()) return r;
         if (!(r = up(&d_grid_tiles_, grid_tiles_)).empty()) return r;
@@ -707,6 +741,10 @@ std::string Plan::execute(bool timed)
         a.grid_targets = 0; a.ntiles = (int)near_tiles_.size(); a.tiles = d_near_tiles_;
         a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_nclu_off_; a.clu_id = d_nclu_id_;
         a.out1 = d_pot_;
+        if (p.precision == PREC_MIXED) {
+            a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
+            a.r2_exact = r2_exact_;
+        }
         std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));

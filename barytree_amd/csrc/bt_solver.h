@@ -175,6 +175,8 @@ private:
     std::vector<btk::Tile> near_tiles_, grid_tiles_, leaf_tiles_;
     // grid keys write rows of partial grid values (row of key k at
     // grid_row_[k]); node n sums rows [node_rows_off_[n], node_rows_off_[n+1])
+    std::vector<double> near_ctr_;  // per near key: centre of its target node (x, y, z, 0)
+    double r2_exact_ = 0;           // mixed near field: pairs below this r^2 use coordinate differences
     std::vector<long long> grid_row_;
     std::vector<int> node_rows_off_;
     long long grid_rows_len_ = 0;
@@ -230,6 +232,7 @@ private:
     btk::Seg *d_gsegs_ = nullptr;
     double *d_pot_ = nullptr, *d_out_ = nullptr;
     int *d_cursor_ = nullptr;  // tile-queue cursors: near, then far
+    double *d_near_ctr_ = nullptr;
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;
     double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;

@@ -156,3 +156,37 @@ def test_against_reference_outputs(bt):
         t, s = uniform_cube(N, M, seed)
         got = bt.solve(t, s, *args)
         assert rel_l2(g[name], got) <= TOL_FP64, name
+
+
+@pytest.mark.parametrize("ct,k,s", [(1, 1, 2), (3, 1, 2), (2, 2, 1), (1, 3, 1), (3, 2, 2)])
+def test_mixed_near_field_close_and_coincident_pairs(bt, oracle_mod, ct, k, s):
+    # targets that coincide with sources (skipped) and targets 1e-9 and 1e-6
+    # away from a source: the mixed near field must recompute such pairs from
+    # coordinate differences instead of the expanded r^2 of the matrix cores
+    print(f"case ct={ct} k={k} s={s}", flush=True)
+    t, src = uniform_cube(6000, 4000, seed=31)
+    rng = np.random.default_rng(5)
+    pick = rng.choice(6000, 1500, replace=False)
+    for j, i in enumerate(pick[:2000]):
+        d = 0.0 if j % 3 == 0 else (1e-9 if j % 3 == 1 else 1e-6)
+        t[0][j] = src[0][i] + d
+        t[1][j] = src[1][i]
+        t[2][j] = src[2][i] - d
+    args = (k, KP[k][s], s, 1, ct, 0.7, 4, 80, 80)
+    ref = oracle_mod.treecode(*t, *src, *args)
+    got = bt.solve(t, src, *args, precision=bt.PRECISION_MIXED)
+    err = rel_l2(ref, got)
+    # at a planted close target the close pair dominates; its FP32 term has an
+    # error of a few 1e-7 of the magnitude of its parts: |q_s w_s| (+ |q_t w_s|
+    # for subtraction) over r
+    worst = 0.0
+    for j, i in enumerate(pick[:1500]):
+        if j % 3 == 0:
+            continue
+        r = np.sqrt(2.0) * (1e-9 if j % 3 == 1 else 1e-6)
+        mag = abs(src[3][i] * (src[4][i] if s == 2 else 1.0))
+        if s == 2:
+            mag += abs(t[3][j] * src[4][i])
+        worst = max(worst, abs(ref[j] - got[j]) / (mag / r))
+    print(f"relL2 {err:.2e}, worst close-pair error / term magnitude {worst:.2e}")
+    assert err <= 1e-6 and worst <= 2e-6
