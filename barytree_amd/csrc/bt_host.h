@@ -109,7 +109,7 @@ void make_cc_lists(Lists &L, const Tree &src, const Tree &tgt, const Params &p);
 void node_table(const Tree &t, int degree, std::vector<double> &tab);
 
 // Source-range flags: a source may coincide with a target (the singularity
-// skip may trigger); a pair may come closer than the plan's r2_exact.
+// skip may trigger); a pair may come closer than its key's threshold.
 constexpr int SEG_COINCIDE = 1, SEG_CLOSE = 2;
 
 // Device-side work description (CSR), built on the host.

@@ -14,7 +14,7 @@ struct alignas(16) Tile { int x, y, z, w; };
 struct alignas(8) Seg { int x, y; };
 
 // Seg.y = count | flags: the singularity skip may trigger in the range;
-// a pair of the range may come closer than InteractArgs::r2_exact.
+// a pair of the range may come closer than its key's threshold (key_ctr).
 constexpr unsigned SEG_COINCIDE_BIT = 0x80000000u;
 constexpr unsigned SEG_CLOSE_BIT = 0x40000000u;
 constexpr unsigned SEG_COUNT_MASK = 0x3fffffffu;
@@ -68,12 +68,12 @@ struct InteractArgs {
     const double *dcx[MAX_DEGREE + 1], *dcy[MAX_DEGREE + 1], *dcz[MAX_DEGREE + 1];
     const double *dcq[MAX_DEGREE + 1], *dcw[MAX_DEGREE + 1];
     const double *dtab[MAX_DEGREE + 1];  // target slot node tables
-    // particle targets, mixed precision: when key_ctr is set, r^2 is formed on
-    // the FP64 matrix cores from coordinates relative to the key's centre
-    // key_ctr[k] (x, y, z, unused); pairs with r^2 < r2_exact are recomputed
-    // from the coordinate differences
+    // particle targets, mixed precision (required there): r^2 is formed as
+    // |t-c|^2 + |s-c|^2 - 2 (t-c).(s-c) from coordinates relative to the key's
+    // centre c = key_ctr[k].xyz; pairs with r^2 below |key_ctr[k].w| (in
+    // ranges flagged SEG_CLOSE, and in the key's clusters if .w < 0) are
+    // recomputed from the coordinate differences
     const double4 *key_ctr;
-    double r2_exact;
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);

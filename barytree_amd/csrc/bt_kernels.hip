@@ -159,21 +159,27 @@ __device__ __forceinline__ void pair_pot32(float r2, double a, float af, float b
 //   Coulomb : (a - tq b 2^(-r'^2)) / r'          = (a - tq b e^(-r^2/k^2)) / r
 //   Yukawa  : (a - tq b) 2^(-r') / r'            = (a - tq b) e^(-k r) / r
 //   RegCoul : (a - tq b 2^(-r'^2)) / sqrt(r'^2 + e'^2)
-// (r' = lam r, e'^2 = lam^2 eps^2).  Returns the FP32 term.
+// (r' = lam r, e'^2 = lam^2 eps^2).  The two parts are summed apart, p1 the
+// a terms and p2 the b terms, and the target combines them as p1 - tq p2
+// (one instruction less per pair than forming the difference per pair).
 template <int KER, bool CHECK>
-__device__ __forceinline__ float pair_ss32(float r2s, float af, float bf, float tqf, float eps2s)
+__device__ __forceinline__ void pair_ss32(float r2s, float af, float bf, float eps2s, float &p1, float &p2)
 {
     if constexpr (KER == KC) {
         float ir = rsq32(r2s);
         if constexpr (CHECK) ir = r2s > 0.0f ? ir : 0.0f;
-        return fmaf(-tqf * bf, exp2_32(-r2s), af) * ir;
+        p1 = fmaf(af, ir, p1);
+        p2 = fmaf(bf, exp2_32(-r2s) * ir, p2);
     } else if constexpr (KER == KY) {
         const float ir = rsq32(r2s);
         float e = exp2_32(-(r2s * ir)) * ir;
         if constexpr (CHECK) e = r2s > 0.0f ? e : 0.0f;
-        return fmaf(-tqf, bf, af) * e;
+        p1 = fmaf(af, e, p1);
+        p2 = fmaf(bf, e, p2);
     } else {
-        return fmaf(-tqf * bf, exp2_32(-r2s), af) * rsq32(r2s + eps2s);
+        const float ir = rsq32(r2s + eps2s);
+        p1 = fmaf(af, ir, p1);
+        p2 = fmaf(bf, exp2_32(-r2s) * ir, p2);
     }
 }
 
@@ -253,16 +259,32 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         Tqf[u] = (float)Tq[u];
         acc[u] = 0.0; accw[u] = 0.0;
     }
-    // mixed subtraction kernels on particle targets: pre-scaled coordinates
-    // and charges (pair_ss32), FP32 sums over blocks of 16 sources
+    // Mixed subtraction kernels on particle targets: coordinates relative to
+    // the key's centre c and pre-scaled by lam (pair_ss32), T' = lam (t - c)
+    // and S' = lam (s - c); r'^2 = |T'|^2 + |S'|^2 - 2 T'.S' (four FP64
+    // instructions; T[xyz] hold -2 T', T2 holds |T'|^2, a staged source's w
+    // holds |S'|^2).  Its rounding error is below 1e-15 (|T'|^2 + |S'|^2), so
+    // in ranges flagged SEG_CLOSE a block of 16 sources with a pair below the
+    // key's threshold (lam^2 |key_ctr.w|, less e'^2 for the regularized
+    // kernel) is evaluated again from coordinate differences, with the skip of
+    // coincident pairs.  FP32 sums over blocks of 16 sources.
     constexpr bool SCALED = MIXED && SS && !GRID;
-    double lam = 1.0;
+    double lam = 1.0, th = 0.0, T2[TPT];
     float eps2s = 0.0f;
+    double4 ctr = make_double4(0.0, 0.0, 0.0, 0.0);
+    bool can_close = false;
     if constexpr (SCALED) {
         lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : sqrt(1.4426950408889634 * A.kp.inv_a2);
         eps2s = (float)(lam * lam * A.kp.eps2);
+        ctr = A.key_ctr[key];
+        th = lam * lam * (fabs(ctr.w) - (KER == KR ? A.kp.eps2 : 0.0));
+        can_close = th > 0.0;
 #pragma unroll
-        for (int u = 0; u < TPT; u++) { Tx[u] *= lam; Ty[u] *= lam; Tz[u] *= lam; }
+        for (int u = 0; u < TPT; u++) {
+            const double x = lam * (Tx[u] - ctr.x), y = lam * (Ty[u] - ctr.y), z = lam * (Tz[u] - ctr.z);
+            T2[u] = fma(z, z, fma(y, y, x * x));
+            Tx[u] = -2.0 * x; Ty[u] = -2.0 * y; Tz[u] = -2.0 * z;
+        }
     }
 
     // n staged sources against the first NU target slots
@@ -270,24 +292,44 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         constexpr bool CHECK = decltype(check_tag)::value;
         constexpr int NU = decltype(nu_tag)::value;
         if constexpr (SCALED) {
+            // CHECK: the range may hold pairs below the threshold
             for (int jb = 0; jb < n; jb += 16) {
                 const int je = min(jb + 16, n);
-                float part[NU];
+                float p1[NU], p2[NU];
 #pragma unroll
-                for (int u = 0; u < NU; u++) part[u] = 0.0f;
+                for (int u = 0; u < NU; u++) { p1[u] = 0.0f; p2[u] = 0.0f; }
+                bool close = false;
 #pragma unroll 2
                 for (int j = jb; j < je; ++j) {
                     const double4 s = Ls[j];
                     const float2 f = Lf[j];
 #pragma unroll
                     for (int u = 0; u < NU; u++) {
-                        const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
-                        const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
-                        part[u] += pair_ss32<KER, CHECK>(r2, f.x, f.y, Tqf[u], eps2s);
+                        const double r2 = fma(Tx[u], s.x, fma(Ty[u], s.y, fma(Tz[u], s.z, T2[u] + s.w)));
+                        if constexpr (CHECK) close |= r2 < th;
+                        pair_ss32<KER, false>((float)r2, f.x, f.y, eps2s, p1[u], p2[u]);
+                    }
+                }
+                if constexpr (CHECK) {
+                    if (__any(close)) {
+                        // the block again from coordinate differences (T' = -T/2 exactly)
+#pragma unroll
+                        for (int u = 0; u < NU; u++) { p1[u] = 0.0f; p2[u] = 0.0f; }
+                        for (int j = jb; j < je; ++j) {
+                            const double4 s = Ls[j];
+                            const float2 f = Lf[j];
+#pragma unroll
+                            for (int u = 0; u < NU; u++) {
+                                const double dx = fma(-0.5, Tx[u], -s.x), dy = fma(-0.5, Ty[u], -s.y),
+                                             dz = fma(-0.5, Tz[u], -s.z);
+                                const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
+                                pair_ss32<KER, KER != KR>(r2, f.x, f.y, eps2s, p1[u], p2[u]);
+                            }
+                        }
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < NU; u++) acc[u] += (double)part[u];
+                for (int u = 0; u < NU; u++) acc[u] += (double)p1[u] - Tq[u] * (double)p2[u];
             }
             return;
         }
@@ -295,8 +337,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         for (int j = 0; j < n; ++j) {
             const double4 s = Ls[j];
             const double b = SS ? Lb[j] : 0.0;
-            float2 f = make_float2(0.f, 0.f);
-            if constexpr (MIXED && SS && !GRID) f = Lf[j];
+            const float2 f = make_float2(0.f, 0.f);
 #pragma unroll
             for (int u = 0; u < NU; u++) {
                 if constexpr (MIXED) {
@@ -318,7 +359,20 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         else if (TPT > 1 && nu == 1) consume_nu(n, check_tag, std::integral_constant<int, 1>{});
     };
     using NoCheck = std::integral_constant<bool, false>;
-    using Check = std::integral_constant<bool, KER != KR>;
+    // scaled: the threshold vote; otherwise the skip of coincident pairs
+    using Check = std::integral_constant<bool, SCALED || KER != KR>;
+    // one staged source point (thread tid)
+    auto stage = [&](double x, double y, double z, double a, double b) {
+        if constexpr (SCALED) {
+            x = lam * (x - ctr.x); y = lam * (y - ctr.y); z = lam * (z - ctr.z);
+            Ls[tid] = make_double4(x, y, z, fma(z, z, fma(y, y, x * x)));
+            Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
+        } else {
+            Ls[tid] = make_double4(x, y, z, a);
+            if constexpr (SS) Lb[tid] = b;
+        }
+    };
+    const bool clu_check = SCALED && can_close && ctr.w < 0.0;
 
     // source clusters (PC in PC mode, CC in CC mode)
     const int c0 = A.clu_off[key], c1 = A.clu_off[key + 1];
@@ -340,23 +394,19 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
             __syncthreads();
             if (tid < n) {
                 const size_t g = base + off + tid;
-                const double a = cq[g];
-                Ls[tid] = make_double4(lam * cx[g], lam * cy[g], lam * cz[g], a);
-                if constexpr (SS) {
-                    const double b = cw[g];
-                    Lb[tid] = b;
-                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
-                }
+                stage(cx[g], cy[g], cz[g], cq[g], SS ? cw[g] : 0.0);
             }
             __syncthreads();
-            consume(n, NoCheck{});
+            if (clu_check) consume(n, Check{});
+            else consume(n, NoCheck{});
         }
     }
     // source particle ranges (PP, or CP into grids)
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
-        const bool check = !GRID && ((unsigned)sg.y & SEG_COINCIDE_BIT);
+        const bool check = SCALED ? can_close && ((unsigned)sg.y & SEG_CLOSE_BIT) != 0
+                                  : !GRID && ((unsigned)sg.y & SEG_COINCIDE_BIT) != 0;
         const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
         for (int off = 0; off < cnt; off += TPB) {
             const int n = min(TPB, cnt - off);
@@ -366,11 +416,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
                 const double q = A.sq[g];
                 double a = q, b = 0.0;
                 if constexpr (SS) { b = A.sw[g]; a = q * b; }
-                Ls[tid] = make_double4(lam * A.sx[g], lam * A.sy[g], lam * A.sz[g], a);
-                if constexpr (SS) {
-                    Lb[tid] = b;
-                    if constexpr (MIXED && !GRID) Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
-                }
+                stage(A.sx[g], A.sy[g], A.sz[g], a, b);
             }
             __syncthreads();
             if (check) consume(n, Check{});
@@ -445,9 +491,11 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 //   C[i][j]  = |t_i - c|^2
 // so D[i][j] = |t_i - s_j|^2 in FP64, c the key's centre; lane l holds
 // D[l/16 + 4v][l%16], v = 0..3.  The FP64 rounding error of D is below
-// 6e-15 (|t-c|^2 + |s-c|^2), so pairs with D < r2_exact (set from the
-// domain size) take the coordinate differences instead, and a coincident
-// pair (r^2 = 0) is skipped as the reference skips it.  The VALU then only
+// 6e-15 (|t-c|^2 + |s-c|^2), so pairs with D below the key's threshold
+// (|key_ctr.w|, 2e-7 R^2 for R the key's extent around c; less eps^2 for the
+// regularized kernel, whose sensitivity is to r^2 + eps^2) take the
+// coordinate differences instead, and a coincident pair (r^2 = 0) is
+// skipped as the reference skips it.  The VALU then only
 // rounds r^2 to FP32, evaluates the pair kernel and accumulates in FP64.
 // Each target's sum is spread over the 16 lanes of its row group and added
 // with a butterfly at the end of the tile.
@@ -466,6 +514,8 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
     const double4 ctr = A.key_ctr[key];
     const KP32 k32{(float)(-1.4426950408889634 * A.kp.inv_a2), (float)(-1.4426950408889634 * A.kp.p0),
                    (float)A.kp.eps2};
+    const double th = fabs(ctr.w) - (KER == KR ? A.kp.eps2 : 0.0);
+    const bool can_close = th > 0.0;
     // centred targets of the tile (LT[d*TT + i])
     for (int i = tid; i < TT; i += TPB) {
         const int g = tile.y + (i < tile.z ? i : 0);
@@ -526,20 +576,21 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
 #pragma unroll
                 for (int b = 0; b < NB; b++)
 #pragma unroll
-                    for (int v = 0; v < 4; v++) close |= D[b][v] < A.r2_exact;
+                    for (int v = 0; v < 4; v++) close |= D[b][v] < th;
                 if (__any(close)) {
 #pragma unroll
                     for (int b = 0; b < NB; b++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            if (D[b][v] < A.r2_exact) {
+                            if (D[b][v] < th) {
                                 const int r = tb + 16 * b + kk + 4 * v;
                                 const double ex = LT[r] - LB[j], ey = LT[TT + r] - LB[TPB + j],
                                              ez = LT[2 * TT + r] - LB[2 * TPB + j];
                                 const double r2 = fma(ez, ez, fma(ey, ey, ex * ex));
                                 // coincident: no contribution (rsq of FLT_MAX and the
-                                // exponentials below underflow to zero)
-                                D[b][v] = r2 > 0.0 ? r2 : 3.4e38;
+                                // exponentials below underflow to zero); the
+                                // regularized kernel has no singularity to skip
+                                D[b][v] = (KER == KR || r2 > 0.0) ? r2 : 3.4e38;
                             }
                         }
                 }
@@ -581,7 +632,7 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
         } else {
             base = (size_t)id * P;
         }
-        stream(P, false, [&](int t, double &x, double &y, double &z, double &a, double &bw) {
+        stream(P, can_close && ctr.w < 0.0, [&](int t, double &x, double &y, double &z, double &a, double &bw) {
             const size_t g = base + t;
             x = cx[g]; y = cy[g]; z = cz[g]; a = cq[g];
             if constexpr (SS) bw = cw[g];
@@ -591,10 +642,9 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
-        // coordinate differences below r2_exact (and the skip of coincident
-        // pairs, which are closer still); the regularized kernel has neither
-        // a singularity nor a sensitivity to r^2 below eps^2
-        const bool check = KER != KR && ((unsigned)sg.y & (SEG_COINCIDE_BIT | SEG_CLOSE_BIT)) != 0;
+        // coordinate differences below the threshold (and the skip of
+        // coincident pairs, which are closer still: SEG_CLOSE covers them)
+        const bool check = can_close && ((unsigned)sg.y & SEG_CLOSE_BIT) != 0;
         const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
         stream(cnt, check, [&](int t, double &x, double &y, double &z, double &a, double &bw) {
             const int g = sg.x + t;
@@ -735,7 +785,9 @@ hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
 {
     // the matrix-core near field pays where the pair kernel is short (the
     // skipping kernels); the subtraction kernels stay on the VALU kernel
-    if (!a.grid_targets && a.precision == 1 && a.key_ctr && !SS) {
+    // mixed particle-target launches form r^2 around the key centres
+    if (!a.grid_targets && a.precision == 1 && !a.key_ctr) return hipErrorInvalidValue;
+    if (!a.grid_targets && a.precision == 1 && !SS) {
         // 4 row blocks of 16 targets per wave: 256 targets per pass of a tile
         return near_mfma_launch<KER, SS, 4>(a, st);
     }

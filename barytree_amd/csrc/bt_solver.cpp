@@ -173,58 +173,74 @@ void Plan::build_work()
     near_ = WorkCSR();
     grid_ = WorkCSR();
     near_ctr_.clear();
-    r2_exact_ = 0.0;
     near_.seg_off.push_back(0); near_.clu_off.push_back(0);
     grid_.seg_off.push_back(0); grid_.clu_off.push_back(0);
 
     for (const System &sy : sys_) {
         const Tree &st = sy.stree, &tt = sy.ttree;
         const Lists &L = sy.L;
-        double d2sys = 0.0;
-        // mixed near field on the matrix cores: r^2 = |t-c|^2 + |s-c|^2 - 2(t-c).(s-c)
-        // carries an FP64 rounding error below 6e-15 (|t-c|^2 + |s-c|^2) <= 1.2e-14 D^2
-        // (D the system's diameter); below r2_exact = 6e-8 D^2 that is more than a
-        // 2e-7 relative error, so such pairs take the coordinate differences
-        {
-            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-            for (const Tree *T : {&st, &tt})
-                for (int i = 0; i < T->n; i++) {
-                    lo[0] = std::min(lo[0], T->xmin[i]); hi[0] = std::max(hi[0], T->xmax[i]);
-                    lo[1] = std::min(lo[1], T->ymin[i]); hi[1] = std::max(hi[1], T->ymax[i]);
-                    lo[2] = std::min(lo[2], T->zmin[i]); hi[2] = std::max(hi[2], T->zmax[i]);
-                }
-            for (int d = 0; d < 3; d++) d2sys += (hi[d] - lo[d]) * (hi[d] - lo[d]);
-            r2_exact_ = std::max(r2_exact_, 6e-8 * d2sys);
-        }
-        auto add_ctr = [&](int t) {
-            near_ctr_.push_back(tt.xmid[t]); near_ctr_.push_back(tt.ymid[t]);
-            near_ctr_.push_back(tt.zmid[t]); near_ctr_.push_back(0.0);
-        };
         const int so = sy.soff, to = sy.toff, sn = sy.snode_off, tn = sy.tnode_off;
+        // Mixed near field: r^2 is formed from coordinates relative to the
+        // key's centre c (its target node's centre) as
+        // |t-c|^2 + |s-c|^2 - 2 (t-c).(s-c), on the FP64 matrix cores or the
+        // FP64 VALU, with a rounding error below 6e-15 (|t-c|^2 + |s-c|^2).
+        // R is the largest distance from c to the key's target box and source
+        // particle boxes: pairs with r^2 >= thresh = 2e-7 R^2 keep a relative
+        // error below 6e-8 (one FP32 rounding), closer pairs are recomputed
+        // from coordinate differences.  The acceptance criterion keeps source
+        // clusters away from the targets; a cluster that is not far enough
+        // (6e-15 (|t-c|^2 + |s-c|^2) above 6e-8 of its closest approach^2)
+        // widens R to its far corner, and the key's clusters are then checked
+        // too (the key's thresh is stored negated).
+        auto far2 = [](double cx, double cy, double cz, const Tree &T, int n) {
+            const double dx = std::max(std::fabs(T.xmin[n] - cx), std::fabs(T.xmax[n] - cx));
+            const double dy = std::max(std::fabs(T.ymin[n] - cy), std::fabs(T.ymax[n] - cy));
+            const double dz = std::max(std::fabs(T.zmin[n] - cz), std::fabs(T.zmax[n] - cz));
+            return dx * dx + dy * dy + dz * dz;
+        };
+        auto gap2 = [&](int t, int s) {
+            const double gx = std::max(0.0, std::max(tt.xmin[t] - st.xmax[s], st.xmin[s] - tt.xmax[t]));
+            const double gy = std::max(0.0, std::max(tt.ymin[t] - st.ymax[s], st.ymin[s] - tt.ymax[t]));
+            const double gz = std::max(0.0, std::max(tt.zmin[t] - st.zmax[s], st.zmin[s] - tt.zmax[t]));
+            return gx * gx + gy * gy + gz * gz;
+        };
         // Flags of a source range (node s) for a target node t: SEG_COINCIDE
         // if a source may coincide with a target (closed boxes within 1e-150:
         // the singularity skip can only trigger there), SEG_CLOSE if a pair may
-        // come closer than sqrt(r2_exact) (the matrix-core r^2 of the mixed
-        // near field is replaced by coordinate differences below r2_exact).
-        const double gclose = std::max(1e-150, std::sqrt(6e-8 * d2sys));
+        // come closer than sqrt(thresh).
         auto within = [&](int t, int s, double g) {
             return tt.xmin[t] - st.xmax[s] <= g && st.xmin[s] - tt.xmax[t] <= g &&
                    tt.ymin[t] - st.ymax[s] <= g && st.ymin[s] - tt.ymax[t] <= g &&
                    tt.zmin[t] - st.zmax[s] <= g && st.zmin[s] - tt.zmax[t] <= g;
         };
-        auto touch = [&](int t, int s) {
-            return (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0);
+        // one near key: target node t (particles [tbeg, tbeg + tcnt) of the
+        // concatenation) <- source clusters clus and source particle nodes segs
+        auto near_key = [&](int t, int tbeg, int tcnt, const std::vector<int> &clus,
+                            const std::vector<int> &segs) {
+            const double cx = tt.xmid[t], cy = tt.ymid[t], cz = tt.zmid[t];
+            const double rt2 = far2(cx, cy, cz, tt, t);
+            double R2 = rt2;
+            for (int s : segs) R2 = std::max(R2, far2(cx, cy, cz, st, s));
+            bool clu_check = false;
+            for (int c : clus) {
+                const double f2 = far2(cx, cy, cz, st, c);
+                if (rt2 + f2 > 1e7 * gap2(t, c)) { clu_check = true; R2 = std::max(R2, f2); }
+            }
+            const double thresh = std::max(2e-7 * R2, 1e-300);
+            const double gclose = std::max(1e-150, std::sqrt(thresh));
+            const long long nt = tcnt;
+            add_key(near_, tn + t, tbeg, tcnt);
+            for (int c : clus) add_clu(near_, sn + c, nt);
+            for (int s : segs)
+                add_seg(near_, so + st.ibeg[s], st.numpar[s], nt,
+                        (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0));
+            close_key(near_);
+            near_ctr_.push_back(cx); near_ctr_.push_back(cy); near_ctr_.push_back(cz);
+            near_ctr_.push_back(clu_check ? -thresh : thresh);
         };
         if (p.compute_type == C_PC) {
             // keys: target batches; pc list -> clusters, pp list -> source leaves
-            for (int b = 0; b < tt.n; b++) {
-                const long long nt = tt.numpar[b];
-                add_key(near_, tn + b, to + tt.ibeg[b], tt.numpar[b]);
-                add_ctr(b);
-                for (int c : L.approx[b]) add_clu(near_, sn + c, nt);
-                for (int s : L.pp[b]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(b, s));
-                close_key(near_);
-            }
+            for (int b = 0; b < tt.n; b++) near_key(b, to + tt.ibeg[b], tt.numpar[b], L.approx[b], L.pp[b]);
         } else if (p.compute_type == C_CP) {
             // lists keyed by source batch; invert to target-node keys
             std::vector<std::vector<int>> pp_of(tt.n), cp_of(tt.n);
@@ -232,13 +248,8 @@ void Plan::build_work()
                 for (int t : L.pp[b]) pp_of[t].push_back(b);
                 for (int t : L.approx[b]) cp_of[t].push_back(b);
             }
-            for (int leaf : tt.leaves) {
-                const long long nt = tt.numpar[leaf];
-                add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
-                add_ctr(leaf);
-                for (int b : pp_of[leaf]) add_seg(near_, so + st.ibeg[b], st.numpar[b], nt, touch(leaf, b));
-                close_key(near_);
-            }
+            const std::vector<int> none;
+            for (int leaf : tt.leaves) near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], none, pp_of[leaf]);
             for (int t = 0; t < tt.n; t++) {
                 if (cp_of[t].empty()) continue;
                 add_key(grid_, tn + t, 0, P);
@@ -248,15 +259,14 @@ void Plan::build_work()
         } else {
             // CC: a target leaf gets the PP and source-approx (PC) lists of
             // itself and of every ancestor; grids get CC and target-approx (CP)
+            std::vector<int> clus, segs;
             for (int leaf : tt.leaves) {
-                const long long nt = tt.numpar[leaf];
-                add_key(near_, tn + leaf, to + tt.ibeg[leaf], tt.numpar[leaf]);
-                add_ctr(leaf);
+                clus.clear(); segs.clear();
                 for (int a = leaf; a >= 0; a = tt.parent[a]) {
-                    for (int c : L.cc_pc[a]) add_clu(near_, sn + c, nt);
-                    for (int s : L.pp[a]) add_seg(near_, so + st.ibeg[s], st.numpar[s], nt, touch(leaf, s));
+                    clus.insert(clus.end(), L.cc_pc[a].begin(), L.cc_pc[a].end());
+                    segs.insert(segs.end(), L.pp[a].begin(), L.pp[a].end());
                 }
-                close_key(near_);
+                near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], clus, segs);
             }
             for (int t = 0; t < tt.n; t++) {
                 if (L.approx[t].empty() && L.cc_cp[t].empty()) continue;
@@ -743,7 +753,6 @@ std::string Plan::execute(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.r2_exact = r2_exact_;
         }
         std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
         a.cursor = d_cursor_;

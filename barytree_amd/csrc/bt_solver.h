@@ -175,8 +175,10 @@ private:
     std::vector<btk::Tile> near_tiles_, grid_tiles_, leaf_tiles_;
     // grid keys write rows of partial grid values (row of key k at
     // grid_row_[k]); node n sums rows [node_rows_off_[n], node_rows_off_[n+1])
-    std::vector<double> near_ctr_;  // per near key: centre of its target node (x, y, z, 0)
-    double r2_exact_ = 0;           // mixed near field: pairs below this r^2 use coordinate differences
+    // per near key: centre of its target node and the r^2 below which the
+    // mixed near field recomputes a pair from coordinate differences
+    // (negated: the key's clusters are checked too)
+    std::vector<double> near_ctr_;
     std::vector<long long> grid_row_;
     std::vector<int> node_rows_off_;
     long long grid_rows_len_ = 0;

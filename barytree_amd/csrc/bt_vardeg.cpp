@@ -557,7 +557,6 @@ std::string Plan::execute_var(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.r2_exact = r2_exact_;
         }
         std::copy(vnear_qlen_, vnear_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_;
