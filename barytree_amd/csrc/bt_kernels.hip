@@ -267,7 +267,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     // in ranges flagged SEG_CLOSE a block of 16 sources with a pair below the
     // key's threshold (lam^2 |key_ctr.w|, less e'^2 for the regularized
     // kernel) is evaluated again from coordinate differences, with the skip of
-    // coincident pairs.  FP32 sums over blocks of 16 sources.
+    // coincident pairs.  FP32 sums over blocks of 32 sources.
     constexpr bool SCALED = MIXED && SS && !GRID;
     double lam = 1.0, th = 0.0, T2[TPT];
     float eps2s = 0.0f;
@@ -292,26 +292,37 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         constexpr bool CHECK = decltype(check_tag)::value;
         constexpr int NU = decltype(nu_tag)::value;
         if constexpr (SCALED) {
-            // CHECK: the range may hold pairs below the threshold
-            for (int jb = 0; jb < n; jb += 16) {
-                const int je = min(jb + 16, n);
+            // CHECK: the range may hold pairs below the threshold (compared
+            // in FP32; a threshold below FLT_MIN still catches r'^2 <= 0)
+            constexpr int BLK = 32;
+            const float thf = fmaxf((float)th, 1.17549435e-38f);
+            for (int jb = 0; jb < n; jb += BLK) {
+                const int je = min(jb + BLK, n);
                 float p1[NU], p2[NU];
 #pragma unroll
                 for (int u = 0; u < NU; u++) { p1[u] = 0.0f; p2[u] = 0.0f; }
-                bool close = false;
-#pragma unroll 2
-                for (int j = jb; j < je; ++j) {
+                float rmin = 3.0e38f;  // least r'^2 of the block (CHECK)
+                auto body = [&](int j) {
                     const double4 s = Ls[j];
                     const float2 f = Lf[j];
 #pragma unroll
                     for (int u = 0; u < NU; u++) {
-                        const double r2 = fma(Tx[u], s.x, fma(Ty[u], s.y, fma(Tz[u], s.z, T2[u] + s.w)));
-                        if constexpr (CHECK) close |= r2 < th;
-                        pair_ss32<KER, false>((float)r2, f.x, f.y, eps2s, p1[u], p2[u]);
+                        const float r2 =
+                            (float)fma(Tx[u], s.x, fma(Ty[u], s.y, fma(Tz[u], s.z, T2[u] + s.w)));
+                        if constexpr (CHECK) rmin = fminf(rmin, r2);
+                        pair_ss32<KER, false>(r2, f.x, f.y, eps2s, p1[u], p2[u]);
                     }
+                };
+                if (je - jb == BLK) {
+                    // full block: constant LDS offsets from one base address
+#pragma unroll 8
+                    for (int j = 0; j < BLK; ++j) body(jb + j);
+                } else {
+#pragma unroll 2
+                    for (int j = jb; j < je; ++j) body(j);
                 }
                 if constexpr (CHECK) {
-                    if (__any(close)) {
+                    if (__any(rmin < thf)) {
                         // the block again from coordinate differences (T' = -T/2 exactly)
 #pragma unroll
                         for (int u = 0; u < NU; u++) { p1[u] = 0.0f; p2[u] = 0.0f; }

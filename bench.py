@@ -92,6 +92,41 @@ def near_traffic():
     return None
 
 
+def near_valu_mix():
+    """Per-kind VALU wave-instruction counts of one launch of the mixed
+    near-field kernel at the headline size, from the committed rocprofv3
+    summary (SQ_INSTS_VALU_* counters); None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_near_kernel_coulomb_ss_cc_n16m.md")
+    try:
+        for line in open(path):
+            if line.startswith("valu_mix_per_launch:"):
+                return json.loads(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+# SQ_INSTS_VALU_* counter -> issue-cost probe kind (BaryTree_ProbeVALU)
+MIX_COST = {"SQ_INSTS_VALU_FMA_F64": "fma_f64", "SQ_INSTS_VALU_ADD_F64": "add_f64",
+            "SQ_INSTS_VALU_MUL_F64": "fma_f64", "SQ_INSTS_VALU_FMA_F32": "fma_f32",
+            "SQ_INSTS_VALU_ADD_F32": "mul_f32", "SQ_INSTS_VALU_MUL_F32": "mul_f32",
+            "SQ_INSTS_VALU_TRANS_F32": "trans_f32", "SQ_INSTS_VALU_CVT": "cvt_f64_f32",
+            "other": "fma_f32"}
+
+
+def issue_model(mix, probe, simds):
+    """Seconds the VALU needs to issue a launch's instruction mix, each kind
+    at its cost measured alone: the sum over kinds of wave instructions x ns
+    per wave instruction and SIMD, over the SIMDs of the GPU.
+    Transcendentals are priced at the mean of exp2 and rsq; instructions
+    outside the counted kinds at an FP32 fma.  Kinds issued side by side can
+    overlap, so a kernel may run below this."""
+    cost = dict(probe)
+    cost["trans_f32"] = 0.5 * (probe["exp_f32"] + probe["rsq_f32"])
+    ns = sum(mix[k] * cost[MIX_COST[k]] for k in MIX_COST if k in mix)
+    return ns * 1e-9 / simds
+
+
 def cpu_baseline(n_evals):
     """Times the oracle (FP64 restatement of the reference kernels, 1 core)
     on a bounded sample and scales to the workload's evaluation count."""
@@ -295,7 +330,13 @@ def main():
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of 2 GiB (HBM-traffic counter calibration)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only print the measured issue cost of the VALU instruction kinds")
     args = ap.parse_args()
+    if args.valu_probe:
+        import barytree_amd as bt
+        print(json.dumps({"valu_ns_per_wave_instruction_per_simd": bt.probe_valu()}))
+        return
     if args.calibrate_copy:
         import torch
         a = torch.empty(2 ** 28, dtype=torch.float64, device="cuda").uniform_()
@@ -376,6 +417,16 @@ def main():
         "frac": round(achieved / PEAK["fp32"], 4),
         "traffic": near_traffic() if n == 16_000_000 else None,
     }
+    # the VALU issue time of the kernel's instruction mix (kinds priced alone)
+    mix = near_valu_mix() if n == 16_000_000 else None
+    if mix:
+        probe = bt.probe_valu()
+        simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        ceil_s = issue_model(mix, probe, simds)
+        roofline["valu_issue_model_s"] = round(ceil_s, 4)
+        roofline["valu_issue_model_over_measured"] = round(ceil_s / near_fast, 4)
+        log(f"[bench] near kernel {near_fast:.3f} s, VALU issue time of its instruction mix "
+            f"(kinds priced alone) {ceil_s:.3f} s; probe ns/instr {probe}")
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU oracle sample")

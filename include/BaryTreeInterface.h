@@ -343,6 +343,16 @@ int BaryTree_Autotune(int numSystems, const int *targetOffsets, const int *sourc
                       int compute_type, double theta, int interpOrder, double sizeCheck,
                       double beta, int precision, BaryTreeSchedule *schedule);
 
+/* Diagnostics (no counterpart in the reference).  Issue cost of the VALU
+ * instruction kinds the pair kernels are made of, measured on the current
+ * GPU: ns per wave instruction and SIMD of FP64 fma, FP64 add, FP32 fma,
+ * FP32 mul, FP32 exp2, FP32 rsq and FP64<->FP32 conversion, in that order
+ * (up to n of them), each kind measured alone.  Times a kernel's per-kind
+ * instruction counts, this models the time the VALU needs to issue that
+ * kernel (kinds issued side by side may overlap, so a kernel can beat it).  Returns the number filled,
+ * or a negative value. */
+int BaryTree_ProbeVALU(double *nsPerInstruction, int n);
+
 #ifdef __cplusplus
 }
 #endif
