@@ -6,11 +6,12 @@
  * library.  Compile with -ffp-contract=off so the arithmetic is the
  * reference's arithmetic without fused multiply-adds.
  *
- * Scope: Lagrange approximation; PC, CP and CC compute types; the Coulomb,
+ * Scope: Lagrange and Hermite approximations; PC, CP and CC compute types; the Coulomb,
  * Yukawa and regularized-Coulomb kernels with skipping and subtraction, in
  * exactly the combinations the reference dispatchers accept
- * (src/interaction_compute/interaction_compute_{pc,cp,cc}.c).  Hermite is
- * rejected with an error code ("parity unpinned" -- not restated here).
+ * (src/interaction_compute/interaction_compute_{pc,cp,cc}.c), and the
+ * Hermite approximation where those dispatchers accept it (PC with either
+ * singularity, CP with skipping).
  *
  * Every function cites the reference lines it follows.  Deviations, each
  * deliberate:
@@ -473,15 +474,22 @@ typedef struct {
 
 static void clus_free(Clus *c) { free(c->x); free(c->y); free(c->z); free(c->q); free(c->w); memset(c, 0, sizeof(*c)); }
 
-static int clus_alloc(Clus *c, int numnodes, int P, int with_w)
+/* qplanes (wplanes) values per grid point: 1 for Lagrange, 8 for Hermite
+ * (run_params.c:122-128); wplanes 0: no weights */
+static int clus_alloc_planes(Clus *c, int numnodes, int P, int qplanes, int wplanes)
 {
     size_t n = (size_t)numnodes * (size_t)P;
     c->numnodes = numnodes; c->P = P;
     c->x = calloc(n, sizeof(double)); c->y = calloc(n, sizeof(double)); c->z = calloc(n, sizeof(double));
-    c->q = calloc(n, sizeof(double));
-    c->w = with_w ? calloc(n, sizeof(double)) : NULL;
-    if (!c->x || !c->y || !c->z || !c->q || (with_w && !c->w)) return ORC_ERR_NOMEM;
+    c->q = calloc(n * qplanes, sizeof(double));
+    c->w = wplanes ? calloc(n * wplanes, sizeof(double)) : NULL;
+    if (!c->x || !c->y || !c->z || !c->q || (wplanes && !c->w)) return ORC_ERR_NOMEM;
     return 0;
+}
+
+static int clus_alloc(Clus *c, int numnodes, int P, int with_w)
+{
+    return clus_alloc_planes(c, numnodes, P, 1, with_w ? 1 : 0);
 }
 
 /*
@@ -607,7 +615,8 @@ static int clusters_sources(Clus *c, const Tree *t, const Parts *s, const Prm *p
 static int clusters_targets(Clus *c, const Tree *t, const Prm *prm)
 {
     int P = prm->P, n = prm->degree, nl = n + 1;
-    int e = clus_alloc(c, t->n, P, prm->singularity == SNG_SUB);
+    int planes = prm->approximation == APX_HERMITE ? 8 : 1;
+    int e = clus_alloc_planes(c, t->n, P, planes, prm->singularity == SNG_SUB ? planes : 0);
     if (e) return e;
     double nodeX[32], nodeY[32], nodeZ[32];
     for (int idx = 0; idx < t->n; idx++) {
@@ -903,6 +912,332 @@ static void downpass(const Tree *t, Clus *c, const Parts *T, const Prm *prm, dou
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Hermite approximation: 8 planes per grid point (value, the three     */
+/* first, the three mixed second and the mixed third derivative terms,  */
+/* in the order 0, x, y, z, xy, yz, xz, xyz).  Planes of node c sit at  */
+/* q[8*P*c + m*P + j]; grid coordinates at x[P*c + j].                  */
+/* ------------------------------------------------------------------ */
+
+/* Nodes, Hermite barycentric weights dj and w of one dimension of the box
+ * [a0,a1] (clusters.c:1296-1318, interaction_compute_downpass.c:913-933). */
+static void herm_weights(int n, double a0, double a1, double *node, double *dj, double *w)
+{
+    double ww[32];
+    for (int i = 0; i < n + 1; i++) {
+        double xx = i * M_PI / n;
+        double tt = cos(xx);
+        ww[i] = -cos(xx) / (2 * sin(xx) * sin(xx));
+        node[i] = a0 + (tt + 1.0) / 2.0 * (a1 - a0);
+    }
+    ww[0] = 0.25 * (n * n / 3.0 + 1.0 / 6.0);
+    ww[n] = -ww[0];
+    for (int j = 0; j < n + 1; j++) {
+        dj[j] = 1.0;
+        w[j] = -4.0 * ww[j] / (a1 - a0);
+    }
+    dj[0] = 0.25;
+    dj[n] = 0.25;
+}
+
+/* The 8 numerators of grid point (k1,k2,k3) for a point at displacement
+ * (dx,dy,dz) from it, with exact-hit indices e* (-1: none):
+ * clusters.c:1392-1462, interaction_compute_downpass.c:1010-1082. */
+static void herm_numerators(int k1, int k2, int k3, double dx, double dy, double dz, int ex, int ey, int ez,
+                            const double *djx, const double *wx, const double *wy, const double *wz,
+                            double *num)
+{
+    for (int m = 0; m < 8; m++) num[m] = 1.0;
+    double Ax = djx[k1] / (dx * dx) + wx[k1] / dx;
+    double Ay = djx[k2] / (dy * dy) + wy[k2] / dy;
+    double Az = djx[k3] / (dz * dz) + wz[k3] / dz;
+    double Bx = djx[k1] / dx, By = djx[k2] / dy, Bz = djx[k3] / dz;
+    if (ex == -1) {
+        num[0] *= Ax; num[1] *= Bx; num[2] *= Ax; num[3] *= Ax;
+        num[4] *= Bx; num[5] *= Ax; num[6] *= Bx; num[7] *= Bx;
+    } else if (ex != k1) {
+        for (int m = 0; m < 8; m++) num[m] *= 0;
+    } else {
+        num[1] *= 0; num[4] *= 0; num[6] *= 0; num[7] *= 0;
+    }
+    if (ey == -1) {
+        num[0] *= Ay; num[1] *= Ay; num[2] *= By; num[3] *= Ay;
+        num[4] *= By; num[5] *= By; num[6] *= Ay; num[7] *= By;
+    } else if (ey != k2) {
+        for (int m = 0; m < 8; m++) num[m] *= 0;
+    } else {
+        num[2] *= 0; num[4] *= 0; num[5] *= 0; num[7] *= 0;
+    }
+    if (ez == -1) {
+        num[0] *= Az; num[1] *= Az; num[2] *= Az; num[3] *= Bz;
+        num[4] *= Az; num[5] *= Bz; num[6] *= Bz; num[7] *= Bz;
+    } else if (ez != k3) {
+        for (int m = 0; m < 8; m++) num[m] *= 0;
+    } else {
+        num[3] *= 0; num[5] *= 0; num[6] *= 0; num[7] *= 0;
+    }
+}
+
+/* Upward pass of one node straight from its particles: clusters.c:1224-1836
+ * (pc_comp_ms_modifiedF_hermite, _hermite_SS).  f1 = q (or q*w), f2 = w
+ * (subtraction) or NULL; Q (and W) receive 8 planes of P values. */
+static int anterp_hermite(int n, double x0, double x1, double y0, double y1, double z0, double z1, int m,
+                          const double *px, const double *py, const double *pz, const double *f1in,
+                          const double *f2in, double *gx, double *gy, double *gz, double *Q, double *W)
+{
+    int nl = n + 1, P = nl * nl * nl;
+    double nodeX[32], nodeY[32], nodeZ[32], dj[32], wx[32], wy[32], wz[32];
+    double *f1 = malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    double *f2 = f2in ? malloc((size_t)(m > 0 ? m : 1) * sizeof(double)) : NULL;
+    int *ex = malloc((size_t)(m > 0 ? m : 1) * 3 * sizeof(int));
+    if (!f1 || (f2in && !f2) || !ex) { free(f1); free(f2); free(ex); return ORC_ERR_NOMEM; }
+    int *ey = ex + m, *ez = ex + 2 * m;
+    herm_weights(n, x0, x1, nodeX, dj, wx);
+    herm_weights(n, y0, y1, nodeY, dj, wy);
+    herm_weights(n, z0, z1, nodeZ, dj, wz);
+    for (int i = 0; i < m; i++) {
+        f1[i] = f1in[i];
+        if (f2) f2[i] = f2in[i];
+        ex[i] = ey[i] = ez[i] = -1;
+        double sumX = 0.0, sumY = 0.0, sumZ = 0.0;
+        for (int j = 0; j < nl; j++) {
+            double dx = px[i] - nodeX[j], dy = py[i] - nodeY[j], dz = pz[i] - nodeZ[j];
+            if (fabs(dx) < DBL_MIN) ex[i] = j;
+            if (fabs(dy) < DBL_MIN) ey[i] = j;
+            if (fabs(dz) < DBL_MIN) ez[i] = j;
+            sumX += dj[j] / (dx * dx) + wx[j] / dx;
+            sumY += dj[j] / (dy * dy) + wy[j] / dy;
+            sumZ += dj[j] / (dz * dz) + wz[j] / dz;
+        }
+        double denominator = 1.0;
+        if (ex[i] == -1) denominator *= sumX;
+        if (ey[i] == -1) denominator *= sumY;
+        if (ez[i] == -1) denominator *= sumZ;
+        f1[i] /= denominator;
+        if (f2) f2[i] /= denominator;
+    }
+    for (int j = 0; j < P; j++) {
+        int k1 = j % nl;
+        int kk = (j - k1) / nl;
+        int k2 = kk % nl;
+        kk = kk - k2;
+        int k3 = kk / nl;
+        double cx = nodeX[k1], cy = nodeY[k2], cz = nodeZ[k3];
+        gx[j] = cx; gy[j] = cy; gz[j] = cz;
+        double tq[8] = {0}, tw[8] = {0}, num[8];
+        for (int i = 0; i < m; i++) {
+            herm_numerators(k1, k2, k3, px[i] - cx, py[i] - cy, pz[i] - cz, ex[i], ey[i], ez[i],
+                            dj, wx, wy, wz, num);
+            for (int q = 0; q < 8; q++) tq[q] += num[q] * f1[i];
+            if (f2)
+                for (int q = 0; q < 8; q++) tw[q] += num[q] * f2[i];
+        }
+        for (int q = 0; q < 8; q++) Q[q * P + j] += tq[q];
+        if (f2)
+            for (int q = 0; q < 8; q++) W[q * P + j] += tw[q];
+    }
+    free(f1); free(f2); free(ex);
+    return 0;
+}
+
+/* clusters.c:182-190: every node from its own particles */
+static int clusters_sources_hermite(Clus *c, const Tree *t, const Parts *s, const Prm *prm)
+{
+    int sub = prm->singularity == SNG_SUB;
+    int P = prm->P, n = prm->degree;
+    int e = clus_alloc_planes(c, t->n, P, 8, sub ? 8 : 0);
+    if (e) return e;
+    double *qw = NULL;
+    if (sub) {
+        qw = malloc((size_t)s->n * sizeof(double));
+        if (!qw) return ORC_ERR_NOMEM;
+        for (int i = 0; i < s->n; i++) qw[i] = s->q[i] * s->w[i];
+    }
+    for (int idx = 0; idx < t->n && !e; idx++) {
+        int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
+        size_t cs = (size_t)idx * P;
+        e = anterp_hermite(n, t->xmin[idx], t->xmax[idx], t->ymin[idx], t->ymax[idx], t->zmin[idx],
+                           t->zmax[idx], m, s->x + b, s->y + b, s->z + b, sub ? qw + b : s->q + b,
+                           sub ? s->w + b : NULL, c->x + cs, c->y + cs, c->z + cs, c->q + 8 * cs,
+                           sub ? c->w + 8 * cs : NULL);
+    }
+    free(qw);
+    return e;
+}
+
+/* Target particles against a source cluster's 8 planes: K_Coulomb_PC_Hermite
+ * (coulomb_pc.c:66-135), K_Yukawa_PC_Hermite (yukawa_pc.c:61-140),
+ * K_RegularizedCoulomb_PC_Hermite (regularized-coulomb_pc.c:67-139) and the
+ * subtraction forms (coulomb_ss_pc.c:63-163, yukawa_ss_pc.c:61-167,
+ * regularized-coulomb_ss_pc.c:63-160). */
+static void k_pc_hermite(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *T, const Clus *C,
+                         double *pot)
+{
+    int sub = prm->singularity == SNG_SUB;
+    const double *q0 = C->q + 8 * cs, *q1 = q0 + P, *q2 = q0 + 2 * P, *q3 = q0 + 3 * P, *q4 = q0 + 4 * P,
+                 *q5 = q0 + 5 * P, *q6 = q0 + 6 * P, *q7 = q0 + 7 * P;
+    const double *w0 = NULL, *w1 = NULL, *w2 = NULL, *w3 = NULL, *w4 = NULL, *w5 = NULL, *w6 = NULL, *w7 = NULL;
+    if (sub) {
+        w0 = C->w + 8 * cs; w1 = w0 + P; w2 = w0 + 2 * P; w3 = w0 + 3 * P;
+        w4 = w0 + 4 * P; w5 = w0 + 5 * P; w6 = w0 + 6 * P; w7 = w0 + 7 * P;
+    }
+    for (int i = 0; i < nt; i++) {
+        int ii = ts + i;
+        double temporary_potential = 0.0;
+        double tx = T->x[ii], ty = T->y[ii], tz = T->z[ii], tq = T->q[ii];
+        for (int j = 0; j < P; j++) {
+            size_t jj = cs + j;
+            double dx = tx - C->x[jj], dy = ty - C->y[jj], dz = tz - C->z[jj];
+            if (prm->kernel == KER_YUKAWA) {
+                double k = prm->kp[0], k2 = k * k, k3 = k * k2;
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                double r2 = r * r, r3 = r2 * r;
+                double rinv = 1 / r;
+                double r3inv = rinv * rinv * rinv, r5inv = r3inv * rinv * rinv, r7inv = r5inv * rinv * rinv;
+                if (!sub) {
+                    temporary_potential += exp(-k * r)
+                        * (rinv * (q0[j])
+                        + r3inv * (1 + k * r) * (q1[j] * dx + q2[j] * dy + q3[j] * dz)
+                        + r5inv * (3 + 3 * k * r + k2 * r2) * (q4[j] * dx * dy + q5[j] * dy * dz + q6[j] * dx * dz)
+                        + r7inv * (15 + 15 * k * r + 6 * k2 * r2 + k3 * r3) * q7[j] * dx * dy * dz);
+                } else {
+                    double kr = k * r, k2r2 = kr * kr, k3r3 = k2r2 * kr;
+                    (void)k2r2; (void)k3r3;
+                    double d0 = q0[j] - w0[j] * tq, d1 = q1[j] - w1[j] * tq, d2 = q2[j] - w2[j] * tq,
+                           d3 = q3[j] - w3[j] * tq, d4 = q4[j] - w4[j] * tq, d5 = q5[j] - w5[j] * tq,
+                           d6 = q6[j] - w6[j] * tq, d7 = q7[j] - w7[j] * tq;
+                    temporary_potential += exp(-k * r)
+                        * (rinv * (d0)
+                        + r3inv * (1 + k * r) * (d1 * dx + d2 * dy + d3 * dz)
+                        + r5inv * (3 + 3 * k * r + k2 * r2) * (d4 * dx * dy + d5 * dy * dz + d6 * dx * dz)
+                        + r7inv * (15 + 15 * k * r + 6 * k2 * r2 + k3 * r3) * d7 * dx * dy * dz);
+                }
+            } else {
+                /* Coulomb and regularized Coulomb: the charge part is the
+                 * derivative expansion of 1/r (r including eps^2 for the
+                 * regularized kernel); subtraction adds that of exp(-r^2/k^2)/r */
+                double r;
+                if (prm->kernel == KER_COULOMB) r = sqrt(dx * dx + dy * dy + dz * dz);
+                else r = sqrt(dx * dx + dy * dy + dz * dz + (sub ? prm->kp[1] * prm->kp[1] : prm->kp[0] * prm->kp[0]));
+                double rinv = 1 / r;
+                double r3inv = rinv * rinv * rinv, r5inv = r3inv * rinv * rinv, r7inv = r5inv * rinv * rinv;
+                if (!sub) {
+                    temporary_potential += rinv * (q0[j])
+                        + r3inv * (q1[j] * dx + q2[j] * dy + q3[j] * dz)
+                        + 3 * r5inv * (q4[j] * dx * dy + q5[j] * dy * dz + q6[j] * dx * dz)
+                        + 15 * r7inv * q7[j] * dx * dy * dz;
+                } else {
+                    double k2 = prm->kp[0] * prm->kp[0];
+                    double r2 = r * r;
+                    double rk2 = r2 / k2, rk4 = rk2 * rk2, rk6 = rk4 * rk2;
+                    temporary_potential += rinv * (q0[j])
+                        + r3inv * (q1[j] * dx + q2[j] * dy + q3[j] * dz)
+                        + 3 * r5inv * (q4[j] * dx * dy + q5[j] * dy * dz + q6[j] * dx * dz)
+                        + 15 * r7inv * q7[j] * dx * dy * dz
+                        - tq * exp(-rk2)
+                            * (rinv * (w0[j])
+                            + r3inv * (1 + 2 * rk2) * (w1[j] * dx + w2[j] * dy + w3[j] * dz)
+                            + r5inv * (3 + 4 * rk2 + 4 * rk4) * (w4[j] * dx * dy + w5[j] * dy * dz + w6[j] * dx * dz)
+                            + r7inv * (15 + 18 * rk2 + 12 * rk4 + 8 * rk6) * w7[j] * dx * dy * dz);
+                }
+            }
+        }
+        pot[ii] += temporary_potential;
+    }
+}
+
+/* Target cluster grid (8 planes) against source particles:
+ * K_Coulomb_CP_Hermite (coulomb_cp.c:66-184), K_Yukawa_CP_Hermite
+ * (yukawa_cp.c:68-194), K_RegularizedCoulomb_CP_Hermite
+ * (regularized-coulomb_cp.c:68-188). */
+static void k_cp_hermite(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *S, Clus *C)
+{
+    double *q0 = C->q + 8 * cs;
+    for (int i = 0; i < P; i++) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0, t5 = 0.0, t6 = 0.0, t7 = 0.0;
+        double cx = C->x[cs + i], cy = C->y[cs + i], cz = C->z[cs + i];
+        for (int j = 0; j < ns; j++) {
+            int jj = ss + j;
+            double dx = S->x[jj] - cx, dy = S->y[jj] - cy, dz = S->z[jj] - cz;
+            if (prm->kernel == KER_YUKAWA) {
+                double k = prm->kp[0], k2 = k * k, k3 = k * k2;
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double r = sqrt(r2), r3 = r2 * r;
+                double r2inv = 1 / r2;
+                double rinvq = S->q[jj] / r * exp(-k * r);
+                double r3inv = rinvq * r2inv, r5inv = r3inv * r2inv, r7inv = r5inv * r2inv;
+                double d1 = r3inv * (1 + k * r);
+                double d2 = r5inv * (3 + 3 * k * r + k2 * r2);
+                double d3 = r7inv * (15 + 15 * k * r + 6 * k2 * r2 + k3 * r3);
+                t0 += rinvq;
+                t1 += d1 * dx; t2 += d1 * dy; t3 += d1 * dz;
+                t4 += d2 * dx * dy; t5 += d2 * dy * dz; t6 += d2 * dx * dz;
+                t7 += d3 * dx * dy * dz;
+            } else {
+                double r2 = dx * dx + dy * dy + dz * dz;
+                if (prm->kernel == KER_REGCOULOMB) r2 += prm->kp[0] * prm->kp[0];
+                double r2inv = 1 / r2;
+                double rinvq = S->q[jj] / sqrt(r2);
+                double r3inv = rinvq * r2inv, r5inv = r3inv * r2inv, r7inv = r5inv * r2inv;
+                r5inv *= 3.0;
+                t0 += rinvq;
+                t1 += r3inv * dx; t2 += r3inv * dy; t3 += r3inv * dz;
+                t4 += r5inv * dx * dy; t5 += r5inv * dy * dz; t6 += r5inv * dx * dz;
+                t7 += r7inv * dx * dy * dz * 15.0;
+            }
+        }
+        q0[i] += t0; q0[P + i] += t1; q0[2 * P + i] += t2; q0[3 * P + i] += t3;
+        q0[4 * P + i] += t4; q0[5 * P + i] += t5; q0[6 * P + i] += t6; q0[7 * P + i] += t7;
+    }
+}
+
+/* Downpass: every target node's 8 planes to its own particles
+ * (interaction_compute_downpass.c:117-120, 860-1109). */
+static void downpass_hermite(const Tree *t, const Clus *c, const Parts *T, const Prm *prm, double *pot)
+{
+    int n = prm->degree, nl = n + 1, P = prm->P;
+    for (int idx = 0; idx < t->n; idx++) {
+        int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
+        const double *q0 = c->q + (size_t)8 * P * idx;
+        double nodeX[32], nodeY[32], nodeZ[32], dj[32], wx[32], wy[32], wz[32];
+        herm_weights(n, t->xmin[idx], t->xmax[idx], nodeX, dj, wx);
+        herm_weights(n, t->ymin[idx], t->ymax[idx], nodeY, dj, wy);
+        herm_weights(n, t->zmin[idx], t->zmax[idx], nodeZ, dj, wz);
+        for (int i = 0; i < m; i++) {
+            double tx = T->x[b + i], ty = T->y[b + i], tz = T->z[b + i];
+            double sumX = 0.0, sumY = 0.0, sumZ = 0.0;
+            int eix = -1, eiy = -1, eiz = -1;
+            for (int j = 0; j < nl; j++) {
+                double cx = tx - nodeX[j], cy = ty - nodeY[j], cz = tz - nodeZ[j];
+                if (fabs(cx) < DBL_MIN) eix = j;
+                if (fabs(cy) < DBL_MIN) eiy = j;
+                if (fabs(cz) < DBL_MIN) eiz = j;
+                sumX += dj[j] / (cx * cx) + wx[j] / cx;
+                sumY += dj[j] / (cy * cy) + wy[j] / cy;
+                sumZ += dj[j] / (cz * cz) + wz[j] / cz;
+            }
+            double denominator = 1.0;
+            if (eix == -1) denominator /= sumX;
+            if (eiy == -1) denominator /= sumY;
+            if (eiz == -1) denominator /= sumZ;
+            double temp = 0.0, num[8];
+            for (int j = 0; j < P; j++) {
+                int k1 = j % nl;
+                int kk = (j - k1) / nl;
+                int k2 = kk % nl;
+                kk = kk - k2;
+                int k3 = kk / nl;
+                herm_numerators(k1, k2, k3, tx - nodeX[k1], ty - nodeY[k2], tz - nodeZ[k3], eix, eiy, eiz,
+                                dj, wx, wy, wz, num);
+                temp += denominator * (num[0] * q0[j] + num[1] * q0[P + j] + num[2] * q0[2 * P + j]
+                                     + num[3] * q0[3 * P + j] + num[4] * q0[4 * P + j] + num[5] * q0[5 * P + j]
+                                     + num[6] * q0[6 * P + j] + num[7] * q0[7 * P + j]);
+            }
+            pot[b + i] += temp;
+        }
+    }
+}
+
 /* interaction_compute_correction.c:22-45 and the *_ss_correction.c kernels */
 static void ss_correction(const Prm *prm, const Parts *T, double *pot)
 {
@@ -914,15 +1249,22 @@ static void ss_correction(const Prm *prm, const Parts *T, double *pot)
     for (int i = 0; i < T->n; i++) pot[i] += param * T->q[i];
 }
 
-/* Which (kernel, singularity, compute type) the reference dispatchers accept
- * for the Lagrange approximation (interaction_compute_{pc,cp,cc}.c). */
+/* Which (kernel, singularity, approximation, compute type) the reference
+ * dispatchers accept (interaction_compute_{pc,cp,cc}.c; Hermite: PC with
+ * either singularity, CP with skipping (interaction_compute_cp.c:133, 187,
+ * interaction_compute_downpass.c:126-128), no CC (interaction_compute_cc.c:146)). */
 static int supported(const Prm *p)
 {
-    if (p->approximation != APX_LAGRANGE) return 0;
+    if (p->approximation != APX_LAGRANGE && p->approximation != APX_HERMITE) return 0;
     if (p->kernel != KER_COULOMB && p->kernel != KER_YUKAWA && p->kernel != KER_REGCOULOMB) return 0;
     if (p->singularity != SNG_SKIP && p->singularity != SNG_SUB) return 0;
     if (p->compute_type < CT_PC || p->compute_type > CT_CC) return 0;
-    if (p->kernel == KER_REGCOULOMB && p->singularity == SNG_SUB && p->compute_type != CT_PC) return 0;
+    if (p->approximation == APX_HERMITE) {
+        if (p->compute_type == CT_CC) return 0;
+        if (p->compute_type == CT_CP && p->singularity == SNG_SUB) return 0;
+    } else if (p->kernel == KER_REGCOULOMB && p->singularity == SNG_SUB && p->compute_type != CT_PC) {
+        return 0;
+    }
     if (p->degree < 1 || p->degree > 30) return 0;
     return 1;
 }
@@ -1009,13 +1351,16 @@ int orc_treecode(int M, int N,
     memset(&sc, 0, sizeof(sc)); memset(&tc, 0, sizeof(tc)); memset(&L, 0, sizeof(L));
     int P = prm.P;
     int stride = g_stride > 0 ? g_stride : 1;
+    int herm = prm.approximation == APX_HERMITE;
 
     if (compute_type == CT_PC) {
         /* treedriver.c:350-627 */
         if ((rc = tree_construct(&src, &S, prm.msl, 0))) goto done;
         if ((rc = tree_construct(&tgt, &T, prm.mtl, 1))) goto done;
         g_stats.t_setup = wall() - t0; t0 = wall();
-        if ((rc = clusters_sources(&sc, &src, &S, &prm))) goto done;
+        if (herm) rc = clusters_sources_hermite(&sc, &src, &S, &prm);
+        else rc = clusters_sources(&sc, &src, &S, &prm);
+        if (rc) goto done;
         g_stats.t_upward = wall() - t0; t0 = wall();
         if ((rc = lists_make(&L, &src, &tgt, &prm))) goto done;
         g_stats.t_lists = wall() - t0; t0 = wall();
@@ -1023,8 +1368,10 @@ int orc_treecode(int M, int N,
         for (int i = 0; i < tgt.n; i++) {
             if (i % stride) continue;
             int bs = tgt.ibeg[i] - 1, nt = tgt.iend[i] - tgt.ibeg[i] + 1;
-            for (int j = 0; j < L.ap[i].n; j++)
-                k_pc(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
+            for (int j = 0; j < L.ap[i].n; j++) {
+                if (herm) k_pc_hermite(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
+                else k_pc(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
+            }
             for (int j = 0; j < L.pp[i].n; j++) {
                 int nd = L.pp[i].v[j];
                 k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, bs, src.ibeg[nd] - 1, &T, &S, pot);
@@ -1043,15 +1390,18 @@ int orc_treecode(int M, int N,
         for (int i = 0; i < src.n; i++) {
             if (i % stride) continue;
             int bs = src.ibeg[i] - 1, ns = src.iend[i] - src.ibeg[i] + 1;
-            for (int j = 0; j < L.ap[i].n; j++)
-                k_cp(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
+            for (int j = 0; j < L.ap[i].n; j++) {
+                if (herm) k_cp_hermite(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
+                else k_cp(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
+            }
             for (int j = 0; j < L.pp[i].n; j++) {
                 int nd = L.pp[i].v[j];
                 k_pp(&prm, tgt.iend[nd] - tgt.ibeg[nd] + 1, ns, tgt.ibeg[nd] - 1, bs, &T, &S, pot);
             }
         }
         g_stats.t_compute = wall() - t0; t0 = wall();
-        downpass(&tgt, &tc, &T, &prm, pot);
+        if (herm) downpass_hermite(&tgt, &tc, &T, &prm, pot);
+        else downpass(&tgt, &tc, &T, &prm, pot);
         g_stats.t_downpass = wall() - t0; t0 = wall();
     } else {
         /* treedriver.c:638-967 */
@@ -1190,7 +1540,8 @@ int orc_treecode_vardeg(int M, int N,
     orc_params_resolve(compute_type, approximation, theta, degree, msl, mtl, size_check, beta,
                        &prm.theta, &prm.degree, &prm.size_check_factor, &prm.msl, &prm.mtl);
     prm.P = (prm.degree + 1) * (prm.degree + 1) * (prm.degree + 1);
-    if (!supported(&prm)) return ORC_ERR_UNSUPPORTED;
+    /* per-pair degrees are a Lagrange extension */
+    if (!supported(&prm) || approximation != APX_LAGRANGE) return ORC_ERR_UNSUPPORTED;
     if (singularity == SNG_SUB && (!sw || !tq)) return ORC_ERR_ARGS;
     int sub = singularity == SNG_SUB;
 
@@ -1309,10 +1660,21 @@ int orc_source_clusters(int n, const double *x, const double *y, const double *z
                         int singularity, double *cx, double *cy, double *cz,
                         double *q_out, double *w_out)
 {
+    return orc_source_clusters_apx(n, x, y, z, q, w, degree, max_leaf, singularity, APX_LAGRANGE, cx, cy,
+                                   cz, q_out, w_out);
+}
+
+int orc_source_clusters_apx(int n, const double *x, const double *y, const double *z,
+                            const double *q, const double *w, int degree, int max_leaf,
+                            int singularity, int approximation, double *cx, double *cy, double *cz,
+                            double *q_out, double *w_out)
+{
     if (n <= 0 || degree < 1 || degree > 30 || max_leaf < 1) return ORC_ERR_ARGS;
+    if (approximation != APX_LAGRANGE && approximation != APX_HERMITE) return ORC_ERR_ARGS;
+    int planes = approximation == APX_HERMITE ? 8 : 1;
     Prm prm;
     memset(&prm, 0, sizeof(prm));
-    prm.singularity = singularity; prm.degree = degree;
+    prm.singularity = singularity; prm.degree = degree; prm.approximation = approximation;
     prm.P = (degree + 1) * (degree + 1) * (degree + 1);
     Parts S;
     memset(&S, 0, sizeof(S));
@@ -1321,16 +1683,17 @@ int orc_source_clusters(int n, const double *x, const double *y, const double *z
     Clus c;
     memset(&t, 0, sizeof(t)); memset(&c, 0, sizeof(c));
     if (!rc) rc = tree_construct(&t, &S, max_leaf, 0);
-    if (!rc && q_out) rc = clusters_sources(&c, &t, &S, &prm);
+    if (!rc && q_out)
+        rc = planes == 8 ? clusters_sources_hermite(&c, &t, &S, &prm) : clusters_sources(&c, &t, &S, &prm);
     if (!rc) {
         rc = t.n;
         if (q_out) {
             size_t tot = (size_t)t.n * prm.P;
-            memcpy(q_out, c.q, tot * sizeof(double));
+            memcpy(q_out, c.q, tot * planes * sizeof(double));
             if (cx) memcpy(cx, c.x, tot * sizeof(double));
             if (cy) memcpy(cy, c.y, tot * sizeof(double));
             if (cz) memcpy(cz, c.z, tot * sizeof(double));
-            if (w_out && c.w) memcpy(w_out, c.w, tot * sizeof(double));
+            if (w_out && c.w) memcpy(w_out, c.w, tot * planes * sizeof(double));
         }
     }
     clus_free(&c); tree_free(&t); parts_free(&S);

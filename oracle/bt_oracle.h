@@ -88,6 +88,12 @@ int orc_source_clusters(int n, const double *x, const double *y, const double *z
                         const double *q, const double *w, int degree, int max_leaf,
                         int singularity, double *cx, double *cy, double *cz,
                         double *q_out, double *w_out);
+/* As orc_source_clusters for approximation 1 (Lagrange) or 2 (Hermite:
+ * q_out and w_out hold 8 planes of P values per node, node-major). */
+int orc_source_clusters_apx(int n, const double *x, const double *y, const double *z,
+                            const double *q, const double *w, int degree, int max_leaf,
+                            int singularity, int approximation, double *cx, double *cy, double *cz,
+                            double *q_out, double *w_out);
 
 /* Variable-degree treecode: as orc_treecode, but far-field pair k (in the
  * order PC per target batch, CP per source batch, CC per target node its

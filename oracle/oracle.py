@@ -138,20 +138,24 @@ def set_sample_stride(stride):
     lib().orc_set_sample_stride(ctypes.c_int(int(stride)))
 
 
-def source_clusters(x, y, z, q, w, degree, max_leaf, singularity):
-    """Upward pass alone: (cx, cy, cz, Q, W) per node, each (numnodes, P)."""
+def source_clusters(x, y, z, q, w, degree, max_leaf, singularity, approximation=1):
+    """Upward pass alone: (cx, cy, cz, Q, W) per node; coordinates (numnodes,
+    P), charges (numnodes, P) for Lagrange, (numnodes, 8, P) for Hermite."""
     x, y, z, q, w = map(_c, (x, y, z, q, w))
     l = lib()
-    nn = l.orc_source_clusters(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
-                               singularity, None, None, None, None, None)
+    nn = l.orc_source_clusters_apx(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
+                                   singularity, approximation, None, None, None, None, None)
     if nn < 0:
         raise RuntimeError(f"oracle returned {nn}")
     P = (degree + 1) ** 3
-    bufs = [np.zeros(nn * P) for _ in range(5)]
-    nn2 = l.orc_source_clusters(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
-                                singularity, *[_p(b) for b in bufs])
+    planes = 8 if approximation == 2 else 1
+    bufs = [np.zeros(nn * P) for _ in range(3)] + [np.zeros(nn * P * planes) for _ in range(2)]
+    nn2 = l.orc_source_clusters_apx(len(x), _p(x), _p(y), _p(z), _p(q), _p(w), degree, max_leaf,
+                                    singularity, approximation, *[_p(b) for b in bufs])
     assert nn2 == nn
-    return [b.reshape(nn, P) for b in bufs]
+    out = [b.reshape(nn, P) for b in bufs[:3]]
+    shape = (nn, 8, P) if planes == 8 else (nn, P)
+    return out + [b.reshape(shape) for b in bufs[3:]]
 
 
 _ip = ctypes.POINTER(ctypes.c_int)

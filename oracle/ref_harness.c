@@ -11,8 +11,9 @@
  * ref_treecode() below runs treedriver's single-rank phase order
  * (src/drivers/treedriver.c:114-967) with those reference functions, and
  * fills the gaps (Tree_Fill and the level lists of tree.c, the batch fill of
- * batches.c, the cluster upward pass of clusters.c, the reorder of
- * particles.c) with the oracle's restatement, which is included below.
+ * batches.c, the cluster upward pass of clusters.c -- Lagrange and Hermite --,
+ * the reorder of particles.c) with the oracle's restatement, which is
+ * included below.
  *
  * Agreement of ref_treecode() with orc_treecode() therefore pins the
  * oracle's partition, tree, interaction-list, kernel, dispatch and downpass
@@ -198,15 +199,19 @@ static struct Clusters *h_clusters(const struct Tree *t, struct Particles *p, co
     h_view(&o, t);
     Clus c;
     memset(&c, 0, sizeof(c));
+    int herm = prm->approximation == APX_HERMITE;
     if (sources) {
         Parts s;
         s.n = p->num; s.x = p->x; s.y = p->y; s.z = p->z; s.q = p->q; s.w = p->w; s.order = p->order;
-        clusters_sources(&c, &o, &s, prm);
+        if (herm) clusters_sources_hermite(&c, &o, &s, prm);
+        else clusters_sources(&c, &o, &s, prm);
     } else {
         clusters_targets(&c, &o, prm);
     }
     struct Clusters *r = calloc(1, sizeof(struct Clusters));
-    r->num = r->num_charges = r->num_weights = t->numnodes * prm->P;
+    r->num = t->numnodes * prm->P;
+    r->num_charges = r->num * (herm ? 8 : 1);
+    r->num_weights = r->num * (herm && prm->singularity == SNG_SUB ? 8 : 1);
     r->x = c.x; r->y = c.y; r->z = c.z; r->q = c.q; r->w = c.w;
     return r;
 }

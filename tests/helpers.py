@@ -42,3 +42,13 @@ def kat_inputs():
 def rel_l2(ref, val):
     ref = np.asarray(ref); val = np.asarray(val)
     return float(np.sqrt(np.sum((ref - val) ** 2) / np.sum(ref ** 2)))
+
+
+def hermite_combos():
+    """(compute_type, kernel, singularity) the reference accepts for Hermite:
+    PC with either singularity, CP with skipping, no CC
+    (src/interaction_compute/interaction_compute_{pc,cp,cc}.c,
+    interaction_compute_downpass.c:126-128)."""
+    out = [(1, k, s) for k in (1, 2, 3) for s in (1, 2)]
+    out += [(2, k, 1) for k in (1, 2, 3)]
+    return out
