@@ -63,8 +63,13 @@ std::string params_check(const Params &p)
         return "INVALID KERNEL (supported: COULOMB, YUKAWA, REGULARIZED_COULOMB)";
     if (p.singularity != S_SKIPPING && p.singularity != S_SUBTRACTION)
         return "INVALID CHOICE OF SINGULARITY";
-    if (p.approximation == A_HERMITE) return "HERMITE approximation is not implemented on this path";
-    if (p.approximation != A_LAGRANGE) return "INVALID CHOICE OF APPROXIMATION";
+    if (p.approximation != A_LAGRANGE && p.approximation != A_HERMITE) return "INVALID CHOICE OF APPROXIMATION";
+    // Hermite: PC with either singularity, CP with skipping, no CC
+    // (interaction_compute_cc.c:146, interaction_compute_cp.c:133, 187,
+    // interaction_compute_downpass.c:126-128)
+    if (p.approximation == A_HERMITE && p.compute_type == C_CC) return "NOT SET UP FOR CC HERMITE";
+    if (p.approximation == A_HERMITE && p.compute_type == C_CP && p.singularity == S_SUBTRACTION)
+        return "NOT SET UP FOR CP HERMITE SS";
     if (p.compute_type != C_PC && p.compute_type != C_CP && p.compute_type != C_CC)
         return "INVALID COMPUTE TYPE";
     // interaction_compute_cp.c:218, interaction_compute_cc.c:241
@@ -395,6 +400,25 @@ void make_cc_lists(Lists &L, const Tree &src, const Tree &tgt, const Params &prm
     L.cc_cp.assign(tgt.n, {});
     const int size_check = (int)(prm.size_check_factor * prm.P);
     cc_rec(L, src, 0, tgt, 0, prm, size_check);
+}
+
+void hermite_weight_table(const Tree &t, int degree, std::vector<double> &tab)
+{
+    const int nl = degree + 1;
+    tab.assign((size_t)t.n * 3 * nl, 0.0);
+    std::vector<double> ww(nl);
+    for (int i = 0; i < nl; i++) {
+        const double xx = i * M_PI / degree;
+        ww[i] = -std::cos(xx) / (2 * std::sin(xx) * std::sin(xx));
+    }
+    ww[0] = 0.25 * (degree * degree / 3.0 + 1.0 / 6.0);
+    ww[degree] = -ww[0];
+    for (int nd = 0; nd < t.n; nd++) {
+        const double len[3] = {t.xmax[nd] - t.xmin[nd], t.ymax[nd] - t.ymin[nd], t.zmax[nd] - t.zmin[nd]};
+        double *o = &tab[(size_t)nd * 3 * nl];
+        for (int d = 0; d < 3; d++)
+            for (int j = 0; j < nl; j++) o[d * nl + j] = -4.0 * ww[j] / len[d];
+    }
 }
 
 void node_table(const Tree &t, int degree, std::vector<double> &tab)

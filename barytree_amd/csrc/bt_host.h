@@ -108,6 +108,12 @@ void make_cc_lists(Lists &L, const Tree &src, const Tree &tgt, const Params &p);
 // y, then z), computed exactly as src/clusters/clusters.c:436-440.
 void node_table(const Tree &t, int degree, std::vector<double> &tab);
 
+// Hermite barycentric weights: per node, 3*(degree+1) values w_d[j] =
+// -4 ww[j] / (max_d - min_d) with ww[j] = -cos(t_j) / (2 sin^2 t_j),
+// t_j = j*pi/degree, and ww[0] = -ww[degree] = (degree^2/3 + 1/6)/4
+// (src/clusters/clusters.c:1296-1318).
+void hermite_weight_table(const Tree &t, int degree, std::vector<double> &tab);
+
 // Source-range flags: a source may coincide with a target (the singularity
 // skip may trigger); a pair may come closer than its key's threshold.
 constexpr int SEG_COINCIDE = 1, SEG_CLOSE = 2;

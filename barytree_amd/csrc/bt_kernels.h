@@ -78,6 +78,60 @@ struct InteractArgs {
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
+// Hermite approximation (bt_hermite.hip), FP64.  Clusters carry 8 planes
+// per grid point; the planes of node c sit at [8*P*c + m*P + j].
+// Upward pass: block (b, slice) = particles [part_beg[b], part_end[b]) of
+// node part_node[b] against grid points slice*256 + thread; the part's row
+// of 8P values goes to Qp[8P*b] (and its weights to Wp).
+struct HermAnterpArgs {
+    int degree, subtraction, nparts;
+    const int *part_node, *part_beg, *part_end;
+    const double *tab, *wtab;  // per node 3(n+1): Chebyshev nodes; Hermite weights w
+    const double *px, *py, *pz, *pq, *pw;
+    double *Qp, *Wp;
+};
+hipError_t launch_herm_anterp(const HermAnterpArgs &a, hipStream_t st);
+// Particle-cluster far field: tile {key, first target, count} against the
+// clusters clu_id[clu_off[key] ..) of its key, added to pot.
+struct HermPCArgs {
+    int kernel, singularity, ntiles, P;
+    const Tile *tiles;
+    const int *clu_off, *clu_id;
+    const double *cx, *cy, *cz, *cq, *cw;
+    const double *tx, *ty, *tz, *tq;
+    KParams kp;
+    double *pot;
+};
+hipError_t launch_herm_pc(const HermPCArgs &a, hipStream_t st);
+// Cluster-particle far field: tile {key, first grid point, count, node}
+// against the source ranges of its key; grid point j's 8 plane sums go to
+// rows[out_row[key] + m*P + j].
+struct HermCPArgs {
+    int kernel, ntiles, nl;
+    const Tile *tiles;
+    const int *seg_off;
+    const Seg *segs;
+    const double *tab;
+    const double *sx, *sy, *sz, *sq;
+    KParams kp;
+    const long long *out_row;
+    double *rows;
+};
+hipError_t launch_herm_cp(const HermCPArgs &a, hipStream_t st);
+// Downpass: tile {leaf, first target, count}; each target adds the 8-plane
+// interpolants of its leaf and every ancestor n with rows (rows_off[n+1] >
+// rows_off[n]) to pot.
+struct HermDownArgs {
+    int degree, ntiles;
+    const Tile *tiles;
+    const int *parent, *rows_off;
+    const double *tab, *wtab;
+    const double *tx, *ty, *tz;
+    const double *Q;
+    double *pot;
+};
+hipError_t launch_herm_down(const HermDownArgs &a, hipStream_t st);
+
 // Issue cost of VALU instruction kinds on the current GPU (bt_probe.hip):
 // ns per wave instruction and SIMD of FP64 fma, FP64 add, FP32 fma, FP32 mul,
 // FP32 exp2, FP32 rsq and FP64<->FP32 conversion, in that order.

@@ -570,8 +570,10 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         }
     }
 
+    herm_ = prm.approximation == A_HERMITE;
     build_work();
     enumerate_pairs();
+    if (herm_) build_hermite_work();
     const bool need_src_clusters = prm.compute_type != C_CP;
     const bool need_tgt_clusters = prm.compute_type != C_PC;
     std::vector<double> stab, ttab, tab;
@@ -598,6 +600,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     double t4 = now();
     tm_.host_work = t4 - t3;
     if (!(e = validate()).empty()) return "internal work-list check failed: " + e;
+    if (herm_ && !(e = validate_hermite()).empty()) return "internal work-list check failed: " + e;
     if (host_only) return "";
 
     // kernel parameters (src/kernels/*/ *_ss_correction.c and pair kernels)
@@ -643,12 +646,13 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
     if (!(r = up(&d_torder_, g_torder_)).empty()) return r;
     const size_t Pz = prm.P;
+    const size_t planes = herm_ ? 8 : 1;  // Hermite: 8 planes per grid point
     if (need_src_clusters) {
         if (!(r = up(&d_stab_, stab)).empty()) return r;
         const size_t np = (size_t)nsnodes_ * Pz;
         d_cx_ = dalloc<double>(np); d_cy_ = dalloc<double>(np); d_cz_ = dalloc<double>(np);
-        d_cq_ = dalloc<double>(np);
-        if (ss) d_cw_ = dalloc<double>(np);
+        d_cq_ = dalloc<double>(np * planes);
+        if (ss) d_cw_ = dalloc<double>(np * planes);
         if (!d_cx_ || !d_cy_ || !d_cz_ || !d_cq_ || (ss && !d_cw_)) return "hipMalloc failed (source clusters)";
         if (!(r = up(&d_sibeg_, g_sibeg_)).empty()) return r;
         if (!(r = up(&d_siend_, g_siend_)).empty()) return r;
@@ -660,8 +664,8 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     if (need_tgt_clusters) {
         if (!(r = up(&d_ttab_, ttab)).empty()) return r;
         const size_t np = (size_t)ntnodes_ * Pz;
-        d_Qt_ = dalloc<double>(np);
-        if (ss) d_Wt_ = dalloc<double>(np);
+        d_Qt_ = dalloc<double>(np * planes);
+        if (ss) d_Wt_ = dalloc<double>(np * planes);
         if (!d_Qt_ || (ss && !d_Wt_)) return "hipMalloc failed (target clusters)";
         if (!(r = up(&d_tparent_, g_tparent_)).empty()) return r;
         if (!(r = up(&d_down_nodes_, down_nodes_)).empty()) return r;
@@ -698,6 +702,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     d_out_ = dalloc<double>(M_);
     d_cursor_ = dalloc<int>(2 * NXCD);
     if (!d_pot_ || !d_out_ || !d_cursor_) return "hipMalloc failed (potential)";
+    if (herm_ && !(r = upload_hermite()).empty()) return r;
     HIPCHK(hipStreamSynchronize(st_));
     tm_.h2d = now() - t4;
     return "";
@@ -714,7 +719,10 @@ std::string Plan::execute(bool timed)
     HIPCHK(hipEventRecord(ev[0], st_));
 
     // 1. upward pass over the source trees (clusters.c:50-200)
-    if (p.compute_type != C_CP) {
+    if (herm_ && p.compute_type == C_PC) {
+        std::string r = hermite_upward();
+        if (!r.empty()) return r;
+    } else if (p.compute_type != C_CP) {
         HIPCHK(btk::launch_cluster_points(nsnodes_, nl, d_stab_, d_cx_, d_cy_, d_cz_, st_));
         btk::AnterpArgs a{};
         a.degree = p.degree; a.subtraction = ss;
@@ -758,10 +766,17 @@ std::string Plan::execute(bool timed)
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
     }
+    if (herm_ && p.compute_type == C_PC) {
+        std::string r = hermite_pc();
+        if (!r.empty()) return r;
+    }
     HIPCHK(hipEventRecord(ev[2], st_));
 
     // 3. far field into target-cluster grids (CP, CC)
-    if (p.compute_type != C_PC) {
+    if (herm_ && p.compute_type == C_CP) {
+        std::string r = hermite_cp();
+        if (!r.empty()) return r;
+    } else if (p.compute_type != C_PC) {
         if (!grid_tiles_.empty()) {
             btk::InteractArgs a = ia;
             a.grid_targets = 1; a.ntiles = (int)grid_tiles_.size(); a.tiles = d_grid_tiles_;
@@ -779,7 +794,10 @@ std::string Plan::execute(bool timed)
     HIPCHK(hipEventRecord(ev[3], st_));
 
     // 4. downpass over the target trees (interaction_compute_downpass.c:37-142)
-    if (p.compute_type != C_PC) {
+    if (herm_ && p.compute_type == C_CP) {
+        std::string r = hermite_down();
+        if (!r.empty()) return r;
+    } else if (p.compute_type != C_PC) {
         btk::DownArgs d{};
         d.degree = p.degree; d.subtraction = ss; d.tab = d_ttab_; d.parent = d_tparent_;
         d.tx = d_tx_; d.ty = d_ty_; d.tz = d_tz_; d.tq = d_tq_; d.Q = d_Qt_; d.W = d_Wt_; d.pot = d_pot_;

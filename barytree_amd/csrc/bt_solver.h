@@ -164,6 +164,14 @@ private:
     void free_var();
     std::string validate_var() const;
     std::string execute_var(bool timed);
+    // Hermite approximation (bt_hermite_plan.cpp)
+    void build_hermite_work();
+    std::string validate_hermite() const;
+    std::string upload_hermite();
+    std::string hermite_upward();
+    std::string hermite_pc();
+    std::string hermite_cp();
+    std::string hermite_down();
 
     Params prm_;
     int M_ = 0, N_ = 0, tpt_ = 2, tile_targets_ = 512;
@@ -238,6 +246,20 @@ private:
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;
     double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;
+    // Hermite: 8 planes per grid point.  Upward pass from every source
+    // node's particles in parts (node n: parts [hp_off_[n], hp_off_[n+1]));
+    // PC: the near keys' clusters in their own launch (hpc_, hpc_tiles_);
+    // CP: grid tiles of 256 points (hcp_tiles_), rows of 8P values; the
+    // downpass of every target node with rows (hdown_tiles_)
+    bool herm_ = false;
+    std::vector<int> hp_node_, hp_beg_, hp_end_, hp_off_;
+    WorkCSR hpc_;
+    std::vector<btk::Tile> hpc_tiles_, hcp_tiles_, hdown_tiles_;
+    std::vector<double> swtab_, twtab_;
+    int *d_hp_node_ = nullptr, *d_hp_beg_ = nullptr, *d_hp_end_ = nullptr, *d_hp_off_ = nullptr;
+    double *d_Qp_ = nullptr, *d_Wp_ = nullptr, *d_swtab_ = nullptr, *d_twtab_ = nullptr;
+    int *d_hpc_clu_off_ = nullptr, *d_hpc_clu_id_ = nullptr;
+    btk::Tile *d_hpc_tiles_ = nullptr, *d_hcp_tiles_ = nullptr, *d_hdown_tiles_ = nullptr;
 };
 
 }  // namespace bt

@@ -136,6 +136,7 @@ std::string Plan::set_pair_degrees(const int *deg)
 {
     free_var();
     if (!deg) return "";
+    if (herm_) return "per-pair degrees need the Lagrange approximation";
     const long long np = (long long)pairs_.size();
     for (long long i = 0; i < np; i++)
         if (deg[i] < 1 || deg[i] > MAXDEG) return "pair degree outside [1, 12]";

@@ -105,8 +105,8 @@ def test_errors_are_reported(bt):
     t, s = uniform_cube(100, 100, seed=1)
     with pytest.raises(RuntimeError, match="weights"):
         bt.solve(t, s[:4] + [None], 1, [0.5], 2, 1, 1, 0.7, 3, 10, 10)
-    with pytest.raises(RuntimeError, match="HERMITE"):
-        bt.solve(t, s, 1, [0.5], 1, 2, 1, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="CC HERMITE"):
+        bt.solve(t, s, 1, [0.5], 1, 2, 3, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="REGULARIZED"):
         bt.solve(t, s, 3, [0.3, 0.05], 2, 1, 3, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError):

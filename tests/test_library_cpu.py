@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from tests.helpers import uniform_cube
+from tests.helpers import KP, hermite_combos, uniform_cube
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "BaryTreeInterface.h")
@@ -68,6 +68,18 @@ def test_host_lists_match_oracle(bt, oracle_mod, ct, n, m, deg, msl, mtl, theta)
     assert a["pp_ptwise"] + a["pc_replaced"] + a["cp_replaced"] + a["cc_replaced"] == n * m
 
 
+@pytest.mark.parametrize("ct,k,s", hermite_combos())
+def test_hermite_host_work_matches_oracle(bt, oracle_mod, ct, k, s):
+    """Hermite plans build (and check) their work lists on the host; the
+    interaction counts are the Lagrange ones (the lists do not depend on the
+    approximation)."""
+    t, src = uniform_cube(4000, 3000, seed=11 + ct + k)
+    a = bt.analyze(t, src, k, KP[k][s], s, 2, ct, 0.7, 3, 60, 50)
+    oracle_mod.treecode(*t, *src, k, KP[k][s], s, 2, ct, 0.7, 3, 60, 50)
+    o = oracle_mod.last_stats()
+    assert {x: a[x] for x in KEYS} == {x: o[x] for x in KEYS}
+
+
 def test_beta_resolution_matches_oracle(bt, oracle_mod):
     t, s = uniform_cube(3000, 3000, seed=4)
     for ct in (1, 2, 3):
@@ -82,8 +94,12 @@ def test_beta_resolution_matches_oracle(bt, oracle_mod):
 
 def test_unsupported_combinations_are_errors(bt):
     t, s = uniform_cube(100, 100, seed=1)
-    with pytest.raises(RuntimeError, match="HERMITE"):
-        bt.analyze(t, s, 1, [0.5], 1, 2, 1, 0.7, 3, 10, 10)
+    # Hermite: no CC, no CP with subtraction (interaction_compute_cc.c:146,
+    # interaction_compute_downpass.c:126-128)
+    with pytest.raises(RuntimeError, match="CC HERMITE"):
+        bt.analyze(t, s, 1, [0.5], 1, 2, 3, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="CP HERMITE SS"):
+        bt.analyze(t, s, 1, [0.5], 2, 2, 2, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="REGULARIZED"):
         bt.analyze(t, s, 3, [0.3, 0.05], 2, 1, 2, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="KERNEL"):
@@ -144,8 +160,8 @@ def test_batched_argument_errors(bt):
         bt.analyze_batched(systems, *args)
     with pytest.raises(RuntimeError, match="targets_per_thread"):
         bt.analyze_batched(systems, *args, schedule=bt.Schedule.make({}, targets_per_thread=3))
-    with pytest.raises(RuntimeError, match="HERMITE"):
-        bt.analyze_batched(systems, 1, [0.5], 1, 2, 1, 0.7, 3, max_source_leaf=10,
+    with pytest.raises(RuntimeError, match="CC HERMITE"):
+        bt.analyze_batched(systems, 1, [0.5], 1, 2, 3, 0.7, 3, max_source_leaf=10,
                            max_target_leaf=10)
     with pytest.raises(RuntimeError, match="leaf sizes must be positive"):
         bt.analyze_batched(systems, *args, max_source_leaf=[10, 0], max_target_leaf=10)
