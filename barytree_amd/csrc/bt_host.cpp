@@ -59,10 +59,24 @@ void params_setup(Params &p, int kernel, int num_kp, const double *kp, int appro
 
 std::string params_check(const Params &p)
 {
-    if (p.kernel != K_COULOMB && p.kernel != K_YUKAWA && p.kernel != K_REG_COULOMB)
-        return "INVALID KERNEL (supported: COULOMB, YUKAWA, REGULARIZED_COULOMB)";
+    const bool other = p.kernel == K_REG_YUKAWA || p.kernel == K_ATAN || p.kernel == K_SIN_OVER_R ||
+                       p.kernel == K_MQ || p.kernel == K_RBS_U || p.kernel == K_RBS_V || p.kernel == K_USER;
+    if (!coulomb_family(p.kernel) && !other)
+        return "INVALID KERNEL (supported: COULOMB, YUKAWA, REGULARIZED_COULOMB, REGULARIZED_YUKAWA, "
+               "ATAN, SIN_OVER_R, MQ, RBS_U, RBS_V, USER)";
     if (p.singularity != S_SKIPPING && p.singularity != S_SUBTRACTION)
         return "INVALID CHOICE OF SINGULARITY";
+    if (other) {
+        // the combinations the dispatchers compute as written
+        // (interaction_compute_pc.c:259-435, _cp.c:255-384, _cc.c:275-403)
+        if (p.approximation != A_LAGRANGE) return "NOT SET UP FOR HERMITE WITH THIS KERNEL";
+        if ((p.kernel == K_ATAN || p.kernel == K_MQ) && p.compute_type != C_PC)
+            return "ATAN AND MQ ARE SET UP FOR PARTICLE-CLUSTER ONLY";
+        if (p.singularity == S_SUBTRACTION && !(p.kernel == K_REG_YUKAWA && p.compute_type == C_PC))
+            return "SUBTRACTION WITH THIS KERNEL IS SET UP FOR PC REGULARIZED_YUKAWA ONLY";
+        const int need = (p.kernel == K_REG_YUKAWA || p.kernel == K_ATAN || p.kernel == K_MQ) ? 2 : 1;
+        if (p.num_kp < need) return "this kernel needs more kernel parameters";
+    }
     if (p.approximation != A_LAGRANGE && p.approximation != A_HERMITE) return "INVALID CHOICE OF APPROXIMATION";
     // Hermite: PC with either singularity, CP with skipping, no CC
     // (interaction_compute_cc.c:146, interaction_compute_cp.c:133, 187,

@@ -15,7 +15,13 @@
 namespace bt {
 
 // Enum values are the reference ABI (src/interface/BaryTreeInterface.h:7-47).
-enum KernelId { K_NONE = 0, K_COULOMB = 1, K_YUKAWA = 2, K_REG_COULOMB = 3 };
+enum KernelId {
+    K_NONE = 0, K_COULOMB = 1, K_YUKAWA = 2, K_REG_COULOMB = 3, K_REG_YUKAWA = 4, K_ATAN = 5,
+    K_SIN_OVER_R = 8, K_MQ = 9, K_RBS_U = 10, K_RBS_V = 11, K_USER = 12
+};
+// the Coulomb family (mixed precision, Hermite, learned degrees); the other
+// kernels run the Lagrange approximation in FP64
+inline bool coulomb_family(int k) { return k == K_COULOMB || k == K_YUKAWA || k == K_REG_COULOMB; }
 enum SingularityId { S_NONE = 0, S_SKIPPING = 1, S_SUBTRACTION = 2 };
 enum ApproxId { A_NONE = 0, A_LAGRANGE = 1, A_HERMITE = 2 };
 enum ComputeId { C_NONE = 0, C_PC = 1, C_CP = 2, C_CC = 3 };

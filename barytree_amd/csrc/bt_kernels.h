@@ -31,9 +31,17 @@ constexpr int SLOT_MASK = (1 << DEG_SHIFT) - 1;
 struct KParams {
     double p0, p1;      // kernel_params[0], kernel_params[1]
     double inv_a2;      // 1/kappa^2 (Coulomb SS), 1/alpha^2 (reg. Coulomb SS)
-    double eps2;        // epsilon^2 (regularized Coulomb)
+    double eps2;        // epsilon^2 (regularized Coulomb and Yukawa)
     double ss_param;    // subtraction correction factor
+    // atan: delta factor sqrt(1 + 1/delta^2) and wadj; mq: (delta/L)^2 and
+    // sqrt(1 + 4 (delta/L)^2); rbs-u, rbs-v: delta^2 (in c1)
+    double c1, c2;
 };
+
+// Kernel ids beside Coulomb (1), Yukawa (2) and regularized Coulomb (3)
+// (src/utilities/enums.h): these run in FP64 whatever the precision asked.
+constexpr int KERNEL_REG_YUKAWA = 4, KERNEL_ATAN = 5, KERNEL_SIN_OVER_R = 8, KERNEL_MQ = 9,
+              KERNEL_RBS_U = 10, KERNEL_RBS_V = 11, KERNEL_USER = 12;
 
 // Interaction work: a list of tiles {key, first target, target count, node}
 // against CSR lists of source particle ranges and source clusters.

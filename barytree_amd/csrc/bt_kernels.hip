@@ -50,6 +50,54 @@ __device__ __forceinline__ double rsq64(double x)
 }
 
 // ---------------------------------------------------------------------
+// The kernels beside the Coulomb family, FP64, one pair's term with the
+// charge a (b, tq: subtraction weight and target charge):
+//   regularized Yukawa SK: a e^{-kr}/sqrt(r^2+e^2)   SS: (a - tq b) e^{-kr}/sqrt(r^2+e^2)
+//   atan (z, period L)   : a (atan(f tan(pi d))/pi - d), d = (dz/L) wrapped to [-1/2, 1/2];
+//                          the sum is scaled by wadj afterwards
+//   sin-over-r           : a sin(kr)/r        (particle sources: skip r <= DBL_MIN)
+//   mq (z, period L)     : a (d c2 / (2 sqrt(d^2 + c1)) - d)
+//   rbs-u, rbs-v (x, y)  : a dy / (2 pi (dx^2+dy^2+delta^2)), -a dx / (2 pi (...))
+//   user                 : a e^{-kr}/r        (particle sources: skip r <= DBL_MIN)
+// (regularized-yukawa_{pp,ss_pp,pc,ss_pc,cp}.c, atan_{pp,pc}.c,
+// sin-over-r_{pp,pc,cp}.c, mq_{pp,pc}.c, rbs-u_*.c, rbs-v_*.c, user_kernel_*.c)
+// ---------------------------------------------------------------------
+constexpr int KRY = KERNEL_REG_YUKAWA, KAT = KERNEL_ATAN, KSR = KERNEL_SIN_OVER_R, KMQ = KERNEL_MQ,
+              KRU = KERNEL_RBS_U, KRV = KERNEL_RBS_V, KUS = KERNEL_USER;
+
+__device__ __forceinline__ double wrap_half(double d)
+{
+    if (d < -0.5) d += 1.0;
+    if (d > 0.5) d -= 1.0;
+    return d;
+}
+
+template <int KER, bool SS, bool CHECK>
+__device__ __forceinline__ double other_term(double dx, double dy, double dz, double r2, double a, double b,
+                                             double tq, const KParams &kp)
+{
+    if constexpr (KER == KRY) {
+        const double r = sqrt(r2);
+        const double e = exp(-kp.p0 * r) / sqrt(r2 + kp.eps2);
+        return (SS ? fma(-tq, b, a) : a) * e;
+    } else if constexpr (KER == KAT) {
+        const double d = wrap_half(dz / kp.p0);
+        return a * (1.0 / M_PI * atan(kp.c1 * tan(M_PI * d)) - d);
+    } else if constexpr (KER == KMQ) {
+        const double d = wrap_half(dz / kp.p0);
+        return a * (0.5 * d * kp.c2 / sqrt(d * d + kp.c1) - d);
+    } else if constexpr (KER == KRU || KER == KRV) {
+        const double r = fma(dy, dy, dx * dx) + kp.c1;
+        return (KER == KRU ? 1.0 : -1.0) / (2.0 * M_PI) * a * (KER == KRU ? dy : dx) / r;
+    } else {  // KSR, KUS
+        const double r = sqrt(r2);
+        double v = (KER == KSR ? sin(kp.p0 * r) : exp(-kp.p0 * r)) / r;
+        if constexpr (CHECK) v = r > DBL_MIN ? v : 0.0;
+        return a * v;
+    }
+}
+
+// ---------------------------------------------------------------------
 // Pair kernels.  A staged source point carries (a, b):
 //   particles: a = q (skipping) or q*w (subtraction), b = w
 //   clusters : a = Q, b = W (subtraction weights)
@@ -81,10 +129,12 @@ __device__ __forceinline__ void pair_pot(double dx, double dy, double dz, double
         if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
         if constexpr (!SS) acc = fma(a, e, acc);
         else acc = fma(fma(-tq, b, a), e, acc);
-    } else {
+    } else if constexpr (KER == KR) {
         const double ir = rsq64(r2 + kp.eps2);
         if constexpr (!SS) acc = fma(a, ir, acc);
         else acc = fma(fma(-tq * b, exp(-r2 * kp.inv_a2), a), ir, acc);
+    } else {
+        acc += other_term<KER, SS, CHECK>(dx, dy, dz, r2, a, b, tq, kp);
     }
 }
 
@@ -109,8 +159,10 @@ __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, doubl
         const double e = exp(-kp.p0 * (r2 * ir)) * ir;
         q = fma(a, e, q);
         if constexpr (SS) w = fma(b, e, w);
-    } else {
+    } else if constexpr (KER == KR) {
         q = fma(a, rsq64(r2 + kp.eps2), q);
+    } else {
+        q += other_term<KER, false, false>(dx, dy, dz, r2, a, 0.0, 0.0, kp);
     }
 }
 
@@ -437,6 +489,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
 #pragma unroll
     for (int u = 0; u < TPT; u++) {
         if (tix[u] < 0) continue;
+        if constexpr (KER == KAT && !GRID) acc[u] *= A.kp.c2;  // atan_pp.c:58, atan_pc.c:58
         if constexpr (GRID) {
             const size_t o = orow + tix[u];
             A.out1[o] = acc[u];
@@ -789,6 +842,19 @@ hipError_t near_mfma_launch(const InteractArgs &a, hipStream_t st)
     k_zero_ints<<<1, 64, 0, st>>>(a.cursor, NQ);
     kern<<<dim3(nb), dim3(TPB), 0, st>>>(a);
     return hipGetLastError();
+}
+
+// The kernels beside the Coulomb family: FP64 whatever the precision asked
+// (no subtraction on grid targets: the reference has none for them).
+template <int KER, bool SS>
+hipError_t plain_dispatch(const InteractArgs &a, hipStream_t st)
+{
+    if (a.grid_targets) {
+        if constexpr (SS) return hipErrorInvalidValue;
+        else return interact_launch<KER, false, true, false, GRID_TPT>(a, st);
+    }
+    if (a.tpt == 1) return interact_launch<KER, SS, false, false, 1>(a, st);
+    return interact_launch<KER, SS, false, false, 2>(a, st);
 }
 
 template <int KER, bool SS>
@@ -1189,6 +1255,7 @@ __global__ __launch_bounds__(256) void k_direct(KParams kp, int m, const double 
             pair_pot<KER, SS, true>(x - s.x, y - s.y, z - s.z, s.w, Lb[j], q, kp, acc);
         }
     }
+    if constexpr (KER == KAT) acc *= kp.c2;
     if (i < m) part[(size_t)blockIdx.y * m + i] = acc;
 }
 
@@ -1212,6 +1279,13 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
         case KC: return ss ? interact_dispatch<KC, true>(a, st) : interact_dispatch<KC, false>(a, st);
         case KY: return ss ? interact_dispatch<KY, true>(a, st) : interact_dispatch<KY, false>(a, st);
         case KR: return ss ? interact_dispatch<KR, true>(a, st) : interact_dispatch<KR, false>(a, st);
+        case KRY: return ss ? plain_dispatch<KRY, true>(a, st) : plain_dispatch<KRY, false>(a, st);
+        case KAT: return ss ? hipErrorInvalidValue : plain_dispatch<KAT, false>(a, st);
+        case KSR: return ss ? hipErrorInvalidValue : plain_dispatch<KSR, false>(a, st);
+        case KMQ: return ss ? hipErrorInvalidValue : plain_dispatch<KMQ, false>(a, st);
+        case KRU: return ss ? hipErrorInvalidValue : plain_dispatch<KRU, false>(a, st);
+        case KRV: return ss ? hipErrorInvalidValue : plain_dispatch<KRV, false>(a, st);
+        case KUS: return ss ? hipErrorInvalidValue : plain_dispatch<KUS, false>(a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1281,6 +1355,13 @@ hipError_t launch_direct(int kernel, int singularity, KParams kp, int m, const d
         case KC: if (ss) L(KC, true); else L(KC, false); break;
         case KY: if (ss) L(KY, true); else L(KY, false); break;
         case KR: if (ss) L(KR, true); else L(KR, false); break;
+        case KRY: if (ss) L(KRY, true); else L(KRY, false); break;
+        case KAT: L(KAT, false); break;
+        case KSR: L(KSR, false); break;
+        case KMQ: L(KMQ, false); break;
+        case KRU: L(KRU, false); break;
+        case KRV: L(KRV, false); break;
+        case KUS: L(KUS, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef L

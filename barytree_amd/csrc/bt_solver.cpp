@@ -612,6 +612,23 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         kp_.ss_param = 2.0 * M_PI * prm.kp[0] * prm.kp[0];
     } else if (prm.kernel == K_YUKAWA) {
         kp_.ss_param = 4.0 * M_PI / prm.kp[0] / prm.kp[0];
+    } else if (prm.kernel == K_REG_YUKAWA) {
+        // (kappa, epsilon); regularized-yukawa_ss_correction.c:12-14
+        kp_.eps2 = prm.kp[1] * prm.kp[1];
+        kp_.ss_param = 4.0 * M_PI / prm.kp[0] / prm.kp[0];
+    } else if (prm.kernel == K_ATAN) {
+        // (domain length, delta): atan_pp.c:24-28
+        const double delta = prm.kp[1];
+        kp_.c1 = std::sqrt(1. + 1.0 / (delta * delta));
+        kp_.c2 = 1. / (1. - delta / std::sqrt(1. + delta * delta));
+    } else if (prm.kernel == K_MQ) {
+        // (domain length, delta): mq_pp.c:24-27
+        kp_.c1 = prm.kp[1] * prm.kp[1] / prm.kp[0] / prm.kp[0];
+        kp_.c2 = std::sqrt(1 + 4 * kp_.c1);
+    } else if (prm.kernel == K_RBS_U || prm.kernel == K_RBS_V) {
+        kp_.c1 = prm.kp[0] * prm.kp[0];  // delta^2
+    } else if (prm.kernel == K_SIN_OVER_R || prm.kernel == K_USER) {
+        // k = kernel_params[0] (p0)
     } else {
         if (prm.singularity == S_SUBTRACTION) {
             kp_.inv_a2 = 1.0 / (prm.kp[0] * prm.kp[0]);

@@ -137,6 +137,8 @@ std::string Plan::set_pair_degrees(const int *deg)
     free_var();
     if (!deg) return "";
     if (herm_) return "per-pair degrees need the Lagrange approximation";
+    if (!coulomb_family(prm_.kernel))
+        return "per-pair degrees are set up for the Coulomb, Yukawa and regularized Coulomb kernels";
     const long long np = (long long)pairs_.size();
     for (long long i = 0; i < np; i++)
         if (deg[i] < 1 || deg[i] > MAXDEG) return "pair degree outside [1, 12]";
@@ -447,6 +449,8 @@ std::string Plan::validate_var() const
 std::string Plan::label_pairs(int first, int count, double tol, int *labels, double *err)
 {
     if (!st_) return "plan has no device data";
+    if (!coulomb_family(prm_.kernel))
+        return "pair labels are set up for the Coulomb, Yukawa and regularized Coulomb kernels";
     if (first < 0 || count < 0 || (long long)first + count > (long long)pairs_.size())
         return "pair range out of bounds";
     if (count == 0) return "";

@@ -38,7 +38,8 @@
 #define M_PI 3.14159265358979323846
 #endif
 
-enum { KER_NONE = 0, KER_COULOMB = 1, KER_YUKAWA = 2, KER_REGCOULOMB = 3 };
+enum { KER_NONE = 0, KER_COULOMB = 1, KER_YUKAWA = 2, KER_REGCOULOMB = 3, KER_REGYUKAWA = 4, KER_ATAN = 5,
+       KER_SINOVERR = 8, KER_MQ = 9, KER_RBSU = 10, KER_RBSV = 11, KER_USER = 12 };
 enum { SNG_NONE = 0, SNG_SKIP = 1, SNG_SUB = 2 };
 enum { APX_NONE = 0, APX_LAGRANGE = 1, APX_HERMITE = 2 };
 enum { CT_NONE = 0, CT_PC = 1, CT_CP = 2, CT_CC = 3 };
@@ -644,10 +645,167 @@ static int clusters_targets(Clus *c, const Tree *t, const Prm *prm)
  * K_*_PP and K_*_SS_PP (coulomb_pp.c:16-58, coulomb_ss_pp.c:18-57,
  * yukawa_pp.c:19-54, yukawa_ss_pp.c:19-57, regularized-coulomb_pp.c:18-58,
  * regularized-coulomb_ss_pp.c:19-56). */
+/* ------------------------------------------------------------------ */
+/* The kernels beside the Coulomb family, one pair's term as each of    */
+/* their sources writes it (src/kernels/<name>/<name>_{pp,pc,cp}.c).    */
+/* ATAN and MQ act on z alone (periodic in domainLength = kp[0], delta  */
+/* = kp[1]); RBS_U and RBS_V on (x, y) (delta = kp[0]); SIN_OVER_R and  */
+/* USER take k = kp[0]; REGULARIZED_YUKAWA (kappa, epsilon).            */
+/* ------------------------------------------------------------------ */
+
+static int other_kernel(int k) { return k == KER_REGYUKAWA || k == KER_ATAN || (k >= KER_SINOVERR && k <= KER_USER); }
+
+/* periodic z offset (atan_pp.c:47-53, mq_pp.c:47-53) */
+static double wrap_dz(double tz, double sz, double L)
+{
+    double dz = (tz - sz) / L;
+    if (dz < -0.5) dz += 1.0;
+    if (dz > 0.5) dz -= 1.0;
+    return dz;
+}
+
+/* the factor ATAN applies to each kernel call's sum (atan_pp.c:26-28, 58) */
+static double atan_wadj(const Prm *prm)
+{
+    double delta = prm->kp[1];
+    return 1. / (1. - delta / sqrt(1. + delta * delta));
+}
+
+/* PP: regularized-yukawa_pp.c:40-43, regularized-yukawa_ss_pp.c:44-48,
+ * atan_pp.c:44-55, sin-over-r_pp.c:40-46, mq_pp.c:44-55, rbs-u_pp.c:41-44,
+ * rbs-v_pp.c:41-44, user_kernel_pp.c:40-46 */
+static double other_pp(const Prm *prm, double tx, double ty, double tz, double tq, double sx, double sy,
+                       double sz, double q, double w)
+{
+    double dx = tx - sx, dy = ty - sy, dz = tz - sz;
+    switch (prm->kernel) {
+    case KER_REGYUKAWA: {
+        double kappa = prm->kp[0];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (prm->singularity == SNG_SUB) {
+            double epsilon = prm->kp[1];
+            return (q - tq) * w * exp(-kappa * r) / sqrt(r * r + epsilon * epsilon);
+        }
+        double epsilon2 = prm->kp[1] * prm->kp[1];
+        return q * exp(-kappa * r) / sqrt(r * r + epsilon2);
+    }
+    case KER_ATAN: {
+        double delta = prm->kp[1];
+        double delta_factor = sqrt(1. + 1.0 / (delta * delta));
+        double d = wrap_dz(tz, sz, prm->kp[0]);
+        return q * (1.0 / M_PI * atan(delta_factor * tan(M_PI * d)) - d);
+    }
+    case KER_MQ: {
+        double L = prm->kp[0], delta = prm->kp[1];
+        double deltaLsq = delta * delta / L / L;
+        double norm_delta_L = sqrt(1 + 4 * deltaLsq);
+        double d = wrap_dz(tz, sz, L);
+        return q * (.5 * d * norm_delta_L / sqrt(d * d + deltaLsq) - d);
+    }
+    case KER_SINOVERR: {
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return r > DBL_MIN ? q * sin(prm->kp[0] * r) / r : 0.0;
+    }
+    case KER_RBSU: {
+        double r = dx * dx + dy * dy + prm->kp[0] * prm->kp[0];
+        return 1. / (2. * M_PI) * q * dy / r;
+    }
+    case KER_RBSV: {
+        double r = dx * dx + dy * dy + prm->kp[0] * prm->kp[0];
+        return -1. / (2. * M_PI) * q * dx / r;
+    }
+    default: { /* KER_USER */
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return r > DBL_MIN ? q * exp(-prm->kp[0] * r) / r : 0.0;
+    }
+    }
+}
+
+/* PC: target against a cluster point with charge Q (and weight W):
+ * regularized-yukawa_pc.c:39-44, regularized-yukawa_ss_pc.c:42-47,
+ * atan_pc.c:44-55, sin-over-r_pc.c:41-46, mq_pc.c:44-55, rbs-u_pc.c:39-44,
+ * rbs-v_pc.c:39-44, user_kernel_pc.c:38-43 (no skip in any of them) */
+static double other_pc(const Prm *prm, double tx, double ty, double tz, double tq, double cx, double cy,
+                       double cz, double Q, double W)
+{
+    double dx = tx - cx, dy = ty - cy, dz = tz - cz;
+    switch (prm->kernel) {
+    case KER_REGYUKAWA: {
+        double kappa = prm->kp[0];
+        if (prm->singularity == SNG_SUB) {
+            double epsilon = prm->kp[1];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            return (Q - tq * W) * exp(-kappa * r) / sqrt(r * r + epsilon * epsilon);
+        }
+        double epsilon2 = prm->kp[1] * prm->kp[1];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double r = sqrt(r2);
+        return Q * exp(-kappa * r) / sqrt(r2 + epsilon2);
+    }
+    case KER_SINOVERR: {
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return Q * sin(prm->kp[0] * r) / r;
+    }
+    case KER_USER: {
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return Q * exp(-prm->kp[0] * r) / r;
+    }
+    default: /* ATAN, MQ, RBS: the PP expressions (no skip there either) */
+        return other_pp(prm, tx, ty, tz, tq, cx, cy, cz, Q, 0.0);
+    }
+}
+
+/* CP (and CC on cluster charges): grid point c against a source s:
+ * regularized-yukawa_cp.c:40-45, sin-over-r_cp.c:40-45, rbs-u_cp.c:41-45,
+ * rbs-v_cp.c:41-45, user_kernel_cp.c:38-43 */
+static double other_cp(const Prm *prm, double cx, double cy, double cz, double sx, double sy, double sz,
+                       double q)
+{
+    double dx = cx - sx, dy = cy - sy, dz = cz - sz;
+    switch (prm->kernel) {
+    case KER_REGYUKAWA: {
+        double kappa = prm->kp[0], epsilon2 = prm->kp[1] * prm->kp[1];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double r = sqrt(r2);
+        return q * exp(-kappa * r) / sqrt(r2 + epsilon2);
+    }
+    case KER_SINOVERR: {
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return q * sin(prm->kp[0] * r) / r;
+    }
+    case KER_RBSU: {
+        double r = dx * dx + dy * dy + prm->kp[0] * prm->kp[0];
+        return 1. / (2. * M_PI) * q * dy / r;
+    }
+    case KER_RBSV: {
+        double r = dx * dx + dy * dy + prm->kp[0] * prm->kp[0];
+        return -1. / (2. * M_PI) * q * dx / r;
+    }
+    default: { /* KER_USER */
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        return q * exp(-prm->kp[0] * r) / r;
+    }
+    }
+}
+
 static void k_pp(const Prm *prm, int nt, int ns, int ts, int ss, const Parts *T, const Parts *S, double *pot)
 {
     const double *sx = S->x, *sy = S->y, *sz = S->z, *sq = S->q, *sw = S->w;
     int sub = prm->singularity == SNG_SUB;
+    if (other_kernel(prm->kernel)) {
+        double wadj = prm->kernel == KER_ATAN ? atan_wadj(prm) : 1.0;
+        for (int i = 0; i < nt; i++) {
+            int ii = ts + i;
+            double temporary_potential = 0.0;
+            for (int j = 0; j < ns; j++) {
+                int jj = ss + j;
+                temporary_potential += other_pp(prm, T->x[ii], T->y[ii], T->z[ii], T->q[ii], sx[jj], sy[jj], sz[jj],
+                                                sq[jj], sw[jj]);
+            }
+            pot[ii] += prm->kernel == KER_ATAN ? wadj * temporary_potential : temporary_potential;
+        }
+        return;
+    }
     for (int i = 0; i < nt; i++) {
         int ii = ts + i;
         double temporary_potential = 0.0;
@@ -698,6 +856,20 @@ static void k_pp(const Prm *prm, int nt, int ns, int ts, int ss, const Parts *T,
 static void k_pc(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *T, const Clus *C, double *pot)
 {
     int sub = prm->singularity == SNG_SUB;
+    if (other_kernel(prm->kernel)) {
+        double wadj = prm->kernel == KER_ATAN ? atan_wadj(prm) : 1.0;
+        for (int i = 0; i < nt; i++) {
+            int ii = ts + i;
+            double temporary_potential = 0.0;
+            for (int j = 0; j < P; j++) {
+                size_t jj = cs + j;
+                temporary_potential += other_pc(prm, T->x[ii], T->y[ii], T->z[ii], T->q[ii], C->x[jj], C->y[jj],
+                                                C->z[jj], C->q[jj], sub ? C->w[jj] : 0.0);
+            }
+            pot[ii] += prm->kernel == KER_ATAN ? wadj * temporary_potential : temporary_potential;
+        }
+        return;
+    }
     for (int i = 0; i < nt; i++) {
         int ii = ts + i;
         double temporary_potential = 0.0;
@@ -742,6 +914,18 @@ static void k_pc(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *
 static void k_cp(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *S, Clus *C)
 {
     int sub = prm->singularity == SNG_SUB;
+    if (other_kernel(prm->kernel)) {
+        for (int i = 0; i < P; i++) {
+            double temporary_potential = 0.0;
+            for (int j = 0; j < ns; j++) {
+                int jj = ss + j;
+                temporary_potential += other_cp(prm, C->x[cs + i], C->y[cs + i], C->z[cs + i], S->x[jj], S->y[jj],
+                                                S->z[jj], S->q[jj]);
+            }
+            C->q[cs + i] += temporary_potential;
+        }
+        return;
+    }
     for (int i = 0; i < P; i++) {
         double temporary_potential = 0.0, temporary_weight = 0.0;
         double cx = C->x[cs + i], cy = C->y[cs + i], cz = C->z[cs + i];
@@ -784,6 +968,18 @@ static void k_cp(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *
 static void k_cc(const Prm *prm, int P, size_t ss, size_t ts, const Clus *S, Clus *C)
 {
     int sub = prm->singularity == SNG_SUB;
+    if (other_kernel(prm->kernel)) {
+        for (int i = 0; i < P; i++) {
+            double temporary_potential = 0.0;
+            for (int j = 0; j < P; j++) {
+                size_t jj = ss + j;
+                temporary_potential += other_cp(prm, C->x[ts + i], C->y[ts + i], C->z[ts + i], S->x[jj], S->y[jj],
+                                                S->z[jj], S->q[jj]);
+            }
+            C->q[ts + i] += temporary_potential;
+        }
+        return;
+    }
     for (int i = 0; i < P; i++) {
         double temporary_potential = 0.0, temporary_weight = 0.0;
         double cx = C->x[ts + i], cy = C->y[ts + i], cz = C->z[ts + i];
@@ -1246,6 +1442,7 @@ static void ss_correction(const Prm *prm, const Parts *T, double *pot)
     if (prm->kernel == KER_COULOMB) param = 2.0 * M_PI * prm->kp[0] * prm->kp[0];
     else if (prm->kernel == KER_REGCOULOMB) param = 2.0 * M_PI * prm->kp[0] * prm->kp[0];
     else if (prm->kernel == KER_YUKAWA) param = 4.0 * M_PI / prm->kp[0] / prm->kp[0];
+    else if (prm->kernel == KER_REGYUKAWA) param = 4.0 * M_PI / prm->kp[0] / prm->kp[0];
     for (int i = 0; i < T->n; i++) pot[i] += param * T->q[i];
 }
 
@@ -1256,6 +1453,18 @@ static void ss_correction(const Prm *prm, const Parts *T, double *pot)
 static int supported(const Prm *p)
 {
     if (p->approximation != APX_LAGRANGE && p->approximation != APX_HERMITE) return 0;
+    if (other_kernel(p->kernel)) {
+        /* the Lagrange combinations the dispatchers compute as written:
+         * ATAN and MQ have PC kernels only; REGULARIZED_YUKAWA also
+         * subtraction in PC; every other combination exits there or
+         * silently drops the far field */
+        if (p->approximation != APX_LAGRANGE) return 0;
+        if (p->compute_type < CT_PC || p->compute_type > CT_CC) return 0;
+        if ((p->kernel == KER_ATAN || p->kernel == KER_MQ) && p->compute_type != CT_PC) return 0;
+        if (p->singularity == SNG_SUB && !(p->kernel == KER_REGYUKAWA && p->compute_type == CT_PC)) return 0;
+        if (p->singularity != SNG_SKIP && p->singularity != SNG_SUB) return 0;
+        return p->degree >= 1 && p->degree <= 30;
+    }
     if (p->kernel != KER_COULOMB && p->kernel != KER_YUKAWA && p->kernel != KER_REGCOULOMB) return 0;
     if (p->singularity != SNG_SKIP && p->singularity != SNG_SUB) return 0;
     if (p->compute_type < CT_PC || p->compute_type > CT_CC) return 0;
@@ -1633,7 +1842,7 @@ int orc_direct(int M, int N,
     memset(&prm, 0, sizeof(prm));
     prm.kernel = kernel; prm.singularity = singularity;
     for (int i = 0; i < nkp && i < 4; i++) prm.kp[i] = kp[i];
-    if (kernel < KER_COULOMB || kernel > KER_REGCOULOMB) return ORC_ERR_UNSUPPORTED;
+    if ((kernel < KER_COULOMB || kernel > KER_REGCOULOMB) && !other_kernel(kernel)) return ORC_ERR_UNSUPPORTED;
     if (singularity != SNG_SKIP && singularity != SNG_SUB) return ORC_ERR_UNSUPPORTED;
     if (singularity == SNG_SUB && (!sw || !tq)) return ORC_ERR_ARGS;
     Parts S, T;

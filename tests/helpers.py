@@ -52,3 +52,16 @@ def hermite_combos():
     out = [(1, k, s) for k in (1, 2, 3) for s in (1, 2)]
     out += [(2, k, 1) for k in (1, 2, 3)]
     return out
+
+
+# the kernels beside the Coulomb family: kernel id -> parameters
+KP_OTHER = {4: [0.5, 0.1], 5: [2.0, 0.3], 8: [1.5], 9: [2.0, 0.3], 10: [0.1], 11: [0.1], 12: [0.5]}
+
+
+def other_combos():
+    """(compute_type, kernel, singularity) for the kernels beside the Coulomb
+    family, where the reference dispatchers compute them as written
+    (src/interaction_compute/interaction_compute_{pc,cp,cc}.c): Lagrange;
+    ATAN and MQ in PC only; subtraction for regularized Yukawa in PC only."""
+    out = [(ct, k, 1) for k in (4, 8, 10, 11, 12) for ct in (1, 2, 3)]
+    return out + [(1, 5, 1), (1, 9, 1), (1, 4, 2)]

@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from tests.helpers import KP, hermite_combos, uniform_cube
+from tests.helpers import KP, KP_OTHER, hermite_combos, other_combos, uniform_cube
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "BaryTreeInterface.h")
@@ -80,6 +80,27 @@ def test_hermite_host_work_matches_oracle(bt, oracle_mod, ct, k, s):
     assert {x: a[x] for x in KEYS} == {x: o[x] for x in KEYS}
 
 
+@pytest.mark.parametrize("ct,k,s", other_combos())
+def test_other_kernels_host_work_matches_oracle(bt, oracle_mod, ct, k, s):
+    t, src = uniform_cube(3000, 2500, seed=21 + ct + k)
+    a = bt.analyze(t, src, k, KP_OTHER[k], s, 1, ct, 0.7, 4, 60, 50)
+    oracle_mod.treecode(*t, *src, k, KP_OTHER[k], s, 1, ct, 0.7, 4, 60, 50)
+    o = oracle_mod.last_stats()
+    assert {x: a[x] for x in KEYS} == {x: o[x] for x in KEYS}
+
+
+def test_other_kernels_rejected_combinations(bt):
+    t, s = uniform_cube(200, 200, seed=3)
+    with pytest.raises(RuntimeError, match="PARTICLE-CLUSTER ONLY"):
+        bt.analyze(t, s, 5, [2.0, 0.3], 1, 1, 2, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="SUBTRACTION WITH THIS KERNEL"):
+        bt.analyze(t, s, 4, [0.5, 0.1], 2, 1, 2, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="HERMITE WITH THIS KERNEL"):
+        bt.analyze(t, s, 8, [1.5], 1, 2, 1, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="more kernel parameters"):
+        bt.analyze(t, s, 9, [2.0], 1, 1, 1, 0.7, 3, 10, 10)
+
+
 def test_beta_resolution_matches_oracle(bt, oracle_mod):
     t, s = uniform_cube(3000, 3000, seed=4)
     for ct in (1, 2, 3):
@@ -103,7 +124,7 @@ def test_unsupported_combinations_are_errors(bt):
     with pytest.raises(RuntimeError, match="REGULARIZED"):
         bt.analyze(t, s, 3, [0.3, 0.05], 2, 1, 2, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="KERNEL"):
-        bt.analyze(t, s, 9, [0.5], 1, 1, 1, 0.7, 3, 10, 10)
+        bt.analyze(t, s, 6, [0.5], 1, 1, 1, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="degree"):
         bt.analyze(t, s, 1, [0.5], 1, 1, 1, 0.7, 13, 10, 10)
     with pytest.raises(RuntimeError, match="positive"):
