@@ -24,7 +24,7 @@ namespace {
 
 constexpr int HTPB = 256;
 constexpr int HCHUNK = 64;  // particles per LDS chunk of the upward pass
-constexpr int HKC = 1, HKY = 2, HKR = 3;
+constexpr int HKC = 1, HKY = 2, HKR = 3, HKS = KERNEL_SIN_OVER_R;
 
 __device__ __forceinline__ double herm_dj(int k, int n) { return (k == 0 || k == n) ? 0.25 : 1.0; }
 
@@ -154,7 +154,21 @@ template <int KER, bool SS>
 __device__ __forceinline__ double herm_pc_pair(double dx, double dy, double dz, const double *q, const double *w,
                                                double tq, const KParams &kp)
 {
-    if constexpr (KER == HKY) {
+    if constexpr (KER == HKS) {
+        // derivative expansion of sin(k r)/r; coincident points are skipped
+        // (sin-over-r_pc.c:68-155)
+        const double k = kp.p0, k2 = k * k, k3 = k * k2;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(r > DBL_MIN)) return 0.0;
+        const double rinv = 1.0 / r;
+        const double r2inv = rinv * rinv, r3inv = rinv * r2inv, r4inv = r2inv * r2inv;
+        const double r5inv = r3inv * r2inv, r6inv = r3inv * r3inv, r7inv = r4inv * r3inv;
+        double sn, cs;
+        sincos(k * r, &sn, &cs);
+        return sn * rinv * q[0] + (sn * r3inv - k * cs * r2inv) * (q[1] * dx + q[2] * dy + q[3] * dz) +
+               (sn * (3 * r5inv - k2 * r3inv) - 3 * k * cs * r4inv) * (q[4] * dx * dy + q[5] * dy * dz + q[6] * dx * dz) +
+               (sn * (15 * r7inv - 6 * k2 * r5inv) + cs * (k3 * r4inv - 15 * k * r6inv)) * q[7] * dx * dy * dz;
+    } else if constexpr (KER == HKY) {
         const double k = kp.p0, k2 = k * k, k3 = k * k2;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         const double r2 = r * r, r3 = r2 * r;
@@ -240,7 +254,8 @@ __global__ __launch_bounds__(HTPB) void k_herm_pc(HermPCArgs a)
 // Cluster-particle far field: tile {key, first grid point, count, node} of
 // up to HTPB grid points (one per thread) of a target node against the
 // source ranges of its key; the 8 plane sums go to the key's output row
-// (coulomb_cp.c:66-184, yukawa_cp.c:68-194, regularized-coulomb_cp.c:68-188).
+// (coulomb_cp.c:66-184, yukawa_cp.c:68-194, regularized-coulomb_cp.c:68-188,
+// sin-over-r_cp.c:68-198).
 // ---------------------------------------------------------------------
 template <int KER>
 __global__ __launch_bounds__(HTPB) void k_herm_cp(HermCPArgs a)
@@ -270,7 +285,23 @@ __global__ __launch_bounds__(HTPB) void k_herm_cp(HermCPArgs a)
             __syncthreads();
             for (int jj = 0; jj < n; jj++) {
                 const double dx = Lx[jj] - cx, dy = Ly[jj] - cy, dz = Lz[jj] - cz;
-                if constexpr (KER == HKY) {
+                if constexpr (KER == HKS) {
+                    const double k = a.kp.p0, k2 = k * k, k3 = k * k2;
+                    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    const double rinv = 1.0 / r;
+                    const double r2inv = rinv * rinv, r3inv = rinv * r2inv, r4inv = r2inv * r2inv;
+                    const double r5inv = r3inv * r2inv, r6inv = r3inv * r3inv, r7inv = r4inv * r3inv;
+                    double sn, cs;
+                    sincos(k * r, &sn, &cs);
+                    sn *= Lq[jj]; cs *= Lq[jj];
+                    const double d1 = sn * r3inv - k * cs * r2inv;
+                    const double d2 = sn * (3 * r5inv - k2 * r3inv) - 3 * k * cs * r4inv;
+                    const double d3 = sn * (15 * r7inv - 6 * k2 * r5inv) + cs * (k3 * r4inv - 15 * k * r6inv);
+                    t[0] += sn * rinv;
+                    t[1] += d1 * dx; t[2] += d1 * dy; t[3] += d1 * dz;
+                    t[4] += d2 * dx * dy; t[5] += d2 * dy * dz; t[6] += d2 * dx * dz;
+                    t[7] += d3 * dx * dy * dz;
+                } else if constexpr (KER == HKY) {
                     const double k = a.kp.p0, k2 = k * k, k3 = k * k2;
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     const double r = sqrt(r2), r3 = r2 * r;
@@ -427,6 +458,7 @@ hipError_t launch_herm_pc(const HermPCArgs &a, hipStream_t st)
         case HKC: if (ss) L(HKC, true); else L(HKC, false); break;
         case HKY: if (ss) L(HKY, true); else L(HKY, false); break;
         case HKR: if (ss) L(HKR, true); else L(HKR, false); break;
+        case HKS: if (ss) return hipErrorInvalidValue; L(HKS, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef L
@@ -441,6 +473,7 @@ hipError_t launch_herm_cp(const HermCPArgs &a, hipStream_t st)
         case HKC: k_herm_cp<HKC><<<g, b, 0, st>>>(a); break;
         case HKY: k_herm_cp<HKY><<<g, b, 0, st>>>(a); break;
         case HKR: k_herm_cp<HKR><<<g, b, 0, st>>>(a); break;
+        case HKS: k_herm_cp<HKS><<<g, b, 0, st>>>(a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -69,7 +69,10 @@ std::string params_check(const Params &p)
     if (other) {
         // the combinations the dispatchers compute as written
         // (interaction_compute_pc.c:259-435, _cp.c:255-384, _cc.c:275-403)
-        if (p.approximation != A_LAGRANGE) return "NOT SET UP FOR HERMITE WITH THIS KERNEL";
+        // Hermite exists for sin-over-r in PC and CP only
+        // (interaction_compute_pc.c:338-362, interaction_compute_cp.c:304-328)
+        if (p.approximation == A_HERMITE && !(p.kernel == K_SIN_OVER_R && p.compute_type != C_CC))
+            return "NOT SET UP FOR HERMITE WITH THIS KERNEL";
         if ((p.kernel == K_ATAN || p.kernel == K_MQ) && p.compute_type != C_PC)
             return "ATAN AND MQ ARE SET UP FOR PARTICLE-CLUSTER ONLY";
         if (p.singularity == S_SUBTRACTION && !(p.kernel == K_REG_YUKAWA && p.compute_type == C_PC))

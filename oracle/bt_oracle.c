@@ -1284,7 +1284,23 @@ static void k_pc_hermite(const Prm *prm, int nt, int P, int ts, size_t cs, const
         for (int j = 0; j < P; j++) {
             size_t jj = cs + j;
             double dx = tx - C->x[jj], dy = ty - C->y[jj], dz = tz - C->z[jj];
-            if (prm->kernel == KER_YUKAWA) {
+            if (prm->kernel == KER_SINOVERR) {
+                /* sin-over-r_pc.c:68-155 */
+                double k = prm->kp[0], k2 = k * k, k3 = k * k2;
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r > DBL_MIN) {
+                    double rinv = 1 / r;
+                    double r2inv = rinv * rinv, r3inv = rinv * r2inv, r4inv = r2inv * r2inv;
+                    double r5inv = r3inv * r2inv, r6inv = r3inv * r3inv, r7inv = r4inv * r3inv;
+                    double sinr = sin(k * r), cosr = cos(k * r);
+                    temporary_potential += sinr * rinv * (q0[j])
+                        + (sinr * r3inv - k * cosr * r2inv) * (q1[j] * dx + q2[j] * dy + q3[j] * dz)
+                        + (sinr * (3 * r5inv - k2 * r3inv) - 3 * k * cosr * r4inv)
+                              * (q4[j] * dx * dy + q5[j] * dy * dz + q6[j] * dx * dz)
+                        + (sinr * (15 * r7inv - 6 * k2 * r5inv) + cosr * (k3 * r4inv - 15 * k * r6inv))
+                              * q7[j] * dx * dy * dz;
+                }
+            } else if (prm->kernel == KER_YUKAWA) {
                 double k = prm->kp[0], k2 = k * k, k3 = k * k2;
                 double r = sqrt(dx * dx + dy * dy + dz * dz);
                 double r2 = r * r, r3 = r2 * r;
@@ -1355,7 +1371,23 @@ static void k_cp_hermite(const Prm *prm, int ns, int P, int ss, size_t cs, const
         for (int j = 0; j < ns; j++) {
             int jj = ss + j;
             double dx = S->x[jj] - cx, dy = S->y[jj] - cy, dz = S->z[jj] - cz;
-            if (prm->kernel == KER_YUKAWA) {
+            if (prm->kernel == KER_SINOVERR) {
+                /* sin-over-r_cp.c:68-198 */
+                double k = prm->kp[0], k2 = k * k, k3 = k * k2;
+                double r = sqrt(dx * dx + dy * dy + dz * dz);
+                double rinv = 1 / r;
+                double r2inv = rinv * rinv, r3inv = rinv * r2inv, r4inv = r2inv * r2inv;
+                double r5inv = r3inv * r2inv, r6inv = r3inv * r3inv, r7inv = r4inv * r3inv;
+                double sinr = sin(k * r) * S->q[jj], cosr = cos(k * r) * S->q[jj];
+                double d0 = sinr * rinv;
+                double d1 = sinr * r3inv - k * cosr * r2inv;
+                double d2 = sinr * (3 * r5inv - k2 * r3inv) - 3 * k * cosr * r4inv;
+                double d3 = sinr * (15 * r7inv - 6 * k2 * r5inv) + cosr * (k3 * r4inv - 15 * k * r6inv);
+                t0 += d0;
+                t1 += d1 * dx; t2 += d1 * dy; t3 += d1 * dz;
+                t4 += d2 * dx * dy; t5 += d2 * dy * dz; t6 += d2 * dx * dz;
+                t7 += d3 * dx * dy * dz;
+            } else if (prm->kernel == KER_YUKAWA) {
                 double k = prm->kp[0], k2 = k * k, k3 = k * k2;
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double r = sqrt(r2), r3 = r2 * r;
@@ -1458,8 +1490,13 @@ static int supported(const Prm *p)
          * ATAN and MQ have PC kernels only; REGULARIZED_YUKAWA also
          * subtraction in PC; every other combination exits there or
          * silently drops the far field */
-        if (p->approximation != APX_LAGRANGE) return 0;
         if (p->compute_type < CT_PC || p->compute_type > CT_CC) return 0;
+        /* Hermite: sin-over-r in PC and CP with skipping only
+         * (interaction_compute_pc.c:351-358, interaction_compute_cp.c:317-324) */
+        if (p->approximation == APX_HERMITE)
+            return p->kernel == KER_SINOVERR && p->compute_type != CT_CC && p->singularity == SNG_SKIP &&
+                   p->degree >= 1 && p->degree <= 30;
+        if (p->approximation != APX_LAGRANGE) return 0;
         if ((p->kernel == KER_ATAN || p->kernel == KER_MQ) && p->compute_type != CT_PC) return 0;
         if (p->singularity == SNG_SUB && !(p->kernel == KER_REGYUKAWA && p->compute_type == CT_PC)) return 0;
         if (p->singularity != SNG_SKIP && p->singularity != SNG_SUB) return 0;

@@ -5,6 +5,7 @@ KP = {  # kernel -> singularity -> parameters
     1: {1: [0.5], 2: [0.5]},
     2: {1: [0.5], 2: [0.5]},
     3: {1: [0.05], 2: [0.3, 0.05]},
+    8: {1: [1.5]},  # sin-over-r (Hermite cases only; skipping)
 }
 
 
@@ -46,12 +47,13 @@ def rel_l2(ref, val):
 
 def hermite_combos():
     """(compute_type, kernel, singularity) the reference accepts for Hermite:
-    PC with either singularity, CP with skipping, no CC
+    PC with either singularity, CP with skipping, no CC, and sin-over-r in
+    PC and CP with skipping
     (src/interaction_compute/interaction_compute_{pc,cp,cc}.c,
     interaction_compute_downpass.c:126-128)."""
     out = [(1, k, s) for k in (1, 2, 3) for s in (1, 2)]
     out += [(2, k, 1) for k in (1, 2, 3)]
-    return out
+    return out + [(1, 8, 1), (2, 8, 1)]
 
 
 # the kernels beside the Coulomb family: kernel id -> parameters

@@ -96,7 +96,9 @@ def test_other_kernels_rejected_combinations(bt):
     with pytest.raises(RuntimeError, match="SUBTRACTION WITH THIS KERNEL"):
         bt.analyze(t, s, 4, [0.5, 0.1], 2, 1, 2, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="HERMITE WITH THIS KERNEL"):
-        bt.analyze(t, s, 8, [1.5], 1, 2, 1, 0.7, 3, 10, 10)
+        bt.analyze(t, s, 4, [0.5, 0.1], 1, 2, 1, 0.7, 3, 10, 10)
+    with pytest.raises(RuntimeError, match="HERMITE WITH THIS KERNEL"):
+        bt.analyze(t, s, 8, [1.5], 1, 2, 3, 0.7, 3, 10, 10)
     with pytest.raises(RuntimeError, match="more kernel parameters"):
         bt.analyze(t, s, 9, [2.0], 1, 1, 1, 0.7, 3, 10, 10)
 

@@ -84,6 +84,6 @@ def test_other_kernels_closed_forms(oracle_mod):
 
 def test_other_kernels_unsupported(oracle_mod):
     t, src = uniform_cube(300, 300, seed=1)
-    for ct, k, s, apx in [(2, 5, 1, 1), (3, 9, 1, 1), (2, 4, 2, 1), (1, 8, 2, 1), (1, 8, 1, 2), (1, 4, 1, 2)]:
+    for ct, k, s, apx in [(2, 5, 1, 1), (3, 9, 1, 1), (2, 4, 2, 1), (1, 8, 2, 1), (3, 8, 1, 2), (1, 4, 1, 2)]:
         with pytest.raises(RuntimeError):
             oracle_mod.treecode(*t, *src, k, KP_OTHER[k], s, apx, ct, 0.7, 3, 50, 50)
