@@ -24,7 +24,8 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function",
               "-fopenmp"]
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-             # packed FP32 instructions stay off
+             # packed FP32 instructions stay off (some of their forms have given
+             # wrong results on gfx950 without any error)
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              # matrix-core results straight into VGPRs (no AGPR copies per pair)
              "-mllvm", "-amdgpu-mfma-vgpr-form",

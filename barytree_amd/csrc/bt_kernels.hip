@@ -610,12 +610,15 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
 
     // staged sources: LB[d*TPB + j] = centred coordinate d (d = 3: |s-c|^2),
     // LA[j] = FP64 charge term, LC[j] = (a, b) in FP32.  Entries past n are
-    // far dummies with zero charge, so 16-wide chunks need no tail handling.
+    // dummies at the centre whose norm entry is 1e30 and whose charges are
+    // zero: every pair with them has r^2 near 1e30 (finite in FP32), adds
+    // exactly zero and is never below the threshold, so 16-wide chunks need
+    // no tail handling.
     auto put = [&](int n, double x, double y, double z, double a, double bw) {
         const bool real = tid < n;
-        const double dx = real ? x - ctr.x : 1.0e3, dy = real ? y - ctr.y : 0.0, dz = real ? z - ctr.z : 0.0;
+        const double dx = real ? x - ctr.x : 0.0, dy = real ? y - ctr.y : 0.0, dz = real ? z - ctr.z : 0.0;
         LB[tid] = dx; LB[TPB + tid] = dy; LB[2 * TPB + tid] = dz;
-        LB[3 * TPB + tid] = fma(dz, dz, fma(dy, dy, dx * dx));
+        LB[3 * TPB + tid] = real ? fma(dz, dz, fma(dy, dy, dx * dx)) : 1.0e30;
         a = real ? a : 0.0;
         bw = real ? bw : 0.0;
         LA[tid] = a;

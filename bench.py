@@ -22,6 +22,13 @@ configuration: regularized Coulomb (epsilon 0.01), N = 4M Plummer sphere
 2000: fixed degree 10 against the per-pair degrees of the shipped model
 (trained on other Plummer realizations), in FP64 and mixed.
 
+--workload yukawa: Yukawa (kappa 0.5), N = 1M uniform, cluster-cluster,
+theta 0.8, degree 8, leaf 4000, plain FP64 against mixed precision.
+
+--workload cpu-case: Coulomb, N = 100k uniform, particle-cluster, theta 0.7,
+degree 8, leaf 2000, skipping, FP64, against the reference's own sources
+compiled for one CPU core.
+
 --workload batched measures the batching extension instead: 256 independent
 Coulomb systems (N log-uniform in [1k, 64k], 8-Gaussian mixtures, sigma 0.05,
 sources = targets), particle-cluster, theta 0.7, degree 6: the autotuned
@@ -316,6 +323,127 @@ def bench_learned(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_yukawa(args):
+    """Yukawa kappa = 0.5, N = 1M uniform in [-1,1]^3 (seed 2), cluster-cluster,
+    theta 0.8, degree 8, max leaf 4000: the plain FP64 form against the mixed
+    form (FP32 pair kernel, FP64 charges and accumulation), each checked
+    against the FP64 direct sum at 1000 stride-sampled targets."""
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+
+    n = args.n if args.n != 16_000_000 else 1_000_000
+    x, y, z, q, w = gen_uniform(n, 2)
+    targets = (x.copy(), y.copy(), z.copy(), q.copy())
+    kargs = (bt.Kernel.YUKAWA, [0.5], bt.Singularity.SKIPPING, bt.Approximation.LAGRANGE,
+             bt.ComputeType.CLUSTER_CLUSTER, 0.8, 8, 4000, 4000)
+    t0 = time.perf_counter()
+    plan = bt.Plan(targets, (x, y, z, q, w), *kargs, precision=bt.PRECISION_MIXED)
+    host_setup = time.perf_counter() - t0
+    idx = np.arange(0, n, n // 1000)[:1000]
+    direct = plan.direct_sample(idx)
+    res = {}
+    for form, prec in (("fp64", bt.PRECISION_FP64), ("mixed", bt.PRECISION_MIXED)):
+        wall, gpu, near, st = time_steps(plan, prec, args.steps, args.warmup)
+        out = plan.execute()
+        err = rel_l2(direct, out[idx])
+        res[form] = {"s": round(wall, 5), "gpu_s": round(gpu, 5), "near_s": round(near, 5),
+                     "rel_l2_err": err,
+                     "phases_s": {k: round(st[k], 5) for k in ("gpu_upward_s", "gpu_near_s",
+                                                               "gpu_far_s", "gpu_down_s")}}
+        log(f"[bench] yukawa {form}: {wall:.4f} s/step (near {near:.4f}), relL2 {err:.2e}")
+    plan.close()
+    near_evals = st["pp_ptwise"] + st["pc_ptwise"]
+    evals = near_evals + st["cp_ptwise"] + st["cc_ptwise"]
+    achieved = 14 * near_evals / res["mixed"]["near_s"] / 1e12
+    line = {
+        "metric": "time per solve, Yukawa CC N=1M, plain FP64 vs mixed precision",
+        "value": res["mixed"]["s"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mixed"]["s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 pair kernel, f64 charges/accumulation",
+        "data": "synthetic",
+        "config": {"workload": (f"Yukawa kappa=0.5, N={n} uniform [-1,1]^3, cluster-cluster "
+                                "theta=0.8 degree 8 max leaf 4000"),
+                   "n": n, "compute_type": "CC", "theta": 0.8, "degree": 8, "max_leaf": 4000},
+        "forms": res,
+        "speedup_mixed_vs_fp64": round(res["fp64"]["s"] / res["mixed"]["s"], 3),
+        "interactions": evals,
+        "roofline": {"bound": "valu", "kernel": "k_near_mfma<Yukawa> (near field, r^2 on the FP64 matrix cores)",
+                     "achieved": round(achieved, 2), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None},
+        "host_setup_s": round(host_setup, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_cpu_case(args):
+    """Coulomb, N = 100k uniform in [-1,1]^3 (seed 1), particle-cluster,
+    theta 0.7, degree 8, max leaf 2000, skipping, FP64: the GPU plain form
+    against the reference's own sources compiled for the CPU (oracle/_ref,
+    one core; the oracle's restatement when that library is absent), on the
+    same particles, with their difference and the error against the FP64
+    direct sum at 1000 stride-sampled targets."""
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from oracle import oracle as O
+
+    n = args.n if args.n != 16_000_000 else 100_000
+    x, y, z, q, w = gen_uniform(n, 1)
+    targets = (x.copy(), y.copy(), z.copy(), q.copy())
+    kargs = (bt.Kernel.COULOMB, [0.0], bt.Singularity.SKIPPING, bt.Approximation.LAGRANGE,
+             bt.ComputeType.PARTICLE_CLUSTER, 0.7, 8, 2000, 2000)
+    bt.solve(targets, (x, y, z, q, w), *kargs)  # warm-up (module load, first allocations)
+    t0 = time.perf_counter()
+    out_gpu = bt.solve(targets, (x, y, z, q, w), *kargs)
+    e2e = time.perf_counter() - t0
+    plan = bt.Plan(targets, (x, y, z, q, w), *kargs)
+    wall, gpu, near, st = time_steps(plan, bt.PRECISION_FP64, args.steps, args.warmup)
+    idx = np.arange(0, n, n // 1000)[:1000]
+    direct = plan.direct_sample(idx)
+    plan.close()
+    kind = "reference" if O.ref_available() else "port"
+    fn = O.ref_treecode if kind == "reference" else O.treecode
+    t0 = time.perf_counter()
+    out_cpu = fn(*targets, x, y, z, q, w, 1, [0.0], 1, 1, 1, 0.7, 8, 2000, 2000)
+    cpu_s = time.perf_counter() - t0
+    diff = rel_l2(out_cpu, out_gpu)
+    log(f"[bench] C1 gpu {wall:.4f} s/step (end to end {e2e:.3f} s), cpu {kind} {cpu_s:.2f} s, "
+        f"gpu vs cpu relL2 {diff:.2e}, vs direct {rel_l2(direct, out_gpu[idx]):.2e}")
+    line = {
+        "metric": "time per solve, Coulomb PC N=100k FP64, GPU vs the reference on one CPU core",
+        "value": round(wall, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"Coulomb, N={n} uniform [-1,1]^3, particle-cluster theta=0.7 "
+                                "degree 8 max leaf 2000, skipping"),
+                   "n": n, "compute_type": "PC", "theta": 0.7, "degree": 8, "max_leaf": 2000},
+        "end_to_end_s": round(e2e, 4),
+        "rel_l2_err": rel_l2(direct, out_gpu[idx]),
+        "rel_l2_gpu_vs_cpu": diff,
+        "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s", "cores": 1, "kind": kind,
+                         "sample": f"the whole treecode on the same {n} particles, one call"},
+        "speedup_vs_cpu": round(cpu_s / wall, 1),
+        "speedup_end_to_end_vs_cpu": round(cpu_s / e2e, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +452,8 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["large", "batched", "learned"], default="large")
+    ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case"],
+                    default="large")
     ap.add_argument("--delta", type=float, default=0.5,
                     help="degree model threshold (probability that a lower degree is not enough)")
     ap.add_argument("--systems", type=int, default=256)
@@ -346,8 +475,8 @@ def main():
         print(json.dumps({"calibration": "device copy", "bytes_read": 2 ** 31, "bytes_written": 2 ** 31}))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md, 'How the path would shard').", file=sys.stderr)
+        print("bench.py: this library runs on one GPU per process; multi-GPU runs "
+              "(a locally essential tree exchange between ranks) are not part of it.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batched":
@@ -355,6 +484,12 @@ def main():
         return
     if args.workload == "learned":
         bench_learned(args)
+        return
+    if args.workload == "yukawa":
+        bench_yukawa(args)
+        return
+    if args.workload == "cpu-case":
+        bench_cpu_case(args)
         return
 
     import torch  # device setup and synchronisation only
