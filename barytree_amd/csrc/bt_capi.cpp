@@ -61,6 +61,7 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->variable_degree = p.variable_degree() ? 1 : 0;
     s->mean_pair_degree = p.mean_pair_degree();
     s->far_ptwise_variable = p.variable_far_evals();
+    s->sym_ptwise = p.sym_pairs();
 }
 
 std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)

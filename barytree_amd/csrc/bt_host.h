@@ -122,7 +122,7 @@ void hermite_weight_table(const Tree &t, int degree, std::vector<double> &tab);
 
 // Source-range flags: a source may coincide with a target (the singularity
 // skip may trigger); a pair may come closer than its key's threshold.
-constexpr int SEG_COINCIDE = 1, SEG_CLOSE = 2;
+constexpr int SEG_COINCIDE = 1, SEG_CLOSE = 2, SEG_SYM = 4;
 
 // Device-side work description (CSR), built on the host.
 struct WorkCSR {
@@ -131,7 +131,7 @@ struct WorkCSR {
     std::vector<int> tbeg, tcnt;   // target range (particles) or grid size
     std::vector<int> seg_off;      // CSR into seg_beg/seg_cnt (source particle ranges)
     std::vector<int> seg_beg, seg_cnt;
-    std::vector<char> seg_chk;     // SEG_COINCIDE | SEG_CLOSE flags
+    std::vector<char> seg_chk;     // SEG_COINCIDE | SEG_CLOSE | SEG_SYM flags
     std::vector<int> clu_off;      // CSR into clu_id (source clusters)
     std::vector<int> clu_id;
     long long pairs = 0;           // point-wise evaluations described

@@ -14,10 +14,13 @@ struct alignas(16) Tile { int x, y, z, w; };
 struct alignas(8) Seg { int x, y; };
 
 // Seg.y = count | flags: the singularity skip may trigger in the range;
-// a pair of the range may come closer than its key's threshold (key_ctr).
+// a pair of the range may come closer than its key's threshold (key_ctr);
+// the range is a leaf whose pairs with the key's leaf the symmetric near
+// field evaluates (launches with skip_sym pass over it).
 constexpr unsigned SEG_COINCIDE_BIT = 0x80000000u;
 constexpr unsigned SEG_CLOSE_BIT = 0x40000000u;
-constexpr unsigned SEG_COUNT_MASK = 0x3fffffffu;
+constexpr unsigned SEG_SYM_BIT = 0x20000000u;
+constexpr unsigned SEG_COUNT_MASK = 0x1fffffffu;
 
 // Tile lists are 8 interleaved queues (tile 8*j + x = j-th tile of queue x),
 // one per XCD of an MI355X.
@@ -82,9 +85,46 @@ struct InteractArgs {
     // ranges flagged SEG_CLOSE, and in the key's clusters if .w < 0) are
     // recomputed from the coordinate differences
     const double4 *key_ctr;
+    int skip_sym;            // pass over source ranges flagged SEG_SYM_BIT
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
+
+// Symmetric near field (mixed precision, Coulomb and Yukawa with
+// subtraction, a system whose targets are its sources).  An item is a pair
+// of distinct leaves A, B, each in the other's near list: every pair (a, b)
+// is evaluated once and gives both a's term from b and b's term from a.
+// a's sums go to slots[rslot + i] (i-th particle of A), b's to
+// slots[cslot + j]; ctr[k] = centre of the pair's boxes and its r^2
+// threshold (negative: pairs below it are recomputed from coordinate
+// differences, with the skip of coincident pairs).
+struct SymItem {
+    int a_beg, a_cnt, b_beg, b_cnt;
+    long long rslot, cslot;
+};
+struct SymArgs {
+    int kernel, nitems;
+    const SymItem *items;
+    const double4 *ctr;
+    const double *x, *y, *z, *q, *w;  // the particles, tree order
+    KParams kp;
+    double *slots;
+};
+hipError_t launch_near_sym(const SymArgs &a, hipStream_t st);
+
+// Adds the slots of every particle to its potential, in list order: tile
+// {list, first particle, count}; particle i of list l adds
+// slots[base[k] + i] for k in [off[l], off[l+1]).
+constexpr int TPB_SYM_REDUCE = 256;  // particles per reduction tile (one per thread)
+struct SymReduceArgs {
+    int ntiles;
+    const Tile *tiles;
+    const int *off;
+    const long long *base;
+    const double *slots;
+    double *pot;
+};
+hipError_t launch_sym_reduce(const SymReduceArgs &a, hipStream_t st);
 
 // Hermite approximation (bt_hermite.hip), FP64.  Clusters carry 8 planes
 // per grid point; the planes of node c sit at [8*P*c + m*P + j].

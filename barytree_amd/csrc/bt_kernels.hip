@@ -468,6 +468,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
+        if (A.skip_sym && ((unsigned)sg.y & SEG_SYM_BIT) != 0) continue;  // the symmetric launch's
         const bool check = SCALED ? can_close && ((unsigned)sg.y & SEG_CLOSE_BIT) != 0
                                   : !GRID && ((unsigned)sg.y & SEG_COINCIDE_BIT) != 0;
         const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
@@ -544,6 +545,185 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
         if (t < 0) break;
         interact_tile<KER, SS, GRID, MIXED, TPT>(A, A.tiles[t], Ls, Lb, Lf);
     }
+}
+
+// ---------------------------------------------------------------------
+// Symmetric near field.  One block per item (leaves A, B).  Coordinates are
+// taken relative to the item's centre c and scaled by lam as in pair_ss32
+// (charges by lam), r'^2 = |T'|^2 + |S'|^2 - 2 T'.S' in FP64 as in the
+// scaled path of interact_tile.  A pass holds SYM_TPT targets of A per
+// lane: lane l of wave w owns targets w*16*TPT + (l%16) + 16k, k < TPT
+// (the four lanes l%16 + 16m hold the same targets), and a staged chunk of
+// up to TPB sources of B is split over the lanes by l/16: at step s the
+// lane takes source 4s + l/16 against its TPT targets.  The targets' sums
+// (a terms and b terms apart, FP32, flushed to FP64 every 32 steps) stay
+// in registers over the whole of B; a source's sums over a lane's targets
+// are added over the 16 lanes of its DPP row (row_shr scan: lane 15 ends
+// with the row's sum), over the 4 waves in LDS, then combined in FP64 as
+// p1 - q p2 into the source's column slot (stored at the first pass over
+// A, added after: the block owns its slots, so the order is fixed).  In
+// items flagged close, a wave whose chunk held a pair below the threshold
+// evaluates the chunk again from coordinate differences (coincident pairs
+// add exactly zero, the reference's skip).
+// ---------------------------------------------------------------------
+constexpr int SYM_TPT = 4;
+
+__device__ __forceinline__ float row_scan16(float v)
+{
+    // inclusive prefix sum within each DPP row of 16 lanes (lanes without a
+    // source lane take 0)
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
+    return v;
+}
+
+template <int KER, int TPT>
+__global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
+{
+    __shared__ double4 Ls[TPB];     // S' and |S'|^2
+    __shared__ float2 Lf[TPB];      // lam q w, lam w of the source
+    __shared__ double Lq[TPB];      // the source's charge as a target
+    __shared__ float2 Lc[4][TPB];   // per wave: the sources' sums (a, b terms)
+    const SymItem it = A.items[blockIdx.x];
+    const double4 ctr = A.ctr[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane & 15, cg = lane >> 4;
+    const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : sqrt(1.4426950408889634 * A.kp.inv_a2);
+    const double th = lam * lam * fabs(ctr.w);
+    const bool check = ctr.w < 0.0 && th > 0.0;
+    const float thf = fmaxf((float)th, 1.17549435e-38f);
+    constexpr int TPASS = 4 * 16 * TPT;
+
+    for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
+        double Tx[TPT], Ty[TPT], Tz[TPT], T2[TPT], Tq[TPT], acc[TPT];
+        float ai[TPT], bi[TPT], p1[TPT], p2[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            const int li = p0 + wv * 16 * TPT + rg + 16 * k;
+            const bool v = li < it.a_cnt;
+            const int g = it.a_beg + (v ? li : 0);
+            const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
+            T2[k] = fma(z, z, fma(y, y, x * x));
+            Tx[k] = -2.0 * x; Ty[k] = -2.0 * y; Tz[k] = -2.0 * z;
+            const double q = A.q[g], w = A.w[g];
+            Tq[k] = q;
+            // a target outside A gives nothing to the sources
+            ai[k] = v ? (float)(lam * q * w) : 0.0f;
+            bi[k] = v ? (float)(lam * w) : 0.0f;
+            acc[k] = 0.0;
+            p1[k] = 0.0f; p2[k] = 0.0f;
+        }
+        for (int off = 0; off < it.b_cnt; off += TPB) {
+            const int n = min(TPB, it.b_cnt - off);
+            __syncthreads();  // the previous chunk's LDS has been read
+            if (tid < n) {
+                const int g = it.b_beg + off + tid;
+                const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
+                const double q = A.q[g], w = A.w[g];
+                Ls[tid] = make_double4(x, y, z, fma(z, z, fma(y, y, x * x)));
+                Lf[tid] = make_float2((float)(lam * q * w), (float)(lam * w));
+                Lq[tid] = q;
+            } else {
+                // dummies: zero charge, |S'|^2 = 1e30 (every term exactly zero)
+                Ls[tid] = make_double4(0.0, 0.0, 0.0, 1.0e30);
+                Lf[tid] = make_float2(0.0f, 0.0f);
+                Lq[tid] = 0.0;
+            }
+            __syncthreads();
+            const int steps = (n + 3) >> 2;
+            double acc0[TPT];
+#pragma unroll
+            for (int k = 0; k < TPT; k++) acc0[k] = acc[k];
+            float rmin = 3.0e38f;
+            auto run = [&](auto exact_tag) {
+                constexpr bool EXACT = decltype(exact_tag)::value;
+                for (int sb = 0; sb < steps; sb += 32) {
+                const int se = min(steps, sb + 32);
+                for (int st = sb; st < se; st++) {
+                    const int j = 4 * st + cg;
+                    const double4 S = Ls[j];
+                    const float2 f = Lf[j];
+                    float c1 = 0.0f, c2 = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) {
+                        float r2;
+                        if constexpr (!EXACT) {
+                            r2 = (float)fma(Tx[k], S.x, fma(Ty[k], S.y, fma(Tz[k], S.z, T2[k] + S.w)));
+                            rmin = fminf(rmin, r2);
+                        } else {
+                            const double dx = fma(-0.5, Tx[k], -S.x), dy = fma(-0.5, Ty[k], -S.y),
+                                         dz = fma(-0.5, Tz[k], -S.z);
+                            r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
+                        }
+                        float ir = rsq32(r2), e;
+                        if constexpr (KER == KC) {
+                            e = exp2_32(-r2) * ir;
+                        } else {
+                            e = exp2_32(-(r2 * ir)) * ir;
+                        }
+                        if constexpr (EXACT) {
+                            ir = r2 > 0.0f ? ir : 0.0f;
+                            e = r2 > 0.0f ? e : 0.0f;
+                        }
+                        if constexpr (KER == KC) {
+                            p1[k] = fmaf(f.x, ir, p1[k]);
+                            c1 = fmaf(ai[k], ir, c1);
+                        } else {
+                            p1[k] = fmaf(f.x, e, p1[k]);
+                            c1 = fmaf(ai[k], e, c1);
+                        }
+                        p2[k] = fmaf(f.y, e, p2[k]);
+                        c2 = fmaf(bi[k], e, c2);
+                    }
+                    c1 = row_scan16(c1);
+                    c2 = row_scan16(c2);
+                    if (rg == 15) Lc[wv][j] = make_float2(c1, c2);
+                }
+                // the FP32 sums of (at most) 32 steps into FP64
+#pragma unroll
+                for (int k = 0; k < TPT; k++) {
+                    acc[k] += (double)p1[k] - Tq[k] * (double)p2[k];
+                    p1[k] = 0.0f; p2[k] = 0.0f;
+                }
+                }
+            };
+            run(std::integral_constant<bool, false>{});
+            if (check && __any(rmin < thf)) {
+#pragma unroll
+                for (int k = 0; k < TPT; k++) { acc[k] = acc0[k]; p1[k] = 0.0f; p2[k] = 0.0f; }
+                run(std::integral_constant<bool, true>{});
+            }
+            __syncthreads();
+            if (tid < n) {
+                double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int w = 0; w < 4; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
+                double *slot = A.slots + it.cslot + off + tid;
+                const double v = s1 - Lq[tid] * s2;
+                *slot = p0 == 0 ? v : *slot + v;
+            }
+        }
+        // a target's sum over its four lanes
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            double r = acc[k];
+            r += __shfl_xor(r, 16);
+            r += __shfl_xor(r, 32);
+            const int li = p0 + wv * 16 * TPT + rg + 16 * k;
+            if (cg == 0 && li < it.a_cnt) A.slots[it.rslot + li] = r;
+        }
+    }
+}
+
+__global__ __launch_bounds__(TPB_SYM_REDUCE) void k_sym_reduce(SymReduceArgs A)
+{
+    const Tile t = A.tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.z) return;
+    const int i = t.y + threadIdx.x;
+    double s = A.pot[i];
+    for (int k = A.off[t.x]; k < A.off[t.x + 1]; k++) s += A.slots[A.base[k] + i];
+    A.pot[i] = s;
 }
 
 // ---------------------------------------------------------------------
@@ -709,6 +889,7 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
     const int s0 = A.seg_off[key], s1 = A.seg_off[key + 1];
     for (int s = s0; s < s1; ++s) {
         const Seg sg = A.segs[s];
+        if (A.skip_sym && ((unsigned)sg.y & SEG_SYM_BIT) != 0) continue;  // the symmetric launch's
         // coordinate differences below the threshold (and the skip of
         // coincident pairs, which are closer still: SEG_CLOSE covers them)
         const bool check = can_close && ((unsigned)sg.y & SEG_CLOSE_BIT) != 0;
@@ -1291,6 +1472,24 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
         case KUS: return ss ? hipErrorInvalidValue : plain_dispatch<KUS, false>(a, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
+{
+    if (a.nitems <= 0) return hipSuccess;
+    switch (a.kernel) {
+        case KC: k_near_sym<KC, SYM_TPT><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
+        case KY: k_near_sym<KY, SYM_TPT><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_reduce(const SymReduceArgs &a, hipStream_t st)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    k_sym_reduce<<<dim3(a.ntiles), dim3(TPB_SYM_REDUCE), 0, st>>>(a);
+    return hipGetLastError();
 }
 
 hipError_t launch_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 namespace bt {
@@ -150,6 +151,88 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
     return "";
 }
 
+// Symmetric near field of one system whose targets are its sources (same
+// particles in the same tree order, same tree), CC: a near pair of distinct
+// leaves (t, s) whose reverse (s, t) is also in the lists becomes one item
+// (the larger-work pairs first), and both source ranges are flagged SEG_SYM
+// in sym_of (parallel to segs_of).  The item's centre c is the centre of the
+// two leaves' joint box and R the largest distance from c to either box:
+// r^2 is formed as in the near keys (|t-c|^2 + |s-c|^2 - 2 (t-c).(s-c),
+// rounding error below 6e-15 (|t-c|^2 + |s-c|^2)), so pairs below 2e-7 R^2
+// are recomputed from differences when the boxes come within that distance.
+void Plan::build_sym(const System &sy, const std::vector<std::vector<int>> &segs_of,
+                     std::vector<std::vector<char>> &sym_of)
+{
+    const Tree &st = sy.stree, &tt = sy.ttree;
+    sym_items_.clear(); sym_ctr_.clear(); sym_red_tiles_.clear();
+    sym_red_off_.clear(); sym_red_base_.clear();
+    sym_slots_len_ = 0; sym_pairs_ = 0;
+    const long long nn = tt.n;
+    std::vector<long long> keys;
+    for (int t : tt.leaves)
+        for (int s : segs_of[t]) keys.push_back((long long)t * nn + s);
+    std::sort(keys.begin(), keys.end());
+    auto has = [&](int t, int s) { return std::binary_search(keys.begin(), keys.end(), (long long)t * nn + s); };
+    std::vector<std::pair<int, int>> cand;
+    for (int t : tt.leaves) {
+        sym_of[t].assign(segs_of[t].size(), 0);
+        for (size_t i = 0; i < segs_of[t].size(); i++) {
+            const int s = segs_of[t][i];
+            if (s == t || st.nchild[s] != 0 || !has(s, t)) continue;
+            sym_of[t][i] = 1;
+            if (t < s) cand.push_back({t, s});
+        }
+    }
+    std::stable_sort(cand.begin(), cand.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+        return (long long)st.numpar[a.first] * st.numpar[a.second] >
+               (long long)st.numpar[b.first] * st.numpar[b.second];
+    });
+    std::vector<int> lidx(tt.n, -1);
+    for (size_t i = 0; i < tt.leaves.size(); i++) lidx[tt.leaves[i]] = (int)i;
+    std::vector<std::vector<long long>> lists(tt.leaves.size());
+    auto far2 = [&](double cx, double cy, double cz, int n) {
+        const double dx = std::max(std::fabs(st.xmin[n] - cx), std::fabs(st.xmax[n] - cx));
+        const double dy = std::max(std::fabs(st.ymin[n] - cy), std::fabs(st.ymax[n] - cy));
+        const double dz = std::max(std::fabs(st.zmin[n] - cz), std::fabs(st.zmax[n] - cz));
+        return dx * dx + dy * dy + dz * dz;
+    };
+    for (const auto &c : cand) {
+        const int t = c.first, s = c.second;
+        btk::SymItem it;
+        it.a_beg = sy.soff + st.ibeg[t]; it.a_cnt = st.numpar[t];
+        it.b_beg = sy.soff + st.ibeg[s]; it.b_cnt = st.numpar[s];
+        it.rslot = sym_slots_len_; sym_slots_len_ += it.a_cnt;
+        it.cslot = sym_slots_len_; sym_slots_len_ += it.b_cnt;
+        const double cx = 0.5 * (std::min(st.xmin[t], st.xmin[s]) + std::max(st.xmax[t], st.xmax[s]));
+        const double cy = 0.5 * (std::min(st.ymin[t], st.ymin[s]) + std::max(st.ymax[t], st.ymax[s]));
+        const double cz = 0.5 * (std::min(st.zmin[t], st.zmin[s]) + std::max(st.zmax[t], st.zmax[s]));
+        const double R2 = std::max(far2(cx, cy, cz, t), far2(cx, cy, cz, s));
+        const double thresh = std::max(2e-7 * R2, 1e-300);
+        const double g = std::max(1e-150, std::sqrt(thresh));
+        const bool close = st.xmin[t] - st.xmax[s] <= g && st.xmin[s] - st.xmax[t] <= g &&
+                           st.ymin[t] - st.ymax[s] <= g && st.ymin[s] - st.ymax[t] <= g &&
+                           st.zmin[t] - st.zmax[s] <= g && st.zmin[s] - st.zmax[t] <= g;
+        sym_ctr_.push_back(cx); sym_ctr_.push_back(cy); sym_ctr_.push_back(cz);
+        sym_ctr_.push_back(close ? -thresh : thresh);
+        // targets are the sources: target index = source index
+        lists[lidx[t]].push_back(it.rslot - (sy.toff + tt.ibeg[t]));
+        lists[lidx[s]].push_back(it.cslot - (sy.toff + tt.ibeg[s]));
+        sym_pairs_ += 2LL * it.a_cnt * it.b_cnt;
+        sym_items_.push_back(it);
+    }
+    sym_red_off_.assign(1, 0);
+    for (size_t l = 0; l < tt.leaves.size(); l++) {
+        const int leaf = tt.leaves[l];
+        sym_red_base_.insert(sym_red_base_.end(), lists[l].begin(), lists[l].end());
+        sym_red_off_.push_back((int)sym_red_base_.size());
+        if (lists[l].empty()) continue;
+        for (int o = 0; o < tt.numpar[leaf]; o += btk::TPB_SYM_REDUCE)
+            sym_red_tiles_.push_back(btk::Tile{(int)l, sy.toff + tt.ibeg[leaf] + o,
+                                               std::min(btk::TPB_SYM_REDUCE, tt.numpar[leaf] - o), 0});
+    }
+    sym_ = !sym_items_.empty();
+}
+
 // Flattens the lists of every system into two global CSR work descriptions:
 //   near_ : target leaves (particle targets) <- source particle ranges + source clusters
 //   grid_ : target nodes (grid targets)      <- source particle ranges + source clusters
@@ -173,6 +256,7 @@ void Plan::build_work()
     near_ = WorkCSR();
     grid_ = WorkCSR();
     near_ctr_.clear();
+    sym_ = false;
     near_.seg_off.push_back(0); near_.clu_off.push_back(0);
     grid_.seg_off.push_back(0); grid_.clu_off.push_back(0);
 
@@ -216,7 +300,7 @@ void Plan::build_work()
         // one near key: target node t (particles [tbeg, tbeg + tcnt) of the
         // concatenation) <- source clusters clus and source particle nodes segs
         auto near_key = [&](int t, int tbeg, int tcnt, const std::vector<int> &clus,
-                            const std::vector<int> &segs) {
+                            const std::vector<int> &segs, const std::vector<char> *sym = nullptr) {
             const double cx = tt.xmid[t], cy = tt.ymid[t], cz = tt.zmid[t];
             const double rt2 = far2(cx, cy, cz, tt, t);
             double R2 = rt2;
@@ -231,9 +315,12 @@ void Plan::build_work()
             const long long nt = tcnt;
             add_key(near_, tn + t, tbeg, tcnt);
             for (int c : clus) add_clu(near_, sn + c, nt);
-            for (int s : segs)
+            for (size_t i = 0; i < segs.size(); i++) {
+                const int s = segs[i];
                 add_seg(near_, so + st.ibeg[s], st.numpar[s], nt,
-                        (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0));
+                        (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0) |
+                            (sym && (*sym)[i] ? SEG_SYM : 0));
+            }
             close_key(near_);
             near_ctr_.push_back(cx); near_ctr_.push_back(cy); near_ctr_.push_back(cz);
             near_ctr_.push_back(clu_check ? -thresh : thresh);
@@ -259,15 +346,18 @@ void Plan::build_work()
         } else {
             // CC: a target leaf gets the PP and source-approx (PC) lists of
             // itself and of every ancestor; grids get CC and target-approx (CP)
-            std::vector<int> clus, segs;
+            std::vector<std::vector<int>> clus_of(tt.n), segs_of(tt.n);
             for (int leaf : tt.leaves) {
-                clus.clear(); segs.clear();
                 for (int a = leaf; a >= 0; a = tt.parent[a]) {
-                    clus.insert(clus.end(), L.cc_pc[a].begin(), L.cc_pc[a].end());
-                    segs.insert(segs.end(), L.pp[a].begin(), L.pp[a].end());
+                    clus_of[leaf].insert(clus_of[leaf].end(), L.cc_pc[a].begin(), L.cc_pc[a].end());
+                    segs_of[leaf].insert(segs_of[leaf].end(), L.pp[a].begin(), L.pp[a].end());
                 }
-                near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], clus, segs);
             }
+            std::vector<std::vector<char>> sym_of(tt.n);
+            if (sym_ok_) build_sym(sy, segs_of, sym_of);
+            for (int leaf : tt.leaves)
+                near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], clus_of[leaf], segs_of[leaf],
+                         sym_ok_ ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (L.approx[t].empty() && L.cc_cp[t].empty()) continue;
                 add_key(grid_, tn + t, 0, P);
@@ -396,6 +486,29 @@ std::string Plan::validate() const
     std::string e = chk_csr(near_, near_tiles_, false);
     if (!e.empty()) return "near " + e;
     if (near_ctr_.size() != 4 * near_.key.size()) return "near centres";
+    if (sym_) {
+        if (sym_ctr_.size() != 4 * sym_items_.size() || sym_items_.size() > (size_t)INT32_MAX) return "sym items";
+        for (const btk::SymItem &it : sym_items_) {
+            if (it.a_beg < 0 || it.a_cnt <= 0 || (long long)it.a_beg + it.a_cnt > N_ || it.b_beg < 0 ||
+                it.b_cnt <= 0 || (long long)it.b_beg + it.b_cnt > N_ || N_ != M_)
+                return "sym item range";
+            if (it.rslot < 0 || it.rslot + it.a_cnt > sym_slots_len_ || it.cslot < 0 ||
+                it.cslot + it.b_cnt > sym_slots_len_)
+                return "sym item slots";
+        }
+        const int nl = (int)sym_red_off_.size() - 1;
+        if (nl < 0 || sym_red_off_[nl] != (int)sym_red_base_.size()) return "sym lists";
+        for (int l = 0; l < nl; l++)
+            if (sym_red_off_[l] > sym_red_off_[l + 1]) return "sym lists";
+        for (const btk::Tile &t : sym_red_tiles_) {
+            if (t.x < 0 || t.x >= nl || t.y < 0 || t.z <= 0 || t.z > btk::TPB_SYM_REDUCE ||
+                (long long)t.y + t.z > M_)
+                return "sym tile";
+            for (int k = sym_red_off_[t.x]; k < sym_red_off_[t.x + 1]; k++)
+                if (sym_red_base_[k] + t.y < 0 || sym_red_base_[k] + t.y + t.z > sym_slots_len_)
+                    return "sym slot base";
+        }
+    }
     if (!(e = chk_csr(grid_, grid_tiles_, true)).empty()) return "far " + e;
     if (grid_row_.size() != grid_.key.size() || (int)node_rows_off_.size() != ntnodes_ + 1 ||
         node_rows_off_[ntnodes_] != (int)grid_.key.size())
@@ -571,6 +684,21 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
 
     herm_ = prm.approximation == A_HERMITE;
+    // symmetric near field: one CC system of Coulomb or Yukawa with
+    // subtraction whose targets are its sources, particle for particle
+    // (after the tree builds: the same order and the same tree)
+    sym_ok_ = false;
+    if (sys_.size() == 1 && prm.compute_type == C_CC && prm.singularity == S_SUBTRACTION &&
+        prm.approximation == A_LAGRANGE && (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA)) {
+        const System &y = sys_[0];
+        const Tree &a = y.stree, &b = y.ttree;
+        auto same = [&](const HostVec<double> &u, const HostVec<double> &v) {
+            return std::memcmp(u.data(), v.data(), (size_t)y.M * sizeof(double)) == 0;
+        };
+        sym_ok_ = y.M == y.N && a.n == b.n && a.ibeg == b.ibeg && a.numpar == b.numpar &&
+                  a.nchild == b.nchild && a.leaves == b.leaves && same(y.S.x, y.T.x) &&
+                  same(y.S.y, y.T.y) && same(y.S.z, y.T.z) && same(y.S.q, y.T.q);
+    }
     build_work();
     enumerate_pairs();
     if (herm_) build_hermite_work();
@@ -693,7 +821,8 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         for (size_t i = 0; i < segs.size(); i++) {
             segs[i].x = w.seg_beg[i];
             segs[i].y = (int)((unsigned)w.seg_cnt[i] | ((w.seg_chk[i] & SEG_COINCIDE) ? btk::SEG_COINCIDE_BIT : 0u) |
-                              ((w.seg_chk[i] & SEG_CLOSE) ? btk::SEG_CLOSE_BIT : 0u));
+                              ((w.seg_chk[i] & SEG_CLOSE) ? btk::SEG_CLOSE_BIT : 0u) |
+                              ((w.seg_chk[i] & SEG_SYM) ? btk::SEG_SYM_BIT : 0u));
         }
         std::string rr;
         if (!(rr = up(so, w.seg_off)).empty()) return rr;
@@ -704,6 +833,13 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     if (!(r = up_csr(near_, &d_nseg_off_, &d_nsegs_, &d_nclu_off_, &d_nclu_id_)).empty()) return r;
     if (!(r = up(&d_near_tiles_, near_tiles_)).empty()) return r;
     if (!(r = up(&d_near_ctr_, near_ctr_)).empty()) return r;
+    if (sym_) {
+        if (!(r = up(&d_sym_items_, sym_items_)).empty()) return r;
+        if (!(r = up(&d_sym_ctr_, sym_ctr_)).empty()) return r;
+        if (!(r = up(&d_sym_red_tiles_, sym_red_tiles_)).empty()) return r;
+        if (!(r = up(&d_sym_red_off_, sym_red_off_)).empty()) return r;
+        if (!(r = up(&d_sym_red_base_, sym_red_base_)).empty()) return r;
+    }
     if (!grid_tiles_.empty()) {
         if (!(r = up_csr(grid_, &d_gseg_off_, &d_gsegs_, &d_gclu_off_, &d_gclu_id_)).empty()) return r;
         if (!(r = up(&d_grid_tiles_, grid_tiles_)).empty()) return r;
@@ -722,6 +858,34 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     if (herm_ && !(r = upload_hermite()).empty()) return r;
     HIPCHK(hipStreamSynchronize(st_));
     tm_.h2d = now() - t4;
+    return "";
+}
+
+// The symmetric near field's items into their slots, then the slots into
+// the potential (after the near launch has stored every target's sum).
+std::string Plan::near_sym()
+{
+    if (!d_sym_slots_) {
+        d_sym_slots_ = dalloc<double>((size_t)sym_slots_len_);
+        if (!d_sym_slots_) return "hipMalloc failed (symmetric near-field slots)";
+    }
+    btk::SymArgs a{};
+    a.kernel = prm_.kernel;
+    a.nitems = (int)sym_items_.size();
+    a.items = d_sym_items_;
+    a.ctr = reinterpret_cast<const double4 *>(d_sym_ctr_);
+    a.x = d_sx_; a.y = d_sy_; a.z = d_sz_; a.q = d_sq_; a.w = d_sw_;
+    a.kp = kp_;
+    a.slots = d_sym_slots_;
+    HIPCHK(btk::launch_near_sym(a, st_));
+    btk::SymReduceArgs r{};
+    r.ntiles = (int)sym_red_tiles_.size();
+    r.tiles = d_sym_red_tiles_;
+    r.off = d_sym_red_off_;
+    r.base = d_sym_red_base_;
+    r.slots = d_sym_slots_;
+    r.pot = d_pot_;
+    HIPCHK(btk::launch_sym_reduce(r, st_));
     return "";
 }
 
@@ -778,10 +942,15 @@ std::string Plan::execute(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
+            a.skip_sym = sym_ ? 1 : 0;
         }
         std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
+        if (a.skip_sym) {
+            std::string r = near_sym();
+            if (!r.empty()) return r;
+        }
     }
     if (herm_ && p.compute_type == C_PC) {
         std::string r = hermite_pc();

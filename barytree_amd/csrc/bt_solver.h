@@ -142,6 +142,8 @@ public:
     const Params &params() const { return prm_; }
     double *out_dev() { return d_out_; }
     long long near_pairs() const { return near_.pairs; }
+    // pairs (ordered) the symmetric near field covers; 0 without it
+    long long sym_pairs() const { return sym_ ? sym_pairs_ : 0; }
     long long far_pairs() const { return grid_.pairs; }
     int source_nodes() const { return nsnodes_; }
     int target_nodes() const { return ntnodes_; }
@@ -159,6 +161,11 @@ private:
     template <class T, class A> std::string up_at(T *d, const std::vector<T, A> &h);
     void free_all();
     void build_work();
+    // the symmetric near field's items, slots and reduction lists (CC, one
+    // system whose targets are its sources): sym_ok_ set by create
+    void build_sym(const System &sy, const std::vector<std::vector<int>> &segs_of,
+                   std::vector<std::vector<char>> &sym_of);
+    std::string near_sym();
     std::string validate() const;
     void enumerate_pairs();
     void free_var();
@@ -187,6 +194,21 @@ private:
     // mixed near field recomputes a pair from coordinate differences
     // (negated: the key's clusters are checked too)
     std::vector<double> near_ctr_;
+    // symmetric near field: mixed precision, Coulomb or Yukawa with
+    // subtraction, CC, one system whose targets are its sources (same
+    // particles, same tree).  Leaf pairs (A, B), each leaf in the other's
+    // near list, are evaluated once for both directions (items, sorted by
+    // work); their source ranges carry SEG_SYM and the near launch passes
+    // over them.  Item k writes A's sums to slots [rslot, +|A|) and B's to
+    // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
+    // slots[base + i] for the bases [sym_red_off_[l], sym_red_off_[l+1]).
+    bool sym_ok_ = false, sym_ = false;
+    std::vector<btk::SymItem> sym_items_;
+    std::vector<double> sym_ctr_;
+    std::vector<btk::Tile> sym_red_tiles_;
+    std::vector<int> sym_red_off_;
+    std::vector<long long> sym_red_base_;
+    long long sym_slots_len_ = 0, sym_pairs_ = 0;
     std::vector<long long> grid_row_;
     std::vector<int> node_rows_off_;
     long long grid_rows_len_ = 0;
@@ -243,6 +265,11 @@ private:
     double *d_pot_ = nullptr, *d_out_ = nullptr;
     int *d_cursor_ = nullptr;  // tile-queue cursors: near, then far
     double *d_near_ctr_ = nullptr;
+    btk::SymItem *d_sym_items_ = nullptr;
+    double *d_sym_ctr_ = nullptr, *d_sym_slots_ = nullptr;  // slots: at the first mixed execution
+    btk::Tile *d_sym_red_tiles_ = nullptr;
+    int *d_sym_red_off_ = nullptr;
+    long long *d_sym_red_base_ = nullptr;
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;
     double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;

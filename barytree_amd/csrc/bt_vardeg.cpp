@@ -562,10 +562,15 @@ std::string Plan::execute_var(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
+            a.skip_sym = sym_ ? 1 : 0;
         }
         std::copy(vnear_qlen_, vnear_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
+        if (a.skip_sym) {
+            std::string r = near_sym();
+            if (!r.empty()) return r;
+        }
     }
     HIPCHK(hipEventRecord(ev[2], st_));
 

@@ -98,6 +98,10 @@ typedef struct BaryTreeStats {
     int variable_degree;     /* 1: far-field pairs carry their own degree */
     double mean_pair_degree; /* mean degree over far-field pairs (variable degree) */
     long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
+    long long sym_ptwise;    /* near-field pairs (both directions counted) that mixed
+                                executions evaluate once for both particles: a CC
+                                system of Coulomb or Yukawa with subtraction whose
+                                targets are its sources; 0 otherwise */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:

@@ -1,0 +1,126 @@
+"""Symmetric near field: mixed executions of a cluster-cluster system of
+Coulomb or Yukawa with subtraction whose targets are its sources evaluate a
+pair of leaves once for both particles.  Checked against the oracle (the
+reference's FP64 arithmetic) at the mixed-precision bar, against the same
+system solved without the symmetric path (a batched plan of two systems
+never takes it), for run-to-run reproducibility, on pairs planted across
+leaf boundaries far closer than the expansion of r^2 resolves, and with
+per-pair degrees.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+KPS = {1: [0.5], 2: [0.5]}
+
+
+@pytest.fixture(scope="module")
+def bt():
+    import barytree_amd
+    barytree_amd.lib()
+    return barytree_amd
+
+
+def same_set(n, seed, wlo=0.5, whi=1.5):
+    """Targets equal to the sources, particle for particle (separate arrays)."""
+    rng = np.random.default_rng(seed)
+    x, y, z = (rng.uniform(-1, 1, n) for _ in range(3))
+    q = rng.uniform(-1, 1, n)
+    w = rng.uniform(wlo, whi, n)
+    return (x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, w)
+
+
+def tiny_system():
+    rng = np.random.default_rng(99)
+    x, y, z, q = (rng.uniform(-1, 1, 50) for _ in range(4))
+    return (x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, np.ones(50))
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_symmetric_matches_oracle_and_is_reproducible(bt, oracle_mod, k):
+    print(f"case kernel={k}", flush=True)
+    t, s = same_set(20000, 10 + k)
+    args = (k, KPS[k], 2, 1, 3, 0.8, 5, 400, 400)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        st = plan.stats()
+        assert st["sym_ptwise"] > 0.8 * st["pp_ptwise"], st
+        got = plan.execute()
+        assert np.array_equal(got, plan.execute())  # every sum has one fixed order
+        plan.set_precision(bt.PRECISION_FP64)
+        fp64 = plan.execute()
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert rel_l2(ref, fp64) <= 1e-12
+    err = rel_l2(ref, got)
+    print(f"symmetric mixed vs oracle: {err:.3e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_symmetric_against_one_directional_mixed(bt, k):
+    t, s = same_set(200000, 20 + k, 1.0, 1.0)
+    args = (k, KPS[k], 2, 1, 3, 0.8, 8, 2000, 2000)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        assert plan.stats()["sym_ptwise"] > 0
+        sym = plan.execute()
+        plan.set_precision(bt.PRECISION_FP64)
+        fp64 = plan.execute()
+    with bt.Plan.batched([(t, s), tiny_system()], *args[:7], 2000, 2000,
+                         precision=bt.PRECISION_MIXED) as bp:
+        assert bp.stats()["sym_ptwise"] == 0
+        one = bp.execute()[:len(t[0])]
+    d_sym, d_one, d_pair = rel_l2(fp64, sym), rel_l2(fp64, one), rel_l2(one, sym)
+    print(f"vs fp64: symmetric {d_sym:.3e}, one-directional {d_one:.3e}; between {d_pair:.3e}", flush=True)
+    assert d_sym < 1e-6 and d_one < 1e-6
+    assert d_sym < 4 * d_one + 1e-9
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_symmetric_close_pairs_across_leaves(bt, oracle_mod, k):
+    """Pairs 2e-10 and 2e-7 apart straddling the root's x split (x = 0: the
+    box is [-1, 1] in x), so each pair's particles sit in different leaves:
+    the expanded r^2 cannot resolve them, the close recomputation must."""
+    t, s = same_set(12000, 30 + k)
+    x, y, z, q, w = s
+    x[0], x[1] = -1.0, 1.0
+    rng = np.random.default_rng(7)
+    for j in range(100):
+        i = 2 + 2 * j
+        d = 1e-10 if j % 2 == 0 else 1e-7
+        x[i], x[i + 1] = -d, d
+        y[i + 1], z[i + 1] = y[i], z[i]
+    t = (x.copy(), y.copy(), z.copy(), q.copy())
+    args = (k, KPS[k], 2, 1, 3, 0.8, 4, 300, 300)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        assert plan.stats()["sym_ptwise"] > 0
+        got = plan.execute()
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert np.all(np.isfinite(got))
+    # a planted particle's sum is dominated by its partner's term, whose FP32
+    # evaluation keeps a few 1e-7 of the magnitude of its parts
+    for j in range(100):
+        i = 2 + 2 * j
+        r = 2e-10 if j % 2 == 0 else 2e-7
+        for a, b in ((i, i + 1), (i + 1, i)):
+            mag = (abs(q[b] * w[b]) + abs(q[a] * w[b])) / r
+            assert abs(got[a] - ref[a]) <= 2e-6 * mag + 1e-6 * abs(ref[a]), (j, a)
+    rest = np.ones(len(got), bool)
+    rest[2:202] = False
+    assert rel_l2(ref[rest], got[rest]) < 1e-6
+
+
+def test_symmetric_with_pair_degrees(bt):
+    t, s = same_set(100000, 40)
+    args = (1, KPS[1], 2, 1, 3, 0.8, 8, 1000, 1000)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        rng = np.random.default_rng(3)
+        deg = rng.integers(6, 11, plan.num_pairs())
+        plan.set_pair_degrees(deg)
+        mixed = plan.execute()
+        plan.set_precision(bt.PRECISION_FP64)
+        fp64 = plan.execute()
+    d = rel_l2(fp64, mixed)
+    print(f"per-pair degrees, symmetric mixed vs fp64: {d:.3e}", flush=True)
+    assert d < 1e-6

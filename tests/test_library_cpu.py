@@ -231,3 +231,26 @@ def test_variable_degree_work_on_host(bt, ct, k, s):
     assert np.isclose(stv["mean_pair_degree"], deg.mean())
     with pytest.raises(RuntimeError, match="degree outside"):
         bt.analyze_degrees(t, src, *args, degrees=np.full(n, 13))
+
+
+def test_symmetric_near_field_eligibility(bt):
+    """The symmetric near field covers nearly every near pair of a CC system
+    of Coulomb or Yukawa with subtraction whose targets are its sources, and
+    nothing otherwise."""
+    rng = np.random.default_rng(8)
+    n = 60000
+    x, y, z, q = (rng.uniform(-1, 1, n) for _ in range(4))
+    w = np.ones(n)
+    same = (x.copy(), y.copy(), z.copy(), q.copy())
+    src = (x, y, z, q, w)
+    for k in (1, 2):
+        a = bt.analyze(same, src, k, [0.5], 2, 1, 3, 0.8, 6, 800, 800)
+        assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], a
+        assert a["sym_ptwise"] % 2 == 0
+    assert bt.analyze(same, src, 1, [0.5], 1, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # skipping
+    assert bt.analyze(same, src, 1, [0.5], 2, 1, 1, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # PC
+    # different trees
+    assert bt.analyze(same, src, 1, [0.5], 2, 1, 3, 0.8, 6, 800, 100)["sym_ptwise"] == 0
+    other = (x.copy(), y.copy(), z.copy(), q.copy())
+    other[0][17] += 1e-12
+    assert bt.analyze(other, src, 1, [0.5], 2, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # positions
