@@ -551,35 +551,42 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // Symmetric near field.  One block per item (leaves A, B).  Coordinates are
 // taken relative to the item's centre c and scaled by lam as in pair_ss32
 // (charges by lam), r'^2 = |T'|^2 + |S'|^2 - 2 T'.S' in FP64 as in the
-// scaled path of interact_tile.  A pass holds SYM_TPT targets of A per
-// lane: lane l of wave w owns targets w*16*TPT + (l%16) + 16k, k < TPT
-// (the four lanes l%16 + 16m hold the same targets), and a staged chunk of
-// up to TPB sources of B is split over the lanes by l/16: at step s the
-// lane takes source 4s + l/16 against its TPT targets.  The targets' sums
-// (a terms and b terms apart, FP32, flushed to FP64 every 32 steps) stay
-// in registers over the whole of B; a source's sums over a lane's targets
-// are added over the 16 lanes of its DPP row (row_shr scan: lane 15 ends
-// with the row's sum), over the 4 waves in LDS, then combined in FP64 as
+// scaled path of interact_tile.  Lanes form groups of G: a pass holds TPT
+// targets of A per lane, lane l of wave w owning targets
+// w*G*TPT + (l%G) + G*k, k < TPT (the 64/G lanes with the same l%G hold the
+// same targets), and a staged chunk of up to TPB sources of B is split over
+// the groups: at step s lane l takes source (64/G)*s + l/G against its TPT
+// targets.  The targets' sums (a terms and b terms apart, FP32, flushed to
+// FP64 every 32 steps) stay in registers over the whole of B; a source's
+// sums over a lane's targets are added over its group (DPP row_shr scan:
+// the group's last lane ends with the sum), over the 4 waves in LDS, then
+// combined in FP64 as
 // p1 - q p2 into the source's column slot (stored at the first pass over
 // A, added after: the block owns its slots, so the order is fixed).  In
 // items flagged close, a wave whose chunk held a pair below the threshold
 // evaluates the chunk again from coordinate differences (coincident pairs
 // add exactly zero, the reference's skip).
 // ---------------------------------------------------------------------
-constexpr int SYM_TPT = 4;
+constexpr int SYM_TPT = 4, SYM_G = 8;
 
-__device__ __forceinline__ float row_scan16(float v)
+// Sums over groups of G consecutive lanes (G = 2, 4, 8 or 16) by a
+// row_shr scan inside each DPP row of 16 lanes (lanes without a source
+// lane take 0): after the shifts 1, 2, .., G/2 the last lane of every group
+// holds the group's sum.
+template <int G>
+__device__ __forceinline__ float group_sum(float v)
 {
-    // inclusive prefix sum within each DPP row of 16 lanes (lanes without a
-    // source lane take 0)
     v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
+    if constexpr (G > 2)
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
+    if constexpr (G > 4)
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
+    if constexpr (G > 8)
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
     return v;
 }
 
-template <int KER, int TPT>
+template <int KER, int TPT, int G>
 __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
 {
     __shared__ double4 Ls[TPB];     // S' and |S'|^2
@@ -588,19 +595,20 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
     __shared__ float2 Lc[4][TPB];   // per wave: the sources' sums (a, b terms)
     const SymItem it = A.items[blockIdx.x];
     const double4 ctr = A.ctr[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane & 15, cg = lane >> 4;
+    constexpr int NG = 64 / G;  // source groups of a wave
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane % G, cg = lane / G;
     const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : sqrt(1.4426950408889634 * A.kp.inv_a2);
     const double th = lam * lam * fabs(ctr.w);
     const bool check = ctr.w < 0.0 && th > 0.0;
     const float thf = fmaxf((float)th, 1.17549435e-38f);
-    constexpr int TPASS = 4 * 16 * TPT;
+    constexpr int TPASS = 4 * G * TPT;
 
     for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
         double Tx[TPT], Ty[TPT], Tz[TPT], T2[TPT], Tq[TPT], acc[TPT];
         float ai[TPT], bi[TPT], p1[TPT], p2[TPT];
 #pragma unroll
         for (int k = 0; k < TPT; k++) {
-            const int li = p0 + wv * 16 * TPT + rg + 16 * k;
+            const int li = p0 + wv * G * TPT + rg + G * k;
             const bool v = li < it.a_cnt;
             const int g = it.a_beg + (v ? li : 0);
             const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
@@ -631,17 +639,19 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
                 Lq[tid] = 0.0;
             }
             __syncthreads();
-            const int steps = (n + 3) >> 2;
+            const int steps = (n + NG - 1) / NG;
             double acc0[TPT];
 #pragma unroll
             for (int k = 0; k < TPT; k++) acc0[k] = acc[k];
             float rmin = 3.0e38f;
-            auto run = [&](auto exact_tag) {
+            // EXACT: r^2 from coordinate differences; TRACK: the least r^2
+            auto run = [&](auto exact_tag, auto track_tag) {
                 constexpr bool EXACT = decltype(exact_tag)::value;
+                constexpr bool TRACK = decltype(track_tag)::value;
                 for (int sb = 0; sb < steps; sb += 32) {
                 const int se = min(steps, sb + 32);
                 for (int st = sb; st < se; st++) {
-                    const int j = 4 * st + cg;
+                    const int j = NG * st + cg;
                     const double4 S = Ls[j];
                     const float2 f = Lf[j];
                     float c1 = 0.0f, c2 = 0.0f;
@@ -650,7 +660,7 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
                         float r2;
                         if constexpr (!EXACT) {
                             r2 = (float)fma(Tx[k], S.x, fma(Ty[k], S.y, fma(Tz[k], S.z, T2[k] + S.w)));
-                            rmin = fminf(rmin, r2);
+                            if constexpr (TRACK) rmin = fminf(rmin, r2);
                         } else {
                             const double dx = fma(-0.5, Tx[k], -S.x), dy = fma(-0.5, Ty[k], -S.y),
                                          dz = fma(-0.5, Tz[k], -S.z);
@@ -676,9 +686,9 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
                         p2[k] = fmaf(f.y, e, p2[k]);
                         c2 = fmaf(bi[k], e, c2);
                     }
-                    c1 = row_scan16(c1);
-                    c2 = row_scan16(c2);
-                    if (rg == 15) Lc[wv][j] = make_float2(c1, c2);
+                    c1 = group_sum<G>(c1);
+                    c2 = group_sum<G>(c2);
+                    if (rg == G - 1) Lc[wv][j] = make_float2(c1, c2);
                 }
                 // the FP32 sums of (at most) 32 steps into FP64
 #pragma unroll
@@ -688,11 +698,17 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
                 }
                 }
             };
-            run(std::integral_constant<bool, false>{});
-            if (check && __any(rmin < thf)) {
+            using F = std::integral_constant<bool, false>;
+            using T = std::integral_constant<bool, true>;
+            if (!check) {
+                run(F{}, F{});
+            } else {
+                run(F{}, T{});
+                if (__any(rmin < thf)) {
 #pragma unroll
-                for (int k = 0; k < TPT; k++) { acc[k] = acc0[k]; p1[k] = 0.0f; p2[k] = 0.0f; }
-                run(std::integral_constant<bool, true>{});
+                    for (int k = 0; k < TPT; k++) { acc[k] = acc0[k]; p1[k] = 0.0f; p2[k] = 0.0f; }
+                    run(T{}, F{});
+                }
             }
             __syncthreads();
             if (tid < n) {
@@ -708,9 +724,9 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
 #pragma unroll
         for (int k = 0; k < TPT; k++) {
             double r = acc[k];
-            r += __shfl_xor(r, 16);
-            r += __shfl_xor(r, 32);
-            const int li = p0 + wv * 16 * TPT + rg + 16 * k;
+#pragma unroll
+            for (int o = G; o < 64; o *= 2) r += __shfl_xor(r, o);
+            const int li = p0 + wv * G * TPT + rg + G * k;
             if (cg == 0 && li < it.a_cnt) A.slots[it.rslot + li] = r;
         }
     }
@@ -1478,8 +1494,8 @@ hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
     switch (a.kernel) {
-        case KC: k_near_sym<KC, SYM_TPT><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
-        case KY: k_near_sym<KY, SYM_TPT><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
+        case KC: k_near_sym<KC, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
+        case KY: k_near_sym<KY, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

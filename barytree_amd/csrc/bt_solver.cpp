@@ -122,6 +122,8 @@ Plan::~Plan() { free_all(); }
 void Plan::free_all()
 {
     free_var();
+    for (hipEvent_t &e : sym_ev_)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
     if (st_) {
@@ -868,6 +870,8 @@ std::string Plan::near_sym()
     if (!d_sym_slots_) {
         d_sym_slots_ = dalloc<double>((size_t)sym_slots_len_);
         if (!d_sym_slots_) return "hipMalloc failed (symmetric near-field slots)";
+        HIPCHK(hipEventCreate(&sym_ev_[0]));
+        HIPCHK(hipEventCreate(&sym_ev_[1]));
     }
     btk::SymArgs a{};
     a.kernel = prm_.kernel;
@@ -877,7 +881,10 @@ std::string Plan::near_sym()
     a.x = d_sx_; a.y = d_sy_; a.z = d_sz_; a.q = d_sq_; a.w = d_sw_;
     a.kp = kp_;
     a.slots = d_sym_slots_;
+    HIPCHK(hipEventRecord(sym_ev_[0], st_));
     HIPCHK(btk::launch_near_sym(a, st_));
+    HIPCHK(hipEventRecord(sym_ev_[1], st_));
+    sym_ran_ = true;
     btk::SymReduceArgs r{};
     r.ntiles = (int)sym_red_tiles_.size();
     r.tiles = d_sym_red_tiles_;
@@ -897,6 +904,7 @@ std::string Plan::execute(bool timed)
     const int nl = p.degree + 1;
     hipEvent_t ev[6];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    sym_ran_ = false;
     HIPCHK(hipEventRecord(ev[0], st_));
 
     // 1. upward pass over the source trees (clusters.c:50-200)
@@ -1008,6 +1016,9 @@ std::string Plan::execute(bool timed)
         tm_.gpu_upward = ms[0] * 1e-3; tm_.gpu_near = ms[1] * 1e-3; tm_.gpu_far = ms[2] * 1e-3;
         tm_.gpu_down = ms[3] * 1e-3; tm_.gpu_finish = ms[4] * 1e-3;
         tm_.gpu_total = (ms[0] + ms[1] + ms[2] + ms[3] + ms[4]) * 1e-3;
+        float msym = 0.0f;
+        if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
+        tm_.gpu_near_sym = msym * 1e-3;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return "";

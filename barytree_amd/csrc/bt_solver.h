@@ -27,6 +27,7 @@ constexpr long long MAX_KEY_SOURCES = 1 << 18;  // sources per grid key (heavier
 struct Timings {
     double host_copy = 0, host_tree = 0, host_lists = 0, host_work = 0, h2d = 0;  // seconds
     double gpu_upward = 0, gpu_near = 0, gpu_far = 0, gpu_down = 0, gpu_finish = 0, gpu_total = 0;
+    double gpu_near_sym = 0;  // the symmetric near-field launch alone (part of gpu_near)
 };
 
 // One independent system of a plan.
@@ -270,6 +271,8 @@ private:
     btk::Tile *d_sym_red_tiles_ = nullptr;
     int *d_sym_red_off_ = nullptr;
     long long *d_sym_red_base_ = nullptr;
+    hipEvent_t sym_ev_[2] = {nullptr, nullptr};  // around the symmetric launch
+    bool sym_ran_ = false;                       // it ran in this execution
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;
     double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;

@@ -520,6 +520,7 @@ std::string Plan::execute_var(bool timed)
     const bool ss = p.singularity == S_SUBTRACTION;
     hipEvent_t ev[6];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    sym_ran_ = false;
     HIPCHK(hipEventRecord(ev[0], st_));
 
     // 1. source slots: modified charges of degree p from the node's particles
@@ -617,6 +618,9 @@ std::string Plan::execute_var(bool timed)
         tm_.gpu_upward = ms[0] * 1e-3; tm_.gpu_near = ms[1] * 1e-3; tm_.gpu_far = ms[2] * 1e-3;
         tm_.gpu_down = ms[3] * 1e-3; tm_.gpu_finish = ms[4] * 1e-3;
         tm_.gpu_total = (ms[0] + ms[1] + ms[2] + ms[3] + ms[4]) * 1e-3;
+        float msym = 0.0f;
+        if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
+        tm_.gpu_near_sym = msym * 1e-3;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return "";

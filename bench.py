@@ -72,24 +72,29 @@ def time_steps(plan, precision, steps, warmup):
     for _ in range(warmup):
         plan.execute(download=False)
     torch.cuda.synchronize()
-    times, near = [], []
+    times, near, sym = [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.execute(download=False)  # synchronizes on its own stream
         st = plan.stats()
         times.append(st["gpu_total_s"])
         near.append(st["gpu_near_s"])
+        sym.append(st["gpu_near_sym_s"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    st["gpu_near_sym_mean_s"] = float(np.mean(sym))
     return wall / steps, float(np.mean(times)), float(np.mean(near)), st
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_k_near_sym_coulomb_ss_cc_n16m.md")
+
+
 def near_traffic():
-    """HBM bytes per launch of the mixed near-field kernel at the headline
-    size, from the committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE,
-    each scaled by the same counter on a copy of known size); None if the
-    summary is absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_near_kernel_coulomb_ss_cc_n16m.md")
+    """HBM bytes per launch of the symmetric near-field kernel at the
+    headline size, from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter on a copy of known size);
+    None if the summary is absent."""
+    path = PMC_SUMMARY
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -100,10 +105,10 @@ def near_traffic():
 
 
 def near_valu_mix():
-    """Per-kind VALU wave-instruction counts of one launch of the mixed
+    """Per-kind VALU wave-instruction counts of one launch of the symmetric
     near-field kernel at the headline size, from the committed rocprofv3
     summary (SQ_INSTS_VALU_* counters); None if the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_near_kernel_coulomb_ss_cc_n16m.md")
+    path = PMC_SUMMARY
     try:
         for line in open(path):
             if line.startswith("valu_mix_per_launch:"):
@@ -541,16 +546,23 @@ def main():
     plan.close()
 
     evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
-    near_evals = st["pp_ptwise"] + st["pc_ptwise"]
-    achieved = FLOPS_PER_EVAL * near_evals / near_fast / 1e12
+    # dominant kernel: the symmetric near field (each leaf pair once for both
+    # particles); its work counted as the reference's, 17 flops per ordered
+    # pair evaluation it delivers
+    sym_evals = st["sym_ptwise"]
+    sym_s = st["gpu_near_sym_mean_s"]
+    achieved = FLOPS_PER_EVAL * sym_evals / sym_s / 1e12 if sym_s > 0 else 0.0
     roofline = {
         "bound": "valu",
-        "kernel": "k_interact<Coulomb,SS,particle targets,mixed> (near field + PC)",
+        "kernel": "k_near_sym<Coulomb> (symmetric near field, mixed precision)",
         "achieved": round(achieved, 2),
         "peak": PEAK["fp32"],
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK["fp32"], 4),
         "traffic": near_traffic() if n == 16_000_000 else None,
+        "kernel_s": round(sym_s, 4),
+        "ordered_pairs": sym_evals,
+        "flops_per_ordered_pair": FLOPS_PER_EVAL,
     }
     # the VALU issue time of the kernel's instruction mix (kinds priced alone)
     mix = near_valu_mix() if n == 16_000_000 else None
@@ -559,8 +571,8 @@ def main():
         simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
         ceil_s = issue_model(mix, probe, simds)
         roofline["valu_issue_model_s"] = round(ceil_s, 4)
-        roofline["valu_issue_model_over_measured"] = round(ceil_s / near_fast, 4)
-        log(f"[bench] near kernel {near_fast:.3f} s, VALU issue time of its instruction mix "
+        roofline["valu_issue_model_over_measured"] = round(ceil_s / sym_s, 4)
+        log(f"[bench] symmetric near kernel {sym_s:.3f} s, VALU issue time of its instruction mix "
             f"(kinds priced alone) {ceil_s:.3f} s; probe ns/instr {probe}")
     cpu = None
     if not args.no_cpu:
@@ -598,7 +610,9 @@ def main():
         "plain_fp64_s": round(wall_plain, 4) if wall_plain else None,
         "plain_fp64_rel_l2_err": err_plain,
         "gpu_time_s": round(gpu_fast, 4),
-        "near_kernel_s": round(near_fast, 4),
+        "near_phase_s": round(near_fast, 4),
+        "near_sym_kernel_s": round(sym_s, 4),
+        "sym_pairs": sym_evals,
         "host_setup_s": round(host_setup, 3),
         "end_to_end_s": round(host_setup + deg_setup + wall_ml, 3),
         "interactions": evals,

@@ -102,6 +102,7 @@ typedef struct BaryTreeStats {
                                 executions evaluate once for both particles: a CC
                                 system of Coulomb or Yukawa with subtraction whose
                                 targets are its sources; 0 otherwise */
+    double gpu_near_sym_s;   /* the symmetric near-field launch alone (within gpu_near_s) */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
