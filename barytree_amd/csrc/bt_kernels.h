@@ -90,8 +90,9 @@ struct InteractArgs {
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
-// Symmetric near field (mixed precision, Coulomb and Yukawa with
-// subtraction, a system whose targets are its sources).  An item is a pair
+// Symmetric near field (mixed precision; Coulomb and Yukawa with either
+// singularity, regularized Coulomb with skipping; a cluster-cluster system
+// whose targets are its sources).  An item is a pair
 // of distinct leaves A, B, each in the other's near list: every pair (a, b)
 // is evaluated once and gives both a's term from b and b's term from a.
 // a's sums go to slots[rslot + i] (i-th particle of A), b's to
@@ -103,10 +104,10 @@ struct SymItem {
     long long rslot, cslot;
 };
 struct SymArgs {
-    int kernel, nitems;
+    int kernel, singularity, nitems;
     const SymItem *items;
     const double4 *ctr;
-    const double *x, *y, *z, *q, *w;  // the particles, tree order
+    const double *x, *y, *z, *q, *w;  // the particles, tree order (w: subtraction only)
     KParams kp;
     double *slots;
 };

@@ -586,22 +586,63 @@ __device__ __forceinline__ float group_sum(float v)
     return v;
 }
 
-template <int KER, int TPT, int G>
+// One pair's terms for both directions, from r'^2 (scaled as in pair_ss32
+// for subtraction; skipping: lam = kappa log2(e) for Yukawa, 1 otherwise):
+// the a-part factor fa and the b-part factor fb (subtraction only).  SKIP:
+// a coincident pair (r^2 = 0) gives zero, the reference's skip (the
+// regularized kernel has none).
+template <int KER, bool SS, bool SKIP>
+__device__ __forceinline__ void sym_pair(float r2, float eps2s, float &fa, float &fb)
+{
+    if constexpr (KER == KR) {
+        fa = rsq32(r2 + eps2s);
+        if constexpr (SS) fb = exp2_32(-r2) * fa;
+        return;
+    }
+    float ir = rsq32(r2);
+    if constexpr (KER == KC) {
+        if constexpr (SS) fb = exp2_32(-r2) * ir;
+        fa = ir;
+    } else {
+        fa = exp2_32(-(r2 * ir)) * ir;
+        fb = fa;
+    }
+    if constexpr (SKIP) {
+        fa = r2 > 0.0f ? fa : 0.0f;
+        if constexpr (SS) fb = r2 > 0.0f ? fb : 0.0f;
+    }
+}
+
+template <int KER, bool SS, int TPT, int G>
 __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
 {
     __shared__ double4 Ls[TPB];     // S' and |S'|^2
-    __shared__ float2 Lf[TPB];      // lam q w, lam w of the source
-    __shared__ double Lq[TPB];      // the source's charge as a target
+    __shared__ float2 Lf[TPB];      // the source's a and b (scaled)
+    __shared__ double Lq[SS ? TPB : 1];  // the source's charge as a target
     __shared__ float2 Lc[4][TPB];   // per wave: the sources' sums (a, b terms)
     const SymItem it = A.items[blockIdx.x];
     const double4 ctr = A.ctr[blockIdx.x];
     constexpr int NG = 64 / G;  // source groups of a wave
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane % G, cg = lane / G;
-    const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : sqrt(1.4426950408889634 * A.kp.inv_a2);
-    const double th = lam * lam * fabs(ctr.w);
+    const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0
+                                 : (SS ? sqrt(1.4426950408889634 * A.kp.inv_a2) : 1.0);
+    const float eps2s = (float)(lam * lam * A.kp.eps2);
+    // the regularized kernel's sensitivity is to r^2 + eps^2
+    const double th = lam * lam * (fabs(ctr.w) - (KER == KR ? A.kp.eps2 : 0.0));
     const bool check = ctr.w < 0.0 && th > 0.0;
     const float thf = fmaxf((float)th, 1.17549435e-38f);
     constexpr int TPASS = 4 * G * TPT;
+    // a and b of a particle as a source: subtraction (lam q w, lam w),
+    // skipping (lam q, 0)
+    auto charges = [&](int g, double &q) {
+        q = A.q[g];
+        if constexpr (SS) {
+            const double w = A.w[g];
+            return make_float2((float)(lam * q * w), (float)(lam * w));
+        } else {
+            return make_float2((float)(lam * q), 0.0f);
+        }
+    };
 
     for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
         double Tx[TPT], Ty[TPT], Tz[TPT], T2[TPT], Tq[TPT], acc[TPT];
@@ -614,11 +655,12 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
             const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
             T2[k] = fma(z, z, fma(y, y, x * x));
             Tx[k] = -2.0 * x; Ty[k] = -2.0 * y; Tz[k] = -2.0 * z;
-            const double q = A.q[g], w = A.w[g];
+            double q;
+            const float2 c = charges(g, q);
             Tq[k] = q;
             // a target outside A gives nothing to the sources
-            ai[k] = v ? (float)(lam * q * w) : 0.0f;
-            bi[k] = v ? (float)(lam * w) : 0.0f;
+            ai[k] = v ? c.x : 0.0f;
+            bi[k] = v ? c.y : 0.0f;
             acc[k] = 0.0;
             p1[k] = 0.0f; p2[k] = 0.0f;
         }
@@ -628,15 +670,15 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
             if (tid < n) {
                 const int g = it.b_beg + off + tid;
                 const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
-                const double q = A.q[g], w = A.w[g];
+                double q;
+                Lf[tid] = charges(g, q);
                 Ls[tid] = make_double4(x, y, z, fma(z, z, fma(y, y, x * x)));
-                Lf[tid] = make_float2((float)(lam * q * w), (float)(lam * w));
-                Lq[tid] = q;
+                if constexpr (SS) Lq[tid] = q;
             } else {
                 // dummies: zero charge, |S'|^2 = 1e30 (every term exactly zero)
                 Ls[tid] = make_double4(0.0, 0.0, 0.0, 1.0e30);
                 Lf[tid] = make_float2(0.0f, 0.0f);
-                Lq[tid] = 0.0;
+                if constexpr (SS) Lq[tid] = 0.0;
             }
             __syncthreads();
             const int steps = (n + NG - 1) / NG;
@@ -666,34 +708,23 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
                                          dz = fma(-0.5, Tz[k], -S.z);
                             r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
                         }
-                        float ir = rsq32(r2), e;
-                        if constexpr (KER == KC) {
-                            e = exp2_32(-r2) * ir;
-                        } else {
-                            e = exp2_32(-(r2 * ir)) * ir;
+                        float fa, fb = 0.0f;
+                        sym_pair<KER, SS, EXACT && KER != KR>(r2, eps2s, fa, fb);
+                        p1[k] = fmaf(f.x, fa, p1[k]);
+                        c1 = fmaf(ai[k], fa, c1);
+                        if constexpr (SS) {
+                            p2[k] = fmaf(f.y, fb, p2[k]);
+                            c2 = fmaf(bi[k], fb, c2);
                         }
-                        if constexpr (EXACT) {
-                            ir = r2 > 0.0f ? ir : 0.0f;
-                            e = r2 > 0.0f ? e : 0.0f;
-                        }
-                        if constexpr (KER == KC) {
-                            p1[k] = fmaf(f.x, ir, p1[k]);
-                            c1 = fmaf(ai[k], ir, c1);
-                        } else {
-                            p1[k] = fmaf(f.x, e, p1[k]);
-                            c1 = fmaf(ai[k], e, c1);
-                        }
-                        p2[k] = fmaf(f.y, e, p2[k]);
-                        c2 = fmaf(bi[k], e, c2);
                     }
                     c1 = group_sum<G>(c1);
-                    c2 = group_sum<G>(c2);
+                    if constexpr (SS) c2 = group_sum<G>(c2);
                     if (rg == G - 1) Lc[wv][j] = make_float2(c1, c2);
                 }
                 // the FP32 sums of (at most) 32 steps into FP64
 #pragma unroll
                 for (int k = 0; k < TPT; k++) {
-                    acc[k] += (double)p1[k] - Tq[k] * (double)p2[k];
+                    acc[k] += SS ? (double)p1[k] - Tq[k] * (double)p2[k] : (double)p1[k];
                     p1[k] = 0.0f; p2[k] = 0.0f;
                 }
                 }
@@ -716,11 +747,11 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
 #pragma unroll
                 for (int w = 0; w < 4; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
                 double *slot = A.slots + it.cslot + off + tid;
-                const double v = s1 - Lq[tid] * s2;
+                const double v = SS ? s1 - Lq[SS ? tid : 0] * s2 : s1;
                 *slot = p0 == 0 ? v : *slot + v;
             }
         }
-        // a target's sum over its four lanes
+        // a target's sum over the lanes that hold it
 #pragma unroll
         for (int k = 0; k < TPT; k++) {
             double r = acc[k];
@@ -1493,9 +1524,13 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
 hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
+    const bool ss = a.singularity == 2;
     switch (a.kernel) {
-        case KC: k_near_sym<KC, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
-        case KY: k_near_sym<KY, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a); break;
+#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a)
+        case KC: if (ss) L(KC, true); else L(KC, false); break;
+        case KY: if (ss) L(KY, true); else L(KY, false); break;
+        case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;
+#undef L
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

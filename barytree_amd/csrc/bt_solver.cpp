@@ -686,12 +686,13 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
 
     herm_ = prm.approximation == A_HERMITE;
-    // symmetric near field: one CC system of Coulomb or Yukawa with
-    // subtraction whose targets are its sources, particle for particle
-    // (after the tree builds: the same order and the same tree)
+    // symmetric near field: one CC system (Coulomb or Yukawa, or regularized
+    // Coulomb with skipping) whose targets are its sources, particle for
+    // particle (after the tree builds: the same order and the same tree)
     sym_ok_ = false;
-    if (sys_.size() == 1 && prm.compute_type == C_CC && prm.singularity == S_SUBTRACTION &&
-        prm.approximation == A_LAGRANGE && (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA)) {
+    if (sys_.size() == 1 && prm.compute_type == C_CC && prm.approximation == A_LAGRANGE &&
+        (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA ||
+         (prm.kernel == K_REG_COULOMB && prm.singularity == S_SKIPPING))) {
         const System &y = sys_[0];
         const Tree &a = y.stree, &b = y.ttree;
         auto same = [&](const HostVec<double> &u, const HostVec<double> &v) {
@@ -875,6 +876,7 @@ std::string Plan::near_sym()
     }
     btk::SymArgs a{};
     a.kernel = prm_.kernel;
+    a.singularity = prm_.singularity;
     a.nitems = (int)sym_items_.size();
     a.items = d_sym_items_;
     a.ctr = reinterpret_cast<const double4 *>(d_sym_ctr_);

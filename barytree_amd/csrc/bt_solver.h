@@ -195,9 +195,9 @@ private:
     // mixed near field recomputes a pair from coordinate differences
     // (negated: the key's clusters are checked too)
     std::vector<double> near_ctr_;
-    // symmetric near field: mixed precision, Coulomb or Yukawa with
-    // subtraction, CC, one system whose targets are its sources (same
-    // particles, same tree).  Leaf pairs (A, B), each leaf in the other's
+    // symmetric near field: mixed precision, Coulomb or Yukawa (either
+    // singularity) or regularized Coulomb (skipping), CC, one system whose
+    // targets are its sources (same particles, same tree).  Leaf pairs (A, B), each leaf in the other's
     // near list, are evaluated once for both directions (items, sorted by
     // work); their source ranges carry SEG_SYM and the near launch passes
     // over them.  Item k writes A's sums to slots [rslot, +|A|) and B's to

@@ -100,8 +100,9 @@ typedef struct BaryTreeStats {
     long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
     long long sym_ptwise;    /* near-field pairs (both directions counted) that mixed
                                 executions evaluate once for both particles: a CC
-                                system of Coulomb or Yukawa with subtraction whose
-                                targets are its sources; 0 otherwise */
+                                system whose targets are its sources (Coulomb,
+                                Yukawa, or regularized Coulomb with skipping);
+                                0 otherwise */
     double gpu_near_sym_s;   /* the symmetric near-field launch alone (within gpu_near_s) */
 } BaryTreeStats;
 

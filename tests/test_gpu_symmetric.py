@@ -1,6 +1,6 @@
-"""Symmetric near field: mixed executions of a cluster-cluster system of
-Coulomb or Yukawa with subtraction whose targets are its sources evaluate a
-pair of leaves once for both particles.  Checked against the oracle (the
+"""Symmetric near field: mixed executions of a cluster-cluster system whose
+targets are its sources (Coulomb, Yukawa, or regularized Coulomb with
+skipping) evaluate a pair of leaves once for both particles.  Checked against the oracle (the
 reference's FP64 arithmetic) at the mixed-precision bar, against the same
 system solved without the symmetric path (a batched plan of two systems
 never takes it), for run-to-run reproducibility, on pairs planted across
@@ -14,7 +14,8 @@ from tests.helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-KPS = {1: [0.5], 2: [0.5]}
+KPS = {1: [0.5], 2: [0.5], 3: [0.05]}
+CASES = [(1, 2), (2, 2), (1, 1), (2, 1), (3, 1)]  # (kernel, singularity)
 
 
 @pytest.fixture(scope="module")
@@ -39,11 +40,11 @@ def tiny_system():
     return (x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, np.ones(50))
 
 
-@pytest.mark.parametrize("k", [1, 2])
-def test_symmetric_matches_oracle_and_is_reproducible(bt, oracle_mod, k):
-    print(f"case kernel={k}", flush=True)
-    t, s = same_set(20000, 10 + k)
-    args = (k, KPS[k], 2, 1, 3, 0.8, 5, 400, 400)
+@pytest.mark.parametrize("k,sg", CASES)
+def test_symmetric_matches_oracle_and_is_reproducible(bt, oracle_mod, k, sg):
+    print(f"case kernel={k} singularity={sg}", flush=True)
+    t, s = same_set(20000, 10 + k + sg)
+    args = (k, KPS[k], sg, 1, 3, 0.8, 5, 400, 400)
     with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
         st = plan.stats()
         assert st["sym_ptwise"] > 0.8 * st["pp_ptwise"], st
@@ -58,10 +59,10 @@ def test_symmetric_matches_oracle_and_is_reproducible(bt, oracle_mod, k):
     assert err < 1e-6
 
 
-@pytest.mark.parametrize("k", [1, 2])
-def test_symmetric_against_one_directional_mixed(bt, k):
-    t, s = same_set(200000, 20 + k, 1.0, 1.0)
-    args = (k, KPS[k], 2, 1, 3, 0.8, 8, 2000, 2000)
+@pytest.mark.parametrize("k,sg", CASES)
+def test_symmetric_against_one_directional_mixed(bt, k, sg):
+    t, s = same_set(200000, 20 + k + sg, 1.0, 1.0)
+    args = (k, KPS[k], sg, 1, 3, 0.8, 8, 2000, 2000)
     with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
         assert plan.stats()["sym_ptwise"] > 0
         sym = plan.execute()
@@ -77,12 +78,12 @@ def test_symmetric_against_one_directional_mixed(bt, k):
     assert d_sym < 4 * d_one + 1e-9
 
 
-@pytest.mark.parametrize("k", [1, 2])
-def test_symmetric_close_pairs_across_leaves(bt, oracle_mod, k):
+@pytest.mark.parametrize("k,sg", CASES)
+def test_symmetric_close_pairs_across_leaves(bt, oracle_mod, k, sg):
     """Pairs 2e-10 and 2e-7 apart straddling the root's x split (x = 0: the
     box is [-1, 1] in x), so each pair's particles sit in different leaves:
     the expanded r^2 cannot resolve them, the close recomputation must."""
-    t, s = same_set(12000, 30 + k)
+    t, s = same_set(12000, 30 + k + sg)
     x, y, z, q, w = s
     x[0], x[1] = -1.0, 1.0
     rng = np.random.default_rng(7)
@@ -92,19 +93,22 @@ def test_symmetric_close_pairs_across_leaves(bt, oracle_mod, k):
         x[i], x[i + 1] = -d, d
         y[i + 1], z[i + 1] = y[i], z[i]
     t = (x.copy(), y.copy(), z.copy(), q.copy())
-    args = (k, KPS[k], 2, 1, 3, 0.8, 4, 300, 300)
+    args = (k, KPS[k], sg, 1, 3, 0.8, 4, 300, 300)
     with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
         assert plan.stats()["sym_ptwise"] > 0
         got = plan.execute()
     ref = oracle_mod.treecode(*t, *s, *args)
     assert np.all(np.isfinite(got))
     # a planted particle's sum is dominated by its partner's term, whose FP32
-    # evaluation keeps a few 1e-7 of the magnitude of its parts
+    # evaluation keeps a few 1e-7 of the magnitude of its parts (regularized:
+    # the pair's term is bounded by 1/eps)
     for j in range(100):
         i = 2 + 2 * j
         r = 2e-10 if j % 2 == 0 else 2e-7
+        if k == 3:
+            r = np.sqrt(r * r + KPS[3][0] ** 2)
         for a, b in ((i, i + 1), (i + 1, i)):
-            mag = (abs(q[b] * w[b]) + abs(q[a] * w[b])) / r
+            mag = (abs(q[b] * w[b]) + (abs(q[a] * w[b]) if sg == 2 else 0.0)) / r
             assert abs(got[a] - ref[a]) <= 2e-6 * mag + 1e-6 * abs(ref[a]), (j, a)
     rest = np.ones(len(got), bool)
     rest[2:202] = False
