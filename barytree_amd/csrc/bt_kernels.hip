@@ -570,19 +570,19 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 constexpr int SYM_TPT = 4, SYM_G = 8;
 
 // Sums over groups of G consecutive lanes (G = 2, 4, 8 or 16) by a
-// row_shr scan inside each DPP row of 16 lanes (lanes without a source
-// lane take 0): after the shifts 1, 2, .., G/2 the last lane of every group
-// holds the group's sum.
+// row_shr scan inside each DPP row of 16 lanes: after the shifts 1, 2, ..,
+// G/2 the last lane of every group holds the group's sum.
 template <int G>
 __device__ __forceinline__ float group_sum(float v)
 {
-    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
+    // bound_ctrl: a lane whose source lane lies outside its row reads 0
+    v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, true));
     if constexpr (G > 2)
-        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, false));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xf, 0xf, true));
     if constexpr (G > 4)
-        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, false));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xf, 0xf, true));
     if constexpr (G > 8)
-        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, false));
+        v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xf, 0xf, true));
     return v;
 }
 

@@ -866,14 +866,27 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
 
 // The symmetric near field's items into their slots, then the slots into
 // the potential (after the near launch has stored every target's sum).
+bool Plan::sym_ready()
+{
+    if (!sym_) return false;
+    if (d_sym_slots_) return true;
+    // the slots at the first mixed execution; without the memory for them
+    // the plan keeps the one-directional near field
+    if (hipEventCreate(&sym_ev_[0]) != hipSuccess || hipEventCreate(&sym_ev_[1]) != hipSuccess) {
+        sym_ = false;
+        return false;
+    }
+    d_sym_slots_ = dalloc<double>((size_t)sym_slots_len_);
+    if (!d_sym_slots_) {
+        (void)hipGetLastError();  // clear the allocation error
+        sym_ = false;
+        return false;
+    }
+    return true;
+}
+
 std::string Plan::near_sym()
 {
-    if (!d_sym_slots_) {
-        d_sym_slots_ = dalloc<double>((size_t)sym_slots_len_);
-        if (!d_sym_slots_) return "hipMalloc failed (symmetric near-field slots)";
-        HIPCHK(hipEventCreate(&sym_ev_[0]));
-        HIPCHK(hipEventCreate(&sym_ev_[1]));
-    }
     btk::SymArgs a{};
     a.kernel = prm_.kernel;
     a.singularity = prm_.singularity;
@@ -952,7 +965,7 @@ std::string Plan::execute(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.skip_sym = sym_ ? 1 : 0;
+            a.skip_sym = sym_ready() ? 1 : 0;
         }
         std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
         a.cursor = d_cursor_;

@@ -166,6 +166,7 @@ private:
     // system whose targets are its sources): sym_ok_ set by create
     void build_sym(const System &sy, const std::vector<std::vector<int>> &segs_of,
                    std::vector<std::vector<char>> &sym_of);
+    bool sym_ready();  // slots allocated (at the first mixed execution)
     std::string near_sym();
     std::string validate() const;
     void enumerate_pairs();

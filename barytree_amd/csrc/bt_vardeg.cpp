@@ -563,7 +563,7 @@ std::string Plan::execute_var(bool timed)
         a.out1 = d_pot_;
         if (p.precision == PREC_MIXED) {
             a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.skip_sym = sym_ ? 1 : 0;
+            a.skip_sym = sym_ready() ? 1 : 0;
         }
         std::copy(vnear_qlen_, vnear_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_;
