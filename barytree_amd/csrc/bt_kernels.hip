@@ -554,12 +554,12 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // scaled path of interact_tile.  Lanes form groups of G: a pass holds TPT
 // targets of A per lane, lane l of wave w owning targets
 // w*G*TPT + (l%G) + G*k, k < TPT (the 64/G lanes with the same l%G hold the
-// same targets), and a staged chunk of up to TPB sources of B is split over
+// same targets), and a staged chunk of up to 64*NW sources of B is split over
 // the groups: at step s lane l takes source (64/G)*s + l/G against its TPT
 // targets.  The targets' sums (a terms and b terms apart, FP32, flushed to
 // FP64 every 32 steps) stay in registers over the whole of B; a source's
 // sums over a lane's targets are added over its group (DPP row_shr scan:
-// the group's last lane ends with the sum), over the 4 waves in LDS, then
+// the group's last lane ends with the sum), over the NW waves in LDS, then
 // combined in FP64 as
 // p1 - q p2 into the source's column slot (stored at the first pass over
 // A, added after: the block owns its slots, so the order is fixed).  In
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // evaluates the chunk again from coordinate differences (coincident pairs
 // add exactly zero, the reference's skip).
 // ---------------------------------------------------------------------
-constexpr int SYM_TPT = 4, SYM_G = 8;
+constexpr int SYM_TPT = 4, SYM_G = 8, SYM_NW = 4;
 
 // Sums over groups of G consecutive lanes (G = 2, 4, 8 or 16) by a
 // row_shr scan inside each DPP row of 16 lanes: after the shifts 1, 2, ..,
@@ -613,13 +613,14 @@ __device__ __forceinline__ void sym_pair(float r2, float eps2s, float &fa, float
     }
 }
 
-template <int KER, bool SS, int TPT, int G>
-__global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
+template <int KER, bool SS, int TPT, int G, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least 4 waves per SIMD
 {
-    __shared__ double4 Ls[TPB];     // S' and |S'|^2
-    __shared__ float2 Lf[TPB];      // the source's a and b (scaled)
-    __shared__ double Lq[SS ? TPB : 1];  // the source's charge as a target
-    __shared__ float2 Lc[4][TPB];   // per wave: the sources' sums (a, b terms)
+    constexpr int NT = 64 * NW;  // threads of the block; sources per staged chunk
+    __shared__ double4 Ls[NT];     // S' and |S'|^2
+    __shared__ float2 Lf[NT];      // the source's a and b (scaled)
+    __shared__ double Lq[SS ? NT : 1];  // the source's charge as a target
+    __shared__ float2 Lc[NW][NT];  // per wave: the sources' sums (a, b terms)
     const SymItem it = A.items[blockIdx.x];
     const double4 ctr = A.ctr[blockIdx.x];
     constexpr int NG = 64 / G;  // source groups of a wave
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
     const double th = lam * lam * (fabs(ctr.w) - (KER == KR ? A.kp.eps2 : 0.0));
     const bool check = ctr.w < 0.0 && th > 0.0;
     const float thf = fmaxf((float)th, 1.17549435e-38f);
-    constexpr int TPASS = 4 * G * TPT;
+    constexpr int TPASS = NW * G * TPT;
     // a and b of a particle as a source: subtraction (lam q w, lam w),
     // skipping (lam q, 0)
     auto charges = [&](int g, double &q) {
@@ -664,8 +665,8 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
             acc[k] = 0.0;
             p1[k] = 0.0f; p2[k] = 0.0f;
         }
-        for (int off = 0; off < it.b_cnt; off += TPB) {
-            const int n = min(TPB, it.b_cnt - off);
+        for (int off = 0; off < it.b_cnt; off += NT) {
+            const int n = min(NT, it.b_cnt - off);
             __syncthreads();  // the previous chunk's LDS has been read
             if (tid < n) {
                 const int g = it.b_beg + off + tid;
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(TPB) void k_near_sym(SymArgs A)
             if (tid < n) {
                 double s1 = 0.0, s2 = 0.0;
 #pragma unroll
-                for (int w = 0; w < 4; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
+                for (int w = 0; w < NW; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
                 double *slot = A.slots + it.cslot + off + tid;
                 const double v = SS ? s1 - Lq[SS ? tid : 0] * s2 : s1;
                 *slot = p0 == 0 ? v : *slot + v;
@@ -1526,7 +1527,7 @@ hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
     if (a.nitems <= 0) return hipSuccess;
     const bool ss = a.singularity == 2;
     switch (a.kernel) {
-#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G><<<dim3(a.nitems), dim3(TPB), 0, st>>>(a)
+#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G, SYM_NW><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
         case KC: if (ss) L(KC, true); else L(KC, false); break;
         case KY: if (ss) L(KY, true); else L(KY, false); break;
         case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;

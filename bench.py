@@ -611,6 +611,8 @@ def main():
         "far_evals_fixed": st["cc_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"],
         "far_evals_learned": st_ml["far_ptwise_variable"],
         "degree_setup_s": round(deg_setup, 3),
+        "phases_s": {k: round(st_ml[k], 4) for k in ("gpu_upward_s", "gpu_near_s", "gpu_near_sym_s",
+                                                      "gpu_far_s", "gpu_down_s", "gpu_finish_s")},
         "plain_fp64_s": round(wall_plain, 4) if wall_plain else None,
         "plain_fp64_rel_l2_err": err_plain,
         "gpu_time_s": round(gpu_fast, 4),
