@@ -86,6 +86,9 @@ struct InteractArgs {
     // recomputed from the coordinate differences
     const double4 *key_ctr;
     int skip_sym;            // pass over source ranges flagged SEG_SYM_BIT
+    // mixed grid targets: r^2 from the expansion around the target node's
+    // centre (valid for acceptance-separated sources, theta < 0.99)
+    int grid_scaled;
 };
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
@@ -227,6 +230,32 @@ struct DownArgs {
     double *pot;
 };
 hipError_t launch_down(const DownArgs &a, hipStream_t st);
+
+// Change of degree of tensor Chebyshev grids (bt_degree.hip), for plans
+// whose far-field pairs carry their own degrees.  E[p] is the (p+1) x (D+1)
+// matrix of the 1D degree-p Lagrange basis at the degree-D Chebyshev
+// points of the reference interval, row major.
+// up: slot s of degree p gets (E x E x E) of the degree-D grid of node[s]
+// (QD, WD: degree-D grids of all nodes; Qp, Wp: the slots' grids).
+struct DegreeUpArgs {
+    int nslots, p, D, subtraction;
+    const int *node;
+    const double *E;
+    const double *QD, *WD;
+    double *Qp, *Wp;
+};
+hipError_t launch_degree_up(const DegreeUpArgs &a, hipStream_t st);
+// down: block b stores into the degree-D grid of node[b] the sum over its
+// items [off[b], off[b+1]) ((p << DEG_SHIFT) | slot, in that order) of
+// (E^T x E^T x E^T) of the slot's degree-p grid (Qs[p], Ws[p]).
+struct DegreeDownArgs {
+    int nnodes, D, subtraction;
+    const int *node, *off, *item;
+    const double *E[MAX_DEGREE + 1];
+    const double *Qs[MAX_DEGREE + 1], *Ws[MAX_DEGREE + 1];
+    double *QD, *WD;
+};
+hipError_t launch_degree_down(const DegreeDownArgs &a, hipStream_t st);
 
 // Subtraction correction and scatter to the caller's order:
 // out[order[i]] = pot[i] + ss_param*tq[i] (ss_param = 0 without subtraction).

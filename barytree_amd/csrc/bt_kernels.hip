@@ -235,6 +235,26 @@ __device__ __forceinline__ void pair_ss32(float r2s, float af, float bf, float e
     }
 }
 
+// Grid targets with coordinates and charges scaled by lam as in pair_ss32
+// (lam = kappa log2(e) for Yukawa; skipping Coulomb and regularized Coulomb:
+// 1): the Q part into p1, the W part (subtraction) into p2.
+template <int KER, bool SS>
+__device__ __forceinline__ void pair_grid_scaled(float r2s, float af, float bf, float eps2s, float &p1, float &p2)
+{
+    if constexpr (KER == KC) {
+        const float ir = rsq32(r2s);
+        p1 = fmaf(af, ir, p1);
+        if constexpr (SS) p2 = fmaf(bf, exp2_32(-r2s) * ir, p2);
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2s);
+        const float e = exp2_32(-(r2s * ir)) * ir;
+        p1 = fmaf(af, e, p1);
+        if constexpr (SS) p2 = fmaf(bf, e, p2);
+    } else {
+        p1 = fmaf(af, rsq32(r2s + eps2s), p1);
+    }
+}
+
 template <int KER, bool SS>
 __device__ __forceinline__ void pair_grid32(float r2, double a, double b, const KP32 &k, double &q,
                                             double &w)
@@ -338,11 +358,68 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
             Tx[u] = -2.0 * x; Ty[u] = -2.0 * y; Tz[u] = -2.0 * z;
         }
     }
+    // Mixed grid targets with grid_scaled: the same centred, scaled expansion
+    // around the target node's centre c (the middle of its Chebyshev range).
+    // Every source of a grid key passed the acceptance criterion,
+    // (R_t + R_s) < theta d, so r >= (1 - theta) d while |t-c|^2 + |s-c|^2 <=
+    // (theta^2 + (1 + theta)^2) d^2: for theta < 0.99 the expansion's relative
+    // error stays below 4e-10 and no pair needs a check.  FP32 sums over
+    // blocks of 32 sources.
+    constexpr bool GS = MIXED && GRID;
+    const bool gscaled = GS && A.grid_scaled;
+    if constexpr (GS) {
+        if (gscaled) {
+            lam = KER == KY ? 1.4426950408889634 * A.kp.p0 : (SS ? sqrt(1.4426950408889634 * A.kp.inv_a2) : 1.0);
+            eps2s = (float)(lam * lam * A.kp.eps2);
+            ctr = make_double4(0.5 * (ttab[0] + ttab[nl - 1]), 0.5 * (ttab[nl] + ttab[2 * nl - 1]),
+                               0.5 * (ttab[2 * nl] + ttab[3 * nl - 1]), 0.0);
+#pragma unroll
+            for (int u = 0; u < TPT; u++) {
+                const double x = lam * (Tx[u] - ctr.x), y = lam * (Ty[u] - ctr.y), z = lam * (Tz[u] - ctr.z);
+                T2[u] = fma(z, z, fma(y, y, x * x));
+                Tx[u] = -2.0 * x; Ty[u] = -2.0 * y; Tz[u] = -2.0 * z;
+            }
+        }
+    }
 
     // n staged sources against the first NU target slots
     auto consume_nu = [&](int n, auto check_tag, auto nu_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
         constexpr int NU = decltype(nu_tag)::value;
+        if constexpr (GS) {
+            if (gscaled) {
+                constexpr int BLK = 32;
+                for (int jb = 0; jb < n; jb += BLK) {
+                    const int je = min(jb + BLK, n);
+                    float p1[NU], p2[NU];
+#pragma unroll
+                    for (int u = 0; u < NU; u++) { p1[u] = 0.0f; p2[u] = 0.0f; }
+                    auto body = [&](int j) {
+                        const double4 s = Ls[j];
+                        const float2 f = Lf[j];
+#pragma unroll
+                        for (int u = 0; u < NU; u++) {
+                            const float r2 =
+                                (float)fma(Tx[u], s.x, fma(Ty[u], s.y, fma(Tz[u], s.z, T2[u] + s.w)));
+                            pair_grid_scaled<KER, SS>(r2, f.x, f.y, eps2s, p1[u], p2[u]);
+                        }
+                    };
+                    if (je - jb == BLK) {
+#pragma unroll 8
+                        for (int j = 0; j < BLK; ++j) body(jb + j);
+                    } else {
+#pragma unroll 2
+                        for (int j = jb; j < je; ++j) body(j);
+                    }
+#pragma unroll
+                    for (int u = 0; u < NU; u++) {
+                        acc[u] += (double)p1[u];
+                        if constexpr (SS) accw[u] += (double)p2[u];
+                    }
+                }
+                return;
+            }
+        }
         if constexpr (SCALED) {
             // CHECK: the range may hold pairs below the threshold (compared
             // in FP32; a threshold below FLT_MIN still catches r'^2 <= 0)
@@ -426,7 +503,7 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     using Check = std::integral_constant<bool, SCALED || KER != KR>;
     // one staged source point (thread tid)
     auto stage = [&](double x, double y, double z, double a, double b) {
-        if constexpr (SCALED) {
+        if (SCALED || gscaled) {
             x = lam * (x - ctr.x); y = lam * (y - ctr.y); z = lam * (z - ctr.z);
             Ls[tid] = make_double4(x, y, z, fma(z, z, fma(y, y, x * x)));
             Lf[tid] = make_float2((float)(lam * a), (float)(lam * b));
@@ -517,7 +594,7 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 {
     __shared__ double4 Ls[TPB];
     __shared__ double Lb[SS ? TPB : 1];
-    __shared__ float2 Lf[(MIXED && SS && !GRID) ? TPB : 1];
+    __shared__ float2 Lf[MIXED && (SS || GRID) ? TPB : 1];
     if (!A.persistent) {
         const Tile tile = A.tiles[blockIdx.x];
         if (tile.z == 0) return;  // padding tile (uniform across the block)

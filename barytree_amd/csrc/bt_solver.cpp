@@ -2,6 +2,7 @@
 #include "bt_solver.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -153,84 +154,101 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
     return "";
 }
 
-// Symmetric near field of one system whose targets are its sources (same
-// particles in the same tree order, same tree), CC: a near pair of distinct
-// leaves (t, s) whose reverse (s, t) is also in the lists becomes one item
-// (the larger-work pairs first), and both source ranges are flagged SEG_SYM
-// in sym_of (parallel to segs_of).  The item's centre c is the centre of the
-// two leaves' joint box and R the largest distance from c to either box:
-// r^2 is formed as in the near keys (|t-c|^2 + |s-c|^2 - 2 (t-c).(s-c),
-// rounding error below 6e-15 (|t-c|^2 + |s-c|^2)), so pairs below 2e-7 R^2
-// are recomputed from differences when the boxes come within that distance.
-void Plan::build_sym(const System &sy, const std::vector<std::vector<int>> &segs_of,
-                     std::vector<std::vector<char>> &sym_of)
+// Symmetric near field of one system whose targets are its sources (the same
+// particles in the same order after the tree builds).  Keys are target
+// nodes (leaves or batches, disjoint particle ranges) with the source nodes
+// of their near lists (segs_of, indexed by target node).  A source node whose
+// particle range is another key's range, while that key lists this key's
+// range, makes a symmetric pair: one item (the larger-work pairs first), and
+// both ranges are flagged SEG_SYM in sym_of (parallel to segs_of).  The
+// item's centre c is the centre of the two boxes' joint box and R the
+// largest distance from c to either box: r^2 is formed as in the near keys
+// (|t-c|^2 + |s-c|^2 - 2 (t-c).(s-c), rounding error below
+// 6e-15 (|t-c|^2 + |s-c|^2)), so pairs below 2e-7 R^2 are recomputed from
+// differences when the boxes come within that distance.
+void Plan::build_sym(const System &sy, const std::vector<int> &keys,
+                     const std::vector<std::vector<int>> &segs_of, std::vector<std::vector<char>> &sym_of)
 {
     const Tree &st = sy.stree, &tt = sy.ttree;
     sym_items_.clear(); sym_ctr_.clear(); sym_red_tiles_.clear();
     sym_red_off_.clear(); sym_red_base_.clear();
     sym_slots_len_ = 0; sym_pairs_ = 0;
-    const long long nn = tt.n;
-    std::vector<long long> keys;
-    for (int t : tt.leaves)
-        for (int s : segs_of[t]) keys.push_back((long long)t * nn + s);
-    std::sort(keys.begin(), keys.end());
-    auto has = [&](int t, int s) { return std::binary_search(keys.begin(), keys.end(), (long long)t * nn + s); };
-    std::vector<std::pair<int, int>> cand;
-    for (int t : tt.leaves) {
+    // key of a particle range [beg, beg + cnt)
+    std::vector<std::array<long long, 2>> key_range;  // (beg << 32 | cnt, key index)
+    for (size_t k = 0; k < keys.size(); k++)
+        key_range.push_back({((long long)tt.ibeg[keys[k]] << 32) | (unsigned)tt.numpar[keys[k]], (long long)k});
+    std::sort(key_range.begin(), key_range.end());
+    auto key_of = [&](int beg, int cnt) {
+        const long long r = ((long long)beg << 32) | (unsigned)cnt;
+        auto it = std::lower_bound(key_range.begin(), key_range.end(), std::array<long long, 2>{r, -1});
+        return (it != key_range.end() && (*it)[0] == r) ? (int)(*it)[1] : -1;
+    };
+    // (key, key of the source range) of every near pair whose source range is a key
+    std::vector<long long> present;
+    const long long nk = (long long)keys.size();
+    for (size_t k = 0; k < keys.size(); k++)
+        for (int s : segs_of[keys[k]]) {
+            const int ks = key_of(st.ibeg[s], st.numpar[s]);
+            if (ks >= 0) present.push_back((long long)k * nk + ks);
+        }
+    std::sort(present.begin(), present.end());
+    auto has = [&](long long a, long long b) { return std::binary_search(present.begin(), present.end(), a * nk + b); };
+    struct Cand { int k, kb, s; };
+    std::vector<Cand> cand;
+    for (size_t k = 0; k < keys.size(); k++) {
+        const int t = keys[k];
         sym_of[t].assign(segs_of[t].size(), 0);
         for (size_t i = 0; i < segs_of[t].size(); i++) {
             const int s = segs_of[t][i];
-            if (s == t || st.nchild[s] != 0 || !has(s, t)) continue;
+            const int ks = key_of(st.ibeg[s], st.numpar[s]);
+            if (ks < 0 || ks == (int)k || !has(ks, (long long)k)) continue;
             sym_of[t][i] = 1;
-            if (t < s) cand.push_back({t, s});
+            if ((int)k < ks) cand.push_back({(int)k, ks, s});
         }
     }
-    std::stable_sort(cand.begin(), cand.end(), [&](const std::pair<int, int> &a, const std::pair<int, int> &b) {
-        return (long long)st.numpar[a.first] * st.numpar[a.second] >
-               (long long)st.numpar[b.first] * st.numpar[b.second];
+    std::stable_sort(cand.begin(), cand.end(), [&](const Cand &a, const Cand &b) {
+        return (long long)tt.numpar[keys[a.k]] * st.numpar[a.s] > (long long)tt.numpar[keys[b.k]] * st.numpar[b.s];
     });
-    std::vector<int> lidx(tt.n, -1);
-    for (size_t i = 0; i < tt.leaves.size(); i++) lidx[tt.leaves[i]] = (int)i;
-    std::vector<std::vector<long long>> lists(tt.leaves.size());
-    auto far2 = [&](double cx, double cy, double cz, int n) {
-        const double dx = std::max(std::fabs(st.xmin[n] - cx), std::fabs(st.xmax[n] - cx));
-        const double dy = std::max(std::fabs(st.ymin[n] - cy), std::fabs(st.ymax[n] - cy));
-        const double dz = std::max(std::fabs(st.zmin[n] - cz), std::fabs(st.zmax[n] - cz));
+    std::vector<std::vector<long long>> lists(keys.size());
+    auto far2 = [](double cx, double cy, double cz, const Tree &T, int n) {
+        const double dx = std::max(std::fabs(T.xmin[n] - cx), std::fabs(T.xmax[n] - cx));
+        const double dy = std::max(std::fabs(T.ymin[n] - cy), std::fabs(T.ymax[n] - cy));
+        const double dz = std::max(std::fabs(T.zmin[n] - cz), std::fabs(T.zmax[n] - cz));
         return dx * dx + dy * dy + dz * dz;
     };
-    for (const auto &c : cand) {
-        const int t = c.first, s = c.second;
+    for (const Cand &c : cand) {
+        const int t = keys[c.k], s = c.s;
         btk::SymItem it;
-        it.a_beg = sy.soff + st.ibeg[t]; it.a_cnt = st.numpar[t];
+        // targets are the sources: target index = source index
+        it.a_beg = sy.soff + tt.ibeg[t]; it.a_cnt = tt.numpar[t];
         it.b_beg = sy.soff + st.ibeg[s]; it.b_cnt = st.numpar[s];
         it.rslot = sym_slots_len_; sym_slots_len_ += it.a_cnt;
         it.cslot = sym_slots_len_; sym_slots_len_ += it.b_cnt;
-        const double cx = 0.5 * (std::min(st.xmin[t], st.xmin[s]) + std::max(st.xmax[t], st.xmax[s]));
-        const double cy = 0.5 * (std::min(st.ymin[t], st.ymin[s]) + std::max(st.ymax[t], st.ymax[s]));
-        const double cz = 0.5 * (std::min(st.zmin[t], st.zmin[s]) + std::max(st.zmax[t], st.zmax[s]));
-        const double R2 = std::max(far2(cx, cy, cz, t), far2(cx, cy, cz, s));
+        const double cx = 0.5 * (std::min(tt.xmin[t], st.xmin[s]) + std::max(tt.xmax[t], st.xmax[s]));
+        const double cy = 0.5 * (std::min(tt.ymin[t], st.ymin[s]) + std::max(tt.ymax[t], st.ymax[s]));
+        const double cz = 0.5 * (std::min(tt.zmin[t], st.zmin[s]) + std::max(tt.zmax[t], st.zmax[s]));
+        const double R2 = std::max(far2(cx, cy, cz, tt, t), far2(cx, cy, cz, st, s));
         const double thresh = std::max(2e-7 * R2, 1e-300);
         const double g = std::max(1e-150, std::sqrt(thresh));
-        const bool close = st.xmin[t] - st.xmax[s] <= g && st.xmin[s] - st.xmax[t] <= g &&
-                           st.ymin[t] - st.ymax[s] <= g && st.ymin[s] - st.ymax[t] <= g &&
-                           st.zmin[t] - st.zmax[s] <= g && st.zmin[s] - st.zmax[t] <= g;
+        const bool close = tt.xmin[t] - st.xmax[s] <= g && st.xmin[s] - tt.xmax[t] <= g &&
+                           tt.ymin[t] - st.ymax[s] <= g && st.ymin[s] - tt.ymax[t] <= g &&
+                           tt.zmin[t] - st.zmax[s] <= g && st.zmin[s] - tt.zmax[t] <= g;
         sym_ctr_.push_back(cx); sym_ctr_.push_back(cy); sym_ctr_.push_back(cz);
         sym_ctr_.push_back(close ? -thresh : thresh);
-        // targets are the sources: target index = source index
-        lists[lidx[t]].push_back(it.rslot - (sy.toff + tt.ibeg[t]));
-        lists[lidx[s]].push_back(it.cslot - (sy.toff + tt.ibeg[s]));
+        lists[c.k].push_back(it.rslot - (sy.toff + tt.ibeg[t]));
+        lists[c.kb].push_back(it.cslot - (sy.toff + st.ibeg[s]));
         sym_pairs_ += 2LL * it.a_cnt * it.b_cnt;
         sym_items_.push_back(it);
     }
     sym_red_off_.assign(1, 0);
-    for (size_t l = 0; l < tt.leaves.size(); l++) {
-        const int leaf = tt.leaves[l];
-        sym_red_base_.insert(sym_red_base_.end(), lists[l].begin(), lists[l].end());
+    for (size_t k = 0; k < keys.size(); k++) {
+        const int t = keys[k];
+        sym_red_base_.insert(sym_red_base_.end(), lists[k].begin(), lists[k].end());
         sym_red_off_.push_back((int)sym_red_base_.size());
-        if (lists[l].empty()) continue;
-        for (int o = 0; o < tt.numpar[leaf]; o += btk::TPB_SYM_REDUCE)
-            sym_red_tiles_.push_back(btk::Tile{(int)l, sy.toff + tt.ibeg[leaf] + o,
-                                               std::min(btk::TPB_SYM_REDUCE, tt.numpar[leaf] - o), 0});
+        if (lists[k].empty()) continue;
+        for (int o = 0; o < tt.numpar[t]; o += btk::TPB_SYM_REDUCE)
+            sym_red_tiles_.push_back(btk::Tile{(int)k, sy.toff + tt.ibeg[t] + o,
+                                               std::min(btk::TPB_SYM_REDUCE, tt.numpar[t] - o), 0});
     }
     sym_ = !sym_items_.empty();
 }
@@ -329,7 +347,14 @@ void Plan::build_work()
         };
         if (p.compute_type == C_PC) {
             // keys: target batches; pc list -> clusters, pp list -> source leaves
-            for (int b = 0; b < tt.n; b++) near_key(b, to + tt.ibeg[b], tt.numpar[b], L.approx[b], L.pp[b]);
+            std::vector<std::vector<char>> sym_of(tt.n);
+            if (sym_ok_) {
+                std::vector<int> keys(tt.n);
+                for (int b = 0; b < tt.n; b++) keys[b] = b;
+                build_sym(sy, keys, L.pp, sym_of);
+            }
+            for (int b = 0; b < tt.n; b++)
+                near_key(b, to + tt.ibeg[b], tt.numpar[b], L.approx[b], L.pp[b], sym_ok_ ? &sym_of[b] : nullptr);
         } else if (p.compute_type == C_CP) {
             // lists keyed by source batch; invert to target-node keys
             std::vector<std::vector<int>> pp_of(tt.n), cp_of(tt.n);
@@ -338,7 +363,11 @@ void Plan::build_work()
                 for (int t : L.approx[b]) cp_of[t].push_back(b);
             }
             const std::vector<int> none;
-            for (int leaf : tt.leaves) near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], none, pp_of[leaf]);
+            std::vector<std::vector<char>> sym_of(tt.n);
+            if (sym_ok_) build_sym(sy, tt.leaves, pp_of, sym_of);
+            for (int leaf : tt.leaves)
+                near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], none, pp_of[leaf],
+                         sym_ok_ ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (cp_of[t].empty()) continue;
                 add_key(grid_, tn + t, 0, P);
@@ -356,7 +385,7 @@ void Plan::build_work()
                 }
             }
             std::vector<std::vector<char>> sym_of(tt.n);
-            if (sym_ok_) build_sym(sy, segs_of, sym_of);
+            if (sym_ok_) build_sym(sy, tt.leaves, segs_of, sym_of);
             for (int leaf : tt.leaves)
                 near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], clus_of[leaf], segs_of[leaf],
                          sym_ok_ ? &sym_of[leaf] : nullptr);
@@ -686,21 +715,19 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
 
     herm_ = prm.approximation == A_HERMITE;
-    // symmetric near field: one CC system (Coulomb or Yukawa, or regularized
+    // symmetric near field: one system (Coulomb or Yukawa, or regularized
     // Coulomb with skipping) whose targets are its sources, particle for
-    // particle (after the tree builds: the same order and the same tree)
+    // particle in tree order (the same order after both tree builds)
     sym_ok_ = false;
-    if (sys_.size() == 1 && prm.compute_type == C_CC && prm.approximation == A_LAGRANGE &&
+    if (sys_.size() == 1 && prm.approximation == A_LAGRANGE &&
         (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA ||
          (prm.kernel == K_REG_COULOMB && prm.singularity == S_SKIPPING))) {
         const System &y = sys_[0];
-        const Tree &a = y.stree, &b = y.ttree;
         auto same = [&](const HostVec<double> &u, const HostVec<double> &v) {
             return std::memcmp(u.data(), v.data(), (size_t)y.M * sizeof(double)) == 0;
         };
-        sym_ok_ = y.M == y.N && a.n == b.n && a.ibeg == b.ibeg && a.numpar == b.numpar &&
-                  a.nchild == b.nchild && a.leaves == b.leaves && same(y.S.x, y.T.x) &&
-                  same(y.S.y, y.T.y) && same(y.S.z, y.T.z) && same(y.S.q, y.T.q);
+        sym_ok_ = y.M == y.N && same(y.S.x, y.T.x) && same(y.S.y, y.T.y) && same(y.S.z, y.T.z) &&
+                  same(y.S.q, y.T.q);
     }
     build_work();
     enumerate_pairs();
@@ -864,6 +891,51 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     return "";
 }
 
+// Upward pass at the plan's degree over the source trees
+// (clusters.c:50-200): grid points, leaves from their particles, then every
+// level's parents from their children's grids, deepest first.
+std::string Plan::upward_fixed()
+{
+    const Params &p = prm_;
+    const int nl = p.degree + 1;
+    HIPCHK(btk::launch_cluster_points(nsnodes_, nl, d_stab_, d_cx_, d_cy_, d_cz_, st_));
+    btk::AnterpArgs a{};
+    a.degree = p.degree; a.subtraction = p.singularity == S_SUBTRACTION;
+    a.ibeg = d_sibeg_; a.iend = d_siend_; a.tab = d_stab_;
+    a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
+    a.Q = d_cq_; a.W = d_cw_;
+    a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)g_sleaves_.size();
+    HIPCHK(btk::launch_anterp(a, st_));
+    for (size_t lv = 0; lv + 1 < up_levels_off_.size(); lv++) {
+        const int b = up_levels_off_[lv], e = up_levels_off_[lv + 1];
+        if (e == b) continue;
+        a.mode = 1; a.nodes = d_up_nodes_ + b; a.nblocks = e - b;
+        a.child_off = d_up_child_off_ + b; a.child_id = d_up_child_id_;
+        HIPCHK(btk::launch_anterp(a, st_));
+    }
+    return "";
+}
+
+// Downpass at the plan's degree over the target trees
+// (interaction_compute_downpass.c:37-142): parents' interpolants onto the
+// children's grids level by level, then the leaves' onto their targets.
+std::string Plan::down_fixed()
+{
+    const Params &p = prm_;
+    btk::DownArgs d{};
+    d.degree = p.degree; d.subtraction = p.singularity == S_SUBTRACTION; d.tab = d_ttab_; d.parent = d_tparent_;
+    d.tx = d_tx_; d.ty = d_ty_; d.tz = d_tz_; d.tq = d_tq_; d.Q = d_Qt_; d.W = d_Wt_; d.pot = d_pot_;
+    for (size_t lv = 0; lv + 1 < down_levels_off_.size(); lv++) {
+        const int b = down_levels_off_[lv], e = down_levels_off_[lv + 1];
+        if (e == b) continue;
+        d.mode = 0; d.nodes = d_down_nodes_ + b; d.nblocks = e - b;
+        HIPCHK(btk::launch_down(d, st_));
+    }
+    d.mode = 1; d.tiles = d_leaf_tiles_; d.nblocks = (int)leaf_tiles_.size();
+    HIPCHK(btk::launch_down(d, st_));
+    return "";
+}
+
 // The symmetric near field's items into their slots, then the slots into
 // the potential (after the near launch has stored every target's sum).
 bool Plan::sym_ready()
@@ -927,21 +999,8 @@ std::string Plan::execute(bool timed)
         std::string r = hermite_upward();
         if (!r.empty()) return r;
     } else if (p.compute_type != C_CP) {
-        HIPCHK(btk::launch_cluster_points(nsnodes_, nl, d_stab_, d_cx_, d_cy_, d_cz_, st_));
-        btk::AnterpArgs a{};
-        a.degree = p.degree; a.subtraction = ss;
-        a.ibeg = d_sibeg_; a.iend = d_siend_; a.tab = d_stab_;
-        a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
-        a.Q = d_cq_; a.W = d_cw_;
-        a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)g_sleaves_.size();
-        HIPCHK(btk::launch_anterp(a, st_));
-        for (size_t lv = 0; lv + 1 < up_levels_off_.size(); lv++) {
-            const int b = up_levels_off_[lv], e = up_levels_off_[lv + 1];
-            if (e == b) continue;
-            a.mode = 1; a.nodes = d_up_nodes_ + b; a.nblocks = e - b;
-            a.child_off = d_up_child_off_ + b; a.child_id = d_up_child_id_;
-            HIPCHK(btk::launch_anterp(a, st_));
-        }
+        std::string r = upward_fixed();
+        if (!r.empty()) return r;
     }
     HIPCHK(hipEventRecord(ev[1], st_));
 
@@ -988,6 +1047,8 @@ std::string Plan::execute(bool timed)
     } else if (p.compute_type != C_PC) {
         if (!grid_tiles_.empty()) {
             btk::InteractArgs a = ia;
+            // acceptance-separated sources: the centred expansion of r^2 (mixed)
+            a.grid_scaled = p.precision == PREC_MIXED && p.theta < 0.99;
             a.grid_targets = 1; a.ntiles = (int)grid_tiles_.size(); a.tiles = d_grid_tiles_;
             a.seg_off = d_gseg_off_; a.segs = d_gsegs_; a.clu_off = d_gclu_off_; a.clu_id = d_gclu_id_;
             a.tnode_tab = d_ttab_;
@@ -1007,17 +1068,8 @@ std::string Plan::execute(bool timed)
         std::string r = hermite_down();
         if (!r.empty()) return r;
     } else if (p.compute_type != C_PC) {
-        btk::DownArgs d{};
-        d.degree = p.degree; d.subtraction = ss; d.tab = d_ttab_; d.parent = d_tparent_;
-        d.tx = d_tx_; d.ty = d_ty_; d.tz = d_tz_; d.tq = d_tq_; d.Q = d_Qt_; d.W = d_Wt_; d.pot = d_pot_;
-        for (size_t lv = 0; lv + 1 < down_levels_off_.size(); lv++) {
-            const int b = down_levels_off_[lv], e = down_levels_off_[lv + 1];
-            if (e == b) continue;
-            d.mode = 0; d.nodes = d_down_nodes_ + b; d.nblocks = e - b;
-            HIPCHK(btk::launch_down(d, st_));
-        }
-        d.mode = 1; d.tiles = d_leaf_tiles_; d.nblocks = (int)leaf_tiles_.size();
-        HIPCHK(btk::launch_down(d, st_));
+        std::string r = down_fixed();
+        if (!r.empty()) return r;
     }
     HIPCHK(hipEventRecord(ev[4], st_));
 

@@ -59,6 +59,7 @@ struct DegreeWork {
     int degree = 0, nl = 0, P = 0;
     bool used = false;                     // some pair has this degree
     std::vector<int> s_ibeg, s_iend;       // global particle range per source slot
+    std::vector<int> s_node;               // global source node per source slot
     // a source slot's charges are summed from parts of at most PART_MAX
     // particles (one block each): part k covers [p_ibeg[k], p_iend[k]) of
     // slot p_slot[k]; slot s owns parts [p_off[s], p_off[s+1])
@@ -70,6 +71,7 @@ struct DegreeWork {
     int num_tslots() const { return nl ? (int)(t_tab.size() / (3 * nl)) : 0; }
     // device copies
     int *d_p_slot = nullptr, *d_p_ibeg = nullptr, *d_p_iend = nullptr, *d_p_off = nullptr;
+    int *d_s_node = nullptr;
     double *d_Qp = nullptr, *d_Wp = nullptr;
     double *d_s_tab = nullptr, *d_cx = nullptr, *d_cy = nullptr, *d_cz = nullptr;
     double *d_cq = nullptr, *d_cw = nullptr;
@@ -162,10 +164,12 @@ private:
     template <class T, class A> std::string up_at(T *d, const std::vector<T, A> &h);
     void free_all();
     void build_work();
-    // the symmetric near field's items, slots and reduction lists (CC, one
+    // the symmetric near field's items, slots and reduction lists (one
     // system whose targets are its sources): sym_ok_ set by create
-    void build_sym(const System &sy, const std::vector<std::vector<int>> &segs_of,
+    void build_sym(const System &sy, const std::vector<int> &keys, const std::vector<std::vector<int>> &segs_of,
                    std::vector<std::vector<char>> &sym_of);
+    std::string upward_fixed();  // upward pass at the plan's degree
+    std::string down_fixed();    // downpass at the plan's degree
     bool sym_ready();  // slots allocated (at the first mixed execution)
     std::string near_sym();
     std::string validate() const;
@@ -197,9 +201,11 @@ private:
     // (negated: the key's clusters are checked too)
     std::vector<double> near_ctr_;
     // symmetric near field: mixed precision, Coulomb or Yukawa (either
-    // singularity) or regularized Coulomb (skipping), CC, one system whose
-    // targets are its sources (same particles, same tree).  Leaf pairs (A, B), each leaf in the other's
-    // near list, are evaluated once for both directions (items, sorted by
+    // singularity) or regularized Coulomb (skipping), one system whose
+    // targets are its sources (the same particles in the same tree order).
+    // Pairs of near-field keys (A, B), each key's particle range a source
+    // range of the other's near list, are evaluated once for both
+    // directions (items, sorted by
     // work); their source ranges carry SEG_SYM and the near launch passes
     // over them.  Item k writes A's sums to slots [rslot, +|A|) and B's to
     // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
@@ -233,6 +239,18 @@ private:
     double var_mean_deg_ = 0;
     long long var_evals_ = 0;
     std::vector<DegreeWork> dw_;
+    // Slots of degree p <= the plan's degree D go through the plan's own
+    // degree-D grids: source slots take (E_p x E_p x E_p) of their node's
+    // degree-D charges (fold_up_), target slots are carried onto their node's
+    // degree-D grid and the plan's downpass (fold_down_); target node
+    // fold_node_[b] sums the slots fold_item_[fold_off_[b] .. fold_off_[b+1]).
+    // emat_ holds E_p (p = 1..D, (p+1) x (D+1) each) at emat_off_[p].
+    bool fold_up_ = false, fold_down_ = false;
+    std::vector<int> fold_node_, fold_off_, fold_item_;
+    std::vector<double> emat_;
+    std::vector<size_t> emat_off_;
+    int *d_fold_node_ = nullptr, *d_fold_off_ = nullptr, *d_fold_item_ = nullptr;
+    double *d_emat_ = nullptr;
     // one near launch (particle ranges + source slots of every degree) and
     // one far launch (every target slot) for all degrees
     WorkCSR vnear_, vfar_;

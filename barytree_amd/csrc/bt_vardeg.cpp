@@ -126,6 +126,11 @@ void Plan::free_var()
     d_vf_tiles_ = nullptr;
     d_vf_row_ = nullptr;
     d_vf_R_ = d_vf_RW_ = nullptr;
+    fold_up_ = fold_down_ = false;
+    fold_node_.clear(); fold_off_.clear(); fold_item_.clear();
+    emat_.clear(); emat_off_.clear();
+    d_fold_node_ = d_fold_off_ = d_fold_item_ = nullptr;
+    d_emat_ = nullptr;
 }
 
 namespace {
@@ -166,6 +171,7 @@ std::string Plan::set_pair_degrees(const int *deg)
         if (sl < 0) {
             DegreeWork &w = dw_[p];
             sl = (int)w.s_ibeg.size();
+            w.s_node.push_back(y.snode_off + s);
             w.s_ibeg.push_back(y.soff + y.stree.ibeg[s]);
             w.s_iend.push_back(y.soff + y.stree.iend[s]);
             if (stab[sy][p].empty()) node_table(y.stree, p, stab[sy][p]);
@@ -277,12 +283,52 @@ std::string Plan::set_pair_degrees(const int *deg)
     }
     make_queue_tiles(vfar_, GRID_TILE, true, 0, vfar_tiles_, vfar_qlen_, true);
 
+    // degrees up to the plan's own go through its degree-D grids (exact in
+    // exact arithmetic; see bt_degree.hip)
+    const int D = prm_.degree;
+    {
+        std::vector<std::vector<int>> items(ntnodes_);
+        for (int p = 1; p <= std::min(D, MAXDEG); p++) {
+            if (!used[p]) continue;
+            if (!dw_[p].s_ibeg.empty()) fold_up_ = true;
+            for (size_t sl = 0; sl < slot_node[p].size(); sl++) {
+                items[sys_[slot_sys[p][sl]].tnode_off + slot_node[p][sl]].push_back(multi_id(p, (int)sl));
+                fold_down_ = true;
+            }
+        }
+        fold_off_.assign(1, 0);
+        for (int n = 0; n < ntnodes_; n++) {
+            if (items[n].empty()) continue;
+            fold_node_.push_back(n);
+            fold_item_.insert(fold_item_.end(), items[n].begin(), items[n].end());
+            fold_off_.push_back((int)fold_item_.size());
+        }
+        // E_p[k][m] = l^p_k(t^D_m), t^n_i = cos(i pi / n) (the node tables'
+        // reference points), by the product formula in long double
+        emat_off_.assign(D + 1, 0);
+        for (int p = 1; p <= D; p++) {
+            emat_off_[p] = emat_.size();
+            for (int k = 0; k <= p; k++)
+                for (int m = 0; m <= D; m++) {
+                    const long double t = cosl((long double)m * M_PI / D);
+                    const long double tk = cosl((long double)k * M_PI / p);
+                    long double v = 1.0L;
+                    for (int j = 0; j <= p; j++) {
+                        if (j == k) continue;
+                        const long double tj = cosl((long double)j * M_PI / p);
+                        v *= (t - tj) / (tk - tj);
+                    }
+                    emat_.push_back((double)v);
+                }
+        }
+    }
     for (int p = 1; p <= MAXDEG; p++) {
         if (!used[p]) continue;
         DegreeWork &w = dw_[p];
         // downpass: each target slot's interpolant at its node's particles,
-        // one launch per tree level (nodes of a level have disjoint ranges)
-        const int T = (int)slot_node[p].size();
+        // one launch per tree level (nodes of a level have disjoint ranges);
+        // degrees up to D are folded instead
+        const int T = p <= D ? 0 : (int)slot_node[p].size();
         w.down.assign(tdepth_ + 1, {});
         for (int sl = 0; sl < T; sl++) {
             const System &y = sys_[slot_sys[p][sl]];
@@ -297,10 +343,10 @@ std::string Plan::set_pair_degrees(const int *deg)
                 w.down[lv].push_back(tl);
             }
         }
-        // parts of the source slots
+        // parts of the source slots (degrees above D)
         w.p_slot.clear(); w.p_ibeg.clear(); w.p_iend.clear(); w.p_off.assign(1, 0);
         for (size_t sl = 0; sl < w.s_ibeg.size(); sl++) {
-            for (int b = w.s_ibeg[sl]; b < w.s_iend[sl]; b += PART_MAX) {
+            for (int b = w.s_ibeg[sl]; p > D && b < w.s_iend[sl]; b += PART_MAX) {
                 w.p_slot.push_back((int)sl);
                 w.p_ibeg.push_back(b);
                 w.p_iend.push_back(std::min(b + PART_MAX, w.s_iend[sl]));
@@ -363,6 +409,15 @@ std::string Plan::set_pair_degrees(const int *deg)
         w.d_down.assign(w.down.size(), nullptr);
         for (size_t lv = 0; lv < w.down.size(); lv++)
             if (!w.down[lv].empty() && !(r = up(&w.d_down[lv], w.down[lv])).empty()) return fail(r);
+        if (S > 0 && p <= D && !(r = up(&w.d_s_node, w.s_node)).empty()) return fail(r);
+    }
+    if (fold_up_ || fold_down_) {
+        if (!(r = up(&d_emat_, emat_)).empty()) return fail(r);
+    }
+    if (fold_down_) {
+        if (!(r = up(&d_fold_node_, fold_node_)).empty()) return fail(r);
+        if (!(r = up(&d_fold_off_, fold_off_)).empty()) return fail(r);
+        if (!(r = up(&d_fold_item_, fold_item_)).empty()) return fail(r);
     }
     alloc_var_ = false;
     HIPCHK(hipStreamSynchronize(st_));
@@ -442,7 +497,24 @@ std::string Plan::validate_var() const
             for (const btk::Tile &t : lv)
                 if (t.x < 0 || t.x >= T || t.y < 0 || t.z <= 0 || t.z > DOWN_TILE || (long long)t.y + t.z > M_)
                     return "down tile";
+        if ((int)w.s_node.size() != S) return "source slot nodes";
+        for (int n : w.s_node)
+            if (n < 0 || n >= nsnodes_) return "source slot node";
     }
+    const int D = prm_.degree;
+    if (fold_off_.empty() || fold_off_.size() != fold_node_.size() + 1 || fold_off_[0] != 0 ||
+        fold_off_.back() != (int)fold_item_.size())
+        return "fold lists";
+    for (size_t b = 0; b < fold_node_.size(); b++) {
+        if (fold_node_[b] < 0 || fold_node_[b] >= ntnodes_ || fold_off_[b] > fold_off_[b + 1]) return "fold node";
+        for (int k = fold_off_[b]; k < fold_off_[b + 1]; k++)
+            if ((fold_item_[k] >> btk::DEG_SHIFT) > D || !slot_ok(fold_item_[k], true)) return "fold item";
+    }
+    if ((fold_up_ || fold_down_) && (D < 1 || D > btk::MAX_DEGREE || (int)emat_off_.size() != D + 1)) return "fold degree";
+    for (int p = 1; p <= D && p < (int)emat_off_.size(); p++)
+        if (emat_off_[p] + (size_t)(p + 1) * (D + 1) > emat_.size()) return "fold matrices";
+    if (fold_up_ && prm_.compute_type == C_CP) return "fold of source slots in a CP plan";
+    if (fold_down_ && prm_.compute_type == C_PC) return "fold of target slots in a PC plan";
     return "";
 }
 
@@ -523,11 +595,26 @@ std::string Plan::execute_var(bool timed)
     sym_ran_ = false;
     HIPCHK(hipEventRecord(ev[0], st_));
 
-    // 1. source slots: modified charges of degree p from the node's particles
+    // 1. source slots: modified charges of degree p -- up to the plan's
+    //    degree from the node's own degree-D charges, above it from the
+    //    node's particles
+    const int D = p.degree;
+    if (fold_up_) {
+        std::string r = upward_fixed();
+        if (!r.empty()) return r;
+    }
     for (const DegreeWork &w : dw_) {
         const int S = (int)w.s_ibeg.size();
         if (!w.used || S == 0) continue;
         HIPCHK(btk::launch_cluster_points(S, w.nl, w.d_s_tab, w.d_cx, w.d_cy, w.d_cz, st_));
+        if (w.degree <= D) {
+            btk::DegreeUpArgs u{};
+            u.nslots = S; u.p = w.degree; u.D = D; u.subtraction = ss ? 1 : 0;
+            u.node = w.d_s_node; u.E = d_emat_ + emat_off_[w.degree];
+            u.QD = d_cq_; u.WD = d_cw_; u.Qp = w.d_cq; u.Wp = w.d_cw;
+            HIPCHK(btk::launch_degree_up(u, st_));
+            continue;
+        }
         btk::AnterpArgs a{};
         a.degree = w.degree; a.subtraction = ss;
         a.ibeg = w.d_p_ibeg; a.iend = w.d_p_iend; a.tab = w.d_s_tab;
@@ -578,6 +665,8 @@ std::string Plan::execute_var(bool timed)
     // 3. every target slot's grid values (rows), then the sum per slot
     if (!vfar_.key.empty()) {
         btk::InteractArgs a = ia;
+        // acceptance-separated sources: the centred expansion of r^2 (mixed)
+        a.grid_scaled = p.precision == PREC_MIXED && p.theta < 0.99;
         a.grid_targets = 1; a.ntiles = (int)vfar_tiles_.size(); a.tiles = d_vf_tiles_;
         a.seg_off = d_vf_seg_off_; a.segs = d_vf_segs_; a.clu_off = d_vf_clu_off_; a.clu_id = d_vf_clu_id_;
         a.out1 = d_vf_R_; a.out2 = d_vf_RW_; a.out_row = d_vf_row_;
@@ -593,7 +682,26 @@ std::string Plan::execute_var(bool timed)
     }
     HIPCHK(hipEventRecord(ev[3], st_));
 
-    // 4. each target slot's interpolant at its node's particles
+    // 4. target slots up to the plan's degree onto their nodes' degree-D
+    //    grids and through the plan's downpass; above it, each slot's
+    //    interpolant at its node's particles
+    if (fold_down_) {
+        const size_t nq = (size_t)ntnodes_ * p.P;
+        HIPCHK(hipMemsetAsync(d_Qt_, 0, nq * sizeof(double), st_));
+        if (ss) HIPCHK(hipMemsetAsync(d_Wt_, 0, nq * sizeof(double), st_));
+        btk::DegreeDownArgs f{};
+        f.nnodes = (int)fold_node_.size(); f.D = D; f.subtraction = ss ? 1 : 0;
+        f.node = d_fold_node_; f.off = d_fold_off_; f.item = d_fold_item_;
+        for (const DegreeWork &w : dw_) {
+            if (!w.used || w.degree > D) continue;
+            f.E[w.degree] = d_emat_ + emat_off_[w.degree];
+            f.Qs[w.degree] = w.d_Qt; f.Ws[w.degree] = w.d_Wt;
+        }
+        f.QD = d_Qt_; f.WD = d_Wt_;
+        HIPCHK(btk::launch_degree_down(f, st_));
+        std::string r = down_fixed();
+        if (!r.empty()) return r;
+    }
     for (const DegreeWork &w : dw_) {
         if (!w.used) continue;
         for (size_t lv = 0; lv < w.down.size(); lv++) {

@@ -43,6 +43,23 @@ def test_uniform_pair_degree_equals_fixed(bt, ct, k, s):
         assert np.array_equal(p.execute(), fixed)
 
 
+@pytest.mark.parametrize("ct,k,s", [(1, 1, 2), (2, 2, 1), (3, 1, 2), (3, 3, 1)])
+def test_pair_degrees_below_and_above_the_plan_degree(bt, oracle_mod, ct, k, s):
+    """Degrees up to the plan's own (7 here) go through the plan's degree-7
+    grids (change of degree on the source and target side), higher ones
+    from and to the particles; both against the oracle."""
+    print(f"case ct={ct} k={k} s={s}", flush=True)
+    t, src = uniform_cube(2500, 2000, seed=90 + 10 * ct + k)
+    args = (k, KP[k][s], s, 1, ct, 0.7, 7, 60, 50)
+    with bt.Plan(t, src, *args) as p:
+        n = p.num_pairs()
+        deg = np.random.default_rng(ct * 7 + k).integers(1, 13, n)
+        p.set_pair_degrees(deg)
+        got = p.execute()
+        ref = oracle_mod.treecode_vardeg(*t, *src, *args, deg)
+        assert rel_l2(ref, got) <= 1e-12
+
+
 @pytest.mark.parametrize("ct,k,s", supported_combos())
 def test_random_pair_degrees_match_oracle(bt, oracle_mod, ct, k, s):
     print(f"case ct={ct} k={k} s={s}", flush=True)

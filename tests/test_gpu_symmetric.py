@@ -1,6 +1,7 @@
-"""Symmetric near field: mixed executions of a cluster-cluster system whose
-targets are its sources (Coulomb, Yukawa, or regularized Coulomb with
-skipping) evaluate a pair of leaves once for both particles.  Checked against the oracle (the
+"""Symmetric near field: mixed executions of a system whose targets are its
+sources in the same tree order (Coulomb, Yukawa, or regularized Coulomb with
+skipping; PC, CP or CC with equal leaf sizes) evaluate a pair of leaves once
+for both particles.  Checked against the oracle (the
 reference's FP64 arithmetic) at the mixed-precision bar, against the same
 system solved without the symmetric path (a batched plan of two systems
 never takes it), for run-to-run reproducibility, on pairs planted across
@@ -54,6 +55,24 @@ def test_symmetric_matches_oracle_and_is_reproducible(bt, oracle_mod, k, sg):
         fp64 = plan.execute()
     ref = oracle_mod.treecode(*t, *s, *args)
     assert rel_l2(ref, fp64) <= 1e-12
+    err = rel_l2(ref, got)
+    print(f"symmetric mixed vs oracle: {err:.3e}", flush=True)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("ct,k,sg", [(1, 1, 2), (1, 2, 1), (1, 3, 1), (2, 1, 1), (2, 2, 2), (2, 3, 1)])
+def test_symmetric_particle_cluster_and_cluster_particle(bt, oracle_mod, ct, k, sg):
+    """PC (target batches) and CP (source batches) plans with equal leaf
+    sizes order targets and sources alike and take the symmetric path."""
+    print(f"case ct={ct} kernel={k} singularity={sg}", flush=True)
+    t, s = same_set(20000, 50 + 3 * ct + k + sg)
+    args = (k, KPS[k], sg, 1, ct, 0.7, 5, 300, 300)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        st = plan.stats()
+        assert st["sym_ptwise"] > 0.8 * st["pp_ptwise"], st
+        got = plan.execute()
+        assert np.array_equal(got, plan.execute())
+    ref = oracle_mod.treecode(*t, *s, *args)
     err = rel_l2(ref, got)
     print(f"symmetric mixed vs oracle: {err:.3e}", flush=True)
     assert err < 1e-6

@@ -234,9 +234,9 @@ def test_variable_degree_work_on_host(bt, ct, k, s):
 
 
 def test_symmetric_near_field_eligibility(bt):
-    """The symmetric near field covers nearly every near pair of a CC system
+    """The symmetric near field covers nearly every near pair of a system
     (Coulomb, Yukawa, or regularized Coulomb with skipping) whose targets are
-    its sources, and nothing otherwise."""
+    its sources in the same tree order, and nothing otherwise."""
     rng = np.random.default_rng(8)
     n = 60000
     x, y, z, q = (rng.uniform(-1, 1, n) for _ in range(4))
@@ -247,8 +247,11 @@ def test_symmetric_near_field_eligibility(bt):
         a = bt.analyze(same, src, k, [0.5], s, 1, 3, 0.8, 6, 800, 800)
         assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], a
         assert a["sym_ptwise"] % 2 == 0
-    assert bt.analyze(same, src, 1, [0.5], 2, 1, 1, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # PC
+    for ct in (1, 2):  # particle-cluster and cluster-particle with equal leaf sizes
+        a = bt.analyze(same, src, 1, [0.5], 1, 1, ct, 0.8, 6, 800, 800)
+        assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], (ct, a)
     assert bt.analyze(same, src, 1, [0.5], 1, 2, 2, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # CP Hermite
+    assert bt.analyze(same, src, 1, [0.5], 1, 1, 1, 0.8, 6, 800, 100)["sym_ptwise"] == 0  # PC, other order
     assert bt.analyze(same, src, 8, [1.5], 1, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # sin-over-r
     # different trees
     assert bt.analyze(same, src, 1, [0.5], 2, 1, 3, 0.8, 6, 800, 100)["sym_ptwise"] == 0
