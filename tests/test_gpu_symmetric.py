@@ -134,6 +134,19 @@ def test_symmetric_close_pairs_across_leaves(bt, oracle_mod, k, sg):
     assert rel_l2(ref[rest], got[rest]) < 1e-6
 
 
+@pytest.mark.parametrize("leaf", [1, 7, 33])
+def test_symmetric_tiny_leaves(bt, oracle_mod, leaf):
+    """Leaves far smaller than a pass (128 targets) or a staged chunk (256
+    sources): every item is mostly padding lanes and dummy sources."""
+    t, s = same_set(3000, 60 + leaf)
+    args = (1, KPS[1], 2, 1, 3, 0.8, 3, leaf, leaf)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        assert plan.stats()["sym_ptwise"] > 0
+        got = plan.execute()
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert rel_l2(ref, got) < 1e-6
+
+
 def test_symmetric_with_pair_degrees(bt):
     t, s = same_set(100000, 40)
     args = (1, KPS[1], 2, 1, 3, 0.8, 8, 1000, 1000)
