@@ -931,6 +931,33 @@ __device__ __forceinline__ void near_mfma_tile(const InteractArgs &A, const Tile
     };
     auto consume = [&](int n, bool check) {
         if (nbv == 0) return;  // no target of this wave in the tile
+        if (!check) {
+            // two chunks a step: the second chunk's matrix products run while
+            // the first chunk's pairs are evaluated (sc <= 224, so both chunks
+            // lie within the TPB staged entries; entries past n are dummies)
+            for (int sc = 0; sc < n; sc += 32) {
+                const int j0 = sc + ii, j1 = sc + 16 + ii;
+                const double bop0 = LB[kk * TPB + j0], bop1 = LB[kk * TPB + j1];
+                const float2 f0 = LC[j0], f1 = LC[j1];
+                const double a0 = LA[j0], a1 = LA[j1];
+                v4d D0[NB], D1[NB];
+#pragma unroll
+                for (int b = 0; b < NB; b++) D0[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[b], bop0, Cop[b], 0, 0, 0);
+#pragma unroll
+                for (int b = 0; b < NB; b++) D1[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[b], bop1, Cop[b], 0, 0, 0);
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        pair_pot32<KER, SS, false>((float)D0[b][v], a0, f0.x, f0.y, tq[b][v], k32, acc[b][v]);
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        pair_pot32<KER, SS, false>((float)D1[b][v], a1, f1.x, f1.y, tq[b][v], k32, acc[b][v]);
+            }
+            return;
+        }
         for (int sc = 0; sc < n; sc += 16) {
             const int j = sc + ii;
             const double bop = LB[kk * TPB + j];
