@@ -296,7 +296,6 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     const int nu = max(0, min(TPT, (tile.z - wbase + TPB - 1) / TPB));
 
     double Tx[TPT], Ty[TPT], Tz[TPT], Tq[TPT], acc[TPT], accw[TPT];
-    float Tqf[TPT];
     int tix[TPT];
     // target grid: degree, node tables and output row of this tile
     int nl = A.nl;
@@ -328,7 +327,6 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
             Tx[u] = A.tx[i]; Ty[u] = A.ty[i]; Tz[u] = A.tz[i];
             Tq[u] = SS ? A.tq[i] : 0.0;
         }
-        Tqf[u] = (float)Tq[u];
         acc[u] = 0.0; accw[u] = 0.0;
     }
     // Mixed subtraction kernels on particle targets: coordinates relative to
@@ -477,15 +475,17 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
         for (int j = 0; j < n; ++j) {
             const double4 s = Ls[j];
             const double b = SS ? Lb[j] : 0.0;
-            const float2 f = make_float2(0.f, 0.f);
 #pragma unroll
             for (int u = 0; u < NU; u++) {
                 if constexpr (MIXED) {
-                    // FP64 displacement and r^2 (full-rate FP64 on gfx950), rounded once
-                    const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
-                    const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
-                    if constexpr (GRID) pair_grid32<KER, SS>(r2, s.w, b, k32, acc[u], accw[u]);
-                    else pair_pot32<KER, SS, CHECK>(r2, s.w, f.x, f.y, Tqf[u], k32, acc[u]);
+                    // grid targets outside the scaled form (theta >= 0.99): FP64
+                    // displacement and r^2, rounded once (mixed particle targets
+                    // have returned from the scaled path above)
+                    if constexpr (GRID) {
+                        const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
+                        const float r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
+                        pair_grid32<KER, SS>(r2, s.w, b, k32, acc[u], accw[u]);
+                    }
                 } else {
                     const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
                     if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
@@ -1196,23 +1196,24 @@ hipError_t plain_dispatch(const InteractArgs &a, hipStream_t st)
 template <int KER, bool SS>
 hipError_t interact_dispatch(const InteractArgs &a, hipStream_t st)
 {
-    // the matrix-core near field pays where the pair kernel is short (the
-    // skipping kernels); the subtraction kernels stay on the VALU kernel
-    // mixed particle-target launches form r^2 around the key centres
-    if (!a.grid_targets && a.precision == 1 && !a.key_ctr) return hipErrorInvalidValue;
-    if (!a.grid_targets && a.precision == 1 && !SS) {
-        // 4 row blocks of 16 targets per wave: 256 targets per pass of a tile
-        return near_mfma_launch<KER, SS, 4>(a, st);
-    }
     if (a.grid_targets) {
         if (a.precision == 1) return interact_launch<KER, SS, true, true, GRID_TPT>(a, st);
         return interact_launch<KER, SS, true, false, GRID_TPT>(a, st);
     }
-    if (a.tpt == 1) {
-        if (a.precision == 1) return interact_launch<KER, SS, false, true, 1>(a, st);
-        return interact_launch<KER, SS, false, false, 1>(a, st);
+    if (a.precision == 1) {
+        // mixed particle-target launches form r^2 around the key centres: the
+        // skipping kernels on the matrix cores (4 row blocks of 16 targets a
+        // wave, 256 targets per pass of a tile), the subtraction kernels on
+        // the scaled VALU path
+        if (!a.key_ctr) return hipErrorInvalidValue;
+        if constexpr (!SS) {
+            return near_mfma_launch<KER, SS, 4>(a, st);
+        } else {
+            if (a.tpt == 1) return interact_launch<KER, SS, false, true, 1>(a, st);
+            return interact_launch<KER, SS, false, true, 2>(a, st);
+        }
     }
-    if (a.precision == 1) return interact_launch<KER, SS, false, true, 2>(a, st);
+    if (a.tpt == 1) return interact_launch<KER, SS, false, false, 1>(a, st);
     return interact_launch<KER, SS, false, false, 2>(a, st);
 }
 
