@@ -1,6 +1,7 @@
 """The benchmark configurations at parity-test size (against the oracle,
 <= 1e-12) and at full size (against the FP64 direct sum at 1000 sampled
-targets, the survey's error metric).  Distributions: barytree_amd/workloads.py."""
+targets, the error metric of examples/support_fns.c:694-730).
+Distributions: barytree_amd/workloads.py."""
 import numpy as np
 import pytest
 
