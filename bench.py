@@ -29,6 +29,10 @@ theta 0.8, degree 8, leaf 4000, plain FP64 against mixed precision.
 degree 8, leaf 2000, skipping, FP64, against the reference's own sources
 compiled for one CPU core.
 
+--workload hermite: Coulomb with the Hermite approximation, particle-cluster
+and cluster-particle, degree 5, against the reference's own sources on one
+CPU core at N = 100k, and at N = 2M on the GPU.
+
 --workload batched measures the batching extension instead: 256 independent
 Coulomb systems (N log-uniform in [1k, 64k], 8-Gaussian mixtures, sigma 0.05,
 sources = targets), particle-cluster, theta 0.7, degree 6: the autotuned
@@ -453,6 +457,68 @@ def bench_cpu_case(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_hermite(args):
+    """Hermite approximation, Coulomb with skipping, uniform in [-1,1]^3
+    (seed 7), theta 0.7, degree 5, leaf 2000, FP64: particle-cluster and
+    cluster-particle at N = 100k against the reference's own sources on one
+    CPU core (oracle/_ref; particle-cluster), and at N = 2M on the GPU alone,
+    each checked against the FP64 direct sum at 1000 stride-sampled targets."""
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from oracle import oracle as O
+
+    res = {}
+    for n in (100_000, 2_000_000):
+        x, y, z, q, w = gen_uniform(n, 7)
+        targets = (x.copy(), y.copy(), z.copy(), q.copy())
+        idx = np.arange(0, n, n // 1000)[:1000]
+        for name, ct in (("pc", bt.ComputeType.PARTICLE_CLUSTER), ("cp", bt.ComputeType.CLUSTER_PARTICLE)):
+            kargs = (bt.Kernel.COULOMB, [0.5], bt.Singularity.SKIPPING, bt.Approximation.HERMITE, ct,
+                     0.7, 5, 2000, 2000)
+            plan = bt.Plan(targets, (x, y, z, q, w), *kargs)
+            wall, gpu, near, st = time_steps(plan, bt.PRECISION_FP64, args.steps, args.warmup)
+            out = plan.execute()
+            err = rel_l2(plan.direct_sample(idx), out[idx])
+            plan.close()
+            res[f"{name}_n{n}"] = {"s": round(wall, 5), "rel_l2_err": err, "out": out}
+            log(f"[bench] hermite {name} N={n}: {wall:.4f} s/step, relL2 {err:.2e}")
+    n = 100_000
+    x, y, z, q, w = gen_uniform(n, 7)
+    kind = "reference" if O.ref_available() else "port"
+    fn = O.ref_treecode if kind == "reference" else O.treecode
+    t0 = time.perf_counter()
+    ref = fn(x.copy(), y.copy(), z.copy(), q.copy(), x, y, z, q, w, 1, [0.5], 1, 2, 1, 0.7, 5, 2000, 2000)
+    cpu_s = time.perf_counter() - t0
+    diff = rel_l2(ref, res["pc_n100000"]["out"])
+    log(f"[bench] hermite pc N={n}: cpu {kind} {cpu_s:.2f} s, gpu vs cpu relL2 {diff:.2e}")
+    for v in res.values():
+        v.pop("out")
+    gpu_s = res["pc_n100000"]["s"]
+    line = {
+        "metric": "time per solve, Coulomb Hermite particle-cluster N=100k FP64, GPU vs the reference on one CPU core",
+        "value": gpu_s,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu_s * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Coulomb skipping, Hermite, N=100k (and 2M) uniform [-1,1]^3, theta=0.7 degree 5 leaf 2000",
+                   "n": n, "compute_type": "PC", "theta": 0.7, "degree": 5, "max_leaf": 2000},
+        "forms": res,
+        "rel_l2_gpu_vs_cpu": diff,
+        "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s", "cores": 1, "kind": kind,
+                         "sample": f"the whole Hermite treecode (particle-cluster) on the same {n} particles, one call"},
+        "speedup_vs_cpu": round(cpu_s / gpu_s, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,7 +527,7 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case"],
+    ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case", "hermite"],
                     default="large")
     ap.add_argument("--delta", type=float, default=0.5,
                     help="degree model threshold (probability that a lower degree is not enough)")
@@ -499,6 +565,9 @@ def main():
         return
     if args.workload == "cpu-case":
         bench_cpu_case(args)
+        return
+    if args.workload == "hermite":
+        bench_hermite(args)
         return
 
     import torch  # device setup and synchronisation only
