@@ -33,6 +33,9 @@ compiled for one CPU core.
 and cluster-particle, degree 5, against the reference's own sources on one
 CPU core at N = 100k, and at N = 2M on the GPU.
 
+--workload kernels: the seven kernels beside the Coulomb family at N = 1M on
+the GPU and at N = 20k against the reference's own sources on one CPU core.
+
 --workload batched measures the batching extension instead: 256 independent
 Coulomb systems (N log-uniform in [1k, 64k], 8-Gaussian mixtures, sigma 0.05,
 sources = targets), particle-cluster, theta 0.7, degree 6: the autotuned
@@ -519,6 +522,73 @@ def bench_hermite(args):
     print(json.dumps(line), flush=True)
 
 
+# the kernels beside the Coulomb family: id -> (name, kernel parameters)
+OTHER_KERNELS = {4: ("regularized_yukawa", [0.5, 0.1]), 5: ("atan", [2.0, 0.3]), 8: ("sin_over_r", [1.5]),
+                 9: ("mq", [2.0, 0.3]), 10: ("rbs_u", [0.1]), 11: ("rbs_v", [0.1]), 12: ("user", [0.5])}
+
+
+def bench_kernels(args):
+    """The kernels beside the Coulomb family, particle-cluster with skipping,
+    uniform in [-1,1]^3 (seed 8), theta 0.7, degree 5, leaf 2000, FP64: each
+    at N = 1M on the GPU against the FP64 direct sum at 1000 stride-sampled
+    targets, and at N = 20k against the reference's own sources on one CPU
+    core (oracle/_ref)."""
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from oracle import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    fn = O.ref_treecode if kind == "reference" else O.treecode
+    res = {}
+    big, small = 1_000_000, 20_000
+    xb = gen_uniform(big, 8)
+    xs = gen_uniform(small, 8)
+    idx = np.arange(0, big, big // 1000)[:1000]
+    cpu_total = 0.0
+    for k, (name, kp) in OTHER_KERNELS.items():
+        kargs = (k, kp, 1, 1, 1, 0.7, 5, 2000, 2000)
+        x, y, z, q, w = xb
+        plan = bt.Plan((x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, w), *kargs)
+        wall, gpu, near, st = time_steps(plan, bt.PRECISION_FP64, args.steps, args.warmup)
+        out = plan.execute()
+        err = rel_l2(plan.direct_sample(idx), out[idx])
+        plan.close()
+        x, y, z, q, w = xs
+        t = (x.copy(), y.copy(), z.copy(), q.copy())
+        got = bt.solve(t, (x, y, z, q, w), *kargs)
+        t0 = time.perf_counter()
+        ref = fn(*t, x, y, z, q, w, *kargs)
+        cpu_s = time.perf_counter() - t0
+        cpu_total += cpu_s
+        res[name] = {"s_n1m": round(wall, 5), "rel_l2_err_n1m": err, "cpu_s_n20k": round(cpu_s, 3),
+                     "rel_l2_gpu_vs_cpu_n20k": rel_l2(ref, got)}
+        log(f"[bench] {name}: N=1M {wall:.4f} s/step relL2 {err:.2e}; N=20k cpu {kind} {cpu_s:.2f} s, "
+            f"gpu vs cpu {res[name]['rel_l2_gpu_vs_cpu_n20k']:.2e}")
+    total = round(sum(v["s_n1m"] for v in res.values()), 5)
+    line = {
+        "metric": "time per solve of the seven kernels beside the Coulomb family, N=1M PC FP64 (sum)",
+        "value": total,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "regularized Yukawa, atan, sin-over-r, MQ, RBS-u, RBS-v, user example; "
+                               "N=1M uniform [-1,1]^3, particle-cluster theta=0.7 degree 5 leaf 2000, skipping",
+                   "n": big, "compute_type": "PC", "theta": 0.7, "degree": 5, "max_leaf": 2000},
+        "kernels": res,
+        "cpu_baseline": {"value": round(cpu_total, 3), "unit": "s", "cores": 1, "kind": kind,
+                         "sample": f"the whole treecode of each kernel on {small} particles, one call each (sum)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -527,7 +597,8 @@ def main():
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case", "hermite"],
+    ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case", "hermite",
+                                           "kernels"],
                     default="large")
     ap.add_argument("--delta", type=float, default=0.5,
                     help="degree model threshold (probability that a lower degree is not enough)")
@@ -568,6 +639,9 @@ def main():
         return
     if args.workload == "hermite":
         bench_hermite(args)
+        return
+    if args.workload == "kernels":
+        bench_kernels(args)
         return
 
     import torch  # device setup and synchronisation only
