@@ -63,6 +63,7 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->far_ptwise_variable = p.variable_far_evals();
     s->sym_ptwise = p.sym_pairs();
     s->gpu_near_sym_s = t.gpu_near_sym;
+    s->sym_slots = p.sym_slots();
 }
 
 std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)

@@ -147,6 +147,7 @@ public:
     long long near_pairs() const { return near_.pairs; }
     // pairs (ordered) the symmetric near field covers; 0 without it
     long long sym_pairs() const { return sym_ ? sym_pairs_ : 0; }
+    long long sym_slots() const { return sym_ ? sym_slots_len_ : 0; }
     long long far_pairs() const { return grid_.pairs; }
     int source_nodes() const { return nsnodes_; }
     int target_nodes() const { return ntnodes_; }

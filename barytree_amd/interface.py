@@ -77,7 +77,8 @@ class Stats(ctypes.Structure):
         (n, ctypes.c_int) for n in ("num_systems", "targets_per_thread", "persistent")] + [
         ("num_far_pairs", ctypes.c_longlong), ("variable_degree", ctypes.c_int),
         ("mean_pair_degree", ctypes.c_double), ("far_ptwise_variable", ctypes.c_longlong),
-        ("sym_ptwise", ctypes.c_longlong), ("gpu_near_sym_s", ctypes.c_double)]
+        ("sym_ptwise", ctypes.c_longlong), ("gpu_near_sym_s", ctypes.c_double),
+        ("sym_slots", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

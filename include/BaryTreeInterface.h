@@ -99,11 +99,14 @@ typedef struct BaryTreeStats {
     double mean_pair_degree; /* mean degree over far-field pairs (variable degree) */
     long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
     long long sym_ptwise;    /* near-field pairs (both directions counted) that mixed
-                                executions evaluate once for both particles: a CC
-                                system whose targets are its sources (Coulomb,
-                                Yukawa, or regularized Coulomb with skipping);
-                                0 otherwise */
+                                executions evaluate once for both particles: a
+                                single system whose targets are its sources in
+                                the same tree order (Coulomb, Yukawa, or
+                                regularized Coulomb with skipping); 0 otherwise */
     double gpu_near_sym_s;   /* the symmetric near-field launch alone (within gpu_near_s) */
+    long long sym_slots;     /* FP64 partial sums the symmetric launch writes (8 bytes
+                                of device memory each, allocated at the first
+                                mixed execution); 0 when sym_ptwise is 0 */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:

@@ -247,10 +247,13 @@ def test_symmetric_near_field_eligibility(bt):
         a = bt.analyze(same, src, k, [0.5], s, 1, 3, 0.8, 6, 800, 800)
         assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], a
         assert a["sym_ptwise"] % 2 == 0
+        # each item holds one slot per particle of both ranges: pairs = sum |A||B|
+        assert 0 < a["sym_slots"] and a["sym_slots"] ** 2 >= 2 * a["sym_ptwise"], a
     for ct in (1, 2):  # particle-cluster and cluster-particle with equal leaf sizes
         a = bt.analyze(same, src, 1, [0.5], 1, 1, ct, 0.8, 6, 800, 800)
         assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], (ct, a)
-    assert bt.analyze(same, src, 1, [0.5], 1, 2, 2, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # CP Hermite
+    herm = bt.analyze(same, src, 1, [0.5], 1, 2, 2, 0.8, 6, 800, 800)  # CP Hermite
+    assert herm["sym_ptwise"] == 0 and herm["sym_slots"] == 0
     assert bt.analyze(same, src, 1, [0.5], 1, 1, 1, 0.8, 6, 800, 100)["sym_ptwise"] == 0  # PC, other order
     assert bt.analyze(same, src, 8, [1.5], 1, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # sin-over-r
     # different trees
