@@ -170,9 +170,8 @@ __device__ __forceinline__ double herm_pc_pair(double dx, double dy, double dz, 
                (sn * (15 * r7inv - 6 * k2 * r5inv) + cs * (k3 * r4inv - 15 * k * r6inv)) * q[7] * dx * dy * dz;
     } else if constexpr (KER == HKY) {
         const double k = kp.p0, k2 = k * k, k3 = k * k2;
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const double r2 = r * r, r3 = r2 * r;
-        const double rinv = 1.0 / r;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rinv = rsqrt(r2), r = r2 * rinv, r3 = r2 * r;
         const double r3inv = rinv * rinv * rinv, r5inv = r3inv * rinv * rinv, r7inv = r5inv * rinv * rinv;
         double c[8];
 #pragma unroll
@@ -187,14 +186,12 @@ __device__ __forceinline__ double herm_pc_pair(double dx, double dy, double dz, 
         // exp(-r^2/k^2)/r weighted by -tq
         double r2 = dx * dx + dy * dy + dz * dz;
         if constexpr (KER == HKR) r2 += kp.eps2;
-        const double r = sqrt(r2);
-        const double rinv = 1.0 / r;
+        const double rinv = rsqrt(r2);
         const double r3inv = rinv * rinv * rinv, r5inv = r3inv * rinv * rinv, r7inv = r5inv * rinv * rinv;
         double v = rinv * q[0] + r3inv * (q[1] * dx + q[2] * dy + q[3] * dz) +
                    3 * r5inv * (q[4] * dx * dy + q[5] * dy * dz + q[6] * dx * dz) + 15 * r7inv * q[7] * dx * dy * dz;
         if constexpr (SS) {
-            const double rr = r * r;
-            const double rk2 = rr * kp.inv_a2, rk4 = rk2 * rk2, rk6 = rk4 * rk2;
+            const double rk2 = r2 * kp.inv_a2, rk4 = rk2 * rk2, rk6 = rk4 * rk2;
             v -= tq * exp(-rk2) *
                  (rinv * w[0] + r3inv * (1 + 2 * rk2) * (w[1] * dx + w[2] * dy + w[3] * dz) +
                   r5inv * (3 + 4 * rk2 + 4 * rk4) * (w[4] * dx * dy + w[5] * dy * dz + w[6] * dx * dz) +
@@ -304,9 +301,9 @@ __global__ __launch_bounds__(HTPB) void k_herm_cp(HermCPArgs a)
                 } else if constexpr (KER == HKY) {
                     const double k = a.kp.p0, k2 = k * k, k3 = k * k2;
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    const double r = sqrt(r2), r3 = r2 * r;
-                    const double r2inv = 1.0 / r2;
-                    const double rinvq = Lq[jj] / r * exp(-k * r);
+                    const double rinv = rsqrt(r2), r = r2 * rinv, r3 = r2 * r;
+                    const double r2inv = rinv * rinv;
+                    const double rinvq = Lq[jj] * rinv * exp(-k * r);
                     const double r3inv = rinvq * r2inv, r5inv = r3inv * r2inv, r7inv = r5inv * r2inv;
                     const double d1 = r3inv * (1 + k * r);
                     const double d2 = r5inv * (3 + 3 * k * r + k2 * r2);
@@ -318,8 +315,9 @@ __global__ __launch_bounds__(HTPB) void k_herm_cp(HermCPArgs a)
                 } else {
                     double r2 = dx * dx + dy * dy + dz * dz;
                     if constexpr (KER == HKR) r2 += a.kp.eps2;
-                    const double r2inv = 1.0 / r2;
-                    const double rinvq = Lq[jj] / sqrt(r2);
+                    const double rinv = rsqrt(r2);
+                    const double r2inv = rinv * rinv;
+                    const double rinvq = Lq[jj] * rinv;
                     const double r3inv = rinvq * r2inv, r5inv = 3.0 * (r3inv * r2inv), r7inv = r3inv * r2inv * r2inv;
                     t[0] += rinvq;
                     t[1] += r3inv * dx; t[2] += r3inv * dy; t[3] += r3inv * dz;

@@ -27,6 +27,9 @@ namespace {
 
 constexpr int HERM_TILE = 256;  // targets (or grid points) per Hermite tile
 constexpr int HERM_PLANES = 8;
+// sources per Hermite CP key: heavier keys are split (into rows of their
+// own), so that the launch has many blocks of similar work
+constexpr long long HERM_KEY_SOURCES = 1 << 14;
 }  // namespace
 
 void Plan::build_hermite_work()
@@ -82,15 +85,34 @@ void Plan::build_hermite_work()
         near_ = std::move(nn);
         make_queue_tiles(near_, tile_targets_, false, P, near_tiles_, near_qlen_);
     } else if (p.compute_type == C_CP) {
+        // grid keys split finer than the Lagrange ones; chunks of a key keep
+        // its node and follow it, so each node's keys stay consecutive
+        {
+            WorkCSR raw = std::move(grid_);
+            split_keys(raw, HERM_KEY_SOURCES, P, false, grid_);
+        }
+        node_rows_off_.assign(ntnodes_ + 1, 0);
+        for (int node : grid_.key) node_rows_off_[node + 1]++;
+        for (int n = 0; n < ntnodes_; n++) node_rows_off_[n + 1] += node_rows_off_[n];
+        make_queue_tiles(grid_, GRID_TILE, true, P, grid_tiles_, grid_qlen_);
         // grid keys write rows of 8 planes
+        grid_row_.assign(grid_.key.size(), 0);
         grid_rows_len_ = 0;
         for (size_t k = 0; k < grid_.key.size(); k++) {
             grid_row_[k] = grid_rows_len_;
             grid_rows_len_ += (long long)HERM_PLANES * grid_.tcnt[k];
         }
+        // tiles of the heaviest keys first (the order of the rows' sums is
+        // fixed by the rows, not by the tiles)
+        std::vector<long long> load(grid_.key.size(), 0);
         for (size_t k = 0; k < grid_.key.size(); k++)
+            for (int s = grid_.seg_off[k]; s < grid_.seg_off[k + 1]; s++) load[k] += grid_.seg_cnt[s];
+        std::vector<int> order(grid_.key.size());
+        for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return load[a] > load[b]; });
+        for (int k : order)
             for (int o = 0; o < grid_.tcnt[k]; o += HERM_TILE)
-                hcp_tiles_.push_back(btk::Tile{(int)k, o, std::min(HERM_TILE, grid_.tcnt[k] - o), grid_.key[k]});
+                hcp_tiles_.push_back(btk::Tile{k, o, std::min(HERM_TILE, grid_.tcnt[k] - o), grid_.key[k]});
         // downpass: tiles of each target leaf; a target sums its leaf's and
         // its ancestors' interpolants
         for (const System &sy : sys_) {

@@ -1034,14 +1034,14 @@ std::string Plan::execute(bool timed)
             if (!r.empty()) return r;
         }
     }
+    HIPCHK(hipEventRecord(ev[2], st_));
+
+    // 3. far field: Hermite PC clusters into the potential (after the near
+    // launch has stored it); grids of target clusters (CP, CC)
     if (herm_ && p.compute_type == C_PC) {
         std::string r = hermite_pc();
         if (!r.empty()) return r;
-    }
-    HIPCHK(hipEventRecord(ev[2], st_));
-
-    // 3. far field into target-cluster grids (CP, CC)
-    if (herm_ && p.compute_type == C_CP) {
+    } else if (herm_ && p.compute_type == C_CP) {
         std::string r = hermite_cp();
         if (!r.empty()) return r;
     } else if (p.compute_type != C_PC) {

@@ -55,6 +55,12 @@ sys.path.insert(0, ROOT)
 
 FLOPS_PER_EVAL = 17  # Coulomb-SS, operations as written in coulomb_ss_pp.c:40-47
 PEAK = {"fp32": 157.3, "fp64": 78.6}  # MI355X vector TFLOP/s
+# FP64 operations of one pair as the reference's formulas write them (sqrt,
+# division counted as one each): Coulomb near field, coulomb_pp.c (3
+# differences, 5 for r^2, sqrt, q/r, the sum); Hermite particle-cluster,
+# coulomb_pc.c:111-126 (42); Hermite cluster-particle, coulomb_cp.c:127-146
+# (36)
+FLOPS_PP_COULOMB, FLOPS_HERM_PC, FLOPS_HERM_CP = 11, 42, 36
 
 
 def log(*a):
@@ -79,7 +85,7 @@ def time_steps(plan, precision, steps, warmup):
     for _ in range(warmup):
         plan.execute(download=False)
     torch.cuda.synchronize()
-    times, near, sym = [], [], []
+    times, near, sym, far = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.execute(download=False)  # synchronizes on its own stream
@@ -87,9 +93,11 @@ def time_steps(plan, precision, steps, warmup):
         times.append(st["gpu_total_s"])
         near.append(st["gpu_near_s"])
         sym.append(st["gpu_near_sym_s"])
+        far.append(st["gpu_far_s"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st["gpu_near_sym_mean_s"] = float(np.mean(sym))
+    st["gpu_far_mean_s"] = float(np.mean(far))
     return wall / steps, float(np.mean(times)), float(np.mean(near)), st
 
 
@@ -484,8 +492,11 @@ def bench_hermite(args):
             out = plan.execute()
             err = rel_l2(plan.direct_sample(idx), out[idx])
             plan.close()
-            res[f"{name}_n{n}"] = {"s": round(wall, 5), "rel_l2_err": err, "out": out}
-            log(f"[bench] hermite {name} N={n}: {wall:.4f} s/step, relL2 {err:.2e}")
+            res[f"{name}_n{n}"] = {"s": round(wall, 5), "rel_l2_err": err, "out": out,
+                                   "near_s": round(near, 6), "far_s": round(st["gpu_far_mean_s"], 6),
+                                   "pp_ptwise": st["pp_ptwise"], "far_ptwise": st[f"{name}_ptwise"]}
+            log(f"[bench] hermite {name} N={n}: {wall:.4f} s/step (near launch {near:.4f} s, "
+                f"Hermite far-field launch {st['gpu_far_mean_s']:.4f} s), relL2 {err:.2e}")
     n = 100_000
     x, y, z, q, w = gen_uniform(n, 7)
     kind = "reference" if O.ref_available() else "port"
@@ -498,6 +509,21 @@ def bench_hermite(args):
     for v in res.values():
         v.pop("out")
     gpu_s = res["pc_n100000"]["s"]
+
+    def roof(name, kernel, flops, pairs, secs):
+        achieved = flops * pairs / secs / 1e12
+        return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 2), "peak": PEAK["fp64"],
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK["fp64"], 4), "traffic": None,
+                "kernel_s": round(secs, 6), "pairs": pairs, "flops_per_pair": flops, "run": name}
+
+    pc, cp = res["pc_n100000"], res["cp_n2000000"]
+    # the metric's workload spends most of its time in the FP64 near-field
+    # launch; the Hermite far-field launches follow at N = 2M
+    roofline = roof("pc_n100000", "k_interact<Coulomb, skipping, particle targets, FP64> (near field)",
+                    FLOPS_PP_COULOMB, pc["pp_ptwise"], pc["near_s"])
+    herm = [roof("pc_n2000000", "k_herm_pc<Coulomb, skipping>", FLOPS_HERM_PC,
+                 res["pc_n2000000"]["far_ptwise"], res["pc_n2000000"]["far_s"]),
+            roof("cp_n2000000", "k_herm_cp<Coulomb>", FLOPS_HERM_CP, cp["far_ptwise"], cp["far_s"])]
     line = {
         "metric": "time per solve, Coulomb Hermite particle-cluster N=100k FP64, GPU vs the reference on one CPU core",
         "value": gpu_s,
@@ -515,6 +541,8 @@ def bench_hermite(args):
                    "n": n, "compute_type": "PC", "theta": 0.7, "degree": 5, "max_leaf": 2000},
         "forms": res,
         "rel_l2_gpu_vs_cpu": diff,
+        "roofline": roofline,
+        "roofline_hermite": herm,
         "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s", "cores": 1, "kind": kind,
                          "sample": f"the whole Hermite treecode (particle-cluster) on the same {n} particles, one call"},
         "speedup_vs_cpu": round(cpu_s / gpu_s, 1),

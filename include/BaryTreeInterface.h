@@ -90,6 +90,10 @@ typedef struct BaryTreeStats {
     int max_per_source_leaf, max_per_target_leaf;
     int near_tiles, far_tiles;
     double host_copy_s, host_tree_s, host_lists_s, host_work_s, h2d_s;
+    /* GPU phases of the last execution: upward pass; near field (with the
+       Lagrange particle-cluster far field, which shares its launch); the
+       other far field (grids of target clusters, or the Hermite
+       particle-cluster launch); downpass; correction and scatter */
     double gpu_upward_s, gpu_near_s, gpu_far_s, gpu_down_s, gpu_finish_s, gpu_total_s;
     int num_systems;         /* independent systems in the plan (1 unless batched) */
     int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets */
