@@ -574,7 +574,9 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
 {
     const int toffs[2] = {0, M}, soffs[2] = {0, N};
     PlanInput in;
-    in.nsys = 1; in.toffs = toffs; in.soffs = soffs; in.tpt = 2;
+    // below SMALL_TARGETS targets, tiles of 256 targets (one per thread) give
+    // the near launch twice the blocks: about 2 per CU at 128k targets
+    in.nsys = 1; in.toffs = toffs; in.soffs = soffs; in.tpt = M < SMALL_TARGETS ? 1 : 2;
     in.tx = tx; in.ty = ty; in.tz = tz; in.tq = tq;
     in.sx = sx; in.sy = sy; in.sz = sz; in.sq = sq; in.sw = sw;
     return create(prm, in, host_only);

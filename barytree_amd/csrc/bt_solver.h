@@ -23,6 +23,7 @@ constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
 constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
 constexpr int PART_MAX = 16384;  // particles per block of a source slot's charges
 constexpr long long MAX_KEY_SOURCES = 1 << 18;  // sources per grid key (heavier keys are split)
+constexpr int SMALL_TARGETS = 1 << 18;  // single-system plans below this many targets: 1 target per thread
 
 struct Timings {
     double host_copy = 0, host_tree = 0, host_lists = 0, host_work = 0, h2d = 0;  // seconds
