@@ -61,6 +61,10 @@ __device__ __forceinline__ double rsq64(double x)
 //   user                 : a e^{-kr}/r        (particle sources: skip r <= DBL_MIN)
 // (regularized-yukawa_{pp,ss_pp,pc,ss_pc,cp}.c, atan_{pp,pc}.c,
 // sin-over-r_{pp,pc,cp}.c, mq_{pp,pc}.c, rbs-u_*.c, rbs-v_*.c, user_kernel_*.c)
+// A division by a square root is a product with rsq64 (about 1 ulp, against
+// two correctly rounded operations in the reference's form).  dz/L stays a
+// division: atan and MQ jump at d = +-1/2, where the wrap must see the
+// reference's rounding of d.
 // ---------------------------------------------------------------------
 constexpr int KRY = KERNEL_REG_YUKAWA, KAT = KERNEL_ATAN, KSR = KERNEL_SIN_OVER_R, KMQ = KERNEL_MQ,
               KRU = KERNEL_RBS_U, KRV = KERNEL_RBS_V, KUS = KERNEL_USER;
@@ -78,20 +82,20 @@ __device__ __forceinline__ double other_term(double dx, double dy, double dz, do
 {
     if constexpr (KER == KRY) {
         const double r = sqrt(r2);
-        const double e = exp(-kp.p0 * r) / sqrt(r2 + kp.eps2);
+        const double e = exp(-kp.p0 * r) * rsq64(r2 + kp.eps2);
         return (SS ? fma(-tq, b, a) : a) * e;
     } else if constexpr (KER == KAT) {
         const double d = wrap_half(dz / kp.p0);
         return a * (1.0 / M_PI * atan(kp.c1 * tan(M_PI * d)) - d);
     } else if constexpr (KER == KMQ) {
         const double d = wrap_half(dz / kp.p0);
-        return a * (0.5 * d * kp.c2 / sqrt(d * d + kp.c1) - d);
+        return a * (0.5 * d * kp.c2 * rsq64(d * d + kp.c1) - d);
     } else if constexpr (KER == KRU || KER == KRV) {
         const double r = fma(dy, dy, dx * dx) + kp.c1;
         return (KER == KRU ? 1.0 : -1.0) / (2.0 * M_PI) * a * (KER == KRU ? dy : dx) / r;
     } else {  // KSR, KUS
-        const double r = sqrt(r2);
-        double v = (KER == KSR ? sin(kp.p0 * r) : exp(-kp.p0 * r)) / r;
+        const double r = sqrt(r2), ir = rsq64(r2);
+        double v = (KER == KSR ? sin(kp.p0 * r) : exp(-kp.p0 * r)) * ir;
         if constexpr (CHECK) v = r > DBL_MIN ? v : 0.0;
         return a * v;
     }
