@@ -96,7 +96,9 @@ typedef struct BaryTreeStats {
        particle-cluster launch); downpass; correction and scatter */
     double gpu_upward_s, gpu_near_s, gpu_far_s, gpu_down_s, gpu_finish_s, gpu_total_s;
     int num_systems;         /* independent systems in the plan (1 unless batched) */
-    int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets */
+    int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets
+                                (single-system plans: 1 below 262144 targets, else 2;
+                                batched plans: the schedule's) */
     int persistent;          /* 1: persistent interaction launches */
     long long num_far_pairs; /* entries of the approximation lists */
     int variable_degree;     /* 1: far-field pairs carry their own degree */
