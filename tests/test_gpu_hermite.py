@@ -58,6 +58,25 @@ def test_hermite_large_nodes_in_parts(bt, oracle_mod):
         assert rel_l2(ref, got) <= TOL_FP64
 
 
+def test_hermite_cp_keys_split_by_sources(bt, oracle_mod):
+    """Cluster-particle keys with more than 16384 sources run as several keys
+    of the same node (rows summed in order): more grid keys than target
+    nodes, and the plain-mode bar against the oracle."""
+    print("case Hermite CP N=40000 leaf 200 degree 3", flush=True)
+    rng = np.random.default_rng(5)
+    x, y, z, q = (rng.uniform(-1, 1, 40000) for _ in range(4))
+    t, src = (x.copy(), y.copy(), z.copy(), q.copy()), (x, y, z, q, np.ones(40000))
+    args = (1, [0.5], 1, 2, 2, 0.7, 3, 200, 200)
+    a = bt.analyze(t, src, *args)
+    # one tile per key (64 grid points), at most 7 padding slots
+    assert a["far_tiles"] - 7 > a["target_nodes"], a
+    ref = oracle_mod.treecode(*t, *src, *args)
+    got = bt.solve(t, src, *args)
+    err = rel_l2(ref, got)
+    print(f"relL2={err:.3e}", flush=True)
+    assert err <= TOL_FP64
+
+
 def test_hermite_batched_equals_single_solves(bt):
     systems = []
     for i, n in enumerate([300, 2500, 40, 1800]):
