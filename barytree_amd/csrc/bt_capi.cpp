@@ -930,13 +930,17 @@ int BaryTree_AnalyzeDegrees(int numTargets, int numSources, const double *target
                                 sourceX, sourceY, sourceZ, zs.data(), nullptr, true);
     if (!e.empty()) return fail(e);
     const int n = (int)plan.far_pairs_list().size();
+    // features and degrees both hold numPairs entries: a count that differs
+    // from the plan's would overrun the caller's buffer or read past it
+    if ((features || degrees) && numPairs != n)
+        return fail("numPairs does not match the plan's far-field pairs (call with features and "
+                    "degrees NULL to learn the count)");
     if (features) {
         std::vector<float> f;
         plan.pair_features(f);
         std::memcpy(features, f.data(), f.size() * sizeof(float));
     }
     if (degrees) {
-        if (numPairs != n) return fail("numPairs does not match the plan's far-field pairs");
         e = plan.set_pair_degrees(degrees);
         if (!e.empty()) return fail(e);
     }

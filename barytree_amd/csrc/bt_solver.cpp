@@ -125,12 +125,23 @@ void Plan::free_all()
     free_var();
     for (hipEvent_t &e : sym_ev_)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    for (hipEvent_t &e : ev_)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
     if (st_) {
         (void)hipStreamDestroy(st_);
         st_ = nullptr;
     }
+}
+
+// The phase events of execute(), created at the first execution and kept
+// until the plan is freed (an execution that fails part way leaks nothing).
+std::string Plan::phase_events()
+{
+    for (hipEvent_t &e : ev_)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    return "";
 }
 
 template <class T> T *Plan::dalloc(size_t n)
@@ -991,8 +1002,9 @@ std::string Plan::execute(bool timed)
     const Params &p = prm_;
     const bool ss = p.singularity == S_SUBTRACTION;
     const int nl = p.degree + 1;
-    hipEvent_t ev[6];
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    std::string er = phase_events();
+    if (!er.empty()) return er;
+    hipEvent_t *ev = ev_;
     sym_ran_ = false;
     HIPCHK(hipEventRecord(ev[0], st_));
 
@@ -1089,7 +1101,6 @@ std::string Plan::execute(bool timed)
         if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
         tm_.gpu_near_sym = msym * 1e-3;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return "";
 }
 

@@ -173,6 +173,7 @@ private:
     std::string upward_fixed();  // upward pass at the plan's degree
     std::string down_fixed();    // downpass at the plan's degree
     bool sym_ready();  // slots allocated (at the first mixed execution)
+    std::string phase_events();
     std::string near_sym();
     std::string validate() const;
     void enumerate_pairs();
@@ -293,6 +294,7 @@ private:
     int *d_sym_red_off_ = nullptr;
     long long *d_sym_red_base_ = nullptr;
     hipEvent_t sym_ev_[2] = {nullptr, nullptr};  // around the symmetric launch
+    hipEvent_t ev_[6] = {};                      // phase boundaries of execute()
     bool sym_ran_ = false;                       // it ran in this execution
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;

@@ -590,8 +590,9 @@ std::string Plan::execute_var(bool timed)
 {
     const Params &p = prm_;
     const bool ss = p.singularity == S_SUBTRACTION;
-    hipEvent_t ev[6];
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    std::string er = phase_events();
+    if (!er.empty()) return er;
+    hipEvent_t *ev = ev_;
     sym_ran_ = false;
     HIPCHK(hipEventRecord(ev[0], st_));
 
@@ -730,7 +731,6 @@ std::string Plan::execute_var(bool timed)
         if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
         tm_.gpu_near_sym = msym * 1e-3;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     return "";
 }
 

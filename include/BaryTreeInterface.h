@@ -277,8 +277,11 @@ int BaryTreePlan_SetPairDegrees(BaryTreePlan *plan, const int *degrees);
  * would build, their features (optional, BARYTREE_PAIR_FEATURES per pair),
  * and, if degrees is not NULL (numPairs entries), the variable-degree work
  * built and checked for those degrees (stats: variable_degree,
- * mean_pair_degree, far_ptwise_variable).  numPairs may be 0 with degrees
- * NULL to learn the count (stats->num_far_pairs). */
+ * mean_pair_degree, far_ptwise_variable).  Call first with degrees and
+ * features NULL to learn the count (stats->num_far_pairs); when either is
+ * given, numPairs must equal that count (else an error, nothing written):
+ * features then holds numPairs * BARYTREE_PAIR_FEATURES floats, degrees
+ * numPairs ints. */
 int BaryTree_AnalyzeDegrees(int numTargets, int numSources, const double *targetX,
                             const double *targetY, const double *targetZ, const double *sourceX,
                             const double *sourceY, const double *sourceZ, int kernel,

@@ -233,6 +233,32 @@ def test_variable_degree_work_on_host(bt, ct, k, s):
         bt.analyze_degrees(t, src, *args, degrees=np.full(n, 13))
 
 
+def test_analyze_degrees_checks_the_pair_count_of_every_buffer(bt):
+    # a C caller that passes a features buffer on the counting call (numPairs
+    # 0) or with a stale count gets an error and an untouched buffer
+    import ctypes
+    from barytree_amd.interface import Stats, _kp, _ptr
+    t, src = uniform_cube(3000, 3000, seed=31)
+    kp = _kp([0.5])
+    args = (ctypes.c_int(3000), ctypes.c_int(3000), *[_ptr(a) for a in t[:3]], *[_ptr(a) for a in src[:3]],
+            ctypes.c_int(1), ctypes.c_int(1), _ptr(kp), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(3),
+            ctypes.c_double(0.7), ctypes.c_int(4), ctypes.c_int(50), ctypes.c_int(50), ctypes.c_double(1.0),
+            ctypes.c_double(-1.0))
+    s = Stats()
+    assert bt.lib().BaryTree_AnalyzeDegrees(*args, 0, None, None, ctypes.byref(s)) == 0
+    n = s.num_far_pairs
+    assert n > 1
+    fptr = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))  # noqa: E731
+    for count in (0, n - 1, n + 1):
+        f = np.full((n + 2) * 8, -7.0, dtype=np.float32)
+        rc = bt.lib().BaryTree_AnalyzeDegrees(*args, ctypes.c_int(count), None, fptr(f), ctypes.byref(s))
+        assert rc != 0 and "numPairs" in bt.last_error()
+        assert np.all(f == -7.0)
+    f = np.full((n + 2) * 8, -7.0, dtype=np.float32)
+    assert bt.lib().BaryTree_AnalyzeDegrees(*args, ctypes.c_int(n), None, fptr(f), ctypes.byref(s)) == 0
+    assert np.all(f[n * 8:] == -7.0) and np.all(np.isfinite(f[:n * 8]))
+
+
 def test_symmetric_near_field_eligibility(bt):
     """The symmetric near field covers nearly every near pair of a system
     (Coulomb, Yukawa, or regularized Coulomb with skipping) whose targets are
