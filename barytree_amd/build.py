@@ -16,9 +16,9 @@ LIB = os.path.join(HERE, "libBaryTree_gpu.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("BARYTREE_ARCH", "gfx950")
 
-HOST_SRCS = ["bt_host.cpp", "bt_solver.cpp", "bt_vardeg.cpp", "bt_hermite_plan.cpp", "bt_model.cpp",
+HOST_SRCS = ["bt_host.cpp", "bt_devtree.cpp", "bt_solver.cpp", "bt_vardeg.cpp", "bt_hermite_plan.cpp", "bt_model.cpp",
              "bt_capi.cpp"]
-HIP_SRCS = ["bt_kernels.hip", "bt_learn.hip", "bt_probe.hip", "bt_hermite.hip", "bt_degree.hip"]
+HIP_SRCS = ["bt_tree.hip", "bt_kernels.hip", "bt_learn.hip", "bt_probe.hip", "bt_hermite.hip", "bt_degree.hip"]
 
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function",

@@ -114,6 +114,44 @@ int Tree::add_node()
     return i;
 }
 
+bool node_split(int numpar, int maxpar, NodeRec &r, int dv[3])
+{
+    // tree_linked_list.c:53-69
+    const double xl = r.hi[0] - r.lo[0], yl = r.hi[1] - r.lo[1], zl = r.hi[2] - r.lo[2];
+    for (int d = 0; d < 3; d++) r.mid[d] = (r.hi[d] + r.lo[d]) / 2.0;
+    r.radius = std::sqrt(xl * xl + yl * yl + zl * zl) / 2.0;
+    dv[0] = dv[1] = dv[2] = 0;
+    if (!(numpar > maxpar)) return false;
+
+    // tree_linked_list.c:81-91 and partition.c:178-286
+    const int maxch = numpar < 2 * maxpar ? 2 : (numpar < 4 * maxpar ? 4 : 8);
+    double crit = xl;
+    if (crit < yl) crit = yl;
+    if (crit < zl) crit = zl;
+    crit = crit / std::sqrt(2.0);
+    dv[0] = xl >= crit; dv[1] = yl >= crit; dv[2] = zl >= crit;
+    const double len[3] = {xl, yl, zl};
+    if (maxch == 4) {
+        for (int d = 0; d < 3; d++) {
+            const double a = len[d], o1 = len[(d + 1) % 3], o2 = len[(d + 2) % 3];
+            if (a < o1 && a < o2) dv[d] = 0;
+        }
+        if (dv[0] + dv[1] + dv[2] == 3) dv[0] = 0;
+    } else if (maxch == 2) {
+        for (int d = 0; d < 3; d++) {
+            const double a = len[d], o1 = len[(d + 1) % 3], o2 = len[(d + 2) % 3];
+            if (a < o1 || a < o2) dv[d] = 0;
+        }
+        if (dv[0] + dv[1] + dv[2] == 3) { dv[0] = 0; dv[1] = 0; }
+        if (dv[0] + dv[1] + dv[2] == 2) {
+            if (dv[0] == 0) dv[1] = 0;
+            else if (dv[1] == 0) dv[2] = 0;
+            else dv[0] = 0;
+        }
+    }
+    return true;
+}
+
 namespace {
 
 // Splits [b, e) (0-based, half open) around val along key array `a`,
@@ -159,14 +197,6 @@ struct Splitter {
     }
 };
 
-// One node of a subtree under construction; child ids are local to the
-// subtree's own node vector (preorder, root = 0).
-struct NodeRec {
-    int b, e, level, nchild;
-    int child[8];
-    double lo[3], hi[3], mid[3], radius;
-};
-
 // Ranges at least this long are split or descended in parallel tasks.
 constexpr int PAR_RANGE = 1 << 17;
 
@@ -175,6 +205,10 @@ struct Builder {
     Splitter sp;
     int maxpar;
     bool leaves_only;
+    // a split left the whole range in one child: more coincident particles
+    // than maxpar, which no split separates (the reference's recursion never
+    // ends there)
+    bool stuck = false;
 
     void bounds(int b, int e, double lo[3], double hi[3]) const
     {
@@ -205,46 +239,17 @@ struct Builder {
     // the result is the serial result whatever runs in parallel.
     void build(int b, int e, int level, std::vector<NodeRec> &out, int &max_depth)
     {
-        const int numpar = e - b;
         NodeRec r{};
         r.b = b; r.e = e; r.level = level; r.nchild = 0;
         bounds(b, e, r.lo, r.hi);
-        // tree_linked_list.c:53-69
-        const double xl = r.hi[0] - r.lo[0], yl = r.hi[1] - r.lo[1], zl = r.hi[2] - r.lo[2];
-        for (int d = 0; d < 3; d++) r.mid[d] = (r.hi[d] + r.lo[d]) / 2.0;
-        r.radius = std::sqrt(xl * xl + yl * yl + zl * zl) / 2.0;
-        const bool leaf = !(numpar > maxpar);
+        int dv[3];
+        const bool leaf = !node_split(e - b, maxpar, r, dv);
+        const int numpar = e - b;
         max_depth = std::max(max_depth, level + 1);
         if (!leaves_only || leaf) out.push_back(r);
         if (leaf) return;
         const size_t self = out.size() - 1;
 
-        // tree_linked_list.c:81-91 and partition.c:178-286
-        const int maxch = numpar < 2 * maxpar ? 2 : (numpar < 4 * maxpar ? 4 : 8);
-        double crit = xl;
-        if (crit < yl) crit = yl;
-        if (crit < zl) crit = zl;
-        crit = crit / std::sqrt(2.0);
-        int dv[3] = {xl >= crit, yl >= crit, zl >= crit};
-        const double len[3] = {xl, yl, zl};
-        if (maxch == 4) {
-            for (int d = 0; d < 3; d++) {
-                const double a = len[d], o1 = len[(d + 1) % 3], o2 = len[(d + 2) % 3];
-                if (a < o1 && a < o2) dv[d] = 0;
-            }
-            if (dv[0] + dv[1] + dv[2] == 3) dv[0] = 0;
-        } else if (maxch == 2) {
-            for (int d = 0; d < 3; d++) {
-                const double a = len[d], o1 = len[(d + 1) % 3], o2 = len[(d + 2) % 3];
-                if (a < o1 || a < o2) dv[d] = 0;
-            }
-            if (dv[0] + dv[1] + dv[2] == 3) { dv[0] = 0; dv[1] = 0; }
-            if (dv[0] + dv[1] + dv[2] == 2) {
-                if (dv[0] == 0) dv[1] = 0;
-                else if (dv[1] == 0) dv[2] = 0;
-                else dv[0] = 0;
-            }
-        }
         int rb[8], re[8], nr = 1;
         rb[0] = b; re[0] = e;
         for (int d = 0; d < 3; d++) {
@@ -264,6 +269,11 @@ struct Builder {
         int nc = 0, cb[8], ce[8];
         for (int i = 0; i < nr; i++)
             if (rb[i] < re[i]) { cb[nc] = rb[i]; ce[nc] = re[i]; nc++; }
+        if (nc == 1) {
+#pragma omp atomic write
+            stuck = true;
+            return;
+        }
         for (int c = 0; c < nc; c++) {
             depth[c] = 0;
 #pragma omp task shared(sub, depth) if (ce[c] - cb[c] >= PAR_RANGE)
@@ -295,8 +305,6 @@ void levels_rec(Tree &t, int idx, int level)
 
 }  // namespace
 
-namespace {
-
 void finish_tree(Tree &t, const std::vector<NodeRec> &nodes, int max_depth, bool leaves_only)
 {
     t.max_depth = max_depth;
@@ -319,9 +327,7 @@ void finish_tree(Tree &t, const std::vector<NodeRec> &nodes, int max_depth, bool
     levels_rec(t, 0, 0);
 }
 
-}  // namespace
-
-void build_trees(const std::vector<TreeJob> &jobs)
+std::string build_trees(const std::vector<TreeJob> &jobs)
 {
     const size_t nj = jobs.size();
     std::vector<std::vector<NodeRec>> nodes(nj);
@@ -342,12 +348,15 @@ void build_trees(const std::vector<TreeJob> &jobs)
 #pragma omp taskwait
     }
     for (size_t k = 0; k < nj; k++)
+        if (blds[k].stuck) return COINCIDENT_ERROR;
+    for (size_t k = 0; k < nj; k++)
         if (jobs[k].p->n > 0) finish_tree(*jobs[k].t, nodes[k], depth[k], jobs[k].leaves_only);
+    return "";
 }
 
-void build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only)
+std::string build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only)
 {
-    build_trees({TreeJob{&t, &p, max_per_leaf, leaves_only}});
+    return build_trees({TreeJob{&t, &p, max_per_leaf, leaves_only}});
 }
 
 namespace {

@@ -87,9 +87,33 @@ struct Tree {
     int add_node();
 };
 
+// One node of a tree under construction; child ids are local to the tree's
+// own node vector (preorder, root = 0).  b and e are 0-based [b, e).
+struct NodeRec {
+    int b, e, level, nchild;
+    int child[8];
+    double lo[3], hi[3], mid[3], radius;
+};
+
+// From a node's tight box (r.lo, r.hi): its centre and radius
+// (tree_linked_list.c:53-69); if numpar > maxpar the dimensions to split, in
+// the order x, y, z (dv[d] = 1), by the child count and tie rules of
+// tree_linked_list.c:81-91 and partition.c:178-286.  Returns false for a
+// leaf (dv all 0).
+bool node_split(int numpar, int maxpar, NodeRec &r, int dv[3]);
+
+// Fills a Tree from node records in preorder (leaves_only: the records are
+// the leaves only, in DFS order).
+void finish_tree(Tree &t, const std::vector<NodeRec> &nodes, int max_depth, bool leaves_only);
+
+// A node of more coincident particles than the leaf size can never be split
+// (the reference recurses without end there); the builders report this.
+constexpr const char *COINCIDENT_ERROR =
+    "more coincident particles than the maximum leaf size: the tree cannot be split further";
+
 // Build the tree over p (permuting p).  leaves_only keeps only the leaves
-// (batches, src/tree/batches.c:25-90).
-void build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only);
+// (batches, src/tree/batches.c:25-90).  Returns "" or COINCIDENT_ERROR.
+std::string build_tree(Tree &t, Particles &p, int max_per_leaf, bool leaves_only);
 
 // Several trees at once (for example the source and target trees of a
 // cluster-cluster run), sharing one pool of OpenMP threads.
@@ -99,7 +123,7 @@ struct TreeJob {
     int max_per_leaf;
     bool leaves_only;
 };
-void build_trees(const std::vector<TreeJob> &jobs);
+std::string build_trees(const std::vector<TreeJob> &jobs);
 
 // Jagged interaction lists (src/interaction_lists/interaction_lists.c),
 // keyed by the "target" argument of InteractionLists_Make.

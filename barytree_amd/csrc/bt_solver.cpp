@@ -1,6 +1,9 @@
 // bt_solver.cpp -- plan (host setup + upload) and execute (GPU hot path).
 #include "bt_solver.h"
 
+#include "bt_devtree.h"
+#include "bt_tree.h"
+
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -150,6 +153,19 @@ template <class T> T *Plan::dalloc(size_t n)
     if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
     (alloc_var_ ? var_allocs_ : allocs_).push_back(p);
     return static_cast<T *>(p);
+}
+
+void Plan::dfree(void *p)
+{
+    if (!p) return;
+    for (auto *v : {&allocs_, &var_allocs_}) {
+        auto it = std::find(v->begin(), v->end(), p);
+        if (it != v->end()) {
+            (void)hipFree(p);
+            v->erase(it);
+            return;
+        }
+    }
 }
 
 template <class T, class A> std::string Plan::up(T **d, const std::vector<T, A> &h)
@@ -573,9 +589,12 @@ std::string Plan::validate() const
         if (c < 0 || c >= ntnodes_ || g_tparent_[c] < 0 || g_tparent_[c] >= ntnodes_) return "downward child";
     for (const btk::Tile &t : leaf_tiles_)
         if (t.x < 0 || t.x >= ntnodes_ || t.y < 0 || (long long)t.y + t.z > M_ || t.z <= 0) return "leaf tile";
-    if ((int)g_torder_.size() != M_) return "target order size";
-    for (int i = 0; i < M_; i++)
-        if (g_torder_[i] < 0 || g_torder_[i] >= M_) return "target order";
+    // (a device-built order is a stable partition of 0..M-1: a permutation)
+    if (!dev_tree_) {
+        if ((int)g_torder_.size() != M_) return "target order size";
+        for (int i = 0; i < M_; i++)
+            if (g_torder_[i] < 0 || g_torder_[i] >= M_) return "target order";
+    }
     return "";
 }
 
@@ -620,7 +639,6 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
     if (!host_only) HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
 
-    double t0 = now();
     tm_ = Timings();
     sys_.assign(in.nsys, System());
     for (int s = 0; s < in.nsys; s++) {
@@ -629,6 +647,27 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         y.M = in.toffs[s + 1] - y.toff; y.N = in.soffs[s + 1] - y.soff;
         y.msl = in.msl ? in.msl[s] : prm.max_source_leaf;
         y.mtl = in.mtl ? in.mtl[s] : prm.max_target_leaf;
+    }
+    dev_tree_ = !host_only;
+    shared_ = false;
+    if (dev_tree_) {
+        // GPU plans: the particles go to the device as they are and the
+        // trees are built there
+        if (!(e = device_trees(in)).empty()) return e;
+    } else {
+        if (!(e = host_trees(in)).empty()) return e;
+    }
+    double t2 = now();
+    return finish_create(in, host_only, t2);
+}
+
+// Host-only plans (BaryTree_Analyze): private tree-ordered copies of the
+// particles, trees built on the host.
+std::string Plan::host_trees(const PlanInput &in)
+{
+    const Params &prm = prm_;
+    double t0 = now();
+    for (System &y : sys_) {
         y.S.n = y.N; y.T.n = y.M;
         for (auto *v : {&y.S.x, &y.S.y, &y.S.z, &y.S.q, &y.S.w}) v->resize(y.N);
         for (auto *v : {&y.T.x, &y.T.y, &y.T.z, &y.T.q}) v->resize(y.M);
@@ -675,9 +714,147 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
             jobs.push_back({&y.ttree, &y.T, y.mtl, false});
         }
     }
-    build_trees(jobs);
-    double t2 = now();
-    tm_.host_tree = t2 - t1;
+    std::string e = build_trees(jobs);
+    tm_.host_tree = now() - t1;
+    return e;
+}
+
+// GPU plans: the caller's arrays go to the device unchanged, the trees are
+// built there (device_build_trees) and the charges, weights and the target
+// order are gathered into tree order.  When every system's targets are its
+// sources, bit for bit, and its leaf sizes are equal, one tree and one set
+// of arrays serve both roles (the target batches of PC and the source
+// batches of CP are the leaves of that tree).
+std::string Plan::device_trees(const PlanInput &in)
+{
+    const Params &prm = prm_;
+    const bool ss = prm.singularity == S_SUBTRACTION;
+    double t0 = now();
+    bool same = M_ == N_ && in.tq != nullptr;
+    for (int s = 0; same && s < (int)sys_.size(); s++)
+        same = in.toffs[s] == in.soffs[s] && sys_[s].msl == sys_[s].mtl;
+    if (same && !(in.tx == in.sx && in.ty == in.sy && in.tz == in.sz && in.tq == in.sq)) {
+        // bitwise equality, in parallel blocks
+        const long long n = M_;
+        const long long blk = 1 << 18, nb = (n + blk - 1) / blk;
+        int diff = 0;
+#pragma omp parallel for reduction(| : diff) schedule(static) if (nb > 1)
+        for (long long b = 0; b < nb; b++) {
+            const size_t o = (size_t)(b * blk), len = (size_t)std::min(blk, n - b * blk) * sizeof(double);
+            diff |= std::memcmp(in.tx + o, in.sx + o, len) | std::memcmp(in.ty + o, in.sy + o, len) |
+                    std::memcmp(in.tz + o, in.sz + o, len) | std::memcmp(in.tq + o, in.sq + o, len);
+        }
+        same = diff == 0;
+    }
+    shared_ = same;
+    const double t1 = now();
+    tm_.host_copy = t1 - t0;
+
+    // one particle set: coordinates uploaded, caller indices 0..n-1
+    auto upload_set = [&](DevParticles &p, int n, const double *x, const double *y, const double *z) -> std::string {
+        p.n = n;
+        p.x = dalloc<double>(n); p.y = dalloc<double>(n); p.z = dalloc<double>(n); p.idx = dalloc<int>(n);
+        p.x2 = dalloc<double>(n); p.y2 = dalloc<double>(n); p.z2 = dalloc<double>(n); p.idx2 = dalloc<int>(n);
+        if (!p.x || !p.y || !p.z || !p.idx || !p.x2 || !p.y2 || !p.z2 || !p.idx2) return "hipMalloc failed (particles)";
+        const size_t b = (size_t)n * sizeof(double);
+        HIPCHK(hipMemcpyAsync(p.x, x, b, hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(p.y, y, b, hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(p.z, z, b, hipMemcpyHostToDevice, st_));
+        HIPCHK(btk::launch_iota(n, p.idx, st_));
+        return "";
+    };
+    // the trees of one set: per system its range and leaf size
+    auto build = [&](DevParticles &p, bool src, bool leaves_only, std::vector<std::vector<NodeRec>> &recs,
+                     std::vector<int> &depth) -> std::string {
+        std::vector<DevRoot> roots;
+        for (const System &y : sys_) {
+            DevRoot r;
+            r.b = src ? y.soff : y.toff;
+            r.e = r.b + (src ? y.N : y.M);
+            r.maxpar = src ? y.msl : y.mtl;
+            roots.push_back(r);
+        }
+        return device_build_trees(p, roots, leaves_only, st_, recs, depth);
+    };
+    // after the build: the coordinates stay, the second buffers are scratch
+    auto release_scratch = [&](DevParticles &p) {
+        dfree(p.x2); dfree(p.y2); dfree(p.z2); dfree(p.idx2);
+        p.x2 = p.y2 = p.z2 = nullptr;
+        p.idx2 = nullptr;
+    };
+    std::string e;
+    const bool src_batches = prm.compute_type == C_CP, tgt_batches = prm.compute_type == C_PC;
+    DevParticles S, T;
+    if (!(e = upload_set(S, N_, in.sx, in.sy, in.sz)).empty()) return e;
+    std::vector<std::vector<NodeRec>> srec, trec;
+    std::vector<int> sdep, tdep;
+    double tb = now();
+    if (!(e = build(S, true, shared_ ? false : src_batches, srec, sdep)).empty()) return e;
+    double tree_s = now() - tb;
+    // charges and weights in tree order: the raw arrays go to the scratch
+    // buffers, then a gather
+    d_sx_ = S.x; d_sy_ = S.y; d_sz_ = S.z;
+    d_sq_ = dalloc<double>(N_);
+    if (ss) d_sw_ = dalloc<double>(N_);
+    if (!d_sq_ || (ss && !d_sw_)) return "hipMalloc failed (charges)";
+    const size_t nb = (size_t)N_ * sizeof(double);
+    HIPCHK(hipMemcpyAsync(S.x2, in.sq, nb, hipMemcpyHostToDevice, st_));
+    if (ss) HIPCHK(hipMemcpyAsync(S.y2, in.sw, nb, hipMemcpyHostToDevice, st_));
+    HIPCHK(btk::launch_tree_gather(N_, S.idx, S.x2, d_sq_, ss ? S.y2 : nullptr, 1.0, ss ? d_sw_ : nullptr, st_));
+    if (shared_) {
+        d_tx_ = d_sx_; d_ty_ = d_sy_; d_tz_ = d_sz_; d_tq_ = d_sq_;
+        d_torder_ = S.idx;
+        S.idx = nullptr;
+        HIPCHK(hipStreamSynchronize(st_));
+        release_scratch(S);
+    } else {
+        HIPCHK(hipStreamSynchronize(st_));
+        release_scratch(S);
+        dfree(S.idx);
+        if (!(e = upload_set(T, M_, in.tx, in.ty, in.tz)).empty()) return e;
+        tb = now();
+        if (!(e = build(T, false, tgt_batches, trec, tdep)).empty()) return e;
+        tree_s += now() - tb;
+        d_tx_ = T.x; d_ty_ = T.y; d_tz_ = T.z;
+        d_tq_ = dalloc<double>(M_);
+        if (!d_tq_) return "hipMalloc failed (target values)";
+        if (in.tq) {
+            HIPCHK(hipMemcpyAsync(T.x2, in.tq, (size_t)M_ * sizeof(double), hipMemcpyHostToDevice, st_));
+            HIPCHK(btk::launch_tree_gather(M_, T.idx, T.x2, d_tq_, nullptr, 0.0, nullptr, st_));
+        } else {
+            HIPCHK(hipMemsetAsync(d_tq_, 0, (size_t)M_ * sizeof(double), st_));
+        }
+        d_torder_ = T.idx;
+        T.idx = nullptr;
+        HIPCHK(hipStreamSynchronize(st_));
+        release_scratch(T);
+    }
+    // the host trees from the device records
+    double tf = now();
+    for (size_t k = 0; k < sys_.size(); k++) {
+        System &y = sys_[k];
+        if (shared_) {
+            const std::vector<NodeRec> leaves = leaf_records(srec[k]);
+            // PC: source tree + target batches; CP: target tree + source
+            // batches; CC: both trees
+            finish_tree(y.stree, src_batches ? leaves : srec[k], sdep[k], src_batches);
+            finish_tree(y.ttree, tgt_batches ? leaves : srec[k], sdep[k], tgt_batches);
+        } else {
+            finish_tree(y.stree, srec[k], sdep[k], src_batches);
+            finish_tree(y.ttree, trec[k], tdep[k], tgt_batches);
+        }
+    }
+    tree_s += now() - tf;
+    tm_.host_tree = tree_s;
+    tm_.h2d = now() - t1 - tree_s;
+    return "";
+}
+
+std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
+{
+    const Params &prm = prm_;
+    const int ns = (int)sys_.size();
+    std::string e;
 
     // interaction lists (treedriver.c:181, 462, 753)
 #pragma omp parallel for schedule(dynamic) if (ns > 1)
@@ -736,11 +913,16 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA ||
          (prm.kernel == K_REG_COULOMB && prm.singularity == S_SKIPPING))) {
         const System &y = sys_[0];
-        auto same = [&](const HostVec<double> &u, const HostVec<double> &v) {
-            return std::memcmp(u.data(), v.data(), (size_t)y.M * sizeof(double)) == 0;
-        };
-        sym_ok_ = y.M == y.N && same(y.S.x, y.T.x) && same(y.S.y, y.T.y) && same(y.S.z, y.T.z) &&
-                  same(y.S.q, y.T.q);
+        if (dev_tree_) {
+            // one tree and one set of arrays serve as sources and targets
+            sym_ok_ = shared_;
+        } else {
+            auto same = [&](const HostVec<double> &u, const HostVec<double> &v) {
+                return std::memcmp(u.data(), v.data(), (size_t)y.M * sizeof(double)) == 0;
+            };
+            sym_ok_ = y.M == y.N && same(y.S.x, y.T.x) && same(y.S.y, y.T.y) && same(y.S.z, y.T.z) &&
+                      same(y.S.q, y.T.q);
+        }
     }
     build_work();
     enumerate_pairs();
@@ -758,15 +940,17 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
             ttab.insert(ttab.end(), tab.begin(), tab.end());
         }
     }
-    g_torder_.resize(M_);
+    if (!dev_tree_) {
+        g_torder_.resize(M_);
 #pragma omp parallel for schedule(dynamic) if (ns > 1)
-    for (int s = 0; s < ns; s++) {
-        const System &y = sys_[s];
-        int *o = g_torder_.data() + y.toff;
-        const int *lo = y.T.order.data();
-        const int off = y.toff, m = y.M;
+        for (int s = 0; s < ns; s++) {
+            const System &y = sys_[s];
+            int *o = g_torder_.data() + y.toff;
+            const int *lo = y.T.order.data();
+            const int off = y.toff, m = y.M;
 #pragma omp parallel for schedule(static) if (m > 65536)
-        for (int i = 0; i < m; i++) o[i] = off + lo[i];
+            for (int i = 0; i < m; i++) o[i] = off + lo[i];
+        }
     }
     double t4 = now();
     tm_.host_work = t4 - t3;
@@ -811,28 +995,10 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     }
     if (prm.singularity != S_SUBTRACTION) kp_.ss_param = 0.0;
 
-    // upload: particles straight from each system's sorted copies
+    // the particles and the target order are on the device already
+    // (device_trees); the work lists and tables follow
     const bool ss = prm.singularity == S_SUBTRACTION;
     std::string r;
-    d_tx_ = dalloc<double>(M_); d_ty_ = dalloc<double>(M_); d_tz_ = dalloc<double>(M_);
-    d_tq_ = dalloc<double>(M_);
-    d_sx_ = dalloc<double>(N_); d_sy_ = dalloc<double>(N_); d_sz_ = dalloc<double>(N_);
-    d_sq_ = dalloc<double>(N_);
-    if (ss) d_sw_ = dalloc<double>(N_);
-    if (!d_tx_ || !d_ty_ || !d_tz_ || !d_tq_ || !d_sx_ || !d_sy_ || !d_sz_ || !d_sq_ || (ss && !d_sw_))
-        return "hipMalloc failed (particles)";
-    for (const System &y : sys_) {
-        if (!(r = up_at(d_tx_ + y.toff, y.T.x)).empty()) return r;
-        if (!(r = up_at(d_ty_ + y.toff, y.T.y)).empty()) return r;
-        if (!(r = up_at(d_tz_ + y.toff, y.T.z)).empty()) return r;
-        if (!(r = up_at(d_tq_ + y.toff, y.T.q)).empty()) return r;
-        if (!(r = up_at(d_sx_ + y.soff, y.S.x)).empty()) return r;
-        if (!(r = up_at(d_sy_ + y.soff, y.S.y)).empty()) return r;
-        if (!(r = up_at(d_sz_ + y.soff, y.S.z)).empty()) return r;
-        if (!(r = up_at(d_sq_ + y.soff, y.S.q)).empty()) return r;
-        if (ss && !(r = up_at(d_sw_ + y.soff, y.S.w)).empty()) return r;
-    }
-    if (!(r = up(&d_torder_, g_torder_)).empty()) return r;
     const size_t Pz = prm.P;
     const size_t planes = herm_ ? 8 : 1;  // Hermite: 8 planes per grid point
     if (need_src_clusters) {
@@ -900,7 +1066,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
     if (!d_pot_ || !d_out_ || !d_cursor_) return "hipMalloc failed (potential)";
     if (herm_ && !(r = upload_hermite()).empty()) return r;
     HIPCHK(hipStreamSynchronize(st_));
-    tm_.h2d = now() - t4;
+    tm_.h2d += now() - t4;
     return "";
 }
 
@@ -1118,9 +1284,11 @@ std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
         if (idx[k] < 0 || idx[k] >= M_) return "target index out of range";
     out.assign(m, 0.0);
     if (m == 0) return "";
-    // position of each caller target in the sorted arrays
-    std::vector<int> inv(M_);
-    for (int i = 0; i < M_; i++) inv[g_torder_[i]] = i;
+    if (!st_) return "the plan has no GPU work (host-only plan)";
+    // position of each caller target in the sorted arrays, on the device
+    int *dinv = dalloc<int>(M_);
+    if (!dinv) return "hipMalloc failed";
+    HIPCHK(btk::launch_invert(M_, d_torder_, dinv, st_));
     double *dw = d_sw_;
     std::string r;
     if (!dw) {
@@ -1129,21 +1297,16 @@ std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
     }
     // each target against the sources of its own system
     for (const System &y : sys_) {
-        std::vector<int> which;
+        std::vector<int> which, sel;
         for (int k = 0; k < m; k++)
-            if (idx[k] >= y.toff && idx[k] < y.toff + y.M) which.push_back(k);
+            if (idx[k] >= y.toff && idx[k] < y.toff + y.M) { which.push_back(k); sel.push_back(idx[k]); }
         if (which.empty()) continue;
         const int mm = (int)which.size();
-        std::vector<double> hx(mm), hy(mm), hz(mm), hq(mm);
-        for (int k = 0; k < mm; k++) {
-            const int i = inv[idx[which[k]]] - y.toff;  // local sorted index
-            hx[k] = y.T.x[i]; hy[k] = y.T.y[i]; hz[k] = y.T.z[i]; hq[k] = y.T.q[i];
-        }
-        double *dx = nullptr, *dy = nullptr, *dz = nullptr, *dq = nullptr;
-        if (!(r = up(&dx, hx)).empty()) return r;
-        if (!(r = up(&dy, hy)).empty()) return r;
-        if (!(r = up(&dz, hz)).empty()) return r;
-        if (!(r = up(&dq, hq)).empty()) return r;
+        int *dsel = nullptr;
+        if (!(r = up(&dsel, sel)).empty()) return r;
+        double *dx = dalloc<double>(mm), *dy = dalloc<double>(mm), *dz = dalloc<double>(mm), *dq = dalloc<double>(mm);
+        if (!dx || !dy || !dz || !dq) return "hipMalloc failed";
+        HIPCHK(btk::launch_gather_sel(mm, dsel, dinv, d_tx_, d_ty_, d_tz_, d_tq_, dx, dy, dz, dq, st_));
         double *dout = dalloc<double>(mm);
         // enough source chunks for about 2048 blocks, each chunk at least 4096 sources
         const int tblocks = (mm + 255) / 256;
@@ -1157,7 +1320,11 @@ std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
         HIPCHK(hipMemcpyAsync(res.data(), dout, (size_t)mm * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipStreamSynchronize(st_));
         for (int k = 0; k < mm; k++) out[which[k]] = res[k];
+        for (void *t : {(void *)dsel, (void *)dx, (void *)dy, (void *)dz, (void *)dq, (void *)dout, (void *)dpart})
+            dfree(t);
     }
+    dfree(dinv);
+    if (dw != d_sw_) dfree(dw);
     return "";
 }
 

@@ -162,6 +162,12 @@ public:
 
 private:
     template <class T> T *dalloc(size_t n);
+    void dfree(void *p);  // frees a dalloc buffer before the plan ends
+    // trees: on the host from private copies (host-only plans) or on the
+    // device (GPU plans); then lists, work and uploads
+    std::string host_trees(const PlanInput &in);
+    std::string device_trees(const PlanInput &in);
+    std::string finish_create(const PlanInput &in, bool host_only, double t2);
     template <class T, class A> std::string up(T **d, const std::vector<T, A> &h);
     template <class T, class A> std::string up_at(T *d, const std::vector<T, A> &h);
     void free_all();
@@ -214,6 +220,10 @@ private:
     // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
     // slots[base + i] for the bases [sym_red_off_[l], sym_red_off_[l+1]).
     bool sym_ok_ = false, sym_ = false;
+    // the trees were built on the device; one tree and one set of particle
+    // arrays serve as sources and targets (every system's targets are its
+    // sources and its leaf sizes are equal)
+    bool dev_tree_ = false, shared_ = false;
     std::vector<btk::SymItem> sym_items_;
     std::vector<double> sym_ctr_;
     std::vector<btk::Tile> sym_red_tiles_;
