@@ -9,4 +9,4 @@ Python front end (interface.py).
 from .interface import (  # noqa: F401
     LIB_PATH, PRECISION_FP64, PRECISION_MIXED, Approximation, ComputeType, Kernel, Plan,
     DegreeModel, Schedule, Singularity, analyze, analyze_batched, analyze_degrees, autotune, callTreedriver,
-    label_pairs, last_error, last_stats, lib, probe_valu, solve, solve_batched, solve_learned)
+    label_pairs, last_error, last_stats, lib, solve, solve_batched, solve_learned)

@@ -18,7 +18,10 @@ ARCH = os.environ.get("BARYTREE_ARCH", "gfx950")
 
 HOST_SRCS = ["bt_host.cpp", "bt_devtree.cpp", "bt_solver.cpp", "bt_vardeg.cpp", "bt_hermite_plan.cpp", "bt_model.cpp",
              "bt_capi.cpp"]
-HIP_SRCS = ["bt_tree.hip", "bt_kernels.hip", "bt_learn.hip", "bt_probe.hip", "bt_hermite.hip", "bt_degree.hip"]
+HIP_SRCS = ["bt_tree.hip", "bt_kernels.hip", "bt_learn.hip", "bt_hermite.hip", "bt_degree.hip"]
+# measurement tooling for the benchmark, in a library of its own
+PROBE_SRC = "bt_probe.hip"
+PROBE_LIB = os.path.join(HERE, "libbt_probe.so")
 
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-function",
@@ -56,8 +59,11 @@ def build(verbose=False):
     run(["g++", "-shared", "-fPIC", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64", "-fopenmp",
          "-Wl,-rpath," + f"{ROCM}/lib"])
     os.replace(tmp, LIB)
+    tmp = PROBE_LIB + ".tmp"
+    run([f"{ROCM}/bin/hipcc", *HIP_FLAGS, "-shared", os.path.join(CSRC, PROBE_SRC), "-o", tmp])
+    os.replace(tmp, PROBE_LIB)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", PROBE_LIB)
     return LIB
 
 

@@ -423,14 +423,6 @@ int BaryTree_Solve(int numTargets, int numSources, const double *targetX, const 
 
 const char *BaryTree_GetLastError(void) { return g_last_error.c_str(); }
 
-int BaryTree_ProbeVALU(double *nsPerInstruction, int n)
-{
-    if (!nsPerInstruction || n <= 0) return fail("ProbeVALU: no output");
-    hipError_t e = btk::probe_valu_issue(nsPerInstruction, std::min(n, btk::PROBE_KINDS));
-    if (e != hipSuccess) return fail(std::string("ProbeVALU: ") + hipGetErrorString(e));
-    return std::min(n, btk::PROBE_KINDS);
-}
-
 int BaryTree_GetLastStats(BaryTreeStats *stats)
 {
     if (!stats || !g_have_stats) return -1;

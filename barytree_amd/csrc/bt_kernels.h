@@ -184,12 +184,6 @@ struct HermDownArgs {
 };
 hipError_t launch_herm_down(const HermDownArgs &a, hipStream_t st);
 
-// Issue cost of VALU instruction kinds on the current GPU (bt_probe.hip):
-// ns per wave instruction and SIMD of FP64 fma, FP64 add, FP32 fma, FP32 mul,
-// FP32 exp2, FP32 rsq and FP64<->FP32 conversion, in that order.
-constexpr int PROBE_KINDS = 7;
-hipError_t probe_valu_issue(double *ns_per_instr, int n);
-
 // Source grid point coordinates from node tables: c{x,y,z}[node*P + j].
 hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double *cx, double *cy,
                                  double *cz, hipStream_t st);

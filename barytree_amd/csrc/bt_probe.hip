@@ -7,11 +7,17 @@
 // Multiplying a kernel's per-kind instruction counts (SQ_INSTS_VALU_*
 // counters) by these costs gives the time the VALU needs to issue them:
 // the attainable ceiling of that kernel's instruction mix.
+//
+// Measurement tooling for the benchmark: built into its own library
+// (libbt_probe.so), not into the drop-in libBaryTree_gpu.so.
 #include <hip/hip_runtime.h>
 
-#include "bt_kernels.h"
-
 namespace btk {
+
+// FP64 fma, FP64 add, FP32 fma, FP32 mul, FP32 exp2, FP32 rsq and
+// FP64<->FP32 conversion, in that order
+constexpr int PROBE_KINDS = 7;
+
 namespace {
 
 constexpr int PROBE_ITER = 2048;
@@ -107,3 +113,12 @@ hipError_t probe_valu_issue(double *ns_per_instr, int n)
 }
 
 }  // namespace btk
+
+// ns per wave instruction and SIMD of each kind on the current GPU; returns
+// the number of kinds filled, or -1 on a HIP error.
+extern "C" int bt_probe_valu(double *ns_per_instr, int n)
+{
+    if (!ns_per_instr || n <= 0) return -1;
+    const int k = n < btk::PROBE_KINDS ? n : btk::PROBE_KINDS;
+    return btk::probe_valu_issue(ns_per_instr, k) == hipSuccess ? k : -1;
+}

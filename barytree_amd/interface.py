@@ -133,7 +133,7 @@ def lib():
                   "BaryTree_LabelPairs", "BaryTreeModel_Create", "BaryTreeModel_Load",
                   "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
                   "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
-                  "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees", "BaryTree_ProbeVALU"):
+                  "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
@@ -596,18 +596,6 @@ def solve_learned(targets, sources, kernel, kernel_params, singularity, approxim
     if rc != 0:
         raise RuntimeError(f"BaryTree_SolveLearned failed: {last_error()}")
     return out
-
-
-VALU_KINDS = ("fma_f64", "add_f64", "fma_f32", "mul_f32", "exp_f32", "rsq_f32", "cvt_f64_f32")
-
-
-def probe_valu():
-    """BaryTree_ProbeVALU: {kind: ns per wave instruction and SIMD} on the GPU."""
-    out = np.zeros(len(VALU_KINDS))
-    n = lib().BaryTree_ProbeVALU(_ptr(out), ctypes.c_int(len(out)))
-    if n != len(out):
-        raise RuntimeError(f"BaryTree_ProbeVALU failed: {last_error()}")
-    return dict(zip(VALU_KINDS, out.tolist()))
 
 
 def callTreedriver(numTargets, numSources,

@@ -133,7 +133,7 @@ def near_valu_mix():
     return None
 
 
-# SQ_INSTS_VALU_* counter -> issue-cost probe kind (BaryTree_ProbeVALU)
+# SQ_INSTS_VALU_* counter -> issue-cost probe kind (barytree_amd.probe)
 MIX_COST = {"SQ_INSTS_VALU_FMA_F64": "fma_f64", "SQ_INSTS_VALU_ADD_F64": "add_f64",
             "SQ_INSTS_VALU_MUL_F64": "fma_f64", "SQ_INSTS_VALU_FMA_F32": "fma_f32",
             "SQ_INSTS_VALU_ADD_F32": "mul_f32", "SQ_INSTS_VALU_MUL_F32": "mul_f32",
@@ -637,8 +637,8 @@ def main():
                     help="only print the measured issue cost of the VALU instruction kinds")
     args = ap.parse_args()
     if args.valu_probe:
-        import barytree_amd as bt
-        print(json.dumps({"valu_ns_per_wave_instruction_per_simd": bt.probe_valu()}))
+        from barytree_amd.probe import probe_valu
+        print(json.dumps({"valu_ns_per_wave_instruction_per_simd": probe_valu()}))
         return
     if args.calibrate_copy:
         import torch
@@ -742,7 +742,8 @@ def main():
     # the VALU issue time of the kernel's instruction mix (kinds priced alone)
     mix = near_valu_mix() if n == 16_000_000 else None
     if mix:
-        probe = bt.probe_valu()
+        from barytree_amd.probe import probe_valu
+        probe = probe_valu()
         simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
         ceil_s = issue_model(mix, probe, simds)
         roofline["valu_issue_model_s"] = round(ceil_s, 4)

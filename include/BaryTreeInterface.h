@@ -55,12 +55,22 @@ typedef enum COMPUTE_TYPE {
 /* Reference entry point (src/interface/BaryTreeInterface.h:52-59,
  * src/interface/BaryTreeInterface.c:18-62): zeroes outputArray, then adds the
  * treecode potential at every target, in the caller's target order.
- * Supported: LAGRANGE; COULOMB, YUKAWA, REGULARIZED_COULOMB; SKIPPING and
- * SUBTRACTION; PARTICLE_CLUSTER, CLUSTER_PARTICLE, CLUSTER_CLUSTER, in the
- * combinations the reference dispatchers accept; degree 1..12.  Where the
- * reference prints an error and calls exit(1), this prints the error (with
- * the "[BaryTree]" prefix), leaves outputArray zero and records the message
- * for BaryTree_GetLastError(); the process keeps running. */
+ * Supported, in the combinations the reference dispatchers accept
+ * (src/interaction_compute/interaction_compute_{pc,cp,cc}.c):
+ *   LAGRANGE: COULOMB, YUKAWA, REGULARIZED_COULOMB with SKIPPING or
+ *     SUBTRACTION in PARTICLE_CLUSTER, CLUSTER_PARTICLE, CLUSTER_CLUSTER
+ *     (REGULARIZED_COULOMB SUBTRACTION in PARTICLE_CLUSTER only);
+ *     REGULARIZED_YUKAWA, SIN_OVER_R, RBS_U, RBS_V, USER in all three
+ *     compute types with SKIPPING (REGULARIZED_YUKAWA also PARTICLE_CLUSTER
+ *     SUBTRACTION); ATAN and MQ in PARTICLE_CLUSTER with SKIPPING;
+ *   HERMITE: COULOMB, YUKAWA, REGULARIZED_COULOMB in PARTICLE_CLUSTER (either
+ *     singularity) and CLUSTER_PARTICLE (SKIPPING); SIN_OVER_R in
+ *     PARTICLE_CLUSTER and CLUSTER_PARTICLE with SKIPPING; no CLUSTER_CLUSTER;
+ * degree 1..12.  Where the reference prints an error and calls exit(1), this
+ * prints the error (with the "[BaryTree]" prefix), leaves outputArray zero
+ * and records the message for BaryTree_GetLastError(); the process keeps
+ * running.  More coincident particles than a leaf size are refused the same
+ * way (the reference's tree recursion does not end there). */
 void BaryTreeInterface(int numTargets, int numSources,
                        double *targetX, double *targetY, double *targetZ, double *targetValue,
                        double *sourceX, double *sourceY, double *sourceZ, double *sourceValue,
@@ -361,15 +371,6 @@ int BaryTree_Autotune(int numSystems, const int *targetOffsets, const int *sourc
                       int compute_type, double theta, int interpOrder, double sizeCheck,
                       double beta, int precision, BaryTreeSchedule *schedule);
 
-/* Diagnostics (no counterpart in the reference).  Issue cost of the VALU
- * instruction kinds the pair kernels are made of, measured on the current
- * GPU: ns per wave instruction and SIMD of FP64 fma, FP64 add, FP32 fma,
- * FP32 mul, FP32 exp2, FP32 rsq and FP64<->FP32 conversion, in that order
- * (up to n of them), each kind measured alone.  Times a kernel's per-kind
- * instruction counts, this models the time the VALU needs to issue that
- * kernel (kinds issued side by side may overlap, so a kernel can beat it).  Returns the number filled,
- * or a negative value. */
-int BaryTree_ProbeVALU(double *nsPerInstruction, int n);
 
 #ifdef __cplusplus
 }

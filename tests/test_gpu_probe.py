@@ -1,5 +1,5 @@
-"""GPU test of the VALU issue-cost probe (BaryTree_ProbeVALU), which prices
-the near-field kernel's instruction mix in bench.py."""
+"""GPU test of the VALU issue-cost probe (libbt_probe.so, barytree_amd.probe),
+which prices the near-field kernel's instruction mix in bench.py."""
 import math
 
 import pytest
@@ -8,9 +8,9 @@ pytestmark = pytest.mark.gpu
 
 
 def test_probe_valu_costs_are_sane():
-    import barytree_amd as bt
-    p = bt.probe_valu()
-    assert set(p) == set(bt.interface.VALU_KINDS)
+    from barytree_amd import probe
+    p = probe.probe_valu()
+    assert set(p) == set(probe.VALU_KINDS)
     for k, v in p.items():
         # a wave instruction takes between 1 and 64 cycles at 0.5-3 GHz
         assert math.isfinite(v) and 0.3 < v < 130.0, (k, v)
