@@ -1,4 +1,6 @@
 """Shared input generators and error metrics for the tests."""
+import ctypes
+
 import numpy as np
 
 KP = {  # kernel -> singularity -> parameters
@@ -67,3 +69,48 @@ def other_combos():
     ATAN and MQ in PC only; subtraction for regularized Yukawa in PC only."""
     out = [(ct, k, 1) for k in (4, 8, 10, 11, 12) for ct in (1, 2, 3)]
     return out + [(1, 5, 1), (1, 9, 1), (1, 4, 2)]
+
+
+def glibc_rand_particles(n, with_targets=False):
+    """srand(1); rand()/RAND_MAX draws as in tests/serial_tests.c:580-588 and
+    :1452-1465 (glibc's rand, so the same stream as the reference test)."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    rmax = 2147483647.0
+
+    def r():
+        return libc.rand() / rmax
+
+    if with_targets:
+        t = [np.zeros(n) for _ in range(4)]
+        s = [np.zeros(n) for _ in range(5)]
+        for i in range(n):
+            for a in t:
+                a[i] = r() * 2.0 - 1.0
+            for a in s[:4]:
+                a[i] = r() * 2.0 - 1.0
+            s[4][i] = r()
+        return t, s
+    s = [np.zeros(n) for _ in range(5)]
+    for i in range(n):
+        for a in s[:4]:
+            a[i] = r() * 2.0 - 1.0
+        s[4][i] = r()
+    return s
+
+
+# The reference's treecode tests (tests/serial_tests.c): approximation
+# (1 Lagrange, 2 Hermite) x kernel (1 Coulomb, 2 Yukawa) x singularity
+# (1 skipping, 2 subtraction), kernel parameter 0.5.
+SCENARIO_COMBOS = [(ap, k, s) for ap in (1, 2) for k in (1, 2) for s in (1, 2)]
+# serial_tests.c:536-840: PC, theta 0.8, degree 4, leaves 100/100, size_check 0
+ONE_TARGET_ARGS = (1, 0.8, 4, 100, 100, 0.0)
+ONE_TARGET_TOL = {1: 2e-4, 2: 2e-5}
+# serial_tests.c:965-1060: (theta, degree) of the three parameter sets
+MONOTONE_SETS = [(0.9, 3), (0.9, 6), (0.4, 3)]
+
+
+def one_target():
+    """The single target of serial_tests.c:571-577: the origin, charge 1."""
+    return [np.zeros(1), np.zeros(1), np.zeros(1), np.ones(1)]

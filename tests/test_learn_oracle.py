@@ -34,6 +34,26 @@ def test_vardeg_uniform_degree_equals_fixed(oracle_mod, ct, k, s):
     assert rel_l2(ref, got) <= 1e-13
 
 
+@pytest.mark.parametrize("p", [2, 6])
+@pytest.mark.parametrize("ct,k,s", supported_combos())
+def test_vardeg_uniform_other_degree_equals_fixed_at_that_degree(oracle_mod, ct, k, s, p):
+    """Every far-field pair at degree p != the plan's degree 4 equals the
+    fixed-degree oracle run at degree p.  size_check 0 makes the
+    interaction lists independent of the degree (interaction_lists.c:484-486,
+    557), so both runs see the same pairs; a per-pair degree that the
+    variable-degree path dropped, or a grid it built at the plan's degree,
+    would show here."""
+    t, src = uniform_cube(1500, 1200, seed=60 + 10 * ct + k)
+    args = (k, KP[k][s], s, 1, ct, 0.7)
+    ref = oracle_mod.treecode(*t, *src, *args, p, 40, 40, 0.0)
+    npairs_p = n_far_pairs(oracle_mod.last_stats(), ct)
+    oracle_mod.treecode(*t, *src, *args, 4, 40, 40, 0.0)
+    npairs = n_far_pairs(oracle_mod.last_stats(), ct)
+    assert npairs == npairs_p and npairs > 0
+    got = oracle_mod.treecode_vardeg(*t, *src, *args, 4, 40, 40, np.full(npairs, p), 0.0)
+    assert rel_l2(ref, got) <= 1e-13
+
+
 @pytest.mark.parametrize("ct", [1, 2, 3])
 def test_vardeg_higher_degree_lowers_error(oracle_mod, ct):
     t, src = uniform_cube(2000, 2000, seed=7 + ct)

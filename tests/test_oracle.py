@@ -7,42 +7,13 @@ the known-answer values of interfaces/python/testBaryTreeInterface.py, the
 reference's own compiled sources (oracle/_ref, when built), and invariants
 that a dropped term, a wrong sign or a transposed index would break.
 """
-import ctypes
 import math
 
 import numpy as np
 import pytest
 
-from tests.helpers import KP, kat_inputs, rel_l2, supported_combos, uniform_cube
-
-
-def glibc_rand_particles(n, with_targets=False):
-    """srand(1); rand()/RAND_MAX draws as in tests/serial_tests.c:580-588 and
-    :1452-1465 (glibc's rand, so the same stream as the reference test)."""
-    libc = ctypes.CDLL("libc.so.6")
-    libc.rand.restype = ctypes.c_int
-    libc.srand(1)
-    rmax = 2147483647.0
-
-    def r():
-        return libc.rand() / rmax
-
-    if with_targets:
-        t = [np.zeros(n) for _ in range(4)]
-        s = [np.zeros(n) for _ in range(5)]
-        for i in range(n):
-            for a in t:
-                a[i] = r() * 2.0 - 1.0
-            for a in s[:4]:
-                a[i] = r() * 2.0 - 1.0
-            s[4][i] = r()
-        return t, s
-    s = [np.zeros(n) for _ in range(5)]
-    for i in range(n):
-        for a in s[:4]:
-            a[i] = r() * 2.0 - 1.0
-        s[4][i] = r()
-    return s
+from tests.helpers import (KP, MONOTONE_SETS, ONE_TARGET_ARGS, ONE_TARGET_TOL, SCENARIO_COMBOS,
+                           glibc_rand_particles, one_target, kat_inputs, rel_l2, supported_combos, uniform_cube)
 
 
 # ---------------------------------------------------------------- closed forms
@@ -181,29 +152,30 @@ def test_caller_order_restored(oracle_mod):
 
 
 # ------------------------------------------------- treecode vs direct (reference)
-@pytest.mark.parametrize("k,s", [(1, 1), (1, 2), (2, 1), (2, 2)])
-def test_treecode_vs_direct_one_target(oracle_mod, k, s):
+@pytest.mark.parametrize("ap,k,s", SCENARIO_COMBOS)
+def test_treecode_vs_direct_one_target(oracle_mod, ap, k, s):
     """tests/serial_tests.c:536-840: one target at the origin, 10k sources
     from srand(1), PC, theta 0.8, degree 4, leaves 100/100, size_check 0;
-    Lagrange relative error < 2e-4."""
+    relative error < 2e-4 (Lagrange) or < 2e-5 (Hermite)."""
     src = glibc_rand_particles(10000)
-    t = [np.zeros(1), np.zeros(1), np.zeros(1), np.ones(1)]
-    tree = oracle_mod.treecode(*t, *src, k, [0.5], s, 1, 1, 0.8, 4, 100, 100, 0.0)
+    t = one_target()
+    tree = oracle_mod.treecode(*t, *src, k, [0.5], s, ap, *ONE_TARGET_ARGS)
     d = oracle_mod.direct(*t, *src, k, [0.5], s)
-    assert abs(tree[0] - d[0]) / abs(d[0]) < 2e-4
+    assert abs(tree[0] - d[0]) / abs(d[0]) < ONE_TARGET_TOL[ap]
 
 
-@pytest.mark.parametrize("k,s", [(1, 1), (1, 2), (2, 1), (2, 2)])
-def test_accuracy_improves_with_degree_and_theta(oracle_mod, k, s):
+@pytest.mark.parametrize("ap,k,s", SCENARIO_COMBOS)
+def test_accuracy_improves_with_degree_and_theta(oracle_mod, ap, k, s):
     """tests/serial_tests.c:965-1411: degree 3 -> 6 and theta 0.9 -> 0.4 each
-    reduce the error (1 target, 5000 sources, leaves 50/5, size_check 0)."""
+    reduce the error (1 target, 5000 sources, leaves 50/5, size_check 0), for
+    all eight Lagrange/Hermite x Coulomb/Yukawa x skipping/subtraction
+    cases."""
     src = glibc_rand_particles(5000)
-    t = [np.zeros(1), np.zeros(1), np.zeros(1), np.ones(1)]
+    t = one_target()
     d = oracle_mod.direct(*t, *src, k, [0.5], s)[0]
-    e1 = abs(oracle_mod.treecode(*t, *src, k, [0.5], s, 1, 1, 0.9, 3, 50, 5, 0.0)[0] - d)
-    e2 = abs(oracle_mod.treecode(*t, *src, k, [0.5], s, 1, 1, 0.9, 6, 50, 5, 0.0)[0] - d)
-    e3 = abs(oracle_mod.treecode(*t, *src, k, [0.5], s, 1, 1, 0.4, 3, 50, 5, 0.0)[0] - d)
-    assert e2 < e1 and e3 < e1
+    e = [abs(oracle_mod.treecode(*t, *src, k, [0.5], s, ap, 1, th, deg, 50, 5, 0.0)[0] - d)
+         for th, deg in MONOTONE_SETS]
+    assert e[1] < e[0] and e[2] < e[0]
 
 
 def test_cluster_cluster_summed_potential(oracle_mod):
