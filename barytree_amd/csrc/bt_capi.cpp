@@ -477,6 +477,15 @@ int BaryTreePlan_SetPrecision(BaryTreePlan *plan, int precision)
     return 0;
 }
 
+int BaryTreePlan_SetSymmetric(BaryTreePlan *plan, int enable)
+{
+    g_last_error.clear();
+    if (!plan) return fail("plan is NULL");
+    if (enable != 0 && enable != 1) return fail("enable must be 0 or 1");
+    plan->plan.set_symmetric(enable != 0);
+    return 0;
+}
+
 int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats)
 {
     if (!plan || !stats) return fail("NULL argument");

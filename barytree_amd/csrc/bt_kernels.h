@@ -108,6 +108,7 @@ struct SymItem {
 };
 struct SymArgs {
     int kernel, singularity, nitems;
+    int fp64;  // 1: FP64 arithmetic (plain executions), 0: mixed precision
     const SymItem *items;
     const double4 *ctr;
     const double *x, *y, *z, *q, *w;  // the particles, tree order (w: subtraction only)

@@ -845,6 +845,156 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
     }
 }
 
+// ---------------------------------------------------------------------
+// Symmetric near field in FP64 (plain executions).  The item layout, lane
+// groups and slots are those of k_near_sym; the arithmetic is that of
+// pair_pot, evaluated once per unordered pair for both particles: the
+// displacement of the reverse pair is the negated one, so r^2, the rsq64
+// estimate and the exponential are the same numbers in both directions,
+// and each direction's term is formed as pair_pot forms it with its own
+// target charge.  Only the order of the sums differs from the
+// one-directional launch.  Sums stay FP64 throughout: a target's over all of
+// B in registers, a source's over its group's lanes by a DPP row_shr scan
+// of the two 32-bit halves, then over the waves in LDS.  CHECK (items whose
+// boxes touch) applies the reference's skip of coincident pairs.
+// ---------------------------------------------------------------------
+constexpr int SYM64_TPT = 4, SYM64_G = 8, SYM64_NW = 4;
+
+template <int G>
+__device__ __forceinline__ double group_sum64(double v)
+{
+    auto shr = [](double x, auto ctl_tag) {
+        constexpr int CTL = decltype(ctl_tag)::value;
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTL, 0xf, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTL, 0xf, 0xf, true);
+        return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    };
+    v += shr(v, std::integral_constant<int, 0x111>{});
+    if constexpr (G > 2) v += shr(v, std::integral_constant<int, 0x112>{});
+    if constexpr (G > 4) v += shr(v, std::integral_constant<int, 0x114>{});
+    if constexpr (G > 8) v += shr(v, std::integral_constant<int, 0x118>{});
+    return v;
+}
+
+template <int KER, bool SS, int TPT, int G, int NW>
+__global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
+{
+    constexpr int NT = 64 * NW;
+    __shared__ double4 Ls[NT];          // x, y, z and the source's a (q w or q)
+    __shared__ double2 Lw[SS ? NT : 1]; // the source's w and its charge as a target
+    __shared__ double Lc[NW][NT];       // per wave: the sources' sums
+    const SymItem it = A.items[blockIdx.x];
+    const bool check = A.ctr[blockIdx.x].w < 0.0;
+    constexpr int NG = 64 / G;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane % G, cg = lane / G;
+    constexpr int TPASS = NW * G * TPT;
+    const KParams kp = A.kp;
+
+    for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
+        // targets: coordinates, -q (subtraction), a = q w (q), w; acc
+        double Tx[TPT], Ty[TPT], Tz[TPT], Tnq[TPT], Ta[TPT], Tw[TPT], acc[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            const int li = p0 + wv * G * TPT + rg + G * k;
+            const bool v = li < it.a_cnt;
+            const int g = it.a_beg + (v ? li : 0);
+            Tx[k] = A.x[g]; Ty[k] = A.y[g]; Tz[k] = A.z[g];
+            const double q = A.q[g];
+            const double w = SS ? A.w[g] : 1.0;
+            Tnq[k] = -q;
+            // a target outside A gives nothing to the sources
+            Ta[k] = v ? (SS ? q * w : q) : 0.0;
+            Tw[k] = v ? w : 0.0;
+            acc[k] = 0.0;
+        }
+        for (int off = 0; off < it.b_cnt; off += NT) {
+            const int n = min(NT, it.b_cnt - off);
+            __syncthreads();  // the previous chunk's LDS has been read
+            if (tid < n) {
+                const int g = it.b_beg + off + tid;
+                const double q = A.q[g];
+                if constexpr (SS) {
+                    const double w = A.w[g];
+                    Ls[tid] = make_double4(A.x[g], A.y[g], A.z[g], q * w);
+                    Lw[tid] = make_double2(w, q);
+                } else {
+                    Ls[tid] = make_double4(A.x[g], A.y[g], A.z[g], q);
+                }
+            } else {
+                // dummies: zero charge and weight, far away (their column
+                // sums are never stored)
+                Ls[tid] = make_double4(1.0e100, 1.0e100, 1.0e100, 0.0);
+                if constexpr (SS) Lw[tid] = make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            const int steps = (n + NG - 1) / NG;
+            auto run = [&](auto check_tag) {
+                constexpr bool CHECK = decltype(check_tag)::value;
+                for (int st = 0; st < steps; st++) {
+                    const int j = NG * st + cg;
+                    const double4 S = Ls[j];
+                    double sw = 1.0, sq = 0.0;
+                    if constexpr (SS) { const double2 u = Lw[j]; sw = u.x; sq = u.y; }
+                    double c = 0.0;
+#pragma unroll
+                    for (int k = 0; k < TPT; k++) {
+                        const double dx = Tx[k] - S.x, dy = Ty[k] - S.y, dz = Tz[k] - S.z;
+                        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                        if constexpr (KER == KC) {
+                            double ir = rsq64(r2);
+                            if constexpr (CHECK) ir = r2 > (SS ? 0.0 : DBL_MIN) ? ir : 0.0;
+                            if constexpr (!SS) {
+                                acc[k] = fma(S.w, ir, acc[k]);
+                                c = fma(Ta[k], ir, c);
+                            } else {
+                                const double e = exp(-r2 * kp.inv_a2);
+                                acc[k] = fma(fma(Tnq[k] * sw, e, S.w), ir, acc[k]);
+                                c = fma(fma(-sq * Tw[k], e, Ta[k]), ir, c);
+                            }
+                        } else if constexpr (KER == KY) {
+                            const double ir = rsq64(r2);
+                            double e = exp(-kp.p0 * (r2 * ir)) * ir;
+                            if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
+                            if constexpr (!SS) {
+                                acc[k] = fma(S.w, e, acc[k]);
+                                c = fma(Ta[k], e, c);
+                            } else {
+                                acc[k] = fma(fma(Tnq[k], sw, S.w), e, acc[k]);
+                                c = fma(fma(-sq, Tw[k], Ta[k]), e, c);
+                            }
+                        } else {  // regularized Coulomb, skipping
+                            const double ir = rsq64(r2 + kp.eps2);
+                            acc[k] = fma(S.w, ir, acc[k]);
+                            c = fma(Ta[k], ir, c);
+                        }
+                    }
+                    c = group_sum64<G>(c);
+                    if (rg == G - 1) Lc[wv][j] = c;
+                }
+            };
+            if (check && KER != KR) run(std::integral_constant<bool, true>{});
+            else run(std::integral_constant<bool, false>{});
+            __syncthreads();
+            if (tid < n) {
+                double s = 0.0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) s += Lc[w][tid];
+                double *slot = A.slots + it.cslot + off + tid;
+                *slot = p0 == 0 ? s : *slot + s;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            double r = acc[k];
+#pragma unroll
+            for (int o = G; o < 64; o *= 2) r += __shfl_xor(r, o);
+            const int li = p0 + wv * G * TPT + rg + G * k;
+            if (cg == 0 && li < it.a_cnt) A.slots[it.rslot + li] = r;
+        }
+    }
+}
+
 __global__ __launch_bounds__(TPB_SYM_REDUCE) void k_sym_reduce(SymReduceArgs A)
 {
     const Tile t = A.tiles[blockIdx.x];
@@ -1635,6 +1785,17 @@ hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
 {
     if (a.nitems <= 0) return hipSuccess;
     const bool ss = a.singularity == 2;
+    if (a.fp64) {
+        switch (a.kernel) {
+#define L(K, S) k_near_sym64<K, S, SYM64_TPT, SYM64_G, SYM64_NW><<<dim3(a.nitems), dim3(64 * SYM64_NW), 0, st>>>(a)
+            case KC: if (ss) L(KC, true); else L(KC, false); break;
+            case KY: if (ss) L(KY, true); else L(KY, false); break;
+            case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;
+#undef L
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.kernel) {
 #define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G, SYM_NW><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
         case KC: if (ss) L(KC, true); else L(KC, false); break;

@@ -1119,10 +1119,10 @@ std::string Plan::down_fixed()
 // the potential (after the near launch has stored every target's sum).
 bool Plan::sym_ready()
 {
-    if (!sym_) return false;
+    if (!sym_ || !sym_enabled_) return false;
     if (d_sym_slots_) return true;
-    // the slots at the first mixed execution; without the memory for them
-    // the plan keeps the one-directional near field
+    // the slots at the first execution that uses them; without the memory
+    // for them the plan keeps the one-directional near field
     if (hipEventCreate(&sym_ev_[0]) != hipSuccess || hipEventCreate(&sym_ev_[1]) != hipSuccess) {
         sym_ = false;
         return false;
@@ -1141,6 +1141,7 @@ std::string Plan::near_sym()
     btk::SymArgs a{};
     a.kernel = prm_.kernel;
     a.singularity = prm_.singularity;
+    a.fp64 = prm_.precision == PREC_MIXED ? 0 : 1;
     a.nitems = (int)sym_items_.size();
     a.items = d_sym_items_;
     a.ctr = reinterpret_cast<const double4 *>(d_sym_ctr_);
@@ -1202,10 +1203,8 @@ std::string Plan::execute(bool timed)
         a.grid_targets = 0; a.ntiles = (int)near_tiles_.size(); a.tiles = d_near_tiles_;
         a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_nclu_off_; a.clu_id = d_nclu_id_;
         a.out1 = d_pot_;
-        if (p.precision == PREC_MIXED) {
-            a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.skip_sym = sym_ready() ? 1 : 0;
-        }
+        if (p.precision == PREC_MIXED) a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
+        a.skip_sym = sym_ready() ? 1 : 0;
         std::copy(near_qlen_, near_qlen_ + NXCD, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));

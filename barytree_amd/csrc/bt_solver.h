@@ -147,8 +147,8 @@ public:
     double *out_dev() { return d_out_; }
     long long near_pairs() const { return near_.pairs; }
     // pairs (ordered) the symmetric near field covers; 0 without it
-    long long sym_pairs() const { return sym_ ? sym_pairs_ : 0; }
-    long long sym_slots() const { return sym_ ? sym_slots_len_ : 0; }
+    long long sym_pairs() const { return sym_ && sym_enabled_ ? sym_pairs_ : 0; }
+    long long sym_slots() const { return sym_ && sym_enabled_ ? sym_slots_len_ : 0; }
     long long far_pairs() const { return grid_.pairs; }
     int source_nodes() const { return nsnodes_; }
     int target_nodes() const { return ntnodes_; }
@@ -156,6 +156,9 @@ public:
     int tpt() const { return tpt_; }
     bool persistent() const { return persistent_; }
     void set_persistent(bool p) { persistent_ = p; }
+    // the symmetric near field on (default) or off for later executions
+    void set_symmetric(bool on) { sym_enabled_ = on; }
+    bool symmetric() const { return sym_enabled_; }
     int num_near_tiles() const { return (int)near_tiles_.size(); }
     int num_far_tiles() const { return (int)grid_tiles_.size(); }
     hipStream_t stream() const { return st_; }
@@ -178,7 +181,7 @@ private:
                    std::vector<std::vector<char>> &sym_of);
     std::string upward_fixed();  // upward pass at the plan's degree
     std::string down_fixed();    // downpass at the plan's degree
-    bool sym_ready();  // slots allocated (at the first mixed execution)
+    bool sym_ready();  // enabled, and the slots allocated (at the first execution that uses them)
     std::string phase_events();
     std::string near_sym();
     std::string validate() const;
@@ -219,7 +222,7 @@ private:
     // over them.  Item k writes A's sums to slots [rslot, +|A|) and B's to
     // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
     // slots[base + i] for the bases [sym_red_off_[l], sym_red_off_[l+1]).
-    bool sym_ok_ = false, sym_ = false;
+    bool sym_ok_ = false, sym_ = false, sym_enabled_ = true;
     // the trees were built on the device; one tree and one set of particle
     // arrays serve as sources and targets (every system's targets are its
     // sources and its leaf sizes are equal)

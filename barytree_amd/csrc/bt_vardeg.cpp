@@ -649,10 +649,8 @@ std::string Plan::execute_var(bool timed)
         a.grid_targets = 0; a.ntiles = (int)vnear_tiles_.size(); a.tiles = d_vn_tiles_;
         a.seg_off = d_nseg_off_; a.segs = d_nsegs_; a.clu_off = d_vn_clu_off_; a.clu_id = d_vn_clu_id_;
         a.out1 = d_pot_;
-        if (p.precision == PREC_MIXED) {
-            a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
-            a.skip_sym = sym_ready() ? 1 : 0;
-        }
+        if (p.precision == PREC_MIXED) a.key_ctr = reinterpret_cast<const double4 *>(d_near_ctr_);
+        a.skip_sym = sym_ready() ? 1 : 0;
         std::copy(vnear_qlen_, vnear_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));

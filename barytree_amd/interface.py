@@ -125,7 +125,7 @@ def lib():
         l.BaryTree_GetLastError.restype = ctypes.c_char_p
         for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
-                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision",
+                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision", "BaryTreePlan_SetSymmetric",
                   "BaryTree_SolveBatched", "BaryTreePlan_CreateBatched", "BaryTree_Autotune",
                   "BaryTree_AnalyzeBatched", "BaryTreePlan_NumPairs", "BaryTreePlan_PairInfo",
                   "BaryTreePlan_PairFeatures", "BaryTreePlan_LabelPairs",
@@ -400,6 +400,12 @@ class Plan:
     def set_precision(self, precision):
         if lib().BaryTreePlan_SetPrecision(self._h, ctypes.c_int(precision)) != 0:
             raise RuntimeError(f"BaryTreePlan_SetPrecision failed: {last_error()}")
+
+    def set_symmetric(self, enable):
+        """Symmetric near field of later executions on (default) or off (one
+        evaluation per ordered pair, as the reference)."""
+        if lib().BaryTreePlan_SetSymmetric(self._h, ctypes.c_int(1 if enable else 0)) != 0:
+            raise RuntimeError(f"BaryTreePlan_SetSymmetric failed: {last_error()}")
 
     def stats(self):
         s = Stats()

@@ -114,15 +114,18 @@ typedef struct BaryTreeStats {
     int variable_degree;     /* 1: far-field pairs carry their own degree */
     double mean_pair_degree; /* mean degree over far-field pairs (variable degree) */
     long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
-    long long sym_ptwise;    /* near-field pairs (both directions counted) that mixed
-                                executions evaluate once for both particles: a
+    long long sym_ptwise;    /* near-field pairs (both directions counted) that
+                                executions evaluate once for both particles (the
+                                symmetric near field, either precision): a
                                 single system whose targets are its sources in
-                                the same tree order (Coulomb, Yukawa, or
-                                regularized Coulomb with skipping); 0 otherwise */
+                                the same tree order (Lagrange; Coulomb, Yukawa,
+                                or regularized Coulomb with skipping); 0
+                                otherwise or when switched off
+                                (BaryTreePlan_SetSymmetric) */
     double gpu_near_sym_s;   /* the symmetric near-field launch alone (within gpu_near_s) */
     long long sym_slots;     /* FP64 partial sums the symmetric launch writes (8 bytes
                                 of device memory each, allocated at the first
-                                mixed execution); 0 when sym_ptwise is 0 */
+                                execution that uses them); 0 when sym_ptwise is 0 */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
@@ -176,6 +179,12 @@ int BaryTreePlan_Execute(BaryTreePlan *plan, double *outputArray);
 /* Switches the pair arithmetic of later Execute calls (BARYTREE_PRECISION_*)
  * without rebuilding the plan. */
 int BaryTreePlan_SetPrecision(BaryTreePlan *plan, int precision);
+/* The symmetric near field of later Execute calls: 1 (the default) evaluates
+ * each near-field pair of two distinct leaves once for both particles where
+ * the plan allows it (see sym_ptwise); 0 keeps one evaluation per ordered
+ * pair, as src/interaction_compute/interaction_compute_cc.c:1019-1200 does.
+ * Results differ only by the order of the sums (FP64: rounding). */
+int BaryTreePlan_SetSymmetric(BaryTreePlan *plan, int enable);
 int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats);
 /* FP64 direct sum on the GPU (directdriver semantics,
  * src/drivers/directdriver.c:23-122) at m targets given by caller-order

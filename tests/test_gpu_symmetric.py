@@ -160,3 +160,90 @@ def test_symmetric_with_pair_degrees(bt):
     d = rel_l2(fp64, mixed)
     print(f"per-pair degrees, symmetric mixed vs fp64: {d:.3e}", flush=True)
     assert d < 1e-6
+
+
+# ---- FP64 (plain) executions: the same items in FP64 arithmetic ----------
+
+@pytest.mark.parametrize("ct,k,sg", [(3, 1, 2), (3, 1, 1), (3, 2, 2), (3, 2, 1), (3, 3, 1),
+                                     (1, 1, 2), (2, 2, 1), (1, 3, 1)])
+def test_symmetric_fp64_matches_oracle(bt, oracle_mod, ct, k, sg):
+    """Plain executions take the symmetric path too: the plain-mode bar
+    (1e-12 against the reference's arithmetic), repeat runs bit-equal, and
+    the one-directional FP64 run of the same plan agrees to rounding."""
+    print(f"case ct={ct} kernel={k} singularity={sg}", flush=True)
+    t, s = same_set(20000, 70 + 3 * ct + k + sg)
+    args = (k, KPS[k], sg, 1, ct, 0.8, 5, 300, 300)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_FP64) as plan:
+        st = plan.stats()
+        assert st["sym_ptwise"] > 0.8 * st["pp_ptwise"], st
+        got = plan.execute()
+        st = plan.stats()
+        assert st["gpu_near_sym_s"] > 0.0
+        assert np.array_equal(got, plan.execute())
+        plan.set_symmetric(False)
+        assert plan.stats()["sym_ptwise"] == 0
+        one = plan.execute()
+        assert plan.stats()["gpu_near_sym_s"] == 0.0
+    ref = oracle_mod.treecode(*t, *s, *args)
+    e_sym, e_one, e_pair = rel_l2(ref, got), rel_l2(ref, one), rel_l2(one, got)
+    print(f"fp64 vs oracle: symmetric {e_sym:.3e}, one-directional {e_one:.3e}; between {e_pair:.3e}",
+          flush=True)
+    assert e_sym <= 1e-12 and e_one <= 1e-12
+    assert e_pair <= 1e-13
+
+
+@pytest.mark.parametrize("k,sg", CASES)
+def test_symmetric_fp64_close_pairs_across_leaves(bt, oracle_mod, k, sg):
+    """Planted pairs 2e-10 and 2e-7 apart across the root's x split: FP64
+    differences resolve them; every particle within 1e-12 of the oracle
+    relative to the magnitude of its terms."""
+    t, s = same_set(12000, 80 + k + sg)
+    x, y, z, q, w = s
+    x[0], x[1] = -1.0, 1.0
+    for j in range(100):
+        i = 2 + 2 * j
+        d = 1e-10 if j % 2 == 0 else 1e-7
+        x[i], x[i + 1] = -d, d
+        y[i + 1], z[i + 1] = y[i], z[i]
+    t = (x.copy(), y.copy(), z.copy(), q.copy())
+    args = (k, KPS[k], sg, 1, 3, 0.8, 4, 300, 300)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_FP64) as plan:
+        assert plan.stats()["sym_ptwise"] > 0
+        got = plan.execute()
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert np.all(np.isfinite(got))
+    for j in range(100):
+        i = 2 + 2 * j
+        r = 2e-10 if j % 2 == 0 else 2e-7
+        if k == 3:
+            r = np.sqrt(r * r + KPS[3][0] ** 2)
+        for a, b in ((i, i + 1), (i + 1, i)):
+            mag = (abs(q[b] * w[b]) + (abs(q[a] * w[b]) if sg == 2 else 0.0)) / r
+            assert abs(got[a] - ref[a]) <= 1e-12 * (mag + abs(ref[a])), (j, a)
+    assert rel_l2(ref, got) <= 1e-12
+
+
+@pytest.mark.parametrize("leaf", [1, 7, 33])
+def test_symmetric_fp64_tiny_leaves(bt, oracle_mod, leaf):
+    t, s = same_set(3000, 90 + leaf)
+    args = (1, KPS[1], 2, 1, 3, 0.8, 3, leaf, leaf)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_FP64) as plan:
+        assert plan.stats()["sym_ptwise"] > 0
+        got = plan.execute()
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert rel_l2(ref, got) <= 1e-12
+
+
+def test_symmetric_fp64_through_the_reference_entry_point(bt, oracle_mod):
+    """The drop-in BaryTreeInterface (plain mode) on a system whose targets
+    are its sources takes the symmetric path and keeps the plain-mode bar."""
+    t, s = same_set(30000, 95)
+    args = (1, KPS[1], 2, 1, 3, 0.8, 6, 500, 500)
+    got = bt.callTreedriver(len(t[0]), len(s[0]), *t, *s[:4], s[4], bt.Kernel.COULOMB, 1, KPS[1],
+                            bt.Singularity.SUBTRACTION, bt.Approximation.LAGRANGE,
+                            bt.ComputeType.CLUSTER_CLUSTER, True, 0, theta=0.8, degree=6,
+                            sourceLeafSize=500, targetLeafSize=500)
+    st = bt.last_stats()
+    assert st["sym_ptwise"] > 0, st
+    ref = oracle_mod.treecode(*t, *s, *args)
+    assert rel_l2(ref, got) <= 1e-12
