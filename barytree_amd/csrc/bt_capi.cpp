@@ -64,8 +64,11 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->sym_ptwise = p.sym_pairs();
     s->gpu_near_sym_s = t.gpu_near_sym;
     s->sym_slots = p.sym_slots();
+    s->model_fallback_pairs = p.variable_degree() ? p.model_fallback_pairs() : 0;
 }
 
+// Degrees from the model; a pair whose features lie outside the model's
+// training range keeps the plan's fixed degree (the reference's method).
 std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)
 {
     std::vector<float> f;
@@ -74,7 +77,15 @@ std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)
     std::vector<int> deg(n);
     std::string e = m.predict(n, f.data(), delta, deg.data(), nullptr);
     if (!e.empty()) return e;
-    return plan.set_pair_degrees(deg.data());
+    long long fallback = 0;
+    for (int i = 0; i < n; i++)
+        if (m.outside_range(f.data() + (size_t)i * btl::MLP_IN)) {
+            deg[i] = plan.params().degree;
+            fallback++;
+        }
+    e = plan.set_pair_degrees(deg.data());
+    if (e.empty()) plan.set_model_fallback_pairs(fallback);
+    return e;
 }
 
 int plan_create(bt::Plan &plan, int M, int N, const double *tx, const double *ty, const double *tz,
@@ -837,6 +848,29 @@ int BaryTreeModel_SetParams(BaryTreeDegreeModel *model, const float *params,
     if (params) std::memcpy(model->m.params.data(), params, btl::MLP_PARAMS * sizeof(float));
     if (featureMean) std::memcpy(model->m.mean, featureMean, sizeof(model->m.mean));
     if (featureScale) std::memcpy(model->m.scale, featureScale, sizeof(model->m.scale));
+    return 0;
+}
+
+int BaryTreeModel_GetFeatureRange(const BaryTreeDegreeModel *model, float *lo, float *hi)
+{
+    if (!model) return fail("model is NULL");
+    if (lo) std::memcpy(lo, model->m.lo, sizeof(model->m.lo));
+    if (hi) std::memcpy(hi, model->m.hi, sizeof(model->m.hi));
+    return model->m.has_range() ? 1 : 0;
+}
+
+int BaryTreeModel_SetFeatureRange(BaryTreeDegreeModel *model, const float *lo, const float *hi)
+{
+    g_last_error.clear();
+    if (!model) return fail("model is NULL");
+    if (!lo || !hi) {
+        model->m.clear_range();
+        return 0;
+    }
+    for (int f = 0; f < btl::MLP_IN; f++)
+        if (!(lo[f] <= hi[f])) return fail("each feature range needs lo <= hi");
+    std::memcpy(model->m.lo, lo, sizeof(model->m.lo));
+    std::memcpy(model->m.hi, hi, sizeof(model->m.hi));
     return 0;
 }
 

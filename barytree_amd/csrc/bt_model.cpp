@@ -14,7 +14,9 @@ namespace bt {
 
 namespace {
 
-constexpr char MAGIC[8] = {'B', 'T', 'D', 'M', 'L', 'P', '0', '1'};
+// file layouts: 01 without the training range, 02 with it
+constexpr char MAGIC1[8] = {'B', 'T', 'D', 'M', 'L', 'P', '0', '1'};
+constexpr char MAGIC[8] = {'B', 'T', 'D', 'M', 'L', 'P', '0', '2'};
 
 // Device buffers owned by one call.
 struct DevBufs {
@@ -64,6 +66,44 @@ void DegreeModel::init(unsigned long long seed)
     steps = 0;
 }
 
+void DegreeModel::clear_range()
+{
+    for (int f = 0; f < btl::MLP_IN; f++) { lo[f] = 1.0f; hi[f] = -1.0f; }
+}
+
+bool DegreeModel::has_range() const
+{
+    for (int f = 0; f < btl::MLP_IN; f++)
+        if (!(lo[f] <= hi[f])) return false;
+    return true;
+}
+
+void DegreeModel::widen_range(int n, const float *x)
+{
+    for (int f = 0; f < btl::MLP_IN; f++) {
+        float a = lo[f], b = hi[f];
+        bool have = a <= b;
+        for (int i = 0; i < n; i++) {
+            const float v = x[(size_t)i * btl::MLP_IN + f];
+            if (!std::isfinite(v)) continue;
+            if (!have) { a = b = v; have = true; continue; }
+            a = std::min(a, v);
+            b = std::max(b, v);
+        }
+        if (have) { lo[f] = a; hi[f] = b; }
+    }
+}
+
+bool DegreeModel::outside_range(const float *x) const
+{
+    if (!has_range()) return false;
+    for (int f = 0; f < btl::MLP_IN; f++) {
+        const float m = 0.05f * (hi[f] - lo[f]);
+        if (!(x[f] >= lo[f] - m && x[f] <= hi[f] + m)) return true;  // NaN counts as outside
+    }
+    return false;
+}
+
 bool DegreeModel::has_scaling() const
 {
     for (float s : scale)
@@ -96,6 +136,8 @@ std::string DegreeModel::save(const std::string &path) const
               std::fwrite(&steps, sizeof(int), 1, f) == 1 &&
               std::fwrite(mean, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
               std::fwrite(scale, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
+              std::fwrite(lo, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
+              std::fwrite(hi, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
               std::fwrite(params.data(), sizeof(float), params.size(), f) == params.size();
     ok = (std::fclose(f) == 0) && ok;
     return ok ? "" : "write to " + path + " failed";
@@ -108,13 +150,16 @@ std::string DegreeModel::load(const std::string &path)
     char mg[8];
     int hdr[4];
     DegreeModel m;
-    bool ok = std::fread(mg, 1, 8, f) == 8 && std::memcmp(mg, MAGIC, 8) == 0 &&
-              std::fread(hdr, sizeof(int), 4, f) == 4 && hdr[0] == btl::MLP_IN &&
-              hdr[1] == btl::MLP_H && hdr[2] == btl::MLP_OUT && hdr[3] == btl::MLP_PARAMS &&
-              std::fread(&m.steps, sizeof(int), 1, f) == 1 &&
-              std::fread(m.mean, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
-              std::fread(m.scale, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
-              std::fread(m.params.data(), sizeof(float), m.params.size(), f) == m.params.size();
+    bool ok = std::fread(mg, 1, 8, f) == 8 && (std::memcmp(mg, MAGIC, 8) == 0 || std::memcmp(mg, MAGIC1, 8) == 0);
+    const bool ranged = ok && std::memcmp(mg, MAGIC, 8) == 0;
+    ok = ok && std::fread(hdr, sizeof(int), 4, f) == 4 && hdr[0] == btl::MLP_IN &&
+         hdr[1] == btl::MLP_H && hdr[2] == btl::MLP_OUT && hdr[3] == btl::MLP_PARAMS &&
+         std::fread(&m.steps, sizeof(int), 1, f) == 1 &&
+         std::fread(m.mean, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
+         std::fread(m.scale, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
+         (!ranged || (std::fread(m.lo, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN &&
+                      std::fread(m.hi, sizeof(float), btl::MLP_IN, f) == (size_t)btl::MLP_IN)) &&
+         std::fread(m.params.data(), sizeof(float), m.params.size(), f) == m.params.size();
     std::fclose(f);
     if (!ok) return path + " is not a degree model file of this layout";
     *this = m;
@@ -155,6 +200,7 @@ std::string DegreeModel::train(int n, const float *x, const int *labels, int epo
 {
     if (n <= 0 || epochs <= 0 || batch <= 0) return "bad training arguments";
     if (!has_scaling()) set_scaling(n, x);
+    widen_range(n, x);
     batch = std::min(batch, n);
     DevBufs db;
     float *dp = db.get<float>(btl::MLP_PARAMS), *dmu = db.get<float>(btl::MLP_IN),
@@ -200,6 +246,7 @@ std::string DegreeModel::train(int n, const float *x, const int *labels, int epo
 std::string DegreeModel::predict(int n, const float *x, float delta, int *degree, float *logits) const
 {
     if (n < 0) return "bad sample count";
+    if (!(delta > 0.0f && delta < 1.0f)) return "delta must lie in (0, 1)";
     if (n == 0) return "";
     if (!has_scaling()) return "model has no feature scaling (train it or set it first)";
     // in chunks, so the activation buffers stay small

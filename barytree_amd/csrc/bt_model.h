@@ -15,7 +15,18 @@ struct DegreeModel {
     std::vector<float> params = std::vector<float>(btl::MLP_PARAMS, 0.0f);
     float mean[btl::MLP_IN] = {};
     float scale[btl::MLP_IN] = {};  // all zero until set from training data
+    // the range of every feature over the training data (lo > hi: unknown;
+    // models saved before ranges were recorded have none)
+    float lo[btl::MLP_IN], hi[btl::MLP_IN];
     int steps = 0;                  // Adam steps taken
+
+    DegreeModel() { clear_range(); }
+    void clear_range();
+    bool has_range() const;
+    void widen_range(int n, const float *x);  // the union with the range of x
+    // 1 if a feature row lies outside the training range by more than 5% of
+    // the range's width (the guard falls back to the fixed degree there)
+    bool outside_range(const float *x) const;
 
     void init(unsigned long long seed);  // He-uniform weights, zero biases
     bool has_scaling() const;

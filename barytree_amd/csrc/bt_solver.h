@@ -133,6 +133,10 @@ public:
     // Gives every far-field pair its own degree (1..12) and builds and
     // uploads the per-degree work; nullptr returns to the fixed degree.
     std::string set_pair_degrees(const int *deg);
+    // far-field pairs a degree model left at the fixed degree (features
+    // outside its training range); set by the caller of set_pair_degrees
+    void set_model_fallback_pairs(long long n) { model_fallback_pairs_ = n; }
+    long long model_fallback_pairs() const { return model_fallback_pairs_; }
     bool variable_degree() const { return var_planned_; }
     double mean_pair_degree() const { return var_mean_deg_; }
     long long variable_far_evals() const { return var_evals_; }
@@ -223,6 +227,7 @@ private:
     // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
     // slots[base + i] for the bases [sym_red_off_[l], sym_red_off_[l+1]).
     bool sym_ok_ = false, sym_ = false, sym_enabled_ = true;
+    long long model_fallback_pairs_ = 0;
     // the trees were built on the device; one tree and one set of particle
     // arrays serve as sources and targets (every system's targets are its
     // sources and its leaf sizes are equal)

@@ -78,7 +78,7 @@ class Stats(ctypes.Structure):
         ("num_far_pairs", ctypes.c_longlong), ("variable_degree", ctypes.c_int),
         ("mean_pair_degree", ctypes.c_double), ("far_ptwise_variable", ctypes.c_longlong),
         ("sym_ptwise", ctypes.c_longlong), ("gpu_near_sym_s", ctypes.c_double),
-        ("sym_slots", ctypes.c_longlong)]
+        ("sym_slots", ctypes.c_longlong), ("model_fallback_pairs", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -133,6 +133,7 @@ def lib():
                   "BaryTree_LabelPairs", "BaryTreeModel_Create", "BaryTreeModel_Load",
                   "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
                   "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
+                  "BaryTreeModel_GetFeatureRange", "BaryTreeModel_SetFeatureRange",
                   "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
@@ -537,6 +538,26 @@ class DegreeModel:
                 for a in (params, mean, scale)]
         lib().BaryTreeModel_SetParams(self._h, *[None if a is None else a.ctypes.data_as(_fp)
                                                  for a in conv])
+
+    def feature_range(self):
+        """(lo, hi) of every feature over the training data, or None."""
+        lo = np.zeros(PAIR_FEATURES, dtype=np.float32)
+        hi = np.zeros(PAIR_FEATURES, dtype=np.float32)
+        rc = lib().BaryTreeModel_GetFeatureRange(self._h, lo.ctypes.data_as(_fp), hi.ctypes.data_as(_fp))
+        if rc < 0:
+            raise RuntimeError(f"BaryTreeModel_GetFeatureRange failed: {last_error()}")
+        return (lo, hi) if rc == 1 else None
+
+    def set_feature_range(self, lo, hi):
+        """Sets the training range (None removes it and the guard)."""
+        if lo is None or hi is None:
+            rc = lib().BaryTreeModel_SetFeatureRange(self._h, None, None)
+        else:
+            lo = np.ascontiguousarray(lo, dtype=np.float32)
+            hi = np.ascontiguousarray(hi, dtype=np.float32)
+            rc = lib().BaryTreeModel_SetFeatureRange(self._h, lo.ctypes.data_as(_fp), hi.ctypes.data_as(_fp))
+        if rc != 0:
+            raise RuntimeError(f"BaryTreeModel_SetFeatureRange failed: {last_error()}")
 
     def gradient(self, features, labels):
         x = np.ascontiguousarray(features, dtype=np.float32)

@@ -89,6 +89,23 @@ def training_set(cfg, log=print):
     return x, y
 
 
+def feature_range(cfg, log=print):
+    """(lo, hi) of every pair feature over all far-field pairs of the
+    configuration's training particle sets (host-only analysis: the same
+    trees and lists a plan builds)."""
+    lo = hi = None
+    for seed in cfg["seeds"]:
+        x, y, z, q, w = particles(cfg["dist"], cfg["n"], seed)
+        _, f = bt.analyze_degrees((x, y, z, q), (x, y, z, q, w), cfg["kernel"], cfg["kp"], cfg["sing"],
+                                  bt.Approximation.LAGRANGE, cfg["ct"], cfg["theta"], cfg["degree"],
+                                  cfg["leaf"], cfg["leaf"])
+        a, b = f.min(axis=0), f.max(axis=0)
+        lo = a if lo is None else np.minimum(lo, a)
+        hi = b if hi is None else np.maximum(hi, b)
+        log(f"[learned] seed {seed}: {len(f)} pairs")
+    return lo, hi
+
+
 def train(x, y, epochs=12, batch=4096, lr=2e-3, seed=0):
     m = bt.DegreeModel(seed=seed)
     losses = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
@@ -100,8 +117,18 @@ def main():
     ap.add_argument("--model", choices=sorted(MODELS), required=True)
     ap.add_argument("--out", default=None)
     ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--set-range", action="store_true",
+                    help="only record the training range of the features in an existing model file")
     args = ap.parse_args()
     cfg = MODELS[args.model]
+    if args.set_range:
+        path = args.out or model_path(args.model)
+        m = bt.DegreeModel(path=model_path(args.model))
+        lo, hi = feature_range(cfg)
+        m.set_feature_range(lo, hi)
+        m.save(path)
+        print(f"[learned] {args.model}: feature range {lo.tolist()} .. {hi.tolist()}; wrote {path}", flush=True)
+        return
     t0 = time.perf_counter()
     x, y = training_set(cfg)
     t1 = time.perf_counter()

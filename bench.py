@@ -5,16 +5,30 @@ subtraction (kappa = 0.5), N = 16M particles uniform in [-1,1]^3 (sources =
 targets, separate arrays), charges uniform in [-1,1], weights 1,
 cluster-cluster treecode, theta = 0.8, degree 10, max leaf 8000.
 
-A step is one execution of the GPU path (upward pass, near and far field,
-downpass, correction, scatter to caller order) with its inputs resident in
-HBM; the host work (trees, interaction lists, upload) is done once by the
-plan and reported separately.  Three forms are timed on the same plan: the
-plain FP64 form; the mixed form (FP32 pair evaluation with FP64 charges and
-accumulation); and the mixed, learned-degree form (each far-field pair at the
-degree the shipped degree model picks) -- the one the metric is quoted on.
-Each is checked against an FP64 direct sum at 1000 stride-sampled targets.
+A step is one BaryTree_SolveLearned call: host arrays in, potential (caller
+order) out -- trees built on the GPU, interaction lists on the host, the
+per-pair degrees the shipped degree model picks, the whole GPU path in
+mixed precision, the copy back.  `value` is seconds per call (the model is
+loaded once, outside the timed region).  Beside it:
+  * the plain FP64 time-to-solution (one BaryTree_Solve call, the drop-in
+    path) and the speedup over it;
+  * the GPU path alone with its inputs resident in HBM (one plan): mixed +
+    learned, mixed at the fixed degree, plain FP64 with the symmetric near
+    field, plain FP64 with one evaluation per ordered near-field pair (the
+    reference's form);
+  * every result's relative L2 error against the FP64 direct sum at 1000
+    stride-sampled targets;
+  * the roofline of the dominant kernel (the symmetric near field);
+  * CPU baselines: the reference's own sources on one core (a sample at
+    500k, extrapolated phase by phase) and the oracle on all host cores at
+    the full size (interactions sampled);
+  * `configs`: the other configurations (C1 Coulomb PC 100k against the
+    reference on one core, C2 Yukawa CC 1M, C3 256 batched systems, C4
+    learned degrees on a 4M Plummer sphere), each with its own timings,
+    errors and baselines.
 
-Prints one JSON line on stdout.
+Prints one JSON line on stdout.  --no-configs skips the other
+configurations, --no-cpu the CPU baselines.
 
 --workload learned measures the learned degree selection on its own
 configuration: regularized Coulomb (epsilon 0.01), N = 4M Plummer sphere
@@ -23,11 +37,12 @@ configuration: regularized Coulomb (epsilon 0.01), N = 4M Plummer sphere
 (trained on other Plummer realizations), in FP64 and mixed.
 
 --workload yukawa: Yukawa (kappa 0.5), N = 1M uniform, cluster-cluster,
-theta 0.8, degree 8, leaf 4000, plain FP64 against mixed precision.
+theta 0.8, degree 8, leaf 4000, plain FP64 (symmetric and one-directional
+near field) against mixed precision.
 
 --workload cpu-case: Coulomb, N = 100k uniform, particle-cluster, theta 0.7,
 degree 8, leaf 2000, skipping, FP64, against the reference's own sources
-compiled for one CPU core.
+compiled for one CPU core and the oracle on all host cores.
 
 --workload hermite: Coulomb with the Hermite approximation, particle-cluster
 and cluster-particle, degree 5, against the reference's own sources on one
@@ -45,6 +60,7 @@ plain single-system solves with the same leaf sizes.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -154,30 +170,6 @@ def issue_model(mix, probe, simds):
     return ns * 1e-9 / simds
 
 
-def cpu_baseline(n_evals):
-    """Times the oracle (FP64 restatement of the reference kernels, 1 core)
-    on a bounded sample and scales to the workload's evaluation count."""
-    from oracle import oracle as O
-    if not os.path.exists(O.LIB_PATH):
-        O.build()
-    m, n = 1000, 2_000_000
-    x, y, z, q, w = gen_uniform(n, 99)
-    t0 = time.perf_counter()
-    O.direct(x[:m], y[:m], z[:m], q[:m], x, y, z, q, w, 1, [0.5], 2)
-    dt = time.perf_counter() - t0
-    rate = m * n / dt
-    return {
-        "value": n_evals / rate,
-        "unit": "s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"oracle FP64 Coulomb-SS pair kernel (coulomb_ss_pp.c arithmetic), "
-                   f"{m} targets x {n} sources = {m * n:.1e} evaluations in {dt:.1f} s "
-                   f"({rate:.3e} eval/s, 1 core); workload time = its {n_evals:.3e} "
-                   f"point-wise evaluations / that rate (upward/downward passes excluded)"),
-    }
-
-
 def bench_batched(args):
     import torch  # device setup and synchronisation only
     assert torch.cuda.is_available(), "no GPU"
@@ -229,8 +221,11 @@ def bench_batched(args):
     seq = min(r[0] for r in seq_runs)
     seq_gpu = min(r[2] for r in seq_runs)
     outs_seq = seq_runs[-1][1]
-    exact = all(np.array_equal(a, b) for a, b in zip(outs_seq, out_fp64))
-    log(f"[bench] sequential {seq:.3f}s (sum of gpu phases {seq_gpu:.4f}s), batched==sequential: {exact}")
+    # single-system solves take the symmetric near field (targets are the
+    # sources), batched plans do not: the sums' order differs
+    diff_seq = rel_l2(np.concatenate(outs_seq), np.concatenate(out_fp64))
+    log(f"[bench] sequential {seq:.3f}s (sum of gpu phases {seq_gpu:.4f}s), batched vs sequential "
+        f"relL2 {diff_seq:.2e}")
 
     # resident-input GPU time: batched plan vs per-system plans
     plan = bt.Plan.batched(systems, *kargs, schedule=sc)
@@ -268,7 +263,7 @@ def bench_batched(args):
         "sequential_s": round(seq, 4),
         "speedup_vs_sequential": round(seq / e2e_fp64, 3),
         "batched_mixed_s": round(e2e_mixed, 4),
-        "batched_equals_sequential_bitwise": exact,
+        "rel_l2_batched_vs_sequential": diff_seq,
         "gpu_batched_fp64_s": round(gpu_b, 5),
         "gpu_batched_mixed_s": round(gpu_bm, 5),
         "gpu_sequential_sum_s": round(seq_gpu, 5),
@@ -277,7 +272,7 @@ def bench_batched(args):
         "interactions": evals,
         "interactions_per_s_gpu_fp64": evals / gpu_b,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_learned(args):
@@ -340,7 +335,7 @@ def bench_learned(args):
         "speedup_learned_mixed_vs_fixed_fp64": round(res["fixed_fp64"]["s"] / res["learned_mixed"]["s"], 3),
         "host_setup_s": round(host_setup, 3),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_yukawa(args):
@@ -363,7 +358,9 @@ def bench_yukawa(args):
     idx = np.arange(0, n, n // 1000)[:1000]
     direct = plan.direct_sample(idx)
     res = {}
-    for form, prec in (("fp64", bt.PRECISION_FP64), ("mixed", bt.PRECISION_MIXED)):
+    for form, prec, sym in (("fp64_one_directional", bt.PRECISION_FP64, False),
+                            ("fp64", bt.PRECISION_FP64, True), ("mixed", bt.PRECISION_MIXED, True)):
+        plan.set_symmetric(sym)
         wall, gpu, near, st = time_steps(plan, prec, args.steps, args.warmup)
         out = plan.execute()
         err = rel_l2(direct, out[idx])
@@ -397,6 +394,7 @@ def bench_yukawa(args):
                    "n": n, "compute_type": "CC", "theta": 0.8, "degree": 8, "max_leaf": 4000},
         "forms": res,
         "speedup_mixed_vs_fp64": round(res["fp64"]["s"] / res["mixed"]["s"], 3),
+        "speedup_mixed_vs_fp64_one_directional": round(res["fp64_one_directional"]["s"] / res["mixed"]["s"], 3),
         "interactions": evals,
         "roofline": {"bound": "valu", "kernel": "k_near_sym<Yukawa, skipping> (symmetric near field, mixed)",
                      "achieved": round(achieved, 2), "peak": PEAK["fp32"], "unit": "TFLOP/s",
@@ -404,7 +402,7 @@ def bench_yukawa(args):
                      "kernel_s": round(sym_s, 5), "ordered_pairs": st["sym_ptwise"]},
         "host_setup_s": round(host_setup, 3),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_cpu_case(args):
@@ -441,6 +439,9 @@ def bench_cpu_case(args):
     diff = rel_l2(out_cpu, out_gpu)
     log(f"[bench] C1 gpu {wall:.4f} s/step (end to end {e2e:.3f} s), cpu {kind} {cpu_s:.2f} s, "
         f"gpu vs cpu relL2 {diff:.2e}, vs direct {rel_l2(direct, out_gpu[idx]):.2e}")
+    threads = host_threads()
+    allc = cpu_run("port", (x, y, z, q, w), (1, [0.0], 1, 1, 1, 0.7, 8, 2000, 2000), 1, threads)
+    log(f"[bench] C1 oracle on {threads} threads {allc['wall']:.2f} s")
     line = {
         "metric": "time per solve, Coulomb PC N=100k FP64, GPU vs the reference on one CPU core",
         "value": round(wall, 5),
@@ -461,11 +462,15 @@ def bench_cpu_case(args):
         "rel_l2_err": rel_l2(direct, out_gpu[idx]),
         "rel_l2_gpu_vs_cpu": diff,
         "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s", "cores": 1, "kind": kind,
-                         "sample": f"the whole treecode on the same {n} particles, one call"},
+                         "sample": f"the whole treecode on the same {n} particles, one call",
+                         "all_cores": {"value": round(allc["wall"], 3), "unit": "s", "cores": threads,
+                                       "kind": "port", "sample": "the oracle on the same particles, one call",
+                                       "phases_s": {k: round(v, 4) for k, v in allc["phases"].items()}},
+                         "host": host_cpu()},
         "speedup_vs_cpu": round(cpu_s / wall, 1),
         "speedup_end_to_end_vs_cpu": round(cpu_s / e2e, 1),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def bench_hermite(args):
@@ -547,7 +552,7 @@ def bench_hermite(args):
                          "sample": f"the whole Hermite treecode (particle-cluster) on the same {n} particles, one call"},
         "speedup_vs_cpu": round(cpu_s / gpu_s, 1),
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 # the kernels beside the Coulomb family: id -> (name, kernel parameters)
@@ -614,7 +619,345 @@ def bench_kernels(args):
         "cpu_baseline": {"value": round(cpu_total, 3), "unit": "s", "cores": 1, "kind": kind,
                          "sample": f"the whole treecode of each kernel on {small} particles, one call each (sum)"},
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def host_threads():
+    """Threads for the all-core CPU runs: OMP_NUM_THREADS when set (the
+    host's share of cores), else the core count, at most 16."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(16, n if n > 0 else (os.cpu_count() or 1)))
+
+
+def host_cpu():
+    """Core count and CPU model of this host (nproc, lscpu)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        pass
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return info
+
+
+def cpu_run(kind, pts, kargs, stride=1, threads=1):
+    """One CPU treecode call on (x, y, z, q, w), targets = sources: kind
+    "reference" runs the reference's own sources (oracle/_ref), "port" the
+    oracle.  With stride k only every k-th key's interaction lists are
+    computed (the trees, clusters, lists and downpass run in full).  Returns
+    the wall time, the phase times and the point-wise evaluation counts
+    (computed and full)."""
+    from oracle import oracle as O
+    x, y, z, q, w = pts
+    t = (x.copy(), y.copy(), z.copy(), q.copy())
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.ref_set_sample_stride(stride)
+        O.ref_treecode(*t, x, y, z, q, w, *kargs)
+        wall = time.perf_counter() - t0
+        O.ref_set_sample_stride(1)
+        r = O.ref_last_timing()
+        done, total = r["near_done"] + r["far_done"], r["near_total"] + r["far_total"]
+        nodes = None
+    else:
+        O.set_sample_stride(stride)
+        O.treecode(*t, x, y, z, q, w, *kargs)
+        wall = time.perf_counter() - t0
+        O.set_sample_stride(1)
+        r = O.last_stats()
+        done = r["near_done"] + r["far_done"]
+        total = r["pp_ptwise"] + r["pc_ptwise"] + r["cp_ptwise"] + r["cc_ptwise"]
+        nodes = r["source_nodes"] + r["target_nodes"]
+    O.set_threads(O.max_threads())
+    phases = {k: r[k] for k in ("t_setup", "t_upward", "t_lists", "t_compute", "t_downpass", "t_finish")}
+    return {"wall": wall, "phases": phases, "evals_done": done, "evals_total": total, "nodes": nodes}
+
+
+def same_n_estimate(r):
+    """Seconds of the whole treecode from a sampled run at the same size: every
+    phase as measured, the interaction phase scaled by evaluations."""
+    ph = r["phases"]
+    rest = sum(v for k, v in ph.items() if k != "t_compute")
+    return rest + ph["t_compute"] * r["evals_total"] / max(r["evals_done"], 1)
+
+
+def cpu_baseline_large(n, kargs, st16, threads):
+    """CPU baselines of the headline workload.
+    reference: the reference's sources on one core at N_s = 250k (same
+    parameters and generator), the interactions of one target leaf computed
+    (the last node in preorder), the other phases in full; extrapolated to
+    N = 16M phase by phase (interactions by the point-wise counts of the 16M
+    plan, upward pass and downpass by N P + nodes P^2, trees by N log N,
+    lists by nodes).
+    port: the oracle on all host cores at the full N = 16M, the interactions
+    of every (nodes/3)-th target node computed (the rest of the treecode in
+    full), interactions scaled by evaluations.  calibration: the oracle over
+    the reference, both on one core on the 250k sample."""
+    import barytree_amd as bt
+    from oracle import oracle as O
+    ns = 250_000
+    pts = gen_uniform(ns, 5)
+    st_s = bt.analyze(pts[:4], pts, *kargs)
+    stride = max(1, st_s["target_nodes"] - 1)  # the root (no lists) and the last leaf
+    P = (kargs[6] + 1) ** 3
+    ref = cpu_run("reference" if O.ref_available() else "port", pts, kargs, stride, 1)
+    port1 = cpu_run("port", pts, kargs, stride, 1)
+    nodes_s = port1["nodes"]
+    nodes16 = st16["source_nodes"] + st16["target_nodes"]
+    evals16 = st16["pp_ptwise"] + st16["pc_ptwise"] + st16["cp_ptwise"] + st16["cc_ptwise"]
+
+    def extrapolate(r):
+        ph = r["phases"]
+        work = lambda nn, nodes: nn * P + nodes * P * P
+        est = {
+            "t_setup": ph["t_setup"] * (n * np.log(n)) / (ns * np.log(ns)),
+            "t_upward": ph["t_upward"] * work(n, nodes16 / 2) / work(ns, nodes_s / 2),
+            "t_lists": ph["t_lists"] * nodes16 / nodes_s,
+            "t_compute": ph["t_compute"] * evals16 / max(r["evals_done"], 1),
+            "t_downpass": ph["t_downpass"] * work(n, nodes16 / 2) / work(ns, nodes_s / 2),
+            "t_finish": ph["t_finish"] * n / ns,
+        }
+        return est, sum(est.values())
+
+    ref_est, ref_total = extrapolate(ref)
+    port1_est, port1_total = extrapolate(port1)
+    calib = port1["wall"] / ref["wall"]
+    log(f"[bench] cpu: reference 1 core on N={ns} (stride {stride}) {ref['wall']:.1f} s -> "
+        f"{ref_total:.0f} s at N={n} (extrapolated); port/reference {calib:.3f}")
+    allc = None
+    if n == 16_000_000 and threads > 1:
+        big = gen_uniform(n, 5)
+        stride16 = max(1, st16["target_nodes"] // 3)
+        allc = cpu_run("port", big, kargs, stride16, threads)
+        allc_total = same_n_estimate(allc)
+        log(f"[bench] cpu: oracle on {threads} threads at N={n} (stride {stride16}) {allc['wall']:.1f} s -> "
+            f"{allc_total:.0f} s (interactions extrapolated)")
+        del big
+    rate_ref = ref["evals_done"] / max(ref["phases"]["t_compute"], 1e-12)
+    out = {
+        "value": round(ref_total, 1),
+        "unit": "s",
+        "cores": 1,
+        "kind": "reference" if O.ref_available() else "port",
+        "sample": (f"the reference's own sources (oracle/_ref) on one core, N={ns} with the headline's "
+                   f"parameters, interactions of the last of its {st_s['target_nodes']} target "
+                   f"nodes ({ref['evals_done']:.3e} "
+                   f"evaluations at {rate_ref:.3e}/s), the other phases in full ({ref['wall']:.1f} s in all); "
+                   f"extrapolated to N={n}: interactions by the plan's {evals16:.3e} point-wise "
+                   f"evaluations, upward pass and downpass by N P + nodes P^2, trees by N log N"),
+        "extrapolated": True,
+        "phases_s": {k: round(v, 2) for k, v in ref_est.items()},
+        "sample_phases_s": {k: round(v, 3) for k, v in ref["phases"].items()},
+        "calibration_port_over_reference": round(calib, 4),
+        "port_one_core_extrapolated_s": round(port1_total, 1),
+        "host": host_cpu(),
+    }
+    if allc is not None:
+        out["all_cores"] = {
+            "value": round(allc_total, 1), "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"the oracle (OpenMP over targets and nodes, per-target sums in serial order) at "
+                       f"N={n}, interactions of every {stride16}th target node ({allc['evals_done']:.3e} of "
+                       f"{allc['evals_total']:.3e} evaluations), the other phases in full "
+                       f"({allc['wall']:.1f} s in all)"),
+            "phases_s": {k: round(v, 2) for k, v in allc["phases"].items()},
+            "reference_equivalent_s": round(allc_total / calib, 1),
+        }
+    return out
+
+
+def summary(line):
+    """A workload's line without the keys every line repeats."""
+    drop = ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "data",
+            "ms_per_step")
+    return {k: v for k, v in line.items() if k not in drop}
+
+
+def bench_large(args):
+    """The headline workload (docstring at the top of the file)."""
+    import torch  # device setup and synchronisation only
+    assert torch.cuda.is_available(), "no GPU"
+    import barytree_amd as bt
+    from barytree_amd import learned
+
+    n = args.n
+    log(f"[bench] generating N={n}")
+    x, y, z, q, w = gen_uniform(n, 5)
+    targets = (x.copy(), y.copy(), z.copy(), q.copy())
+    sources = (x, y, z, q, w)
+    kargs = (bt.Kernel.COULOMB, [0.5], bt.Singularity.SUBTRACTION,
+             bt.Approximation.LAGRANGE, bt.ComputeType.CLUSTER_CLUSTER, 0.8, 10, 8000, 8000)
+    model = learned.load_model("coulomb_cc")  # loaded once, outside the timed region
+
+    # 1. time-to-solution: one BaryTree_SolveLearned call per step (host
+    # arrays in, potential out: trees on the GPU, lists, degrees from the
+    # model, the GPU path, the copy back)
+    def tts_learned():
+        t0 = time.perf_counter()
+        out = bt.solve_learned(targets, sources, *kargs, model=model, delta=args.delta,
+                               precision=bt.PRECISION_MIXED)
+        return time.perf_counter() - t0, out, bt.last_stats()
+    for _ in range(args.warmup):
+        tts_learned()
+    torch.cuda.synchronize()
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        runs.append(tts_learned())
+    torch.cuda.synchronize()
+    tts = (time.perf_counter() - t0) / args.steps
+    out_tts, st_tts = runs[-1][1], runs[-1][2]
+    host_s = sum(st_tts[k] for k in ("host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s"))
+    log(f"[bench] time-to-solution (mixed + learned, one call): {tts:.3f} s "
+        f"(trees {st_tts['host_tree_s']:.3f} lists {st_tts['host_lists_s']:.3f} work {st_tts['host_work_s']:.3f} "
+        f"h2d {st_tts['h2d_s']:.3f} gpu {st_tts['gpu_total_s']:.3f})")
+
+    # plain FP64 time-to-solution (BaryTree_Solve, the drop-in path)
+    plain_tts, out_plain_tts = None, None
+    if args.plain_steps > 0:
+        ts = []
+        for _ in range(args.plain_steps):
+            t0 = time.perf_counter()
+            out_plain_tts = bt.solve(targets, sources, *kargs)
+            ts.append(time.perf_counter() - t0)
+        plain_tts = float(np.mean(ts))
+        log(f"[bench] time-to-solution plain FP64 (one BaryTree_Solve call): {plain_tts:.3f} s")
+
+    # 2. the GPU path alone, inputs resident in HBM (one plan, several forms)
+    t0 = time.perf_counter()
+    plan = bt.Plan(targets, sources, *kargs, precision=bt.PRECISION_MIXED)
+    plan_setup = time.perf_counter() - t0
+    st0 = plan.stats()
+    log(f"[bench] plan {plan_setup:.2f}s (tree {st0['host_tree_s']:.2f} lists "
+        f"{st0['host_lists_s']:.2f} work {st0['host_work_s']:.2f} h2d {st0['h2d_s']:.2f})")
+    idx = np.arange(0, n, n // 1000)[:1000]
+    direct = plan.direct_sample(idx)
+    gsteps = max(1, min(args.steps, 3))
+    wall_fast, gpu_fast, near_fast, st = time_steps(plan, bt.PRECISION_MIXED, gsteps, 1)
+    err_fast = rel_l2(direct, plan.execute()[idx])
+    log(f"[bench] mixed: {wall_fast:.3f} s/step (gpu {gpu_fast:.3f}, near {near_fast:.3f}), relL2 {err_fast:.3e}")
+    t0 = time.perf_counter()
+    plan.use_model(model, args.delta)
+    deg_setup = time.perf_counter() - t0
+    wall_ml, gpu_ml, near_ml, st_ml = time_steps(plan, bt.PRECISION_MIXED, gsteps, 1)
+    err_ml = rel_l2(direct, plan.execute()[idx])
+    log(f"[bench] mixed+learned: {wall_ml:.3f} s/step (gpu {gpu_ml:.3f}, far {st_ml['gpu_far_s']:.3f}, "
+        f"mean degree {st_ml['mean_pair_degree']:.2f}), relL2 {err_ml:.3e}, degree setup {deg_setup:.2f}s")
+    plan.set_pair_degrees(None)
+    wall_plain = wall_plain1 = err_plain = None
+    if args.plain_steps > 0:
+        wall_plain, gpu_plain, _, st_p = time_steps(plan, bt.PRECISION_FP64, args.plain_steps, 1)
+        err_plain = rel_l2(direct, plan.execute()[idx])
+        log(f"[bench] fp64 (symmetric near field): {wall_plain:.3f} s/step (symmetric launch "
+            f"{st_p['gpu_near_sym_mean_s']:.3f}), relL2 {err_plain:.3e}")
+        # the reference's form: one evaluation per ordered near-field pair
+        plan.set_symmetric(False)
+        wall_plain1, _, _, _ = time_steps(plan, bt.PRECISION_FP64, 1, 0)
+        plan.set_symmetric(True)
+        log(f"[bench] fp64 one-directional: {wall_plain1:.3f} s/step")
+    plan.close()
+    err_tts = rel_l2(direct, out_tts[idx])
+    err_plain_tts = rel_l2(direct, out_plain_tts[idx]) if out_plain_tts is not None else None
+
+    evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
+    evals_learned = st["pp_ptwise"] + st_ml["far_ptwise_variable"]
+    # dominant kernel: the symmetric near field (each leaf pair once for both
+    # particles); its work counted as the reference's, 17 flops per ordered
+    # pair evaluation it delivers
+    sym_evals = st["sym_ptwise"]
+    sym_s = st_ml["gpu_near_sym_mean_s"]
+    achieved = FLOPS_PER_EVAL * sym_evals / sym_s / 1e12 if sym_s > 0 else 0.0
+    roofline = {
+        "bound": "valu",
+        "kernel": "k_near_sym<Coulomb, subtraction> (symmetric near field, mixed precision)",
+        "achieved": round(achieved, 2),
+        "peak": PEAK["fp32"],
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK["fp32"], 4),
+        "frac_executed": round(achieved / 2 / PEAK["fp32"], 4),
+        "traffic": near_traffic() if n == 16_000_000 else None,
+        "kernel_s": round(sym_s, 4),
+        "ordered_pairs": sym_evals,
+        "flops_per_ordered_pair": FLOPS_PER_EVAL,
+    }
+    mix = near_valu_mix() if n == 16_000_000 else None
+    if mix:
+        from barytree_amd.probe import probe_valu
+        probe = probe_valu()
+        simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
+        ceil_s = issue_model(mix, probe, simds)
+        roofline["valu_issue_model_s"] = round(ceil_s, 4)
+        roofline["valu_issue_model_over_measured"] = round(ceil_s / sym_s, 4)
+        log(f"[bench] symmetric near kernel {sym_s:.3f} s, VALU issue time of its instruction mix "
+            f"(kinds priced alone) {ceil_s:.3f} s; probe ns/instr {probe}")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baselines")
+        cpu = cpu_baseline_large(n, kargs, st, host_threads())
+
+    line = {
+        "metric": "time-to-solution N=16M at rel-L2 err <=1e-6 vs FP64 direct; speedup vs plain",
+        "value": round(tts, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tts * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 pair kernel, f64 charges/accumulation",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"Coulomb singularity subtraction kappa=0.5, N={n} uniform [-1,1]^3, "
+                         "cluster-cluster theta=0.8 degree 10 max leaf 8000"),
+            "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
+        },
+        "form": ("one BaryTree_SolveLearned call per step: host arrays in, potential out; mixed precision "
+                 "+ learned per-pair degree (shipped coulomb_cc model, loaded once)"),
+        "rel_l2_err": err_tts,
+        "time_to_solution_phases_s": {k: round(st_tts[k], 4) for k in (
+            "host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s", "gpu_total_s")},
+        "host_setup_s": round(host_s, 4),
+        "plain_fp64_time_to_solution_s": round(plain_tts, 4) if plain_tts else None,
+        "plain_fp64_time_to_solution_rel_l2_err": err_plain_tts,
+        "speedup_vs_plain": round(plain_tts / tts, 3) if plain_tts else None,
+        "gpu_step_s": {
+            "mixed_learned": round(wall_ml, 4),
+            "mixed_fixed_degree": round(wall_fast, 4),
+            "plain_fp64_symmetric": round(wall_plain, 4) if wall_plain else None,
+            "plain_fp64_one_directional": round(wall_plain1, 4) if wall_plain1 else None,
+        },
+        "gpu_step_speedup_vs_plain_fp64_symmetric": round(wall_plain / wall_ml, 3) if wall_plain else None,
+        "gpu_step_speedup_vs_plain_fp64_one_directional": round(wall_plain1 / wall_ml, 3) if wall_plain1 else None,
+        "rel_l2_err_gpu_step": {"mixed_learned": err_ml, "mixed_fixed_degree": err_fast,
+                                "plain_fp64": err_plain},
+        "mean_pair_degree": st_ml["mean_pair_degree"],
+        "far_evals_fixed": st["cc_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"],
+        "far_evals_learned": st_ml["far_ptwise_variable"],
+        "degree_setup_s": round(deg_setup, 3),
+        "phases_s": {k: round(st_ml[k], 4) for k in ("gpu_upward_s", "gpu_near_s", "gpu_near_sym_s",
+                                                      "gpu_far_s", "gpu_down_s", "gpu_finish_s")},
+        "sym_pairs": sym_evals,
+        "interactions": evals_learned,
+        "interactions_per_s": evals_learned / tts,
+        "interactions_per_s_gpu_step": evals_learned / wall_ml,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_reference_one_core"] = round(cpu["value"] / tts, 1)
+        if "all_cores" in cpu:
+            line["speedup_vs_cpu_all_cores"] = round(cpu["all_cores"]["value"] / tts, 1)
+    return line
 
 
 def main():
@@ -624,7 +967,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16_000_000)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline only: skip the other configurations' sub-objects")
     ap.add_argument("--workload", choices=["large", "batched", "learned", "yukawa", "cpu-case", "hermite",
                                            "kernels"],
                     default="large")
@@ -653,151 +998,23 @@ def main():
               "(a locally essential tree exchange between ranks) are not part of it.", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "batched":
-        bench_batched(args)
-        return
-    if args.workload == "learned":
-        bench_learned(args)
-        return
-    if args.workload == "yukawa":
-        bench_yukawa(args)
-        return
-    if args.workload == "cpu-case":
-        bench_cpu_case(args)
-        return
-    if args.workload == "hermite":
-        bench_hermite(args)
-        return
-    if args.workload == "kernels":
-        bench_kernels(args)
+    single = {"batched": bench_batched, "learned": bench_learned, "yukawa": bench_yukawa,
+              "cpu-case": bench_cpu_case, "hermite": bench_hermite, "kernels": bench_kernels}
+    if args.workload in single:
+        print(json.dumps(single[args.workload](args)), flush=True)
         return
 
-    import torch  # device setup and synchronisation only
-    assert torch.cuda.is_available(), "no GPU"
-    import barytree_amd as bt
-
-    n = args.n
-    log(f"[bench] generating N={n}")
-    x, y, z, q, w = gen_uniform(n, 5)
-    targets = (x.copy(), y.copy(), z.copy(), q.copy())
-    sources = (x, y, z, q, w)
-    kernel_args = (bt.Kernel.COULOMB, [0.5], bt.Singularity.SUBTRACTION,
-                   bt.Approximation.LAGRANGE, bt.ComputeType.CLUSTER_CLUSTER, 0.8, 10, 8000, 8000)
-    t0 = time.perf_counter()
-    plan = bt.Plan(targets, sources, *kernel_args, precision=bt.PRECISION_MIXED)
-    host_setup = time.perf_counter() - t0
-    st0 = plan.stats()
-    log(f"[bench] plan {host_setup:.2f}s (tree {st0['host_tree_s']:.2f} lists "
-        f"{st0['host_lists_s']:.2f} work {st0['host_work_s']:.2f} h2d {st0['h2d_s']:.2f})")
-
-    # fast form (the metric)
-    wall_fast, gpu_fast, near_fast, st = time_steps(plan, bt.PRECISION_MIXED, args.steps, args.warmup)
-    log(f"[bench] mixed: {wall_fast:.3f} s/step (gpu {gpu_fast:.3f}, near {near_fast:.3f})")
-    idx = np.arange(0, n, n // 1000)[:1000]
-    out_fast = plan.execute()
-    direct = plan.direct_sample(idx)
-    err_fast = rel_l2(direct, out_fast[idx])
-    log(f"[bench] mixed relL2 vs FP64 direct (1000 targets): {err_fast:.3e}")
-
-    # mixed, learned-degree form (the metric)
-    from barytree_amd import learned
-    t0 = time.perf_counter()
-    plan.use_model(learned.load_model("coulomb_cc"), args.delta)
-    deg_setup = time.perf_counter() - t0
-    wall_ml, gpu_ml, near_ml, st_ml = time_steps(plan, bt.PRECISION_MIXED, args.steps, args.warmup)
-    out_ml = plan.execute()
-    err_ml = rel_l2(direct, out_ml[idx])
-    log(f"[bench] mixed+learned: {wall_ml:.3f} s/step (gpu {gpu_ml:.3f}, far {st_ml['gpu_far_s']:.3f}, "
-        f"mean degree {st_ml['mean_pair_degree']:.2f}), relL2 {err_ml:.3e}, degree setup {deg_setup:.2f}s")
-    plan.set_pair_degrees(None)
-
-    # plain FP64 form on the same plan
-    wall_plain, err_plain = None, None
-    if args.plain_steps > 0:
-        wall_plain, gpu_plain, near_plain, _ = time_steps(plan, bt.PRECISION_FP64, args.plain_steps, 1)
-        out_plain = plan.execute()
-        err_plain = rel_l2(direct, out_plain[idx])
-        log(f"[bench] fp64: {wall_plain:.3f} s/step (gpu {gpu_plain:.3f}), relL2 {err_plain:.3e}")
-    plan.close()
-
-    evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
-    # dominant kernel: the symmetric near field (each leaf pair once for both
-    # particles); its work counted as the reference's, 17 flops per ordered
-    # pair evaluation it delivers
-    sym_evals = st["sym_ptwise"]
-    sym_s = st["gpu_near_sym_mean_s"]
-    achieved = FLOPS_PER_EVAL * sym_evals / sym_s / 1e12 if sym_s > 0 else 0.0
-    roofline = {
-        "bound": "valu",
-        "kernel": "k_near_sym<Coulomb> (symmetric near field, mixed precision)",
-        "achieved": round(achieved, 2),
-        "peak": PEAK["fp32"],
-        "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK["fp32"], 4),
-        "traffic": near_traffic() if n == 16_000_000 else None,
-        "kernel_s": round(sym_s, 4),
-        "ordered_pairs": sym_evals,
-        "flops_per_ordered_pair": FLOPS_PER_EVAL,
-    }
-    # the VALU issue time of the kernel's instruction mix (kinds priced alone)
-    mix = near_valu_mix() if n == 16_000_000 else None
-    if mix:
-        from barytree_amd.probe import probe_valu
-        probe = probe_valu()
-        simds = 4 * torch.cuda.get_device_properties(0).multi_processor_count
-        ceil_s = issue_model(mix, probe, simds)
-        roofline["valu_issue_model_s"] = round(ceil_s, 4)
-        roofline["valu_issue_model_over_measured"] = round(ceil_s / sym_s, 4)
-        log(f"[bench] symmetric near kernel {sym_s:.3f} s, VALU issue time of its instruction mix "
-            f"(kinds priced alone) {ceil_s:.3f} s; probe ns/instr {probe}")
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] timing the CPU oracle sample")
-        cpu = cpu_baseline(evals)
-        log(f"[bench] cpu baseline {cpu['value']:.1f} s (extrapolated)")
-
-    line = {
-        "metric": "time-to-solution N=16M at rel-L2 err <=1e-6 vs FP64 direct; speedup vs plain",
-        "value": round(wall_ml, 4),
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall_ml * 1e3, 2),
-        "higher_is_better": False,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "f32 pair kernel, f64 charges/accumulation",
-        "data": "synthetic",
-        "config": {
-            "workload": (f"Coulomb singularity subtraction kappa=0.5, N={n} uniform [-1,1]^3, "
-                         "cluster-cluster theta=0.8 degree 10 max leaf 8000"),
-            "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
-        },
-        "form": "mixed precision + learned per-pair degree (shipped coulomb_cc model)",
-        "rel_l2_err": err_ml,
-        "speedup_vs_plain": round(wall_plain / wall_ml, 3) if wall_plain else None,
-        "mixed_fixed_degree_s": round(wall_fast, 4),
-        "mixed_fixed_degree_rel_l2_err": err_fast,
-        "mean_pair_degree": st_ml["mean_pair_degree"],
-        "far_evals_fixed": st["cc_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"],
-        "far_evals_learned": st_ml["far_ptwise_variable"],
-        "degree_setup_s": round(deg_setup, 3),
-        "phases_s": {k: round(st_ml[k], 4) for k in ("gpu_upward_s", "gpu_near_s", "gpu_near_sym_s",
-                                                      "gpu_far_s", "gpu_down_s", "gpu_finish_s")},
-        "plain_fp64_s": round(wall_plain, 4) if wall_plain else None,
-        "plain_fp64_rel_l2_err": err_plain,
-        "gpu_time_s": round(gpu_fast, 4),
-        "near_phase_s": round(near_fast, 4),
-        "near_sym_kernel_s": round(sym_s, 4),
-        "sym_pairs": sym_evals,
-        "host_setup_s": round(host_setup, 3),
-        "end_to_end_s": round(host_setup + deg_setup + wall_ml, 3),
-        "interactions": evals,
-        "interactions_per_s": evals / wall_fast,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    line = bench_large(args)
+    if not args.no_configs:
+        # the other configurations, each at its own size with a few steps
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.n = 3, 1, 16_000_000
+        configs = {}
+        for key, fn in (("C1_cpu_case", bench_cpu_case), ("C2_yukawa", bench_yukawa),
+                        ("C3_batched", bench_batched), ("C4_learned", bench_learned)):
+            log(f"[bench] configuration {key}")
+            configs[key] = summary(fn(sub))
+        line["configs"] = configs
     print(json.dumps(line), flush=True)
 
 

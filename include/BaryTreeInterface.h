@@ -126,6 +126,9 @@ typedef struct BaryTreeStats {
     long long sym_slots;     /* FP64 partial sums the symmetric launch writes (8 bytes
                                 of device memory each, allocated at the first
                                 execution that uses them); 0 when sym_ptwise is 0 */
+    long long model_fallback_pairs; /* far-field pairs a degree model left at the
+                                fixed degree: their features lie outside the
+                                model's training range (variable degree only) */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
@@ -345,10 +348,20 @@ int BaryTreeModel_Gradient(const BaryTreeDegreeModel *model, int n, const float 
 int BaryTreeModel_Train(BaryTreeDegreeModel *model, int n, const float *features,
                         const int *labels, int epochs, int batch, float learningRate,
                         unsigned long long seed, float *epochLoss);
-/* degree = 3+k for the largest k with sigmoid(logit_k) >= delta, else 2;
- * logits (optional): 10 per sample */
+/* degree = 3+k for the largest k with sigmoid(logit_k) >= delta, else 2
+ * (0 < delta < 1, else an error); logits (optional): 10 per sample */
 int BaryTreeModel_Predict(const BaryTreeDegreeModel *model, int n, const float *features,
                           float delta, int *degrees, float *logits);
+/* The range of every feature over the model's training data
+ * (BARYTREE_PAIR_FEATURES floats each).  Training widens it to cover its
+ * data; BaryTreeModel_Save/Load keep it (files written before it was
+ * recorded load without one).  A pair whose features lie outside it by more
+ * than 5% of its width keeps the plan's fixed degree
+ * (BaryTreePlan_UseDegreeModel, BaryTree_SolveLearned; counted in
+ * model_fallback_pairs).  Get returns 1 if the model has a range, else 0
+ * (then there is no guard); Set with lo or hi NULL removes it. */
+int BaryTreeModel_GetFeatureRange(const BaryTreeDegreeModel *model, float *lo, float *hi);
+int BaryTreeModel_SetFeatureRange(BaryTreeDegreeModel *model, const float *lo, const float *hi);
 void BaryTreeModel_Destroy(BaryTreeDegreeModel *model);
 /* features -> model -> SetPairDegrees */
 int BaryTreePlan_UseDegreeModel(BaryTreePlan *plan, const BaryTreeDegreeModel *model, float delta);

@@ -31,6 +31,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "bt_oracle.h"
 
@@ -586,24 +589,30 @@ static int clusters_sources(Clus *c, const Tree *t, const Parts *s, const Prm *p
         if (!qw) return ORC_ERR_NOMEM;
         for (int i = 0; i < s->n; i++) qw[i] = s->q[i] * s->w[i];
     }
-    for (int li = 0; li < t->nleaves && !e; li++) {
+    /* leaves, then parents level by level: every node's grid is written by
+       one thread, its children in order (errors: the least code) */
+#pragma omp parallel for schedule(dynamic) reduction(min : e)
+    for (int li = 0; li < t->nleaves; li++) {
         int idx = t->leaves[li];
         int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
         size_t cs = (size_t)idx * P;
-        e = anterp(n, t->xmin[idx], t->xmax[idx], t->ymin[idx], t->ymax[idx], t->zmin[idx], t->zmax[idx],
-                   m, s->x + b, s->y + b, s->z + b, sub ? qw + b : s->q + b, sub ? s->w + b : NULL,
-                   c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL);
+        int r = anterp(n, t->xmin[idx], t->xmax[idx], t->ymin[idx], t->ymax[idx], t->zmin[idx], t->zmax[idx],
+                       m, s->x + b, s->y + b, s->z + b, sub ? qw + b : s->q + b, sub ? s->w + b : NULL,
+                       c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL);
+        if (r < e) e = r;
     }
     for (int level = t->max_depth - 1; level >= 0 && !e; --level) {
-        for (int ci = 0; ci < t->lev_n[level] && !e; ++ci) {
+#pragma omp parallel for schedule(dynamic) reduction(min : e)
+        for (int ci = 0; ci < t->lev_n[level]; ++ci) {
             int par = t->lev[level][ci];
             size_t ps = (size_t)par * P;
-            for (int k = 0; k < t->nchild[par] && !e; ++k) {
+            for (int k = 0; k < t->nchild[par]; ++k) {
                 int ch = t->child[8 * par + k];
                 size_t cs = (size_t)ch * P;
-                e = anterp(n, t->xmin[par], t->xmax[par], t->ymin[par], t->ymax[par], t->zmin[par], t->zmax[par],
-                           P, c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL,
-                           c->x + ps, c->y + ps, c->z + ps, c->q + ps, sub ? c->w + ps : NULL);
+                int r = anterp(n, t->xmin[par], t->xmax[par], t->ymin[par], t->ymax[par], t->zmin[par],
+                               t->zmax[par], P, c->x + cs, c->y + cs, c->z + cs, c->q + cs, sub ? c->w + cs : NULL,
+                               c->x + ps, c->y + ps, c->z + ps, c->q + ps, sub ? c->w + ps : NULL);
+                if (r < e) e = r;
             }
         }
     }
@@ -794,6 +803,7 @@ static void k_pp(const Prm *prm, int nt, int ns, int ts, int ss, const Parts *T,
     int sub = prm->singularity == SNG_SUB;
     if (other_kernel(prm->kernel)) {
         double wadj = prm->kernel == KER_ATAN ? atan_wadj(prm) : 1.0;
+#pragma omp parallel for schedule(static) if (nt >= 256)
         for (int i = 0; i < nt; i++) {
             int ii = ts + i;
             double temporary_potential = 0.0;
@@ -806,6 +816,7 @@ static void k_pp(const Prm *prm, int nt, int ns, int ts, int ss, const Parts *T,
         }
         return;
     }
+#pragma omp parallel for schedule(static) if (nt >= 256)
     for (int i = 0; i < nt; i++) {
         int ii = ts + i;
         double temporary_potential = 0.0;
@@ -858,6 +869,7 @@ static void k_pc(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *
     int sub = prm->singularity == SNG_SUB;
     if (other_kernel(prm->kernel)) {
         double wadj = prm->kernel == KER_ATAN ? atan_wadj(prm) : 1.0;
+#pragma omp parallel for schedule(static) if (nt >= 256)
         for (int i = 0; i < nt; i++) {
             int ii = ts + i;
             double temporary_potential = 0.0;
@@ -870,6 +882,7 @@ static void k_pc(const Prm *prm, int nt, int P, int ts, size_t cs, const Parts *
         }
         return;
     }
+#pragma omp parallel for schedule(static) if (nt >= 256)
     for (int i = 0; i < nt; i++) {
         int ii = ts + i;
         double temporary_potential = 0.0;
@@ -915,6 +928,7 @@ static void k_cp(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *
 {
     int sub = prm->singularity == SNG_SUB;
     if (other_kernel(prm->kernel)) {
+#pragma omp parallel for schedule(static) if (P >= 256)
         for (int i = 0; i < P; i++) {
             double temporary_potential = 0.0;
             for (int j = 0; j < ns; j++) {
@@ -926,6 +940,7 @@ static void k_cp(const Prm *prm, int ns, int P, int ss, size_t cs, const Parts *
         }
         return;
     }
+#pragma omp parallel for schedule(static) if (P >= 256)
     for (int i = 0; i < P; i++) {
         double temporary_potential = 0.0, temporary_weight = 0.0;
         double cx = C->x[cs + i], cy = C->y[cs + i], cz = C->z[cs + i];
@@ -969,6 +984,7 @@ static void k_cc(const Prm *prm, int P, size_t ss, size_t ts, const Clus *S, Clu
 {
     int sub = prm->singularity == SNG_SUB;
     if (other_kernel(prm->kernel)) {
+#pragma omp parallel for schedule(static) if (P >= 256)
         for (int i = 0; i < P; i++) {
             double temporary_potential = 0.0;
             for (int j = 0; j < P; j++) {
@@ -980,6 +996,7 @@ static void k_cc(const Prm *prm, int P, size_t ss, size_t ts, const Clus *S, Clu
         }
         return;
     }
+#pragma omp parallel for schedule(static) if (P >= 256)
     for (int i = 0; i < P; i++) {
         double temporary_potential = 0.0, temporary_weight = 0.0;
         double cx = C->x[ts + i], cy = C->y[ts + i], cz = C->z[ts + i];
@@ -1085,7 +1102,9 @@ static void downpass(const Tree *t, Clus *c, const Parts *T, const Prm *prm, dou
 {
     int P = prm->P, n = prm->degree;
     int sub = prm->singularity == SNG_SUB;
+    /* parents write their own children's grids, leaves their own targets */
     for (int level = 0; level < t->max_depth; ++level) {
+#pragma omp parallel for schedule(dynamic)
         for (int ci = 0; ci < t->lev_n[level]; ++ci) {
             int par = t->lev[level][ci];
             size_t ps = (size_t)par * P;
@@ -1098,6 +1117,7 @@ static void downpass(const Tree *t, Clus *c, const Parts *T, const Prm *prm, dou
             }
         }
     }
+#pragma omp parallel for schedule(dynamic)
     for (int li = 0; li < t->nleaves; ++li) {
         int idx = t->leaves[li];
         int b = t->ibeg[idx] - 1, m = t->iend[idx] - t->ibeg[idx] + 1;
@@ -1617,10 +1637,12 @@ int orc_treecode(int M, int N,
             for (int j = 0; j < L.ap[i].n; j++) {
                 if (herm) k_pc_hermite(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
                 else k_pc(&prm, nt, P, bs, (size_t)P * L.ap[i].v[j], &T, &sc, pot);
+                g_stats.far_done += (long long)nt * P;
             }
             for (int j = 0; j < L.pp[i].n; j++) {
                 int nd = L.pp[i].v[j];
                 k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, bs, src.ibeg[nd] - 1, &T, &S, pot);
+                g_stats.near_done += (long long)nt * (src.iend[nd] - src.ibeg[nd] + 1);
             }
         }
         g_stats.t_compute = wall() - t0; t0 = wall();
@@ -1639,10 +1661,12 @@ int orc_treecode(int M, int N,
             for (int j = 0; j < L.ap[i].n; j++) {
                 if (herm) k_cp_hermite(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
                 else k_cp(&prm, ns, P, bs, (size_t)P * L.ap[i].v[j], &S, &tc);
+                g_stats.far_done += (long long)ns * P;
             }
             for (int j = 0; j < L.pp[i].n; j++) {
                 int nd = L.pp[i].v[j];
                 k_pp(&prm, tgt.iend[nd] - tgt.ibeg[nd] + 1, ns, tgt.ibeg[nd] - 1, bs, &T, &S, pot);
+                g_stats.near_done += (long long)ns * (tgt.iend[nd] - tgt.ibeg[nd] + 1);
             }
         }
         g_stats.t_compute = wall() - t0; t0 = wall();
@@ -1669,13 +1693,16 @@ int orc_treecode(int M, int N,
                 k_cc(&prm, P, (size_t)P * L.ap[i].v[j], tcs, &sc, &tc);
             for (int j = 0; j < L.spc[i].n; j++)
                 k_pc(&prm, nt, P, ts, (size_t)P * L.spc[i].v[j], &T, &sc, pot);
+            g_stats.far_done += (long long)L.ap[i].n * P * P + (long long)L.spc[i].n * nt * P;
             for (int j = 0; j < L.tcp[i].n; j++) {
                 int nd = L.tcp[i].v[j];
                 k_cp(&prm, src.iend[nd] - src.ibeg[nd] + 1, P, src.ibeg[nd] - 1, tcs, &S, &tc);
+                g_stats.far_done += (long long)P * (src.iend[nd] - src.ibeg[nd] + 1);
             }
             for (int j = 0; j < L.pp[i].n; j++) {
                 int nd = L.pp[i].v[j];
                 k_pp(&prm, nt, src.iend[nd] - src.ibeg[nd] + 1, ts, src.ibeg[nd] - 1, &T, &S, pot);
+                g_stats.near_done += (long long)nt * (src.iend[nd] - src.ibeg[nd] + 1);
             }
         }
         g_stats.t_compute = wall() - t0; t0 = wall();
@@ -1900,6 +1927,27 @@ int orc_direct(int M, int N,
 void orc_last_stats(orc_stats *s) { *s = g_stats; }
 
 void orc_set_sample_stride(int stride) { g_stride = stride > 0 ? stride : 1; }
+
+/* Threads of the parallel loops (pair kernels over their targets or grid
+ * points, upward pass and downpass over nodes); each target's or grid
+ * point's sum keeps its serial order, so results do not depend on it. */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 int orc_source_clusters(int n, const double *x, const double *y, const double *z,
                         const double *q, const double *w, int degree, int max_leaf,

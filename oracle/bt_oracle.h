@@ -39,6 +39,9 @@ typedef struct {
     double theta;
     int max_per_source_leaf, max_per_target_leaf;
     double t_setup, t_upward, t_lists, t_compute, t_downpass, t_finish;
+    /* point-wise evaluations the interaction phase computed (all of them
+       unless a sample stride is set): near field, far field */
+    long long near_done, far_done;
 } orc_stats;
 
 /* BaryTreeInterface semantics (src/interface/BaryTreeInterface.c:18-62) with
@@ -71,6 +74,9 @@ void orc_last_stats(orc_stats *s);
  * phase (the upward pass and the downpass still run in full).  Used only to
  * time a bounded sample of a large problem; 1 restores the full sum. */
 void orc_set_sample_stride(int stride);
+/* threads of the oracle's parallel loops (results do not depend on it) */
+void orc_set_threads(int n);
+int orc_max_threads(void);
 
 /* RunParams_Setup semantics (src/run_params/run_params.c:13-131): writes
  * theta, degree, size_check_factor, source and target leaf sizes. */

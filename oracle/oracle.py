@@ -28,7 +28,8 @@ class Stats(ctypes.Structure):
         ("theta", ctypes.c_double),
         ("max_per_source_leaf", ctypes.c_int), ("max_per_target_leaf", ctypes.c_int)] + [
         (n, ctypes.c_double) for n in (
-            "t_setup", "t_upward", "t_lists", "t_compute", "t_downpass", "t_finish")]
+            "t_setup", "t_upward", "t_lists", "t_compute", "t_downpass", "t_finish")] + [
+        ("near_done", ctypes.c_longlong), ("far_done", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -136,6 +137,40 @@ def last_stats():
 
 def set_sample_stride(stride):
     lib().orc_set_sample_stride(ctypes.c_int(int(stride)))
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (also those the reference-parts
+    library runs through the restatement); results do not depend on it."""
+    lib().orc_set_threads(ctypes.c_int(int(n)))
+    if ref_available():
+        ref_lib().orc_set_threads(ctypes.c_int(int(n)))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
+
+
+class RefTiming(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "t_setup", "t_upward", "t_lists", "t_compute", "t_downpass", "t_finish")] + [
+        (n, ctypes.c_longlong) for n in ("near_done", "far_done", "near_total", "far_total")]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+def ref_set_sample_stride(stride):
+    """oracle/_ref: compute only every stride-th key's interaction lists."""
+    ref_lib().ref_set_sample_stride(ctypes.c_int(int(stride)))
+
+
+def ref_last_timing():
+    """Phase times and point-wise counts (computed and full) of the last
+    ref_treecode call."""
+    t = RefTiming()
+    ref_lib().ref_last_timing(ctypes.byref(t))
+    return t.as_dict()
 
 
 def source_clusters(x, y, z, q, w, degree, max_leaf, singularity, approximation=1):

@@ -230,6 +230,53 @@ static struct Particles *h_parts(int n, const double *x, const double *y, const 
     return p;
 }
 
+/* Phase times of the last ref_treecode call (seconds) and its point-wise
+ * evaluation counts: the computed ones (a sample when a stride is set) and
+ * those of the full lists.  With stride k > 1 only every k-th key (target
+ * batch for PC, source batch for CP, target node for CC) keeps its
+ * interaction lists, so the compute phase runs a sample of the treecode's
+ * evaluations; the trees, clusters, lists and downpass run in full. */
+typedef struct {
+    double t_setup, t_upward, t_lists, t_compute, t_downpass, t_finish;
+    long long near_done, far_done, near_total, far_total;
+} ref_timing;
+static __thread ref_timing g_rt;
+static __thread int g_ref_stride = 1;
+
+void ref_set_sample_stride(int stride) { g_ref_stride = stride > 0 ? stride : 1; }
+void ref_last_timing(ref_timing *t) { *t = g_rt; }
+
+/* point-wise evaluations of key i's lists (treedriver.c:191-200, 472-483,
+ * 763-792), and the sampling cut of the lists */
+static void h_count_and_sample(struct InteractionLists *L, int nkeys, const struct Tree *keys,
+                               const struct Tree *ent, int P, int ct)
+{
+    for (int i = 0; i < nkeys; i++) {
+        long long nn = 0, nf = 0;
+        const long long nk = keys->numpar[i];
+        for (int j = 0; j < L->num_pp[i]; j++) nn += nk * ent->numpar[L->pp_interactions[i][j]];
+        /* PC lists only num_pc, CP only num_cp (keys: source batches), CC all */
+        if (ct == CT_CC) {
+            nf += (long long)L->num_cc[i] * P * P;
+            nf += (long long)L->num_pc[i] * nk * P;
+            for (int j = 0; j < L->num_cp[i]; j++) nf += (long long)P * ent->numpar[L->cp_interactions[i][j]];
+        } else if (ct == CT_PC) {
+            nf += (long long)L->num_pc[i] * nk * P;
+        } else {
+            nf += (long long)L->num_cp[i] * nk * P;
+        }
+        g_rt.near_total += nn; g_rt.far_total += nf;
+        if (i % g_ref_stride) {
+            L->num_pp[i] = 0;
+            if (L->num_cc) L->num_cc[i] = 0;
+            if (L->num_pc) L->num_pc[i] = 0;
+            if (L->num_cp) L->num_cp[i] = 0;
+        } else {
+            g_rt.near_done += nn; g_rt.far_done += nf;
+        }
+    }
+}
+
 int ref_treecode(int M, int N,
                  const double *tx, const double *ty, const double *tz, const double *tq,
                  const double *sx, const double *sy, const double *sz, const double *sq,
@@ -259,30 +306,50 @@ int ref_treecode(int M, int N,
     double *pot = dvec(M);
     struct InteractionLists *L = NULL;
 
+    memset(&g_rt, 0, sizeof(g_rt));
+    double t0 = wall();
+    /* the phases of treedriver.c:114-967 in its order */
     if (compute_type == CT_PC) {
         struct Tree *tree = h_tree_construct(S, rp->max_per_source_leaf, 1);
         struct Tree *batches = h_batches(T, rp->max_per_target_leaf, 0);
+        g_rt.t_setup = wall() - t0; t0 = wall();
         struct Clusters *cl = h_clusters(tree, S, &prm, 1);
+        g_rt.t_upward = wall() - t0; t0 = wall();
         InteractionLists_Make(&L, tree, batches, rp);
+        h_count_and_sample(L, batches->numnodes, batches, tree, prm.P, compute_type);
+        g_rt.t_lists = wall() - t0; t0 = wall();
         InteractionCompute_PC(pot, tree, batches, L, S, T, cl, rp);
+        g_rt.t_compute = wall() - t0; t0 = wall();
     } else if (compute_type == CT_CP) {
         struct Tree *tree = h_tree_construct(T, rp->max_per_target_leaf, 0);
         struct Tree *batches = h_batches(S, rp->max_per_source_leaf, 1);
         struct Clusters *cl = h_clusters(tree, T, &prm, 0);
+        g_rt.t_setup = wall() - t0; t0 = wall();
         InteractionLists_Make(&L, tree, batches, rp);
+        h_count_and_sample(L, batches->numnodes, batches, tree, prm.P, compute_type);
+        g_rt.t_lists = wall() - t0; t0 = wall();
         InteractionCompute_CP(pot, tree, batches, L, S, T, cl, rp);
+        g_rt.t_compute = wall() - t0; t0 = wall();
         InteractionCompute_Downpass(pot, tree, T, cl, rp);
+        g_rt.t_downpass = wall() - t0; t0 = wall();
     } else {
         struct Tree *stree = h_tree_construct(S, rp->max_per_source_leaf, 1);
         struct Tree *ttree = h_tree_construct(T, rp->max_per_target_leaf, 0);
+        g_rt.t_setup = wall() - t0; t0 = wall();
         struct Clusters *scl = h_clusters(stree, S, &prm, 1);
         struct Clusters *tcl = h_clusters(ttree, T, &prm, 0);
+        g_rt.t_upward = wall() - t0; t0 = wall();
         InteractionLists_Make(&L, stree, ttree, rp);
+        h_count_and_sample(L, ttree->numnodes, ttree, stree, prm.P, compute_type);
+        g_rt.t_lists = wall() - t0; t0 = wall();
         InteractionCompute_CC(pot, stree, ttree, L, S, T, scl, tcl, rp);
+        g_rt.t_compute = wall() - t0; t0 = wall();
         InteractionCompute_Downpass(pot, ttree, T, tcl, rp);
+        g_rt.t_downpass = wall() - t0; t0 = wall();
     }
     InteractionCompute_SubtractionPotentialCorrection(pot, T, rp);
     for (int i = 0; i < M; i++) out[T->order[i] - 1] = pot[i];
+    g_rt.t_finish = wall() - t0;
     /* Test harness: the small allocations above are left to process exit. */
     return 0;
 }

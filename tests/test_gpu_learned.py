@@ -210,3 +210,44 @@ def test_trained_network_gain_over_fixed_degree(bt):
     assert ev_learned < 0.7 * ev_fixed
     assert e_learned <= 1e-6
     assert d_eq is not None and e_learned < e_eq
+
+
+def test_model_guard_falls_back_outside_the_training_range(bt):
+    """Pairs whose features lie outside the model's training range (by more
+    than 5% of its width) keep the fixed degree: the count equals the pairs
+    outside the shipped range computed here from the plan's features; with a
+    range no pair fits, the learned run equals the fixed-degree run to
+    rounding; without a range there is no guard."""
+    from barytree_amd import learned
+    print("case: regularized Coulomb CP Plummer 200k, shipped model with its range, then an empty range",
+          flush=True)
+    cfg = dict(learned.MODELS["regularized_coulomb_cp"], leaf=1000)
+    m = learned.load_model("regularized_coulomb_cp")
+    rng_lo, rng_hi = m.feature_range()
+    pts = learned.particles("plummer", 200_000, 77)
+    with learned.plan_for(cfg, *pts) as p:
+        fixed = p.execute()
+        n = p.num_pairs()
+        f = p.pair_features()
+        mg = 0.05 * (rng_hi - rng_lo)
+        outside = int(np.sum(np.any(~((f >= rng_lo - mg) & (f <= rng_hi + mg)), axis=1)))
+        p.use_model(m, 0.5)
+        st = p.stats()
+        assert st["variable_degree"] == 1
+        print(f"pairs {n}, outside the shipped range {outside}", flush=True)
+        assert st["model_fallback_pairs"] == outside
+        # a range far from every feature: all pairs keep the fixed degree
+        m.set_feature_range(rng_lo + 100.0, rng_hi + 100.0)
+        p.use_model(m, 0.5)
+        st = p.stats()
+        assert st["model_fallback_pairs"] == n
+        assert st["mean_pair_degree"] == cfg["degree"]
+        assert rel_l2(fixed, p.execute()) <= 1e-12
+        # no range: no guard
+        m.set_feature_range(None, None)
+        assert m.feature_range() is None
+        p.use_model(m, 0.5)
+        assert p.stats()["model_fallback_pairs"] == 0
+        for bad in (0.0, 1.0, -0.5, float("nan")):
+            with pytest.raises(RuntimeError, match="delta"):
+                p.use_model(m, bad)

@@ -287,3 +287,29 @@ def test_symmetric_near_field_eligibility(bt):
     other = (x.copy(), y.copy(), z.copy(), q.copy())
     other[0][17] += 1e-12
     assert bt.analyze(other, src, 1, [0.5], 2, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # positions
+
+
+def test_model_feature_range_round_trip(bt, tmp_path):
+    """The training range is saved and loaded with the model; a file of the
+    earlier layout (no range) loads without one; bad ranges are refused."""
+    import struct
+    m = bt.DegreeModel(seed=3)
+    assert m.feature_range() is None
+    lo = np.linspace(-1, 0, 8).astype(np.float32)
+    hi = lo + np.arange(1, 9, dtype=np.float32)
+    m.set_feature_range(lo, hi)
+    path = tmp_path / "m.bin"
+    m.save(path)
+    r = bt.DegreeModel(path=path).feature_range()
+    assert r is not None and np.array_equal(r[0], lo) and np.array_equal(r[1], hi)
+    with pytest.raises(RuntimeError, match="lo <= hi"):
+        m.set_feature_range(hi, lo)
+    # the earlier layout: magic ...01, header, steps, mean, scale, params
+    p, mean, scale = m.get_params()
+    old = tmp_path / "old.bin"
+    with open(old, "wb") as f:
+        f.write(b"BTDMLP01" + struct.pack("4i", 8, 64, 10, len(p)) + struct.pack("i", 0))
+        f.write(mean.tobytes() + scale.tobytes() + p.tobytes())
+    o = bt.DegreeModel(path=old)
+    assert o.feature_range() is None
+    assert np.array_equal(o.get_params()[0], p)
