@@ -14,6 +14,7 @@
 #include "../../include/BaryTreeInterface.h"
 #include "bt_learn.h"
 #include "bt_model.h"
+#include "bt_noalloc.h"
 #include "bt_solver.h"
 
 struct BaryTreePlan {
@@ -513,6 +514,21 @@ int BaryTreePlan_DirectSample(BaryTreePlan *plan, int m, const int *targetIndex,
     std::string e = plan->plan.direct(idx, res);
     if (!e.empty()) return fail(e);
     std::memcpy(out, res.data(), (size_t)m * sizeof(double));
+    return 0;
+}
+
+int BaryTreePlan_DirectSampleNoAlloc(BaryTreePlan *plan, int m, const int *targetIndex, double *out)
+{
+    g_last_error.clear();
+    if (!plan || !targetIndex || !out || m < 0) return fail("bad argument");
+    if (m > btk::NOALLOC_MAX_SAMPLE)
+        return fail("BaryTreePlan_DirectSampleNoAlloc takes at most 8192 targets (the size of its scratch)");
+    void *scratch = nullptr;
+    size_t bytes = 0;
+    hipError_t he = btk::noalloc_scratch(&scratch, &bytes);
+    if (he != hipSuccess) return fail(std::string("scratch lookup failed: ") + hipGetErrorString(he));
+    std::string e = plan->plan.direct_noalloc(m, targetIndex, out, scratch, bytes);
+    if (!e.empty()) return fail(e);
     return 0;
 }
 

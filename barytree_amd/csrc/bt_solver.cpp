@@ -2,6 +2,7 @@
 #include "bt_solver.h"
 
 #include "bt_devtree.h"
+#include "bt_noalloc.h"
 #include "bt_tree.h"
 
 #include <algorithm>
@@ -1324,6 +1325,69 @@ std::string Plan::direct(const std::vector<int> &idx, std::vector<double> &out)
     }
     dfree(dinv);
     if (dw != d_sw_) dfree(dw);
+    return "";
+}
+
+// Caller indices sorted (with their slots) on the host; per system, its
+// sorted subrange is found in tree order on the device (k_find_sel over the
+// system's positions), gathered, and summed by the kernels direct() uses with
+// the same chunking, so the results are the same bit for bit.  Scratch
+// layout (each part 256-byte aligned): sel (m ints), pos (m ints), target
+// x, y, z, q and the sums (m doubles each), the partial sums.
+std::string Plan::direct_noalloc(int m, const int *idx, double *out, void *scratch, size_t bytes)
+{
+    if (m < 0 || m > btk::NOALLOC_MAX_SAMPLE) return "at most 8192 targets (the scratch's size)";
+    for (int k = 0; k < m; k++)
+        if (idx[k] < 0 || idx[k] >= M_) return "target index out of range";
+    if (m == 0) return "";
+    if (!st_) return "the plan has no GPU work (host-only plan)";
+    // host work in fixed per-thread buffers: (index, slot) pairs sorted by index
+    static thread_local std::array<std::pair<int, int>, btk::NOALLOC_MAX_SAMPLE> srt;
+    static thread_local std::array<int, btk::NOALLOC_MAX_SAMPLE> sel;
+    static thread_local std::array<double, btk::NOALLOC_MAX_SAMPLE> res;
+    for (int k = 0; k < m; k++) srt[k] = {idx[k], k};
+    std::sort(srt.begin(), srt.begin() + m);
+    for (int k = 0; k < m; k++) sel[k] = srt[k].first;
+    auto carve = [&](size_t &at, size_t n) {
+        char *p = static_cast<char *>(scratch) + at;
+        at += (n + 255) / 256 * 256;
+        return p;
+    };
+    size_t at = 0;
+    int *d_sel = reinterpret_cast<int *>(carve(at, (size_t)m * sizeof(int)));
+    int *d_pos = reinterpret_cast<int *>(carve(at, (size_t)m * sizeof(int)));
+    double *dx = reinterpret_cast<double *>(carve(at, (size_t)m * sizeof(double)));
+    double *dy = reinterpret_cast<double *>(carve(at, (size_t)m * sizeof(double)));
+    double *dz = reinterpret_cast<double *>(carve(at, (size_t)m * sizeof(double)));
+    double *dq = reinterpret_cast<double *>(carve(at, (size_t)m * sizeof(double)));
+    double *dout = reinterpret_cast<double *>(carve(at, (size_t)m * sizeof(double)));
+    const size_t part_at = at;
+    HIPCHK(hipMemcpyAsync(d_sel, sel.data(), (size_t)m * sizeof(int), hipMemcpyHostToDevice, st_));
+    HIPCHK(btk::launch_fill_neg(m, d_pos, st_));
+    int k0 = 0;
+    for (const System &y : sys_) {
+        // this system's part of the sorted indices
+        int k1 = k0;
+        while (k1 < m && sel[k1] < y.toff + y.M) k1++;
+        const int mm = k1 - k0;
+        if (mm > 0) {
+            const int tblocks = (mm + 255) / 256;
+            const int nchunks = std::max(1, std::min((2048 + tblocks - 1) / tblocks, (y.N + 4095) / 4096));
+            if (part_at + (size_t)nchunks * mm * sizeof(double) > bytes) return "scratch too small";
+            double *dpart = reinterpret_cast<double *>(static_cast<char *>(scratch) + part_at);
+            HIPCHK(btk::launch_find_sel(y.toff, y.toff + y.M, d_torder_, mm, d_sel + k0, d_pos + k0, st_));
+            HIPCHK(btk::launch_gather_pos(mm, d_pos + k0, d_tx_, d_ty_, d_tz_, d_tq_, dx + k0, dy + k0, dz + k0,
+                                          dq + k0, st_));
+            // without subtraction the kernels read no weights
+            HIPCHK(btk::launch_direct(prm_.kernel, prm_.singularity, kp_, mm, dx + k0, dy + k0, dz + k0, dq + k0,
+                                      y.N, d_sx_ + y.soff, d_sy_ + y.soff, d_sz_ + y.soff, d_sq_ + y.soff,
+                                      d_sw_ ? d_sw_ + y.soff : nullptr, dpart, nchunks, dout + k0, st_));
+        }
+        k0 = k1;
+    }
+    HIPCHK(hipMemcpyAsync(res.data(), dout, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    for (int k = 0; k < m; k++) out[srt[k].second] = res[k];
     return "";
 }
 

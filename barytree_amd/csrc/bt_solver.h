@@ -121,6 +121,10 @@ public:
     // FP64 direct sum on the GPU for caller-order target indices `idx`
     // (each target against the sources of its own system).
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
+    // The same sums without device allocation: every device buffer lies in
+    // the scratch [scratch, scratch + bytes) (m <= NOALLOC_MAX_SAMPLE);
+    // no host heap either.  Results equal direct()'s bit for bit.
+    std::string direct_noalloc(int m, const int *idx, double *out, void *scratch, size_t bytes);
 
     void set_precision(int p) { prm_.precision = p; }
 

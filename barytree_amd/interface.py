@@ -125,7 +125,8 @@ def lib():
         l.BaryTree_GetLastError.restype = ctypes.c_char_p
         for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
-                  "BaryTreePlan_DirectSample", "BaryTreePlan_SetPrecision", "BaryTreePlan_SetSymmetric",
+                  "BaryTreePlan_DirectSample", "BaryTreePlan_DirectSampleNoAlloc", "BaryTreePlan_SetPrecision",
+                  "BaryTreePlan_SetSymmetric",
                   "BaryTree_SolveBatched", "BaryTreePlan_CreateBatched", "BaryTree_Autotune",
                   "BaryTree_AnalyzeBatched", "BaryTreePlan_NumPairs", "BaryTreePlan_PairInfo",
                   "BaryTreePlan_PairFeatures", "BaryTreePlan_LabelPairs",

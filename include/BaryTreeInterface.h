@@ -193,6 +193,16 @@ int BaryTreePlan_GetStats(const BaryTreePlan *plan, BaryTreeStats *stats);
  * src/drivers/directdriver.c:23-122) at m targets given by caller-order
  * indices; for error checks of the plan's result. */
 int BaryTreePlan_DirectSample(BaryTreePlan *plan, int m, const int *targetIndex, double *out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  BaryTreePlan_DirectSample without any device
+ * allocation: its device buffers live in one fixed scratch array per GPU
+ * (5 MiB, held until the process ends), so m is at most 8192 (larger m is
+ * an error, nothing computed).  Results equal BaryTreePlan_DirectSample's
+ * bit for bit.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (other host threads) race and must be serialised by the
+ * caller. */
+int BaryTreePlan_DirectSampleNoAlloc(BaryTreePlan *plan, int m, const int *targetIndex, double *out);
 void BaryTreePlan_Destroy(BaryTreePlan *plan);
 
 /* ---- Batching: many independent systems in one set of launches -------
