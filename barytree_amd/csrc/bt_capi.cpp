@@ -435,6 +435,15 @@ int BaryTree_Solve(int numTargets, int numSources, const double *targetX, const 
 
 const char *BaryTree_GetLastError(void) { return g_last_error.c_str(); }
 
+int BaryTree_SetBetaLeafSizes(int which)
+{
+    g_last_error.clear();
+    if (which != BARYTREE_BETA_LEAVES_CPU && which != BARYTREE_BETA_LEAVES_GPU)
+        return fail("which must be BARYTREE_BETA_LEAVES_CPU or BARYTREE_BETA_LEAVES_GPU");
+    bt::set_beta_leaf_mode(which);
+    return 0;
+}
+
 int BaryTree_GetLastStats(BaryTreeStats *stats)
 {
     if (!stats || !g_have_stats) return -1;
@@ -741,20 +750,22 @@ int BaryTreePlan_UseDegreeModel(BaryTreePlan *plan, const BaryTreeDegreeModel *m
     return 0;
 }
 
-int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
-                        const int *targetCount, const int *sourceBegin, const int *sourceCount,
-                        const double *targetBox, const double *sourceBox, int numTargets,
-                        const double *targetX, const double *targetY, const double *targetZ,
-                        int numSources, const double *sourceX, const double *sourceY,
-                        const double *sourceZ, const double *sourceValue, int kernel,
-                        double kernelParam, double tol, int *labels, double *errors)
+// Labels of explicitly given pairs; ss: the subtraction kernel (target
+// charges tq, source weights sw, p1 the Gaussian's 1/width^2).
+static int label_explicit(int numPairs, const int *kind, const int *targetBegin, const int *targetCount,
+                          const int *sourceBegin, const int *sourceCount, const double *targetBox,
+                          const double *sourceBox, int numTargets, const double *targetX, const double *targetY,
+                          const double *targetZ, const double *tq, int numSources, const double *sourceX,
+                          const double *sourceY, const double *sourceZ, const double *sourceValue,
+                          const double *sw, int kernel, double kernelParam, int ss, double p1, double tol,
+                          int *labels, double *errors)
 {
     g_last_error.clear();
     if (numPairs < 0 || !labels) return fail("bad argument");
     if (numPairs == 0) return 0;
     if (!kind || !targetBegin || !targetCount || !sourceBegin || !sourceCount || !targetBox ||
         !sourceBox || !targetX || !targetY || !targetZ || !sourceX || !sourceY || !sourceZ ||
-        !sourceValue)
+        !sourceValue || (ss && (!tq || !sw)))
         return fail("NULL argument");
     if (kernel < 1 || kernel > 3) return fail("INVALID KERNEL (supported: COULOMB, YUKAWA, REGULARIZED_COULOMB)");
     for (int i = 0; i < numPairs; i++) {
@@ -790,11 +801,17 @@ int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
     a.sy = (const double *)put(sourceY, (size_t)numSources * sizeof(double));
     a.sz = (const double *)put(sourceZ, (size_t)numSources * sizeof(double));
     a.sq = (const double *)put(sourceValue, (size_t)numSources * sizeof(double));
+    if (ss) {
+        a.ss = 1;
+        a.tq = (const double *)put(tq, (size_t)numTargets * sizeof(double));
+        a.sw = (const double *)put(sw, (size_t)numSources * sizeof(double));
+        a.p1 = p1;
+    }
     void *dl = nullptr, *de = nullptr;
     if (hipMalloc(&dl, np * sizeof(int)) == hipSuccess) tmp.push_back(dl); else dl = nullptr;
     if (hipMalloc(&de, np * btl::LABEL_ND * sizeof(double)) == hipSuccess) tmp.push_back(de); else de = nullptr;
     if (!a.kind || !a.tbeg || !a.tcnt || !a.sbeg || !a.scnt || !a.tbox || !a.sbox || !a.tx || !a.ty ||
-        !a.tz || !a.sx || !a.sy || !a.sz || !a.sq || !dl || !de) {
+        !a.tz || !a.sx || !a.sy || !a.sz || !a.sq || !dl || !de || (ss && (!a.tq || !a.sw))) {
         cleanup();
         return fail("device buffers for labels failed");
     }
@@ -810,6 +827,47 @@ int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
     cleanup();
     if (he != hipSuccess) return fail(std::string("label kernel: ") + hipGetErrorString(he));
     return 0;
+}
+
+int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
+                        const int *targetCount, const int *sourceBegin, const int *sourceCount,
+                        const double *targetBox, const double *sourceBox, int numTargets,
+                        const double *targetX, const double *targetY, const double *targetZ,
+                        int numSources, const double *sourceX, const double *sourceY,
+                        const double *sourceZ, const double *sourceValue, int kernel,
+                        double kernelParam, double tol, int *labels, double *errors)
+{
+    return label_explicit(numPairs, kind, targetBegin, targetCount, sourceBegin, sourceCount, targetBox,
+                          sourceBox, numTargets, targetX, targetY, targetZ, nullptr, numSources, sourceX,
+                          sourceY, sourceZ, sourceValue, nullptr, kernel, kernelParam, 0, 0.0, tol, labels,
+                          errors);
+}
+
+int BaryTree_LabelPairsSubtraction(int numPairs, const int *kind, const int *targetBegin,
+                                   const int *targetCount, const int *sourceBegin, const int *sourceCount,
+                                   const double *targetBox, const double *sourceBox, int numTargets,
+                                   const double *targetX, const double *targetY, const double *targetZ,
+                                   const double *targetValue, int numSources, const double *sourceX,
+                                   const double *sourceY, const double *sourceZ, const double *sourceValue,
+                                   const double *sourceWeight, int kernel, int numKernelParams,
+                                   const double *kernelParams, double tol, int *labels, double *errors)
+{
+    g_last_error.clear();
+    if (!kernelParams || numKernelParams < (kernel == 3 ? 2 : 1)) return fail("kernel parameters missing");
+    double p0 = 0.0, p1 = 0.0;
+    if (kernel == 1) {
+        p1 = 1.0 / (kernelParams[0] * kernelParams[0]);
+    } else if (kernel == 2) {
+        p0 = kernelParams[0];
+    } else if (kernel == 3) {
+        p0 = kernelParams[1];
+        p1 = 1.0 / (kernelParams[0] * kernelParams[0]);
+    } else {
+        return fail("kernel must be 1, 2 or 3");
+    }
+    return label_explicit(numPairs, kind, targetBegin, targetCount, sourceBegin, sourceCount, targetBox,
+                          sourceBox, numTargets, targetX, targetY, targetZ, targetValue, numSources, sourceX,
+                          sourceY, sourceZ, sourceValue, sourceWeight, kernel, p0, 1, p1, tol, labels, errors);
 }
 
 int BaryTreeModel_Create(BaryTreeDegreeModel **model, unsigned long long seed)

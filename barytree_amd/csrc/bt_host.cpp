@@ -14,6 +14,13 @@
 
 namespace bt {
 
+namespace {
+thread_local int g_beta_leaf_mode = 0;
+}
+
+void set_beta_leaf_mode(int mode) { g_beta_leaf_mode = mode; }
+int beta_leaf_mode() { return g_beta_leaf_mode; }
+
 void params_setup(Params &p, int kernel, int num_kp, const double *kp, int approximation,
                   int singularity, int compute_type, double theta, int degree, int msl, int mtl,
                   double size_check, double beta, int verbosity)
@@ -34,8 +41,9 @@ void params_setup(Params &p, int kernel, int num_kp, const double *kp, int appro
         p.max_source_leaf = msl;
         p.max_target_leaf = mtl;
     } else {
-        // run_params.c:51-120 (the CPU build's leaf sizes, so that results
-        // agree with the reference CPU run)
+        // run_params.c:51-120 (leaf sizes of the CPU build by default, so
+        // that results agree with the reference CPU run; of the GPU build
+        // on request)
         p.beta = beta;
         double tmin = 0.55, tmax = 0.95, nmin = 1, nmax = 12, es = 2, et = 3;
         p.size_check_factor = 1.0;
@@ -49,7 +57,8 @@ void params_setup(Params &p, int kernel, int num_kp, const double *kp, int appro
         }
         p.theta = tmax - (tmax - tmin) * std::pow(beta, es);
         p.degree = (int)(nmax - (nmax - nmin) * std::pow(1. - beta, et));
-        if (compute_type == C_PC) { p.max_source_leaf = 50; p.max_target_leaf = 5; }
+        if (g_beta_leaf_mode == 1) { p.max_source_leaf = 3000; p.max_target_leaf = 3000; }
+        else if (compute_type == C_PC) { p.max_source_leaf = 50; p.max_target_leaf = 5; }
         else if (compute_type == C_CP) { p.max_source_leaf = 5; p.max_target_leaf = 50; }
         else { p.max_source_leaf = 50; p.max_target_leaf = 50; }
     }

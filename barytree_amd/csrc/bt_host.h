@@ -44,6 +44,12 @@ struct Params {
     int precision = PREC_FP64;
 };
 
+// Leaf sizes of the beta auto-tune on this thread (run_params.c:100-117):
+// 0 the reference CPU build's (PC 50/5, CP 5/50, CC 50/50; the default, so
+// beta runs reproduce the reference CPU results), 1 its GPU build's
+// (3000/3000).
+void set_beta_leaf_mode(int mode);
+int beta_leaf_mode();
 // Resolve (theta, degree, leaf sizes) from beta exactly as RunParams_Setup.
 void params_setup(Params &p, int kernel, int num_kp, const double *kp, int approximation,
                   int singularity, int compute_type, double theta, int degree, int msl, int mtl,

@@ -1399,13 +1399,22 @@ __device__ __forceinline__ double bary_w(int j, int n)
 }
 
 // ---------------------------------------------------------------------
-// Upward pass (anterpolation), one block of 256 threads per output node.
-// Points arrive in chunks of CH; for each chunk, 3*CH threads compute the
-// per-dimension factors w_k/(x_d - node_d[k]) (or the exact-hit Kronecker
-// row) into LDS, CH threads form the modified charge q/prod(sums), and
-// every thread accumulates its grid points j = tid + 256*m as
-//   Q[j] += ((Ax[k1]*Ay[k2])*Az[k3]) * qmod
-// which is the reference's numerator order (clusters.c:506-565).
+// Upward pass (anterpolation) on the FP64 matrix cores, one block of 256
+// threads (4 waves) per output node.  Points (a node's particles, or the
+// grid points of its children) arrive in chunks of ACH; for each chunk 3*ACH
+// threads compute the per-dimension factors A_d[k][p] = w_k/(x_d - node_d[k])
+// (or the exact-hit Kronecker row; clusters.c:506-565, 379-582) into LDS and
+// ACH threads the modified charges f1 = q/den (f2 = w/den), den the product
+// of the three factor sums.  The grid's charges are then the product
+//   Q[(k1,k2), k3] += sum_p (A_x[k1][p] A_y[k2][p]) (A_z[k3][p] f1[p])
+// of an NL^2 x ACH matrix and an ACH x NL matrix; with subtraction the
+// weights W are the columns NL..2NL-1 of the same product (f2 instead of
+// f1).  Tiles of 16x16 go through v_mfma_f64_16x16x4f64 (4 points per
+// instruction): wave w owns column tile w % CT and every (4/CT)-th row tile
+// (CT = 1 or 2 column tiles), so a step reads each B operand once.
+// Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16]; it holds
+// D[l/16 + 4v][l%16], v = 0..3.  Out-of-range rows and columns and points
+// past the chunk's end supply zeros.
 // ---------------------------------------------------------------------
 constexpr int ACH = 64;
 
@@ -1413,27 +1422,39 @@ template <int NL, bool SS>
 __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
 {
     constexpr int P = NL * NL * NL;
-    constexpr int MJ = (P + 255) / 256;
+    constexpr int R = NL * NL;                       // rows (k1, k2)
+    constexpr int C = SS ? 2 * NL : NL;              // columns: k3 (Q), NL + k3 (W)
+    constexpr int RT = (R + 15) / 16, CT = (C + 15) / 16;
+    static_assert(CT == 1 || CT == 2, "column tiles");
+    constexpr int RSTEP = 4 / CT;                    // row tiles between a wave's tiles
+    constexpr int TPW = (RT + RSTEP - 1) / RSTEP;    // row tiles per wave (at most)
     constexpr int LDA = ACH + 1;
     constexpr int n = NL - 1;
     __shared__ double LA[3 * NL * LDA];
     __shared__ double Lsum[3 * ACH];
-    __shared__ double Lf1[ACH], Lf2[ACH];
+    __shared__ double Lf[2 * ACH];
     __shared__ double Lnode[3 * NL];
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int node = A.nodes[blockIdx.x];
     if (tid < 3 * NL) Lnode[tid] = A.tab[(size_t)node * 3 * NL + tid];
 
-    int o1[MJ], o2[MJ], o3[MJ];
-    double accq[MJ], accw[MJ];
+    // this lane's B column: k3 and which charge (0: f1, 1: f2); -1 outside
+    const int ct = wv % CT;
+    const int bc = ct * 16 + li;
+    const int b_k3 = bc < C ? bc % NL : -1, b_f = bc < C ? bc / NL : 0;
+    const int rt0 = wv / CT;
+    // this lane's A rows, one per row tile of the wave: k1 and k2 (-1 outside)
+    int a_k1[TPW], a_k2[TPW];
+    v4d acc[TPW];
 #pragma unroll
-    for (int m = 0; m < MJ; m++) {
-        const int j = tid + 256 * m;
-        const int jj = j < P ? j : 0;
-        const int k1 = jj % NL, k2 = (jj / NL) % NL, k3 = jj / (NL * NL);
-        o1[m] = k1 * LDA; o2[m] = (NL + k2) * LDA; o3[m] = (2 * NL + k3) * LDA;
-        accq[m] = 0.0; accw[m] = 0.0;
+    for (int t = 0; t < TPW; t++) {
+        const int r = (rt0 + RSTEP * t) * 16 + li;
+        const bool ok = rt0 + RSTEP * t < RT && r < R;
+        a_k1[t] = ok ? r % NL : -1;
+        a_k2[t] = ok ? r / NL : 0;
+        acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
     // point sets: mode 0 one range of particles (the node's); mode 2 one
@@ -1454,7 +1475,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
             child = A.child_id[A.child_off[blockIdx.x] + set];
         }
         for (int c0 = 0; c0 < pcnt; c0 += ACH) {
-            __syncthreads();
+            __syncthreads();  // the previous chunk's factors have been read
             if (tid < 3 * ACH) {
                 const int p = tid % ACH, d = tid / ACH;
                 const bool valid = c0 + p < pcnt;
@@ -1508,29 +1529,34 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
                     f1 /= den;
                     if (SS) f2 /= den;
                 }
-                Lf1[p] = f1;
-                Lf2[p] = f2;
+                Lf[p] = f1;
+                Lf[ACH + p] = f2;
             }
             __syncthreads();
-            const int np = min(ACH, pcnt - c0);
-            for (int p = 0; p < np; ++p) {
-                const double f1 = Lf1[p];
-                const double f2 = SS ? Lf2[p] : 0.0;
+            // points past the end have zero factors and charges
+            const int np4 = min(ACH, (pcnt - c0 + 3) & ~3);
+            for (int p0 = 0; p0 < np4; p0 += 4) {
+                const int p = p0 + lk;
+                const double bop = b_k3 >= 0 ? LA[(2 * NL + b_k3) * LDA + p] * Lf[b_f * ACH + p] : 0.0;
 #pragma unroll
-                for (int m = 0; m < MJ; m++) {
-                    const double num = (LA[o1[m] + p] * LA[o2[m] + p]) * LA[o3[m] + p];
-                    accq[m] = fma(num, f1, accq[m]);
-                    if (SS) accw[m] = fma(num, f2, accw[m]);
+                for (int t = 0; t < TPW; t++) {
+                    if (rt0 + RSTEP * t >= RT) continue;  // wave-uniform
+                    const double aop = a_k1[t] >= 0 ? LA[a_k1[t] * LDA + p] * LA[(NL + a_k2[t]) * LDA + p] : 0.0;
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bop, acc[t], 0, 0, 0);
                 }
             }
         }
     }
+    // lane l holds rows (k1, k2) = r, r = rt*16 + l/16 + 4v, of column bc
+    if (b_k3 < 0) return;
+    double *out = b_f == 0 ? A.Q : A.W;
 #pragma unroll
-    for (int m = 0; m < MJ; m++) {
-        const int j = tid + 256 * m;
-        if (j < P) {
-            A.Q[(size_t)orow * P + j] = accq[m];
-            if (SS) A.W[(size_t)orow * P + j] = accw[m];
+    for (int t = 0; t < TPW; t++) {
+        if (rt0 + RSTEP * t >= RT) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = (rt0 + RSTEP * t) * 16 + lk + 4 * v;
+            if (r < R) out[(size_t)orow * P + r + R * b_k3] = acc[t][v];
         }
     }
 }
@@ -1566,17 +1592,37 @@ __device__ __forceinline__ void point_factors(double x, const double *nodes, dou
     hit_any = hit_any || hit >= 0;
 }
 
-constexpr int DTPB = 128;  // threads per block of the downward pass
+constexpr int DTPB = 256;  // threads per block of the downward pass (4 waves)
+constexpr int DCH = 64;    // points per chunk (16 per wave)
 
+// Downward pass on the FP64 matrix cores.  For a chunk of points (a
+// child's grid points, or a tile of a leaf's targets) 3*DCH threads put the
+// per-dimension factors into LDS (point_factors: exact hits become Kronecker
+// rows, den collects 1/sum over the other dimensions).  Wave w takes points
+// 16w..16w+15 of the chunk and forms
+//   T[p][(k2,k3)] = sum_k1 wx[p][k1] Q[k1 + NL (k2 + NL k3)]
+// in 16x16 tiles with v_mfma_f64_16x16x4f64 (K = k1, padded to a multiple
+// of 4 with zeros), then each lane weights its entries of T by
+// wy[p][k2] wz[p][k3] and the 16 lanes of a row add them (DPP row_shr
+// scan); the value at p is den * that sum
+// (interaction_compute_downpass.c:210-303, 395-479; only the order of the
+// sums differs).
 template <int NL, bool SS>
 __global__ __launch_bounds__(DTPB) void k_down(DownArgs A)
 {
     constexpr int P = NL * NL * NL;
+    constexpr int N2 = NL * NL;                      // columns (k2, k3)
+    constexpr int CT = (N2 + 15) / 16;               // column tiles
+    constexpr int KS = (NL + 3) / 4;                 // k1 steps of 4
+    constexpr int LDW = DCH + 1;
     __shared__ double LQ[P];
     __shared__ double LW[SS ? P : 1];
     __shared__ double Lnode[3 * NL];
-    __shared__ double Lwy[NL * DTPB], Lwz[NL * DTPB];  // per-thread y and z factors
-    const int tid = threadIdx.x;
+    __shared__ double Lw[3 * NL * LDW];              // factors [d][k][p]
+    __shared__ double Lsum[3 * DCH];                 // 1/sum of a dimension (1 on a hit)
+    __shared__ double Lv[2 * DCH];                   // the chunk's results (Q, W)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     int src_node, pbeg = 0, pcnt = P, dst_node = -1;
     if (A.mode == 0) {
         dst_node = A.nodes[blockIdx.x];
@@ -1590,57 +1636,87 @@ __global__ __launch_bounds__(DTPB) void k_down(DownArgs A)
         if (SS) LW[j] = A.W[(size_t)src_node * P + j];
     }
     if (tid < 3 * NL) Lnode[tid] = A.tab[(size_t)src_node * 3 * NL + tid];
-    __syncthreads();
-    for (int i = tid; i < pcnt; i += DTPB) {
-        double x, y, z;
-        if (A.mode == 0) {
-            const int k1 = i % NL, k2 = (i / NL) % NL, k3 = i / (NL * NL);
-            const double *tb = A.tab + (size_t)dst_node * 3 * NL;
-            x = tb[k1]; y = tb[NL + k2]; z = tb[2 * NL + k3];
-        } else {
-            const int g = pbeg + i;
-            x = A.tx[g]; y = A.ty[g]; z = A.tz[g];
-        }
-        double wx[NL], wt[NL];
-        double den = 1.0;
-        bool h = false;
-        point_factors<NL>(x, Lnode, wx, den, h);
-        point_factors<NL>(y, Lnode + NL, wt, den, h);
-#pragma unroll
-        for (int k = 0; k < NL; k++) Lwy[k * DTPB + tid] = wt[k];
-        point_factors<NL>(z, Lnode + 2 * NL, wt, den, h);
-#pragma unroll
-        for (int k = 0; k < NL; k++) Lwz[k * DTPB + tid] = wt[k];
-        double sq = 0.0, sw = 0.0;
-#pragma unroll 1
-        for (int k3 = 0; k3 < NL; k3++) {
-            double s2q = 0.0, s2w = 0.0;
-#pragma unroll 1
-            for (int k2 = 0; k2 < NL; k2++) {
-                double s1q = 0.0, s1w = 0.0;
-                const int base = (k3 * NL + k2) * NL;
-#pragma unroll
-                for (int k1 = 0; k1 < NL; k1++) {
-                    s1q = fma(wx[k1], LQ[base + k1], s1q);
-                    if (SS) s1w = fma(wx[k1], LW[base + k1], s1w);
+    for (int c0 = 0; c0 < pcnt; c0 += DCH) {
+        __syncthreads();  // LQ, Lnode ready; the previous chunk's factors read
+        if (tid < 3 * DCH) {
+            const int p = tid % DCH, d = tid / DCH, i = c0 + p;
+            const bool valid = i < pcnt;
+            double x = 0.0;
+            if (valid) {
+                if (A.mode == 0) {
+                    const int kd = d == 0 ? i % NL : (d == 1 ? (i / NL) % NL : i / (NL * NL));
+                    x = A.tab[(size_t)dst_node * 3 * NL + d * NL + kd];
+                } else {
+                    const int g = pbeg + i;
+                    x = d == 0 ? A.tx[g] : (d == 1 ? A.ty[g] : A.tz[g]);
                 }
-                const double wyk = Lwy[k2 * DTPB + tid];
-                s2q = fma(wyk, s1q, s2q);
-                if (SS) s2w = fma(wyk, s1w, s2w);
             }
-            const double wzk = Lwz[k3 * DTPB + tid];
-            sq = fma(wzk, s2q, sq);
-            if (SS) sw = fma(wzk, s2w, sw);
+            double wk[NL];
+            double den = 1.0;
+            bool h = false;
+            point_factors<NL>(x, Lnode + d * NL, wk, den, h);
+#pragma unroll
+            for (int k = 0; k < NL; k++) Lw[(d * NL + k) * LDW + p] = valid ? wk[k] : 0.0;
+            Lsum[d * DCH + p] = den;
         }
-        const double vq = den * sq, vw = den * sw;
-        if (A.mode == 0) {
-            const size_t o = (size_t)dst_node * P + i;
-            A.Q[o] += vq;
-            if (SS) A.W[o] += vw;
-        } else {
-            const int g = pbeg + i;
-            // subtraction: interpolate Q - tq*W (downpass.c:836)
-            A.pot[g] += SS ? fma(-A.tq[g], vw, vq) : vq;
+        __syncthreads();
+        // stage 1 and 2 for points pw + 4v + lk (rows of the tile), pw = 16 wv
+        const int pw = 16 * wv;
+        double sq[4] = {0.0, 0.0, 0.0, 0.0}, sw[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c0 + pw < pcnt) {  // wave-uniform: the wave has points
+#pragma unroll 1
+            for (int ct = 0; ct < CT; ct++) {
+                const int col = ct * 16 + li;             // B column / this lane's column of T
+                const bool cok = col < N2;
+                v4d dq = {0.0, 0.0, 0.0, 0.0}, dw = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    const int kA = ks * 4 + lk;           // k1 of this lane's A and B entries
+                    const double aop = kA < NL ? Lw[kA * LDW + pw + li] : 0.0;
+                    const double bq = (kA < NL && cok) ? LQ[kA + NL * col] : 0.0;
+                    dq = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bq, dq, 0, 0, 0);
+                    if (SS) {
+                        const double bw = (kA < NL && cok) ? LW[kA + NL * col] : 0.0;
+                        dw = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, bw, dw, 0, 0, 0);
+                    }
+                }
+                if (cok) {
+                    const int k2 = col % NL, k3 = col / NL;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int p = pw + lk + 4 * v;
+                        const double f = Lw[(NL + k2) * LDW + p] * Lw[(2 * NL + k3) * LDW + p];
+                        sq[v] = fma(f, dq[v], sq[v]);
+                        if (SS) sw[v] = fma(f, dw[v], sw[v]);
+                    }
+                }
+            }
+            // the 16 lanes of a row: lane 15 of the row ends with the sum
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                sq[v] = group_sum64<16>(sq[v]);
+                if (SS) sw[v] = group_sum64<16>(sw[v]);
+                if (li == 15) {
+                    const int p = pw + lk + 4 * v;
+                    const double den = Lsum[p] * Lsum[DCH + p] * Lsum[2 * DCH + p];
+                    Lv[p] = den * sq[v];
+                    if (SS) Lv[DCH + p] = den * sw[v];
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < DCH && c0 + tid < pcnt) {
+            const int i = c0 + tid;
+            const double vq = Lv[tid], vw = SS ? Lv[DCH + tid] : 0.0;
+            if (A.mode == 0) {
+                const size_t o = (size_t)dst_node * P + i;
+                A.Q[o] += vq;
+                if (SS) A.W[o] += vw;
+            } else {
+                const int g = pbeg + i;
+                // subtraction: interpolate Q - tq*W (downpass.c:836)
+                A.pot[g] += SS ? fma(-A.tq[g], vw, vq) : vq;
+            }
         }
     }
 }

@@ -28,6 +28,13 @@ struct LabelArgs {
     const double *sx, *sy, *sz, *sq;
     int kernel;                 // 1 Coulomb, 2 Yukawa (p0 = kappa), 3 regularized Coulomb (p0 = epsilon)
     double p0;
+    // subtraction (ss = 1): the pair's potential is that of the subtraction
+    // kernel, sum_s (q_s - q_t g_w(r)/g(r)) w_s g(r): target charges tq and
+    // source weights sw, p1 = 1/kappa^2 (Coulomb) or 1/alpha^2 (regularized
+    // Coulomb) for the Gaussian of g_w; Yukawa's g_w is g
+    int ss;
+    const double *tq, *sw;
+    double p1;
     double tol;
     int *labels;
     double *errors;             // LABEL_ND per pair (NaN where not computed)

@@ -61,17 +61,32 @@ __device__ __forceinline__ double group_sum(double v)
     return v;
 }
 
+// Subtraction kernels: the second term's kernel, which multiplies the
+// source weight and, at the target, minus the target's charge
+// (coulomb_ss_pc.c:47, yukawa_ss_pc.c:46, regularized-coulomb_ss_pc.c,
+// coulomb_ss_cp.c:54-55): Coulomb e^{-r^2/k^2}/r, Yukawa e^{-kappa r}/r (the
+// same kernel), regularized Coulomb e^{-r^2/a^2}/sqrt(r^2 + eps^2);
+// p1 = 1/k^2 or 1/a^2.
+__device__ __forceinline__ double green_w(int kernel, double p0, double p1, double dx, double dy, double dz)
+{
+    if (kernel == 2) return green(kernel, p0, dx, dy, dz);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    return exp(-r2 * p1) * green(kernel, p0, dx, dy, dz);
+}
+
 __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
 {
     constexpr int NT = LABEL_NT, NS = LABEL_NS;
-    __shared__ double Ltx[NT], Lty[NT], Ltz[NT], Lex[NT], Lred[2 * NT];
-    __shared__ double Lsx[NS], Lsy[NS], Lsz[NS], Lsq[NS];
+    __shared__ double Ltx[NT], Lty[NT], Ltz[NT], Ltq[NT], Lex[NT], Lred[2 * NT];
+    __shared__ double Lsx[NS], Lsy[NS], Lsz[NS], Lsq[NS], Lsw[NS];
     __shared__ double Lnt[3 * NLMAX], Lns[3 * NLMAX];
     __shared__ double Bs[NS * 3 * NLMAX];
     __shared__ double Bt[NT * 3 * NLMAX];
-    __shared__ double G1[PMAX], G2[PMAX];
+    // grid values of the two terms (the second only with subtraction)
+    __shared__ double G1[PMAX], G2[PMAX], G1w[PMAX], G2w[PMAX];
     __shared__ int Ldone;
 
+    const bool ss = A.ss != 0;
     const int pi = blockIdx.x, tid = threadIdx.x;
     const int kind = A.kind[pi];
     const int tb0 = A.tbeg[pi], tc = A.tcnt[pi], sb0 = A.sbeg[pi], sc = A.scnt[pi];
@@ -80,10 +95,15 @@ __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
     if (tid < nt) {
         const int i = tb0 + (int)(((long long)tid * tc) / nt);
         Ltx[tid] = A.tx[i]; Lty[tid] = A.ty[i]; Ltz[tid] = A.tz[i];
+        Ltq[tid] = ss ? A.tq[i] : 0.0;
     }
     if (tid < ns) {
         const int s = sb0 + (int)(((long long)tid * sc) / ns);
-        Lsx[tid] = A.sx[s]; Lsy[tid] = A.sy[s]; Lsz[tid] = A.sz[s]; Lsq[tid] = A.sq[s];
+        Lsx[tid] = A.sx[s]; Lsy[tid] = A.sy[s]; Lsz[tid] = A.sz[s];
+        // subtraction: the first term's charge is q w, the second's w
+        const double w = ss ? A.sw[s] : 1.0;
+        Lsq[tid] = ss ? A.sq[s] * w : A.sq[s];
+        Lsw[tid] = w;
     }
     if (tid == 0) Ldone = 0;
     __syncthreads();
@@ -91,12 +111,16 @@ __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
     const int g = tid / LPT, l = tid % LPT;  // target of this lane group, lane in group
     // exact sums at the sampled targets
     {
-        double v = 0.0;
+        double v = 0.0, vw = 0.0;
         if (g < nt)
-            for (int s = l; s < ns; s += LPT)
-                v += Lsq[s] * green(A.kernel, A.p0, Ltx[g] - Lsx[s], Lty[g] - Lsy[s], Ltz[g] - Lsz[s]);
+            for (int s = l; s < ns; s += LPT) {
+                const double dx = Ltx[g] - Lsx[s], dy = Lty[g] - Lsy[s], dz = Ltz[g] - Lsz[s];
+                v += Lsq[s] * green(A.kernel, A.p0, dx, dy, dz);
+                if (ss) vw += Lsw[s] * green_w(A.kernel, A.p0, A.p1, dx, dy, dz);
+            }
         v = group_sum(v);
-        if (l == 0 && g < nt) Lex[g] = v;
+        if (ss) vw = group_sum(vw);
+        if (l == 0 && g < nt) Lex[g] = ss ? v - Ltq[g] * vw : v;
     }
     __syncthreads();
 
@@ -129,11 +153,15 @@ __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
             // modified charges of the source grid: Q_k = sum_s q_s l_k(y_s)
             for (int j = tid; j < P; j += LTPB) {
                 const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                double q = 0.0;
-                for (int s = 0; s < ns; s++)
-                    q += Lsq[s] * (Bs[(s * 3) * NLMAX + k1] * Bs[(s * 3 + 1) * NLMAX + k2] *
-                                   Bs[(s * 3 + 2) * NLMAX + k3]);
+                double q = 0.0, qw = 0.0;
+                for (int s = 0; s < ns; s++) {
+                    const double b = Bs[(s * 3) * NLMAX + k1] * Bs[(s * 3 + 1) * NLMAX + k2] *
+                                     Bs[(s * 3 + 2) * NLMAX + k3];
+                    q += Lsq[s] * b;
+                    if (ss) qw += Lsw[s] * b;
+                }
                 G1[j] = q;
+                G1w[j] = qw;
             }
             __syncthreads();
         }
@@ -142,10 +170,14 @@ __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
             for (int j = tid; j < P; j += LTPB) {
                 const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
                 const double gx = Lnt[k1], gy = Lnt[nl + k2], gz = Lnt[2 * nl + k3];
-                double v = 0.0;
-                for (int s = 0; s < ns; s++)
-                    v += Lsq[s] * green(A.kernel, A.p0, gx - Lsx[s], gy - Lsy[s], gz - Lsz[s]);
+                double v = 0.0, vw = 0.0;
+                for (int s = 0; s < ns; s++) {
+                    const double dx = gx - Lsx[s], dy = gy - Lsy[s], dz = gz - Lsz[s];
+                    v += Lsq[s] * green(A.kernel, A.p0, dx, dy, dz);
+                    if (ss) vw += Lsw[s] * green_w(A.kernel, A.p0, A.p1, dx, dy, dz);
+                }
                 G2[j] = v;
+                G2w[j] = vw;
             }
             __syncthreads();
         } else if (kind == 3) {
@@ -153,35 +185,42 @@ __global__ __launch_bounds__(LTPB) void k_label(LabelArgs A)
             for (int j = tid; j < P; j += LTPB) {
                 const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
                 const double gx = Lnt[k1], gy = Lnt[nl + k2], gz = Lnt[2 * nl + k3];
-                double v = 0.0;
+                double v = 0.0, vw = 0.0;
                 for (int k = 0; k < P; k++) {
                     const int m1 = k % nl, m2 = (k / nl) % nl, m3 = k / (nl * nl);
-                    v += G1[k] * green(A.kernel, A.p0, gx - Lns[m1], gy - Lns[nl + m2], gz - Lns[2 * nl + m3]);
+                    const double dx = gx - Lns[m1], dy = gy - Lns[nl + m2], dz = gz - Lns[2 * nl + m3];
+                    v += G1[k] * green(A.kernel, A.p0, dx, dy, dz);
+                    if (ss) vw += G1w[k] * green_w(A.kernel, A.p0, A.p1, dx, dy, dz);
                 }
                 G2[j] = v;
+                G2w[j] = vw;
             }
             __syncthreads();
         }
         // approximation at the sampled targets
-        double v = 0.0;
+        double v = 0.0, vw = 0.0;
         if (g < nt) {
             if (kind == 1) {
                 for (int k = l; k < P; k += LPT) {
                     const int m1 = k % nl, m2 = (k / nl) % nl, m3 = k / (nl * nl);
-                    v += G1[k] * green(A.kernel, A.p0, Ltx[g] - Lns[m1], Lty[g] - Lns[nl + m2],
-                                       Ltz[g] - Lns[2 * nl + m3]);
+                    const double dx = Ltx[g] - Lns[m1], dy = Lty[g] - Lns[nl + m2], dz = Ltz[g] - Lns[2 * nl + m3];
+                    v += G1[k] * green(A.kernel, A.p0, dx, dy, dz);
+                    if (ss) vw += G1w[k] * green_w(A.kernel, A.p0, A.p1, dx, dy, dz);
                 }
             } else {
                 const double *b = Bt + (g * 3) * NLMAX;
                 for (int j = l; j < P; j += LPT) {
                     const int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                    v += (b[k1] * b[NLMAX + k2] * b[2 * NLMAX + k3]) * G2[j];
+                    const double bb = b[k1] * b[NLMAX + k2] * b[2 * NLMAX + k3];
+                    v += bb * G2[j];
+                    if (ss) vw += bb * G2w[j];
                 }
             }
         }
         v = group_sum(v);
+        if (ss) vw = group_sum(vw);
         if (l == 0 && g < NT) {
-            const double e = g < nt ? v - Lex[g] : 0.0;
+            const double e = g < nt ? (ss ? v - Ltq[g] * vw : v) - Lex[g] : 0.0;
             Lred[g] = e * e;
             Lred[NT + g] = g < nt ? Lex[g] * Lex[g] : 0.0;
         }

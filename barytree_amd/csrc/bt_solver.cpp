@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 
 namespace bt {
 
@@ -29,6 +30,10 @@ double now()
     } while (0)
 
 constexpr int NXCD = btk::TILE_QUEUES;  // XCDs of an MI355X, one tile queue each
+// upward pass: leaves are split into parts when there are fewer than this
+// many (four blocks per CU), parts of at least UP_PART_MIN particles
+constexpr int UP_SPLIT_BLOCKS = 1024;
+constexpr int UP_PART_MIN = 256;
 
 }  // namespace
 
@@ -268,6 +273,45 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         sym_pairs_ += 2LL * it.a_cnt * it.b_cnt;
         sym_items_.push_back(it);
     }
+    // Launch order.  Workgroups are dealt round-robin over the 8 XCDs, so
+    // block b runs on XCD b % 8.  A block streams its B leaf once per pass
+    // over A; items that share a B leaf go to one XCD back to back, so their
+    // passes re-read B from that XCD's L2 rather than from HBM.  Groups
+    // (heaviest first) go to the least loaded XCD; the queues are
+    // interleaved (block 8j + x: item j of XCD x) and padded with empty
+    // items.  Slots and sums do not depend on the order.
+    {
+        std::map<int, std::vector<int>> by_b;
+        for (size_t i = 0; i < cand.size(); i++) by_b[cand[i].kb].push_back((int)i);
+        std::vector<std::pair<long long, const std::vector<int> *>> groups;
+        for (const auto &g : by_b) {
+            long long w = 0;
+            for (int i : g.second) w += (long long)sym_items_[i].a_cnt * sym_items_[i].b_cnt;
+            groups.push_back({w, &g.second});
+        }
+        std::stable_sort(groups.begin(), groups.end(),
+                         [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::vector<std::vector<int>> q(NXCD);
+        long long load[NXCD] = {};
+        for (const auto &g : groups) {
+            const int x = (int)(std::min_element(load, load + NXCD) - load);
+            load[x] += g.first;
+            q[x].insert(q[x].end(), g.second->begin(), g.second->end());
+        }
+        size_t qmax = 0;
+        for (const auto &v : q) qmax = std::max(qmax, v.size());
+        std::vector<btk::SymItem> items(qmax * NXCD, btk::SymItem{0, 0, 0, 0, 0, 0});
+        std::vector<double> ctr(4 * qmax * NXCD, 0.0);
+        for (int x = 0; x < NXCD; x++)
+            for (size_t j = 0; j < q[x].size(); j++) {
+                const size_t slot = j * NXCD + x;
+                items[slot] = sym_items_[q[x][j]];
+                std::copy(sym_ctr_.begin() + 4 * (size_t)q[x][j], sym_ctr_.begin() + 4 * (size_t)q[x][j] + 4,
+                          ctr.begin() + 4 * slot);
+            }
+        sym_items_.swap(items);
+        sym_ctr_.swap(ctr);
+    }
     sym_red_off_.assign(1, 0);
     for (size_t k = 0; k < keys.size(); k++) {
         const int t = keys[k];
@@ -502,6 +546,30 @@ void Plan::build_work()
         for (int i = 0; i < sy.ttree.n; i++)
             g_tparent_.push_back(sy.ttree.parent[i] < 0 ? -1 : sy.tnode_off + sy.ttree.parent[i]);
     }
+    // leaves' modified charges: one block per leaf, or, when the leaves are
+    // too few to fill the GPU (fewer than UP_SPLIT_BLOCKS), parts of at least
+    // UP_PART_MIN particles (multiples of the 64-particle chunk) whose rows
+    // are then added per leaf in part order
+    up_part_node_.clear(); up_part_beg_.clear(); up_part_end_.clear(); up_part_off_.clear();
+    if (p.compute_type != C_CP && p.approximation == A_LAGRANGE && (int)g_sleaves_.size() < UP_SPLIT_BLOCKS) {
+        long long np = 0;
+        for (int l : g_sleaves_) np += g_siend_[l] - g_sibeg_[l];
+        int part = (int)std::max<long long>(UP_PART_MIN, (np + UP_SPLIT_BLOCKS - 1) / UP_SPLIT_BLOCKS);
+        part = (part + 63) / 64 * 64;
+        std::vector<int> first(nsnodes_ + 1, 0);
+        std::vector<std::vector<std::array<int, 2>>> of(nsnodes_);
+        for (int l : g_sleaves_)
+            for (int b = g_sibeg_[l]; b < g_siend_[l]; b += part) of[l].push_back({b, std::min(b + part, g_siend_[l])});
+        up_part_off_.assign(1, 0);
+        for (int n = 0; n < nsnodes_; n++) {
+            for (const auto &r : of[n]) {
+                up_part_node_.push_back(n);
+                up_part_beg_.push_back(r[0]);
+                up_part_end_.push_back(r[1]);
+            }
+            up_part_off_.push_back((int)up_part_node_.size());
+        }
+    }
 }
 
 // Every index a kernel will dereference is checked here, on the host,
@@ -548,6 +616,7 @@ std::string Plan::validate() const
     if (sym_) {
         if (sym_ctr_.size() != 4 * sym_items_.size() || sym_items_.size() > (size_t)INT32_MAX) return "sym items";
         for (const btk::SymItem &it : sym_items_) {
+            if (it.a_cnt == 0 && it.b_cnt == 0 && it.rslot == 0 && it.cslot == 0) continue;  // padding
             if (it.a_beg < 0 || it.a_cnt <= 0 || (long long)it.a_beg + it.a_cnt > N_ || it.b_beg < 0 ||
                 it.b_cnt <= 0 || (long long)it.b_beg + it.b_cnt > N_ || N_ != M_)
                 return "sym item range";
@@ -582,6 +651,15 @@ std::string Plan::validate() const
         if (g_sibeg_[i] < 0 || g_siend_[i] > N_ || g_sibeg_[i] >= g_siend_[i]) return "source node range";
     for (int l : g_sleaves_)
         if (l < 0 || l >= nsnodes_) return "source leaf";
+    if (!up_part_node_.empty()) {
+        if ((int)up_part_off_.size() != nsnodes_ + 1 || up_part_off_[nsnodes_] != (int)up_part_node_.size())
+            return "upward parts";
+        for (int n = 0; n < nsnodes_; n++)
+            for (int k = up_part_off_[n]; k < up_part_off_[n + 1]; k++)
+                if (up_part_node_[k] != n || up_part_beg_[k] < g_sibeg_[n] || up_part_end_[k] > g_siend_[n] ||
+                    up_part_beg_[k] >= up_part_end_[k])
+                    return "upward part range";
+    }
     for (int c : up_child_id_)
         if (c < 0 || c >= nsnodes_) return "upward child";
     for (int n : up_nodes_)
@@ -1012,6 +1090,16 @@ std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
         if (!(r = up(&d_sibeg_, g_sibeg_)).empty()) return r;
         if (!(r = up(&d_siend_, g_siend_)).empty()) return r;
         if (!(r = up(&d_sleaves_, g_sleaves_)).empty()) return r;
+        if (!up_part_node_.empty()) {
+            if (!(r = up(&d_up_part_node_, up_part_node_)).empty()) return r;
+            if (!(r = up(&d_up_part_beg_, up_part_beg_)).empty()) return r;
+            if (!(r = up(&d_up_part_end_, up_part_end_)).empty()) return r;
+            if (!(r = up(&d_up_part_off_, up_part_off_)).empty()) return r;
+            const size_t npp = up_part_node_.size() * Pz;
+            d_up_Qp_ = dalloc<double>(npp);
+            if (ss) d_up_Wp_ = dalloc<double>(npp);
+            if (!d_up_Qp_ || (ss && !d_up_Wp_)) return "hipMalloc failed (upward parts)";
+        }
         if (!(r = up(&d_up_nodes_, up_nodes_)).empty()) return r;
         if (!(r = up(&d_up_child_off_, up_child_off_)).empty()) return r;
         if (!(r = up(&d_up_child_id_, up_child_id_)).empty()) return r;
@@ -1084,8 +1172,19 @@ std::string Plan::upward_fixed()
     a.ibeg = d_sibeg_; a.iend = d_siend_; a.tab = d_stab_;
     a.px = d_sx_; a.py = d_sy_; a.pz = d_sz_; a.pq = d_sq_; a.pw = d_sw_;
     a.Q = d_cq_; a.W = d_cw_;
-    a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)g_sleaves_.size();
-    HIPCHK(btk::launch_anterp(a, st_));
+    if (up_part_node_.empty()) {
+        a.mode = 0; a.nodes = d_sleaves_; a.nblocks = (int)g_sleaves_.size();
+        HIPCHK(btk::launch_anterp(a, st_));
+    } else {
+        // the leaves in parts, then each leaf's rows in part order (every
+        // other node's row is cleared here and written by its level below)
+        btk::AnterpArgs pa = a;
+        pa.mode = 2; pa.nodes = d_up_part_node_; pa.ibeg = d_up_part_beg_; pa.iend = d_up_part_end_;
+        pa.Q = d_up_Qp_; pa.W = d_up_Wp_; pa.nblocks = (int)up_part_node_.size();
+        HIPCHK(btk::launch_anterp(pa, st_));
+        HIPCHK(btk::launch_sum_parts(nsnodes_, p.P, d_up_part_off_, d_up_Qp_, d_up_Wp_, d_cq_,
+                                     a.subtraction ? d_cw_ : nullptr, st_));
+    }
     for (size_t lv = 0; lv + 1 < up_levels_off_.size(); lv++) {
         const int b = up_levels_off_[lv], e = up_levels_off_[lv + 1];
         if (e == b) continue;

@@ -250,6 +250,9 @@ private:
     std::vector<int> down_levels_off_, down_nodes_;
     // global (all systems) host arrays for the kernels
     std::vector<int> g_sibeg_, g_siend_, g_sleaves_, g_tparent_;
+    // upward pass of the leaves in parts (few leaves): part -> leaf node,
+    // particle range, CSR of parts per node
+    std::vector<int> up_part_node_, up_part_beg_, up_part_end_, up_part_off_;
     HostVec<int> g_torder_;
     Counts counts_;
     Timings tm_;
@@ -301,6 +304,8 @@ private:
     double *d_Qt_ = nullptr, *d_Wt_ = nullptr;
     int *d_sibeg_ = nullptr, *d_siend_ = nullptr, *d_tparent_ = nullptr;
     int *d_sleaves_ = nullptr, *d_up_nodes_ = nullptr, *d_up_child_off_ = nullptr, *d_up_child_id_ = nullptr;
+    int *d_up_part_node_ = nullptr, *d_up_part_beg_ = nullptr, *d_up_part_end_ = nullptr, *d_up_part_off_ = nullptr;
+    double *d_up_Qp_ = nullptr, *d_up_Wp_ = nullptr;
     int *d_down_nodes_ = nullptr;
     btk::Tile *d_near_tiles_ = nullptr, *d_grid_tiles_ = nullptr, *d_leaf_tiles_ = nullptr;
     int *d_nseg_off_ = nullptr, *d_nclu_off_ = nullptr, *d_nclu_id_ = nullptr;

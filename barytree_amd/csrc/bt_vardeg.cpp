@@ -572,6 +572,12 @@ std::string Plan::label_pairs(int first, int count, double tol, int *labels, dou
     a.kernel = prm_.kernel;
     a.p0 = prm_.kernel == K_YUKAWA ? prm_.kp[0]
          : (prm_.kernel == K_REG_COULOMB ? (prm_.singularity == S_SUBTRACTION ? prm_.kp[1] : prm_.kp[0]) : 0.0);
+    // subtraction plans: labels of the subtraction kernel (both terms)
+    if (prm_.singularity == S_SUBTRACTION) {
+        a.ss = 1;
+        a.tq = d_tq_; a.sw = d_sw_;
+        a.p1 = prm_.kernel == K_YUKAWA ? 0.0 : 1.0 / (prm_.kp[0] * prm_.kp[0]);
+    }
     a.tol = tol;
     a.labels = (int *)dl;
     a.errors = (double *)de;

@@ -126,12 +126,12 @@ def lib():
         for f in ("BaryTree_Solve", "BaryTree_Analyze", "BaryTree_GetLastStats",
                   "BaryTreePlan_Create", "BaryTreePlan_Execute", "BaryTreePlan_GetStats",
                   "BaryTreePlan_DirectSample", "BaryTreePlan_DirectSampleNoAlloc", "BaryTreePlan_SetPrecision",
-                  "BaryTreePlan_SetSymmetric",
+                  "BaryTreePlan_SetSymmetric", "BaryTree_SetBetaLeafSizes",
                   "BaryTree_SolveBatched", "BaryTreePlan_CreateBatched", "BaryTree_Autotune",
                   "BaryTree_AnalyzeBatched", "BaryTreePlan_NumPairs", "BaryTreePlan_PairInfo",
                   "BaryTreePlan_PairFeatures", "BaryTreePlan_LabelPairs",
                   "BaryTreePlan_SetPairDegrees", "BaryTreePlan_UseDegreeModel",
-                  "BaryTree_LabelPairs", "BaryTreeModel_Create", "BaryTreeModel_Load",
+                  "BaryTree_LabelPairs", "BaryTree_LabelPairsSubtraction", "BaryTreeModel_Create", "BaryTreeModel_Load",
                   "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
                   "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
                   "BaryTreeModel_GetFeatureRange", "BaryTreeModel_SetFeatureRange",
@@ -191,6 +191,17 @@ def solve(targets, sources, kernel, kernel_params, singularity, approximation, c
     if rc != 0:
         raise RuntimeError(f"BaryTree_Solve failed: {last_error()}")
     return out
+
+
+BETA_LEAVES_CPU, BETA_LEAVES_GPU = 0, 1
+
+
+def set_beta_leaf_sizes(which):
+    """Leaf sizes of the beta auto-tune for later calls on this thread:
+    BETA_LEAVES_CPU (the reference CPU build's, default) or BETA_LEAVES_GPU
+    (its GPU build's, 3000/3000)."""
+    if lib().BaryTree_SetBetaLeafSizes(ctypes.c_int(int(which))) != 0:
+        raise RuntimeError(f"BaryTree_SetBetaLeafSizes failed: {last_error()}")
 
 
 def last_stats():
@@ -506,6 +517,29 @@ def label_pairs(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, targets, sources, kern
         _iptr(labels), _ptr(errors))
     if rc != 0:
         raise RuntimeError(f"BaryTree_LabelPairs failed: {last_error()}")
+    return labels, errors
+
+
+def label_pairs_subtraction(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, targets, sources, kernel,
+                            kernel_params, tol):
+    """BaryTree_LabelPairsSubtraction: targets = (x, y, z, q), sources =
+    (x, y, z, q, w); kernel_params {kappa} (Coulomb, Yukawa) or {alpha,
+    epsilon} (regularized Coulomb).  Returns (labels, errors)."""
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (kind, tbeg, tcnt, sbeg, scnt)]
+    tbox, sbox = (np.ascontiguousarray(a, dtype=np.float64) for a in (tbox, sbox))
+    tx, ty, tz, tq = map(_arr, targets[:4])
+    sx, sy, sz, sq, sw = map(_arr, sources[:5])
+    kp = _kp(kernel_params)
+    n = len(ints[0])
+    labels = np.zeros(n, dtype=np.int32)
+    errors = np.zeros((n, LABEL_DEGREES))
+    rc = lib().BaryTree_LabelPairsSubtraction(
+        ctypes.c_int(n), *[_iptr(a) for a in ints], _ptr(tbox), _ptr(sbox), ctypes.c_int(len(tx)),
+        _ptr(tx), _ptr(ty), _ptr(tz), _ptr(tq), ctypes.c_int(len(sx)), _ptr(sx), _ptr(sy), _ptr(sz),
+        _ptr(sq), _ptr(sw), ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp), ctypes.c_double(tol),
+        _iptr(labels), _ptr(errors))
+    if rc != 0:
+        raise RuntimeError(f"BaryTree_LabelPairsSubtraction failed: {last_error()}")
     return labels, errors
 
 

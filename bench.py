@@ -442,6 +442,25 @@ def bench_cpu_case(args):
     threads = host_threads()
     allc = cpu_run("port", (x, y, z, q, w), (1, [0.0], 1, 1, 1, 0.7, 8, 2000, 2000), 1, threads)
     log(f"[bench] C1 oracle on {threads} threads {allc['wall']:.2f} s")
+    # the beta auto-tune (beta = 0.5) with the leaf sizes of the reference's
+    # CPU build and of its GPU build, one call each after a warm-up
+    beta = {}
+    bkargs = (bt.Kernel.COULOMB, [0.0], bt.Singularity.SKIPPING, bt.Approximation.LAGRANGE,
+              bt.ComputeType.PARTICLE_CLUSTER, -1.0, -1, 1, 1)
+    try:
+        for name, which in (("cpu_build_leaves", bt.BETA_LEAVES_CPU), ("gpu_build_leaves", bt.BETA_LEAVES_GPU)):
+            bt.set_beta_leaf_sizes(which)
+            bt.solve(targets, (x, y, z, q, w), *bkargs, beta=0.5)
+            t0 = time.perf_counter()
+            out_b = bt.solve(targets, (x, y, z, q, w), *bkargs, beta=0.5)
+            dt = time.perf_counter() - t0
+            sb = bt.last_stats()
+            beta[name] = {"s": round(dt, 4), "theta": sb["theta"], "degree": sb["interp_degree"],
+                          "leaves": [sb["max_per_source_leaf"], sb["max_per_target_leaf"]],
+                          "rel_l2_err": rel_l2(direct, out_b[idx])}
+            log(f"[bench] C1 beta 0.5 {name}: {dt:.4f} s, {beta[name]}")
+    finally:
+        bt.set_beta_leaf_sizes(bt.BETA_LEAVES_CPU)
     line = {
         "metric": "time per solve, Coulomb PC N=100k FP64, GPU vs the reference on one CPU core",
         "value": round(wall, 5),
@@ -469,6 +488,7 @@ def bench_cpu_case(args):
                          "host": host_cpu()},
         "speedup_vs_cpu": round(cpu_s / wall, 1),
         "speedup_end_to_end_vs_cpu": round(cpu_s / e2e, 1),
+        "beta_0_5_one_call": beta,
     }
     return line
 

@@ -147,6 +147,16 @@ int BaryTree_Solve(int numTargets, int numSources,
 /* Message of the last failed call on this thread ("" if none). */
 const char *BaryTree_GetLastError(void);
 
+/* Leaf sizes the beta auto-tune (0 <= beta <= 1) picks for later calls on
+ * this thread, as src/run_params/run_params.c:100-117 sets them:
+ * BARYTREE_BETA_LEAVES_CPU (the default) those of the reference CPU build
+ * (PC 50/5, CP 5/50, CC 50/50), so beta runs reproduce the reference CPU
+ * results; BARYTREE_BETA_LEAVES_GPU those of the reference GPU build
+ * (3000/3000), far faster here.  Returns 0, or -1 for another value. */
+#define BARYTREE_BETA_LEAVES_CPU 0
+#define BARYTREE_BETA_LEAVES_GPU 1
+int BaryTree_SetBetaLeafSizes(int which);
+
 /* Statistics of the last successful BaryTreeInterface/BaryTree_Solve call on
  * this thread.  Returns 0, or -1 if there was none. */
 int BaryTree_GetLastStats(BaryTreeStats *stats);
@@ -297,8 +307,9 @@ int BaryTreePlan_PairFeatures(const BaryTreePlan *plan, float *features);
  * error is <= tol, 13 if none.  Pair error at degree p: relative L2 error,
  * over up to 32 stride-sampled targets of the target node, of the degree-p
  * approximation of the potential of up to 128 stride-sampled sources of the
- * source node (SK kernel, FP64).  errors (optional): 11 per pair, degrees
- * 2..12, NaN beyond the label. */
+ * source node (FP64; the plan's kernel with skipping, or with subtraction
+ * both of its terms with the targets' charges and the sources' weights).
+ * errors (optional): 11 per pair, degrees 2..12, NaN beyond the label. */
 int BaryTreePlan_LabelPairs(BaryTreePlan *plan, int first, int count, double tol, int *labels,
                             double *errors);
 /* One degree (1..12) per pair for later Execute calls; NULL restores the
@@ -335,6 +346,19 @@ int BaryTree_LabelPairs(int numPairs, const int *kind, const int *targetBegin,
                         int numSources, const double *sourceX, const double *sourceY,
                         const double *sourceZ, const double *sourceValue, int kernel,
                         double kernelParam, double tol, int *labels, double *errors);
+
+/* BaryTree_LabelPairs with the subtraction kernel (both of its terms, as
+ * BaryTreePlan_LabelPairs labels subtraction plans): the targets' charges
+ * targetValue and the sources' weights sourceWeight; kernel 1 or 2 with
+ * kernelParams {kappa}, 3 (regularized Coulomb) with {alpha, epsilon}. */
+int BaryTree_LabelPairsSubtraction(int numPairs, const int *kind, const int *targetBegin,
+                                   const int *targetCount, const int *sourceBegin, const int *sourceCount,
+                                   const double *targetBox, const double *sourceBox, int numTargets,
+                                   const double *targetX, const double *targetY, const double *targetZ,
+                                   const double *targetValue, int numSources, const double *sourceX,
+                                   const double *sourceY, const double *sourceZ, const double *sourceValue,
+                                   const double *sourceWeight, int kernel, int numKernelParams,
+                                   const double *kernelParams, double tol, int *labels, double *errors);
 
 /* The degree model: MLP 8 -> 64 -> 64 -> 64 -> 10 (ReLU, FP32).  Output k
  * is the logit of "the pair needs a degree above 2+k".  Training (Adam on

@@ -15,7 +15,11 @@
  *     over the node boxes, nodes x0 + (cos(k pi/p)+1)/2 (x1-x0)
  *     (src/clusters/clusters.c:436-440), barycentric Lagrange basis with
  *     weights (-1)^k d_k, d_0 = d_p = 1/2, and the exact-hit Kronecker row
- *     when |x - x_k| < DBL_MIN (clusters.c:446-520);
+ *     when |x - x_k| < DBL_MIN (clusters.c:446-520); with subtraction the
+ *     potential is that of the subtraction kernel (both terms: the charges
+ *     q w with the kernel, the weights w with its Gaussian-weighted form
+ *     times minus the target's charge, each interpolated as the treecode
+ *     does);
  *   - the degree model's loss and gradient: MLP 8 -> 64 -> 64 -> 64 -> 10
  *     with ReLU, batch-mean binary cross-entropy of the 10 "degree above
  *     2+k" outputs, in FP64.
@@ -70,31 +74,55 @@ static void cheb_nodes(const double *box, int p, double *nodes /* 3*(p+1) */)
         }
 }
 
-int orc_label_pairs(int npairs, const int *kind, const int *tbeg, const int *tcnt, const int *sbeg,
-                    const int *scnt, const double *tbox, const double *sbox, const double *tx,
-                    const double *ty, const double *tz, const double *sx, const double *sy,
-                    const double *sz, const double *sq, int kernel, double p0, double tol,
-                    int *labels, double *errors)
+/* Subtraction kernels' second term (the weight's kernel, multiplied at the
+ * target by minus the target's charge): Coulomb e^{-r^2/k^2}/r
+ * (coulomb_ss_pc.c:47), Yukawa e^{-kappa r}/r (yukawa_ss_pc.c:46: the same
+ * kernel), regularized Coulomb e^{-r^2/a^2}/sqrt(r^2+eps^2); p1 = 1/k^2 or
+ * 1/a^2. */
+static double green_w(int kernel, double p0, double p1, double dx, double dy, double dz)
 {
-    double *G1 = malloc(sizeof(double) * NLMAX * NLMAX * NLMAX);
-    double *G2 = malloc(sizeof(double) * NLMAX * NLMAX * NLMAX);
-    if (!G1 || !G2) { free(G1); free(G2); return -1; }
+    if (kernel == 2) return green(kernel, p0, dx, dy, dz);
+    double r2 = dx * dx + dy * dy + dz * dz;
+    return exp(-r2 * p1) * green(kernel, p0, dx, dy, dz);
+}
+
+/* The pair potential at target t: sum_s c0_s g(t - s) - ct_t sum_s c1_s g_w(t - s)
+ * with, for subtraction, c0 = q w, c1 = w and ct = the target's charge (the
+ * second sum absent without subtraction); the approximations interpolate
+ * each sum as the treecode does. */
+int orc_label_pairs_ss(int npairs, const int *kind, const int *tbeg, const int *tcnt, const int *sbeg,
+                       const int *scnt, const double *tbox, const double *sbox, const double *tx,
+                       const double *ty, const double *tz, const double *tq, const double *sx,
+                       const double *sy, const double *sz, const double *sq, const double *sw, int kernel,
+                       double p0, int ss, double p1, double tol, int *labels, double *errors)
+{
+    size_t pm = (size_t)NLMAX * NLMAX * NLMAX;
+    double *G1 = malloc(sizeof(double) * pm), *G2 = malloc(sizeof(double) * pm);
+    double *G1w = malloc(sizeof(double) * pm), *G2w = malloc(sizeof(double) * pm);
+    if (!G1 || !G2 || !G1w || !G2w) { free(G1); free(G2); free(G1w); free(G2w); return -1; }
+    if (ss && (!tq || !sw)) { free(G1); free(G2); free(G1w); free(G2w); return -1; }
     for (int pi = 0; pi < npairs; pi++) {
         int nt = tcnt[pi] < NT ? tcnt[pi] : NT, ns = scnt[pi] < NS ? scnt[pi] : NS;
-        double X[NT][3], Y[NS][3], Q[NS], ex[NT];
+        double X[NT][3], CT[NT], Y[NS][3], Q[NS], W[NS], ex[NT];
         for (int k = 0; k < nt; k++) {
             int i = tbeg[pi] + (int)(((long long)k * tcnt[pi]) / nt);
             X[k][0] = tx[i]; X[k][1] = ty[i]; X[k][2] = tz[i];
+            CT[k] = ss ? tq[i] : 0.0;
         }
         for (int k = 0; k < ns; k++) {
             int s = sbeg[pi] + (int)(((long long)k * scnt[pi]) / ns);
-            Y[k][0] = sx[s]; Y[k][1] = sy[s]; Y[k][2] = sz[s]; Q[k] = sq[s];
+            Y[k][0] = sx[s]; Y[k][1] = sy[s]; Y[k][2] = sz[s];
+            W[k] = ss ? sw[s] : 1.0;
+            Q[k] = ss ? sq[s] * W[k] : sq[s];
         }
         for (int i = 0; i < nt; i++) {
-            double v = 0.0;
-            for (int s = 0; s < ns; s++)
-                v += Q[s] * green(kernel, p0, X[i][0] - Y[s][0], X[i][1] - Y[s][1], X[i][2] - Y[s][2]);
-            ex[i] = v;
+            double v = 0.0, vw = 0.0;
+            for (int s = 0; s < ns; s++) {
+                double dx = X[i][0] - Y[s][0], dy = X[i][1] - Y[s][1], dz = X[i][2] - Y[s][2];
+                v += Q[s] * green(kernel, p0, dx, dy, dz);
+                if (ss) vw += W[s] * green_w(kernel, p0, p1, dx, dy, dz);
+            }
+            ex[i] = ss ? v - CT[i] * vw : v;
         }
         int label = DMAX + 1;
         for (int p = DMIN; p <= DMAX; p++) errors[(size_t)pi * ND + p - DMIN] = NAN;
@@ -103,57 +131,73 @@ int orc_label_pairs(int npairs, const int *kind, const int *tbeg, const int *tcn
             double nt_nodes[3 * NLMAX], ns_nodes[3 * NLMAX];
             cheb_nodes(tbox + 6 * (size_t)pi, p, nt_nodes);
             cheb_nodes(sbox + 6 * (size_t)pi, p, ns_nodes);
-            if (kind[pi] != 2) {  /* source grid charges */
+            if (kind[pi] != 2) {  /* source grid charges of each term */
                 double B[NS][3][NLMAX];
                 for (int s = 0; s < ns; s++)
                     for (int d = 0; d < 3; d++) lagrange_row(Y[s][d], ns_nodes + d * nl, nl, B[s][d]);
                 for (int j = 0; j < P; j++) {
                     int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                    double q = 0.0;
-                    for (int s = 0; s < ns; s++) q += Q[s] * (B[s][0][k1] * B[s][1][k2] * B[s][2][k3]);
+                    double q = 0.0, qw = 0.0;
+                    for (int s = 0; s < ns; s++) {
+                        double b = B[s][0][k1] * B[s][1][k2] * B[s][2][k3];
+                        q += Q[s] * b;
+                        qw += W[s] * b;
+                    }
                     G1[j] = q;
+                    G1w[j] = qw;
                 }
             }
             if (kind[pi] == 2) {
                 for (int j = 0; j < P; j++) {
                     int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                    double v = 0.0;
-                    for (int s = 0; s < ns; s++)
-                        v += Q[s] * green(kernel, p0, nt_nodes[k1] - Y[s][0], nt_nodes[nl + k2] - Y[s][1],
-                                          nt_nodes[2 * nl + k3] - Y[s][2]);
+                    double v = 0.0, vw = 0.0;
+                    for (int s = 0; s < ns; s++) {
+                        double dx = nt_nodes[k1] - Y[s][0], dy = nt_nodes[nl + k2] - Y[s][1],
+                               dz = nt_nodes[2 * nl + k3] - Y[s][2];
+                        v += Q[s] * green(kernel, p0, dx, dy, dz);
+                        if (ss) vw += W[s] * green_w(kernel, p0, p1, dx, dy, dz);
+                    }
                     G2[j] = v;
+                    G2w[j] = vw;
                 }
             } else if (kind[pi] == 3) {
                 for (int j = 0; j < P; j++) {
                     int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                    double v = 0.0;
+                    double v = 0.0, vw = 0.0;
                     for (int k = 0; k < P; k++) {
                         int m1 = k % nl, m2 = (k / nl) % nl, m3 = k / (nl * nl);
-                        v += G1[k] * green(kernel, p0, nt_nodes[k1] - ns_nodes[m1],
-                                           nt_nodes[nl + k2] - ns_nodes[nl + m2],
-                                           nt_nodes[2 * nl + k3] - ns_nodes[2 * nl + m3]);
+                        double dx = nt_nodes[k1] - ns_nodes[m1], dy = nt_nodes[nl + k2] - ns_nodes[nl + m2],
+                               dz = nt_nodes[2 * nl + k3] - ns_nodes[2 * nl + m3];
+                        v += G1[k] * green(kernel, p0, dx, dy, dz);
+                        if (ss) vw += G1w[k] * green_w(kernel, p0, p1, dx, dy, dz);
                     }
                     G2[j] = v;
+                    G2w[j] = vw;
                 }
             }
             double num = 0.0, den = 0.0;
             for (int i = 0; i < nt; i++) {
-                double v = 0.0;
+                double v = 0.0, vw = 0.0;
                 if (kind[pi] == 1) {
                     for (int k = 0; k < P; k++) {
                         int m1 = k % nl, m2 = (k / nl) % nl, m3 = k / (nl * nl);
-                        v += G1[k] * green(kernel, p0, X[i][0] - ns_nodes[m1], X[i][1] - ns_nodes[nl + m2],
-                                           X[i][2] - ns_nodes[2 * nl + m3]);
+                        double dx = X[i][0] - ns_nodes[m1], dy = X[i][1] - ns_nodes[nl + m2],
+                               dz = X[i][2] - ns_nodes[2 * nl + m3];
+                        v += G1[k] * green(kernel, p0, dx, dy, dz);
+                        if (ss) vw += G1w[k] * green_w(kernel, p0, p1, dx, dy, dz);
                     }
                 } else {
                     double b[3][NLMAX];
                     for (int d = 0; d < 3; d++) lagrange_row(X[i][d], nt_nodes + d * nl, nl, b[d]);
                     for (int j = 0; j < P; j++) {
                         int k1 = j % nl, k2 = (j / nl) % nl, k3 = j / (nl * nl);
-                        v += (b[0][k1] * b[1][k2] * b[2][k3]) * G2[j];
+                        double bb = b[0][k1] * b[1][k2] * b[2][k3];
+                        v += bb * G2[j];
+                        vw += bb * G2w[j];
                     }
                 }
-                num += (v - ex[i]) * (v - ex[i]);
+                double a = ss ? v - CT[i] * vw : v;
+                num += (a - ex[i]) * (a - ex[i]);
                 den += ex[i] * ex[i];
             }
             double err = den > 0.0 ? sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
@@ -162,9 +206,18 @@ int orc_label_pairs(int npairs, const int *kind, const int *tbeg, const int *tcn
         }
         labels[pi] = label;
     }
-    free(G1);
-    free(G2);
+    free(G1); free(G2); free(G1w); free(G2w);
     return 0;
+}
+
+int orc_label_pairs(int npairs, const int *kind, const int *tbeg, const int *tcnt, const int *sbeg,
+                    const int *scnt, const double *tbox, const double *sbox, const double *tx,
+                    const double *ty, const double *tz, const double *sx, const double *sy,
+                    const double *sz, const double *sq, int kernel, double p0, double tol,
+                    int *labels, double *errors)
+{
+    return orc_label_pairs_ss(npairs, kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, tx, ty, tz, NULL, sx, sy, sz,
+                              sq, NULL, kernel, p0, 0, 0.0, tol, labels, errors);
 }
 
 /* ---- degree model -------------------------------------------------------- */

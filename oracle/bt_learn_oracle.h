@@ -20,6 +20,15 @@ int orc_label_pairs(int npairs, const int *kind, const int *tbeg, const int *tcn
                     const double *sz, const double *sq, int kernel, double p0, double tol,
                     int *labels, double *errors);
 
+/* The same with the subtraction kernel when ss = 1: target charges tq,
+ * source weights sw, p1 = 1/kappa^2 (Coulomb) or 1/alpha^2 (regularized
+ * Coulomb; p0 = epsilon); Yukawa ignores p1. */
+int orc_label_pairs_ss(int npairs, const int *kind, const int *tbeg, const int *tcnt, const int *sbeg,
+                       const int *scnt, const double *tbox, const double *sbox, const double *tx,
+                       const double *ty, const double *tz, const double *tq, const double *sx,
+                       const double *sy, const double *sz, const double *sq, const double *sw, int kernel,
+                       double p0, int ss, double p1, double tol, int *labels, double *errors);
+
 /* Degree model in FP64: parameter count, logits (10 per sample), and the
  * batch-mean loss with its gradient (grad may be NULL). */
 int orc_mlp_num_params(void);

@@ -216,6 +216,27 @@ def label_pairs(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, tx, ty, tz, sx, sy, sz
     return labels, errors
 
 
+def label_pairs_ss(kind, tbeg, tcnt, sbeg, scnt, tbox, sbox, tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, p0,
+                   p1, tol):
+    """Pair labels of the subtraction kernel (both terms): target charges tq,
+    source weights sw, p1 = 1/kappa^2 (Coulomb) or 1/alpha^2 (regularized
+    Coulomb, p0 = epsilon); Yukawa ignores p1."""
+    l = lib()
+    ints = [np.ascontiguousarray(a, dtype=np.int32) for a in (kind, tbeg, tcnt, sbeg, scnt)]
+    dbl = [np.ascontiguousarray(a, dtype=np.float64) for a in (tbox, sbox, tx, ty, tz, tq, sx, sy, sz, sq, sw)]
+    n = len(ints[0])
+    labels = np.zeros(n, dtype=np.int32)
+    errors = np.zeros((n, 11))
+    l.orc_label_pairs_ss.restype = ctypes.c_int
+    rc = l.orc_label_pairs_ss(ctypes.c_int(n), *[a.ctypes.data_as(_ip) for a in ints],
+                              *[a.ctypes.data_as(_dp) for a in dbl], ctypes.c_int(int(kernel)),
+                              ctypes.c_double(p0), ctypes.c_int(1), ctypes.c_double(p1), ctypes.c_double(tol),
+                              labels.ctypes.data_as(_ip), errors.ctypes.data_as(_dp))
+    if rc != 0:
+        raise RuntimeError("oracle label_pairs_ss failed")
+    return labels, errors
+
+
 def mlp_num_params():
     l = lib()
     l.orc_mlp_num_params.restype = ctypes.c_int

@@ -115,6 +115,70 @@ def test_labels_match_oracle(bt, oracle_mod):
         assert np.array_equal(lg, lo) or border
 
 
+def test_subtraction_labels_match_oracle(bt, oracle_mod):
+    """Labels of the subtraction kernels (both terms) against the oracle's
+    restatement, which scipy's interpolation pins."""
+    rng = np.random.default_rng(22)
+    kinds, tb, tc, sb, sc, tbox, sbox, T, S, Q = make_pairs(rng, 60)
+    tq = rng.uniform(-1, 1, len(T))
+    w = rng.uniform(0.5, 1.5, len(S))
+    for kernel, kp, tol in ((1, [0.5], 1e-7), (2, [0.5], 1e-6), (3, [0.3, 0.01], 1e-8)):
+        print(f"case kernel={kernel} kp={kp}", flush=True)
+        lg, eg = bt.label_pairs_subtraction(kinds, tb, tc, sb, sc, tbox, sbox, (T[:, 0], T[:, 1], T[:, 2], tq),
+                                            (S[:, 0], S[:, 1], S[:, 2], Q, w), kernel, kp, tol)
+        p0 = {1: 0.0, 2: kp[0], 3: kp[-1]}[kernel]
+        p1 = {1: 1.0 / kp[0] ** 2, 2: 0.0, 3: 1.0 / kp[0] ** 2}[kernel]
+        lo, eo = oracle_mod.label_pairs_ss(kinds, tb, tc, sb, sc, tbox, sbox, T[:, 0], T[:, 1], T[:, 2], tq,
+                                           S[:, 0], S[:, 1], S[:, 2], Q, w, kernel, p0, p1, tol)
+        both = ~np.isnan(eo) & ~np.isnan(eg)
+        assert np.allclose(eg[both], eo[both], rtol=1e-8, atol=1e-13)
+        border = np.any(np.abs(eo[both] - tol) <= 1e-13)
+        assert np.array_equal(lg, lo) or border
+
+
+def test_subtraction_plan_labels_follow_the_rule(bt):
+    """A subtraction plan labels its pairs with the subtraction kernel: the
+    rule holds, and zero target charges give the skipping kernel's labels."""
+    t, src = uniform_cube(20000, 20000, seed=6)
+    args = (1, [0.5], 2, 1, 3, 0.8, 6, 300, 300)
+    with bt.Plan(t, src, *args) as p:
+        lab, err = p.label_pairs(1e-7, with_errors=True)
+        for i in range(len(lab)):
+            L = lab[i]
+            if L <= 12:
+                assert err[i, L - 2] <= 1e-7 and np.all(err[i, :L - 2] > 1e-7)
+    t0 = (t[0], t[1], t[2], np.zeros(len(t[0])))
+    ones = (src[0], src[1], src[2], src[3], np.ones(len(src[0])))
+    with bt.Plan(t0, ones, *args) as p0, bt.Plan(t0, ones, 1, [0.5], 1, 1, 3, 0.8, 6, 300, 300) as pk:
+        l0, e0 = p0.label_pairs(1e-7, with_errors=True)
+        lk, ek = pk.label_pairs(1e-7, with_errors=True)
+    assert np.array_equal(l0, lk)
+
+
+def test_learned_subtraction_small_kappa_against_direct(bt):
+    """The shipped Coulomb subtraction model (trained at kappa 0.5) on a
+    system with kappa 0.1: the learned mixed run keeps the error bar against
+    the FP64 direct sum."""
+    from barytree_amd import learned
+    n = 400_000
+    rng = np.random.default_rng(31)
+    x, y, z = (rng.uniform(-1, 1, n) for _ in range(3))
+    q = rng.uniform(-1, 1, n)
+    src = (x, y, z, q, np.ones(n))
+    tgt = (x.copy(), y.copy(), z.copy(), q.copy())
+    m = learned.load_model("coulomb_cc")
+    idx = np.arange(0, n, n // 1000)[:1000]
+    with bt.Plan(tgt, src, 1, [0.1], 2, 1, 3, 0.8, 10, 2000, 2000, precision=bt.PRECISION_MIXED) as p:
+        direct = p.direct_sample(idx)
+        p.use_model(m, 0.5)
+        st = p.stats()
+        out = p.execute()
+    e = rel_l2(direct, out[idx])
+    print(f"kappa 0.1 learned mixed: rel-L2 {e:.2e}, mean degree {st['mean_pair_degree']:.2f}, "
+          f"fallback pairs {st['model_fallback_pairs']}", flush=True)
+    assert e <= 1e-6
+
+
 def test_plan_labels_follow_the_rule(bt):
     t, src = uniform_cube(20000, 20000, seed=3)
     with bt.Plan(t, src, 1, [0.0], 1, 1, 3, 0.8, 6, 300, 300) as p:

@@ -159,6 +159,69 @@ def test_label_errors_match_scipy_interpolation(oracle_mod):
                 assert abs(e - errors[i, p - 2]) <= 1e-12 + 1e-7 * e, (kernel, i, p, e, errors[i, p - 2])
 
 
+def green_w(kernel, p0, p1, d):
+    """The subtraction kernels' second term (the weight's kernel)."""
+    if kernel == 2:
+        return green(kernel, p0, d)
+    return np.exp(-np.sum(d * d, axis=-1) * p1) * green(kernel, p0, d)
+
+
+def scipy_term(kind, X, Y, c, tbox, sbox, p, kern):
+    """(exact, approximation) at X of the potential of charges c at Y with
+    kernel kern(d), interpolated as the treecode does, basis from scipy."""
+    exact = np.array([np.sum(c * kern(x - Y)) for x in X])
+    nl = p + 1
+    gs = np.stack(np.meshgrid(*cheb(sbox, p), indexing="ij"), -1)
+    gt = np.stack(np.meshgrid(*cheb(tbox, p), indexing="ij"), -1)
+    if kind != 2:
+        Bs = [basis_rows(cheb(sbox, p)[d], Y[:, d]) for d in range(3)]
+        Q = np.einsum("si,sj,sk,s->ijk", Bs[0], Bs[1], Bs[2], c)
+    if kind == 1:
+        return exact, np.array([np.sum(Q * kern(x - gs)) for x in X])
+    if kind == 2:
+        V = np.array([np.sum(c * kern(g - Y)) for g in gt.reshape(-1, 3)])
+    else:
+        V = np.array([np.sum(Q.reshape(-1) * kern(g - gs.reshape(-1, 3))) for g in gt.reshape(-1, 3)])
+    Bt = [basis_rows(cheb(tbox, p)[d], X[:, d]) for d in range(3)]
+    return exact, np.einsum("ti,tj,tk,ijk->t", Bt[0], Bt[1], Bt[2], V.reshape(nl, nl, nl))
+
+
+def test_label_errors_ss_match_scipy_interpolation(oracle_mod):
+    """Subtraction labels: the pair potential sum_s (q_s - q_t g_w/g) w_s g,
+    each term interpolated on its own; errors against an independent
+    evaluation through scipy's BarycentricInterpolator, and with zero target
+    charges and unit weights the labels of the skipping kernel."""
+    rng = np.random.default_rng(13)
+    kinds, tb, tc, sb, sc, tbox, sbox, T, S, Q = make_pairs(rng, 9)
+    tq = rng.uniform(-1, 1, len(T))
+    w = rng.uniform(0.5, 1.5, len(S))
+    for kernel, p0, p1 in ((1, 0.0, 4.0), (2, 0.5, 0.0), (3, 0.05, 1.0 / 0.09)):
+        labels, errors = oracle_mod.label_pairs_ss(kinds, tb, tc, sb, sc, tbox, sbox, T[:, 0], T[:, 1], T[:, 2],
+                                                   tq, S[:, 0], S[:, 1], S[:, 2], Q, w, kernel, p0, p1, 1e-30)
+        assert np.all(labels == 13)
+        for i in range(len(kinds)):
+            nt, ns = min(tc[i], 32), min(sc[i], 128)
+            ti = tb[i] + (np.arange(nt) * tc[i]) // nt
+            si = sb[i] + (np.arange(ns) * sc[i]) // ns
+            X, Y = T[ti], S[si]
+            for p in (2, 5, 9):
+                bx, by = tbox[6 * i:6 * i + 6], sbox[6 * i:6 * i + 6]
+                e0, a0 = scipy_term(kinds[i], X, Y, Q[si] * w[si], bx, by, p, lambda d: green(kernel, p0, d))
+                e1, a1 = scipy_term(kinds[i], X, Y, w[si], bx, by, p, lambda d: green_w(kernel, p0, p1, d))
+                ex, ap = e0 - tq[ti] * e1, a0 - tq[ti] * a1
+                e = np.sqrt(np.sum((ap - ex) ** 2) / np.sum(ex ** 2))
+                assert abs(e - errors[i, p - 2]) <= 1e-12 + 1e-7 * e, (kernel, i, p, e, errors[i, p - 2])
+        # the second term vanishes with zero target charges: the skipping kernel's labels
+        l0, e0 = oracle_mod.label_pairs_ss(kinds, tb, tc, sb, sc, tbox, sbox, T[:, 0], T[:, 1], T[:, 2],
+                                           np.zeros(len(T)), S[:, 0], S[:, 1], S[:, 2], Q, np.ones(len(S)),
+                                           kernel, p0, p1, 1e-7)
+        lk, ek = oracle_mod.label_pairs(kinds, tb, tc, sb, sc, tbox, sbox, T[:, 0], T[:, 1], T[:, 2],
+                                        S[:, 0], S[:, 1], S[:, 2], Q, kernel, p0, 1e-7)
+        assert np.array_equal(l0, lk)
+        assert np.array_equal(np.isnan(e0), np.isnan(ek)) and np.allclose(e0[~np.isnan(e0)], ek[~np.isnan(ek)],
+                                                                          rtol=1e-14, atol=0)
+
+
 def test_label_rule_and_convergence(oracle_mod):
     rng = np.random.default_rng(5)
     kinds, tb, tc, sb, sc, tbox, sbox, T, S, Q = make_pairs(rng, 12, spread=4.0)

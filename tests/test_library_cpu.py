@@ -115,6 +115,26 @@ def test_beta_resolution_matches_oracle(bt, oracle_mod):
         assert {k: a[k] for k in KEYS} == {k: o[k] for k in KEYS}
 
 
+def test_beta_leaf_sizes_of_the_gpu_build(bt):
+    """The GPU build's beta leaf sizes (run_params.c:100-103: 3000/3000)
+    on request, the CPU build's by default; theta and degree unchanged."""
+    t, s = uniform_cube(20000, 20000, seed=5)
+    try:
+        cpu = [bt.analyze(t, s, 1, [0.5], 1, 1, ct, -1.0, -1, 1, 1, 1.0, 0.5) for ct in (1, 2, 3)]
+        bt.set_beta_leaf_sizes(bt.BETA_LEAVES_GPU)
+        gpu = [bt.analyze(t, s, 1, [0.5], 1, 1, ct, -1.0, -1, 1, 1, 1.0, 0.5) for ct in (1, 2, 3)]
+        with pytest.raises(RuntimeError):
+            bt.set_beta_leaf_sizes(7)
+    finally:
+        bt.set_beta_leaf_sizes(bt.BETA_LEAVES_CPU)
+    for c, g in zip(cpu, gpu):
+        assert (g["max_per_source_leaf"], g["max_per_target_leaf"]) == (3000, 3000)
+        assert (c["max_per_source_leaf"], c["max_per_target_leaf"]) != (3000, 3000)
+        assert (g["theta"], g["interp_degree"]) == (c["theta"], c["interp_degree"])
+    again = bt.analyze(t, s, 1, [0.5], 1, 1, 3, -1.0, -1, 1, 1, 1.0, 0.5)
+    assert (again["max_per_source_leaf"], again["max_per_target_leaf"]) == (50, 50)
+
+
 def test_unsupported_combinations_are_errors(bt):
     t, s = uniform_cube(100, 100, seed=1)
     # Hermite: no CC, no CP with subtraction (interaction_compute_cc.c:146,
