@@ -66,6 +66,7 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
     s->gpu_near_sym_s = t.gpu_near_sym;
     s->sym_slots = p.sym_slots();
     s->model_fallback_pairs = p.variable_degree() ? p.model_fallback_pairs() : 0;
+    s->sym_evals = p.sym_evals();
 }
 
 // Degrees from the model; a pair whose features lie outside the model's
@@ -93,7 +94,8 @@ int plan_create(bt::Plan &plan, int M, int N, const double *tx, const double *ty
                 const double *tq, const double *sx, const double *sy, const double *sz,
                 const double *sq, const double *sw, int kernel, int nkp, const double *kp,
                 int singularity, int approximation, int compute_type, double theta, int degree,
-                int msl, int mtl, double size_check, double beta, int verbosity, int precision)
+                int msl, int mtl, double size_check, double beta, int verbosity, int precision,
+                bool device = false)
 {
     bt::Params prm;
     bt::params_setup(prm, kernel, nkp, kp, approximation, singularity, compute_type, theta, degree,
@@ -102,42 +104,110 @@ int plan_create(bt::Plan &plan, int M, int N, const double *tx, const double *ty
         return fail("unknown precision mode");
     prm.precision = precision;
     if (verbosity > 0) {
-        std::printf("[BaryTree]\n[BaryTree] RunParams: kernel=%d approximation=%d singularity=%d "
+        std::printf("[BaryTree]\n[BaryTree] BaryTree has started.\n");
+        std::printf("[BaryTree] RunParams: kernel=%d approximation=%d singularity=%d "
                     "compute_type=%d theta=%f degree=%d leaves=%d/%d size_check=%f beta=%f "
                     "precision=%d\n",
                     prm.kernel, prm.approximation, prm.singularity, prm.compute_type, prm.theta,
                     prm.degree, prm.max_source_leaf, prm.max_target_leaf, prm.size_check_factor,
                     prm.beta, prm.precision);
     }
-    std::string e = plan.create(prm, M, N, tx, ty, tz, tq, sx, sy, sz, sq, sw);
+    std::string e = plan.create(prm, M, N, tx, ty, tz, tq, sx, sy, sz, sq, sw, false, device);
     if (!e.empty()) return fail(e);
     return 0;
+}
+
+// Diagnostics after a solve, at the reference's verbosity levels
+// (treedriver.c:53-86, 989-1136): > 0 start and end lines, > 1 sizes and
+// the total time, > 2 the clusterwise and pointwise interaction counts,
+// the point-wise interactions each kind of interaction accounts for, and
+// the time of every phase.  One GPU, one rank: the per-rank minimum and
+// maximum columns equal the totals.
+void print_diagnostics(const BaryTreeStats &s, int M, int N, double total_s, int verbosity)
+{
+    if (verbosity <= 0) return;
+    const char *bar = "|------------------------------------------------------------------------------|";
+    if (verbosity > 1) {
+        std::printf("[BaryTree] %d target and %d source particles on one GPU (one rank).\n", M, N);
+        std::printf("[BaryTree] Total BaryTree time: %1.3f seconds.\n", total_s);
+    }
+    if (verbosity > 2) {
+        auto table = [&](const char *title, const long long v[5]) {
+            std::printf("[BaryTree]\n[BaryTree] %s:\n[BaryTree]\n[BaryTree] %s\n", title, bar);
+            std::printf("[BaryTree] |       |             Cumulative       Minimum per rank       Maximum per rank |\n");
+            const char *names[5] = {"Total", "PP   ", "PC   ", "CP   ", "CC   "};
+            for (int i = 0; i < 5; i++) {
+                std::printf("[BaryTree] %s\n", bar);
+                std::printf("[BaryTree] | %s | %22lld %22lld %22lld |\n", names[i], v[i], v[i], v[i]);
+            }
+            std::printf("[BaryTree] %s\n[BaryTree]\n", bar);
+        };
+        const long long cw[5] = {s.num_pp + s.num_pc + s.num_cp + s.num_cc, s.num_pp, s.num_pc, s.num_cp,
+                                 s.num_cc};
+        table("Clusterwise interaction information", cw);
+        const long long pw[5] = {s.pp_ptwise + s.pc_ptwise + s.cp_ptwise + s.cc_ptwise, s.pp_ptwise,
+                                 s.pc_ptwise, s.cp_ptwise, s.cc_ptwise};
+        table("Pointwise interaction information", pw);
+        const long long pure = s.pp_ptwise + s.pc_replaced + s.cp_replaced + s.cc_replaced;
+        const long long acc[4] = {s.pp_ptwise, s.pc_replaced, s.cp_replaced, s.cc_replaced};
+        const char *names[4] = {"PP   ", "PC   ", "CP   ", "CC   "};
+        const char *bar2 = "|-------------------------------------------------------|";
+        std::printf("[BaryTree]\n[BaryTree] Pointwise interactions accounted for:\n[BaryTree]\n");
+        std::printf("[BaryTree] %s\n[BaryTree] |       |                 Number             Percentage |\n", bar2);
+        std::printf("[BaryTree] %s\n[BaryTree] | Total | %22lld            100.000000%% |\n", bar2, pure);
+        for (int i = 0; i < 4; i++)
+            std::printf("[BaryTree] %s\n[BaryTree] | %s | %22lld %21.6f%% |\n", bar2, names[i], acc[i],
+                        pure > 0 ? 100.0 * (double)acc[i] / (double)pure : 0.0);
+        std::printf("[BaryTree] %s\n[BaryTree]\n", bar2);
+        // phase times (this build's phases; the reference's are its own)
+        std::printf("[BaryTree] Phase times (seconds):\n");
+        const struct { const char *name; double t; } ph[] = {
+            {"host: argument checks and copies", s.host_copy_s}, {"trees", s.host_tree_s},
+            {"host: interaction lists", s.host_lists_s}, {"host: work lists and tables", s.host_work_s},
+            {"uploads", s.h2d_s}, {"GPU: upward pass", s.gpu_upward_s},
+            {"GPU: near field (and PC far field)", s.gpu_near_s}, {"GPU: far field", s.gpu_far_s},
+            {"GPU: downpass", s.gpu_down_s}, {"GPU: correction and caller order", s.gpu_finish_s}};
+        for (const auto &x : ph) std::printf("[BaryTree]   %-38s %10.6f\n", x.name, x.t);
+        std::printf("[BaryTree]\n");
+    }
+    std::printf("[BaryTree] BaryTree has finished.\n");
+    std::fflush(stdout);
+}
+
+// The output array is zeroed first (host memory, or device memory).
+int zero_output(double *out, int M, bool device)
+{
+    if (M <= 0) return 0;
+    if (!device) {
+        std::memset(out, 0, (size_t)M * sizeof(double));
+        return 0;
+    }
+    hipError_t e = hipMemset(out, 0, (size_t)M * sizeof(double));
+    return e == hipSuccess ? 0 : fail(std::string("outputArray: ") + hipGetErrorString(e));
 }
 
 int solve(int M, int N, const double *tx, const double *ty, const double *tz, const double *tq,
           const double *sx, const double *sy, const double *sz, const double *sq, const double *sw,
           double *out, int kernel, int nkp, const double *kp, int singularity, int approximation,
           int compute_type, double theta, int degree, int msl, int mtl, double size_check,
-          double beta, int verbosity, int precision)
+          double beta, int verbosity, int precision, bool device = false)
 {
     g_last_error.clear();
     if (!out) return fail("outputArray is NULL");
-    if (M > 0) std::memset(out, 0, (size_t)M * sizeof(double));
+    const auto t_start = std::chrono::steady_clock::now();
+    if (int z = zero_output(out, M, device)) return z;
     bt::Plan plan;
     int rc = plan_create(plan, M, N, tx, ty, tz, tq, sx, sy, sz, sq, sw, kernel, nkp, kp, singularity,
                          approximation, compute_type, theta, degree, msl, mtl, size_check, beta,
-                         verbosity, precision);
+                         verbosity, precision, device);
     if (rc) return rc;
     std::string e = plan.execute(true);
-    if (e.empty()) e = plan.download(out);
+    if (e.empty()) e = plan.download(out, device);
     if (!e.empty()) return fail(e);
     fill_stats(plan, &g_stats);
     g_have_stats = true;
-    if (verbosity > 0) {
-        std::printf("[BaryTree] host tree %.3fs lists %.3fs; gpu total %.3fs (near %.3fs far %.3fs)\n",
-                    g_stats.host_tree_s, g_stats.host_lists_s, g_stats.gpu_total_s,
-                    g_stats.gpu_near_s, g_stats.gpu_far_s);
-    }
+    print_diagnostics(g_stats, M, N,
+                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(), verbosity);
     return 0;
 }
 
@@ -581,6 +651,7 @@ int BaryTree_SolveBatched(int numSystems, const int *targetOffsets, const int *s
 {
     g_last_error.clear();
     if (!outputArray) return fail("outputArray is NULL");
+    const auto t_start = std::chrono::steady_clock::now();
     std::string e = check_offsets(numSystems, targetOffsets, sourceOffsets);
     if (!e.empty()) return fail(e);
     std::memset(outputArray, 0, (size_t)targetOffsets[numSystems] * sizeof(double));
@@ -599,10 +670,9 @@ int BaryTree_SolveBatched(int numSystems, const int *targetOffsets, const int *s
     if (!e.empty()) return fail(e);
     fill_stats(plan, &g_stats);
     g_have_stats = true;
-    if (verbosity > 0) {
-        std::printf("[BaryTree] batched: %d systems, host tree %.3fs lists %.3fs; gpu total %.3fs\n",
-                    numSystems, g_stats.host_tree_s, g_stats.host_lists_s, g_stats.gpu_total_s);
-    }
+    if (verbosity > 1) std::printf("[BaryTree] batched: %d independent systems.\n", numSystems);
+    print_diagnostics(g_stats, targetOffsets[numSystems], sourceOffsets[numSystems],
+                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(), verbosity);
     return 0;
 }
 
@@ -987,6 +1057,39 @@ int BaryTreeModel_Predict(const BaryTreeDegreeModel *model, int n, const float *
 
 void BaryTreeModel_Destroy(BaryTreeDegreeModel *model) { delete model; }
 
+static int solve_learned(int numTargets, int numSources, const double *targetX,
+                          const double *targetY, const double *targetZ, const double *targetValue,
+                          const double *sourceX, const double *sourceY, const double *sourceZ,
+                          const double *sourceValue, const double *sourceWeight,
+                          double *outputArray, int kernel, int numKernelParams,
+                          const double *kernelParams, int singularity, int approximation,
+                          int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                          int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                          int precision, const BaryTreeDegreeModel *model, float delta, bool device)
+{
+    g_last_error.clear();
+    if (!outputArray) return fail("outputArray is NULL");
+    const auto t_start = std::chrono::steady_clock::now();
+    if (int z = zero_output(outputArray, numTargets, device)) return z;
+    bt::Plan plan;
+    int rc = plan_create(plan, numTargets, numSources, targetX, targetY, targetZ, targetValue,
+                         sourceX, sourceY, sourceZ, sourceValue, sourceWeight, kernel,
+                         numKernelParams, kernelParams, singularity, approximation, compute_type,
+                         theta, interpOrder, maxPerSourceLeaf, maxPerTargetLeaf, sizeCheck, beta,
+                         verbosity, precision, device);
+    if (rc) return rc;
+    std::string e;
+    if (model) e = use_model(plan, model->m, delta);
+    if (e.empty()) e = plan.execute(true);
+    if (e.empty()) e = plan.download(outputArray, device);
+    if (!e.empty()) return fail(e);
+    fill_stats(plan, &g_stats);
+    g_have_stats = true;
+    print_diagnostics(g_stats, numTargets, numSources,
+                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(), verbosity);
+    return 0;
+}
+
 int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
                           const double *targetY, const double *targetZ, const double *targetValue,
                           const double *sourceX, const double *sourceY, const double *sourceZ,
@@ -997,24 +1100,41 @@ int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
                           int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
                           int precision, const BaryTreeDegreeModel *model, float delta)
 {
-    g_last_error.clear();
-    if (!outputArray) return fail("outputArray is NULL");
-    if (numTargets > 0) std::memset(outputArray, 0, (size_t)numTargets * sizeof(double));
-    bt::Plan plan;
-    int rc = plan_create(plan, numTargets, numSources, targetX, targetY, targetZ, targetValue,
-                         sourceX, sourceY, sourceZ, sourceValue, sourceWeight, kernel,
-                         numKernelParams, kernelParams, singularity, approximation, compute_type,
-                         theta, interpOrder, maxPerSourceLeaf, maxPerTargetLeaf, sizeCheck, beta,
-                         verbosity, precision);
-    if (rc) return rc;
-    std::string e;
-    if (model) e = use_model(plan, model->m, delta);
-    if (e.empty()) e = plan.execute(true);
-    if (e.empty()) e = plan.download(outputArray);
-    if (!e.empty()) return fail(e);
-    fill_stats(plan, &g_stats);
-    g_have_stats = true;
-    return 0;
+    return solve_learned(numTargets, numSources, targetX, targetY, targetZ, targetValue, sourceX, sourceY,
+                         sourceZ, sourceValue, sourceWeight, outputArray, kernel, numKernelParams, kernelParams,
+                         singularity, approximation, compute_type, theta, interpOrder, maxPerSourceLeaf,
+                         maxPerTargetLeaf, sizeCheck, beta, verbosity, precision, model, delta, false);
+}
+
+int BaryTree_SolveLearnedDevice(int numTargets, int numSources, const double *targetX,
+                                const double *targetY, const double *targetZ, const double *targetValue,
+                                const double *sourceX, const double *sourceY, const double *sourceZ,
+                                const double *sourceValue, const double *sourceWeight,
+                                double *outputArray, int kernel, int numKernelParams,
+                                const double *kernelParams, int singularity, int approximation,
+                                int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                                int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                                int precision, const BaryTreeDegreeModel *model, float delta)
+{
+    return solve_learned(numTargets, numSources, targetX, targetY, targetZ, targetValue, sourceX, sourceY,
+                         sourceZ, sourceValue, sourceWeight, outputArray, kernel, numKernelParams, kernelParams,
+                         singularity, approximation, compute_type, theta, interpOrder, maxPerSourceLeaf,
+                         maxPerTargetLeaf, sizeCheck, beta, verbosity, precision, model, delta, true);
+}
+
+int BaryTree_SolveDevice(int numTargets, int numSources, const double *targetX, const double *targetY,
+                         const double *targetZ, const double *targetValue, const double *sourceX,
+                         const double *sourceY, const double *sourceZ, const double *sourceValue,
+                         const double *sourceWeight, double *outputArray, int kernel,
+                         int numKernelParams, const double *kernelParams, int singularity,
+                         int approximation, int compute_type, double theta, int interpOrder,
+                         int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck, double beta,
+                         int verbosity, int precision)
+{
+    return solve(numTargets, numSources, targetX, targetY, targetZ, targetValue, sourceX, sourceY,
+                 sourceZ, sourceValue, sourceWeight, outputArray, kernel, numKernelParams,
+                 kernelParams, singularity, approximation, compute_type, theta, interpOrder,
+                 maxPerSourceLeaf, maxPerTargetLeaf, sizeCheck, beta, verbosity, precision, true);
 }
 
 

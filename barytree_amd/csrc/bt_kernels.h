@@ -95,13 +95,15 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
 // Symmetric near field (mixed precision; Coulomb and Yukawa with either
 // singularity, regularized Coulomb with skipping; a cluster-cluster system
-// whose targets are its sources).  An item is a pair
+// whose targets are its sources).  A two-sided item (cslot >= 0) is a pair
 // of distinct leaves A, B, each in the other's near list: every pair (a, b)
 // is evaluated once and gives both a's term from b and b's term from a.
-// a's sums go to slots[rslot + i] (i-th particle of A), b's to
-// slots[cslot + j]; ctr[k] = centre of the pair's boxes and its r^2
-// threshold (negative: pairs below it are recomputed from coordinate
-// differences, with the skip of coincident pairs).
+// A one-sided item (cslot < 0) is a leaf A with a source range B of its near
+// list that does not list A back (or A itself): A's terms only.  a's sums go
+// to slots[rslot + i] (i-th particle of A), b's to slots[cslot + j];
+// ctr[k] = centre of the pair's boxes and its r^2 threshold (negative:
+// pairs below it are recomputed from coordinate differences, with the skip
+// of coincident pairs).
 struct SymItem {
     int a_beg, a_cnt, b_beg, b_cnt;
     long long rslot, cslot;
@@ -194,6 +196,11 @@ hipError_t launch_cluster_points(int numnodes, int nl, const double *tab, double
 // parent in `nodes`, points = the grids of its children (child CSR
 // child_off/child_id); mode 2: block b takes the node tables of nodes[b],
 // points = particles [ibeg[b], iend[b]), output row b (partial charges).
+// mode 0: block b anterpolates the particles of node nodes[b] into its row;
+// mode 2: the particles [ibeg[b], iend[b]) of node nodes[b] into row b;
+// mode 1: the grids of node nodes[b]'s children child_id[child_off[b] ..
+// child_off[b+1]) into its row; mode 3: as mode 1 into row child_off[b] of
+// Qo (Wo), the children's charges read from Q (W).
 struct AnterpArgs {
     int mode, degree, subtraction, nblocks;
     const int *nodes;
@@ -202,11 +209,13 @@ struct AnterpArgs {
     const double *tab;                                  // node tables
     const double *px, *py, *pz, *pq, *pw;              // particles (mode 0)
     double *Q, *W;                                      // cluster charges (in/out)
+    double *Qo, *Wo;  // mode 3: the entries' rows (the children's charges are read from Q, W)
 };
 hipError_t launch_anterp(const AnterpArgs &a, hipStream_t st);
-// Q[s][j] = sum_{k in [off[s], off[s+1])} Qp[k][j] (W likewise if W is set), in k order.
+// Q[d][j] = sum_{k in [off[s], off[s+1])} Qp[k][j] (W likewise if W is set), in k order,
+// d = dst[s] (s without dst).
 hipError_t launch_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
-                            double *Q, double *W, hipStream_t st);
+                            double *Q, double *W, hipStream_t st, const int *dst = nullptr);
 // Q[s][j] = sum_{k in [off[s], off[s+1])} R[row[k] + j] (W from RW likewise), in k order;
 // zero where a slot has no rows.
 hipError_t launch_sum_rows(int nslots, int P, const int *off, const long long *row, const double *R,

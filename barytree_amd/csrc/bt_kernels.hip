@@ -646,7 +646,8 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // A, added after: the block owns its slots, so the order is fixed).  In
 // items flagged close, a wave whose chunk held a pair below the threshold
 // evaluates the chunk again from coordinate differences (coincident pairs
-// add exactly zero, the reference's skip).
+// add exactly zero, the reference's skip).  One-sided items run the same
+// loop without the sources' sums.
 // ---------------------------------------------------------------------
 constexpr int SYM_TPT = 4, SYM_G = 8, SYM_NW = 4;
 
@@ -726,6 +727,9 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
         }
     };
 
+    // TWO: the sources' sums as well (block-uniform)
+    auto body = [&](auto two_tag) {
+    constexpr bool TWO = decltype(two_tag)::value;
     for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
         double Tx[TPT], Ty[TPT], Tz[TPT], T2[TPT], Tq[TPT], acc[TPT];
         float ai[TPT], bi[TPT], p1[TPT], p2[TPT];
@@ -778,7 +782,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
                     const int j = NG * st + cg;
                     const double4 S = Ls[j];
                     const float2 f = Lf[j];
-                    float c1 = 0.0f, c2 = 0.0f;
+                    [[maybe_unused]] float c1 = 0.0f, c2 = 0.0f;
 #pragma unroll
                     for (int k = 0; k < TPT; k++) {
                         float r2;
@@ -793,15 +797,17 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
                         float fa, fb = 0.0f;
                         sym_pair<KER, SS, EXACT && KER != KR>(r2, eps2s, fa, fb);
                         p1[k] = fmaf(f.x, fa, p1[k]);
-                        c1 = fmaf(ai[k], fa, c1);
+                        if constexpr (TWO) c1 = fmaf(ai[k], fa, c1);
                         if constexpr (SS) {
                             p2[k] = fmaf(f.y, fb, p2[k]);
-                            c2 = fmaf(bi[k], fb, c2);
+                            if constexpr (TWO) c2 = fmaf(bi[k], fb, c2);
                         }
                     }
-                    c1 = group_sum<G>(c1);
-                    if constexpr (SS) c2 = group_sum<G>(c2);
-                    if (rg == G - 1) Lc[wv][j] = make_float2(c1, c2);
+                    if constexpr (TWO) {
+                        c1 = group_sum<G>(c1);
+                        if constexpr (SS) c2 = group_sum<G>(c2);
+                        if (rg == G - 1) Lc[wv][j] = make_float2(c1, c2);
+                    }
                 }
                 // the FP32 sums of (at most) 32 steps into FP64
 #pragma unroll
@@ -823,14 +829,16 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
                     run(T{}, F{});
                 }
             }
-            __syncthreads();
-            if (tid < n) {
-                double s1 = 0.0, s2 = 0.0;
+            if constexpr (TWO) {
+                __syncthreads();
+                if (tid < n) {
+                    double s1 = 0.0, s2 = 0.0;
 #pragma unroll
-                for (int w = 0; w < NW; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
-                double *slot = A.slots + it.cslot + off + tid;
-                const double v = SS ? s1 - Lq[SS ? tid : 0] * s2 : s1;
-                *slot = p0 == 0 ? v : *slot + v;
+                    for (int w = 0; w < NW; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
+                    double *slot = A.slots + it.cslot + off + tid;
+                    const double v = SS ? s1 - Lq[SS ? tid : 0] * s2 : s1;
+                    *slot = p0 == 0 ? v : *slot + v;
+                }
             }
         }
         // a target's sum over the lanes that hold it
@@ -843,6 +851,9 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
             if (cg == 0 && li < it.a_cnt) A.slots[it.rslot + li] = r;
         }
     }
+    };
+    if (it.cslot >= 0) body(std::true_type{});
+    else body(std::false_type{});
 }
 
 // ---------------------------------------------------------------------
@@ -856,7 +867,9 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
 // one-directional launch.  Sums stay FP64 throughout: a target's over all of
 // B in registers, a source's over its group's lanes by a DPP row_shr scan
 // of the two 32-bit halves, then over the waves in LDS.  CHECK (items whose
-// boxes touch) applies the reference's skip of coincident pairs.
+// boxes touch) applies the reference's skip of coincident pairs.  One-sided
+// items (a leaf and a source range that does not list it, or itself) form
+// the targets' sums only.
 // ---------------------------------------------------------------------
 constexpr int SYM64_TPT = 4, SYM64_G = 8, SYM64_NW = 4;
 
@@ -891,6 +904,9 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
     constexpr int TPASS = NW * G * TPT;
     const KParams kp = A.kp;
 
+    // TWO: the sources' sums as well (block-uniform)
+    auto body = [&](auto two_tag) {
+    constexpr bool TWO = decltype(two_tag)::value;
     for (int p0 = 0; p0 < it.a_cnt; p0 += TPASS) {
         // targets: coordinates, -q (subtraction), a = q w (q), w; acc
         double Tx[TPT], Ty[TPT], Tz[TPT], Tnq[TPT], Ta[TPT], Tw[TPT], acc[TPT];
@@ -936,7 +952,7 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
                     const double4 S = Ls[j];
                     double sw = 1.0, sq = 0.0;
                     if constexpr (SS) { const double2 u = Lw[j]; sw = u.x; sq = u.y; }
-                    double c = 0.0;
+                    [[maybe_unused]] double c = 0.0;
 #pragma unroll
                     for (int k = 0; k < TPT; k++) {
                         const double dx = Tx[k] - S.x, dy = Ty[k] - S.y, dz = Tz[k] - S.z;
@@ -946,11 +962,11 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
                             if constexpr (CHECK) ir = r2 > (SS ? 0.0 : DBL_MIN) ? ir : 0.0;
                             if constexpr (!SS) {
                                 acc[k] = fma(S.w, ir, acc[k]);
-                                c = fma(Ta[k], ir, c);
+                                if constexpr (TWO) c = fma(Ta[k], ir, c);
                             } else {
                                 const double e = exp(-r2 * kp.inv_a2);
                                 acc[k] = fma(fma(Tnq[k] * sw, e, S.w), ir, acc[k]);
-                                c = fma(fma(-sq * Tw[k], e, Ta[k]), ir, c);
+                                if constexpr (TWO) c = fma(fma(-sq * Tw[k], e, Ta[k]), ir, c);
                             }
                         } else if constexpr (KER == KY) {
                             const double ir = rsq64(r2);
@@ -958,30 +974,34 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
                             if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
                             if constexpr (!SS) {
                                 acc[k] = fma(S.w, e, acc[k]);
-                                c = fma(Ta[k], e, c);
+                                if constexpr (TWO) c = fma(Ta[k], e, c);
                             } else {
                                 acc[k] = fma(fma(Tnq[k], sw, S.w), e, acc[k]);
-                                c = fma(fma(-sq, Tw[k], Ta[k]), e, c);
+                                if constexpr (TWO) c = fma(fma(-sq, Tw[k], Ta[k]), e, c);
                             }
                         } else {  // regularized Coulomb, skipping
                             const double ir = rsq64(r2 + kp.eps2);
                             acc[k] = fma(S.w, ir, acc[k]);
-                            c = fma(Ta[k], ir, c);
+                            if constexpr (TWO) c = fma(Ta[k], ir, c);
                         }
                     }
-                    c = group_sum64<G>(c);
-                    if (rg == G - 1) Lc[wv][j] = c;
+                    if constexpr (TWO) {
+                        c = group_sum64<G>(c);
+                        if (rg == G - 1) Lc[wv][j] = c;
+                    }
                 }
             };
             if (check && KER != KR) run(std::integral_constant<bool, true>{});
             else run(std::integral_constant<bool, false>{});
-            __syncthreads();
-            if (tid < n) {
-                double s = 0.0;
+            if constexpr (TWO) {
+                __syncthreads();
+                if (tid < n) {
+                    double s = 0.0;
 #pragma unroll
-                for (int w = 0; w < NW; w++) s += Lc[w][tid];
-                double *slot = A.slots + it.cslot + off + tid;
-                *slot = p0 == 0 ? s : *slot + s;
+                    for (int w = 0; w < NW; w++) s += Lc[w][tid];
+                    double *slot = A.slots + it.cslot + off + tid;
+                    *slot = p0 == 0 ? s : *slot + s;
+                }
             }
         }
 #pragma unroll
@@ -993,6 +1013,9 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
             if (cg == 0 && li < it.a_cnt) A.slots[it.rslot + li] = r;
         }
     }
+    };
+    if (it.cslot >= 0) body(std::true_type{});
+    else body(std::false_type{});
 }
 
 __global__ __launch_bounds__(TPB_SYM_REDUCE) void k_sym_reduce(SymReduceArgs A)
@@ -1459,10 +1482,12 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
 
     // point sets: mode 0 one range of particles (the node's); mode 2 one
     // range of particles (the block's part of a node); mode 1 the grids of
-    // all children
+    // all children; mode 3 the grids of the block's children (one entry of
+    // the parent-child list), into the entry's row child_off[b]
+    const bool kids = A.mode == 1 || A.mode == 3;
     int nsets = 1;
-    if (A.mode == 1) nsets = A.child_off[blockIdx.x + 1] - A.child_off[blockIdx.x];
-    const int orow = A.mode == 2 ? (int)blockIdx.x : node;  // output row
+    if (kids) nsets = A.child_off[blockIdx.x + 1] - A.child_off[blockIdx.x];
+    const int orow = A.mode == 2 ? (int)blockIdx.x : (A.mode == 3 ? A.child_off[blockIdx.x] : node);
     for (int set = 0; set < nsets; ++set) {
         int pbeg = 0, pcnt = P, child = -1;
         if (A.mode == 0) {
@@ -1481,7 +1506,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
                 const bool valid = c0 + p < pcnt;
                 double coord = 0.0;
                 if (valid) {
-                    if (A.mode != 1) {
+                    if (!kids) {
                         const int g = pbeg + c0 + p;
                         coord = d == 0 ? A.px[g] : (d == 1 ? A.py[g] : A.pz[g]);
                     } else {
@@ -1516,7 +1541,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
                     den *= Lsum[p];
                     den *= Lsum[ACH + p];
                     den *= Lsum[2 * ACH + p];
-                    if (A.mode != 1) {
+                    if (!kids) {
                         const int g = pbeg + c0 + p;
                         const double q = A.pq[g];
                         if (SS) { const double w = A.pw[g]; f1 = q * w; f2 = w; }
@@ -1549,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_anterp(AnterpArgs A)
     }
     // lane l holds rows (k1, k2) = r, r = rt*16 + l/16 + 4v, of column bc
     if (b_k3 < 0) return;
-    double *out = b_f == 0 ? A.Q : A.W;
+    double *out = A.mode == 3 ? (b_f == 0 ? A.Qo : A.Wo) : (b_f == 0 ? A.Q : A.W);
 #pragma unroll
     for (int t = 0; t < TPW; t++) {
         if (rt0 + RSTEP * t >= RT) continue;
@@ -1734,10 +1759,10 @@ hipError_t anterp_dispatch(const AnterpArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-// Q[s*P + j] = sum over parts k in [off[s], off[s+1]) of Qp[k*P + j] (and W),
-// in part order
+// Q[d*P + j] = sum over parts k in [off[s], off[s+1]) of Qp[k*P + j] (and W),
+// in part order; d = dst[s], or s without dst
 __global__ void k_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
-                            double *Q, double *W)
+                            double *Q, double *W, const int *dst)
 {
     const size_t total = (size_t)nslots * P;
     for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -1748,8 +1773,9 @@ __global__ void k_sum_parts(int nslots, int P, const int *off, const double *Qp,
             q += Qp[(size_t)k * P + j];
             if (W) w += Wp[(size_t)k * P + j];
         }
-        Q[g] = q;
-        if (W) W[g] = w;
+        const size_t o = dst ? (size_t)dst[s] * P + j : g;
+        Q[o] = q;
+        if (W) W[o] = w;
     }
 }
 
@@ -1891,12 +1917,12 @@ hipError_t launch_sym_reduce(const SymReduceArgs &a, hipStream_t st)
 }
 
 hipError_t launch_sum_parts(int nslots, int P, const int *off, const double *Qp, const double *Wp,
-                            double *Q, double *W, hipStream_t st)
+                            double *Q, double *W, hipStream_t st, const int *dst)
 {
     const size_t total = (size_t)nslots * P;
     if (total == 0) return hipSuccess;
     int blocks = (int)std::min<size_t>((total + 255) / 256, 65535);
-    k_sum_parts<<<blocks, 256, 0, st>>>(nslots, P, off, Qp, Wp, Q, W);
+    k_sum_parts<<<blocks, 256, 0, st>>>(nslots, P, off, Qp, Wp, Q, W, dst);
     return hipGetLastError();
 }
 

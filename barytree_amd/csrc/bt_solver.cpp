@@ -192,8 +192,10 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
 // nodes (leaves or batches, disjoint particle ranges) with the source nodes
 // of their near lists (segs_of, indexed by target node).  A source node whose
 // particle range is another key's range, while that key lists this key's
-// range, makes a symmetric pair: one item (the larger-work pairs first), and
-// both ranges are flagged SEG_SYM in sym_of (parallel to segs_of).  The
+// range, makes a symmetric pair: one two-sided item.  Every other source
+// node (one that does not list the key back, and the key's own range) makes
+// a one-sided item.  Every range is flagged SEG_SYM in sym_of (parallel to
+// segs_of): the near launch leaves the particle pairs to the items.  The
 // item's centre c is the centre of the two boxes' joint box and R the
 // largest distance from c to either box: r^2 is formed as in the near keys
 // (|t-c|^2 + |s-c|^2 - 2 (t-c).(s-c), rounding error below
@@ -205,7 +207,7 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
     const Tree &st = sy.stree, &tt = sy.ttree;
     sym_items_.clear(); sym_ctr_.clear(); sym_red_tiles_.clear();
     sym_red_off_.clear(); sym_red_base_.clear();
-    sym_slots_len_ = 0; sym_pairs_ = 0;
+    sym_slots_len_ = 0; sym_pairs_ = 0; sym_evals_ = 0;
     // key of a particle range [beg, beg + cnt)
     std::vector<std::array<long long, 2>> key_range;  // (beg << 32 | cnt, key index)
     for (size_t k = 0; k < keys.size(); k++)
@@ -226,22 +228,25 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         }
     std::sort(present.begin(), present.end());
     auto has = [&](long long a, long long b) { return std::binary_search(present.begin(), present.end(), a * nk + b); };
+    // kb >= 0: two-sided (the pair of keys k < kb), kb < 0: one-sided
     struct Cand { int k, kb, s; };
     std::vector<Cand> cand;
     for (size_t k = 0; k < keys.size(); k++) {
         const int t = keys[k];
-        sym_of[t].assign(segs_of[t].size(), 0);
+        sym_of[t].assign(segs_of[t].size(), 1);
         for (size_t i = 0; i < segs_of[t].size(); i++) {
             const int s = segs_of[t][i];
             const int ks = key_of(st.ibeg[s], st.numpar[s]);
-            if (ks < 0 || ks == (int)k || !has(ks, (long long)k)) continue;
-            sym_of[t][i] = 1;
-            if ((int)k < ks) cand.push_back({(int)k, ks, s});
+            const bool two = ks >= 0 && ks != (int)k && has(ks, (long long)k);
+            if (!two) cand.push_back({(int)k, -1, s});
+            else if ((int)k < ks) cand.push_back({(int)k, ks, s});
         }
     }
-    std::stable_sort(cand.begin(), cand.end(), [&](const Cand &a, const Cand &b) {
-        return (long long)tt.numpar[keys[a.k]] * st.numpar[a.s] > (long long)tt.numpar[keys[b.k]] * st.numpar[b.s];
-    });
+    // work of an item: a two-sided pair costs about twice a one-sided one
+    auto work = [&](const Cand &c) {
+        return (long long)tt.numpar[keys[c.k]] * st.numpar[c.s] * (c.kb >= 0 ? 2 : 1);
+    };
+    std::stable_sort(cand.begin(), cand.end(), [&](const Cand &a, const Cand &b) { return work(a) > work(b); });
     std::vector<std::vector<long long>> lists(keys.size());
     auto far2 = [](double cx, double cy, double cz, const Tree &T, int n) {
         const double dx = std::max(std::fabs(T.xmin[n] - cx), std::fabs(T.xmax[n] - cx));
@@ -256,7 +261,8 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         it.a_beg = sy.soff + tt.ibeg[t]; it.a_cnt = tt.numpar[t];
         it.b_beg = sy.soff + st.ibeg[s]; it.b_cnt = st.numpar[s];
         it.rslot = sym_slots_len_; sym_slots_len_ += it.a_cnt;
-        it.cslot = sym_slots_len_; sym_slots_len_ += it.b_cnt;
+        it.cslot = -1;
+        if (c.kb >= 0) { it.cslot = sym_slots_len_; sym_slots_len_ += it.b_cnt; }
         const double cx = 0.5 * (std::min(tt.xmin[t], st.xmin[s]) + std::max(tt.xmax[t], st.xmax[s]));
         const double cy = 0.5 * (std::min(tt.ymin[t], st.ymin[s]) + std::max(tt.ymax[t], st.ymax[s]));
         const double cz = 0.5 * (std::min(tt.zmin[t], st.zmin[s]) + std::max(tt.zmax[t], st.zmax[s]));
@@ -269,8 +275,9 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         sym_ctr_.push_back(cx); sym_ctr_.push_back(cy); sym_ctr_.push_back(cz);
         sym_ctr_.push_back(close ? -thresh : thresh);
         lists[c.k].push_back(it.rslot - (sy.toff + tt.ibeg[t]));
-        lists[c.kb].push_back(it.cslot - (sy.toff + st.ibeg[s]));
-        sym_pairs_ += 2LL * it.a_cnt * it.b_cnt;
+        if (c.kb >= 0) lists[c.kb].push_back(it.cslot - (sy.toff + st.ibeg[s]));
+        sym_pairs_ += (c.kb >= 0 ? 2LL : 1LL) * it.a_cnt * it.b_cnt;
+        sym_evals_ += (long long)it.a_cnt * it.b_cnt;
         sym_items_.push_back(it);
     }
     // Launch order.  Workgroups are dealt round-robin over the 8 XCDs, so
@@ -282,11 +289,11 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
     // items.  Slots and sums do not depend on the order.
     {
         std::map<int, std::vector<int>> by_b;
-        for (size_t i = 0; i < cand.size(); i++) by_b[cand[i].kb].push_back((int)i);
+        for (size_t i = 0; i < cand.size(); i++) by_b[sym_items_[i].b_beg].push_back((int)i);
         std::vector<std::pair<long long, const std::vector<int> *>> groups;
         for (const auto &g : by_b) {
             long long w = 0;
-            for (int i : g.second) w += (long long)sym_items_[i].a_cnt * sym_items_[i].b_cnt;
+            for (int i : g.second) w += work(cand[i]);
             groups.push_back({w, &g.second});
         }
         std::stable_sort(groups.begin(), groups.end(),
@@ -620,8 +627,8 @@ std::string Plan::validate() const
             if (it.a_beg < 0 || it.a_cnt <= 0 || (long long)it.a_beg + it.a_cnt > N_ || it.b_beg < 0 ||
                 it.b_cnt <= 0 || (long long)it.b_beg + it.b_cnt > N_ || N_ != M_)
                 return "sym item range";
-            if (it.rslot < 0 || it.rslot + it.a_cnt > sym_slots_len_ || it.cslot < 0 ||
-                it.cslot + it.b_cnt > sym_slots_len_)
+            if (it.rslot < 0 || it.rslot + it.a_cnt > sym_slots_len_ || it.cslot < -1 ||
+                (it.cslot >= 0 && it.cslot + it.b_cnt > sym_slots_len_))
                 return "sym item slots";
         }
         const int nl = (int)sym_red_off_.size() - 1;
@@ -679,7 +686,8 @@ std::string Plan::validate() const
 
 std::string Plan::create(const Params &prm, int M, int N, const double *tx, const double *ty,
                          const double *tz, const double *tq, const double *sx, const double *sy,
-                         const double *sz, const double *sq, const double *sw, bool host_only)
+                         const double *sz, const double *sq, const double *sw, bool host_only,
+                         bool device_inputs)
 {
     const int toffs[2] = {0, M}, soffs[2] = {0, N};
     PlanInput in;
@@ -688,6 +696,7 @@ std::string Plan::create(const Params &prm, int M, int N, const double *tx, cons
     in.nsys = 1; in.toffs = toffs; in.soffs = soffs; in.tpt = M < SMALL_TARGETS ? 1 : 2;
     in.tx = tx; in.ty = ty; in.tz = tz; in.tq = tq;
     in.sx = sx; in.sy = sy; in.sz = sz; in.sq = sq; in.sw = sw;
+    in.device = device_inputs;
     return create(prm, in, host_only);
 }
 
@@ -706,6 +715,7 @@ std::string Plan::create(const Params &prm, const PlanInput &in, bool host_only)
         if (in.toffs[s + 1] - in.toffs[s] <= 0 || in.soffs[s + 1] - in.soffs[s] <= 0)
             return "numTargets and numSources must be positive (every system)";
     if (in.toffs[0] != 0 || in.soffs[0] != 0) return "system offsets must start at 0";
+    if (in.device && host_only) return "device-resident particles need a GPU plan";
     M_ = in.toffs[in.nsys];
     N_ = in.soffs[in.nsys];
     if (!in.tx || !in.ty || !in.tz || !in.sx || !in.sy || !in.sz || !in.sq)
@@ -812,7 +822,22 @@ std::string Plan::device_trees(const PlanInput &in)
     bool same = M_ == N_ && in.tq != nullptr;
     for (int s = 0; same && s < (int)sys_.size(); s++)
         same = in.toffs[s] == in.soffs[s] && sys_[s].msl == sys_[s].mtl;
-    if (same && !(in.tx == in.sx && in.ty == in.sy && in.tz == in.sz && in.tq == in.sq)) {
+    const bool alias = in.tx == in.sx && in.ty == in.sy && in.tz == in.sz && in.tq == in.sq;
+    if (same && !alias && in.device) {
+        // bitwise equality on the device
+        int *flag = dalloc<int>(1);
+        if (!flag) return "hipMalloc failed (comparison)";
+        HIPCHK(hipMemsetAsync(flag, 0, sizeof(int), st_));
+        HIPCHK(btk::launch_differ(M_, in.tx, in.sx, flag, st_));
+        HIPCHK(btk::launch_differ(M_, in.ty, in.sy, flag, st_));
+        HIPCHK(btk::launch_differ(M_, in.tz, in.sz, flag, st_));
+        HIPCHK(btk::launch_differ(M_, in.tq, in.sq, flag, st_));
+        int differ = 0;
+        HIPCHK(hipMemcpyAsync(&differ, flag, sizeof(int), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        dfree(flag);
+        same = differ == 0;
+    } else if (same && !alias) {
         // bitwise equality, in parallel blocks
         const long long n = M_;
         const long long blk = 1 << 18, nb = (n + blk - 1) / blk;
@@ -828,6 +853,7 @@ std::string Plan::device_trees(const PlanInput &in)
     shared_ = same;
     const double t1 = now();
     tm_.host_copy = t1 - t0;
+    const hipMemcpyKind kind = in.device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
 
     // one particle set: coordinates uploaded, caller indices 0..n-1
     auto upload_set = [&](DevParticles &p, int n, const double *x, const double *y, const double *z) -> std::string {
@@ -836,9 +862,9 @@ std::string Plan::device_trees(const PlanInput &in)
         p.x2 = dalloc<double>(n); p.y2 = dalloc<double>(n); p.z2 = dalloc<double>(n); p.idx2 = dalloc<int>(n);
         if (!p.x || !p.y || !p.z || !p.idx || !p.x2 || !p.y2 || !p.z2 || !p.idx2) return "hipMalloc failed (particles)";
         const size_t b = (size_t)n * sizeof(double);
-        HIPCHK(hipMemcpyAsync(p.x, x, b, hipMemcpyHostToDevice, st_));
-        HIPCHK(hipMemcpyAsync(p.y, y, b, hipMemcpyHostToDevice, st_));
-        HIPCHK(hipMemcpyAsync(p.z, z, b, hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(p.x, x, b, kind, st_));
+        HIPCHK(hipMemcpyAsync(p.y, y, b, kind, st_));
+        HIPCHK(hipMemcpyAsync(p.z, z, b, kind, st_));
         HIPCHK(btk::launch_iota(n, p.idx, st_));
         return "";
     };
@@ -877,8 +903,8 @@ std::string Plan::device_trees(const PlanInput &in)
     if (ss) d_sw_ = dalloc<double>(N_);
     if (!d_sq_ || (ss && !d_sw_)) return "hipMalloc failed (charges)";
     const size_t nb = (size_t)N_ * sizeof(double);
-    HIPCHK(hipMemcpyAsync(S.x2, in.sq, nb, hipMemcpyHostToDevice, st_));
-    if (ss) HIPCHK(hipMemcpyAsync(S.y2, in.sw, nb, hipMemcpyHostToDevice, st_));
+    HIPCHK(hipMemcpyAsync(S.x2, in.sq, nb, kind, st_));
+    if (ss) HIPCHK(hipMemcpyAsync(S.y2, in.sw, nb, kind, st_));
     HIPCHK(btk::launch_tree_gather(N_, S.idx, S.x2, d_sq_, ss ? S.y2 : nullptr, 1.0, ss ? d_sw_ : nullptr, st_));
     if (shared_) {
         d_tx_ = d_sx_; d_ty_ = d_sy_; d_tz_ = d_sz_; d_tq_ = d_sq_;
@@ -898,7 +924,7 @@ std::string Plan::device_trees(const PlanInput &in)
         d_tq_ = dalloc<double>(M_);
         if (!d_tq_) return "hipMalloc failed (target values)";
         if (in.tq) {
-            HIPCHK(hipMemcpyAsync(T.x2, in.tq, (size_t)M_ * sizeof(double), hipMemcpyHostToDevice, st_));
+            HIPCHK(hipMemcpyAsync(T.x2, in.tq, (size_t)M_ * sizeof(double), kind, st_));
             HIPCHK(btk::launch_tree_gather(M_, T.idx, T.x2, d_tq_, nullptr, 0.0, nullptr, st_));
         } else {
             HIPCHK(hipMemsetAsync(d_tq_, 0, (size_t)M_ * sizeof(double), st_));
@@ -1103,6 +1129,19 @@ std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
         if (!(r = up(&d_up_nodes_, up_nodes_)).empty()) return r;
         if (!(r = up(&d_up_child_off_, up_child_off_)).empty()) return r;
         if (!(r = up(&d_up_child_id_, up_child_id_)).empty()) return r;
+        if (!up_child_id_.empty()) {
+            // per parent-child entry: its parent, and the identity offsets
+            std::vector<int> ent_parent(up_child_id_.size()), ent_off(up_child_id_.size() + 1);
+            for (size_t k = 0; k + 1 < up_child_off_.size(); k++)
+                for (int m = up_child_off_[k]; m < up_child_off_[k + 1]; m++) ent_parent[m] = up_nodes_[k];
+            for (size_t m = 0; m < ent_off.size(); m++) ent_off[m] = (int)m;
+            if (!(r = up(&d_up_entry_parent_, ent_parent)).empty()) return r;
+            if (!(r = up(&d_up_entry_off_, ent_off)).empty()) return r;
+            const size_t ne = up_child_id_.size() * Pz;
+            d_up_Qe_ = dalloc<double>(ne);
+            if (ss) d_up_We_ = dalloc<double>(ne);
+            if (!d_up_Qe_ || (ss && !d_up_We_)) return "hipMalloc failed (upward entries)";
+        }
     }
     if (need_tgt_clusters) {
         if (!(r = up(&d_ttab_, ttab)).empty()) return r;
@@ -1185,12 +1224,19 @@ std::string Plan::upward_fixed()
         HIPCHK(btk::launch_sum_parts(nsnodes_, p.P, d_up_part_off_, d_up_Qp_, d_up_Wp_, d_cq_,
                                      a.subtraction ? d_cw_ : nullptr, st_));
     }
+    // parents level by level: one block per (parent, child) entry into the
+    // entry's row, then each parent's rows in child order
     for (size_t lv = 0; lv + 1 < up_levels_off_.size(); lv++) {
         const int b = up_levels_off_[lv], e = up_levels_off_[lv + 1];
         if (e == b) continue;
-        a.mode = 1; a.nodes = d_up_nodes_ + b; a.nblocks = e - b;
-        a.child_off = d_up_child_off_ + b; a.child_id = d_up_child_id_;
-        HIPCHK(btk::launch_anterp(a, st_));
+        const int e0 = up_child_off_[b], e1 = up_child_off_[e];
+        btk::AnterpArgs ca = a;
+        ca.mode = 3; ca.nodes = d_up_entry_parent_ + e0; ca.nblocks = e1 - e0;
+        ca.child_off = d_up_entry_off_ + e0; ca.child_id = d_up_child_id_;
+        ca.Qo = d_up_Qe_; ca.Wo = d_up_We_;
+        HIPCHK(btk::launch_anterp(ca, st_));
+        HIPCHK(btk::launch_sum_parts(e - b, p.P, d_up_child_off_ + b, d_up_Qe_, d_up_We_, d_cq_,
+                                     a.subtraction ? d_cw_ : nullptr, st_, d_up_nodes_ + b));
     }
     return "";
 }
@@ -1369,9 +1415,10 @@ std::string Plan::execute(bool timed)
     return "";
 }
 
-std::string Plan::download(double *out)
+std::string Plan::download(double *out, bool device)
 {
-    HIPCHK(hipMemcpyAsync(out, d_out_, (size_t)M_ * sizeof(double), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipMemcpyAsync(out, d_out_, (size_t)M_ * sizeof(double),
+                          device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
     HIPCHK(hipStreamSynchronize(st_));
     return "";
 }

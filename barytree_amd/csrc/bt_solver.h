@@ -102,6 +102,7 @@ struct PlanInput {
     int persistent = 0;                            // persistent interaction launches
     const double *tx = nullptr, *ty = nullptr, *tz = nullptr, *tq = nullptr;
     const double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sq = nullptr, *sw = nullptr;
+    bool device = false;  // the particle arrays are device pointers (read on the device only)
 };
 
 class Plan {
@@ -113,11 +114,11 @@ public:
     std::string create(const Params &prm, int M, int N, const double *tx, const double *ty,
                        const double *tz, const double *tq, const double *sx, const double *sy,
                        const double *sz, const double *sq, const double *sw,
-                       bool host_only = false);
+                       bool host_only = false, bool device_inputs = false);
     // Runs the GPU path; result in caller order, on the device (out_dev()).
     std::string execute(bool timed);
-    // Copies the device result to host memory.
-    std::string download(double *out);
+    // Copies the result to `out` (host memory, or device memory if `device`).
+    std::string download(double *out, bool device = false);
     // FP64 direct sum on the GPU for caller-order target indices `idx`
     // (each target against the sources of its own system).
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
@@ -156,6 +157,7 @@ public:
     long long near_pairs() const { return near_.pairs; }
     // pairs (ordered) the symmetric near field covers; 0 without it
     long long sym_pairs() const { return sym_ && sym_enabled_ ? sym_pairs_ : 0; }
+    long long sym_evals() const { return sym_ && sym_enabled_ ? sym_evals_ : 0; }
     long long sym_slots() const { return sym_ && sym_enabled_ ? sym_slots_len_ : 0; }
     long long far_pairs() const { return grid_.pairs; }
     int source_nodes() const { return nsnodes_; }
@@ -241,7 +243,7 @@ private:
     std::vector<btk::Tile> sym_red_tiles_;
     std::vector<int> sym_red_off_;
     std::vector<long long> sym_red_base_;
-    long long sym_slots_len_ = 0, sym_pairs_ = 0;
+    long long sym_slots_len_ = 0, sym_pairs_ = 0, sym_evals_ = 0;
     std::vector<long long> grid_row_;
     std::vector<int> node_rows_off_;
     long long grid_rows_len_ = 0;
@@ -306,6 +308,9 @@ private:
     int *d_sleaves_ = nullptr, *d_up_nodes_ = nullptr, *d_up_child_off_ = nullptr, *d_up_child_id_ = nullptr;
     int *d_up_part_node_ = nullptr, *d_up_part_beg_ = nullptr, *d_up_part_end_ = nullptr, *d_up_part_off_ = nullptr;
     double *d_up_Qp_ = nullptr, *d_up_Wp_ = nullptr;
+    // child-to-parent entries: parent of each, identity offsets, rows
+    int *d_up_entry_parent_ = nullptr, *d_up_entry_off_ = nullptr;
+    double *d_up_Qe_ = nullptr, *d_up_We_ = nullptr;
     int *d_down_nodes_ = nullptr;
     btk::Tile *d_near_tiles_ = nullptr, *d_grid_tiles_ = nullptr, *d_leaf_tiles_ = nullptr;
     int *d_nseg_off_ = nullptr, *d_nclu_off_ = nullptr, *d_nclu_id_ = nullptr;

@@ -43,6 +43,8 @@ hipError_t launch_tree_scatter(int nchunks, const TreeChunk *chunks, const TreeS
 // out_a[i] = a[idx[i]]; out_b[i] = b ? b[idx[i]] : fill_b (out_b may be null).
 hipError_t launch_tree_gather(int n, const int *idx, const double *a, double *out_a, const double *b,
                               double fill_b, double *out_b, hipStream_t st);
+// *flag = 1 if a[i] and b[i] differ in any bit for some i < n (else untouched).
+hipError_t launch_differ(int n, const double *a, const double *b, int *flag, hipStream_t st);
 // idx[i] = i (caller order before the build).
 hipError_t launch_iota(int n, int *idx, hipStream_t st);
 // inv[order[i]] = i.

@@ -159,6 +159,14 @@ __global__ void k_tree_gather(int n, const int *__restrict__ idx, const double *
     }
 }
 
+__global__ void k_differ(int n, const double *__restrict__ a, const double *__restrict__ b, int *flag)
+{
+    bool d = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        d |= __double_as_longlong(a[i]) != __double_as_longlong(b[i]);
+    if (__any(d) && (threadIdx.x & 63) == 0) *flag = 1;  // every writer stores the same value
+}
+
 __global__ void k_iota(int n, int *__restrict__ idx)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) idx[i] = i;
@@ -216,6 +224,13 @@ hipError_t launch_tree_gather(int n, const int *idx, const double *a, double *ou
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tree_gather, dim3(grid_for(n)), dim3(TB), 0, st, n, idx, a, out_a, b, fill_b, out_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_differ(int n, const double *a, const double *b, int *flag, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_differ, dim3(grid_for(n)), dim3(TB), 0, st, n, a, b, flag);
     return hipGetLastError();
 }
 

@@ -78,7 +78,8 @@ class Stats(ctypes.Structure):
         ("num_far_pairs", ctypes.c_longlong), ("variable_degree", ctypes.c_int),
         ("mean_pair_degree", ctypes.c_double), ("far_ptwise_variable", ctypes.c_longlong),
         ("sym_ptwise", ctypes.c_longlong), ("gpu_near_sym_s", ctypes.c_double),
-        ("sym_slots", ctypes.c_longlong), ("model_fallback_pairs", ctypes.c_longlong)]
+        ("sym_slots", ctypes.c_longlong), ("model_fallback_pairs", ctypes.c_longlong),
+        ("sym_evals", ctypes.c_longlong)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -135,7 +136,8 @@ def lib():
                   "BaryTreeModel_Save", "BaryTreeModel_GetParams", "BaryTreeModel_SetParams",
                   "BaryTreeModel_Gradient", "BaryTreeModel_Train", "BaryTreeModel_Predict",
                   "BaryTreeModel_GetFeatureRange", "BaryTreeModel_SetFeatureRange",
-                  "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees"):
+                  "BaryTree_SolveLearned", "BaryTree_AnalyzeDegrees", "BaryTree_SolveDevice",
+                  "BaryTree_SolveLearnedDevice"):
             getattr(l, f).restype = ctypes.c_int
         l.BaryTreeInterface.restype = None
         l.BaryTreePlan_Destroy.restype = None
@@ -277,6 +279,25 @@ def _batch(systems):
     return (np.asarray(toffs, dtype=np.int32), np.asarray(soffs, dtype=np.int32), arrays)
 
 
+class Batch:
+    """Systems [(targets, sources), ...] concatenated once into the arrays
+    BaryTree_SolveBatched takes (offsets, then targets x, y, z, q and sources
+    x, y, z, q, w): a caller that solves the same batch layout again passes
+    a Batch wherever a list of systems is accepted and skips the copies."""
+
+    def __init__(self, systems):
+        self.toffs, self.soffs, self.arrays = _batch(systems)
+
+    def __len__(self):
+        return len(self.toffs) - 1
+
+
+def _batched(systems):
+    if isinstance(systems, Batch):
+        return systems.toffs, systems.soffs, systems.arrays
+    return _batch(systems)
+
+
 def _ints(a, n):
     if a is None:
         return None
@@ -291,11 +312,12 @@ def _iptr(a):
 def solve_batched(systems, kernel, kernel_params, singularity, approximation, compute_type,
                   theta, degree, max_source_leaf=None, max_target_leaf=None, schedule=None,
                   size_check=1.0, beta=-1.0, precision=PRECISION_FP64, verbosity=0):
-    """BaryTree_SolveBatched over systems = [(targets, sources), ...].  Leaf
+    """BaryTree_SolveBatched over systems = [(targets, sources), ...] (or a
+    Batch).  Leaf
     sizes: per-system arrays (or one int for all), else the schedule, else the
     autotuner.  Returns one potential array per system."""
-    toffs, soffs, arr = _batch(systems)
-    ns = len(systems)
+    toffs, soffs, arr = _batched(systems)
+    ns = len(toffs) - 1
     kp = _kp(kernel_params)
     msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
     out = np.zeros(int(toffs[-1]))
@@ -317,8 +339,8 @@ def analyze_batched(systems, kernel, kernel_params, singularity, approximation, 
                     size_check=1.0, beta=-1.0):
     """BaryTree_AnalyzeBatched: host-only trees and lists, counts summed over
     the systems (no GPU work)."""
-    toffs, soffs, arr = _batch(systems)
-    ns = len(systems)
+    toffs, soffs, arr = _batched(systems)
+    ns = len(toffs) - 1
     kp = _kp(kernel_params)
     msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
     s = Stats()
@@ -338,7 +360,7 @@ def analyze_batched(systems, kernel, kernel_params, singularity, approximation, 
 def autotune(systems, kernel, kernel_params, singularity, approximation, compute_type, theta,
              degree, size_check=1.0, beta=-1.0, precision=PRECISION_FP64):
     """BaryTree_Autotune: returns the Schedule picked for this batch."""
-    toffs, soffs, arr = _batch(systems)
+    toffs, soffs, arr = _batched(systems)
     kp = _kp(kernel_params)
     sc = Schedule()
     rc = lib().BaryTree_Autotune(
@@ -362,8 +384,8 @@ class Plan:
                 size_check=1.0, beta=-1.0, precision=PRECISION_FP64):
         """BaryTreePlan_CreateBatched; execute() returns the concatenated
         potential (split with .offsets)."""
-        toffs, soffs, arr = _batch(systems)
-        ns = len(systems)
+        toffs, soffs, arr = _batched(systems)
+        ns = len(toffs) - 1
         kp = _kp(kernel_params)
         msl, mtl = _ints(max_source_leaf, ns), _ints(max_target_leaf, ns)
         self = cls.__new__(cls)
@@ -657,6 +679,88 @@ def solve_learned(targets, sources, kernel, kernel_params, singularity, approxim
         None if model is None else model._h, ctypes.c_float(delta))
     if rc != 0:
         raise RuntimeError(f"BaryTree_SolveLearned failed: {last_error()}")
+    return out
+
+
+class DeviceBuffer:
+    """float64 device memory allocated with the HIP runtime the library
+    itself runs on (for callers without torch): from_numpy, to_numpy, free.
+    Any object with data_ptr() and numel() (a torch CUDA tensor) serves the
+    *_device calls as well."""
+
+    def __init__(self, n):
+        self._hip = ctypes.CDLL("libamdhip64.so.7")
+        self.n = int(n)
+        self.ptr = ctypes.c_void_p()
+        if self._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.n, 1) * 8)) != 0:
+            raise RuntimeError("hipMalloc failed")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = cls(len(a))
+        if b._hip.hipMemcpy(b.ptr, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes), 1) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return b
+
+    def to_numpy(self):
+        a = np.empty(self.n)
+        if self._hip.hipMemcpy(a.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(a.nbytes), 2) != 0:
+            raise RuntimeError("hipMemcpy failed")
+        return a
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def numel(self):
+        return self.n
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dev_ptr(a):
+    """The device address of a device array (None passes through)."""
+    return None if a is None else ctypes.c_void_p(a.data_ptr())
+
+
+def solve_device(targets, sources, out, kernel, kernel_params, singularity, approximation, compute_type,
+                 theta, degree, max_source_leaf, max_target_leaf, size_check=1.0, beta=-1.0,
+                 precision=PRECISION_FP64, verbosity=0, model=None, delta=0.5):
+    """BaryTree_SolveDevice (model None) or BaryTree_SolveLearnedDevice:
+    targets = (x, y, z, q), sources = (x, y, z, q, w) and out are device
+    arrays -- DeviceBuffers, or contiguous float64 torch CUDA tensors when
+    torch's HIP runtime is the process's (torch initialised first) -- q, w
+    may be None; the potential is written to out."""
+    arrs = list(targets[:4]) + list(sources[:5])
+    for a in arrs + [out]:
+        if a is None or isinstance(a, DeviceBuffer):
+            continue
+        if (not getattr(a, "is_cuda", False) or not a.is_contiguous() or str(a.dtype) != "torch.float64"):
+            raise ValueError("device arrays must be DeviceBuffers or contiguous float64 CUDA tensors")
+    kp = _kp(kernel_params)
+    args = [ctypes.c_int(int(targets[0].numel())), ctypes.c_int(int(sources[0].numel()))]
+    args += [_dev_ptr(a) for a in arrs] + [_dev_ptr(out)]
+    args += [ctypes.c_int(int(kernel)), ctypes.c_int(len(kp)), _ptr(kp), ctypes.c_int(int(singularity)),
+             ctypes.c_int(int(approximation)), ctypes.c_int(int(compute_type)), ctypes.c_double(theta),
+             ctypes.c_int(degree), ctypes.c_int(max_source_leaf), ctypes.c_int(max_target_leaf),
+             ctypes.c_double(size_check), ctypes.c_double(beta), ctypes.c_int(verbosity), ctypes.c_int(precision)]
+    if model is None:
+        rc = lib().BaryTree_SolveDevice(*args)
+        name = "BaryTree_SolveDevice"
+    else:
+        rc = lib().BaryTree_SolveLearnedDevice(*args, model._h, ctypes.c_float(delta))
+        name = "BaryTree_SolveLearnedDevice"
+    if rc != 0:
+        raise RuntimeError(f"{name} failed: {last_error()}")
     return out
 
 

@@ -5,13 +5,16 @@ subtraction (kappa = 0.5), N = 16M particles uniform in [-1,1]^3 (sources =
 targets, separate arrays), charges uniform in [-1,1], weights 1,
 cluster-cluster treecode, theta = 0.8, degree 10, max leaf 8000.
 
-A step is one BaryTree_SolveLearned call: host arrays in, potential (caller
-order) out -- trees built on the GPU, interaction lists on the host, the
-per-pair degrees the shipped degree model picks, the whole GPU path in
-mixed precision, the copy back.  `value` is seconds per call (the model is
-loaded once, outside the timed region).  Beside it:
-  * the plain FP64 time-to-solution (one BaryTree_Solve call, the drop-in
-    path) and the speedup over it;
+A step is one BaryTree_SolveLearnedDevice call with the particles resident
+in HBM: device arrays in, potential (caller order) out on the device --
+trees built on the GPU, interaction lists on the host, the per-pair degrees
+the shipped degree model picks, the whole GPU path in mixed precision.
+`value` is seconds per call (the model is loaded once, outside the timed
+region).  Beside it:
+  * the same call on host arrays (PCIe in and out, BaryTree_SolveLearned);
+  * the plain FP64 time-to-solution (one call on HBM-resident particles,
+    and BaryTree_Solve on host arrays, the drop-in path) and the speedups
+    over them;
   * the GPU path alone with its inputs resident in HBM (one plan): mixed +
     learned, mixed at the fixed degree, plain FP64 with the symmetric near
     field, plain FP64 with one evaluation per ordered near-field pair (the
@@ -192,10 +195,13 @@ def bench_batched(args):
     log(f"[bench] autotune {tune_s:.2f}s: {sc.as_dict()}")
     leaf_of = [sc.leaf[int(np.floor(np.log2(n)))] for n in sizes]
 
-    # one-shot batched solve, end to end (host trees and lists, upload, GPU, download)
+    # one-shot batched solve, end to end (host trees and lists, upload, GPU,
+    # download) on the concatenated arrays BaryTree_SolveBatched takes
+    batch = bt.Batch(systems)
+
     def batched_once(precision):
         t0 = time.perf_counter()
-        out = bt.solve_batched(systems, *kargs, schedule=sc, precision=precision)
+        out = bt.solve_batched(batch, *kargs, schedule=sc, precision=precision)
         return time.perf_counter() - t0, out, bt.last_stats()
     batched_once(bt.PRECISION_FP64)  # warm-up
     e2e = [batched_once(bt.PRECISION_FP64) for _ in range(args.steps)]
@@ -204,9 +210,9 @@ def bench_batched(args):
     e2e_mixed_runs = [batched_once(bt.PRECISION_MIXED) for _ in range(args.steps)]
     e2e_mixed = min(r[0] for r in e2e_mixed_runs)
     out_mixed = e2e_mixed_runs[-1][1]
-    log(f"[bench] batched solve fp64 {e2e_fp64:.3f}s mixed {e2e_mixed:.3f}s "
-        f"(host tree {st_b['host_tree_s']:.3f} lists {st_b['host_lists_s']:.3f} "
-        f"gpu {st_b['gpu_total_s']:.4f})")
+    phases = {k: round(st_b[k], 4) for k in ("host_copy_s", "host_tree_s", "host_lists_s", "host_work_s",
+                                             "h2d_s", "gpu_total_s")}
+    log(f"[bench] batched solve fp64 {e2e_fp64:.3f}s mixed {e2e_mixed:.3f}s (phases {phases})")
 
     # 256 sequential plain solves, same leaf sizes
     def sequential_once():
@@ -263,6 +269,7 @@ def bench_batched(args):
         "sequential_s": round(seq, 4),
         "speedup_vs_sequential": round(seq / e2e_fp64, 3),
         "batched_mixed_s": round(e2e_mixed, 4),
+        "time_to_solution_phases_s": phases,
         "rel_l2_batched_vs_sequential": diff_seq,
         "gpu_batched_fp64_s": round(gpu_b, 5),
         "gpu_batched_mixed_s": round(gpu_bm, 5),
@@ -310,8 +317,10 @@ def bench_learned(args):
                      "far_evals": far, "degree_setup_s": round(setup, 4),
                      "mean_pair_degree": st["mean_pair_degree"] if use else 10.0,
                      "phases_s": {k: round(st[k], 5) for k in ("gpu_upward_s", "gpu_near_s",
-                                                               "gpu_far_s", "gpu_down_s")}}
-        log(f"[bench] {form}: {wall:.4f} s/step, relL2 {err:.2e}, far evals {far:.3e}")
+                                                               "gpu_far_s", "gpu_down_s")},
+                     "near_symmetric_launch_s": round(st["gpu_near_sym_mean_s"], 5)}
+        log(f"[bench] {form}: {wall:.4f} s/step, relL2 {err:.2e}, far evals {far:.3e}, "
+            f"phases {res[form]['phases_s']}, symmetric near launch {st['gpu_near_sym_mean_s']:.4f}")
     plan.close()
     line = {
         "metric": "time-to-solution, learned per-pair degree vs fixed degree 10 (C4)",
@@ -398,8 +407,10 @@ def bench_yukawa(args):
         "interactions": evals,
         "roofline": {"bound": "valu", "kernel": "k_near_sym<Yukawa, skipping> (symmetric near field, mixed)",
                      "achieved": round(achieved, 2), "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["fp32"], 4), "traffic": None,
-                     "kernel_s": round(sym_s, 5), "ordered_pairs": st["sym_ptwise"]},
+                     "frac": round(achieved / PEAK["fp32"], 4),
+                     "frac_executed": round(achieved * st["sym_evals"] / max(st["sym_ptwise"], 1) / PEAK["fp32"], 4),
+                     "traffic": None, "kernel_s": round(sym_s, 5), "ordered_pairs": st["sym_ptwise"],
+                     "executed_pair_evaluations": st["sym_evals"]},
         "host_setup_s": round(host_setup, 3),
     }
     return line
@@ -817,39 +828,55 @@ def bench_large(args):
              bt.Approximation.LAGRANGE, bt.ComputeType.CLUSTER_CLUSTER, 0.8, 10, 8000, 8000)
     model = learned.load_model("coulomb_cc")  # loaded once, outside the timed region
 
-    # 1. time-to-solution: one BaryTree_SolveLearned call per step (host
-    # arrays in, potential out: trees on the GPU, lists, degrees from the
-    # model, the GPU path, the copy back)
-    def tts_learned():
-        t0 = time.perf_counter()
-        out = bt.solve_learned(targets, sources, *kargs, model=model, delta=args.delta,
-                               precision=bt.PRECISION_MIXED)
-        return time.perf_counter() - t0, out, bt.last_stats()
-    for _ in range(args.warmup):
-        tts_learned()
+    # 1. time-to-solution: one BaryTree_SolveLearnedDevice call per step with
+    # the particles resident in HBM (device arrays in, potential out on the
+    # device: trees on the GPU, lists on the host, degrees from the model,
+    # the GPU path); the same call on host arrays (PCIe-inclusive) beside it
+    dt = [torch.from_numpy(a).cuda() for a in targets]
+    ds = [torch.from_numpy(a).cuda() for a in sources]
+    dout = torch.zeros(n, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
-    runs = []
+
+    def tts_device(model_, precision):
+        bt.solve_device(dt, ds, dout, *kargs, model=model_, delta=args.delta, precision=precision)
+        return bt.last_stats()
+    for _ in range(args.warmup):
+        tts_device(model, bt.PRECISION_MIXED)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        runs.append(tts_learned())
+        st_tts = tts_device(model, bt.PRECISION_MIXED)
     torch.cuda.synchronize()
     tts = (time.perf_counter() - t0) / args.steps
-    out_tts, st_tts = runs[-1][1], runs[-1][2]
+    out_tts = dout.cpu().numpy()
     host_s = sum(st_tts[k] for k in ("host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s"))
-    log(f"[bench] time-to-solution (mixed + learned, one call): {tts:.3f} s "
+    log(f"[bench] time-to-solution (mixed + learned, one call, inputs in HBM): {tts:.3f} s "
         f"(trees {st_tts['host_tree_s']:.3f} lists {st_tts['host_lists_s']:.3f} work {st_tts['host_work_s']:.3f} "
-        f"h2d {st_tts['h2d_s']:.3f} gpu {st_tts['gpu_total_s']:.3f})")
+        f"copies {st_tts['h2d_s']:.3f} gpu {st_tts['gpu_total_s']:.3f})")
+    t0 = time.perf_counter()
+    out_host = bt.solve_learned(targets, sources, *kargs, model=model, delta=args.delta,
+                                precision=bt.PRECISION_MIXED)
+    tts_host = time.perf_counter() - t0
+    host_equal = bool(np.array_equal(out_host, out_tts))
+    log(f"[bench] the same call on host arrays (PCIe in and out): {tts_host:.3f} s (results equal: {host_equal})")
 
-    # plain FP64 time-to-solution (BaryTree_Solve, the drop-in path)
-    plain_tts, out_plain_tts = None, None
+    # plain FP64 time-to-solution (the drop-in path's arithmetic)
+    plain_tts = plain_tts_host = err_plain_tts = None
     if args.plain_steps > 0:
         ts = []
         for _ in range(args.plain_steps):
             t0 = time.perf_counter()
-            out_plain_tts = bt.solve(targets, sources, *kargs)
+            tts_device(None, bt.PRECISION_FP64)
+            torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
         plain_tts = float(np.mean(ts))
-        log(f"[bench] time-to-solution plain FP64 (one BaryTree_Solve call): {plain_tts:.3f} s")
+        out_plain_tts = dout.cpu().numpy()
+        t0 = time.perf_counter()
+        bt.solve(targets, sources, *kargs)
+        plain_tts_host = time.perf_counter() - t0
+        log(f"[bench] time-to-solution plain FP64 (one call, inputs in HBM): {plain_tts:.3f} s; "
+            f"BaryTree_Solve on host arrays {plain_tts_host:.3f} s")
+    del dt, ds, dout
 
     # 2. the GPU path alone, inputs resident in HBM (one plan, several forms)
     t0 = time.perf_counter()
@@ -885,7 +912,8 @@ def bench_large(args):
         log(f"[bench] fp64 one-directional: {wall_plain1:.3f} s/step")
     plan.close()
     err_tts = rel_l2(direct, out_tts[idx])
-    err_plain_tts = rel_l2(direct, out_plain_tts[idx]) if out_plain_tts is not None else None
+    if plain_tts:
+        err_plain_tts = rel_l2(direct, out_plain_tts[idx])
 
     evals = st["pp_ptwise"] + st["pc_ptwise"] + st["cp_ptwise"] + st["cc_ptwise"]
     evals_learned = st["pp_ptwise"] + st_ml["far_ptwise_variable"]
@@ -902,10 +930,11 @@ def bench_large(args):
         "peak": PEAK["fp32"],
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK["fp32"], 4),
-        "frac_executed": round(achieved / 2 / PEAK["fp32"], 4),
+        "frac_executed": round(achieved * st["sym_evals"] / max(sym_evals, 1) / PEAK["fp32"], 4),
         "traffic": near_traffic() if n == 16_000_000 else None,
         "kernel_s": round(sym_s, 4),
         "ordered_pairs": sym_evals,
+        "executed_pair_evaluations": st["sym_evals"],
         "flops_per_ordered_pair": FLOPS_PER_EVAL,
     }
     mix = near_valu_mix() if n == 16_000_000 else None
@@ -941,15 +970,20 @@ def bench_large(args):
                          "cluster-cluster theta=0.8 degree 10 max leaf 8000"),
             "n": n, "compute_type": "CC", "theta": 0.8, "degree": 10, "max_leaf": 8000,
         },
-        "form": ("one BaryTree_SolveLearned call per step: host arrays in, potential out; mixed precision "
-                 "+ learned per-pair degree (shipped coulomb_cc model, loaded once)"),
+        "form": ("one BaryTree_SolveLearnedDevice call per step, particles resident in HBM, potential "
+                 "written to HBM: trees, lists, per-pair degrees (shipped coulomb_cc model, loaded once) and "
+                 "the GPU path in mixed precision"),
         "rel_l2_err": err_tts,
         "time_to_solution_phases_s": {k: round(st_tts[k], 4) for k in (
             "host_copy_s", "host_tree_s", "host_lists_s", "host_work_s", "h2d_s", "gpu_total_s")},
         "host_setup_s": round(host_s, 4),
+        "time_to_solution_host_arrays_s": round(tts_host, 4),
+        "host_arrays_result_equal": host_equal,
         "plain_fp64_time_to_solution_s": round(plain_tts, 4) if plain_tts else None,
+        "plain_fp64_time_to_solution_host_arrays_s": round(plain_tts_host, 4) if plain_tts_host else None,
         "plain_fp64_time_to_solution_rel_l2_err": err_plain_tts,
         "speedup_vs_plain": round(plain_tts / tts, 3) if plain_tts else None,
+        "speedup_vs_plain_host_arrays": round(plain_tts_host / tts_host, 3) if plain_tts_host else None,
         "gpu_step_s": {
             "mixed_learned": round(wall_ml, 4),
             "mixed_fixed_degree": round(wall_fast, 4),

@@ -114,14 +114,15 @@ typedef struct BaryTreeStats {
     int variable_degree;     /* 1: far-field pairs carry their own degree */
     double mean_pair_degree; /* mean degree over far-field pairs (variable degree) */
     long long far_ptwise_variable; /* point-wise evaluations of the far field (variable degree) */
-    long long sym_ptwise;    /* near-field pairs (both directions counted) that
-                                executions evaluate once for both particles (the
-                                symmetric near field, either precision): a
-                                single system whose targets are its sources in
+    long long sym_ptwise;    /* near-field particle pairs (ordered) that the
+                                symmetric near field delivers (either precision):
+                                a single system whose targets are its sources in
                                 the same tree order (Lagrange; Coulomb, Yukawa,
-                                or regularized Coulomb with skipping); 0
-                                otherwise or when switched off
-                                (BaryTreePlan_SetSymmetric) */
+                                or regularized Coulomb with skipping); pairs of
+                                two leaves in each other's near list are
+                                evaluated once for both particles, the rest once
+                                for their target; 0 otherwise or when switched
+                                off (BaryTreePlan_SetSymmetric) */
     double gpu_near_sym_s;   /* the symmetric near-field launch alone (within gpu_near_s) */
     long long sym_slots;     /* FP64 partial sums the symmetric launch writes (8 bytes
                                 of device memory each, allocated at the first
@@ -129,6 +130,9 @@ typedef struct BaryTreeStats {
     long long model_fallback_pairs; /* far-field pairs a degree model left at the
                                 fixed degree: their features lie outside the
                                 model's training range (variable degree only) */
+    long long sym_evals;     /* pair evaluations the symmetric near field
+                                executes for its sym_ptwise pairs (between half
+                                of sym_ptwise and sym_ptwise) */
 } BaryTreeStats;
 
 /* For callers that want an error code instead of a printed message:
@@ -143,6 +147,19 @@ int BaryTree_Solve(int numTargets, int numSources,
                    int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
                    int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
                    int precision);
+
+/* BaryTree_Solve for callers whose particles already live in GPU memory:
+ * every array argument (targets, sources, outputArray) is a device pointer
+ * on the current GPU; nothing crosses PCIe but the trees' small node
+ * records.  Results equal BaryTree_Solve's bit for bit. */
+int BaryTree_SolveDevice(int numTargets, int numSources, const double *targetX, const double *targetY,
+                         const double *targetZ, const double *targetValue, const double *sourceX,
+                         const double *sourceY, const double *sourceZ, const double *sourceValue,
+                         const double *sourceWeight, double *outputArray, int kernel,
+                         int numKernelParams, const double *kernelParams, int singularity,
+                         int approximation, int compute_type, double theta, int interpOrder,
+                         int maxPerSourceLeaf, int maxPerTargetLeaf, double sizeCheck, double beta,
+                         int verbosity, int precision);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *BaryTree_GetLastError(void);
@@ -411,6 +428,18 @@ int BaryTree_SolveLearned(int numTargets, int numSources, const double *targetX,
                           int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
                           int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
                           int precision, const BaryTreeDegreeModel *model, float delta);
+
+/* BaryTree_SolveLearned with device pointers for every particle array and
+ * outputArray (as BaryTree_SolveDevice); kernelParams stays a host array. */
+int BaryTree_SolveLearnedDevice(int numTargets, int numSources, const double *targetX,
+                                const double *targetY, const double *targetZ, const double *targetValue,
+                                const double *sourceX, const double *sourceY, const double *sourceZ,
+                                const double *sourceValue, const double *sourceWeight,
+                                double *outputArray, int kernel, int numKernelParams,
+                                const double *kernelParams, int singularity, int approximation,
+                                int compute_type, double theta, int interpOrder, int maxPerSourceLeaf,
+                                int maxPerTargetLeaf, double sizeCheck, double beta, int verbosity,
+                                int precision, const BaryTreeDegreeModel *model, float delta);
 
 /* Autotuner: picks the leaf size of every size class present in the batch
  * (candidates P/2, P, 2P, 4P, 8P with P = (interpOrder+1)^3), then the tile

@@ -190,3 +190,29 @@ def test_mixed_near_field_close_and_coincident_pairs(bt, oracle_mod, ct, k, s):
         worst = max(worst, abs(ref[j] - got[j]) / (mag / r))
     print(f"relL2 {err:.2e}, worst close-pair error / term magnitude {worst:.2e}")
     assert err <= 1e-6 and worst <= 2e-6
+
+
+def test_verbosity_diagnostics(bt, capfd):
+    """verbosity > 2 prints the reference's diagnostics (treedriver.c:53-86,
+    989-1136): start and end lines, sizes, total time, the clusterwise and
+    pointwise interaction tables with the counts of the stats, and the
+    point-wise interactions each kind accounts for; verbosity 0 prints
+    nothing."""
+    import re
+    t, s = uniform_cube(20000, 20000, seed=17)
+    args = (1, KP[1][2], 2, 1, 3, 0.8, 5, 500, 500)
+    bt.solve(t, s, *args, verbosity=3)
+    out = capfd.readouterr().out
+    st = bt.last_stats()
+    assert "BaryTree has started" in out and "BaryTree has finished" in out
+    assert "20000 target and 20000 source particles" in out and "Total BaryTree time" in out
+    cw = out[out.index("Clusterwise interaction information"):out.index("Pointwise interaction information")]
+    rows = {m.group(1): int(m.group(2)) for m in re.finditer(r"\| (PP|PC|CP|CC) +\| +(\d+)", cw)}
+    assert rows["PP"] == st["num_pp"] and rows["CC"] == st["num_cc"]
+    pt = re.search(r"Pointwise interaction information.*?\| PP +\| +(\d+)", out, re.S)
+    assert int(pt.group(1)) == st["pp_ptwise"]
+    acc = re.search(r"accounted for.*?\| Total \| +(\d+)", out, re.S)
+    assert int(acc.group(1)) == st["pp_ptwise"] + st["pc_replaced"] + st["cp_replaced"] + st["cc_replaced"]
+    assert "GPU: near field" in out
+    bt.solve(t, s, *args, verbosity=0)
+    assert capfd.readouterr().out == ""

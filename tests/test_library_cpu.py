@@ -280,9 +280,11 @@ def test_analyze_degrees_checks_the_pair_count_of_every_buffer(bt):
 
 
 def test_symmetric_near_field_eligibility(bt):
-    """The symmetric near field covers nearly every near pair of a system
+    """The symmetric near field covers every near particle pair of a system
     (Coulomb, Yukawa, or regularized Coulomb with skipping) whose targets are
-    its sources in the same tree order, and nothing otherwise."""
+    its sources in the same tree order, and nothing otherwise: pairs of
+    leaves in each other's near list once for both particles, the rest
+    (each leaf with itself among them) once for the target."""
     rng = np.random.default_rng(8)
     n = 60000
     x, y, z, q = (rng.uniform(-1, 1, n) for _ in range(4))
@@ -291,13 +293,16 @@ def test_symmetric_near_field_eligibility(bt):
     src = (x, y, z, q, w)
     for k, s in ((1, 2), (2, 2), (1, 1), (2, 1), (3, 1)):
         a = bt.analyze(same, src, k, [0.5], s, 1, 3, 0.8, 6, 800, 800)
-        assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], a
-        assert a["sym_ptwise"] % 2 == 0
-        # each item holds one slot per particle of both ranges: pairs = sum |A||B|
-        assert 0 < a["sym_slots"] and a["sym_slots"] ** 2 >= 2 * a["sym_ptwise"], a
+        assert a["sym_ptwise"] == a["pp_ptwise"], a
+        # most pairs run once for both particles; the one-sided ones
+        # (ordered pairs evaluated once: 2 evals - ptwise) include every
+        # particle with itself, and every particle has a slot of its own leaf
+        assert a["pp_ptwise"] // 2 < a["sym_evals"] < 0.6 * a["pp_ptwise"], a
+        assert 2 * a["sym_evals"] - a["sym_ptwise"] >= n, a
+        assert n <= a["sym_slots"] < a["sym_evals"], a
     for ct in (1, 2):  # particle-cluster and cluster-particle with equal leaf sizes
         a = bt.analyze(same, src, 1, [0.5], 1, 1, ct, 0.8, 6, 800, 800)
-        assert 0.9 * a["pp_ptwise"] < a["sym_ptwise"] <= a["pp_ptwise"], (ct, a)
+        assert a["sym_ptwise"] == a["pp_ptwise"] and a["sym_evals"] < 0.6 * a["pp_ptwise"], (ct, a)
     herm = bt.analyze(same, src, 1, [0.5], 1, 2, 2, 0.8, 6, 800, 800)  # CP Hermite
     assert herm["sym_ptwise"] == 0 and herm["sym_slots"] == 0
     assert bt.analyze(same, src, 1, [0.5], 1, 1, 1, 0.8, 6, 800, 100)["sym_ptwise"] == 0  # PC, other order
