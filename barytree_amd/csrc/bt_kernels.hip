@@ -50,6 +50,38 @@ __device__ __forceinline__ double rsq64(double x)
 }
 
 // ---------------------------------------------------------------------
+// e^x in FP64 for x <= 0, with a table of 2^(j/64), j < 64 (EXP_TAB
+// doubles, filled by exp_table).  x is clamped at -746, where e^x is below
+// half the least subnormal.  x = (64 k + j) ln2/64 + f with t = 64 k + j
+// the integer nearest x 64/ln2 and |f| <= ln2/128 (ln2/64 split in a
+// 36-bit head, exact times |t| < 2^17, and a tail); e^f by its Taylor
+// polynomial to degree 5 (truncation below 3.5e-17 relative); the result
+// 2^k tab[j] e^f is within about 2 ulp of e^x.
+// ---------------------------------------------------------------------
+constexpr int EXP_TAB = 64;
+
+__device__ __forceinline__ void exp_table(double *tab, int tid)
+{
+    if (tid < EXP_TAB) tab[tid] = exp2((double)tid * (1.0 / EXP_TAB));
+}
+
+__device__ __forceinline__ double exp_nonpos(double x, const double *tab)
+{
+    constexpr double INV = 92.33248261689366;                // 64 / ln 2
+    constexpr double HI = 0x1.62e42fefa0000p-7, LO = 2.572804622327669e-14;  // ln2/64
+    x = fmax(x, -746.0);
+    const double t = __builtin_rint(x * INV);
+    const double f = fma(-t, LO, fma(-t, HI, x));
+    const int n = (int)t;
+    double p = fma(f, 1.0 / 120.0, 1.0 / 24.0);
+    p = fma(f, p, 1.0 / 6.0);
+    p = fma(f, p, 0.5);
+    p = fma(f, p, 1.0);
+    p = fma(f, p, 1.0);
+    return __builtin_amdgcn_ldexp(tab[n & (EXP_TAB - 1)] * p, n >> 6);
+}
+
+// ---------------------------------------------------------------------
 // The kernels beside the Coulomb family, FP64, one pair's term with the
 // charge a (b, tq: subtraction weight and target charge):
 //   regularized Yukawa SK: a e^{-kr}/sqrt(r^2+e^2)   SS: (a - tq b) e^{-kr}/sqrt(r^2+e^2)
@@ -897,12 +929,15 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
     __shared__ double4 Ls[NT];          // x, y, z and the source's a (q w or q)
     __shared__ double2 Lw[SS ? NT : 1]; // the source's w and its charge as a target
     __shared__ double Lc[NW][NT];       // per wave: the sources' sums
+    __shared__ double Ltab[EXP_TAB];     // 2^(j/64) for exp_nonpos
     const SymItem it = A.items[blockIdx.x];
     const bool check = A.ctr[blockIdx.x].w < 0.0;
     constexpr int NG = 64 / G;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, rg = lane % G, cg = lane / G;
     constexpr int TPASS = NW * G * TPT;
     const KParams kp = A.kp;
+    // read after the first chunk's barriers
+    if constexpr (SS || KER == KY) exp_table(Ltab, tid);
 
     // TWO: the sources' sums as well (block-uniform)
     auto body = [&](auto two_tag) {
@@ -964,13 +999,13 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
                                 acc[k] = fma(S.w, ir, acc[k]);
                                 if constexpr (TWO) c = fma(Ta[k], ir, c);
                             } else {
-                                const double e = exp(-r2 * kp.inv_a2);
+                                const double e = exp_nonpos(-r2 * kp.inv_a2, Ltab);
                                 acc[k] = fma(fma(Tnq[k] * sw, e, S.w), ir, acc[k]);
                                 if constexpr (TWO) c = fma(fma(-sq * Tw[k], e, Ta[k]), ir, c);
                             }
                         } else if constexpr (KER == KY) {
                             const double ir = rsq64(r2);
-                            double e = exp(-kp.p0 * (r2 * ir)) * ir;
+                            double e = exp_nonpos(-kp.p0 * (r2 * ir), Ltab) * ir;
                             if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
                             if constexpr (!SS) {
                                 acc[k] = fma(S.w, e, acc[k]);
