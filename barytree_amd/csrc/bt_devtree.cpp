@@ -246,12 +246,15 @@ std::string device_build_trees(DevParticles &p, const std::vector<DevRoot> &root
     // preorder records per root, ranges relative to the root
     nodes.assign(roots.size(), {});
     depth.assign(roots.size(), 0);
-    for (size_t r = 0; r < roots.size(); r++) {
+    // index of every node's record within its root's records
+    std::vector<int> rec_of(all.size(), -1);
+    const int nroots = (int)roots.size();
+#pragma omp parallel for schedule(dynamic) if (nroots > 1)
+    for (int r = 0; r < nroots; r++) {
         std::vector<NodeRec> &out = nodes[r];
         const int base = roots[r].b;
-        // iterative DFS: (BNode id, index of its record in out)
-        std::vector<int> stack = {(int)r};
-        std::vector<int> rec_of(all.size(), -1);
+        // iterative DFS over this root's nodes (ids in preorder)
+        std::vector<int> stack = {r}, order;
         while (!stack.empty()) {
             const int id = stack.back();
             stack.pop_back();
@@ -261,13 +264,13 @@ std::string device_build_trees(DevParticles &p, const std::vector<DevRoot> &root
             rec.b = n.b - base; rec.e = n.e - base; rec.nchild = 0;
             out.push_back(rec);
             rec_of[id] = (int)out.size() - 1;
+            order.push_back(id);
             for (size_t k = n.kids.size(); k-- > 0;) stack.push_back(n.kids[k]);
         }
         // child links: a node's children follow in preorder
-        for (size_t i = 0; i < all.size(); i++) {
-            if (rec_of[i] < 0) continue;
-            NodeRec &pr = out[rec_of[i]];
-            for (int c : all[i].kids) pr.child[pr.nchild++] = rec_of[c];
+        for (int id : order) {
+            NodeRec &pr = out[rec_of[id]];
+            for (int c : all[id].kids) pr.child[pr.nchild++] = rec_of[c];
         }
         if (leaves_only) out = leaf_records(out);
     }

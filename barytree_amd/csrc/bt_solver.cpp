@@ -356,10 +356,8 @@ void Plan::build_work()
     grid_ = WorkCSR();
     near_ctr_.clear();
     sym_ = false;
-    near_.seg_off.push_back(0); near_.clu_off.push_back(0);
-    grid_.seg_off.push_back(0); grid_.clu_off.push_back(0);
-
-    for (const System &sy : sys_) {
+    // the work of one system into (near keys, grid keys, near centres)
+    auto per_system = [&](const System &sy, WorkCSR &nw, WorkCSR &gw, std::vector<double> &nctr) {
         const Tree &st = sy.stree, &tt = sy.ttree;
         const Lists &L = sy.L;
         const int so = sy.soff, to = sy.toff, sn = sy.snode_off, tn = sy.tnode_off;
@@ -412,17 +410,17 @@ void Plan::build_work()
             const double thresh = std::max(2e-7 * R2, 1e-300);
             const double gclose = std::max(1e-150, std::sqrt(thresh));
             const long long nt = tcnt;
-            add_key(near_, tn + t, tbeg, tcnt);
-            for (int c : clus) add_clu(near_, sn + c, nt);
+            add_key(nw, tn + t, tbeg, tcnt);
+            for (int c : clus) add_clu(nw, sn + c, nt);
             for (size_t i = 0; i < segs.size(); i++) {
                 const int s = segs[i];
-                add_seg(near_, so + st.ibeg[s], st.numpar[s], nt,
+                add_seg(nw, so + st.ibeg[s], st.numpar[s], nt,
                         (within(t, s, 1e-150) ? SEG_COINCIDE : 0) | (within(t, s, gclose) ? SEG_CLOSE : 0) |
                             (sym && (*sym)[i] ? SEG_SYM : 0));
             }
-            close_key(near_);
-            near_ctr_.push_back(cx); near_ctr_.push_back(cy); near_ctr_.push_back(cz);
-            near_ctr_.push_back(clu_check ? -thresh : thresh);
+            close_key(nw);
+            nctr.push_back(cx); nctr.push_back(cy); nctr.push_back(cz);
+            nctr.push_back(clu_check ? -thresh : thresh);
         };
         if (p.compute_type == C_PC) {
             // keys: target batches; pc list -> clusters, pp list -> source leaves
@@ -449,9 +447,9 @@ void Plan::build_work()
                          sym_ok_ ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (cp_of[t].empty()) continue;
-                add_key(grid_, tn + t, 0, P);
-                for (int b : cp_of[t]) add_seg(grid_, so + st.ibeg[b], st.numpar[b], P);
-                close_key(grid_);
+                add_key(gw, tn + t, 0, P);
+                for (int b : cp_of[t]) add_seg(gw, so + st.ibeg[b], st.numpar[b], P);
+                close_key(gw);
             }
         } else {
             // CC: a target leaf gets the PP and source-approx (PC) lists of
@@ -470,11 +468,47 @@ void Plan::build_work()
                          sym_ok_ ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (L.approx[t].empty() && L.cc_cp[t].empty()) continue;
-                add_key(grid_, tn + t, 0, P);
-                for (int c : L.approx[t]) add_clu(grid_, sn + c, P);
-                for (int s : L.cc_cp[t]) add_seg(grid_, so + st.ibeg[s], st.numpar[s], P);
-                close_key(grid_);
+                add_key(gw, tn + t, 0, P);
+                for (int c : L.approx[t]) add_clu(gw, sn + c, P);
+                for (int s : L.cc_cp[t]) add_seg(gw, so + st.ibeg[s], st.numpar[s], P);
+                close_key(gw);
             }
+        }
+    };
+    auto csr_start = [](WorkCSR &w) { w.seg_off.assign(1, 0); w.clu_off.assign(1, 0); };
+    csr_start(near_);
+    csr_start(grid_);
+    const int ns = (int)sys_.size();
+    if (ns == 1) {
+        per_system(sys_[0], near_, grid_, near_ctr_);
+    } else {
+        // systems in parallel (the symmetric near field needs a single
+        // system), then concatenated in system order
+        std::vector<WorkCSR> nws(ns), gws(ns);
+        std::vector<std::vector<double>> ctrs(ns);
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < ns; s++) {
+            csr_start(nws[s]);
+            csr_start(gws[s]);
+            per_system(sys_[s], nws[s], gws[s], ctrs[s]);
+        }
+        auto append = [](WorkCSR &w, const WorkCSR &f) {
+            const int so = (int)w.seg_beg.size(), co = (int)w.clu_id.size();
+            w.key.insert(w.key.end(), f.key.begin(), f.key.end());
+            w.tbeg.insert(w.tbeg.end(), f.tbeg.begin(), f.tbeg.end());
+            w.tcnt.insert(w.tcnt.end(), f.tcnt.begin(), f.tcnt.end());
+            for (size_t k = 1; k < f.seg_off.size(); k++) w.seg_off.push_back(so + f.seg_off[k]);
+            for (size_t k = 1; k < f.clu_off.size(); k++) w.clu_off.push_back(co + f.clu_off[k]);
+            w.seg_beg.insert(w.seg_beg.end(), f.seg_beg.begin(), f.seg_beg.end());
+            w.seg_cnt.insert(w.seg_cnt.end(), f.seg_cnt.begin(), f.seg_cnt.end());
+            w.seg_chk.insert(w.seg_chk.end(), f.seg_chk.begin(), f.seg_chk.end());
+            w.clu_id.insert(w.clu_id.end(), f.clu_id.begin(), f.clu_id.end());
+            w.pairs += f.pairs;
+        };
+        for (int s = 0; s < ns; s++) {
+            append(near_, nws[s]);
+            append(grid_, gws[s]);
+            near_ctr_.insert(near_ctr_.end(), ctrs[s].begin(), ctrs[s].end());
         }
     }
 
@@ -936,7 +970,9 @@ std::string Plan::device_trees(const PlanInput &in)
     }
     // the host trees from the device records
     double tf = now();
-    for (size_t k = 0; k < sys_.size(); k++) {
+    const int nsys = (int)sys_.size();
+#pragma omp parallel for schedule(dynamic) if (nsys > 1)
+    for (int k = 0; k < nsys; k++) {
         System &y = sys_[k];
         if (shared_) {
             const std::vector<NodeRec> leaves = leaf_records(srec[k]);
