@@ -81,6 +81,14 @@ __device__ __forceinline__ double exp_nonpos(double x, const double *tab)
     return __builtin_amdgcn_ldexp(tab[n & (EXP_TAB - 1)] * p, n >> 6);
 }
 
+// e^x, x <= 0: the table form (TAB) or the library's
+template <bool TAB>
+__device__ __forceinline__ double exp_np(double x, const double *tab)
+{
+    if constexpr (TAB) return exp_nonpos(x, tab);
+    else return exp(x);
+}
+
 // ---------------------------------------------------------------------
 // The kernels beside the Coulomb family, FP64, one pair's term with the
 // charge a (b, tq: subtraction weight and target charge):
@@ -149,26 +157,28 @@ __device__ __forceinline__ double other_term(double dx, double dy, double dz, do
 // can never trigger; cluster sources never need it (an accepted cluster is
 // separated from its targets).
 // ---------------------------------------------------------------------
-template <int KER, bool SS, bool CHECK>
+// TAB: exponentials by exp_nonpos from the table tab.
+template <int KER, bool SS, bool CHECK, bool TAB = false>
 __device__ __forceinline__ void pair_pot(double dx, double dy, double dz, double a, double b,
-                                         double tq, const KParams &kp, double &acc)
+                                         double tq, const KParams &kp, double &acc,
+                                         const double *tab = nullptr)
 {
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if constexpr (KER == KC) {
         double ir = rsq64(r2);
         if constexpr (CHECK) ir = r2 > (SS ? 0.0 : DBL_MIN) ? ir : 0.0;
         if constexpr (!SS) acc = fma(a, ir, acc);
-        else acc = fma(fma(-tq * b, exp(-r2 * kp.inv_a2), a), ir, acc);
+        else acc = fma(fma(-tq * b, exp_np<TAB>(-r2 * kp.inv_a2, tab), a), ir, acc);
     } else if constexpr (KER == KY) {
         const double ir = rsq64(r2);
-        double e = exp(-kp.p0 * (r2 * ir)) * ir;
+        double e = exp_np<TAB>(-kp.p0 * (r2 * ir), tab) * ir;
         if constexpr (CHECK) e = r2 > 0.0 ? e : 0.0;
         if constexpr (!SS) acc = fma(a, e, acc);
         else acc = fma(fma(-tq, b, a), e, acc);
     } else if constexpr (KER == KR) {
         const double ir = rsq64(r2 + kp.eps2);
         if constexpr (!SS) acc = fma(a, ir, acc);
-        else acc = fma(fma(-tq * b, exp(-r2 * kp.inv_a2), a), ir, acc);
+        else acc = fma(fma(-tq * b, exp_np<TAB>(-r2 * kp.inv_a2, tab), a), ir, acc);
     } else {
         acc += other_term<KER, SS, CHECK>(dx, dy, dz, r2, a, b, tq, kp);
     }
@@ -181,18 +191,19 @@ __device__ __forceinline__ void pair_pot(double dx, double dy, double dz, double
 // (coulomb_cp.c:46-55, coulomb_ss_cp.c:54-55, coulomb_ss_cc.c:54-55,
 //  yukawa_cp.c:52, yukawa_ss_cp.c:55-56, yukawa_ss_cc.c:55-56, regularized-coulomb_cp.c:52)
 // The reference applies no skip in these kernels.
-template <int KER, bool SS>
+template <int KER, bool SS, bool TAB = false>
 __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, double a, double b,
-                                          const KParams &kp, double &q, double &w)
+                                          const KParams &kp, double &q, double &w,
+                                          const double *tab = nullptr)
 {
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if constexpr (KER == KC) {
         const double ir = rsq64(r2);
         q = fma(a, ir, q);
-        if constexpr (SS) w = fma(exp(-r2 * kp.inv_a2) * b, ir, w);
+        if constexpr (SS) w = fma(exp_np<TAB>(-r2 * kp.inv_a2, tab) * b, ir, w);
     } else if constexpr (KER == KY) {
         const double ir = rsq64(r2);
-        const double e = exp(-kp.p0 * (r2 * ir)) * ir;
+        const double e = exp_np<TAB>(-kp.p0 * (r2 * ir), tab) * ir;
         q = fma(a, e, q);
         if constexpr (SS) w = fma(b, e, w);
     } else if constexpr (KER == KR) {
@@ -323,7 +334,7 @@ __device__ __forceinline__ void pair_grid32(float r2, double a, double b, const 
 
 template <int KER, bool SS, bool GRID, bool MIXED, int TPT>
 __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile tile, double4 *Ls,
-                                              double *Lb, float2 *Lf)
+                                              double *Lb, float2 *Lf, const double *Ltab)
 {
     const int key = tile.x;
     const int tid = threadIdx.x;
@@ -524,8 +535,8 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
                     }
                 } else {
                     const double dx = Tx[u] - s.x, dy = Ty[u] - s.y, dz = Tz[u] - s.z;
-                    if constexpr (GRID) pair_grid<KER, SS>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u]);
-                    else pair_pot<KER, SS, CHECK>(dx, dy, dz, s.w, b, Tq[u], A.kp, acc[u]);
+                    if constexpr (GRID) pair_grid<KER, SS, true>(dx, dy, dz, s.w, b, A.kp, acc[u], accw[u], Ltab);
+                    else pair_pot<KER, SS, CHECK, true>(dx, dy, dz, s.w, b, Tq[u], A.kp, acc[u], Ltab);
                 }
             }
         }
@@ -631,10 +642,13 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
     __shared__ double4 Ls[TPB];
     __shared__ double Lb[SS ? TPB : 1];
     __shared__ float2 Lf[MIXED && (SS || GRID) ? TPB : 1];
+    // 2^(j/64) for the FP64 exponentials (read after the first staging barrier)
+    __shared__ double Ltab[EXP_TAB];
+    if constexpr (!MIXED && (SS || KER == KY)) exp_table(Ltab, threadIdx.x);
     if (!A.persistent) {
         const Tile tile = A.tiles[blockIdx.x];
         if (tile.z == 0) return;  // padding tile (uniform across the block)
-        interact_tile<KER, SS, GRID, MIXED, TPT>(A, tile, Ls, Lb, Lf);
+        interact_tile<KER, SS, GRID, MIXED, TPT>(A, tile, Ls, Lb, Lf, Ltab);
         return;
     }
     __shared__ int Lt;
@@ -656,7 +670,7 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
         const int t = Lt;
         __syncthreads();
         if (t < 0) break;
-        interact_tile<KER, SS, GRID, MIXED, TPT>(A, A.tiles[t], Ls, Lb, Lf);
+        interact_tile<KER, SS, GRID, MIXED, TPT>(A, A.tiles[t], Ls, Lb, Lf, Ltab);
     }
 }
 

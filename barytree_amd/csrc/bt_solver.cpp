@@ -1047,10 +1047,11 @@ std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
 
     herm_ = prm.approximation == A_HERMITE;
     // symmetric near field: one system (Coulomb or Yukawa, or regularized
-    // Coulomb with skipping) whose targets are its sources, particle for
-    // particle in tree order (the same order after both tree builds)
+    // Coulomb with skipping; either approximation, which only concerns the
+    // far field) whose targets are its sources, particle for particle in
+    // tree order (the same order after both tree builds)
     sym_ok_ = false;
-    if (sys_.size() == 1 && prm.approximation == A_LAGRANGE &&
+    if (sys_.size() == 1 &&
         (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA ||
          (prm.kernel == K_REG_COULOMB && prm.singularity == S_SKIPPING))) {
         const System &y = sys_[0];

@@ -117,8 +117,9 @@ typedef struct BaryTreeStats {
     long long sym_ptwise;    /* near-field particle pairs (ordered) that the
                                 symmetric near field delivers (either precision):
                                 a single system whose targets are its sources in
-                                the same tree order (Lagrange; Coulomb, Yukawa,
-                                or regularized Coulomb with skipping); pairs of
+                                the same tree order (Coulomb, Yukawa, or
+                                regularized Coulomb with skipping; Lagrange or
+                                Hermite); pairs of
                                 two leaves in each other's near list are
                                 evaluated once for both particles, the rest once
                                 for their target; 0 otherwise or when switched

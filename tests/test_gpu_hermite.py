@@ -77,6 +77,29 @@ def test_hermite_cp_keys_split_by_sources(bt, oracle_mod):
     assert err <= TOL_FP64
 
 
+@pytest.mark.parametrize("ct,k,s", [(1, 1, 2), (1, 2, 1), (1, 3, 1), (2, 1, 1), (2, 2, 1)])
+def test_hermite_targets_are_sources(bt, oracle_mod, ct, k, s):
+    """Targets equal to the sources: the near field runs in the symmetric
+    launch (the approximation concerns only the far field).  The plain-mode
+    bar against the oracle, and the one-directional near field (symmetric
+    form off) gives the same sums up to their order."""
+    print(f"case Hermite ct={ct} k={k} s={s} targets=sources N=6000", flush=True)
+    _, src = uniform_cube(6000, 1, seed=700 + 10 * ct + k)
+    t = [a.copy() for a in src[:4]]
+    args = (k, KP[k][s], s, 2, ct, 0.7, 4, 300, 300)
+    ref = oracle_mod.treecode(*t, *src, *args)
+    with bt.Plan(t, src, *args) as p:
+        got = p.execute()
+        st = p.stats()
+        p.set_symmetric(False)
+        one = p.execute()
+    assert st["sym_ptwise"] == st["pp_ptwise"] > 0, st
+    err = rel_l2(ref, got)
+    print(f"relL2={err:.3e}", flush=True)
+    assert err <= TOL_FP64
+    assert rel_l2(one, got) <= 1e-13
+
+
 def test_hermite_batched_equals_single_solves(bt):
     systems = []
     for i, n in enumerate([300, 2500, 40, 1800]):

@@ -303,8 +303,11 @@ def test_symmetric_near_field_eligibility(bt):
     for ct in (1, 2):  # particle-cluster and cluster-particle with equal leaf sizes
         a = bt.analyze(same, src, 1, [0.5], 1, 1, ct, 0.8, 6, 800, 800)
         assert a["sym_ptwise"] == a["pp_ptwise"] and a["sym_evals"] < 0.6 * a["pp_ptwise"], (ct, a)
+    # the approximation concerns only the far field: Hermite plans too
     herm = bt.analyze(same, src, 1, [0.5], 1, 2, 2, 0.8, 6, 800, 800)  # CP Hermite
-    assert herm["sym_ptwise"] == 0 and herm["sym_slots"] == 0
+    lag = bt.analyze(same, src, 1, [0.5], 1, 1, 2, 0.8, 6, 800, 800)
+    assert herm["sym_ptwise"] == herm["pp_ptwise"] == lag["sym_ptwise"]
+    assert herm["sym_slots"] == lag["sym_slots"] > 0
     assert bt.analyze(same, src, 1, [0.5], 1, 1, 1, 0.8, 6, 800, 100)["sym_ptwise"] == 0  # PC, other order
     assert bt.analyze(same, src, 8, [1.5], 1, 1, 3, 0.8, 6, 800, 800)["sym_ptwise"] == 0  # sin-over-r
     # different trees
