@@ -15,8 +15,8 @@ struct alignas(8) Seg { int x, y; };
 
 // Seg.y = count | flags: the singularity skip may trigger in the range;
 // a pair of the range may come closer than its key's threshold (key_ctr);
-// the range is a leaf whose pairs with the key's leaf the symmetric near
-// field evaluates (launches with skip_sym pass over it).
+// the range's pairs with the key's particles run in the item launch
+// (near launches with skip_sym pass over it).
 constexpr unsigned SEG_COINCIDE_BIT = 0x80000000u;
 constexpr unsigned SEG_CLOSE_BIT = 0x40000000u;
 constexpr unsigned SEG_SYM_BIT = 0x20000000u;
@@ -93,9 +93,10 @@ struct InteractArgs {
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
-// Symmetric near field (mixed precision; Coulomb and Yukawa with either
-// singularity, regularized Coulomb with skipping; a cluster-cluster system
-// whose targets are its sources).  A two-sided item (cslot >= 0) is a pair
+// Item launch of the near field's particle pairs (Coulomb and Yukawa with
+// either singularity, regularized Coulomb with skipping; FP64 or mixed
+// precision): one system whose targets are its sources, or (one-sided items
+// only) one system with other targets.  A two-sided item (cslot >= 0) is a pair
 // of distinct leaves A, B, each in the other's near list: every pair (a, b)
 // is evaluated once and gives both a's term from b and b's term from a.
 // A one-sided item (cslot < 0) is a leaf A with a source range B of its near
@@ -113,7 +114,8 @@ struct SymArgs {
     int fp64;  // 1: FP64 arithmetic (plain executions), 0: mixed precision
     const SymItem *items;
     const double4 *ctr;
-    const double *x, *y, *z, *q, *w;  // the particles, tree order (w: subtraction only)
+    const double *x, *y, *z, *q, *w;  // the sources, tree order (w: subtraction only)
+    const double *tx, *ty, *tz, *tq;   // the targets, tree order (symmetric plans: the sources)
     KParams kp;
     double *slots;
 };

@@ -784,15 +784,20 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
             const int li = p0 + wv * G * TPT + rg + G * k;
             const bool v = li < it.a_cnt;
             const int g = it.a_beg + (v ? li : 0);
-            const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
+            const double x = lam * (A.tx[g] - ctr.x), y = lam * (A.ty[g] - ctr.y), z = lam * (A.tz[g] - ctr.z);
             T2[k] = fma(z, z, fma(y, y, x * x));
             Tx[k] = -2.0 * x; Ty[k] = -2.0 * y; Tz[k] = -2.0 * z;
-            double q;
-            const float2 c = charges(g, q);
-            Tq[k] = q;
-            // a target outside A gives nothing to the sources
-            ai[k] = v ? c.x : 0.0f;
-            bi[k] = v ? c.y : 0.0f;
+            Tq[k] = SS ? A.tq[g] : 0.0;
+            if constexpr (TWO) {
+                // the target as a source (two-sided items: targets are the
+                // sources); a target outside A gives nothing to the sources
+                double q;
+                const float2 c = charges(g, q);
+                ai[k] = v ? c.x : 0.0f;
+                bi[k] = v ? c.y : 0.0f;
+            } else {
+                ai[k] = 0.0f; bi[k] = 0.0f;
+            }
             acc[k] = 0.0;
             p1[k] = 0.0f; p2[k] = 0.0f;
         }
@@ -964,13 +969,18 @@ __global__ __launch_bounds__(64 * NW) void k_near_sym64(SymArgs A)
             const int li = p0 + wv * G * TPT + rg + G * k;
             const bool v = li < it.a_cnt;
             const int g = it.a_beg + (v ? li : 0);
-            Tx[k] = A.x[g]; Ty[k] = A.y[g]; Tz[k] = A.z[g];
-            const double q = A.q[g];
-            const double w = SS ? A.w[g] : 1.0;
-            Tnq[k] = -q;
-            // a target outside A gives nothing to the sources
-            Ta[k] = v ? (SS ? q * w : q) : 0.0;
-            Tw[k] = v ? w : 0.0;
+            Tx[k] = A.tx[g]; Ty[k] = A.ty[g]; Tz[k] = A.tz[g];
+            Tnq[k] = SS ? -A.tq[g] : 0.0;
+            if constexpr (TWO) {
+                // the target as a source (two-sided items: targets are the
+                // sources); a target outside A gives nothing to the sources
+                const double q = A.q[g];
+                const double w = SS ? A.w[g] : 1.0;
+                Ta[k] = v ? (SS ? q * w : q) : 0.0;
+                Tw[k] = v ? w : 0.0;
+            } else {
+                Ta[k] = 0.0; Tw[k] = 0.0;
+            }
             acc[k] = 0.0;
         }
         for (int off = 0; off < it.b_cnt; off += NT) {

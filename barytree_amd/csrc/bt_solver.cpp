@@ -202,7 +202,8 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
 // 6e-15 (|t-c|^2 + |s-c|^2)), so pairs below 2e-7 R^2 are recomputed from
 // differences when the boxes come within that distance.
 void Plan::build_sym(const System &sy, const std::vector<int> &keys,
-                     const std::vector<std::vector<int>> &segs_of, std::vector<std::vector<char>> &sym_of)
+                     const std::vector<std::vector<int>> &segs_of, std::vector<std::vector<char>> &sym_of,
+                     bool symmetric)
 {
     const Tree &st = sy.stree, &tt = sy.ttree;
     sym_items_.clear(); sym_ctr_.clear(); sym_red_tiles_.clear();
@@ -221,7 +222,7 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
     // (key, key of the source range) of every near pair whose source range is a key
     std::vector<long long> present;
     const long long nk = (long long)keys.size();
-    for (size_t k = 0; k < keys.size(); k++)
+    for (size_t k = 0; symmetric && k < keys.size(); k++)
         for (int s : segs_of[keys[k]]) {
             const int ks = key_of(st.ibeg[s], st.numpar[s]);
             if (ks >= 0) present.push_back((long long)k * nk + ks);
@@ -236,7 +237,7 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         sym_of[t].assign(segs_of[t].size(), 1);
         for (size_t i = 0; i < segs_of[t].size(); i++) {
             const int s = segs_of[t][i];
-            const int ks = key_of(st.ibeg[s], st.numpar[s]);
+            const int ks = symmetric ? key_of(st.ibeg[s], st.numpar[s]) : -1;
             const bool two = ks >= 0 && ks != (int)k && has(ks, (long long)k);
             if (!two) cand.push_back({(int)k, -1, s});
             else if ((int)k < ks) cand.push_back({(int)k, ks, s});
@@ -257,8 +258,8 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
     for (const Cand &c : cand) {
         const int t = keys[c.k], s = c.s;
         btk::SymItem it;
-        // targets are the sources: target index = source index
-        it.a_beg = sy.soff + tt.ibeg[t]; it.a_cnt = tt.numpar[t];
+        // A: targets, B: sources (symmetric plans: the same particles)
+        it.a_beg = sy.toff + tt.ibeg[t]; it.a_cnt = tt.numpar[t];
         it.b_beg = sy.soff + st.ibeg[s]; it.b_cnt = st.numpar[s];
         it.rslot = sym_slots_len_; sym_slots_len_ += it.a_cnt;
         it.cslot = -1;
@@ -356,6 +357,8 @@ void Plan::build_work()
     grid_ = WorkCSR();
     near_ctr_.clear();
     sym_ = false;
+    // items for the near field's particle pairs (one system)
+    const bool items = sym_ok_ || items_ok_;
     // the work of one system into (near keys, grid keys, near centres)
     auto per_system = [&](const System &sy, WorkCSR &nw, WorkCSR &gw, std::vector<double> &nctr) {
         const Tree &st = sy.stree, &tt = sy.ttree;
@@ -425,13 +428,13 @@ void Plan::build_work()
         if (p.compute_type == C_PC) {
             // keys: target batches; pc list -> clusters, pp list -> source leaves
             std::vector<std::vector<char>> sym_of(tt.n);
-            if (sym_ok_) {
+            if (items) {
                 std::vector<int> keys(tt.n);
                 for (int b = 0; b < tt.n; b++) keys[b] = b;
-                build_sym(sy, keys, L.pp, sym_of);
+                build_sym(sy, keys, L.pp, sym_of, sym_ok_);
             }
             for (int b = 0; b < tt.n; b++)
-                near_key(b, to + tt.ibeg[b], tt.numpar[b], L.approx[b], L.pp[b], sym_ok_ ? &sym_of[b] : nullptr);
+                near_key(b, to + tt.ibeg[b], tt.numpar[b], L.approx[b], L.pp[b], items ? &sym_of[b] : nullptr);
         } else if (p.compute_type == C_CP) {
             // lists keyed by source batch; invert to target-node keys
             std::vector<std::vector<int>> pp_of(tt.n), cp_of(tt.n);
@@ -441,10 +444,10 @@ void Plan::build_work()
             }
             const std::vector<int> none;
             std::vector<std::vector<char>> sym_of(tt.n);
-            if (sym_ok_) build_sym(sy, tt.leaves, pp_of, sym_of);
+            if (items) build_sym(sy, tt.leaves, pp_of, sym_of, sym_ok_);
             for (int leaf : tt.leaves)
                 near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], none, pp_of[leaf],
-                         sym_ok_ ? &sym_of[leaf] : nullptr);
+                         items ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (cp_of[t].empty()) continue;
                 add_key(gw, tn + t, 0, P);
@@ -462,10 +465,10 @@ void Plan::build_work()
                 }
             }
             std::vector<std::vector<char>> sym_of(tt.n);
-            if (sym_ok_) build_sym(sy, tt.leaves, segs_of, sym_of);
+            if (items) build_sym(sy, tt.leaves, segs_of, sym_of, sym_ok_);
             for (int leaf : tt.leaves)
                 near_key(leaf, to + tt.ibeg[leaf], tt.numpar[leaf], clus_of[leaf], segs_of[leaf],
-                         sym_ok_ ? &sym_of[leaf] : nullptr);
+                         items ? &sym_of[leaf] : nullptr);
             for (int t = 0; t < tt.n; t++) {
                 if (L.approx[t].empty() && L.cc_cp[t].empty()) continue;
                 add_key(gw, tn + t, 0, P);
@@ -658,8 +661,8 @@ std::string Plan::validate() const
         if (sym_ctr_.size() != 4 * sym_items_.size() || sym_items_.size() > (size_t)INT32_MAX) return "sym items";
         for (const btk::SymItem &it : sym_items_) {
             if (it.a_cnt == 0 && it.b_cnt == 0 && it.rslot == 0 && it.cslot == 0) continue;  // padding
-            if (it.a_beg < 0 || it.a_cnt <= 0 || (long long)it.a_beg + it.a_cnt > N_ || it.b_beg < 0 ||
-                it.b_cnt <= 0 || (long long)it.b_beg + it.b_cnt > N_ || N_ != M_)
+            if (it.a_beg < 0 || it.a_cnt <= 0 || (long long)it.a_beg + it.a_cnt > M_ || it.b_beg < 0 ||
+                it.b_cnt <= 0 || (long long)it.b_beg + it.b_cnt > N_ || (it.cslot >= 0 && (!sym_ok_ || N_ != M_)))
                 return "sym item range";
             if (it.rslot < 0 || it.rslot + it.a_cnt > sym_slots_len_ || it.cslot < -1 ||
                 (it.cslot >= 0 && it.cslot + it.b_cnt > sym_slots_len_))
@@ -1066,6 +1069,11 @@ std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
                       same(y.S.q, y.T.q);
         }
     }
+    // one system with other targets, skipping kernels: one-sided items
+    // (run in mixed precision, in place of the one-directional particle
+    // pairs of the near launch)
+    items_ok_ = !sym_ok_ && sys_.size() == 1 && prm.singularity == S_SKIPPING &&
+                (prm.kernel == K_COULOMB || prm.kernel == K_YUKAWA || prm.kernel == K_REG_COULOMB);
     build_work();
     enumerate_pairs();
     if (herm_) build_hermite_work();
@@ -1302,7 +1310,7 @@ std::string Plan::down_fixed()
 // the potential (after the near launch has stored every target's sum).
 bool Plan::sym_ready()
 {
-    if (!sym_ || !sym_enabled_) return false;
+    if (!items_run()) return false;
     if (d_sym_slots_) return true;
     // the slots at the first execution that uses them; without the memory
     // for them the plan keeps the one-directional near field
@@ -1329,6 +1337,7 @@ std::string Plan::near_sym()
     a.items = d_sym_items_;
     a.ctr = reinterpret_cast<const double4 *>(d_sym_ctr_);
     a.x = d_sx_; a.y = d_sy_; a.z = d_sz_; a.q = d_sq_; a.w = d_sw_;
+    a.tx = d_tx_; a.ty = d_ty_; a.tz = d_tz_; a.tq = d_tq_;
     a.kp = kp_;
     a.slots = d_sym_slots_;
     HIPCHK(hipEventRecord(sym_ev_[0], st_));

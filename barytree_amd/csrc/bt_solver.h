@@ -155,10 +155,11 @@ public:
     const Params &params() const { return prm_; }
     double *out_dev() { return d_out_; }
     long long near_pairs() const { return near_.pairs; }
-    // pairs (ordered) the symmetric near field covers; 0 without it
-    long long sym_pairs() const { return sym_ && sym_enabled_ ? sym_pairs_ : 0; }
-    long long sym_evals() const { return sym_ && sym_enabled_ ? sym_evals_ : 0; }
-    long long sym_slots() const { return sym_ && sym_enabled_ ? sym_slots_len_ : 0; }
+    // pairs (ordered) the item launch delivers at the plan's precision, the
+    // pair evaluations it executes and its slots; 0 without it
+    long long sym_pairs() const { return items_run() ? sym_pairs_ : 0; }
+    long long sym_evals() const { return items_run() ? sym_evals_ : 0; }
+    long long sym_slots() const { return items_run() ? sym_slots_len_ : 0; }
     long long far_pairs() const { return grid_.pairs; }
     int source_nodes() const { return nsnodes_; }
     int target_nodes() const { return ntnodes_; }
@@ -185,13 +186,17 @@ private:
     template <class T, class A> std::string up_at(T *d, const std::vector<T, A> &h);
     void free_all();
     void build_work();
-    // the symmetric near field's items, slots and reduction lists (one
-    // system whose targets are its sources): sym_ok_ set by create
+    // the item launch's items, slots and reduction lists (one system):
+    // two-sided and one-sided items when its targets are its sources
+    // (sym_ok_), one-sided items only otherwise (items_ok_); set by create
     void build_sym(const System &sy, const std::vector<int> &keys, const std::vector<std::vector<int>> &segs_of,
-                   std::vector<std::vector<char>> &sym_of);
+                   std::vector<std::vector<char>> &sym_of, bool symmetric);
+    // the item launch runs in this execution's precision: always for a
+    // symmetric plan, in mixed precision for one-sided items only
+    bool items_run() const { return sym_ && sym_enabled_ && (sym_ok_ || prm_.precision == PREC_MIXED); }
     std::string upward_fixed();  // upward pass at the plan's degree
     std::string down_fixed();    // downpass at the plan's degree
-    bool sym_ready();  // enabled, and the slots allocated (at the first execution that uses them)
+    bool sym_ready();  // items_run(), and the slots allocated (at the first execution that uses them)
     std::string phase_events();
     std::string near_sym();
     std::string validate() const;
@@ -222,17 +227,20 @@ private:
     // mixed near field recomputes a pair from coordinate differences
     // (negated: the key's clusters are checked too)
     std::vector<double> near_ctr_;
-    // symmetric near field: mixed precision, Coulomb or Yukawa (either
-    // singularity) or regularized Coulomb (skipping), one system whose
-    // targets are its sources (the same particles in the same tree order).
-    // Pairs of near-field keys (A, B), each key's particle range a source
-    // range of the other's near list, are evaluated once for both
-    // directions (items, sorted by
-    // work); their source ranges carry SEG_SYM and the near launch passes
-    // over them.  Item k writes A's sums to slots [rslot, +|A|) and B's to
-    // [cslot, +|B|); the reduction adds, for each particle of leaf list l,
-    // slots[base + i] for the bases [sym_red_off_[l], sym_red_off_[l+1]).
-    bool sym_ok_ = false, sym_ = false, sym_enabled_ = true;
+    // Item launch of the near field's particle pairs (Coulomb or Yukawa, or
+    // regularized Coulomb with skipping).  Symmetric plans (sym_ok_: one
+    // system whose targets are its sources, the same particles in the same
+    // tree order): pairs of near-field keys (A, B), each key's particle
+    // range a source range of the other's near list, are evaluated once for
+    // both directions (two-sided items), every other range of a key's list
+    // once for the key (one-sided items), in both precisions.  One system
+    // with other targets, skipping, mixed precision (items_ok_): every range
+    // as a one-sided item.  The ranges carry SEG_SYM and the near launch
+    // passes over them when the items run.  Item k writes A's sums to slots
+    // [rslot, +|A|) and B's to [cslot, +|B|); the reduction adds, for each
+    // particle of leaf list l, slots[base + i] for the bases
+    // [sym_red_off_[l], sym_red_off_[l+1]).
+    bool sym_ok_ = false, items_ok_ = false, sym_ = false, sym_enabled_ = true;
     long long model_fallback_pairs_ = 0;
     // the trees were built on the device; one tree and one set of particle
     // arrays serve as sources and targets (every system's targets are its

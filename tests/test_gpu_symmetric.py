@@ -247,3 +247,38 @@ def test_symmetric_fp64_through_the_reference_entry_point(bt, oracle_mod):
     assert st["sym_ptwise"] > 0, st
     ref = oracle_mod.treecode(*t, *s, *args)
     assert rel_l2(ref, got) <= 1e-12
+
+
+@pytest.mark.parametrize("ct,k", [(1, 1), (2, 2), (3, 3), (3, 1)])
+def test_one_sided_items_for_other_targets(bt, oracle_mod, ct, k):
+    """One system whose targets differ from its sources, skipping kernel:
+    in mixed precision every near particle pair runs as a one-sided item
+    (ordered pairs delivered = pairs evaluated = the plan's PP count).
+    Against the oracle and against the one-directional near launch (items
+    switched off) at the mixed bar; FP64 executions keep the one-directional
+    launch (the same bits with items on or off)."""
+    print(f"case ct={ct} kernel={k} other targets, mixed", flush=True)
+    rng = np.random.default_rng(40 + 10 * ct + k)
+    s = tuple(rng.uniform(-1, 1, 20000) for _ in range(4)) + (np.ones(20000),)
+    t = tuple(rng.uniform(-1, 1, 15000) for _ in range(4))
+    args = (k, KPS[k], 1, 1, ct, 0.7, 6, 500, 500)
+    ref = oracle_mod.treecode(*t, *s, *args)
+    with bt.Plan(t, s, *args, precision=bt.PRECISION_MIXED) as plan:
+        got = plan.execute()
+        st = plan.stats()
+        plan.set_symmetric(False)
+        one = plan.execute()
+        plan.set_symmetric(True)
+        plan.set_precision(bt.PRECISION_FP64)
+        fp64 = plan.execute()
+        st64 = plan.stats()
+        plan.set_symmetric(False)
+        fp64_off = plan.execute()
+    assert st["sym_ptwise"] == st["sym_evals"] == st["pp_ptwise"] > 0, st
+    assert st["gpu_near_sym_s"] > 0.0
+    assert st64["sym_ptwise"] == 0 and st64["sym_slots"] == 0
+    assert np.array_equal(fp64, fp64_off)
+    assert rel_l2(ref, fp64) <= 1e-12
+    err, diff = rel_l2(ref, got), rel_l2(one, got)
+    print(f"items vs oracle {err:.3e}, vs one-directional launch {diff:.3e}", flush=True)
+    assert err < 1e-6 and diff < 1e-6
