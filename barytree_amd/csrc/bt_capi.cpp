@@ -201,13 +201,22 @@ int solve(int M, int N, const double *tx, const double *ty, const double *tz, co
                          approximation, compute_type, theta, degree, msl, mtl, size_check, beta,
                          verbosity, precision, device);
     if (rc) return rc;
+    const auto t_created = std::chrono::steady_clock::now();
     std::string e = plan.execute(true);
+    const auto t_exec = std::chrono::steady_clock::now();
     if (e.empty()) e = plan.download(out, device);
     if (!e.empty()) return fail(e);
     fill_stats(plan, &g_stats);
     g_have_stats = true;
-    print_diagnostics(g_stats, M, N,
-                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(), verbosity);
+    const auto t_down = std::chrono::steady_clock::now();
+    plan.release();
+    const auto t_end = std::chrono::steady_clock::now();
+    if (verbosity > 3) {
+        auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+        std::printf("[BaryTree] call: plan %.6f, execution %.6f, download %.6f, release %.6f s\n",
+                    sec(t_start, t_created), sec(t_created, t_exec), sec(t_exec, t_down), sec(t_down, t_end));
+    }
+    print_diagnostics(g_stats, M, N, std::chrono::duration<double>(t_end - t_start).count(), verbosity);
     return 0;
 }
 
@@ -1079,14 +1088,26 @@ static int solve_learned(int numTargets, int numSources, const double *targetX,
                          verbosity, precision, device);
     if (rc) return rc;
     std::string e;
+    const auto t_created = std::chrono::steady_clock::now();
     if (model) e = use_model(plan, model->m, delta);
+    const auto t_model = std::chrono::steady_clock::now();
     if (e.empty()) e = plan.execute(true);
+    const auto t_exec = std::chrono::steady_clock::now();
     if (e.empty()) e = plan.download(outputArray, device);
     if (!e.empty()) return fail(e);
     fill_stats(plan, &g_stats);
     g_have_stats = true;
-    print_diagnostics(g_stats, numTargets, numSources,
-                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count(), verbosity);
+    const auto t_down = std::chrono::steady_clock::now();
+    plan.release();
+    const auto t_end = std::chrono::steady_clock::now();
+    if (verbosity > 3) {
+        auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+        std::printf("[BaryTree] call: plan %.6f, degree model %.6f, execution %.6f, download %.6f, release %.6f s\n",
+                    sec(t_start, t_created), sec(t_created, t_model), sec(t_model, t_exec), sec(t_exec, t_down),
+                    sec(t_down, t_end));
+    }
+    print_diagnostics(g_stats, numTargets, numSources, std::chrono::duration<double>(t_end - t_start).count(),
+                      verbosity);
     return 0;
 }
 

@@ -684,11 +684,12 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // same targets), and a staged chunk of up to 64*NW sources of B is split over
 // the groups: at step s lane l takes source (64/G)*s + l/G against its TPT
 // targets.  The targets' sums (a terms and b terms apart, FP32, flushed to
-// FP64 every 32 steps) stay in registers over the whole of B; a source's
-// sums over a lane's targets are added over its group (DPP row_shr scan:
-// the group's last lane ends with the sum), over the NW waves in LDS, then
-// combined in FP64 as
-// p1 - q p2 into the source's column slot (stored at the first pass over
+// FP64 every 32 steps) stay in registers over the whole of B.  A source's
+// sum over a lane's targets (subtraction: the a terms minus the source's
+// charge times the b terms, combined in FP32 before any lane exchange, so
+// one scan per step serves both parts) is added over its group (DPP row_shr
+// scan: the group's last lane ends with the sum), over the NW waves in LDS
+// in FP64, and into the source's column slot (stored at the first pass over
 // A, added after: the block owns its slots, so the order is fixed).  In
 // items flagged close, a wave whose chunk held a pair below the threshold
 // evaluates the chunk again from coordinate differences (coincident pairs
@@ -741,14 +742,17 @@ __device__ __forceinline__ void sym_pair(float r2, float eps2s, float &fa, float
     }
 }
 
-template <int KER, bool SS, int TPT, int G, int NW>
-__global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least 4 waves per SIMD
+// a source's a and b (scaled) and, subtraction, its charge as a target
+template <bool SS> struct SymSrc { float a, b, q, pad; };
+template <> struct SymSrc<false> { float a, b; };
+
+template <int KER, bool SS, int TPT, int G, int NW, int WPE>
+__global__ __launch_bounds__(64 * NW, WPE) void k_near_sym(SymArgs A)  // at least WPE waves per SIMD
 {
     constexpr int NT = 64 * NW;  // threads of the block; sources per staged chunk
     __shared__ double4 Ls[NT];     // S' and |S'|^2
-    __shared__ float2 Lf[NT];      // the source's a and b (scaled)
-    __shared__ double Lq[SS ? NT : 1];  // the source's charge as a target
-    __shared__ float2 Lc[NW][NT];  // per wave: the sources' sums (a, b terms)
+    __shared__ SymSrc<SS> Lf[NT];
+    __shared__ float Lc[NW][NT];   // per wave: the sources' sums
     const SymItem it = A.items[blockIdx.x];
     const double4 ctr = A.ctr[blockIdx.x];
     constexpr int NG = 64 / G;  // source groups of a wave
@@ -763,13 +767,13 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
     constexpr int TPASS = NW * G * TPT;
     // a and b of a particle as a source: subtraction (lam q w, lam w),
     // skipping (lam q, 0)
-    auto charges = [&](int g, double &q) {
-        q = A.q[g];
+    auto charges = [&](int g) {
+        const double q = A.q[g];
         if constexpr (SS) {
             const double w = A.w[g];
-            return make_float2((float)(lam * q * w), (float)(lam * w));
+            return SymSrc<true>{(float)(lam * q * w), (float)(lam * w), (float)q, 0.0f};
         } else {
-            return make_float2((float)(lam * q), 0.0f);
+            return SymSrc<false>{(float)(lam * q), 0.0f};
         }
     };
 
@@ -791,10 +795,9 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
             if constexpr (TWO) {
                 // the target as a source (two-sided items: targets are the
                 // sources); a target outside A gives nothing to the sources
-                double q;
-                const float2 c = charges(g, q);
-                ai[k] = v ? c.x : 0.0f;
-                bi[k] = v ? c.y : 0.0f;
+                const SymSrc<SS> c = charges(g);
+                ai[k] = v ? c.a : 0.0f;
+                bi[k] = v ? c.b : 0.0f;
             } else {
                 ai[k] = 0.0f; bi[k] = 0.0f;
             }
@@ -807,15 +810,12 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
             if (tid < n) {
                 const int g = it.b_beg + off + tid;
                 const double x = lam * (A.x[g] - ctr.x), y = lam * (A.y[g] - ctr.y), z = lam * (A.z[g] - ctr.z);
-                double q;
-                Lf[tid] = charges(g, q);
+                Lf[tid] = charges(g);
                 Ls[tid] = make_double4(x, y, z, fma(z, z, fma(y, y, x * x)));
-                if constexpr (SS) Lq[tid] = q;
             } else {
                 // dummies: zero charge, |S'|^2 = 1e30 (every term exactly zero)
                 Ls[tid] = make_double4(0.0, 0.0, 0.0, 1.0e30);
-                Lf[tid] = make_float2(0.0f, 0.0f);
-                if constexpr (SS) Lq[tid] = 0.0;
+                Lf[tid] = SymSrc<SS>{};
             }
             __syncthreads();
             const int steps = (n + NG - 1) / NG;
@@ -832,7 +832,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
                 for (int st = sb; st < se; st++) {
                     const int j = NG * st + cg;
                     const double4 S = Ls[j];
-                    const float2 f = Lf[j];
+                    const SymSrc<SS> f = Lf[j];
                     [[maybe_unused]] float c1 = 0.0f, c2 = 0.0f;
 #pragma unroll
                     for (int k = 0; k < TPT; k++) {
@@ -847,17 +847,20 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
                         }
                         float fa, fb = 0.0f;
                         sym_pair<KER, SS, EXACT && KER != KR>(r2, eps2s, fa, fb);
-                        p1[k] = fmaf(f.x, fa, p1[k]);
+                        p1[k] = fmaf(f.a, fa, p1[k]);
                         if constexpr (TWO) c1 = fmaf(ai[k], fa, c1);
                         if constexpr (SS) {
-                            p2[k] = fmaf(f.y, fb, p2[k]);
+                            p2[k] = fmaf(f.b, fb, p2[k]);
                             if constexpr (TWO) c2 = fmaf(bi[k], fb, c2);
                         }
                     }
                     if constexpr (TWO) {
-                        c1 = group_sum<G>(c1);
-                        if constexpr (SS) c2 = group_sum<G>(c2);
-                        if (rg == G - 1) Lc[wv][j] = make_float2(c1, c2);
+                        // the source's sum over the lane's targets (subtraction:
+                        // a part minus its charge times the b part), then over the group
+                        float c = c1;
+                        if constexpr (SS) c = fmaf(-f.q, c2, c1);
+                        c = group_sum<G>(c);
+                        if (rg == G - 1) Lc[wv][j] = c;
                     }
                 }
                 // the FP32 sums of (at most) 32 steps into FP64
@@ -883,11 +886,10 @@ __global__ __launch_bounds__(64 * NW, 4) void k_near_sym(SymArgs A)  // at least
             if constexpr (TWO) {
                 __syncthreads();
                 if (tid < n) {
-                    double s1 = 0.0, s2 = 0.0;
+                    double v = 0.0;
 #pragma unroll
-                    for (int w = 0; w < NW; w++) { s1 += (double)Lc[w][tid].x; s2 += (double)Lc[w][tid].y; }
+                    for (int w = 0; w < NW; w++) v += (double)Lc[w][tid];
                     double *slot = A.slots + it.cslot + off + tid;
-                    const double v = SS ? s1 - Lq[SS ? tid : 0] * s2 : s1;
                     *slot = p0 == 0 ? v : *slot + v;
                 }
             }
@@ -1958,7 +1960,7 @@ hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
         return hipGetLastError();
     }
     switch (a.kernel) {
-#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G, SYM_NW><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
+#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G, SYM_NW, 4><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
         case KC: if (ss) L(KC, true); else L(KC, false); break;
         case KY: if (ss) L(KY, true); else L(KY, false); break;
         case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;

@@ -119,6 +119,9 @@ public:
     std::string execute(bool timed);
     // Copies the result to `out` (host memory, or device memory if `device`).
     std::string download(double *out, bool device = false);
+    // Frees the plan's device memory, events and stream (the destructor does
+    // the same; a released plan cannot execute again).
+    void release() { free_all(); }
     // FP64 direct sum on the GPU for caller-order target indices `idx`
     // (each target against the sources of its own system).
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);

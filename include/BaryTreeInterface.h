@@ -70,7 +70,11 @@ typedef enum COMPUTE_TYPE {
  * prints the error (with the "[BaryTree]" prefix), leaves outputArray zero
  * and records the message for BaryTree_GetLastError(); the process keeps
  * running.  More coincident particles than a leaf size are refused the same
- * way (the reference's tree recursion does not end there). */
+ * way (the reference's tree recursion does not end there).  verbosity > 0
+ * prints start and finish, > 1 the total time, > 2 the reference's
+ * interaction tables and this build's phase times; > 3 (the one-call
+ * BaryTree_Solve* forms) also the wall time of the call's parts: plan,
+ * degree model, execution, download, release. */
 void BaryTreeInterface(int numTargets, int numSources,
                        double *targetX, double *targetY, double *targetZ, double *targetValue,
                        double *sourceX, double *sourceY, double *sourceZ, double *sourceValue,
