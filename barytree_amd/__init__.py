@@ -11,4 +11,5 @@ from .interface import (  # noqa: F401
     DegreeModel, Schedule, Singularity, analyze, analyze_batched, analyze_degrees, autotune, callTreedriver,
     label_pairs, label_pairs_subtraction, last_error, last_stats, lib, solve, solve_batched, solve_learned,
     solve_device, DeviceBuffer, Batch,
-    BETA_LEAVES_CPU, BETA_LEAVES_GPU, set_beta_leaf_sizes)
+    BETA_LEAVES_CPU, BETA_LEAVES_GPU, set_beta_leaf_sizes, empty_device_cache, set_device_cache_limit,
+    cached_device_bytes)

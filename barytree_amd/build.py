@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libBaryTree_gpu.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("BARYTREE_ARCH", "gfx950")
 
-HOST_SRCS = ["bt_host.cpp", "bt_devtree.cpp", "bt_solver.cpp", "bt_vardeg.cpp", "bt_hermite_plan.cpp", "bt_model.cpp",
+HOST_SRCS = ["bt_host.cpp", "bt_devmem.cpp", "bt_devtree.cpp", "bt_solver.cpp", "bt_vardeg.cpp", "bt_hermite_plan.cpp", "bt_model.cpp",
              "bt_capi.cpp"]
 HIP_SRCS = ["bt_tree.hip", "bt_kernels.hip", "bt_learn.hip", "bt_hermite.hip", "bt_degree.hip", "bt_noalloc.hip"]
 # measurement tooling for the benchmark, in a library of its own

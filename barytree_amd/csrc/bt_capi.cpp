@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/BaryTreeInterface.h"
+#include "bt_devmem.h"
 #include "bt_learn.h"
 #include "bt_model.h"
 #include "bt_noalloc.h"
@@ -513,6 +514,16 @@ int BaryTree_Solve(int numTargets, int numSources, const double *targetX, const 
 }
 
 const char *BaryTree_GetLastError(void) { return g_last_error.c_str(); }
+
+unsigned long long BaryTree_EmptyDeviceCache(void) { return btm::empty_cache(); }
+
+int BaryTree_SetDeviceCacheLimit(unsigned long long bytes)
+{
+    btm::set_cache_limit((size_t)bytes);
+    return 0;
+}
+
+unsigned long long BaryTree_CachedDeviceBytes(void) { return btm::cached_bytes(); }
 
 int BaryTree_SetBetaLeafSizes(int which)
 {

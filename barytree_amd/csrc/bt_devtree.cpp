@@ -10,6 +10,7 @@
 // lower-part counts back to place the ranges; a level costs a few
 // synchronisations, a pass a read and a write of 28 bytes per particle.
 #include "bt_devtree.h"
+#include "bt_devmem.h"
 
 #include <algorithm>
 #include <cstring>
@@ -32,16 +33,25 @@ constexpr int CH = btk::TREE_CHUNK;
 template <class T> struct Grow {
     T *p = nullptr;
     size_t cap = 0;
-    ~Grow() { if (p) (void)hipFree(p); }
+    ~Grow() { drop(); }
+    // the buffer's work on the device is finished before it goes back to
+    // the device-memory cache
+    void drop()
+    {
+        if (!p) return;
+        (void)hipDeviceSynchronize();
+        btm::release(p);
+        p = nullptr;
+        cap = 0;
+    }
     hipError_t fit(size_t n)
     {
         if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) cap = n;
-        return e;
+        drop();
+        p = static_cast<T *>(btm::alloc(std::max<size_t>(n, 1) * sizeof(T)));
+        if (!p) return hipErrorOutOfMemory;
+        cap = n;
+        return hipSuccess;
     }
 };
 

@@ -95,6 +95,7 @@ void Plan::build_hermite_work()
         for (int node : grid_.key) node_rows_off_[node + 1]++;
         for (int n = 0; n < ntnodes_; n++) node_rows_off_[n + 1] += node_rows_off_[n];
         make_queue_tiles(grid_, GRID_TILE, true, P, grid_tiles_, grid_qlen_);
+        make_pencil_tiles(grid_, P, false, pencil_tiles_);
         // grid keys write rows of 8 planes
         grid_row_.assign(grid_.key.size(), 0);
         grid_rows_len_ = 0;

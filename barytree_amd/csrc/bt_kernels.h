@@ -93,6 +93,13 @@ struct InteractArgs {
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
+// Mixed-precision grid targets of Coulomb, Yukawa (either singularity) and
+// regularized Coulomb (skipping) with acceptance-separated sources, the
+// grids as tensor products: one block per tile {key, 0, nl^3, node} of
+// a.tiles, nl = a.nl (a.multi: the node field's degree + 1, 2..13); the
+// other fields as for launch_interact's grid targets (grid_scaled).
+hipError_t launch_grid_pencil(const InteractArgs &a, hipStream_t st);
+
 // Item launch of the near field's particle pairs (Coulomb and Yukawa with
 // either singularity, regularized Coulomb with skipping; FP64 or mixed
 // precision): one system whose targets are its sources, or (one-sided items

@@ -627,6 +627,208 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
     }
 }
 
+// ---------------------------------------------------------------------
+// Grid targets as tensor grids (mixed precision, acceptance-separated
+// sources: grid_scaled).  The NL^3 points of a target node's grid are
+// (x_a, y_b, z_c), so r^2 = (x_a - s_x)^2 + [(y_b - s_y)^2 + (z_c - s_z)^2]
+// and the bracket is shared by the NL points of the pencil (b, c).  A unit
+// owns the points a in [g L, (g+1) L) of pencil u % NL^2, g = u / NL^2, with
+// L the least length for which U = NL^2 ceil(NL/L) units fit the 256
+// threads; small grids (U <= 128) split the sources over 256/U groups of U
+// threads instead, whose sums are added in group order at the end.  Per
+// staged source a unit forms the bracket once (two subtractions, a multiply
+// and an fma), then per point one subtraction and one fma before the kernel
+// (against four FP64 instructions and a conversion per pair in the centred
+// expansion of interact_tile).  Coordinates are taken relative to the node's
+// centre and scaled by lam as in that expansion, then rounded to FP32; a
+// difference then carries the rounding of the two centred coordinates
+// (below 6e-8 of their size).  An acceptance-separated pair has r >= (1 -
+// theta) d with centred coordinates below about 2 d, so r^2 keeps a
+// relative error below about 5e-7 / (1 - theta), far less for most pairs.
+// FP32 sums over blocks of 32 sources, FP64 totals; rows as in k_interact.
+// One block per tile {key, 0, NL^3, node}; NL from the node field's degree
+// (several degrees) or the launch's.
+// ---------------------------------------------------------------------
+constexpr int pencil_len(int nl)
+{
+    for (int L = 1; L < nl; L++)
+        if (nl * nl * ((nl + L - 1) / L) <= 256) return L;
+    return nl;
+}
+
+// r2s: the scaled r'^2 (regularized kernel: r'^2 + e'^2)
+template <int KER, bool SS>
+__device__ __forceinline__ void pair_pencil(float r2s, float af, float bf, float &p1, float &p2)
+{
+    if constexpr (KER == KC) {
+        const float ir = rsq32(r2s);
+        p1 = fmaf(af, ir, p1);
+        if constexpr (SS) p2 = fmaf(bf, exp2_32(-r2s) * ir, p2);
+    } else if constexpr (KER == KY) {
+        const float ir = rsq32(r2s);
+        const float e = exp2_32(-(r2s * ir)) * ir;
+        p1 = fmaf(af, e, p1);
+        if constexpr (SS) p2 = fmaf(bf, e, p2);
+    } else {
+        p1 = fmaf(af, rsq32(r2s), p1);
+    }
+}
+
+constexpr int PENCIL_RED = 256;  // doubles per part of the group reduction
+
+template <int KER, bool SS, int NL>
+__device__ __forceinline__ void grid_pencil(const InteractArgs &A, const Tile tile, float4 *Ls, float *Lb,
+                                            double *Lr)
+{
+    constexpr int P2 = NL * NL, L = pencil_len(NL), U = P2 * ((NL + L - 1) / L);
+    constexpr int G = U <= 128 ? 256 / U : 1;  // source groups
+    static_assert(U * G <= 256 && (G == 1 || (G - 1) * U * L <= PENCIL_RED), "pencil layout");
+    const int key = tile.x, tid = threadIdx.x;
+    const double *ttab = A.multi ? A.dtab[NL - 1] + (size_t)(tile.w & SLOT_MASK) * 3 * NL
+                                 : A.tnode_tab + (size_t)tile.w * 3 * NL;
+    const size_t orow = A.out_row ? (size_t)A.out_row[key] : (size_t)tile.w * P2 * NL;
+    const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0
+                                 : (SS ? sqrt(1.4426950408889634 * A.kp.inv_a2) : 1.0);
+    const float eps2s = (float)(lam * lam * A.kp.eps2);
+    const double cx = 0.5 * (ttab[0] + ttab[NL - 1]), cy = 0.5 * (ttab[NL] + ttab[2 * NL - 1]),
+                 cz = 0.5 * (ttab[2 * NL] + ttab[3 * NL - 1]);
+    const bool on = tid < U * G;
+    const int grp = tid / U, unit = tid % U;
+    const int pen = unit % P2, a0 = (unit / P2) * L;
+    const float gy = (float)(lam * (ttab[NL + pen % NL] - cy)), gz = (float)(lam * (ttab[2 * NL + pen / NL] - cz));
+    float gx[L], p1[L], p2[L];
+    double acc[L], accw[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        gx[i] = (float)(lam * (ttab[min(a0 + i, NL - 1)] - cx));
+        p1[i] = 0.0f; p2[i] = 0.0f; acc[i] = 0.0; accw[i] = 0.0;
+    }
+    auto stage = [&](double x, double y, double z, double qa, double qb) {
+        Ls[tid] = make_float4((float)(lam * (x - cx)), (float)(lam * (y - cy)), (float)(lam * (z - cz)),
+                              (float)(lam * qa));
+        if constexpr (SS) Lb[tid] = (float)(lam * qb);
+    };
+    // the staged sources grp, grp + G, ... of n
+    auto consume = [&](int n) {
+        if (!on) return;
+        for (int jb = grp; jb < n; jb += 32 * G) {
+            const int je = min(jb + 32 * G, n);
+            for (int j = jb; j < je; j += G) {
+                const float4 s = Ls[j];
+                const float bf = SS ? Lb[j] : 0.0f;
+                const float dy = gy - s.y, dz = gz - s.z;
+                float dyz = fmaf(dz, dz, dy * dy);
+                if constexpr (KER == KR) dyz += eps2s;
+#pragma unroll
+                for (int i = 0; i < L; i++) {
+                    const float dx = gx[i] - s.x;
+                    pair_pencil<KER, SS>(fmaf(dx, dx, dyz), s.w, bf, p1[i], p2[i]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                acc[i] += (double)p1[i];
+                p1[i] = 0.0f;
+                if constexpr (SS) { accw[i] += (double)p2[i]; p2[i] = 0.0f; }
+            }
+        }
+    };
+    // source clusters (CC)
+    for (int c = A.clu_off[key]; c < A.clu_off[key + 1]; ++c) {
+        const int id = A.clu_id[c];
+        int P = A.P;
+        const double *sx = A.cx, *sy = A.cy, *sz = A.cz, *sq = A.cq, *sw = A.cw;
+        size_t base;
+        if (A.multi) {
+            const int dg = id >> DEG_SHIFT;
+            P = (dg + 1) * (dg + 1) * (dg + 1);
+            sx = A.dcx[dg]; sy = A.dcy[dg]; sz = A.dcz[dg]; sq = A.dcq[dg]; sw = A.dcw[dg];
+            base = (size_t)(id & SLOT_MASK) * P;
+        } else {
+            base = (size_t)id * P;
+        }
+        for (int off = 0; off < P; off += 256) {
+            const int n = min(256, P - off);
+            __syncthreads();
+            if (tid < n) {
+                const size_t g = base + off + tid;
+                stage(sx[g], sy[g], sz[g], sq[g], SS ? sw[g] : 0.0);
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    // source particle ranges (CP)
+    for (int s = A.seg_off[key]; s < A.seg_off[key + 1]; ++s) {
+        const Seg sg = A.segs[s];
+        const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
+        for (int off = 0; off < cnt; off += 256) {
+            const int n = min(256, cnt - off);
+            __syncthreads();
+            if (tid < n) {
+                const int g = sg.x + off + tid;
+                const double q = A.sq[g];
+                double qa = q, qb = 0.0;
+                if constexpr (SS) { qb = A.sw[g]; qa = q * qb; }
+                stage(A.sx[g], A.sy[g], A.sz[g], qa, qb);
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    if constexpr (G > 1) {
+        // the groups' sums, added in group order
+        if (on && grp > 0)
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                Lr[((grp - 1) * U + unit) * L + i] = acc[i];
+                if constexpr (SS) Lr[PENCIL_RED + ((grp - 1) * U + unit) * L + i] = accw[i];
+            }
+        __syncthreads();
+        if (on && grp == 0)
+            for (int g = 1; g < G; g++)
+#pragma unroll
+                for (int i = 0; i < L; i++) {
+                    acc[i] += Lr[((g - 1) * U + unit) * L + i];
+                    if constexpr (SS) accw[i] += Lr[PENCIL_RED + ((g - 1) * U + unit) * L + i];
+                }
+    }
+    if (!on || grp != 0) return;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        if (a0 + i >= NL) continue;
+        const size_t o = orow + a0 + i + (size_t)NL * pen;
+        A.out1[o] = acc[i];
+        if constexpr (SS) A.out2[o] = accw[i];
+    }
+}
+
+template <int KER, bool SS>
+__global__ __launch_bounds__(256) void k_grid_pencil(InteractArgs A)
+{
+    __shared__ float4 Ls[256];          // S' (centred, scaled) and the scaled a charge
+    __shared__ float Lb[SS ? 256 : 1];  // the scaled b charge (subtraction)
+    __shared__ double Lr[(SS ? 2 : 1) * PENCIL_RED];
+    const Tile tile = A.tiles[blockIdx.x];
+    if (tile.z == 0) return;  // padding (block-uniform)
+    const int nl = A.multi ? (tile.w >> DEG_SHIFT) + 1 : A.nl;
+    switch (nl) {
+        case 2: grid_pencil<KER, SS, 2>(A, tile, Ls, Lb, Lr); break;
+        case 3: grid_pencil<KER, SS, 3>(A, tile, Ls, Lb, Lr); break;
+        case 4: grid_pencil<KER, SS, 4>(A, tile, Ls, Lb, Lr); break;
+        case 5: grid_pencil<KER, SS, 5>(A, tile, Ls, Lb, Lr); break;
+        case 6: grid_pencil<KER, SS, 6>(A, tile, Ls, Lb, Lr); break;
+        case 7: grid_pencil<KER, SS, 7>(A, tile, Ls, Lb, Lr); break;
+        case 8: grid_pencil<KER, SS, 8>(A, tile, Ls, Lb, Lr); break;
+        case 9: grid_pencil<KER, SS, 9>(A, tile, Ls, Lb, Lr); break;
+        case 10: grid_pencil<KER, SS, 10>(A, tile, Ls, Lb, Lr); break;
+        case 11: grid_pencil<KER, SS, 11>(A, tile, Ls, Lb, Lr); break;
+        case 12: grid_pencil<KER, SS, 12>(A, tile, Ls, Lb, Lr); break;
+        case 13: grid_pencil<KER, SS, 13>(A, tile, Ls, Lb, Lr); break;
+        default: break;
+    }
+}
+
 // Interaction kernel.  Tiles are laid out as 8 interleaved queues: tile
 // 8*j + x is the j-th tile of queue x, and each queue's tiles read sources
 // that its neighbours in the queue also read.  Workgroups are dispatched
@@ -1942,6 +2144,21 @@ hipError_t launch_interact(const InteractArgs &a, hipStream_t st)
         case KUS: return ss ? hipErrorInvalidValue : plain_dispatch<KUS, false>(a, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_grid_pencil(const InteractArgs &a, hipStream_t st)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    const bool ss = a.singularity == 2;
+    switch (a.kernel) {
+#define L(K, S) k_grid_pencil<K, S><<<dim3(a.ntiles), dim3(256), 0, st>>>(a)
+        case KC: if (ss) L(KC, true); else L(KC, false); break;
+        case KY: if (ss) L(KY, true); else L(KY, false); break;
+        case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;
+#undef L
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)

@@ -1,5 +1,6 @@
 // bt_model.cpp -- host side of the degree model (bt_model.h).
 #include "bt_model.h"
+#include "bt_devmem.h"
 
 #include <hip/hip_runtime_api.h>
 
@@ -21,11 +22,18 @@ constexpr char MAGIC[8] = {'B', 'T', 'D', 'M', 'L', 'P', '0', '2'};
 // Device buffers owned by one call.
 struct DevBufs {
     std::vector<void *> p;
-    ~DevBufs() { for (void *q : p) (void)hipFree(q); }
+    // the call's work on the device is finished before the buffers go back
+    // to the device-memory cache
+    ~DevBufs()
+    {
+        if (p.empty()) return;
+        (void)hipDeviceSynchronize();
+        for (void *q : p) btm::release(q);
+    }
     template <class T> T *get(size_t n)
     {
-        void *q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        void *q = btm::alloc(std::max<size_t>(n, 1) * sizeof(T));
+        if (!q) return nullptr;
         p.push_back(q);
         return static_cast<T *>(q);
     }

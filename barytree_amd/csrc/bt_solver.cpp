@@ -1,6 +1,7 @@
 // bt_solver.cpp -- plan (host setup + upload) and execute (GPU hot path).
 #include "bt_solver.h"
 
+#include "bt_devmem.h"
 #include "bt_devtree.h"
 #include "bt_noalloc.h"
 #include "bt_tree.h"
@@ -12,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <parallel/algorithm>
 
 namespace bt {
 
@@ -127,17 +129,34 @@ void make_queue_tiles(const WorkCSR &w, int tile_t, bool grid, int P, std::vecto
     for (int x = 0; x < NXCD; x++) qlen[x] = (int)lists[x].size();
 }
 
+void make_pencil_tiles(const WorkCSR &w, int P, bool multi, std::vector<btk::Tile> &tiles)
+{
+    std::vector<std::pair<long long, btk::Tile>> ks;
+    for (size_t k = 0; k < w.key.size(); k++) {
+        long long src = 0;
+        for (int s = w.seg_off[k]; s < w.seg_off[k + 1]; s++) src += w.seg_cnt[s];
+        for (int c = w.clu_off[k]; c < w.clu_off[k + 1]; c++) src += cluster_points(w.clu_id[c], P, multi);
+        ks.push_back({(src + 1) * w.tcnt[k], btk::Tile{(int)k, 0, w.tcnt[k], w.key[k]}});
+    }
+    std::stable_sort(ks.begin(), ks.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    tiles.clear();
+    for (const auto &k : ks) tiles.push_back(k.second);
+}
+
 Plan::~Plan() { free_all(); }
 
 void Plan::free_all()
 {
+    // the plan's work is on its stream: once it has finished, the buffers
+    // go back to the device-memory cache
+    if (st_) (void)hipStreamSynchronize(st_);
     free_var();
+    for (void *p : allocs_) btm::release(p);
+    allocs_.clear();
     for (hipEvent_t &e : sym_ev_)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (hipEvent_t &e : ev_)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
-    for (void *p : allocs_) (void)hipFree(p);
-    allocs_.clear();
     if (st_) {
         (void)hipStreamDestroy(st_);
         st_ = nullptr;
@@ -155,8 +174,8 @@ std::string Plan::phase_events()
 
 template <class T> T *Plan::dalloc(size_t n)
 {
-    void *p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    void *p = btm::alloc(std::max<size_t>(n, 1) * sizeof(T));
+    if (!p) return nullptr;
     (alloc_var_ ? var_allocs_ : allocs_).push_back(p);
     return static_cast<T *>(p);
 }
@@ -167,7 +186,8 @@ void Plan::dfree(void *p)
     for (auto *v : {&allocs_, &var_allocs_}) {
         auto it = std::find(v->begin(), v->end(), p);
         if (it != v->end()) {
-            (void)hipFree(p);
+            if (st_) (void)hipStreamSynchronize(st_);
+            btm::release(p);
             v->erase(it);
             return;
         }
@@ -185,6 +205,14 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
 {
     if (!h.empty()) HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st_));
     return "";
+}
+
+bool Plan::pencil_grids() const
+{
+    const Params &p = prm_;
+    return p.precision == PREC_MIXED && p.theta < 0.99 &&
+           (p.kernel == K_COULOMB || p.kernel == K_YUKAWA ||
+            (p.kernel == K_REG_COULOMB && p.singularity == S_SKIPPING));
 }
 
 // Symmetric near field of one system whose targets are its sources (the same
@@ -209,61 +237,106 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
     sym_items_.clear(); sym_ctr_.clear(); sym_red_tiles_.clear();
     sym_red_off_.clear(); sym_red_base_.clear();
     sym_slots_len_ = 0; sym_pairs_ = 0; sym_evals_ = 0;
-    // key of a particle range [beg, beg + cnt)
-    std::vector<std::array<long long, 2>> key_range;  // (beg << 32 | cnt, key index)
-    for (size_t k = 0; k < keys.size(); k++)
-        key_range.push_back({((long long)tt.ibeg[keys[k]] << 32) | (unsigned)tt.numpar[keys[k]], (long long)k});
-    std::sort(key_range.begin(), key_range.end());
-    auto key_of = [&](int beg, int cnt) {
-        const long long r = ((long long)beg << 32) | (unsigned)cnt;
-        auto it = std::lower_bound(key_range.begin(), key_range.end(), std::array<long long, 2>{r, -1});
-        return (it != key_range.end() && (*it)[0] == r) ? (int)(*it)[1] : -1;
-    };
-    // (key, key of the source range) of every near pair whose source range is a key
-    std::vector<long long> present;
-    const long long nk = (long long)keys.size();
-    for (size_t k = 0; symmetric && k < keys.size(); k++)
-        for (int s : segs_of[keys[k]]) {
-            const int ks = key_of(st.ibeg[s], st.numpar[s]);
-            if (ks >= 0) present.push_back((long long)k * nk + ks);
-        }
-    std::sort(present.begin(), present.end());
-    auto has = [&](long long a, long long b) { return std::binary_search(present.begin(), present.end(), a * nk + b); };
-    // kb >= 0: two-sided (the pair of keys k < kb), kb < 0: one-sided
-    struct Cand { int k, kb, s; };
-    std::vector<Cand> cand;
-    for (size_t k = 0; k < keys.size(); k++) {
-        const int t = keys[k];
-        sym_of[t].assign(segs_of[t].size(), 1);
-        for (size_t i = 0; i < segs_of[t].size(); i++) {
-            const int s = segs_of[t][i];
-            const int ks = symmetric ? key_of(st.ibeg[s], st.numpar[s]) : -1;
-            const bool two = ks >= 0 && ks != (int)k && has(ks, (long long)k);
-            if (!two) cand.push_back({(int)k, -1, s});
-            else if ((int)k < ks) cand.push_back({(int)k, ks, s});
+    const int nk = (int)keys.size();
+    // key index of every source node whose particle range is a key's range
+    // (-1 otherwise); keys are disjoint leaves, so a range names one key
+    std::vector<int> key_of_node(st.n, -1);
+    if (symmetric) {
+        std::vector<std::array<long long, 2>> key_range(nk);  // (beg << 32 | cnt, key index)
+        for (int k = 0; k < nk; k++)
+            key_range[k] = {((long long)tt.ibeg[keys[k]] << 32) | (unsigned)tt.numpar[keys[k]], (long long)k};
+        std::sort(key_range.begin(), key_range.end());
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < st.n; s++) {
+            const long long r = ((long long)st.ibeg[s] << 32) | (unsigned)st.numpar[s];
+            auto it = std::lower_bound(key_range.begin(), key_range.end(), std::array<long long, 2>{r, -1});
+            if (it != key_range.end() && (*it)[0] == r) key_of_node[s] = (int)(*it)[1];
         }
     }
-    // work of an item: a two-sided pair costs about twice a one-sided one
+    // per key: the keys among its source ranges, sorted (does ks list k back?)
+    std::vector<std::vector<int>> nbr(nk);
+    if (symmetric) {
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int k = 0; k < nk; k++) {
+            for (int s : segs_of[keys[k]])
+                if (key_of_node[s] >= 0) nbr[k].push_back(key_of_node[s]);
+            std::sort(nbr[k].begin(), nbr[k].end());
+        }
+    }
+    auto has = [&](int a, int b) { return std::binary_search(nbr[a].begin(), nbr[a].end(), b); };
+    // kb >= 0: two-sided (the pair of keys k < kb), kb < 0: one-sided;
+    // candidates in key order, each key's ranges in list order
+    struct Cand { int k, kb, s; };
+    std::vector<int> first(nk + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nk; k++) {
+        const int t = keys[k];
+        sym_of[t].assign(segs_of[t].size(), 1);
+        int c = 0;
+        for (int s : segs_of[t]) {
+            const int ks = key_of_node[s];
+            const bool two = ks >= 0 && ks != k && has(ks, k);
+            c += (!two || k < ks) ? 1 : 0;
+        }
+        first[k + 1] = c;
+    }
+    for (int k = 0; k < nk; k++) first[k + 1] += first[k];
+    std::vector<Cand> cand(first[nk]);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int k = 0; k < nk; k++) {
+        int c = first[k];
+        for (int s : segs_of[keys[k]]) {
+            const int ks = key_of_node[s];
+            const bool two = ks >= 0 && ks != k && has(ks, k);
+            if (!two) cand[c++] = {k, -1, s};
+            else if (k < ks) cand[c++] = {k, ks, s};
+        }
+    }
+    // heaviest first, ties in candidate order (a two-sided pair costs about
+    // twice a one-sided one)
+    const size_t nc = cand.size();
+    {
+        std::vector<std::pair<long long, int>> order(nc);
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < nc; i++) {
+            const Cand &c = cand[i];
+            order[i] = {-(long long)tt.numpar[keys[c.k]] * st.numpar[c.s] * (c.kb >= 0 ? 2 : 1), (int)i};
+        }
+        __gnu_parallel::sort(order.begin(), order.end());
+        std::vector<Cand> sorted(nc);
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < nc; i++) sorted[i] = cand[order[i].second];
+        cand.swap(sorted);
+    }
     auto work = [&](const Cand &c) {
         return (long long)tt.numpar[keys[c.k]] * st.numpar[c.s] * (c.kb >= 0 ? 2 : 1);
     };
-    std::stable_sort(cand.begin(), cand.end(), [&](const Cand &a, const Cand &b) { return work(a) > work(b); });
-    std::vector<std::vector<long long>> lists(keys.size());
     auto far2 = [](double cx, double cy, double cz, const Tree &T, int n) {
         const double dx = std::max(std::fabs(T.xmin[n] - cx), std::fabs(T.xmax[n] - cx));
         const double dy = std::max(std::fabs(T.ymin[n] - cy), std::fabs(T.ymax[n] - cy));
         const double dz = std::max(std::fabs(T.zmin[n] - cz), std::fabs(T.zmax[n] - cz));
         return dx * dx + dy * dy + dz * dz;
     };
-    for (const Cand &c : cand) {
+    // slots in candidate order: A's particles, then B's for a two-sided item
+    std::vector<long long> slot0(nc + 1, 0);
+    for (size_t i = 0; i < nc; i++) {
+        const Cand &c = cand[i];
+        slot0[i + 1] = slot0[i] + tt.numpar[keys[c.k]] + (c.kb >= 0 ? st.numpar[c.s] : 0);
+    }
+    sym_slots_len_ = slot0[nc];
+    sym_items_.resize(nc);
+    sym_ctr_.resize(4 * nc);
+    long long pairs = 0, evals = 0;
+#pragma omp parallel for schedule(static) reduction(+ : pairs, evals)
+    for (size_t i = 0; i < nc; i++) {
+        const Cand &c = cand[i];
         const int t = keys[c.k], s = c.s;
         btk::SymItem it;
         // A: targets, B: sources (symmetric plans: the same particles)
         it.a_beg = sy.toff + tt.ibeg[t]; it.a_cnt = tt.numpar[t];
         it.b_beg = sy.soff + st.ibeg[s]; it.b_cnt = st.numpar[s];
-        it.rslot = sym_slots_len_; sym_slots_len_ += it.a_cnt;
-        it.cslot = -1;
-        if (c.kb >= 0) { it.cslot = sym_slots_len_; sym_slots_len_ += it.b_cnt; }
+        it.rslot = slot0[i];
+        it.cslot = c.kb >= 0 ? slot0[i] + it.a_cnt : -1;
         const double cx = 0.5 * (std::min(tt.xmin[t], st.xmin[s]) + std::max(tt.xmax[t], st.xmax[s]));
         const double cy = 0.5 * (std::min(tt.ymin[t], st.ymin[s]) + std::max(tt.ymax[t], st.ymax[s]));
         const double cz = 0.5 * (std::min(tt.zmin[t], st.zmin[s]) + std::max(tt.zmax[t], st.zmax[s]));
@@ -273,50 +346,64 @@ void Plan::build_sym(const System &sy, const std::vector<int> &keys,
         const bool close = tt.xmin[t] - st.xmax[s] <= g && st.xmin[s] - tt.xmax[t] <= g &&
                            tt.ymin[t] - st.ymax[s] <= g && st.ymin[s] - tt.ymax[t] <= g &&
                            tt.zmin[t] - st.zmax[s] <= g && st.zmin[s] - tt.zmax[t] <= g;
-        sym_ctr_.push_back(cx); sym_ctr_.push_back(cy); sym_ctr_.push_back(cz);
-        sym_ctr_.push_back(close ? -thresh : thresh);
-        lists[c.k].push_back(it.rslot - (sy.toff + tt.ibeg[t]));
-        if (c.kb >= 0) lists[c.kb].push_back(it.cslot - (sy.toff + st.ibeg[s]));
-        sym_pairs_ += (c.kb >= 0 ? 2LL : 1LL) * it.a_cnt * it.b_cnt;
-        sym_evals_ += (long long)it.a_cnt * it.b_cnt;
-        sym_items_.push_back(it);
+        double *ctr = sym_ctr_.data() + 4 * i;
+        ctr[0] = cx; ctr[1] = cy; ctr[2] = cz; ctr[3] = close ? -thresh : thresh;
+        pairs += (c.kb >= 0 ? 2LL : 1LL) * it.a_cnt * it.b_cnt;
+        evals += (long long)it.a_cnt * it.b_cnt;
+        sym_items_[i] = it;
+    }
+    sym_pairs_ = pairs;
+    sym_evals_ = evals;
+    // per key, the bases of its slots in candidate order
+    std::vector<std::vector<long long>> lists(nk);
+    for (size_t i = 0; i < nc; i++) {
+        const Cand &c = cand[i];
+        const btk::SymItem &it = sym_items_[i];
+        lists[c.k].push_back(it.rslot - it.a_beg);
+        if (c.kb >= 0) lists[c.kb].push_back(it.cslot - it.b_beg);
     }
     // Launch order.  Workgroups are dealt round-robin over the 8 XCDs, so
     // block b runs on XCD b % 8.  A block streams its B leaf once per pass
     // over A; items that share a B leaf go to one XCD back to back, so their
     // passes re-read B from that XCD's L2 rather than from HBM.  Groups
-    // (heaviest first) go to the least loaded XCD; the queues are
-    // interleaved (block 8j + x: item j of XCD x) and padded with empty
-    // items.  Slots and sums do not depend on the order.
+    // (heaviest first; ties by first particle of B) go to the least loaded
+    // XCD; the queues are interleaved (block 8j + x: item j of XCD x) and
+    // padded with empty items.  Slots and sums do not depend on the order.
     {
-        std::map<int, std::vector<int>> by_b;
-        for (size_t i = 0; i < cand.size(); i++) by_b[sym_items_[i].b_beg].push_back((int)i);
-        std::vector<std::pair<long long, const std::vector<int> *>> groups;
-        for (const auto &g : by_b) {
+        std::vector<std::pair<int, int>> by_b(nc);  // (first particle of B, item)
+        for (size_t i = 0; i < nc; i++) by_b[i] = {sym_items_[i].b_beg, (int)i};
+        __gnu_parallel::sort(by_b.begin(), by_b.end());
+        struct Group { long long w; size_t beg, end; };
+        std::vector<Group> groups;
+        for (size_t i = 0; i < nc;) {
+            size_t e = i;
             long long w = 0;
-            for (int i : g.second) w += work(cand[i]);
-            groups.push_back({w, &g.second});
+            while (e < nc && by_b[e].first == by_b[i].first) w += work(cand[by_b[e++].second]);
+            groups.push_back({w, i, e});
+            i = e;
         }
-        std::stable_sort(groups.begin(), groups.end(),
-                         [](const auto &a, const auto &b) { return a.first > b.first; });
+        std::stable_sort(groups.begin(), groups.end(), [](const Group &a, const Group &b) { return a.w > b.w; });
         std::vector<std::vector<int>> q(NXCD);
         long long load[NXCD] = {};
-        for (const auto &g : groups) {
+        for (const Group &g : groups) {
             const int x = (int)(std::min_element(load, load + NXCD) - load);
-            load[x] += g.first;
-            q[x].insert(q[x].end(), g.second->begin(), g.second->end());
+            load[x] += g.w;
+            for (size_t e = g.beg; e < g.end; e++) q[x].push_back(by_b[e].second);
         }
         size_t qmax = 0;
         for (const auto &v : q) qmax = std::max(qmax, v.size());
         std::vector<btk::SymItem> items(qmax * NXCD, btk::SymItem{0, 0, 0, 0, 0, 0});
         std::vector<double> ctr(4 * qmax * NXCD, 0.0);
-        for (int x = 0; x < NXCD; x++)
-            for (size_t j = 0; j < q[x].size(); j++) {
-                const size_t slot = j * NXCD + x;
+        for (int x = 0; x < NXCD; x++) {
+            const int len = (int)q[x].size();
+#pragma omp parallel for schedule(static)
+            for (int j = 0; j < len; j++) {
+                const size_t slot = (size_t)j * NXCD + x;
                 items[slot] = sym_items_[q[x][j]];
                 std::copy(sym_ctr_.begin() + 4 * (size_t)q[x][j], sym_ctr_.begin() + 4 * (size_t)q[x][j] + 4,
                           ctr.begin() + 4 * slot);
             }
+        }
         sym_items_.swap(items);
         sym_ctr_.swap(ctr);
     }
@@ -531,6 +618,7 @@ void Plan::build_work()
     for (int n = 0; n < ntnodes_; n++) node_rows_off_[n + 1] += node_rows_off_[n];
     make_queue_tiles(near_, tile_targets_, false, P, near_tiles_, near_qlen_);
     make_queue_tiles(grid_, GRID_TILE, true, P, grid_tiles_, grid_qlen_);
+    make_pencil_tiles(grid_, P, false, pencil_tiles_);
 
     // upward pass (source trees, PC and CC): non-leaf nodes per level,
     // deepest first, every system in the same launch
@@ -682,6 +770,10 @@ std::string Plan::validate() const
         }
     }
     if (!(e = chk_csr(grid_, grid_tiles_, true)).empty()) return "far " + e;
+    if (pencil_tiles_.size() != grid_.key.size()) return "pencil tiles";
+    for (const btk::Tile &t : pencil_tiles_)
+        if (t.x < 0 || t.x >= (int)grid_.key.size() || t.y != 0 || t.z != P || t.w != grid_.key[t.x])
+            return "pencil tile";
     if (grid_row_.size() != grid_.key.size() || (int)node_rows_off_.size() != ntnodes_ + 1 ||
         node_rows_off_[ntnodes_] != (int)grid_.key.size())
         return "grid rows";
@@ -1225,6 +1317,7 @@ std::string Plan::finish_create(const PlanInput &in, bool host_only, double t2)
     if (!grid_tiles_.empty()) {
         if (!(r = up_csr(grid_, &d_gseg_off_, &d_gsegs_, &d_gclu_off_, &d_gclu_id_)).empty()) return r;
         if (!(r = up(&d_grid_tiles_, grid_tiles_)).empty()) return r;
+        if (!(r = up(&d_pencil_tiles_, pencil_tiles_)).empty()) return r;
     }
     if (need_tgt_clusters) {
         if (!(r = up(&d_grid_row_, grid_row_)).empty()) return r;
@@ -1426,7 +1519,12 @@ std::string Plan::execute(bool timed)
             a.out1 = d_Qrows_; a.out2 = d_Wrows_; a.out_row = d_grid_row_;
             std::copy(grid_qlen_, grid_qlen_ + NXCD, a.qlen);
             a.cursor = d_cursor_ + NXCD;
-            HIPCHK(btk::launch_interact(a, st_));
+            if (pencil_grids()) {
+                a.ntiles = (int)pencil_tiles_.size(); a.tiles = d_pencil_tiles_;
+                HIPCHK(btk::launch_grid_pencil(a, st_));
+            } else {
+                HIPCHK(btk::launch_interact(a, st_));
+            }
         }
         // every target grid: the sum of its rows, in row order (zero without rows)
         HIPCHK(btk::launch_sum_rows(ntnodes_, p.P, d_node_rows_off_, d_grid_row_, d_Qrows_, d_Wrows_, d_Qt_,

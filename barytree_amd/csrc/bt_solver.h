@@ -22,7 +22,8 @@ namespace bt {
 constexpr int DOWN_TILE = 1024;  // targets per block of the leaf downpass
 constexpr int GRID_TILE = 512;   // grid points per block of grid-target tiles (2 per thread)
 constexpr int PART_MAX = 16384;  // particles per block of a source slot's charges
-constexpr long long MAX_KEY_SOURCES = 1 << 18;  // sources per grid key (heavier keys are split)
+constexpr long long MAX_KEY_SOURCES = 1 << 14;  // source points per grid key (heavier keys are split:
+                                                 // blocks enough to fill the GPU)
 constexpr int SMALL_TARGETS = 1 << 18;  // single-system plans below this many targets: 1 target per thread
 
 struct Timings {
@@ -88,6 +89,11 @@ struct DegreeWork {
 // (degree+1)^3 points each; else P.
 void make_queue_tiles(const WorkCSR &w, int tile_targets, bool grid, int P,
                       std::vector<btk::Tile> &tiles, int *qlen, bool multi = false);
+
+// One tile {key, 0, points, node} per grid key for the tensor-grid launch
+// (launch_grid_pencil: one block per grid), heaviest first.  With multi,
+// cluster ids carry their degree as in make_queue_tiles.
+void make_pencil_tiles(const WorkCSR &w, int P, bool multi, std::vector<btk::Tile> &tiles);
 
 // Splits every key whose sources (particles plus cluster points) exceed
 // max_src into consecutive chunks of its particle ranges and clusters; the
@@ -201,6 +207,10 @@ private:
     std::string down_fixed();    // downpass at the plan's degree
     bool sym_ready();  // items_run(), and the slots allocated (at the first execution that uses them)
     std::string phase_events();
+    // the tensor-grid launch serves this execution's grid targets: mixed
+    // precision, acceptance-separated sources (theta < 0.99), Coulomb or
+    // Yukawa, or regularized Coulomb with skipping
+    bool pencil_grids() const;
     std::string near_sym();
     std::string validate() const;
     void enumerate_pairs();
@@ -296,6 +306,8 @@ private:
     // one far launch (every target slot) for all degrees
     WorkCSR vnear_, vfar_;
     std::vector<btk::Tile> vnear_tiles_, vfar_tiles_;
+    std::vector<btk::Tile> vpencil_tiles_;  // vfar_ keys, one tile each
+    btk::Tile *d_vp_tiles_ = nullptr;
     int vnear_qlen_[btk::TILE_QUEUES] = {}, vfar_qlen_[btk::TILE_QUEUES] = {};
     std::vector<long long> vfar_row_;
     long long vfar_rows_len_ = 0;
@@ -324,6 +336,9 @@ private:
     double *d_up_Qe_ = nullptr, *d_up_We_ = nullptr;
     int *d_down_nodes_ = nullptr;
     btk::Tile *d_near_tiles_ = nullptr, *d_grid_tiles_ = nullptr, *d_leaf_tiles_ = nullptr;
+    // grid_ keys, one tile each (the tensor-grid launch of mixed executions)
+    std::vector<btk::Tile> pencil_tiles_;
+    btk::Tile *d_pencil_tiles_ = nullptr;
     int *d_nseg_off_ = nullptr, *d_nclu_off_ = nullptr, *d_nclu_id_ = nullptr;
     btk::Seg *d_nsegs_ = nullptr;
     int *d_gseg_off_ = nullptr, *d_gclu_off_ = nullptr, *d_gclu_id_ = nullptr;

@@ -21,6 +21,7 @@
 #include <numeric>
 
 #include "bt_learn.h"
+#include "bt_devmem.h"
 #include "bt_solver.h"
 
 namespace bt {
@@ -106,13 +107,16 @@ void Plan::pair_features(std::vector<float> &f) const
 
 void Plan::free_var()
 {
-    for (void *p : var_allocs_) (void)hipFree(p);
+    if (st_ && !var_allocs_.empty()) (void)hipStreamSynchronize(st_);
+    for (void *p : var_allocs_) btm::release(p);
     var_allocs_.clear();
     dw_.clear();
     vnear_ = WorkCSR();
     vfar_ = WorkCSR();
     vnear_tiles_.clear();
     vfar_tiles_.clear();
+    vpencil_tiles_.clear();
+    d_vp_tiles_ = nullptr;
     vfar_row_.clear();
     vfar_rows_len_ = 0;
     var_ = false;
@@ -282,6 +286,7 @@ std::string Plan::set_pair_degrees(const int *deg)
         }
     }
     make_queue_tiles(vfar_, GRID_TILE, true, 0, vfar_tiles_, vfar_qlen_, true);
+    make_pencil_tiles(vfar_, 0, true, vpencil_tiles_);
 
     // degrees up to the plan's own go through its degree-D grids (exact in
     // exact arithmetic; see bt_degree.hip)
@@ -377,6 +382,7 @@ std::string Plan::set_pair_degrees(const int *deg)
     if (!(r = up(&d_vn_tiles_, vnear_tiles_)).empty()) return fail(r);
     if (!(r = up_csr(vfar_, &d_vf_seg_off_, &d_vf_segs_, &d_vf_clu_off_, &d_vf_clu_id_)).empty()) return fail(r);
     if (!(r = up(&d_vf_tiles_, vfar_tiles_)).empty()) return fail(r);
+    if (!vpencil_tiles_.empty() && !(r = up(&d_vp_tiles_, vpencil_tiles_)).empty()) return fail(r);
     if (!(r = up(&d_vf_row_, vfar_row_)).empty()) return fail(r);
     d_vf_R_ = dalloc<double>(vfar_rows_len_);
     if (ss) d_vf_RW_ = dalloc<double>(vfar_rows_len_);
@@ -467,6 +473,13 @@ std::string Plan::validate_var() const
     std::string e = chk(vnear_, vnear_tiles_, vnear_qlen_, false, tile_targets_);
     if (!e.empty()) return "near " + e;
     if ((e = chk(vfar_, vfar_tiles_, vfar_qlen_, true, GRID_TILE)).empty() == false) return "far " + e;
+    if (vpencil_tiles_.size() != vfar_.key.size()) return "pencil tiles";
+    for (const btk::Tile &t : vpencil_tiles_) {
+        const int d = t.w >> btk::DEG_SHIFT;
+        if (t.x < 0 || t.x >= (int)vfar_.key.size() || t.y != 0 || t.w != vfar_.key[t.x] || d < 1 ||
+            d > btk::MAX_DEGREE || t.z != (d + 1) * (d + 1) * (d + 1))
+            return "pencil tile";
+    }
     if (vfar_row_.size() != vfar_.key.size()) return "far rows";
     for (size_t k = 0; k < vfar_.key.size(); k++)
         if (vfar_row_[k] < 0 || vfar_row_[k] + vfar_.tcnt[k] > vfar_rows_len_) return "far row";
@@ -677,7 +690,12 @@ std::string Plan::execute_var(bool timed)
         a.out1 = d_vf_R_; a.out2 = d_vf_RW_; a.out_row = d_vf_row_;
         std::copy(vfar_qlen_, vfar_qlen_ + NQ, a.qlen);
         a.cursor = d_cursor_ + NQ;
-        HIPCHK(btk::launch_interact(a, st_));
+        if (pencil_grids()) {
+            a.ntiles = (int)vpencil_tiles_.size(); a.tiles = d_vp_tiles_;
+            HIPCHK(btk::launch_grid_pencil(a, st_));
+        } else {
+            HIPCHK(btk::launch_interact(a, st_));
+        }
         for (const DegreeWork &w : dw_) {
             const int T = w.num_tslots();
             if (!w.used || T == 0) continue;

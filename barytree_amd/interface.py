@@ -149,12 +149,31 @@ def lib():
         l.BaryTreeModel_Predict.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
         l.BaryTreePlan_UseDegreeModel.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+        l.BaryTree_EmptyDeviceCache.restype = ctypes.c_ulonglong
+        l.BaryTree_CachedDeviceBytes.restype = ctypes.c_ulonglong
+        l.BaryTree_SetDeviceCacheLimit.restype = ctypes.c_int
+        l.BaryTree_SetDeviceCacheLimit.argtypes = [ctypes.c_ulonglong]
         _lib = l
     return _lib
 
 
 def last_error():
     return lib().BaryTree_GetLastError().decode()
+
+
+def empty_device_cache():
+    """Frees the device blocks that destroyed plans left in the library's
+    cache; returns the bytes freed."""
+    return int(lib().BaryTree_EmptyDeviceCache())
+
+
+def set_device_cache_limit(nbytes):
+    """Most bytes of released plan buffers the library keeps per GPU."""
+    lib().BaryTree_SetDeviceCacheLimit(ctypes.c_ulonglong(int(nbytes)))
+
+
+def cached_device_bytes():
+    return int(lib().BaryTree_CachedDeviceBytes())
 
 
 def _arr(a):

@@ -179,6 +179,20 @@ const char *BaryTree_GetLastError(void);
 #define BARYTREE_BETA_LEAVES_GPU 1
 int BaryTree_SetBetaLeafSizes(int which);
 
+/* Device memory cache.  The buffers of a plan (and of a one-call solve's
+ * internal plan) go back to a per-GPU cache when it is destroyed, and later
+ * plans take blocks of a close size from it instead of allocating: a
+ * one-call solve at N = 16M otherwise spends milliseconds unmapping and
+ * mapping several GB per call.  At most the limit (default 32 GiB) stays
+ * cached per GPU.  BaryTree_EmptyDeviceCache frees every cached block and
+ * returns the bytes freed; BaryTree_SetDeviceCacheLimit sets the limit (0:
+ * nothing is kept) and frees what lies beyond it; BaryTree_CachedDeviceBytes
+ * reports the current GPU's cached bytes.  No counterpart in the reference
+ * (its GPU build allocates through OpenACC data regions per call). */
+unsigned long long BaryTree_EmptyDeviceCache(void);
+int BaryTree_SetDeviceCacheLimit(unsigned long long bytes);
+unsigned long long BaryTree_CachedDeviceBytes(void);
+
 /* Statistics of the last successful BaryTreeInterface/BaryTree_Solve call on
  * this thread.  Returns 0, or -1 if there was none. */
 int BaryTree_GetLastStats(BaryTreeStats *stats);

@@ -2,7 +2,8 @@
 //
 // The five tree kernels of bt_tree.hip are emulated here with plain loops of
 // the same semantics, and the few HIP runtime calls the orchestration makes
-// (allocation, copies, synchronisation) act on host memory.  emul_compare()
+// (allocation, copies, synchronisation; the device-memory cache of
+// bt_devmem.cpp on top of them) act on host memory.  emul_compare()
 // builds the same trees with the host builder (bt_host.cpp) and with the
 // device orchestration, and compares them node by node: ranges, boxes,
 // centres, radii, children, levels, leaves, and the particles of every leaf.
@@ -37,6 +38,13 @@ hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipSt
     return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize() { return hipSuccess; }
+hipError_t hipGetDevice(int *d)
+{
+    *d = 0;
+    return hipSuccess;
+}
+hipError_t hipGetLastError() { return hipSuccess; }
 const char *hipGetErrorString(hipError_t) { return "emulated error"; }
 
 // ---- the kernels ---------------------------------------------------------

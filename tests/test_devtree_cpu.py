@@ -22,7 +22,7 @@ def emul(tmp_path_factory):
     cmd = ["g++", "-O1", "-shared", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
            "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include", "-fopenmp",
            os.path.join(ROOT, "tests", "devtree_emul.cpp"), os.path.join(SRC, "bt_devtree.cpp"),
-           os.path.join(SRC, "bt_host.cpp"), "-o", out]
+           os.path.join(SRC, "bt_host.cpp"), os.path.join(SRC, "bt_devmem.cpp"), "-o", out]
     subprocess.run(cmd, check=True, capture_output=True)
     lib = ctypes.CDLL(out)
     lib.emul_compare.restype = ctypes.c_int

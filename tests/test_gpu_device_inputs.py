@@ -68,3 +68,29 @@ def test_device_solve_errors(bt):
     with pytest.raises(RuntimeError, match="weights"):
         bt.solve_device([dev(a) for a in t], [dev(a) for a in src[:4]] + [None], out, 1, [0.5], 2, 1, 1, 0.7, 4,
                         100, 100)
+
+
+def test_device_memory_cache(bt):
+    """Destroyed plans leave their buffers in the library's per-GPU cache;
+    the next solve of the same size takes them back (results unchanged), a
+    limit of 0 keeps nothing, and emptying the cache frees everything."""
+    print("case device-memory cache, N=200k PC Coulomb", flush=True)
+    t, s = uniform_cube(200_000, 200_000, seed=91)
+    args = (1, [0.5], 1, 1, 1, 0.7, 6, 2000, 2000)
+    bt.empty_device_cache()
+    first = bt.solve(t, s, *args, precision=bt.PRECISION_MIXED)
+    cached = bt.cached_device_bytes()
+    assert cached > 8 * 200_000 * 4
+    second = bt.solve(t, s, *args, precision=bt.PRECISION_MIXED)
+    assert np.array_equal(first, second)
+    assert bt.cached_device_bytes() == cached
+    bt.set_device_cache_limit(0)
+    assert bt.cached_device_bytes() == 0
+    third = bt.solve(t, s, *args, precision=bt.PRECISION_MIXED)
+    assert np.array_equal(first, third)
+    assert bt.cached_device_bytes() == 0
+    bt.set_device_cache_limit(32 << 30)
+    bt.solve(t, s, *args)
+    assert bt.cached_device_bytes() > 0
+    assert bt.empty_device_cache() > 0
+    assert bt.cached_device_bytes() == 0
