@@ -72,21 +72,32 @@ void fill_stats(const bt::Plan &p, BaryTreeStats *s)
 
 // Degrees from the model; a pair whose features lie outside the model's
 // training range keeps the plan's fixed degree (the reference's method).
-std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)
+// The model's degree for every far-field pair of the plan (the fixed degree
+// where the pair's features lie outside the model's training range).
+std::string model_degrees(const bt::Plan &plan, const bt::DegreeModel &m, float delta, std::vector<int> &deg,
+                          long long &fallback)
 {
     std::vector<float> f;
     plan.pair_features(f);
     const int n = (int)plan.far_pairs_list().size();
-    std::vector<int> deg(n);
+    deg.assign(n, 0);
     std::string e = m.predict(n, f.data(), delta, deg.data(), nullptr);
     if (!e.empty()) return e;
-    long long fallback = 0;
+    fallback = 0;
     for (int i = 0; i < n; i++)
         if (m.outside_range(f.data() + (size_t)i * btl::MLP_IN)) {
             deg[i] = plan.params().degree;
             fallback++;
         }
-    e = plan.set_pair_degrees(deg.data());
+    return "";
+}
+
+std::string use_model(bt::Plan &plan, const bt::DegreeModel &m, float delta)
+{
+    std::vector<int> deg;
+    long long fallback = 0;
+    std::string e = model_degrees(plan, m, delta, deg, fallback);
+    if (e.empty()) e = plan.set_pair_degrees(deg.data());
     if (e.empty()) plan.set_model_fallback_pairs(fallback);
     return e;
 }
@@ -1100,7 +1111,17 @@ static int solve_learned(int numTargets, int numSources, const double *targetX,
     if (rc) return rc;
     std::string e;
     const auto t_created = std::chrono::steady_clock::now();
-    if (model) e = use_model(plan, model->m, delta);
+    if (model) {
+        // the degrees (the model's inference runs on the GPU), then the near
+        // field's item launch -- most of the GPU time -- while the host
+        // builds the per-degree work
+        std::vector<int> deg;
+        long long fallback = 0;
+        e = model_degrees(plan, model->m, delta, deg, fallback);
+        if (e.empty()) e = plan.start_items();
+        if (e.empty()) e = plan.set_pair_degrees(deg.data());
+        if (e.empty()) plan.set_model_fallback_pairs(fallback);
+    }
     const auto t_model = std::chrono::steady_clock::now();
     if (e.empty()) e = plan.execute(true);
     const auto t_exec = std::chrono::steady_clock::now();

@@ -128,6 +128,11 @@ public:
     // Frees the plan's device memory, events and stream (the destructor does
     // the same; a released plan cannot execute again).
     void release() { free_all(); }
+    // Enqueues the next execution's near-field item launch now (when the
+    // items run at the plan's current precision), so that host work before
+    // execute() -- a degree model's -- overlaps it; execute() then goes on
+    // from it.  Precision and symmetry must not change in between.
+    std::string start_items();
     // FP64 direct sum on the GPU for caller-order target indices `idx`
     // (each target against the sources of its own system).
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
@@ -354,6 +359,7 @@ private:
     hipEvent_t sym_ev_[2] = {nullptr, nullptr};  // around the symmetric launch
     hipEvent_t ev_[6] = {};                      // phase boundaries of execute()
     bool sym_ran_ = false;                       // it ran in this execution
+    bool items_started_ = false;                 // enqueued ahead of execute() (start_items)
     long long *d_grid_row_ = nullptr;
     int *d_node_rows_off_ = nullptr;
     double *d_Qrows_ = nullptr, *d_Wrows_ = nullptr;
