@@ -898,7 +898,12 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // add exactly zero, the reference's skip).  One-sided items run the same
 // loop without the sources' sums.
 // ---------------------------------------------------------------------
-constexpr int SYM_TPT = 4, SYM_G = 8, SYM_NW = 4;
+constexpr int SYM_G = 8, SYM_NW = 4;
+// targets per lane: 6 for Coulomb with subtraction (the per-step scan and
+// loop cost spread over more pairs; 155 VGPRs, three waves per SIMD: about
+// 1% faster at N = 16M than 4 targets at four waves), else 4
+constexpr int sym_tpt(int ker, bool ss) { return ker == KC && ss ? 6 : 4; }
+constexpr int sym_wpe(int ker, bool ss) { return sym_tpt(ker, ss) == 6 ? 3 : 4; }
 
 // Sums over groups of G consecutive lanes (G = 2, 4, 8 or 16) by a
 // row_shr scan inside each DPP row of 16 lanes: after the shifts 1, 2, ..,
@@ -2177,7 +2182,7 @@ hipError_t launch_near_sym(const SymArgs &a, hipStream_t st)
         return hipGetLastError();
     }
     switch (a.kernel) {
-#define L(K, S) k_near_sym<K, S, SYM_TPT, SYM_G, SYM_NW, 4><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
+#define L(K, S) k_near_sym<K, S, sym_tpt(K, S), SYM_G, SYM_NW, sym_wpe(K, S)><<<dim3(a.nitems), dim3(64 * SYM_NW), 0, st>>>(a)
         case KC: if (ss) L(KC, true); else L(KC, false); break;
         case KY: if (ss) L(KY, true); else L(KY, false); break;
         case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;
