@@ -632,12 +632,11 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
 // sources: grid_scaled).  The NL^3 points of a target node's grid are
 // (x_a, y_b, z_c), so r^2 = (x_a - s_x)^2 + [(y_b - s_y)^2 + (z_c - s_z)^2]
 // and the bracket is shared by the NL points of the pencil (b, c).  A unit
-// owns the points a in [g L, (g+1) L) of pencil u % NL^2, g = u / NL^2, with
-// L the least length for which U = NL^2 ceil(NL/L) units fit the 256
-// threads; small grids (U <= 128) split the sources over 256/U groups of U
-// threads instead, whose sums are added in group order at the end.  Per
-// staged source a unit forms the bracket once (two subtractions, a multiply
-// and an fma), then per point one subtraction and one fma before the kernel
+// (thread) owns a whole pencil, so U = NL^2 units; the 256 threads form
+// G = 256/U groups of U units that take every G-th staged source, and the
+// groups' sums are added in group order at the end.  Per staged source a
+// unit forms the bracket once (two subtractions, a multiply and an fma),
+// then per point one subtraction and one fma before the kernel
 // (against four FP64 instructions and a conversion per pair in the centred
 // expansion of interact_tile).  Coordinates are taken relative to the node's
 // centre and scaled by lam as in that expansion, then rounded to FP32; a
@@ -649,12 +648,6 @@ __device__ __forceinline__ void interact_tile(const InteractArgs &A, const Tile 
 // One block per tile {key, 0, NL^3, node}; NL from the node field's degree
 // (several degrees) or the launch's.
 // ---------------------------------------------------------------------
-constexpr int pencil_len(int nl)
-{
-    for (int L = 1; L < nl; L++)
-        if (nl * nl * ((nl + L - 1) / L) <= 256) return L;
-    return nl;
-}
 
 // r2s: the scaled r'^2 (regularized kernel: r'^2 + e'^2)
 template <int KER, bool SS>
@@ -674,15 +667,15 @@ __device__ __forceinline__ void pair_pencil(float r2s, float af, float bf, float
     }
 }
 
-constexpr int PENCIL_RED = 256;  // doubles per part of the group reduction
+constexpr int PENCIL_RED = 1536;  // doubles per part of the group reduction: max (G-1) U NL
 
 template <int KER, bool SS, int NL>
 __device__ __forceinline__ void grid_pencil(const InteractArgs &A, const Tile tile, float4 *Ls, float *Lb,
                                             double *Lr)
 {
-    constexpr int P2 = NL * NL, L = pencil_len(NL), U = P2 * ((NL + L - 1) / L);
-    constexpr int G = U <= 128 ? 256 / U : 1;  // source groups
-    static_assert(U * G <= 256 && (G == 1 || (G - 1) * U * L <= PENCIL_RED), "pencil layout");
+    constexpr int P2 = NL * NL, L = NL, U = P2;
+    constexpr int G = 256 / U;  // source groups
+    static_assert(U * G <= 256 && G >= 1 && (G - 1) * U * L <= PENCIL_RED, "pencil layout");
     const int key = tile.x, tid = threadIdx.x;
     const double *ttab = A.multi ? A.dtab[NL - 1] + (size_t)(tile.w & SLOT_MASK) * 3 * NL
                                  : A.tnode_tab + (size_t)tile.w * 3 * NL;

@@ -1,7 +1,7 @@
 """The far field of mixed executions on target grids as tensor grids
 (k_grid_pencil): every grid size the degrees 1..12 give (2..13 points per
-dimension: pencil segments of one to all points, and source groups for the
-grids of at most 5 points per dimension), in one launch when pairs carry
+dimension: one thread per pencil, 1 to 64 groups of threads sharing the
+staged sources), in one launch when pairs carry
 their own degrees, for every kernel and singularity the mixed path serves,
 in cluster-particle and cluster-cluster plans.  The mixed result is checked
 against the FP64 execution of the same plan (the FP32 pairs' bar, 1e-6)."""
