@@ -892,10 +892,11 @@ __global__ __launch_bounds__(TPB) void k_interact(InteractArgs A)
 // loop without the sources' sums.
 // ---------------------------------------------------------------------
 constexpr int SYM_G = 8, SYM_NW = 4;
-// targets per lane: 6 for Coulomb with subtraction (the per-step scan and
-// loop cost spread over more pairs; 155 VGPRs, three waves per SIMD: about
-// 1% faster at N = 16M than 4 targets at four waves), else 4
-constexpr int sym_tpt(int ker, bool ss) { return ker == KC && ss ? 6 : 4; }
+// targets per lane: 6 for Coulomb and Yukawa (the per-step scan and loop
+// cost spread over more pairs; up to 155 VGPRs, three waves per SIMD: 1-3%
+// faster than 4 targets at four waves), 4 for the regularized kernel (whose
+// one-sided items, most of its pairs at 4M Plummer, ran 2.5% slower with 6)
+constexpr int sym_tpt(int ker, bool) { return ker == KR ? 4 : 6; }
 constexpr int sym_wpe(int ker, bool ss) { return sym_tpt(ker, ss) == 6 ? 3 : 4; }
 
 // Sums over groups of G consecutive lanes (G = 2, 4, 8 or 16) by a
