@@ -17,18 +17,27 @@
 #include <tuple>
 #include <vector>
 
+#include "../barytree_amd/csrc/bt_devmem.h"
 #include "../barytree_amd/csrc/bt_devtree.h"
 #include "../barytree_amd/csrc/bt_host.h"
 #include "../barytree_amd/csrc/bt_tree.h"
 
 // ---- the HIP runtime calls of the orchestration, on host memory ----------
+static long long g_mallocs = 0, g_frees = 0;
+static size_t g_oom_above = (size_t)-1;  // emulated device size limit (emul_devmem)
 hipError_t hipMalloc(void **p, size_t n)
 {
+    if (n > g_oom_above) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
     *p = std::malloc(n ? n : 1);
+    if (*p) g_mallocs++;
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipFree(void *p)
 {
+    if (p) g_frees++;
     std::free(p);
     return hipSuccess;
 }
@@ -215,5 +224,55 @@ extern "C" int emul_compare(int n, const double *x, const double *y, const doubl
         }
     }
     *nnodes_out = total;
+    return 0;
+}
+
+// The device-memory cache (bt_devmem.cpp) on the emulated allocator: reuse
+// of a released block of a close size, no reuse of a much larger one, the
+// limit, emptying, and the retry after an out-of-memory allocation.
+// Returns 0, or the number of the failed check.
+extern "C" int emul_devmem()
+{
+    btm::set_cache_limit(size_t(32) << 30);
+    btm::empty_cache();
+    const long long m0 = g_mallocs;
+    void *a = btm::alloc(1000);
+    if (!a || g_mallocs != m0 + 1) return 1;
+    btm::release(a);
+    if (btm::cached_bytes() != 1024) return 2;               // 512-byte steps below 1 MiB
+    void *b = btm::alloc(900);                                // same rounded size: reused
+    if (b != a || g_mallocs != m0 + 1 || btm::cached_bytes() != 0) return 3;
+    btm::release(b);
+    void *c = btm::alloc(100000);                             // far larger: a new block
+    if (c == a || g_mallocs != m0 + 2) return 4;
+    btm::release(c);
+    void *d = btm::alloc(3u << 20);                           // 2 MiB steps above 1 MiB: 4 MiB
+    btm::release(d);
+    void *e = btm::alloc((5u << 20) / 2);                     // 2.5 MiB rounds to 4 MiB: reused
+    if (e != d) return 5;
+    void *f = btm::alloc(1u << 20);                           // 2 MiB: a 4 MiB block is within 2x
+    btm::release(e);
+    btm::release(f);
+    const size_t cached = btm::cached_bytes();
+    if (cached != 1024 + 100352 + (size_t(4) << 20) + (size_t(2) << 20)) return 6;
+    const long long f0 = g_frees;
+    btm::set_cache_limit(size_t(1) << 21);                    // frees the largest blocks first
+    if (btm::cached_bytes() > (size_t(1) << 21) || g_frees <= f0) return 7;
+    void *g = btm::alloc(size_t(3) << 21);                    // beyond the limit on release: freed
+    const long long f1 = g_frees;
+    btm::release(g);
+    if (g_frees != f1 + 1) return 8;
+    btm::set_cache_limit(size_t(32) << 30);
+    void *h = btm::alloc(1u << 22);
+    btm::release(h);
+    g_oom_above = (size_t)1 << 23;                            // the next 12 MiB fails at first
+    void *i = btm::alloc(size_t(12) << 20);
+    if (i) return 9;                                          // still too large after emptying
+    if (btm::cached_bytes() != 0) return 10;                  // the cache was emptied for it
+    g_oom_above = (size_t)-1;
+    if (btm::empty_cache() != 0) return 11;
+    void *j = btm::alloc(4096);
+    btm::release(j);
+    if (btm::empty_cache() != 4096) return 12;
     return 0;
 }

@@ -128,3 +128,12 @@ def test_many_systems(emul, leaves_only):
     roots = np.concatenate([[0], np.cumsum(sizes)])
     rc, nn = compare(emul, x, y, z, roots, leaves, leaves_only)
     assert rc == 0 and nn >= len(sizes)
+
+
+def test_device_memory_cache_on_the_emulated_allocator(emul):
+    """bt_devmem.cpp: released blocks are reused for close sizes only,
+    the limit trims the largest cached blocks first and frees a release
+    beyond it, an allocation that runs out of memory empties the cache and
+    tries again, and emptying reports the bytes freed."""
+    emul.emul_devmem.restype = ctypes.c_int
+    assert emul.emul_devmem() == 0
