@@ -920,12 +920,13 @@ __device__ __forceinline__ float group_sum(float v)
 // for subtraction; skipping: lam = kappa log2(e) for Yukawa, 1 otherwise):
 // the a-part factor fa and the b-part factor fb (subtraction only).  SKIP:
 // a coincident pair (r^2 = 0) gives zero, the reference's skip (the
-// regularized kernel has none).
-template <int KER, bool SS, bool SKIP>
+// regularized kernel has none).  EPS_IN: r2 of the regularized kernel
+// already holds e'^2.
+template <int KER, bool SS, bool SKIP, bool EPS_IN = false>
 __device__ __forceinline__ void sym_pair(float r2, float eps2s, float &fa, float &fb)
 {
     if constexpr (KER == KR) {
-        fa = rsq32(r2 + eps2s);
+        fa = rsq32(EPS_IN ? r2 : r2 + eps2s);
         if constexpr (SS) fb = exp2_32(-r2) * fa;
         return;
     }
@@ -961,10 +962,12 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_near_sym(SymArgs A)  // at lea
     const double lam = KER == KY ? 1.4426950408889634 * A.kp.p0
                                  : (SS ? sqrt(1.4426950408889634 * A.kp.inv_a2) : 1.0);
     const float eps2s = (float)(lam * lam * A.kp.eps2);
-    // the regularized kernel's sensitivity is to r^2 + eps^2
+    // the regularized kernel's sensitivity is to r^2 + eps^2, which its
+    // expanded r'^2 holds (e'^2 folded into |T'|^2: no FP32 add per pair)
+    constexpr bool EPS_IN = KER == KR;
     const double th = lam * lam * (fabs(ctr.w) - (KER == KR ? A.kp.eps2 : 0.0));
     const bool check = ctr.w < 0.0 && th > 0.0;
-    const float thf = fmaxf((float)th, 1.17549435e-38f);
+    const float thf = fmaxf((float)(EPS_IN ? lam * lam * fabs(ctr.w) : th), 1.17549435e-38f);
     constexpr int TPASS = NW * G * TPT;
     // a and b of a particle as a source: subtraction (lam q w, lam w),
     // skipping (lam q, 0)
@@ -990,7 +993,7 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_near_sym(SymArgs A)  // at lea
             const bool v = li < it.a_cnt;
             const int g = it.a_beg + (v ? li : 0);
             const double x = lam * (A.tx[g] - ctr.x), y = lam * (A.ty[g] - ctr.y), z = lam * (A.tz[g] - ctr.z);
-            T2[k] = fma(z, z, fma(y, y, x * x));
+            T2[k] = fma(z, z, fma(y, y, x * x)) + (EPS_IN ? lam * lam * A.kp.eps2 : 0.0);
             Tx[k] = -2.0 * x; Ty[k] = -2.0 * y; Tz[k] = -2.0 * z;
             Tq[k] = SS ? A.tq[g] : 0.0;
             if constexpr (TWO) {
@@ -1047,7 +1050,7 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_near_sym(SymArgs A)  // at lea
                             r2 = (float)fma(dz, dz, fma(dy, dy, dx * dx));
                         }
                         float fa, fb = 0.0f;
-                        sym_pair<KER, SS, EXACT && KER != KR>(r2, eps2s, fa, fb);
+                        sym_pair<KER, SS, EXACT && KER != KR, EPS_IN && !EXACT>(r2, eps2s, fa, fb);
                         p1[k] = fmaf(f.a, fa, p1[k]);
                         if constexpr (TWO) c1 = fmaf(ai[k], fa, c1);
                         if constexpr (SS) {
