@@ -17,10 +17,19 @@ struct Device {
     size_t cached = 0;
 };
 
-std::mutex mu;
-std::map<int, Device> devices;
-std::unordered_map<void *, std::pair<size_t, int>> live;  // block -> (size, device)
-size_t limit = size_t(32) << 30;
+// Never destroyed: plans released while the process exits (after static
+// destructors have run) still find the cache.
+struct State {
+    std::mutex mu;
+    std::map<int, Device> devices;
+    std::unordered_map<void *, std::pair<size_t, int>> live;  // block -> (size, device)
+    size_t limit = size_t(32) << 30;
+};
+State &S()
+{
+    static State *s = new State;
+    return *s;
+}
 
 // Sizes in steps of 512 B below 1 MiB, of 2 MiB above: repeated solves of
 // similar problems meet the same sizes.
@@ -51,8 +60,9 @@ void *alloc(size_t bytes)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     const size_t sz = round_up(bytes);
-    std::lock_guard<std::mutex> g(mu);
-    Device &d = devices[dev];
+    State &st = S();
+    std::lock_guard<std::mutex> g(st.mu);
+    Device &d = st.devices[dev];
     // a cached block no more than twice the size (and at most 64 MiB larger)
     auto it = d.free.lower_bound(sz);
     if (it != d.free.end() && it->first <= std::min(2 * sz, sz + (size_t(64) << 20))) {
@@ -60,7 +70,7 @@ void *alloc(size_t bytes)
         const size_t s = it->first;
         d.free.erase(it);
         d.cached -= s;
-        live[p] = {s, dev};
+        st.live[p] = {s, dev};
         return p;
     }
     void *p = nullptr;
@@ -73,23 +83,24 @@ void *alloc(size_t bytes)
             return nullptr;
         }
     }
-    live[p] = {sz, dev};
+    st.live[p] = {sz, dev};
     return p;
 }
 
 void release(void *p)
 {
     if (!p) return;
-    std::lock_guard<std::mutex> g(mu);
-    auto it = live.find(p);
-    if (it == live.end()) {
+    State &st = S();
+    std::lock_guard<std::mutex> g(st.mu);
+    auto it = st.live.find(p);
+    if (it == st.live.end()) {
         (void)hipFree(p);
         return;
     }
     const size_t s = it->second.first;
-    Device &d = devices[it->second.second];
-    live.erase(it);
-    if (d.cached + s > limit) {
+    Device &d = st.devices[it->second.second];
+    st.live.erase(it);
+    if (d.cached + s > st.limit) {
         (void)hipFree(p);
         return;
     }
@@ -99,26 +110,29 @@ void release(void *p)
 
 size_t empty_cache()
 {
-    std::lock_guard<std::mutex> g(mu);
+    State &st = S();
+    std::lock_guard<std::mutex> g(st.mu);
     size_t freed = 0;
-    for (auto &kv : devices) freed += trim(kv.second, 0);
+    for (auto &kv : st.devices) freed += trim(kv.second, 0);
     return freed;
 }
 
 void set_cache_limit(size_t bytes)
 {
-    std::lock_guard<std::mutex> g(mu);
-    limit = bytes;
-    for (auto &kv : devices) trim(kv.second, limit);
+    State &st = S();
+    std::lock_guard<std::mutex> g(st.mu);
+    st.limit = bytes;
+    for (auto &kv : st.devices) trim(kv.second, st.limit);
 }
 
 size_t cached_bytes()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    std::lock_guard<std::mutex> g(mu);
-    auto it = devices.find(dev);
-    return it == devices.end() ? 0 : it->second.cached;
+    State &st = S();
+    std::lock_guard<std::mutex> g(st.mu);
+    auto it = st.devices.find(dev);
+    return it == st.devices.end() ? 0 : it->second.cached;
 }
 
 }  // namespace btm
