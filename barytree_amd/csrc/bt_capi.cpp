@@ -1119,7 +1119,13 @@ static int solve_learned(int numTargets, int numSources, const double *targetX,
         long long fallback = 0;
         e = model_degrees(plan, model->m, delta, deg, fallback);
         if (e.empty()) e = plan.start_items();
-        if (e.empty()) e = plan.set_pair_degrees(deg.data());
+        if (e.empty()) {
+            // the per-degree work's uploads on the stream of the phases
+            // beside the items (they would otherwise wait for the items)
+            plan.side_begin();
+            e = plan.set_pair_degrees(deg.data());
+            plan.side_end();
+        }
         if (e.empty()) plan.set_model_fallback_pairs(fallback);
     }
     const auto t_model = std::chrono::steady_clock::now();

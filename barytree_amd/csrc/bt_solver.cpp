@@ -149,10 +149,18 @@ void Plan::free_all()
 {
     // the plan's work is on its stream: once it has finished, the buffers
     // go back to the device-memory cache
+    if (swapped_) { std::swap(st_, aux_); swapped_ = false; }
     if (st_) (void)hipStreamSynchronize(st_);
+    if (aux_) (void)hipStreamSynchronize(aux_);
     free_var();
     for (void *p : allocs_) btm::release(p);
     allocs_.clear();
+    for (hipEvent_t *e : {&fork_ev_, &join_ev_, &red_ev_})
+        if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
+    if (aux_) {
+        (void)hipStreamDestroy(aux_);
+        aux_ = nullptr;
+    }
     for (hipEvent_t &e : sym_ev_)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (hipEvent_t &e : ev_)
@@ -1420,7 +1428,7 @@ bool Plan::sym_ready()
     return true;
 }
 
-std::string Plan::near_sym()
+std::string Plan::launch_items()
 {
     btk::SymArgs a{};
     a.kernel = prm_.kernel;
@@ -1433,13 +1441,20 @@ std::string Plan::near_sym()
     a.tx = d_tx_; a.ty = d_ty_; a.tz = d_tz_; a.tq = d_tq_;
     a.kp = kp_;
     a.slots = d_sym_slots_;
-    if (!items_started_) {
-        HIPCHK(hipEventRecord(sym_ev_[0], st_));
-        HIPCHK(btk::launch_near_sym(a, st_));
-        HIPCHK(hipEventRecord(sym_ev_[1], st_));
+    forked_ = false;
+    if (fork_ev_) {
+        HIPCHK(hipEventRecord(fork_ev_, st_));
+        forked_ = true;
     }
-    items_started_ = false;
-    sym_ran_ = true;
+    HIPCHK(hipEventRecord(sym_ev_[0], st_));
+    HIPCHK(btk::launch_near_sym(a, st_));
+    HIPCHK(hipEventRecord(sym_ev_[1], st_));
+    items_started_ = true;
+    return "";
+}
+
+std::string Plan::sym_reduce()
+{
     btk::SymReduceArgs r{};
     r.ntiles = (int)sym_red_tiles_.size();
     r.tiles = d_sym_red_tiles_;
@@ -1451,25 +1466,81 @@ std::string Plan::near_sym()
     return "";
 }
 
+std::string Plan::near_sym(bool reduce)
+{
+    if (!items_started_) {
+        std::string r = launch_items();
+        if (!r.empty()) return r;
+    }
+    items_started_ = false;
+    sym_ran_ = true;
+    return reduce ? sym_reduce() : "";
+}
+
 std::string Plan::start_items()
 {
     if (items_started_ || !sym_ready()) return "";
-    btk::SymArgs a{};
-    a.kernel = prm_.kernel;
-    a.singularity = prm_.singularity;
-    a.fp64 = prm_.precision == PREC_MIXED ? 0 : 1;
-    a.nitems = (int)sym_items_.size();
-    a.items = d_sym_items_;
-    a.ctr = reinterpret_cast<const double4 *>(d_sym_ctr_);
-    a.x = d_sx_; a.y = d_sy_; a.z = d_sz_; a.q = d_sq_; a.w = d_sw_;
-    a.tx = d_tx_; a.ty = d_ty_; a.tz = d_tz_; a.tq = d_tq_;
-    a.kp = kp_;
-    a.slots = d_sym_slots_;
-    HIPCHK(hipEventRecord(sym_ev_[0], st_));
-    HIPCHK(btk::launch_near_sym(a, st_));
-    HIPCHK(hipEventRecord(sym_ev_[1], st_));
-    items_started_ = true;
+    (void)overlap_ok();  // the fork event, if the second stream is available
+    return launch_items();
+}
+
+void Plan::side_begin()
+{
+    if (items_started_ && forked_ && !swapped_ && aux_) {
+        std::swap(st_, aux_);
+        swapped_ = true;
+    }
+}
+
+void Plan::side_end()
+{
+    if (swapped_) {
+        std::swap(st_, aux_);
+        swapped_ = false;
+    }
+}
+
+bool Plan::overlap_ok()
+{
+    if (herm_ || !sym_ready()) return false;
+    if (items_started_ && !forked_) return false;  // started before the second stream existed
+    if (!aux_) {
+        if (hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            aux_ = nullptr;
+            return false;
+        }
+        for (hipEvent_t *e : {&fork_ev_, &join_ev_, &red_ev_})
+            if (hipEventCreate(e) != hipSuccess) {
+                (void)hipGetLastError();
+                *e = nullptr;
+            }
+    }
+    return fork_ev_ && join_ev_ && red_ev_;
+}
+
+std::string Plan::fork_phases(bool ov)
+{
+    if (!ov) return "";
+    if (!items_started_) {
+        std::string r = launch_items();
+        if (!r.empty()) return r;
+    }
+    HIPCHK(hipStreamWaitEvent(aux_, fork_ev_, 0));
+    std::swap(st_, aux_);
+    swapped_ = true;
     return "";
+}
+
+std::string Plan::join_phases(bool ov)
+{
+    if (!ov) return "";
+    HIPCHK(hipEventRecord(join_ev_, st_));
+    std::swap(st_, aux_);
+    swapped_ = false;
+    HIPCHK(hipStreamWaitEvent(st_, join_ev_, 0));
+    HIPCHK(hipEventRecord(red_ev_, st_));
+    return sym_reduce();
 }
 
 std::string Plan::execute(bool timed)
@@ -1483,6 +1554,10 @@ std::string Plan::execute(bool timed)
     hipEvent_t *ev = ev_;
     sym_ran_ = false;
     const bool ahead = items_started_;
+    // with the items running on st_, the phases below run on the second stream
+    const bool ov = overlap_ok();
+    AuxGuard guard{*this};
+    if (!(er = fork_phases(ov)).empty()) return er;
     HIPCHK(hipEventRecord(ev[0], st_));
 
     // 1. upward pass over the source trees (clusters.c:50-200)
@@ -1519,7 +1594,7 @@ std::string Plan::execute(bool timed)
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
         if (a.skip_sym) {
-            std::string r = near_sym();
+            std::string r = near_sym(!ov);
             if (!r.empty()) return r;
         }
     }
@@ -1566,6 +1641,7 @@ std::string Plan::execute(bool timed)
         if (!r.empty()) return r;
     }
     HIPCHK(hipEventRecord(ev[4], st_));
+    if (!(er = join_phases(ov)).empty()) return er;
 
     // 5. subtraction correction and the caller's order (treedriver.c:589-592)
     HIPCHK(btk::launch_finish(M_, d_pot_, d_tq_, kp_.ss_param, d_torder_, d_out_, st_));
@@ -1580,7 +1656,16 @@ std::string Plan::execute(bool timed)
         float msym = 0.0f;
         if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
         tm_.gpu_near_sym = msym * 1e-3;
-        if (ahead) {
+        if (ov) {
+            // the items beside the other phases: the near phase counts them,
+            // the total is the elapsed time from their start
+            float mt = 0.0f, mf = 0.0f;
+            HIPCHK(hipEventElapsedTime(&mt, sym_ev_[0], ev[5]));
+            HIPCHK(hipEventElapsedTime(&mf, red_ev_, ev[5]));
+            tm_.gpu_near += tm_.gpu_near_sym;
+            tm_.gpu_finish = mf * 1e-3;
+            tm_.gpu_total = mt * 1e-3;
+        } else if (ahead) {
             // the item launch ran before ev[0]: count it in the near phase
             tm_.gpu_near += tm_.gpu_near_sym;
             tm_.gpu_total += tm_.gpu_near_sym;

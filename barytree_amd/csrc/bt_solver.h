@@ -133,6 +133,13 @@ public:
     // execute() -- a degree model's -- overlaps it; execute() then goes on
     // from it.  Precision and symmetry must not change in between.
     std::string start_items();
+    // Host work between start_items() and execute() that uploads to the
+    // device (a degree model's per-degree work) goes to the stream of the
+    // phases beside the item launch, so its copies do not wait for the
+    // items: side_begin() before it, side_end() after (no-ops when the
+    // items were not started or the second stream is not available).
+    void side_begin();
+    void side_end();
     // FP64 direct sum on the GPU for caller-order target indices `idx`
     // (each target against the sources of its own system).
     std::string direct(const std::vector<int> &idx, std::vector<double> &out);
@@ -216,7 +223,27 @@ private:
     // precision, acceptance-separated sources (theta < 0.99), Coulomb or
     // Yukawa, or regularized Coulomb with skipping
     bool pencil_grids() const;
-    std::string near_sym();
+    // reduce: add the slots to the potential now (else join_phases does)
+    std::string near_sym(bool reduce = true);
+    std::string launch_items();  // the fork event, then the item launch on st_
+    std::string sym_reduce();
+    // The phases beside the item launch: when the items run (and a second
+    // stream is available), they go on st_ and every other phase of the
+    // execution on aux_, which starts at the fork event recorded before the
+    // items; fork_phases swaps st_ and aux_ so the phases' code launches on
+    // aux_, join_phases swaps back, makes st_ wait for aux_ and adds the
+    // slots to the potential (after the downpass: the order of those two
+    // sums is fixed).
+    bool overlap_ok();
+    std::string fork_phases(bool ov);
+    std::string join_phases(bool ov);
+    struct AuxGuard {  // swaps back on an early return
+        Plan &p;
+        ~AuxGuard()
+        {
+            if (p.swapped_) { std::swap(p.st_, p.aux_); p.swapped_ = false; }
+        }
+    };
     std::string validate() const;
     void enumerate_pairs();
     void free_var();
@@ -287,6 +314,10 @@ private:
     btk::KParams kp_{};
 
     hipStream_t st_ = nullptr;
+    hipStream_t aux_ = nullptr;                  // the phases beside the item launch
+    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr, red_ev_ = nullptr;
+    bool swapped_ = false;                       // st_ and aux_ swapped (fork_phases, side_begin)
+    bool forked_ = false;                        // the fork event precedes the started items
     std::vector<void *> allocs_;
     // variable degree
     std::vector<FarPair> pairs_;

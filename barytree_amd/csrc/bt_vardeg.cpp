@@ -614,6 +614,10 @@ std::string Plan::execute_var(bool timed)
     hipEvent_t *ev = ev_;
     sym_ran_ = false;
     const bool ahead = items_started_;
+    // with the items running on st_, the phases below run on the second stream
+    const bool ov = overlap_ok();
+    AuxGuard guard{*this};
+    if (!(er = fork_phases(ov)).empty()) return er;
     HIPCHK(hipEventRecord(ev[0], st_));
 
     // 1. source slots: modified charges of degree p -- up to the plan's
@@ -675,7 +679,7 @@ std::string Plan::execute_var(bool timed)
         a.cursor = d_cursor_;
         HIPCHK(btk::launch_interact(a, st_));
         if (a.skip_sym) {
-            std::string r = near_sym();
+            std::string r = near_sym(!ov);
             if (!r.empty()) return r;
         }
     }
@@ -739,6 +743,7 @@ std::string Plan::execute_var(bool timed)
         }
     }
     HIPCHK(hipEventRecord(ev[4], st_));
+    if (!(er = join_phases(ov)).empty()) return er;
 
     // 5. subtraction correction and the caller's order
     HIPCHK(btk::launch_finish(M_, d_pot_, d_tq_, kp_.ss_param, d_torder_, d_out_, st_));
@@ -753,7 +758,16 @@ std::string Plan::execute_var(bool timed)
         float msym = 0.0f;
         if (sym_ran_) HIPCHK(hipEventElapsedTime(&msym, sym_ev_[0], sym_ev_[1]));
         tm_.gpu_near_sym = msym * 1e-3;
-        if (ahead) {
+        if (ov) {
+            // the items beside the other phases: the near phase counts them,
+            // the total is the elapsed time from their start
+            float mt = 0.0f, mf = 0.0f;
+            HIPCHK(hipEventElapsedTime(&mt, sym_ev_[0], ev[5]));
+            HIPCHK(hipEventElapsedTime(&mf, red_ev_, ev[5]));
+            tm_.gpu_near += tm_.gpu_near_sym;
+            tm_.gpu_finish = mf * 1e-3;
+            tm_.gpu_total = mt * 1e-3;
+        } else if (ahead) {
             // the item launch ran before ev[0]: count it in the near phase
             tm_.gpu_near += tm_.gpu_near_sym;
             tm_.gpu_total += tm_.gpu_near_sym;

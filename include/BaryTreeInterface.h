@@ -107,7 +107,12 @@ typedef struct BaryTreeStats {
     /* GPU phases of the last execution: upward pass; near field (with the
        Lagrange particle-cluster far field, which shares its launch); the
        other far field (grids of target clusters, or the Hermite
-       particle-cluster launch); downpass; correction and scatter */
+       particle-cluster launch); downpass; correction and scatter.  When the
+       symmetric near field's launch runs (sym_ptwise > 0, Lagrange), the
+       upward pass, the far field and the downpass run on a second stream
+       beside it: their times are then elapsed times while sharing the GPU
+       with it, and gpu_total_s is the execution's elapsed time (less than
+       the sum of the phases) */
     double gpu_upward_s, gpu_near_s, gpu_far_s, gpu_down_s, gpu_finish_s, gpu_total_s;
     int num_systems;         /* independent systems in the plan (1 unless batched) */
     int targets_per_thread;  /* near-field tile = 256 * targets_per_thread targets
