@@ -339,6 +339,8 @@ def bench_learned(args):
                                 "cluster-particle theta=0.7, degree 10, leaf 2000"),
                    "n": n, "compute_type": "CP", "theta": 0.7, "degree": 10, "max_leaf": 2000},
         "forms": res,
+        "phases_note": ("the far field, upward pass and downpass run on a second stream beside the symmetric "
+                        "near-field launch: their times are elapsed times while sharing the GPU with it"),
         "delta": args.delta,
         "speedup_learned_vs_fixed_fp64": round(res["fixed_fp64"]["s"] / res["learned_fp64"]["s"], 3),
         "speedup_learned_mixed_vs_fixed_fp64": round(res["fixed_fp64"]["s"] / res["learned_mixed"]["s"], 3),
@@ -1000,6 +1002,8 @@ def bench_large(args):
         "degree_setup_s": round(deg_setup, 3),
         "phases_s": {k: round(st_ml[k], 4) for k in ("gpu_upward_s", "gpu_near_s", "gpu_near_sym_s",
                                                       "gpu_far_s", "gpu_down_s", "gpu_finish_s")},
+        "phases_note": ("upward pass, far field and downpass run on a second stream beside the symmetric "
+                        "near-field launch: their times are elapsed times while sharing the GPU with it"),
         "sym_pairs": sym_evals,
         "interactions": evals_learned,
         "interactions_per_s": evals_learned / tts,
