@@ -282,3 +282,48 @@ def test_one_sided_items_for_other_targets(bt, oracle_mod, ct, k):
     err, diff = rel_l2(ref, got), rel_l2(one, got)
     print(f"items vs oracle {err:.3e}, vs one-directional launch {diff:.3e}", flush=True)
     assert err < 1e-6 and diff < 1e-6
+
+
+def test_two_stream_executions_repeat_across_setting_changes(bt):
+    """A symmetric plan's executions (the item launch on the plan's stream,
+    the other phases on the second stream) give the same bits as fresh
+    plans with the same settings, whatever ran before on the plan: precision
+    switches, the symmetric near field switched off and on, per-pair degrees
+    set and cleared, and the same through a learned one-call solve against
+    its plan form."""
+    from barytree_amd import learned
+    rng = np.random.default_rng(77)
+    n = 60000
+    x, y, z, q = (rng.uniform(-1, 1, n) for _ in range(4))
+    w = np.ones(n)
+    src, tgt = (x, y, z, q, w), (x.copy(), y.copy(), z.copy(), q.copy())
+    args = (1, [0.5], 2, 1, 3, 0.8, 8, 1000, 1000)
+
+    def fresh(precision, sym=True, degrees=None):
+        with bt.Plan(tgt, src, *args, precision=precision) as p:
+            p.set_symmetric(sym)
+            if degrees is not None:
+                p.set_pair_degrees(degrees)
+            return p.execute()
+
+    with bt.Plan(tgt, src, *args) as p:
+        degs = np.random.default_rng(3).integers(2, 12, p.num_pairs())
+        seq = []
+        for prec, sym, d in ((bt.PRECISION_FP64, True, None), (bt.PRECISION_MIXED, True, None),
+                             (bt.PRECISION_MIXED, False, None), (bt.PRECISION_MIXED, True, degs),
+                             (bt.PRECISION_FP64, True, degs), (bt.PRECISION_MIXED, True, None),
+                             (bt.PRECISION_FP64, True, None)):
+            p.set_precision(prec)
+            p.set_symmetric(sym)
+            p.set_pair_degrees(d)
+            seq.append((prec, sym, d, p.execute(), p.stats()))
+    for prec, sym, d, got, st in seq:
+        print(f"case precision={prec} symmetric={sym} degrees={'set' if d is not None else 'fixed'}", flush=True)
+        assert (st["sym_ptwise"] > 0) == sym
+        assert np.array_equal(got, fresh(prec, sym, d))
+    model = learned.load_model("coulomb_cc")
+    one_call = bt.solve_learned(tgt, src, *args, model=model, delta=0.5, precision=bt.PRECISION_MIXED)
+    with bt.Plan(tgt, src, *args, precision=bt.PRECISION_MIXED) as p:
+        p.use_model(model, 0.5)
+        planned = p.execute()
+    assert np.array_equal(one_call, planned)
