@@ -93,11 +93,12 @@ struct InteractArgs {
 
 hipError_t launch_interact(const InteractArgs &a, hipStream_t st);
 
-// Mixed-precision grid targets of Coulomb, Yukawa (either singularity) and
-// regularized Coulomb (skipping) with acceptance-separated sources, the
-// grids as tensor products: one block per tile {key, 0, nl^3, node} of
-// a.tiles, nl = a.nl (a.multi: the node field's degree + 1, 2..13); the
-// other fields as for launch_interact's grid targets (grid_scaled).
+// Grid targets of Coulomb, Yukawa (either singularity) and regularized
+// Coulomb (skipping), the grids as tensor products: one block per tile
+// {key, 0, nl^3, node} of a.tiles, nl = a.nl (a.multi: the node field's
+// degree + 1, 2..13); the other fields as for launch_interact's grid
+// targets.  a.precision 0: FP64 displacements; 1: the centred expansion
+// (grid_scaled), for acceptance-separated sources only.
 hipError_t launch_grid_pencil(const InteractArgs &a, hipStream_t st);
 
 // Item launch of the near field's particle pairs (Coulomb and Yukawa with

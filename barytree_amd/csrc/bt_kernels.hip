@@ -191,12 +191,12 @@ __device__ __forceinline__ void pair_pot(double dx, double dy, double dz, double
 // (coulomb_cp.c:46-55, coulomb_ss_cp.c:54-55, coulomb_ss_cc.c:54-55,
 //  yukawa_cp.c:52, yukawa_ss_cp.c:55-56, yukawa_ss_cc.c:55-56, regularized-coulomb_cp.c:52)
 // The reference applies no skip in these kernels.
-template <int KER, bool SS, bool TAB = false>
-__device__ __forceinline__ void pair_grid(double dx, double dy, double dz, double a, double b,
-                                          const KParams &kp, double &q, double &w,
-                                          const double *tab = nullptr)
+// grid_term: the same from r^2 (Coulomb, Yukawa, regularized Coulomb);
+// EPS_IN: r2 already holds r^2 + e^2 (regularized Coulomb).
+template <int KER, bool SS, bool TAB = false, bool EPS_IN = false>
+__device__ __forceinline__ void grid_term(double r2, double a, double b, const KParams &kp, double &q,
+                                          double &w, const double *tab = nullptr)
 {
-    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
     if constexpr (KER == KC) {
         const double ir = rsq64(r2);
         q = fma(a, ir, q);
@@ -206,8 +206,20 @@ __device__ __forceinline__ void pair_grid(double dx, double dy, double dz, doubl
         const double e = exp_np<TAB>(-kp.p0 * (r2 * ir), tab) * ir;
         q = fma(a, e, q);
         if constexpr (SS) w = fma(b, e, w);
-    } else if constexpr (KER == KR) {
-        q = fma(a, rsq64(r2 + kp.eps2), q);
+    } else {
+        static_assert(KER == KR, "grid_term: Coulomb, Yukawa or regularized Coulomb");
+        q = fma(a, rsq64(EPS_IN ? r2 : r2 + kp.eps2), q);
+    }
+}
+
+template <int KER, bool SS, bool TAB = false>
+__device__ __forceinline__ void pair_grid(double dx, double dy, double dz, double a, double b,
+                                          const KParams &kp, double &q, double &w,
+                                          const double *tab = nullptr)
+{
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    if constexpr (KER == KC || KER == KY || KER == KR) {
+        grid_term<KER, SS, TAB>(r2, a, b, kp, q, w, tab);
     } else {
         q += other_term<KER, false, false>(dx, dy, dz, r2, a, 0.0, 0.0, kp);
     }
@@ -818,6 +830,157 @@ __global__ __launch_bounds__(256) void k_grid_pencil(InteractArgs A)
         case 11: grid_pencil<KER, SS, 11>(A, tile, Ls, Lb, Lr); break;
         case 12: grid_pencil<KER, SS, 12>(A, tile, Ls, Lb, Lr); break;
         case 13: grid_pencil<KER, SS, 13>(A, tile, Ls, Lb, Lr); break;
+        default: break;
+    }
+}
+
+// ---------------------------------------------------------------------
+// FP64 executions on target grids as tensor grids: the pencils and source
+// groups of grid_pencil, with the reference's displacements (grid point
+// minus source, in FP64) and pair expressions (grid_term; coulomb_cp.c:46-55,
+// coulomb_ss_cp.c:54-55, yukawa_cp.c:52, yukawa_ss_cp.c:55-56,
+// regularized-coulomb_cp.c:52 and the CC forms).  dy^2 + dz^2 (+ e^2 for the
+// regularized kernel) is formed once per pencil and source, so a pair costs
+// one subtraction and one fma before the kernel instead of three
+// subtractions, a product and two fma.  r^2 is summed as dx^2 + (dz^2 +
+// dy^2) where the reference sums dx^2 + dy^2 + dz^2: the two differ by
+// rounding (a few ulp of r^2).  FP64 sums per thread; the groups' sums are
+// added in group order.
+// ---------------------------------------------------------------------
+template <int KER, bool SS, int NL>
+__device__ __forceinline__ void grid_pencil64(const InteractArgs &A, const Tile tile, double4 *Ls, double *Lb,
+                                              double *Lr, const double *Ltab)
+{
+    constexpr int U = NL * NL, L = NL;
+    constexpr int G = 256 / U;  // source groups
+    static_assert(U * G <= 256 && G >= 1 && (G - 1) * U * L <= PENCIL_RED, "pencil layout");
+    const int key = tile.x, tid = threadIdx.x;
+    const double *ttab = A.multi ? A.dtab[NL - 1] + (size_t)(tile.w & SLOT_MASK) * 3 * NL
+                                 : A.tnode_tab + (size_t)tile.w * 3 * NL;
+    const size_t orow = A.out_row ? (size_t)A.out_row[key] : (size_t)tile.w * U * NL;
+    const bool on = tid < U * G;
+    const int grp = tid / U, pen = tid % U;
+    const double gy = ttab[NL + pen % NL], gz = ttab[2 * NL + pen / NL];
+    const double e2 = KER == KR ? A.kp.eps2 : 0.0;
+    double gx[L], acc[L], accw[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        gx[i] = ttab[i];
+        acc[i] = 0.0; accw[i] = 0.0;
+    }
+    // the staged sources grp, grp + G, ... of n
+    auto consume = [&](int n) {
+        if (!on) return;
+        for (int j = grp; j < n; j += G) {
+            const double4 s = Ls[j];
+            const double b = SS ? Lb[j] : 0.0;
+            const double dy = gy - s.y, dz = gz - s.z;
+            double dyz = fma(dz, dz, dy * dy);
+            if constexpr (KER == KR) dyz += e2;
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                const double dx = gx[i] - s.x;
+                grid_term<KER, SS, true, KER == KR>(fma(dx, dx, dyz), s.w, b, A.kp, acc[i], accw[i], Ltab);
+            }
+        }
+    };
+    auto stage = [&](double x, double y, double z, double qa, double qb) {
+        Ls[tid] = make_double4(x, y, z, qa);
+        if constexpr (SS) Lb[tid] = qb;
+    };
+    // source clusters (CC)
+    for (int c = A.clu_off[key]; c < A.clu_off[key + 1]; ++c) {
+        const int id = A.clu_id[c];
+        int P = A.P;
+        const double *sx = A.cx, *sy = A.cy, *sz = A.cz, *sq = A.cq, *sw = A.cw;
+        size_t base;
+        if (A.multi) {
+            const int dg = id >> DEG_SHIFT;
+            P = (dg + 1) * (dg + 1) * (dg + 1);
+            sx = A.dcx[dg]; sy = A.dcy[dg]; sz = A.dcz[dg]; sq = A.dcq[dg]; sw = A.dcw[dg];
+            base = (size_t)(id & SLOT_MASK) * P;
+        } else {
+            base = (size_t)id * P;
+        }
+        for (int off = 0; off < P; off += 256) {
+            const int n = min(256, P - off);
+            __syncthreads();
+            if (tid < n) {
+                const size_t g = base + off + tid;
+                stage(sx[g], sy[g], sz[g], sq[g], SS ? sw[g] : 0.0);
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    // source particle ranges (CP)
+    for (int s = A.seg_off[key]; s < A.seg_off[key + 1]; ++s) {
+        const Seg sg = A.segs[s];
+        const int cnt = (int)((unsigned)sg.y & SEG_COUNT_MASK);
+        for (int off = 0; off < cnt; off += 256) {
+            const int n = min(256, cnt - off);
+            __syncthreads();
+            if (tid < n) {
+                const int g = sg.x + off + tid;
+                const double q = A.sq[g];
+                double qa = q, qb = 0.0;
+                if constexpr (SS) { qb = A.sw[g]; qa = q * qb; }
+                stage(A.sx[g], A.sy[g], A.sz[g], qa, qb);
+            }
+            __syncthreads();
+            consume(n);
+        }
+    }
+    if constexpr (G > 1) {
+        // the groups' sums, added in group order
+        if (on && grp > 0)
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                Lr[((grp - 1) * U + pen) * L + i] = acc[i];
+                if constexpr (SS) Lr[PENCIL_RED + ((grp - 1) * U + pen) * L + i] = accw[i];
+            }
+        __syncthreads();
+        if (on && grp == 0)
+            for (int g = 1; g < G; g++)
+#pragma unroll
+                for (int i = 0; i < L; i++) {
+                    acc[i] += Lr[((g - 1) * U + pen) * L + i];
+                    if constexpr (SS) accw[i] += Lr[PENCIL_RED + ((g - 1) * U + pen) * L + i];
+                }
+    }
+    if (!on || grp != 0) return;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        const size_t o = orow + i + (size_t)NL * pen;
+        A.out1[o] = acc[i];
+        if constexpr (SS) A.out2[o] = accw[i];
+    }
+}
+
+template <int KER, bool SS>
+__global__ __launch_bounds__(256) void k_grid_pencil64(InteractArgs A)
+{
+    __shared__ double4 Ls[256];          // source point and its a charge
+    __shared__ double Lb[SS ? 256 : 1];  // its b charge (subtraction)
+    __shared__ double Lr[(SS ? 2 : 1) * PENCIL_RED];
+    __shared__ double Ltab[EXP_TAB];     // 2^(j/64) for exp_nonpos
+    if constexpr (SS || KER == KY) exp_table(Ltab, threadIdx.x);  // read after the first staging barrier
+    const Tile tile = A.tiles[blockIdx.x];
+    if (tile.z == 0) return;  // padding (block-uniform)
+    const int nl = A.multi ? (tile.w >> DEG_SHIFT) + 1 : A.nl;
+    switch (nl) {
+        case 2: grid_pencil64<KER, SS, 2>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 3: grid_pencil64<KER, SS, 3>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 4: grid_pencil64<KER, SS, 4>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 5: grid_pencil64<KER, SS, 5>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 6: grid_pencil64<KER, SS, 6>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 7: grid_pencil64<KER, SS, 7>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 8: grid_pencil64<KER, SS, 8>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 9: grid_pencil64<KER, SS, 9>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 10: grid_pencil64<KER, SS, 10>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 11: grid_pencil64<KER, SS, 11>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 12: grid_pencil64<KER, SS, 12>(A, tile, Ls, Lb, Lr, Ltab); break;
+        case 13: grid_pencil64<KER, SS, 13>(A, tile, Ls, Lb, Lr, Ltab); break;
         default: break;
     }
 }
@@ -2152,6 +2315,17 @@ hipError_t launch_grid_pencil(const InteractArgs &a, hipStream_t st)
 {
     if (a.ntiles <= 0) return hipSuccess;
     const bool ss = a.singularity == 2;
+    if (a.precision == 0) {
+        switch (a.kernel) {
+#define L(K, S) k_grid_pencil64<K, S><<<dim3(a.ntiles), dim3(256), 0, st>>>(a)
+            case KC: if (ss) L(KC, true); else L(KC, false); break;
+            case KY: if (ss) L(KY, true); else L(KY, false); break;
+            case KR: if (ss) return hipErrorInvalidValue; L(KR, false); break;
+#undef L
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (a.kernel) {
 #define L(K, S) k_grid_pencil<K, S><<<dim3(a.ntiles), dim3(256), 0, st>>>(a)
         case KC: if (ss) L(KC, true); else L(KC, false); break;

@@ -218,7 +218,9 @@ template <class T, class A> std::string Plan::up_at(T *d, const std::vector<T, A
 bool Plan::pencil_grids() const
 {
     const Params &p = prm_;
-    return p.precision == PREC_MIXED && p.theta < 0.99 &&
+    // FP64 forms displacements; mixed uses the centred expansion, which needs
+    // acceptance-separated sources (theta < 0.99)
+    return (p.precision != PREC_MIXED || p.theta < 0.99) &&
            (p.kernel == K_COULOMB || p.kernel == K_YUKAWA ||
             (p.kernel == K_REG_COULOMB && p.singularity == S_SKIPPING));
 }

@@ -219,9 +219,9 @@ private:
     std::string down_fixed();    // downpass at the plan's degree
     bool sym_ready();  // items_run(), and the slots allocated (at the first execution that uses them)
     std::string phase_events();
-    // the tensor-grid launch serves this execution's grid targets: mixed
-    // precision, acceptance-separated sources (theta < 0.99), Coulomb or
-    // Yukawa, or regularized Coulomb with skipping
+    // the tensor-grid launch serves this execution's grid targets: Coulomb
+    // or Yukawa, or regularized Coulomb with skipping; FP64, or mixed
+    // precision with acceptance-separated sources (theta < 0.99)
     bool pencil_grids() const;
     // reduce: add the slots to the potential now (else join_phases does)
     std::string near_sym(bool reduce = true);
