@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "bt_kernels.h"
+#include "bt_rsq.h"
 
 namespace btk {
 
@@ -29,25 +30,6 @@ constexpr int GRID_TPT = 2;  // targets per thread of grid-target tiles
 constexpr int NQ = TILE_QUEUES;  // tile queues, one per XCD
 constexpr int KC = 1, KY = 2, KR = 3;
 
-// ---------------------------------------------------------------------
-// 1/sqrt(x) in FP64: hardware estimate refined by Newton steps
-// y <- y + y*(1/2 - (x*y)*(y/2)).  RSQ_NEWTON steps give full FP64 accuracy
-// (about 1 ulp) from the v_rsq_f64 estimate.
-// ---------------------------------------------------------------------
-#ifndef BT_RSQ_NEWTON
-#define BT_RSQ_NEWTON 2
-#endif
-__device__ __forceinline__ double rsq64(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int it = 0; it < BT_RSQ_NEWTON; it++) {
-        const double g = x * y, h = 0.5 * y;
-        const double e = fma(-g, h, 0.5);
-        y = fma(y, e, y);
-    }
-    return y;
-}
 
 // ---------------------------------------------------------------------
 // e^x in FP64 for x <= 0, with a table of 2^(j/64), j < 64 (EXP_TAB
@@ -101,7 +83,7 @@ __device__ __forceinline__ double exp_np(double x, const double *tab)
 //   user                 : a e^{-kr}/r        (particle sources: skip r <= DBL_MIN)
 // (regularized-yukawa_{pp,ss_pp,pc,ss_pc,cp}.c, atan_{pp,pc}.c,
 // sin-over-r_{pp,pc,cp}.c, mq_{pp,pc}.c, rbs-u_*.c, rbs-v_*.c, user_kernel_*.c)
-// A division by a square root is a product with rsq64 (about 1 ulp, against
+// A division by a square root is a product with rsq64 (bt_rsq.h; about 1 ulp, against
 // two correctly rounded operations in the reference's form).  dz/L stays a
 // division: atan and MQ jump at d = +-1/2, where the wrap must see the
 // reference's rounding of d.

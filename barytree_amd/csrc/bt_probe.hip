@@ -8,9 +8,14 @@
 // counters) by these costs gives the time the VALU needs to issue them:
 // the attainable ceiling of that kernel's instruction mix.
 //
+// Also: the FP64 reciprocal square root of the pair kernels (bt_rsq.h)
+// evaluated on given inputs, for the accuracy test of that function.
+//
 // Measurement tooling for the benchmark: built into its own library
 // (libbt_probe.so), not into the drop-in libBaryTree_gpu.so.
 #include <hip/hip_runtime.h>
+
+#include "bt_rsq.h"
 
 namespace btk {
 
@@ -112,6 +117,12 @@ hipError_t probe_valu_issue(double *ns_per_instr, int n)
     return hipSuccess;
 }
 
+__global__ void k_rsq64_eval(int n, const double *x, double *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = rsq64(x[i]);
+}
+
 }  // namespace btk
 
 // ns per wave instruction and SIMD of each kind on the current GPU; returns
@@ -121,4 +132,24 @@ extern "C" int bt_probe_valu(double *ns_per_instr, int n)
     if (!ns_per_instr || n <= 0) return -1;
     const int k = n < btk::PROBE_KINDS ? n : btk::PROBE_KINDS;
     return btk::probe_valu_issue(ns_per_instr, k) == hipSuccess ? k : -1;
+}
+
+// y[i] = rsq64(x[i]) on the current GPU for host arrays x, y of n values;
+// returns 0, or -1 on a HIP error or bad arguments.
+extern "C" int bt_probe_rsq64(const double *x, double *y, int n)
+{
+    if (!x || !y || n <= 0) return -1;
+    double *dx = nullptr, *dy = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    hipError_t e = hipMalloc(&dx, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dy, bytes);
+    if (e == hipSuccess) e = hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        btk::k_rsq64_eval<<<(n + 255) / 256, 256>>>(n, dx, dy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(y, dy, bytes, hipMemcpyDeviceToHost);
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return e == hipSuccess ? 0 : -1;
 }
